@@ -1,0 +1,174 @@
+"""Benchmark of the PIC time step on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload electrode_1024]
+
+A step is one full time step (deposit both species, Poisson + E, push and
+re-bin both species) over the workload's particles, resident in HBM.  Prints
+one JSON line: particle-steps/s (sum over steps and species of live particles
+divided by wall time), per-stage device times, the dominant kernel's roofline
+and the CPU baseline (the oracle's restatement of the reference path, timed
+on a bounded sample on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "bmp-6th-sem_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per particle of the particle kernels (CELL32 / F64 layout):
+# push: read state (cell + 2 offsets + 3 velocities), write state + key.
+PUSH_BYTES = {1: 24 + 24, 0: 44 + 44}
+# bin scatter: read state minus cell + key + rank, write state.
+SCATTER_BYTES = {1: 20 + 8 + 24, 0: 40 + 8 + 44}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, parts, steps=3):
+    """Oracle (C, OpenMP, FP64 AoS, MKL PARDISO) on the same workload."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    import pic_configs as cfg
+
+    t0 = time.perf_counter()
+    sim = O.Sim(w.NX, w.NY, w.width, w.height, w.dt, cfg.EPS, w.bz, w.rects(), bx=w.bx, deposit_energy=1,
+                use_pardiso=True)
+    for s, (q, m, _) in enumerate(w.species()):
+        sim.add_species(q, m, w.n_per_species, cfg.N0, *parts[s])
+    sim.step()                      # warm-up (first solve after factorisation)
+    sim.times()
+    n0 = sum(sim.count(k) for k in range(2))
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    t2 = time.perf_counter()
+    n1 = sum(sim.count(k) for k in range(2))
+    stages = sim.times()
+    pardiso = sim.uses_pardiso
+    sim.close()
+    rate = steps * 0.5 * (n0 + n1) / (t2 - t1)
+    log(f"cpu baseline: {rate:.3e} particle-steps/s over {steps} steps ({t2 - t1:.2f} s, setup {t1 - t0:.1f} s)")
+    return {
+        "value": rate, "unit": "particle-steps/s", "cores": O.threads(), "kind": "port",
+        "sample": f"full {w.name} workload ({w.NX}x{w.NY} cells, {2 * w.n_per_species} particles), {steps} steps "
+                  f"after 1 warm-up step; FP64 AoS, OpenMP, Poisson by "
+                  f"{'MKL PARDISO (factorised before timing)' if pardiso else 'the oracle direct solve'}",
+        "stage_s": {k: round(v / steps, 5) for k, v in stages.items()},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="electrode_1024")
+    ap.add_argument("--layout", type=int, default=None, help="0 F64, 1 CELL32 (default: the workload's)")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    a = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1", "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+
+    import pic_configs as cfg
+    import pic_hip
+
+    w = cfg.WORKLOADS[a.workload]
+    layout = w.layout if a.layout is None else a.layout
+    t0 = time.perf_counter()
+    parts = [cfg.generate(w, s) for s in range(2)]
+    log(f"generated {2 * w.n_per_species} particles in {time.perf_counter() - t0:.1f} s")
+    ctx = cfg.make_context(w, layout=layout, particles=parts, poisson_tol=1e-10)
+    log(f"context ready ({time.perf_counter() - t0:.1f} s)")
+
+    it = 0
+    for _ in range(a.warmup):
+        ctx.step(it)
+        it += 1
+    ctx.sync()
+    n_start = sum(ctx.count(s) for s in range(2))
+    ctx.set_timing(True)
+    ctx.stage_times()
+    stats = []
+    ctx.sync()
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        stats.append(ctx.step(it))
+        it += 1
+    ctx.sync()
+    t2 = time.perf_counter()
+    stages = ctx.stage_times()
+    ctx.set_timing(False)
+    n_end = sum(ctx.count(s) for s in range(2))
+    elapsed = t2 - t1
+    ps = a.steps * 0.5 * (n_start + n_end)
+    value = ps / elapsed
+    ms_step = 1e3 * elapsed / a.steps
+    log(f"gpu: {value:.3e} particle-steps/s, {ms_step:.3f} ms/step")
+
+    # roofline of the push kernel (one launch per species per step)
+    push_ms, push_n = stages["push"]
+    avg_push_s = push_ms / max(push_n, 1) * 1e-3
+    n_avg = 0.5 * (n_start + n_end) / 2           # particles per species per launch
+    push_bytes = PUSH_BYTES[layout] * n_avg
+    achieved = push_bytes / avg_push_s / 1e9 if avg_push_s > 0 else None
+    traffic = None
+    tf = ROOT / "profiles" / "push_traffic.json"
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    roof = {"kernel": "k_push", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+            "bytes_per_particle": PUSH_BYTES[layout], "avg_launch_us": avg_push_s * 1e6}
+
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline(w, parts, a.cpu_steps)
+        except Exception as e:  # the GPU result stands without the baseline
+            log(f"cpu baseline failed: {e}")
+    iters = [s["iterations"] for s in stats]
+    out = {
+        "metric": "particle-steps/s (16M ptcl, 1024^2)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "storage": "int32 cell + f32 offsets/velocities" if layout == 1 else "f64",
+        "data": "synthetic (project generator, seeded)",
+        "config": {"workload": w.name, "cells": f"{w.NX}x{w.NY}", "particles_per_species": w.n_per_species,
+                   "species": 2, "electrode_V": w.electrode_v if w.electrode else None,
+                   "B_T": w.b_total, "B_tilt_deg": w.b_tilt_deg, "poisson_tol": 1e-10},
+        "stages_ms_per_step": {k: round(v[0] / a.steps, 4) for k, v in stages.items()},
+        "poisson": {"iterations_mean": float(np.mean(iters)), "ms_mean": float(np.mean([s["ms"] for s in stats]))},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+    }
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,79 @@
+"""Build libbmp_pic.so (gfx950) in-tree with hipcc.
+
+Usage: python bmp-6th-sem_amd/build.py [--jobs N]
+
+Every source under csrc/ is compiled for --offload-arch=gfx950 and linked into
+bmp-6th-sem_amd/lib/libbmp_pic.so.  Packed FP32 instructions are disabled for
+all device code; the particle kernels are also built with -ffp-contract=off so
+their FP64 expressions round exactly as written.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+LIBDIR = PKG / "lib"
+OBJDIR = PKG / "lib" / "obj"
+LIB = LIBDIR / "libbmp_pic.so"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = [
+    "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument", "-Wno-unknown-warning-option",
+    f"-I{PKG.parent / 'include'}", f"-I{CSRC}",
+]
+
+# per-file extra flags
+EXTRA = {
+    "pic_particles.hip": ["-ffp-contract=off"],
+}
+
+SOURCES = ["pic_particles.hip", "pic_poisson.hip", "pic_host.cpp", "pic_config.cpp", "pic_dropin.cpp"]
+
+
+def _compile(src: str) -> Path:
+    obj = OBJDIR / (src + ".o")
+    cmd = [HIPCC, *COMMON, *EXTRA.get(src, []), "-c", str(CSRC / src), "-o", str(obj)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(jobs: int = 8, verbose: bool = True) -> Path:
+    if not Path(HIPCC).exists():
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    OBJDIR.mkdir(parents=True, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(_compile, SOURCES))
+    tmp = LIBDIR / "libbmp_pic.so.tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp), "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    shutil.rmtree(OBJDIR, ignore_errors=True)
+    if verbose:
+        print(f"built {LIB} ({LIB.stat().st_size // 1024} KiB)")
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=8)
+    a = ap.parse_args()
+    try:
+        build(a.jobs)
+    except RuntimeError as e:
+        print(e, file=sys.stderr)
+        sys.exit(1)

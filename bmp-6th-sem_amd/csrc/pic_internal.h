@@ -1,0 +1,160 @@
+// pic_internal.h — context layout and helpers shared by the host runtime and
+// the gfx950 kernels of the PIC time step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/bmp_pic.h"
+
+namespace pic {
+
+// ------------------------------------------------------------------ errors
+void set_error(const char* fmt, ...);
+
+#define PIC_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            ::pic::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,       \
+                             hipGetErrorString(e_));                           \
+            return PIC_ERR_HIP;                                                \
+        }                                                                      \
+    } while (0)
+
+#define PIC_TRY(expr)                                                          \
+    do {                                                                       \
+        int rc_ = (expr);                                                      \
+        if (rc_ != PIC_OK) return rc_;                                         \
+    } while (0)
+
+// -------------------------------------------------------- device geometry
+// Everything a particle kernel needs about the grid, passed by value.
+struct GridDev {
+    int    gx, gy;          // GRID_X, GRID_Y (nodes)
+    double dx, dy;
+    double width, height;   // GRID_WIDTH, GRID_HEIGHT
+    double inv_dx, inv_dy;  // 1.0 / dx, 1.0 / dy as the reference computes them
+};
+
+// Per-species constants for the particle kernels.
+struct SpeciesDev {
+    double qm;        // q/m (CtoM)
+    double half_qm;   // 0.5 * q/m as the reference groups it: (0.5 * pctm)
+    double denom;     // 1 / |q/m| for the energy weight
+    double dt;
+    double bx, by;    // uniform in-plane B (0 = reference Bz-only push)
+    int    general_b; // 1 when bx or by is non-zero
+};
+
+// Particle storage: structure of arrays, double-buffered for the sort.
+struct ParticleBuf {
+    void*    pa = nullptr;   // F64: x (double)   CELL32: ox (float)
+    void*    pb = nullptr;   // F64: y (double)   CELL32: oy (float)
+    void*    vx = nullptr;   // F64: double       CELL32: float
+    void*    vy = nullptr;
+    void*    vz = nullptr;
+    int32_t* cell = nullptr; // bin key gy*GRID_X + gx
+};
+
+struct Species {
+    double charge = 0, mass = 0, qm = 0, W = 0, sim = 0, actual = 0;
+    int64_t capacity = 0;
+    int64_t n_upper = 0;      // host upper bound on live particles (grid size of particle launches)
+    ParticleBuf buf[2];
+    int cur = 0;              // buffer holding the sorted live particles
+    int32_t* ndev = nullptr;  // device: live particles held in buf[0], buf[1]
+    int32_t* key = nullptr;   // per-particle new key (-1 = absorbed)
+    int32_t* rank = nullptr;  // per-particle slot inside its cell
+    int32_t* counts = nullptr;// GRID_X*GRID_Y
+    int32_t* start = nullptr; // GRID_X*GRID_Y + 1 (start[G] = live count)
+    int64_t* q_charge = nullptr; // fixed-point accumulators (GRID_X*GRID_Y)
+    int64_t* q_energy = nullptr;
+    double*  final_mesh = nullptr;
+    double*  energy_mesh = nullptr;
+    unsigned long long* absorbed = nullptr; // [0] left through x<0, [1] all absorbed
+};
+
+// One multigrid level (FP32 storage).
+struct MgLevel {
+    int gx = 0, gy = 0;
+    float cx = 0, cy = 0, diag = 0;   // 1/hx^2, 1/hy^2, 2(cx+cy)
+    uint8_t* dir = nullptr;           // 1 = Dirichlet node
+    float* e = nullptr;               // correction
+    float* b = nullptr;               // right-hand side
+    float* t = nullptr;               // scratch
+};
+
+struct StageTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double ms = 0;
+    int64_t launches = 0;
+};
+
+struct Geometry {
+    int x1, y1, x2, y2;
+    double potential;
+};
+
+} // namespace pic
+
+struct pic_ctx {
+    pic_params p{};
+    pic::GridDev g{};
+    int G = 0;                 // GRID_X * GRID_Y
+    hipStream_t stream = nullptr;
+    std::vector<pic::Species> sp;
+    std::vector<pic::Geometry> geom;
+    // fields (device)
+    double* rho = nullptr;
+    double* phi = nullptr;
+    double* ex = nullptr;      // separate components for the API / diagnostics
+    double* ey = nullptr;
+    double* bz = nullptr;
+    double4* fld = nullptr;    // packed (Ex, Ey, Bz, 0) per node, padded with zeros
+    uint8_t* dir = nullptr;    // Dirichlet mask (fine grid)
+    double* dval = nullptr;    // Dirichlet value per node (0 elsewhere)
+    // Poisson work vectors (FP64, fine grid)
+    double *pr = nullptr, *pp = nullptr, *pq = nullptr, *pz = nullptr, *pzold = nullptr, *pb = nullptr;
+    double* guess = nullptr;
+    int guess_set = 0;
+    int have_prev_phi = 0;
+    std::vector<pic::MgLevel> mg;
+    double* red = nullptr;          // reduction partials
+    double* scal = nullptr;         // device scalars
+    unsigned int* red_count = nullptr;
+    double* scal_host = nullptr;    // pinned
+    int scan_nb = 0;
+    int32_t* scan_bsum = nullptr;
+    // timing
+    int timing = 0;
+    pic::StageTimer timers[PIC_NSTAGES];
+};
+
+namespace pic {
+
+// kernels / launchers (pic_particles.hip)
+int launch_encode_f64(pic_ctx* c, int s, int64_t n, const double* x, const double* y,
+                      const double* vx, const double* vy, const double* vz);
+int launch_bin(pic_ctx* c, int s);
+int launch_deposit(pic_ctx* c, int s);
+int launch_push(pic_ctx* c, int s);
+int launch_zero_f64(pic_ctx* c, double* p, int64_t n);
+int launch_decode(pic_ctx* c, int s, double* x, double* y, double* vx, double* vy, double* vz);
+
+// Poisson (pic_poisson.hip)
+int poisson_setup(pic_ctx* c);
+void poisson_free(pic_ctx* c);
+int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st);
+int launch_overwrite_phi(pic_ctx* c);
+int launch_overwrite_rho(pic_ctx* c);
+int launch_efield(pic_ctx* c);
+
+// timing helpers (pic_host.cpp)
+int timer_begin(pic_ctx* c, int stage, hipEvent_t* ev);
+int timer_end(pic_ctx* c, int stage, hipEvent_t ev0);
+
+} // namespace pic

@@ -1,0 +1,262 @@
+/*
+ * bmp_pic.h — C ABI of the MI355X (gfx950) 2D3V electrostatic PIC time step.
+ *
+ * Two groups of entry points:
+ *
+ *  1. The drop-in group `sycl_pic_*`: the same symbol names, signatures and
+ *     argument meaning as the reference's plain C wrapper
+ *     (SYCL-PIC-poissonsolver/src/sycl_pic_ctypes.cpp:46-272) plus the two
+ *     extra calls of its CPython module (update_fields_from_phi, grid x/y;
+ *     SYCL-PIC-poissonsolver/src/sycl_pic_module.cpp:110-223).  A ctypes caller
+ *     of the reference library binds these unchanged.  One simulation per
+ *     process, global state, every call synchronous — as in the reference.
+ *
+ *  2. The stage group `pic_*`: the reference's per-stage C++ entry points
+ *     (bin, deposit, Poisson, gather + Boris push, geometry) over an opaque
+ *     context handle instead of `sycl::queue&` / `DeviceView&` / globals.
+ *
+ * Errors: every `pic_*` call returns a status code (PIC_OK = 0, negative on
+ * error) and `pic_last_error()` describes the last failure.  Where the
+ * reference calls `std::exit(code)`, the matching status below is returned
+ * instead; the library never exits the process.
+ *
+ * All pointers in this header are HOST pointers unless stated otherwise.
+ * Grids are row-major, node (gx, gy) at index gy * GRID_X + gx with
+ * GRID_X = NX + 1, GRID_Y = NY + 1 (SYCL-PIC-poissonsolver/src/particles.cpp:356-368).
+ */
+#ifndef BMP_PIC_H
+#define BMP_PIC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define PIC_OK                 0
+#define PIC_ERR_ARG           -1   /* bad argument; also bad device string (sycl_pic_ctypes.cpp:57-59 returns -1) */
+#define PIC_ERR_FILE          -2   /* input file missing (reference exit(1): particles.cpp:326-329, :639-642) */
+#define PIC_ERR_FORMAT        -3   /* malformed M_PICDAT.DAT / geometry.conf (reference exit(2)) */
+#define PIC_ERR_ALLOC         -4   /* allocation failure (reference exit(3)) */
+#define PIC_ERR_SOLVER        -5   /* Poisson solve failed (reference exit(5)/exit(6), poissonSolver.cpp:291-318,511-534) */
+#define PIC_ERR_PERIODIC_GEOM -9   /* periodic flag with > 3 geometries (reference exit(9), particles.cpp:651-654) */
+#define PIC_ERR_HIP          -10   /* HIP runtime error */
+#define PIC_ERR_RANGE        -11   /* capacity / fixed-point headroom exceeded (reference exit(1) on map overflow, mover.cpp:412-419) */
+#define PIC_ERR_UNSUPPORTED  -12   /* configuration this build does not implement (e.g. periodic-y) */
+#define PIC_ERR_STATE        -13   /* call made in the wrong order (e.g. before init) */
+
+/* ------------------------------------------------------- storage layouts */
+/* PIC_LAYOUT_F64: per particle x, y, vx, vy, vz as double plus an int32 cell
+ *   key — the reference's Point precision (point.hpp:11-14) without the unused
+ *   z coordinate and hashmap slot pointer.
+ * PIC_LAYOUT_CELL32: per particle int32 cell key, float in-cell offsets
+ *   ox, oy in [0, 1) (units of dx, dy) and float vx, vy, vz: 24 bytes. */
+#define PIC_LAYOUT_F64     0
+#define PIC_LAYOUT_CELL32  1
+
+/* ----------------------------------------------------------- grid names */
+#define PIC_GRID_RHO        0   /* rho (reference `rho`, types.cpp:33)            */
+#define PIC_GRID_PHI        1   /* phi (reference `phi`)                            */
+#define PIC_GRID_EX         2   /* electricField[].fx                               */
+#define PIC_GRID_EY         3   /* electricField[].fy                               */
+#define PIC_GRID_BZ         4   /* magneticField (Bz on nodes)                      */
+#define PIC_GRID_FINAL0     8   /* + species: Particle::final_mesh (species_ds.hpp:26) */
+#define PIC_GRID_ENERGY0   16   /* + species: Particle::energy_mesh                 */
+
+/* -------------------------------------------------------------- stages */
+#define PIC_STAGE_DEPOSIT   0
+#define PIC_STAGE_POISSON   1
+#define PIC_STAGE_EFIELD    2
+#define PIC_STAGE_PUSH      3
+#define PIC_STAGE_BIN       4
+#define PIC_NSTAGES         5
+
+typedef struct pic_ctx pic_ctx;
+
+/* Simulation parameters.  Field names follow the reference globals
+ * (types.hpp:8-133); fill with pic_params_default() then override. */
+typedef struct pic_params {
+    int    NX, NY;            /* cells in x / y (M_PICDAT.DAT "NX NY")                       */
+    double width, height;     /* GRID_WIDTH, GRID_HEIGHT in m                                 */
+    double dt;                /* time step, s                                                 */
+    double epsilon;           /* permittivity, F/m                                            */
+    double b0;                /* uniform Bz filled into magneticField, T (initMagneticField) */
+    double bx, by;            /* uniform in-plane B, T; 0 = the reference's Bz-only Boris     */
+    double surf_area;         /* surfArea for the weighting factor (particles.cpp:431-432)   */
+    int    periodic;          /* periodicBoundary; only 0 is implemented                     */
+    int    layout;            /* PIC_LAYOUT_*                                                 */
+    int    charge_shift;      /* fixed-point scale 2^-charge_shift of normalised CIC weights  */
+    int    energy_shift;      /* fixed-point scale 2^-energy_shift eV of energy weights        */
+    int    deposit_energy;    /* 1: also build energy_mesh each deposit (reference behaviour) */
+    double poisson_tol;       /* relative residual ||b - A phi|| / ||b|| to stop at            */
+    int    poisson_max_iter;  /* cap on outer (FP64) iterations                                */
+    int    mg_pre, mg_post;   /* weighted-Jacobi sweeps before / after coarse correction      */
+    double mg_omega;          /* Jacobi weight                                                 */
+} pic_params;
+
+/* Per-solve statistics of pic_poisson_solve. */
+typedef struct pic_solve_stats {
+    int    iterations;        /* PCG iterations (FP64 outer loop)                 */
+    double initial_relres;    /* ||b - A phi0|| / ||b|| of the initial guess       */
+    double final_relres;      /* same after the solve                              */
+    double ms;                /* device time of the solve (HIP events)             */
+    int    guess;             /* 0 zero guess, 1 previous phi, 2 caller-set guess  */
+} pic_solve_stats;
+
+/* ====================================================== stage group */
+
+/* Fill defaults (dx = dy = 1e-4 m, 128 x 128 cells, ...). */
+void pic_params_default(pic_params* p);
+
+/* Create a context on HIP device 0.  Allocates grids and the multigrid
+ * hierarchy; geometry defaults to four grounded walls. */
+int  pic_create(const pic_params* p, pic_ctx** out);
+void pic_destroy(pic_ctx* ctx);
+
+/* Geometry (replaces readGeometry, particles.cpp:636-686, and
+ * setPotentialBoundary, poissonSolver.cpp:64-79).  `rects` holds n rows of
+ * (x1, y1, x2, y2, V) as fractions of the domain; node index =
+ * (int)((GRID-1) * fraction); rectangles are inclusive; later rows win. */
+int  pic_set_geometry(pic_ctx* ctx, const double* rects, int n);
+/* Parse a geometry.conf file (same format as the reference). */
+int  pic_read_geometry(pic_ctx* ctx, const char* path);
+/* Node rectangles actually in use: out_nodes n x (x1, y1, x2, y2), out_pot n. */
+int  pic_get_geometry(pic_ctx* ctx, int* out_nodes, double* out_pot, int* n);
+
+/* Register a species (PICDAT species table row, particles.cpp:590-621):
+ * q/m = charge / mass (initCtoM :298-300); weighting factor
+ * W = actual * surf_area / (sim * dx*dy*dx*dy) (updateWeighingFactor :303-305).
+ * `capacity` = max particles ever held. */
+int  pic_add_species(pic_ctx* ctx, double charge, double mass, double sim_count,
+                     double actual_count, int64_t capacity, int* species_out);
+
+/* Project particle generator (host only; replaces initParticlesRandomly,
+ * particles.cpp:127-173, which is time-seeded).  Counter-based: particle i of
+ * (seed, stream) is independent of n and thread count.  Positions uniform in
+ * [0, width) x [0, height), rejecting points inside the node rectangle
+ * excl = (x1, y1, x2, y2) (meters = node * dx) when excl != NULL.
+ * Velocities Maxwellian, sigma = sqrt(|q/m| T) per component. */
+int  pic_generate_particles(uint64_t seed, int stream, int64_t n,
+                            double width, double height, double dx, double dy,
+                            const int* excl, double qm_abs, double temp_eV,
+                            double* x, double* y, double* vx, double* vy, double* vz);
+
+/* Upload n particles (absolute FP64 state) into species s, encode them into
+ * the context layout and bin them (counting sort by the reference key
+ * (int)(x * (1.0/dx)), particles.cpp:229-235).  Replaces the InsertKernel. */
+int  pic_upload_particles(pic_ctx* ctx, int s, int64_t n, const double* x, const double* y,
+                          const double* vx, const double* vy, const double* vz);
+/* Upload raw CELL32 state (cell, ox, oy, vx, vy, vz) and bin it. */
+int  pic_upload_cell32(pic_ctx* ctx, int s, int64_t n, const int32_t* cell,
+                       const float* ox, const float* oy,
+                       const float* vx, const float* vy, const float* vz);
+/* Live particle count of species s (after the last bin). */
+int64_t pic_particle_count(pic_ctx* ctx, int s);
+/* Download the live particles in storage (cell-sorted) order, decoded to
+ * FP64: F64 layout returns the stored values, CELL32 returns
+ * x = (gx + ox) * dx.  `cell` gets the bin key.  Any pointer may be NULL. */
+int  pic_download_particles(pic_ctx* ctx, int s, double* x, double* y,
+                            double* vx, double* vy, double* vz, int32_t* cell);
+/* Download raw CELL32 storage.  Any pointer may be NULL. */
+int  pic_download_cell32(pic_ctx* ctx, int s, int32_t* cell, float* ox, float* oy,
+                         float* vx, float* vy, float* vz);
+
+/* Bin species s: counting sort of the live particles by cell key, dropping
+ * dead ones (replaces hashmap insert/delete + cleanup_map + compaction,
+ * sycl_hashmap.hpp:131-231, utils.cpp:30-44,249-372). */
+int  pic_bin(pic_ctx* ctx, int s);
+/* Per-key particle counts, GRID_X * GRID_Y ints (key_idx after cleanup). */
+int  pic_get_cell_counts(pic_ctx* ctx, int s, int32_t* counts);
+/* Exclusive prefix of the counts, GRID_X * GRID_Y + 1 ints. */
+int  pic_get_cell_start(pic_ctx* ctx, int s, int32_t* start);
+
+/* rho = 0 (sycl_pic_zero_rho, sycl_pic_ctypes.cpp:121-124). */
+int  pic_zero_rho(pic_ctx* ctx);
+/* CIC charge (and energy) deposit of species s (interpolate,
+ * interpolation.cpp:13-176): final_mesh[s], energy_mesh[s], and
+ * rho += final_mesh[s] * (-q_s / epsilon). */
+int  pic_deposit(pic_ctx* ctx, int s);
+/* Raw fixed-point accumulators of the last deposit of species s (GRID_X*GRID_Y
+ * int64 each; either pointer may be NULL). */
+int  pic_get_deposit_fixed(pic_ctx* ctx, int s, int64_t* q_charge, int64_t* q_energy);
+
+/* Poisson solve (poissonSolver, poissonSolver.cpp:459-544): rho is rewritten
+ * in place as the reference does (overwriteRho: rho[geometry] = V, then
+ * rho = -rho), phi solves the 5-point system with Dirichlet nodes held at V,
+ * then E = -grad phi (calculateElectricField :433-456).  iteration == 0 starts
+ * from a zero guess; later iterations warm-start from the previous phi, or
+ * from the guess set with pic_set_guess().  `st` may be NULL. */
+int  pic_poisson_solve(pic_ctx* ctx, int iteration, pic_solve_stats* st);
+/* Use `phi0` (GRID_X*GRID_Y doubles) as the next solve's initial guess. */
+int  pic_set_guess(pic_ctx* ctx, const double* phi0);
+/* overwritePhi + calculateElectricField on the current phi
+ * (py_update_fields_from_phi, sycl_pic_module.cpp:216-223). */
+int  pic_update_fields_from_phi(pic_ctx* ctx);
+
+/* Gather E and Bz at each particle, Boris push, wall absorption, new cell key
+ * (mover / deterministic_mover, mover.cpp:103-176,304-367).  With rebin != 0
+ * the species is re-binned afterwards (the reference re-inserts into its
+ * hashmap inside the mover). */
+int  pic_push(pic_ctx* ctx, int s, int rebin);
+
+/* One full time step in the reference order (main.cpp:130-257):
+ * zero rho; deposit every species; Poisson + E; push + re-bin every species. */
+int  pic_step(pic_ctx* ctx, int iteration, pic_solve_stats* st);
+
+/* Copy a grid (PIC_GRID_*) to / from the host, GRID_X * GRID_Y doubles. */
+int  pic_get_grid(pic_ctx* ctx, int which, double* out);
+int  pic_set_grid(pic_ctx* ctx, int which, const double* in);
+
+/* Absorption counters of species s since creation: particles that left
+ * through x < 0 (the reference `new_additions`, mover.cpp:344-352) and all
+ * absorbed particles.  Either pointer may be NULL. */
+int  pic_get_absorbed(pic_ctx* ctx, int s, int64_t* left, int64_t* total);
+
+/* Block until all work of the context is done. */
+int  pic_sync(pic_ctx* ctx);
+/* Enable per-stage HIP-event timing (0/1), and read the accumulated device
+ * milliseconds and launch counts per stage (PIC_NSTAGES entries each; either
+ * pointer may be NULL).  Reading resets the accumulators. */
+int  pic_set_timing(pic_ctx* ctx, int on);
+int  pic_stage_times(pic_ctx* ctx, double* ms, int64_t* launches);
+
+/* Grid shape of the context: GRID_X, GRID_Y. */
+int  pic_grid_shape(pic_ctx* ctx, int* grid_x, int* grid_y);
+/* Text of the last error (thread-local). */
+const char* pic_last_error(void);
+
+/* =================================================== drop-in group */
+/* Same names/signatures/semantics as SYCL-PIC-poissonsolver/src/sycl_pic_ctypes.cpp.
+ * init reads ./M_PICDAT.DAT and ./geometry.conf from the working directory;
+ * dev_type must be "gpu" ("cpu" returns -1: this library has no CPU path);
+ * wg and ps are accepted and ignored (the counting sort has no probing
+ * width); input1/input2 are ignored as in the reference (particles.cpp:174-176).
+ * Particles are drawn by pic_generate_particles with the seed in the
+ * environment variable BMP_PIC_SEED (default 1). */
+int     sycl_pic_init(const char* dev_type, int wg, int ps, const char* input1, const char* input2); /* sycl_pic_ctypes.cpp:49-88 */
+int     sycl_pic_get_grid_size(void);               /* :91-93   */
+int     sycl_pic_get_grid_x(void);                  /* sycl_pic_module.cpp:110-115 */
+int     sycl_pic_get_grid_y(void);                  /* sycl_pic_module.cpp:117-122 */
+int     sycl_pic_get_timesteps(void);               /* :95-97   */
+int     sycl_pic_get_print_interval(void);          /* :99-101  */
+int     sycl_pic_get_num_species(void);             /* :103-105 */
+int     sycl_pic_get_particle_count(int species);   /* :107-109 */
+double* sycl_pic_get_rho_ptr(void);                 /* :112-114 host mirror, refreshed after each deposit / solve */
+double* sycl_pic_get_phi_ptr(void);                 /* :116-118 host mirror; caller writes are read back by update_fields_from_phi */
+void    sycl_pic_zero_rho(void);                    /* :121-124 */
+double  sycl_pic_charge_deposition(int species);    /* :127-150 */
+double  sycl_pic_poisson_solve(int iteration);      /* :153-158 */
+void    sycl_pic_update_fields_from_phi(void);      /* sycl_pic_module.cpp:216-223 */
+double  sycl_pic_move_particles(int species, int iteration);                          /* :161-176 */
+double  sycl_pic_compact_particles(int species, double* out_alloc_t, double* out_scan_t); /* :179-198 */
+double  sycl_pic_cleanup_map(int species);          /* :201-209 */
+double  sycl_pic_generate_new_particles(void);      /* :212-236 */
+void    sycl_pic_print_diagnostics(const char* dir, int iteration); /* :239-244 */
+void    sycl_pic_save_meshes(void);                 /* :247-264 */
+void    sycl_pic_finalize(void);                    /* :267-270 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BMP_PIC_H */

@@ -1,0 +1,211 @@
+"""GPU parity of each stage against the oracle on identical inputs, through
+the C ABI.  Integer/index results are compared bit-exactly; FP64 deposit
+values to 1e-12 (normwise, charge) and 1e-9 (energy); Poisson to 1e-10
+relative; push and E field bit-exactly (same FP64 expressions on both sides)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+WALLS = [[0, 0, 0, 1, 0.0], [1, 0, 1, 1, 0.0], [0, 1, 1, 1, 0.0], [0, 0, 1, 0, 0.0]]
+ELEC = WALLS + [[0.4375, 0.4375, 0.5625, 0.5625, 100.0]]
+E, ME, MI = 1.602176634e-19, 9.1093837015e-31, 40 * 1.66053906660e-27
+
+
+def _ctx(hip, NX, layout, n=0, geom=WALLS, **kw):
+    dx = 1e-4
+    p = hip.default_params(NX=NX, NY=NX, width=NX * dx, height=NX * dx, dt=1.77e-11, b0=0.01, layout=layout, **kw)
+    c = hip.Context(p)
+    c.set_geometry(geom)
+    return c
+
+
+def _parts(hip, c, n, seed=1, stream=0, qm=E / ME, T=2.0):
+    return hip.generate_particles(seed, stream, n, c.params.width, c.params.height, c.dx, c.dy, qm, T)
+
+
+def _adversarial(c, n=2000, seed=0):
+    """Positions on and next to cell faces (including the far faces)."""
+    rng = np.random.default_rng(seed)
+    NX = c.params.NX
+    k = rng.integers(0, NX, n)
+    j = rng.integers(0, NX, n)
+    x = k * c.dx
+    y = j * c.dy
+    sel = rng.integers(0, 3, n)
+    x = np.where(sel == 1, np.nextafter(x, np.inf), np.where(sel == 2, np.nextafter(x + c.dx, 0), x))
+    y = np.where(sel == 2, np.nextafter(y, np.inf), y)
+    x = np.clip(x, 0, np.nextafter(c.params.width, 0))
+    y = np.clip(y, 0, np.nextafter(c.params.height, 0))
+    v = rng.normal(0, 1e5, (3, n))
+    return x, y, v[0], v[1], v[2]
+
+
+def _canon(*cols):
+    """Rows sorted lexicographically (cell-sorted storage order within a cell is free)."""
+    a = np.stack([np.asarray(c, np.float64) for c in cols], 1)
+    idx = np.lexsort(a.T[::-1])
+    return a[idx]
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_bin_membership_and_counts(hip_mod, oracle_mod, layout):
+    hip, O = hip_mod, oracle_mod
+    c = _ctx(hip, 64, layout)
+    s = c.add_species(-E, ME, 30000, 1e16, 40000)
+    x, y, vx, vy, vz = _parts(hip, c, 30000)
+    ax = _adversarial(c)
+    x, y, vx, vy, vz = [np.concatenate([a, b]) for a, b in zip((x, y, vx, vy, vz), ax)]
+    c.upload(s, x, y, vx, vy, vz)
+    key = O.keys(x, y, c.gx, c.dx, c.dy)
+    counts, start, order = O.bin(key, c.G)
+    assert np.array_equal(c.cell_counts(s), counts)
+    assert np.array_equal(c.cell_start(s), start)
+    if layout == 0:
+        gx_, gy_, gvx, gvy, gvz, gcell = c.download(s)
+        assert np.array_equal(_canon(gcell, gx_, gy_, gvx, gvy, gvz), _canon(key, x, y, vx, vy, vz))
+    else:
+        cell, ox, oy, fvx, fvy, fvz = c.download_cell32(s)
+        rc, rox, roy, rvx, rvy, rvz = O.encode_cell32(x, y, vx, vy, vz, c.gx, c.dx, c.dy)
+        assert np.array_equal(_canon(cell, ox, oy, fvx, fvy, fvz), _canon(rc, rox, roy, rvx, rvy, rvz))
+    # storage is cell-sorted
+    gcell = c.download(s)[5]
+    assert np.all(np.diff(gcell) >= 0)
+    c.close()
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_deposit_parity(hip_mod, oracle_mod, layout):
+    hip, O = hip_mod, oracle_mod
+    c = _ctx(hip, 64, layout)
+    qm = -E / ME
+    s = c.add_species(-E, ME, 50000, 1e16, 50000)
+    x, y, vx, vy, vz = _parts(hip, c, 50000, seed=5)
+    c.upload(s, x, y, vx, vy, vz)
+    c.zero_rho()
+    c.deposit(s)
+    qc, qe = c.deposit_fixed(s)
+    W = 1e16 * (c.params.width * c.params.height) / (50000 * c.dx * c.dy * c.dx * c.dy)
+    if layout == 0:
+        key = O.keys(x, y, c.gx, c.dx, c.dy)
+        rqc, rqe = O.deposit_fixed_f64(x, y, vx, vy, vz, key, c.gx, c.gy, c.dx, c.dy, 40, 32, qm)
+        fm, em, rho = O.deposit_ref(x, y, vx, vy, vz, key, c.gx, c.gy, c.dx, c.dy, W, qm, -E, 8.854e-12)
+    else:
+        cell, ox, oy, fvx, fvy, fvz = O.encode_cell32(x, y, vx, vy, vz, c.gx, c.dx, c.dy)
+        rqc, rqe = O.deposit_fixed_cell32(cell, ox, oy, fvx, fvy, fvz, c.gx, c.gy, 40, 32, qm)
+        xd = ((cell % c.gx) + ox.astype(np.float64)) * c.dx
+        yd = ((cell // c.gx) + oy.astype(np.float64)) * c.dy
+        fm, em, rho = O.deposit_ref(xd, yd, fvx, fvy, fvz, cell, c.gx, c.gy, c.dx, c.dy, W, qm, -E, 8.854e-12)
+    assert np.array_equal(qc, rqc), "fixed-point charge differs from the oracle"
+    assert np.array_equal(qe, rqe), "fixed-point energy differs from the oracle"
+    gfm = c.grid(hip.GRID_FINAL0 + s)
+    gem = c.grid(hip.GRID_ENERGY0 + s)
+    grho = c.grid(hip.GRID_RHO)
+    assert np.abs(gfm - fm).max() / np.abs(fm).max() < 1e-12
+    assert np.abs(grho - rho).max() / np.abs(rho).max() < 1e-12
+    assert np.abs(gem - em).max() / np.abs(em).max() < 1e-9
+    c.close()
+
+
+def test_efield_parity(hip_mod, oracle_mod):
+    hip, O = hip_mod, oracle_mod
+    c = _ctx(hip, 48, 0, geom=ELEC)
+    rng = np.random.default_rng(3)
+    phi = rng.normal(0, 10, c.G)
+    c.set_grid(hip.GRID_PHI, phi)
+    c.update_fields_from_phi()
+    _, _, mask, dval = O.geometry(c.gx, c.gy, ELEC)
+    phi_ref = np.where(mask == 1, dval, phi)
+    ex, ey = O.efield(phi_ref, c.gx, c.gy, c.dx, c.dy)
+    assert np.array_equal(c.grid(hip.GRID_PHI), phi_ref)
+    assert np.array_equal(c.grid(hip.GRID_EX), ex)
+    assert np.array_equal(c.grid(hip.GRID_EY), ey)
+    c.close()
+
+
+@pytest.mark.parametrize("layout,bx", [(0, 0.0), (0, 0.025), (1, 0.0), (1, 0.025)])
+def test_push_parity(hip_mod, oracle_mod, layout, bx):
+    hip, O = hip_mod, oracle_mod
+    NX = 64
+    c = _ctx(hip, NX, layout, bx=bx)
+    rng = np.random.default_rng(8)
+    ex = rng.normal(0, 2e3, c.G)
+    ey = rng.normal(0, 2e3, c.G)
+    bz = 0.0433 + rng.normal(0, 1e-3, c.G)
+    c.set_grid(hip.GRID_EX, ex)
+    c.set_grid(hip.GRID_EY, ey)
+    c.set_grid(hip.GRID_BZ, bz)
+    for sp, (q, m, T) in enumerate([(-E, ME, 2.0), (E, MI, 0.03)]):
+        s = c.add_species(q, m, 40000, 1e16, 40000)
+        x, y, vx, vy, vz = _parts(hip, c, 40000, seed=9, stream=sp, qm=abs(q / m), T=T)
+        # fast particles so some cross walls and cells in one step
+        vx = vx * 30
+        vy = vy * 30
+        c.upload(s, x, y, vx, vy, vz)
+        dt = c.params.dt
+        g = dict(gx=c.gx, gy=c.gy, dx=c.dx, dy=c.dy, width=c.params.width, height=c.params.height)
+        if layout == 0:
+            X, Y, VX, VY, VZ, key, ab = O.push_f64(x, y, vx, vy, vz, ex, ey, bz, qm=q / m, dt=dt, bx=bx, **g)
+        else:
+            cell, ox, oy, fvx, fvy, fvz = O.encode_cell32(x, y, vx, vy, vz, c.gx, c.dx, c.dy)
+            rc, rox, roy, rvx, rvy, rvz, key, ab = O.push_cell32(cell, ox, oy, fvx, fvy, fvz, ex, ey, bz,
+                                                                 qm=q / m, dt=dt, bx=bx, **g)
+        c.push(s, rebin=True)
+        live = key >= 0
+        assert c.count(s) == live.sum()
+        left, total = c.absorbed(s)
+        assert (left, total) == (ab[0], ab[1])
+        if sp == 0:
+            assert total > 0
+        counts, _, _ = O.bin(key, c.G)
+        assert np.array_equal(c.cell_counts(s), counts)
+        if layout == 0:
+            gx_, gy_, gvx, gvy, gvz, gcell = c.download(s)
+            assert np.array_equal(_canon(gcell, gx_, gy_, gvx, gvy, gvz),
+                                  _canon(key[live], X[live], Y[live], VX[live], VY[live], VZ[live]))
+        else:
+            cell2, ox2, oy2, v1, v2, v3 = c.download_cell32(s)
+            assert np.array_equal(_canon(cell2, ox2, oy2, v1, v2, v3),
+                                  _canon(rc[live], rox[live], roy[live], rvx[live], rvy[live], rvz[live]))
+    c.close()
+
+
+@pytest.mark.parametrize("it", [0, 4])
+def test_poisson_matches_reference_pardiso(hip_mod, oracle_mod, it):
+    """validation_dumps (PARDISO) at 257^2: phi to 1e-10, rho rewritten as overwriteRho does."""
+    from conftest import GOLDEN
+
+    hip, O = hip_mod, oracle_mod
+    p = hip.default_params(NX=256, NY=256, width=1.0, height=1.0, dt=2.5e-9, poisson_tol=1e-12)
+    c = hip.Context(p)
+    c.set_geometry(WALLS)
+    rho = np.load(GOLDEN / f"iter{it:04d}_rho_input.npy")
+    ref = np.load(GOLDEN / f"iter{it:04d}_phi_cpp.npy")
+    c.set_grid(hip.GRID_RHO, rho)
+    st = c.poisson_solve(0)
+    phi = c.grid(hip.GRID_PHI)
+    assert st["final_relres"] <= 1e-12
+    assert np.abs(phi - ref).max() / np.abs(ref).max() < 1e-10
+    _, _, mask, dval = O.geometry(257, 257, WALLS)
+    assert np.array_equal(c.grid(hip.GRID_RHO), O.overwrite_rho(rho, mask, dval))
+    ex, ey = O.efield(phi, 257, 257, 1 / 256, 1 / 256)
+    assert np.array_equal(c.grid(hip.GRID_EX), ex) and np.array_equal(c.grid(hip.GRID_EY), ey)
+    c.close()
+
+
+def test_poisson_electrode_vs_direct(hip_mod, oracle_mod):
+    hip, O = hip_mod, oracle_mod
+    NX = 128
+    c = _ctx(hip, NX, 0, geom=ELEC, poisson_tol=1e-12)
+    rng = np.random.default_rng(1)
+    rho = rng.normal(0, 1e6, c.G)
+    c.set_grid(hip.GRID_RHO, rho)
+    st = c.poisson_solve(0)
+    rhs, phi_ref, _, _ = O.poisson_solver(rho, c.gx, c.gy, c.dx, c.dy, ELEC)
+    phi = c.grid(hip.GRID_PHI)
+    assert np.abs(phi - phi_ref).max() / np.abs(phi_ref).max() < 1e-10
+    # warm start from the solution converges immediately
+    c.set_grid(hip.GRID_RHO, rho)
+    st2 = c.poisson_solve(1)
+    assert st2["iterations"] <= 1 and st2["guess"] == 1
+    c.close()

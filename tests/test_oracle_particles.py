@@ -1,0 +1,256 @@
+"""Pins the oracle's particle stages (bin key, membership, CIC deposit, Boris
+push, wall absorption) with brute force, closed forms and invariants."""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+
+def test_key_exact_rational_away_from_faces(oracle_mod):
+    """Away from cell faces the reference key (int)(x*(1.0/dx)) equals the
+    exact floor(x/dx); x and y must not be swapped and the row stride is the
+    node count GRID_X = NX + 1."""
+    O = oracle_mod
+    rng = np.random.default_rng(7)
+    NX, NY, dx, dy = 37, 23, 1e-4, 2.5e-4
+    gx = NX + 1
+    x = rng.uniform(0, NX * dx, 2000)
+    y = rng.uniform(0, NY * dy, 2000)
+    k = O.keys(x, y, gx, dx, dy)
+    for xi, yi, ki in zip(x[:300], y[:300], k[:300]):
+        fx, fy = Fraction(xi) / Fraction(dx), Fraction(yi) / Fraction(dy)
+        if min(fx - int(fx), int(fx) + 1 - fx, fy - int(fy), int(fy) + 1 - fy) < 1e-9:
+            continue
+        assert ki == int(fy) * gx + int(fx)
+
+
+def test_bin_membership_brute_force(oracle_mod):
+    O = oracle_mod
+    rng = np.random.default_rng(3)
+    G = 50
+    key = rng.integers(-1, G, 1000).astype(np.int32)
+    counts, start, order = O.bin(key, G)
+    assert np.array_equal(counts, np.bincount(key[key >= 0], minlength=G))
+    assert start[-1] == (key >= 0).sum()
+    for k in range(G):
+        members = order[start[k]:start[k + 1]]
+        assert np.array_equal(members, np.nonzero(key == k)[0])
+
+
+def _single(O, x, y, gx=5, gy=5, dx=0.5, dy=0.25, W=2.0):
+    key = O.keys(np.array([x]), np.array([y]), gx, dx, dy)
+    fm, em, rho = O.deposit_ref([x], [y], [3.0], [4.0], [0.0], key, gx, gy, dx, dy, W, 1.0, 1.0, 1.0)
+    return key[0], fm.reshape(gy, gx), em.reshape(gy, gx), rho.reshape(gy, gx)
+
+
+def test_deposit_single_particle_weights(oracle_mod):
+    """One particle at offsets (a, b) of cell (1, 2): corner weights
+    (1-a)(1-b), a(1-b), (1-a)b, ab times dx*dy*W; energy weight
+    0.5 v^2 / |q/m| = 12.5; rho = final * (-q/eps)."""
+    O = oracle_mod
+    dx, dy, W = 0.5, 0.25, 2.0
+    a, b = 0.25, 0.75
+    k, fm, em, rho = _single(O, (1 + a) * dx, (2 + b) * dy)
+    assert k == 2 * 5 + 1
+    A = dx * dy * W
+    expect = {(2, 1): (1 - a) * (1 - b), (2, 2): a * (1 - b), (3, 1): (1 - a) * b, (3, 2): a * b}
+    for (j, i), w in expect.items():
+        assert fm[j, i] == pytest.approx(A * w, rel=1e-14)
+        assert em[j, i] == pytest.approx(A * w * 12.5, rel=1e-14)
+    assert np.count_nonzero(fm) == 4
+    assert np.allclose(rho, -fm)
+
+
+def test_deposit_last_column_dropped(oracle_mod):
+    """Keys on the last node column deposit nothing (interpolation.cpp:122)."""
+    O = oracle_mod
+    gx, dx = 5, 0.5
+    x = np.nextafter(4 * dx, 0)           # x < W but x * (1/dx) rounds to NX
+    key = O.keys(np.array([x]), np.array([0.3]), gx, dx, 0.25)
+    if key[0] % gx != gx - 1:
+        pytest.skip("this dx does not round up at the last face")
+    fm, _, _ = O.deposit_ref([x], [0.3], [0.0], [0.0], [0.0], key, gx, 5, dx, 0.25, 1.0, 1.0, 1.0, 1.0)
+    assert not fm.any()
+
+
+def test_deposit_total_and_uniform_lattice(oracle_mod):
+    """Sum over nodes of final_mesh = W*dx*dy*N; a lattice of particles at the
+    same offset in every cell gives every interior node the same density."""
+    O = oracle_mod
+    NX = NY = 16
+    gx = gy = NX + 1
+    dx, dy, W = 1e-4, 1e-4, 3.0
+    jj, ii = np.meshgrid(np.arange(NY), np.arange(NX), indexing="ij")
+    x = ((ii + 0.3) * dx).ravel()
+    y = ((jj + 0.6) * dy).ravel()
+    key = O.keys(x, y, gx, dx, dy)
+    z = np.zeros_like(x)
+    fm, _, _ = O.deposit_ref(x, y, z, z, z, key, gx, gy, dx, dy, W, 1.0, 1.0, 1.0)
+    assert fm.sum() == pytest.approx(W * dx * dy * x.size, rel=1e-13)
+    inner = fm.reshape(gy, gx)[1:-1, 1:-1]
+    assert np.allclose(inner, W * dx * dy, rtol=1e-12)
+
+
+def test_fixed_point_deposit_bound(oracle_mod):
+    """The int64 fixed-point deposit differs from the FP64 one by at most
+    one half unit (2^-(S+1)) per contribution."""
+    O = oracle_mod
+    rng = np.random.default_rng(11)
+    NX = NY = 32
+    gx = gy = NX + 1
+    dx = dy = 1e-4
+    n = 40000
+    x = rng.uniform(0, NX * dx, n)
+    y = rng.uniform(0, NY * dy, n)
+    v = rng.normal(0, 1e5, (3, n))
+    key = O.keys(x, y, gx, dx, dy)
+    qm = 1.7e11
+    for S in (32, 40):
+        qc, qe = O.deposit_fixed_f64(x, y, *v, key, gx, gy, dx, dy, S, 32, qm)
+        fm, _, _ = O.deposit_ref(x, y, *v, key, gx, gy, dx, dy, 1.0, qm, 1.0, 1.0)
+        fixed = qc * (dx * dy) * 2.0**-S
+        ncontrib = np.zeros(gx * gy)
+        kk = key[(key % gx < gx - 1) & (key // gx < gy - 1)]
+        for off in (0, 1, gx, gx + 1):
+            np.add.at(ncontrib, kk + off, 1)
+        bound = ncontrib * 2.0 ** -(S + 1) * dx * dy + 1e-15 * np.abs(fm)
+        assert np.all(np.abs(fixed - fm) <= bound)
+    # charge at 2^-40 meets 1e-12 normwise
+    qc, _ = O.deposit_fixed_f64(x, y, *v, key, gx, gy, dx, dy, 40, 32, qm)
+    assert np.abs(qc * (dx * dy) * 2.0**-40 - fm).max() / np.abs(fm).max() < 1e-12
+
+
+def _uniform_fields(G, ex=0.0, ey=0.0, bz=0.0):
+    return np.full(G, ex), np.full(G, ey), np.full(G, bz)
+
+
+PUSH_GRID = dict(gx=33, gy=33, dx=1e-4, dy=1e-4, width=32e-4, height=32e-4)
+
+
+def test_push_magnetic_rotation_conserves_speed(oracle_mod):
+    """E = 0, uniform Bz: the Boris step is a rotation by 2 atan(t) with
+    t = q B dt / 2m; in-plane speed is conserved and vz is untouched."""
+    O = oracle_mod
+    g = PUSH_GRID
+    G = g["gx"] * g["gy"]
+    rng = np.random.default_rng(5)
+    n = 2000
+    x = rng.uniform(5e-4, 27e-4, n)
+    y = rng.uniform(5e-4, 27e-4, n)
+    vx, vy, vz = rng.normal(0, 1e3, (3, n))
+    qm, dt, B = -1.758820e11, 1e-12, 0.05
+    ex, ey, bz = _uniform_fields(G, bz=B)
+    X, Y, VX, VY, VZ, key, ab = O.push_f64(x, y, vx, vy, vz, ex, ey, bz, qm=qm, dt=dt, **g)
+    assert np.allclose(np.hypot(VX, VY), np.hypot(vx, vy), rtol=1e-13)
+    assert np.array_equal(VZ, vz)
+    t = 0.5 * qm * B * dt
+    ang = np.arctan2(VY, VX) - np.arctan2(vy, vx)
+    ang = (ang + np.pi) % (2 * np.pi) - np.pi
+    assert np.allclose(ang, -2 * np.arctan(t), atol=1e-9)
+    assert np.allclose(X, x + dt * VX, rtol=0, atol=1e-18)
+
+
+def test_push_uniform_e_acceleration(oracle_mod):
+    O = oracle_mod
+    g = PUSH_GRID
+    G = g["gx"] * g["gy"]
+    n = 100
+    x = np.full(n, 10.3e-4)
+    y = np.linspace(3e-4, 29e-4, n)
+    qm, dt, E = 1.5e7, 1e-9, 250.0
+    ex, ey, bz = _uniform_fields(G, ex=E, ey=-2 * E)
+    X, Y, VX, VY, VZ, key, ab = O.push_f64(x, y, np.zeros(n), np.zeros(n), np.zeros(n), ex, ey, bz, qm=qm, dt=dt,
+                                           **g)
+    assert np.allclose(VX, qm * E * dt, rtol=1e-13)
+    assert np.allclose(VY, -2 * qm * E * dt, rtol=1e-13)
+
+
+def test_push_exb_drift(oracle_mod):
+    """Average velocity over many gyro-periods -> E x B / B^2."""
+    O = oracle_mod
+    g = dict(gx=33, gy=33, dx=1.0, dy=1.0, width=32.0, height=32.0)
+    G = g["gx"] * g["gy"]
+    qm, B, E = 1.0, 1.0, 1e-3
+    dt = 0.05
+    ex, ey, bz = _uniform_fields(G, ey=E, bz=B)
+    x, y = np.array([16.0]), np.array([16.0])
+    v = [np.array([0.0]), np.array([0.0]), np.array([0.0])]
+    steps = 4000
+    for _ in range(steps):
+        x, y, v[0], v[1], v[2], key, _ = O.push_f64(x, y, *v, ex, ey, bz, qm=qm, dt=dt, **g)
+    drift = (x[0] - 16.0) / (steps * dt)
+    assert drift == pytest.approx(E / B, rel=2e-2)     # E_y x B_z = +x
+
+
+def test_push_general_b_rotation(oracle_mod):
+    """B along x (tilted path): rotation in the y-z plane, |v| conserved, vx unchanged."""
+    O = oracle_mod
+    g = PUSH_GRID
+    G = g["gx"] * g["gy"]
+    rng = np.random.default_rng(9)
+    n = 500
+    x = rng.uniform(5e-4, 27e-4, n)
+    y = rng.uniform(5e-4, 27e-4, n)
+    vx, vy, vz = rng.normal(0, 1e3, (3, n))
+    ex, ey, bz = _uniform_fields(G)
+    X, Y, VX, VY, VZ, key, ab = O.push_f64(x, y, vx, vy, vz, ex, ey, bz, qm=-1.7e11, dt=1e-12, bx=0.05, **g)
+    assert np.allclose(np.sqrt(VX**2 + VY**2 + VZ**2), np.sqrt(vx**2 + vy**2 + vz**2), rtol=1e-13)
+    assert np.allclose(VX, vx, rtol=1e-14)
+
+
+def test_push_general_path_reduces_to_reference(oracle_mod):
+    """With bx = by = 0 the reference Bz-only expression is used; with a tiny
+    bx the general rotation agrees to rounding."""
+    O = oracle_mod
+    g = PUSH_GRID
+    G = g["gx"] * g["gy"]
+    rng = np.random.default_rng(2)
+    n = 300
+    x = rng.uniform(1e-4, 30e-4, n)
+    y = rng.uniform(1e-4, 30e-4, n)
+    vx, vy, vz = rng.normal(0, 1e4, (3, n))
+    ex, ey = rng.normal(0, 100, (2, G))
+    bz = np.full(G, 0.02)
+    a = O.push_f64(x, y, vx, vy, vz, ex, ey, bz, qm=-1.7e11, dt=1e-12, **g)
+    b = O.push_f64(x, y, vx, vy, vz, ex, ey, bz, qm=-1.7e11, dt=1e-12, bx=1e-30, **g)
+    for u, w in zip(a[:5], b[:5]):
+        assert np.allclose(u, w, rtol=1e-14, atol=0)
+
+
+def test_push_walls_absorb(oracle_mod):
+    """x < 0, x >= W, y < 0, y >= H are absorbed (mover.cpp:325); only x < 0
+    counts as leaving on the left (:344-352)."""
+    O = oracle_mod
+    g = PUSH_GRID
+    G = g["gx"] * g["gy"]
+    dt = 1.0
+    x = np.array([1e-6, 31.5e-4, 10e-4, 10e-4, 10e-4])
+    y = np.array([10e-4, 10e-4, 1e-6, 31.9e-4, 10e-4])
+    vx = np.array([-2e-6, 0.5e-4, 0, 0, 0])
+    vy = np.array([0, 0, -2e-6, 0.1e-4, 0])
+    vz = np.zeros(5)
+    ex, ey, bz = _uniform_fields(G)
+    X, Y, VX, VY, VZ, key, ab = O.push_f64(x, y, vx, vy, vz, ex, ey, bz, qm=1.0, dt=dt, **g)
+    assert list(key[:4]) == [-1, -1, -1, -1] and key[4] >= 0
+    assert list(ab) == [1, 4]
+    # exactly on the far wall is outside
+    g1 = dict(gx=33, gy=33, dx=1.0, dy=1.0, width=32.0, height=32.0)
+    X, Y, VX, VY, VZ, key, ab = O.push_f64(np.array([31.0]), np.array([5.0]), np.array([1.0]), np.zeros(1),
+                                           np.zeros(1), ex, ey, bz, qm=1.0, dt=1.0, **g1)
+    assert X[0] == g1["width"] and key[0] == -1 and list(ab) == [0, 1]
+
+
+def test_cell32_encode_decode(oracle_mod):
+    """CELL32 encoding: cell = reference key, offsets in [0,1), and the decode
+    (gx + ox) * dx lies within one float ulp of a cell of the original x."""
+    O = oracle_mod
+    rng = np.random.default_rng(4)
+    NX, dx = 64, 1e-4
+    x = rng.uniform(0, NX * dx, 5000)
+    y = rng.uniform(0, NX * dx, 5000)
+    z = np.zeros_like(x)
+    cell, ox, oy, *_ = O.encode_cell32(x, y, z, z, z, NX + 1, dx, dx)
+    assert np.array_equal(cell, O.keys(x, y, NX + 1, dx, dx))
+    assert np.all((ox >= 0) & (ox < 1) & (oy >= 0) & (oy < 1))
+    xd = ((cell % (NX + 1)) + ox.astype(np.float64)) * dx
+    assert np.abs(xd - x).max() < 1e-7 * dx
