@@ -91,10 +91,11 @@ extern "C" void pic_destroy(pic_ctx* c) {
     for (auto& s : c->sp) free_species(s);
     poisson_free(c);
     void* ptrs[] = {c->rho, c->phi, c->ex, c->ey, c->bz, c->fld, c->dir, c->dval, c->pr, c->pp, c->pq,
-                    c->pz, c->pzold, c->pb, c->guess, c->red, c->scal, c->red_count, c->scan_bsum};
+                    c->pz, c->pzold, c->pb, c->guess, c->part, c->scal, c->flags, c->scan_bsum};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c->scal_host) (void)hipHostFree(c->scal_host);
+    if (c->flags_host) (void)hipHostFree(c->flags_host);
     for (auto& t : c->timers)
         for (auto& e : t.pending) {
             (void)hipEventDestroy(e.first);
@@ -185,7 +186,11 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
     c->g.dy = p->height / p->NY;
     c->g.inv_dx = 1.0 / c->g.dx;
     c->g.inv_dy = 1.0 / c->g.dy;
+    c->g.nx = p->NX;
+    c->g.ny = p->NY;
+    c->g.rcp_gx = 1.0f / (float)c->g.gx;
     c->G = c->g.gx * c->g.gy;
+    c->g.gp = c->G + c->g.gx + 2;
     const size_t G = c->G;
     auto fail = [&](int rc) { pic_destroy(c); return rc; };
 #define ALLOC(ptr, bytes)                                                      \
@@ -215,9 +220,9 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
     ALLOC(c->pzold, G * sizeof(double));
     ALLOC(c->pb, G * sizeof(double));
     ALLOC(c->guess, G * sizeof(double));
-    ALLOC(c->red, 2 * 1024 * sizeof(double));
+    ALLOC(c->part, 8 * 1024 * sizeof(double));
     ALLOC(c->scal, 16 * sizeof(double));
-    ALLOC(c->red_count, sizeof(unsigned));
+    ALLOC(c->flags, 4 * sizeof(int));
     c->scan_nb = (int)((G + 2047) / 2048);
     if (c->scan_nb > 8192) {
         set_error("pic_create: grid too large for the two-level scan");
@@ -225,7 +230,8 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
     }
     ALLOC(c->scan_bsum, c->scan_nb * sizeof(int32_t));
 #undef ALLOC
-    if (hipHostMalloc((void**)&c->scal_host, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&c->scal_host, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&c->flags_host, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
         set_error("pic_create: hipHostMalloc failed");
         return fail(PIC_ERR_ALLOC);
     }
@@ -234,7 +240,8 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
     (void)hipMemset(c->ex, 0, G * sizeof(double));
     (void)hipMemset(c->ey, 0, G * sizeof(double));
     (void)hipMemset(c->fld, 0, pad * sizeof(double4));
-    (void)hipMemset(c->red_count, 0, sizeof(unsigned));
+    (void)hipMemset(c->flags, 0, 4 * sizeof(int));
+    (void)hipMemset(c->part, 0, 8 * 1024 * sizeof(double));
     (void)hipMemset(c->scal, 0, 16 * sizeof(double));
     {
         std::vector<double> b(G, p->b0);   // initMagneticField: uniform B0 (particles.cpp:709-719)
@@ -375,8 +382,9 @@ extern "C" int pic_add_species(pic_ctx* c, double charge, double mass, double si
     ok = ok && hipMalloc((void**)&s.rank, n * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s.counts, G * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s.start, (G + 1) * sizeof(int32_t)) == hipSuccess;
-    ok = ok && hipMalloc((void**)&s.q_charge, G * sizeof(int64_t)) == hipSuccess;
-    ok = ok && hipMalloc((void**)&s.q_energy, G * sizeof(int64_t)) == hipSuccess;
+    const size_t nacc = 4 * (size_t)c->g.gp;   // four node-indexed corner meshes
+    ok = ok && hipMalloc((void**)&s.q_charge, nacc * sizeof(int64_t)) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s.q_energy, nacc * sizeof(int64_t)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s.final_mesh, G * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s.energy_mesh, G * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s.absorbed, 2 * sizeof(unsigned long long)) == hipSuccess;
@@ -388,8 +396,8 @@ extern "C" int pic_add_species(pic_ctx* c, double charge, double mass, double si
     (void)hipMemset(s.ndev, 0, 2 * sizeof(int32_t));
     (void)hipMemset(s.counts, 0, G * sizeof(int32_t));
     (void)hipMemset(s.start, 0, (G + 1) * sizeof(int32_t));
-    (void)hipMemset(s.q_charge, 0, G * sizeof(int64_t));
-    (void)hipMemset(s.q_energy, 0, G * sizeof(int64_t));
+    (void)hipMemset(s.q_charge, 0, nacc * sizeof(int64_t));
+    (void)hipMemset(s.q_energy, 0, nacc * sizeof(int64_t));
     (void)hipMemset(s.final_mesh, 0, G * sizeof(double));
     (void)hipMemset(s.energy_mesh, 0, G * sizeof(double));
     (void)hipMemset(s.absorbed, 0, 2 * sizeof(unsigned long long));
@@ -472,6 +480,7 @@ extern "C" int pic_upload_cell32(pic_ctx* c, int s, int64_t n, const int32_t* ce
         PIC_HIP(hipMemcpyAsync(b.vz, vz, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     }
     PIC_TRY(launch_key_from_cell(c, s, n));
+    sp.deposit_pending = 0;
     PIC_TRY(launch_bin(c, s));
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
@@ -600,8 +609,15 @@ extern "C" int pic_deposit(pic_ctx* c, int s) {
 extern "C" int pic_get_deposit_fixed(pic_ctx* c, int s, int64_t* qc, int64_t* qe) {
     PIC_TRY(check_species(c, s, "pic_get_deposit_fixed"));
     PIC_HIP(hipStreamSynchronize(c->stream));
-    if (qc) PIC_HIP(hipMemcpy(qc, c->sp[s].q_charge, c->G * sizeof(int64_t), hipMemcpyDeviceToHost));
-    if (qe) PIC_HIP(hipMemcpy(qe, c->sp[s].q_energy, c->G * sizeof(int64_t), hipMemcpyDeviceToHost));
+    const size_t gp = c->g.gp;
+    std::vector<int64_t> tmp(4 * gp);
+    int64_t* outs[2] = {qc, qe};
+    int64_t* srcs[2] = {c->sp[s].q_charge, c->sp[s].q_energy};
+    for (int a = 0; a < 2; ++a) {
+        if (!outs[a]) continue;
+        PIC_HIP(hipMemcpy(tmp.data(), srcs[a], 4 * gp * sizeof(int64_t), hipMemcpyDeviceToHost));
+        for (int n = 0; n < c->G; ++n) outs[a][n] = tmp[n] + tmp[gp + n] + tmp[2 * gp + n] + tmp[3 * gp + n];
+    }
     return PIC_OK;
 }
 
@@ -640,8 +656,10 @@ extern "C" int pic_update_fields_from_phi(pic_ctx* c) {
 
 extern "C" int pic_push(pic_ctx* c, int s, int rebin) {
     PIC_TRY(check_species(c, s, "pic_push"));
-    PIC_TRY(launch_push(c, s));
-    if (rebin) PIC_TRY(launch_bin(c, s));
+    if (rebin)
+        PIC_TRY(launch_move(c, s));
+    else
+        PIC_TRY(launch_push(c, s));
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }
@@ -655,10 +673,7 @@ extern "C" int pic_step(pic_ctx* c, int iteration, pic_solve_stats* st) {
     for (int s = 0; s < (int)c->sp.size(); ++s) PIC_TRY(launch_deposit(c, s));
     PIC_TRY(poisson_solve(c, iteration, st));
     PIC_TRY(launch_efield(c));
-    for (int s = 0; s < (int)c->sp.size(); ++s) {
-        PIC_TRY(launch_push(c, s));
-        PIC_TRY(launch_bin(c, s));
-    }
+    for (int s = 0; s < (int)c->sp.size(); ++s) PIC_TRY(launch_move(c, s));
     return PIC_OK;
 }
 

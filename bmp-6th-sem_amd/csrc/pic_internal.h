@@ -35,9 +35,12 @@ void set_error(const char* fmt, ...);
 // Everything a particle kernel needs about the grid, passed by value.
 struct GridDev {
     int    gx, gy;          // GRID_X, GRID_Y (nodes)
+    int    gp;              // stride of one corner mesh: G + GRID_X + 2
+    float  rcp_gx;          // 1 / GRID_X for the key -> (gx, gy) split
     double dx, dy;
     double width, height;   // GRID_WIDTH, GRID_HEIGHT
     double inv_dx, inv_dy;  // 1.0 / dx, 1.0 / dy as the reference computes them
+    double nx, ny;          // NX, NY (cells): the walls in cell units
 };
 
 // Per-species constants for the particle kernels.
@@ -46,6 +49,7 @@ struct SpeciesDev {
     double half_qm;   // 0.5 * q/m as the reference groups it: (0.5 * pctm)
     double denom;     // 1 / |q/m| for the energy weight
     double dt;
+    double dtdx, dtdy; // dt / dx, dt / dy (cell-unit position update)
     double bx, by;    // uniform in-plane B (0 = reference Bz-only push)
     int    general_b; // 1 when bx or by is non-zero
 };
@@ -71,11 +75,13 @@ struct Species {
     int32_t* rank = nullptr;  // per-particle slot inside its cell
     int32_t* counts = nullptr;// GRID_X*GRID_Y
     int32_t* start = nullptr; // GRID_X*GRID_Y + 1 (start[G] = live count)
-    int64_t* q_charge = nullptr; // fixed-point accumulators (GRID_X*GRID_Y)
+    int64_t* q_charge = nullptr; // fixed-point corner meshes: 4 x (G + GRID_X + 2) int64
     int64_t* q_energy = nullptr;
     double*  final_mesh = nullptr;
     double*  energy_mesh = nullptr;
     unsigned long long* absorbed = nullptr; // [0] left through x<0, [1] all absorbed
+    int deposit_pending = 0;  // q_charge/q_energy hold the deposit of the stored particles
+    int sorted = 0;           // storage is cell-sorted and `start` describes it
 };
 
 // One multigrid level (FP32 storage).
@@ -123,10 +129,11 @@ struct pic_ctx {
     int guess_set = 0;
     int have_prev_phi = 0;
     std::vector<pic::MgLevel> mg;
-    double* red = nullptr;          // reduction partials
+    double* part = nullptr;         // per-block partial sums of the dot products
     double* scal = nullptr;         // device scalars
-    unsigned int* red_count = nullptr;
+    int* flags = nullptr;           // [0] converged / failed, [1] iterations
     double* scal_host = nullptr;    // pinned
+    int* flags_host = nullptr;      // pinned
     int scan_nb = 0;
     int32_t* scan_bsum = nullptr;
     // timing
@@ -142,6 +149,7 @@ int launch_encode_f64(pic_ctx* c, int s, int64_t n, const double* x, const doubl
 int launch_bin(pic_ctx* c, int s);
 int launch_deposit(pic_ctx* c, int s);
 int launch_push(pic_ctx* c, int s);
+int launch_move(pic_ctx* c, int s);
 int launch_zero_f64(pic_ctx* c, double* p, int64_t n);
 int launch_decode(pic_ctx* c, int s, double* x, double* y, double* vx, double* vy, double* vz);
 

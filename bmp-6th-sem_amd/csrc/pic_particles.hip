@@ -1,10 +1,19 @@
 // pic_particles.hip — gfx950 kernels for the particle stages of the PIC step:
-// encode/bin (counting sort), CIC deposit (int64 fixed point), gather + Boris
-// push + wall absorption + re-bin key.
+// encode/bin (counting sort), CIC deposit (int64 fixed point, per-cell corner
+// sums), gather + Boris push + wall absorption + re-bin.
 //
 // Floating-point contraction is disabled for this file (-ffp-contract=off in
-// the build) so that every FP64 expression below rounds exactly as the
-// reference's C++ does on the host; the oracle is compiled the same way.
+// the build) so that every FP64 expression rounds exactly as written; the
+// oracle is compiled the same way.
+//
+// Storage is cell-sorted structure-of-arrays; `start[k]` is the first slot of
+// key k.  Two layouts:
+//   F64    — x, y, vx, vy, vz as double; the push is the reference's FP64
+//            expression sequence (mover.cpp:103-176) bit for bit.
+//   CELL32 — int32 key + float in-cell offsets (units of dx, dy) + float
+//            velocities; the push runs in FP64 in cell units: bilinear weights
+//            from the offsets, the reference's Boris step, and the new position
+//            t = (cell + offset) + v * dt / dx.
 #include "pic_internal.h"
 
 #include <cmath>
@@ -27,16 +36,20 @@ __device__ __forceinline__ float clamp_offset(float o) {
     return o < 1.0f ? o : 0x1.fffffep-1f;
 }
 
-// Reference bin key (particles.cpp:229-235, mover.cpp:323-324,356):
-// gx = (int)(x * inv_dx), gy = (int)(y * inv_dy), key = gy * GRID_X + gx.
-__device__ __forceinline__ int ref_key(double x, double y, const GridDev& g) {
-    int gx = (int)(x * g.inv_dx);
-    int gy = (int)(y * g.inv_dy);
-    return gy * g.gx + gx;
+// key -> (key % GRID_X, key / GRID_X) exactly: float quotient estimate
+// (within one of the true quotient for keys < 2^24) plus an integer fix-up.
+__device__ __forceinline__ void split_key(int k, const GridDev& g, int& kx, int& ky) {
+    int q = (int)((float)k * g.rcp_gx);
+    int r = k - q * g.gx;
+    if (r < 0) { q -= 1; r += g.gx; }
+    else if (r >= g.gx) { q += 1; r -= g.gx; }
+    kx = r;
+    ky = q;
 }
 
 // ================================================================ encode
-// Upload path: absolute FP64 particles -> storage layout + key.
+// Upload path: absolute FP64 particles -> storage layout + reference key
+// (int)(x * inv_dx), (int)(y * inv_dy) (particles.cpp:229-235).
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_encode(
     int n, const double* __restrict__ x, const double* __restrict__ y,
@@ -73,8 +86,8 @@ __global__ void k_copy_i32(int n, const int32_t* __restrict__ a, int32_t* __rest
 __global__ void k_set_i32(int32_t* p, int v) { *p = v; }
 
 // ================================================================== bin
-// Counting sort.  count: per-particle slot inside its cell; scan: exclusive
-// prefix of the counts (and reset of the counts); scatter: copy to the slot.
+// Counting sort of key[] (-1 = absorbed): count (slot inside the cell),
+// exclusive scan of the counts (which also resets them), scatter.
 __global__ __launch_bounds__(kBlock) void k_count(
     const int32_t* __restrict__ nsrc, const int32_t* __restrict__ key,
     int32_t* __restrict__ counts, int32_t* __restrict__ rank)
@@ -194,12 +207,13 @@ __global__ __launch_bounds__(kBlock) void k_scatter(
 }
 
 // ============================================================== deposit
-// CIC weights normalised by dx*dy, converted to int64 at 2^charge_shift and
-// added with 64-bit integer atomics: the sum is independent of order.
-// Weights follow interpolate (interpolation.cpp:72-98): for the particle's
-// key cell (gx, gy) = (key % GRID_X, key / GRID_X), lx = x - gx*dx,
+// CIC weights normalised by dx*dy, rounded to int64 at 2^charge_shift
+// (energy weights at 2^energy_shift eV) and summed as integers: the result
+// does not depend on summation order.  Weights follow interpolate
+// (interpolation.cpp:72-98): for key cell (gx, gy), lx = x - gx*dx,
 // ly = y - gy*dy; corners (gx,gy) (dx-lx)(dy-ly), (gx+1,gy) lx(dy-ly),
-// (gx,gy+1) (dx-lx)ly, (gx+1,gy+1) lx*ly.  Cells on the last node row or
+// (gx,gy+1) (dx-lx)ly, (gx+1,gy+1) lx*ly; CELL32 uses the stored offsets
+// (1-ox)(1-oy), ox(1-oy), (1-ox)oy, ox*oy.  Cells on the last node row or
 // column deposit nothing (InterReductionKernel returns, :122).
 struct DepositArgs {
     double inv_area;   // 1 / (dx*dy)
@@ -230,36 +244,132 @@ __device__ __forceinline__ void cic_weights(const ParticleBuf& b, int i, int gxk
 }
 
 template <int L>
-__global__ __launch_bounds__(kBlock) void k_deposit(
-    const int32_t* __restrict__ nsrc, ParticleBuf b, GridDev g, DepositArgs a,
-    unsigned long long* __restrict__ qc, unsigned long long* __restrict__ qe)
-{
+__device__ __forceinline__ double particle_energy(const ParticleBuf& b, int i, const DepositArgs& a) {
     using V = typename Lay<L>::V;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *nsrc) return;
-    int k = b.cell[i];
-    int gxk = k % g.gx, gyk = k / g.gx;
-    if (gxk + 1 >= g.gx || gyk + 1 >= g.gy) return;
-    double w[4];
-    cic_weights<L>(b, i, gxk, gyk, g, a, w);
-    int node[4] = {k, k + 1, k + g.gx, k + g.gx + 1};
+    double pvx = (double)((const V*)b.vx)[i];
+    double pvy = (double)((const V*)b.vy)[i];
+    double pvz = (double)((const V*)b.vz)[i];
+    double v2 = pvx * pvx + pvy * pvy + pvz * pvz;
+    return 0.5 * v2 * a.denom;
+}
+
+// One workgroup per tile of kDepC consecutive cells of one key row (the
+// cells' particles are contiguous).  Lanes take consecutive particles
+// (coalesced loads); a wave-level segmented scan over the sorted keys sums
+// each cell's corner weights, and the last lane of each cell segment adds the
+// sums to the tile's LDS cell accumulators.  The tile then writes every
+// cell's four corner sums to the node-indexed corner meshes — the reference's
+// per-cell reduction and corner meshes (:56-148): corner c of cell k goes to
+// node k, k+1, k+GX, k+GX+1.  Every cell is written by exactly one tile, so no
+// global atomics; corner-mesh entries that no cell writes stay zero.
+static constexpr int kDepC = 256;
+
+__device__ __forceinline__ long long seg_scan_step(long long v, int d, int lane, int head) {
+    long long t = __shfl_up(v, d, 64);
+    return (lane - d >= head) ? v + t : v;
+}
+
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_deposit_tile(
+    ParticleBuf b, GridDev g, DepositArgs a, int tiles_per_row, const int32_t* __restrict__ start,
+    long long* __restrict__ cq, long long* __restrict__ ce)
+{
+    __shared__ long long acc[8][kDepC];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int r = blockIdx.x / tiles_per_row;
+    const int c0 = (blockIdx.x % tiles_per_row) * kDepC;
+    const int cw = min(kDepC, g.gx - c0);
+    const int k0 = r * g.gx + c0;
+    const int nv = a.energy ? 8 : 4;
+    for (int j = tid; j < 8 * kDepC; j += kBlock) acc[j / kDepC][j % kDepC] = 0;
+    __syncthreads();
+    const bool live_row = r + 1 < g.gy;
+    const int beg = start[k0], end = start[k0 + cw];
+    const unsigned long long le_mask = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    if (live_row) {
+        for (int base = beg; base < end; base += kBlock) {
+            const int i = base + tid;
+            int lc = -1;
+            long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (i < end) {
+                const int kx = b.cell[i] - r * g.gx;
+                lc = kx - c0;
+                if (kx + 1 < g.gx) {
+                    double w[4];
+                    cic_weights<L>(b, i, kx, r, g, a, w);
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-        atomicAdd(qc + node[c], (unsigned long long)__double2ll_rn(w[c] * a.scale_q));
-    if (a.energy) {
-        double pvx = (double)((const V*)b.vx)[i];
-        double pvy = (double)((const V*)b.vy)[i];
-        double pvz = (double)((const V*)b.vz)[i];
-        double v2 = pvx * pvx + pvy * pvy + pvz * pvz;
-        double en = 0.5 * v2 * a.denom;
+                    for (int c = 0; c < 4; ++c) v[c] = __double2ll_rn(w[c] * a.scale_q);
+                    if (a.energy) {
+                        double en = particle_energy<L>(b, i, a);
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
-            atomicAdd(qe + node[c], (unsigned long long)__double2ll_rn((w[c] * en) * a.scale_e));
+                        for (int c = 0; c < 4; ++c) v[4 + c] = __double2ll_rn((w[c] * en) * a.scale_e);
+                    }
+                }
+            }
+            // segment heads: first lane of each run of equal cells
+            const int prev = __shfl_up(lc, 1, 64);
+            const unsigned long long heads = __ballot(lane == 0 || lc != prev);
+            const int head = 63 - __clzll(heads & le_mask);
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (c < nv) v[c] = seg_scan_step(v[c], d, lane, head);
+            }
+            const bool tail = (lane == 63) || ((heads >> (lane + 1)) & 1ull);
+            if (tail && lc >= 0) {
+                for (int c = 0; c < nv; ++c)
+                    if (v[c]) atomicAdd((unsigned long long*)&acc[c][lc], (unsigned long long)v[c]);
+            }
+        }
+    }
+    __syncthreads();
+    const int gp = g.gp;
+    for (int c = tid; c < cw; c += kBlock) {
+        const int k = k0 + c;
+        cq[k] = acc[0][c];
+        cq[gp + k + 1] = acc[1][c];
+        cq[2 * gp + k + g.gx] = acc[2][c];
+        cq[3 * gp + k + g.gx + 1] = acc[3][c];
+        if (a.energy) {
+            ce[k] = acc[4][c];
+            ce[gp + k + 1] = acc[5][c];
+            ce[2 * gp + k + g.gx] = acc[6][c];
+            ce[3 * gp + k + g.gx + 1] = acc[7][c];
+        }
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_zero_u64(unsigned long long* p, int n) {
+// Storage not cell-sorted (after an in-place push): per-particle 64-bit
+// integer atomics into corner mesh 0 (the other three were zeroed).
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_deposit_atomic(
+    const int32_t* __restrict__ nsrc, ParticleBuf b, GridDev g, DepositArgs a,
+    unsigned long long* __restrict__ cq, unsigned long long* __restrict__ ce)
+{
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *nsrc) return;
+    int k = b.cell[i];
+    if (k < 0) return;
+    int kx, ky;
+    split_key(k, g, kx, ky);
+    if (kx + 1 >= g.gx || ky + 1 >= g.gy) return;
+    double w[4];
+    cic_weights<L>(b, i, kx, ky, g, a, w);
+    int node[4] = {k, k + 1, k + g.gx, k + g.gx + 1};
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        atomicAdd(cq + node[c], (unsigned long long)__double2ll_rn(w[c] * a.scale_q));
+    if (a.energy) {
+        double en = particle_energy<L>(b, i, a);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            atomicAdd(ce + node[c], (unsigned long long)__double2ll_rn((w[c] * en) * a.scale_e));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_u64(unsigned long long* p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
 }
 
@@ -268,66 +378,50 @@ __global__ __launch_bounds__(kBlock) void k_zero_f64(double* p, int64_t n) {
     if (i < n) p[i] = 0.0;
 }
 
-// final_mesh = W * dx*dy * 2^-S * Q; rho += final_mesh * (-q/eps)  (RhoKernel :165-171)
+// Node sum of the corner meshes (ReducePartialMeshesKernel :150-163),
+// final_mesh = W * dx*dy * 2^-S * Q, rho += final_mesh * (-q/eps)
+// (RhoKernel :165-171).
 __global__ __launch_bounds__(kBlock) void k_deposit_finalize(
-    int G, const long long* __restrict__ qc, const long long* __restrict__ qe,
-    double cq, double ce, double chrg, int energy,
+    int G, int gp, const long long* __restrict__ cq, const long long* __restrict__ ce,
+    double cqf, double cef, double chrg, int energy,
     double* __restrict__ final_mesh, double* __restrict__ energy_mesh, double* __restrict__ rho)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= G) return;
-    double f = (double)qc[n] * cq;
+    long long q = cq[n] + cq[gp + n] + cq[2 * gp + n] + cq[3 * gp + n];
+    double f = (double)q * cqf;
     final_mesh[n] = f;
     rho[n] += f * chrg;
-    if (energy) energy_mesh[n] = (double)qe[n] * ce;
+    if (energy) {
+        long long e = ce[n] + ce[gp + n] + ce[2 * gp + n] + ce[3 * gp + n];
+        energy_mesh[n] = (double)e * cef;
+    }
 }
 
 // ================================================================= push
-// Gather (deterministic_mover, mover.cpp:103-133): gx = (int)(x/dx) — a
+// F64 gather (deterministic_mover, mover.cpp:103-133): gx = (int)(x/dx) — a
 // division, unlike the bin key — lx = x - gx*dx, bilinear weights over
 // (dx*dy).  The field arrays are padded with zero nodes so the reference's
 // flat indexing (gy*GRID_X + gx + 1 etc.) stays in bounds at the last node.
 // Boris (mover.cpp:151-166) for Bz; when bx or by is non-zero the same
 // rotation is done with the full vector t = (q/m) B dt / 2.
 // Boundary (mover.cpp:325,331-352): absorbed when x >= W, x < 0, y >= H or
-// y < 0; particles leaving through x < 0 are counted separately.
-template <int L, int GENB>
-__global__ __launch_bounds__(kBlock) void k_push(
-    const int32_t* __restrict__ nsrc, ParticleBuf b, GridDev g, SpeciesDev s,
-    const double4* __restrict__ fld, int32_t* __restrict__ key,
-    unsigned long long* __restrict__ absorbed)
-{
+// y < 0 (CELL32: the same walls in cell units); particles leaving through
+// x < 0 are counted separately.
+
+// Particle state after the push, in storage form.
+template <int L> struct Pushed {
     using P = typename Lay<L>::P;
     using V = typename Lay<L>::V;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *nsrc) return;
+    int key;            // new bin key, -1 when absorbed
+    bool left;          // absorbed through x < 0
+    P a, b;             // F64: x, y   CELL32: offsets
+    V vx, vy, vz;
+};
 
-    double px, py;
-    if (L == PIC_LAYOUT_F64) {
-        px = ((const double*)b.pa)[i];
-        py = ((const double*)b.pb)[i];
-    } else {
-        int c = b.cell[i];
-        int cgx = c % g.gx, cgy = c / g.gx;
-        px = ((double)cgx + (double)((const float*)b.pa)[i]) * g.dx;
-        py = ((double)cgy + (double)((const float*)b.pb)[i]) * g.dy;
-    }
-    double pvx = (double)((const V*)b.vx)[i];
-    double pvy = (double)((const V*)b.vy)[i];
-    double pvz = (double)((const V*)b.vz)[i];
-
-    int gx = (int)(px / g.dx);
-    int gy = (int)(py / g.dy);
-    double lx = px - gx * g.dx;
-    double ly = py - gy * g.dy;
-    int i00 = gy * g.gx + gx;
-    double4 f00 = fld[i00], f10 = fld[i00 + 1], f01 = fld[i00 + g.gx], f11 = fld[i00 + g.gx + 1];
-    double wx0 = g.dx - lx, wy0 = g.dy - ly;
-    double area = g.dx * g.dy;
-    double Epx = (f00.x * wx0 * wy0 + f10.x * lx * wy0 + f01.x * wx0 * ly + f11.x * lx * ly) / area;
-    double Epy = (f00.y * wx0 * wy0 + f10.y * lx * wy0 + f01.y * wx0 * ly + f11.y * lx * ly) / area;
-    double Bpz = (f00.z * wx0 * wy0 + f10.z * lx * wy0 + f01.z * wx0 * ly + f11.z * lx * ly) / area;
-
+template <int GENB>
+__device__ __forceinline__ void boris(double Epx, double Epy, double Bpz, const SpeciesDev& s,
+                                      double& pvx, double& pvy, double& pvz) {
     double hx = s.half_qm * Epx * s.dt;     // 0.5 * pctm * Epx * dt
     double hy = s.half_qm * Epy * s.dt;
     double Vxminus = pvx + hx;
@@ -353,29 +447,206 @@ __global__ __launch_bounds__(kBlock) void k_push(
         pvy = Vyminus + (vpz * sx - vpx * sz) + hy;
         pvz = Vzminus + (vpx * sy - vpy * sx);
     }
-    px = px + s.dt * pvx;
-    py = py + s.dt * pvy;
+}
 
-    bool out = (px >= g.width || px < 0.0 || py >= g.height || py < 0);
-    if (out) {
-        key[i] = -1;
-        if (px < 0.0) atomicAdd(absorbed, 1ull);
+template <int L, int GENB>
+__device__ __forceinline__ Pushed<L> push_one(const ParticleBuf& b, int i, const GridDev& g, const SpeciesDev& s,
+                                             const double4* __restrict__ fld) {
+    using P = typename Lay<L>::P;
+    using V = typename Lay<L>::V;
+    double pvx = (double)((const V*)b.vx)[i];
+    double pvy = (double)((const V*)b.vy)[i];
+    double pvz = (double)((const V*)b.vz)[i];
+    Pushed<L> o;
+    if (L == PIC_LAYOUT_F64) {
+        double px = ((const double*)b.pa)[i];
+        double py = ((const double*)b.pb)[i];
+        int gx = (int)(px / g.dx);
+        int gy = (int)(py / g.dy);
+        double lx = px - gx * g.dx;
+        double ly = py - gy * g.dy;
+        int i00 = gy * g.gx + gx;
+        double4 f00 = fld[i00], f10 = fld[i00 + 1], f01 = fld[i00 + g.gx], f11 = fld[i00 + g.gx + 1];
+        double wx0 = g.dx - lx, wy0 = g.dy - ly;
+        double area = g.dx * g.dy;
+        double Epx = (f00.x * wx0 * wy0 + f10.x * lx * wy0 + f01.x * wx0 * ly + f11.x * lx * ly) / area;
+        double Epy = (f00.y * wx0 * wy0 + f10.y * lx * wy0 + f01.y * wx0 * ly + f11.y * lx * ly) / area;
+        double Bpz = (f00.z * wx0 * wy0 + f10.z * lx * wy0 + f01.z * wx0 * ly + f11.z * lx * ly) / area;
+        boris<GENB>(Epx, Epy, Bpz, s, pvx, pvy, pvz);
+        px = px + s.dt * pvx;
+        py = py + s.dt * pvy;
+        bool out = (px >= g.width || px < 0.0 || py >= g.height || py < 0);
+        o.left = px < 0.0;
+        if (out) {
+            o.key = -1;
+            return o;
+        }
+        int ngx = (int)(px * g.inv_dx), ngy = (int)(py * g.inv_dy);
+        o.key = ngy * g.gx + ngx;
+        o.a = (P)px;
+        o.b = (P)py;
+    } else {
+        int c = b.cell[i];
+        int cgx, cgy;
+        split_key(c, g, cgx, cgy);
+        double fx = (double)((const float*)b.pa)[i];
+        double fy = (double)((const float*)b.pb)[i];
+        double w00 = (1.0 - fx) * (1.0 - fy), w10 = fx * (1.0 - fy);
+        double w01 = (1.0 - fx) * fy, w11 = fx * fy;
+        double4 f00 = fld[c], f10 = fld[c + 1], f01 = fld[c + g.gx], f11 = fld[c + g.gx + 1];
+        double Epx = f00.x * w00 + f10.x * w10 + f01.x * w01 + f11.x * w11;
+        double Epy = f00.y * w00 + f10.y * w10 + f01.y * w01 + f11.y * w11;
+        double Bpz = f00.z * w00 + f10.z * w10 + f01.z * w01 + f11.z * w11;
+        boris<GENB>(Epx, Epy, Bpz, s, pvx, pvy, pvz);
+        double tx = ((double)cgx + fx) + pvx * s.dtdx;
+        double ty = ((double)cgy + fy) + pvy * s.dtdy;
+        bool out = (tx >= g.nx || tx < 0.0 || ty >= g.ny || ty < 0.0);
+        o.left = tx < 0.0;
+        if (out) {
+            o.key = -1;
+            return o;
+        }
+        int ngx = (int)tx, ngy = (int)ty;
+        o.key = ngy * g.gx + ngx;
+        o.a = clamp_offset((float)(tx - (double)ngx));
+        o.b = clamp_offset((float)(ty - (double)ngy));
+    }
+    o.vx = (V)pvx;
+    o.vy = (V)pvy;
+    o.vz = (V)pvz;
+    return o;
+}
+
+template <int L>
+__device__ __forceinline__ void store_state(const ParticleBuf& d, int j, const Pushed<L>& o) {
+    using P = typename Lay<L>::P;
+    using V = typename Lay<L>::V;
+    ((P*)d.pa)[j] = o.a;
+    ((P*)d.pb)[j] = o.b;
+    ((V*)d.vx)[j] = o.vx;
+    ((V*)d.vy)[j] = o.vy;
+    ((V*)d.vz)[j] = o.vz;
+}
+
+// In-place push without re-binning (stage API pic_push(ctx, s, 0)); the
+// stored cell becomes the new key (-1 when absorbed).
+template <int L, int GENB>
+__global__ __launch_bounds__(kBlock) void k_push(
+    const int32_t* __restrict__ nsrc, ParticleBuf b, GridDev g, SpeciesDev s,
+    const double4* __restrict__ fld, int32_t* __restrict__ key,
+    unsigned long long* __restrict__ absorbed)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *nsrc) return;
+    Pushed<L> o = push_one<L, GENB>(b, i, g, s, fld);
+    key[i] = o.key;
+    b.cell[i] = o.key;
+    if (o.key < 0) {
+        if (o.left) atomicAdd(absorbed, 1ull);
         atomicAdd(absorbed + 1, 1ull);
         return;
     }
-    double tx = px * g.inv_dx, ty = py * g.inv_dy;
-    int ngx = (int)tx, ngy = (int)ty;
-    key[i] = ngy * g.gx + ngx;
-    if (L == PIC_LAYOUT_F64) {
-        ((P*)b.pa)[i] = (P)px;
-        ((P*)b.pb)[i] = (P)py;
-    } else {
-        ((P*)b.pa)[i] = clamp_offset((float)(tx - (double)ngx));
-        ((P*)b.pb)[i] = clamp_offset((float)(ty - (double)ngy));
+    store_state<L>(b, i, o);
+}
+
+// ====================================================== tiled push + bin
+// One workgroup per tile of kTileC consecutive cells of one key row; the
+// tile's particles are the contiguous range [start[k0], start[k0 + w)).  The
+// workgroup pushes them (no state is written) and counts the new keys in an
+// LDS window over key rows r-1..r+1, columns c0-1..c0+kTileC (a particle
+// moves less than a cell per step).  Lanes of a wave with the same window bin
+// are merged with ballots so each bin takes one LDS atomic per wave; the
+// window then reserves each bin's slots with one global atomic, and every
+// particle gets its slot inside its new cell.  Particles that land outside
+// the window take a slot with a global atomic.  k_scatter_push recomputes the
+// identical push and writes each particle to start[key] + slot.
+static constexpr int kTileC = 256;
+static constexpr int kHW = kTileC + 2, kHR = 3;
+static constexpr int kRankBits = 21;            // window bin (< 1024) in bits 21..30
+static constexpr int kFallback = (int)0x80000000;
+
+template <int L, int GENB>
+__global__ __launch_bounds__(kBlock) void k_push_tile(
+    ParticleBuf b, GridDev g, SpeciesDev s, int tiles_per_row, const double4* __restrict__ fld,
+    const int32_t* __restrict__ start, int32_t* __restrict__ counts, int32_t* __restrict__ rank,
+    unsigned long long* __restrict__ absorbed)
+{
+    __shared__ int hist[kHR * kHW];
+    __shared__ unsigned long long nabs[2];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int r = blockIdx.x / tiles_per_row;
+    const int c0 = (blockIdx.x % tiles_per_row) * kTileC;
+    const int cw = min(kTileC, g.gx - c0);
+    const int k0 = r * g.gx + c0;
+    for (int j = tid; j < kHR * kHW; j += kBlock) hist[j] = 0;
+    if (tid < 2) nabs[tid] = 0ull;
+    __syncthreads();
+    const int beg = start[k0], end = start[k0 + cw];
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    // every lane of a wave runs the same number of iterations (ballots below)
+    for (int base = beg; base < end; base += kBlock) {
+        const int i = base + tid;
+        int bin = -1, code = -1;
+        if (i < end) {
+            Pushed<L> o = push_one<L, GENB>(b, i, g, s, fld);
+            if (o.key < 0) {
+                if (o.left) atomicAdd(&nabs[0], 1ull);
+                atomicAdd(&nabs[1], 1ull);
+            } else {
+                int gxn, gyn;
+                split_key(o.key, g, gxn, gyn);
+                const int dr = gyn - (r - 1), dc = gxn - (c0 - 1);
+                if (dr >= 0 && dr < kHR && dc >= 0 && dc < kHW) bin = dr * kHW + dc;
+                else code = kFallback | atomicAdd(&counts[o.key], 1);
+            }
+        }
+        unsigned long long active = __ballot(bin >= 0);
+        while (active) {
+            const int leader = __ffsll((unsigned long long)active) - 1;
+            const int lb = __shfl(bin, leader, 64);
+            const unsigned long long m = __ballot(bin == lb) & active;
+            int wbase = 0;
+            if (lane == leader) wbase = atomicAdd(&hist[lb], (int)__popcll(m));
+            wbase = __shfl(wbase, leader, 64);
+            if ((m >> lane) & 1ull) code = (bin << kRankBits) | (wbase + (int)__popcll(m & lt_mask));
+            active &= ~m;
+        }
+        if (i < end) rank[i] = code;
     }
-    ((V*)b.vx)[i] = (V)pvx;
-    ((V*)b.vy)[i] = (V)pvy;
-    ((V*)b.vz)[i] = (V)pvz;
+    __syncthreads();
+    for (int j = tid; j < kHR * kHW; j += kBlock) {
+        int cnt = hist[j];
+        if (cnt) {
+            int k = (r - 1 + j / kHW) * g.gx + (c0 - 1 + j % kHW);
+            hist[j] = atomicAdd(&counts[k], cnt);
+        }
+    }
+    if (tid == 0) {
+        if (nabs[0]) atomicAdd(absorbed, nabs[0]);
+        if (nabs[1]) atomicAdd(absorbed + 1, nabs[1]);
+    }
+    __syncthreads();
+    for (int i = beg + tid; i < end; i += kBlock) {
+        int v = rank[i];
+        if (v == -1) continue;
+        if (v & kFallback) rank[i] = v & ~kFallback;
+        else rank[i] = hist[v >> kRankBits] + (v & ((1 << kRankBits) - 1));
+    }
+}
+
+// Recompute the push and write each surviving particle to its sorted slot.
+template <int L, int GENB>
+__global__ __launch_bounds__(kBlock) void k_scatter_push(
+    const int32_t* __restrict__ nsrc, ParticleBuf src, ParticleBuf dst, GridDev g, SpeciesDev s,
+    const double4* __restrict__ fld, const int32_t* __restrict__ rank, const int32_t* __restrict__ start)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *nsrc) return;
+    Pushed<L> o = push_one<L, GENB>(src, i, g, s, fld);
+    if (o.key < 0) return;
+    int d = start[o.key] + rank[i];
+    store_state<L>(dst, d, o);
+    dst.cell[d] = o.key;
 }
 
 // ============================================================ launchers
@@ -401,6 +672,7 @@ int launch_encode_f64(pic_ctx* c, int s, int64_t n, const double* x, const doubl
     hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, c->stream, sp.ndev + sp.cur, (int)n);
     PIC_HIP(hipGetLastError());
     sp.n_upper = n;
+    sp.deposit_pending = 0;
     return PIC_OK;
 }
 
@@ -417,6 +689,19 @@ int launch_key_from_cell(pic_ctx* c, int s, int64_t n) {
     return PIC_OK;
 }
 
+static int scan_counts(pic_ctx* c, Species& sp, int dst) {
+    hipLaunchKernelGGL(k_scan_local, dim3(c->scan_nb), dim3(kBlock), 0, c->stream,
+                       sp.counts, c->G, sp.start, c->scan_bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, c->stream,
+                       c->scan_bsum, c->scan_nb, sp.start, c->G, sp.ndev + dst);
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(c->G)), dim3(kBlock), 0, c->stream,
+                       sp.start, c->G, c->scan_bsum);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+// Counting sort of the current storage by sp.key (upload / pic_bin / after an
+// in-place push).
 int launch_bin(pic_ctx* c, int s) {
     Species& sp = c->sp[s];
     hipEvent_t ev;
@@ -428,15 +713,7 @@ int launch_bin(pic_ctx* c, int s) {
                            sp.ndev + src, sp.key, sp.counts, sp.rank);
         PIC_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_scan_local, dim3(c->scan_nb), dim3(kBlock), 0, c->stream,
-                       sp.counts, c->G, sp.start, c->scan_bsum);
-    PIC_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, c->stream,
-                       c->scan_bsum, c->scan_nb, sp.start, c->G, sp.ndev + dst);
-    PIC_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(c->G)), dim3(kBlock), 0, c->stream,
-                       sp.start, c->G, c->scan_bsum);
-    PIC_HIP(hipGetLastError());
+    PIC_TRY(scan_counts(c, sp, dst));
     if (n > 0) {
         if (c->p.layout == PIC_LAYOUT_F64)
             hipLaunchKernelGGL(k_scatter<PIC_LAYOUT_F64>, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream,
@@ -447,64 +724,101 @@ int launch_bin(pic_ctx* c, int s) {
         PIC_HIP(hipGetLastError());
     }
     sp.cur = dst;
+    sp.sorted = 1;
     PIC_TRY(timer_end(c, PIC_STAGE_BIN, ev));
     return PIC_OK;
 }
 
-int launch_deposit(pic_ctx* c, int s) {
-    Species& sp = c->sp[s];
-    hipEvent_t ev;
-    PIC_TRY(timer_begin(c, PIC_STAGE_DEPOSIT, &ev));
-    const double dx = c->g.dx, dy = c->g.dy;
+static DepositArgs deposit_args(const pic_ctx* c, const Species& sp) {
     DepositArgs a;
+    const double dx = c->g.dx, dy = c->g.dy;
     a.inv_area = 1.0 / (dx * dy);
     a.scale_q = std::ldexp(1.0, c->p.charge_shift);
     a.scale_e = std::ldexp(1.0, c->p.energy_shift);
     a.denom = 1 / std::fabs(sp.qm);
     a.energy = c->p.deposit_energy;
-    hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(c->G)), dim3(kBlock), 0, c->stream,
-                       (unsigned long long*)sp.q_charge, c->G);
-    PIC_HIP(hipGetLastError());
-    if (a.energy) {
-        hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(c->G)), dim3(kBlock), 0, c->stream,
-                           (unsigned long long*)sp.q_energy, c->G);
-        PIC_HIP(hipGetLastError());
-    }
-    const int64_t n = sp.n_upper;
-    if (n > 0) {
-        if (c->p.layout == PIC_LAYOUT_F64)
-            hipLaunchKernelGGL(k_deposit<PIC_LAYOUT_F64>, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream,
-                               sp.ndev + sp.cur, sp.buf[sp.cur], c->g, a,
-                               (unsigned long long*)sp.q_charge, (unsigned long long*)sp.q_energy);
-        else
-            hipLaunchKernelGGL(k_deposit<PIC_LAYOUT_CELL32>, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream,
-                               sp.ndev + sp.cur, sp.buf[sp.cur], c->g, a,
-                               (unsigned long long*)sp.q_charge, (unsigned long long*)sp.q_energy);
-        PIC_HIP(hipGetLastError());
-    }
-    const double cq = sp.W * (dx * dy) * std::ldexp(1.0, -c->p.charge_shift);
-    const double ce = sp.W * (dx * dy) * std::ldexp(1.0, -c->p.energy_shift);
-    const double chrg = -sp.charge / c->p.epsilon;
-    hipLaunchKernelGGL(k_deposit_finalize, dim3(blocks_for(c->G)), dim3(kBlock), 0, c->stream,
-                       c->G, (const long long*)sp.q_charge, (const long long*)sp.q_energy,
-                       cq, ce, chrg, a.energy, sp.final_mesh, sp.energy_mesh, c->rho);
-    PIC_HIP(hipGetLastError());
-    PIC_TRY(timer_end(c, PIC_STAGE_DEPOSIT, ev));
-    return PIC_OK;
+    return a;
 }
 
-int launch_push(pic_ctx* c, int s) {
-    Species& sp = c->sp[s];
-    hipEvent_t ev;
-    PIC_TRY(timer_begin(c, PIC_STAGE_PUSH, &ev));
+static SpeciesDev species_dev(const pic_ctx* c, const Species& sp) {
     SpeciesDev d;
     d.qm = sp.qm;
     d.half_qm = 0.5 * sp.qm;
     d.denom = 1 / std::fabs(sp.qm);
     d.dt = c->p.dt;
+    d.dtdx = c->p.dt * c->g.inv_dx;
+    d.dtdy = c->p.dt * c->g.inv_dy;
     d.bx = c->p.bx;
     d.by = c->p.by;
     d.general_b = (c->p.bx != 0.0 || c->p.by != 0.0);
+    return d;
+}
+
+static int tiles_per_row(const pic_ctx* c) { return (c->g.gx + kTileC - 1) / kTileC; }
+
+static int finalize_deposit(pic_ctx* c, Species& sp) {
+    const double dx = c->g.dx, dy = c->g.dy;
+    const double cqf = sp.W * (dx * dy) * std::ldexp(1.0, -c->p.charge_shift);
+    const double cef = sp.W * (dx * dy) * std::ldexp(1.0, -c->p.energy_shift);
+    const double chrg = -sp.charge / c->p.epsilon;
+    hipLaunchKernelGGL(k_deposit_finalize, dim3(blocks_for(c->G)), dim3(kBlock), 0, c->stream,
+                       c->G, c->g.gp, (const long long*)sp.q_charge, (const long long*)sp.q_energy,
+                       cqf, cef, chrg, c->p.deposit_energy, sp.final_mesh, sp.energy_mesh, c->rho);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+// Charge (and energy) deposit of species s for its current particles:
+// per-cell corner sums on cell-sorted storage, per-particle atomics
+// otherwise, then node sums into final_mesh / energy_mesh and rho.
+int launch_deposit(pic_ctx* c, int s) {
+    Species& sp = c->sp[s];
+    hipEvent_t ev;
+    PIC_TRY(timer_begin(c, PIC_STAGE_DEPOSIT, &ev));
+    DepositArgs a = deposit_args(c, sp);
+    if (!sp.deposit_pending) {
+        if (sp.sorted) {
+            const int tpr = (c->g.gx + kDepC - 1) / kDepC;
+            const unsigned ntiles = (unsigned)(tpr * c->g.gy);
+            if (c->p.layout == PIC_LAYOUT_F64)
+                hipLaunchKernelGGL(k_deposit_tile<PIC_LAYOUT_F64>, dim3(ntiles), dim3(kBlock), 0,
+                                   c->stream, sp.buf[sp.cur], c->g, a, tpr, sp.start, (long long*)sp.q_charge,
+                                   (long long*)sp.q_energy);
+            else
+                hipLaunchKernelGGL(k_deposit_tile<PIC_LAYOUT_CELL32>, dim3(ntiles), dim3(kBlock), 0,
+                                   c->stream, sp.buf[sp.cur], c->g, a, tpr, sp.start, (long long*)sp.q_charge,
+                                   (long long*)sp.q_energy);
+        } else {
+            const int64_t nacc = 4 * (int64_t)c->g.gp;
+            hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(nacc)), dim3(kBlock), 0, c->stream,
+                               (unsigned long long*)sp.q_charge, nacc);
+            hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(nacc)), dim3(kBlock), 0, c->stream,
+                               (unsigned long long*)sp.q_energy, nacc);
+            if (sp.n_upper > 0) {
+                if (c->p.layout == PIC_LAYOUT_F64)
+                    hipLaunchKernelGGL(k_deposit_atomic<PIC_LAYOUT_F64>, dim3(blocks_for(sp.n_upper)), dim3(kBlock),
+                                       0, c->stream, sp.ndev + sp.cur, sp.buf[sp.cur], c->g, a,
+                                       (unsigned long long*)sp.q_charge, (unsigned long long*)sp.q_energy);
+                else
+                    hipLaunchKernelGGL(k_deposit_atomic<PIC_LAYOUT_CELL32>, dim3(blocks_for(sp.n_upper)),
+                                       dim3(kBlock), 0, c->stream, sp.ndev + sp.cur, sp.buf[sp.cur], c->g, a,
+                                       (unsigned long long*)sp.q_charge, (unsigned long long*)sp.q_energy);
+            }
+        }
+        PIC_HIP(hipGetLastError());
+        sp.deposit_pending = 1;
+    }
+    PIC_TRY(finalize_deposit(c, sp));
+    PIC_TRY(timer_end(c, PIC_STAGE_DEPOSIT, ev));
+    return PIC_OK;
+}
+
+// In-place push, no re-bin (keys left in sp.key).
+int launch_push(pic_ctx* c, int s) {
+    Species& sp = c->sp[s];
+    hipEvent_t ev;
+    PIC_TRY(timer_begin(c, PIC_STAGE_PUSH, &ev));
+    SpeciesDev d = species_dev(c, sp);
     const int64_t n = sp.n_upper;
     if (n > 0) {
         dim3 grid(blocks_for(n)), blk(kBlock);
@@ -523,7 +837,57 @@ int launch_push(pic_ctx* c, int s) {
                                sp.ndev + sp.cur, sp.buf[sp.cur], c->g, d, c->fld, sp.key, sp.absorbed);
         PIC_HIP(hipGetLastError());
     }
+    sp.deposit_pending = 0;
+    sp.sorted = 0;
     PIC_TRY(timer_end(c, PIC_STAGE_PUSH, ev));
+    return PIC_OK;
+}
+
+template <int L, int GENB>
+static void tile_kernel(pic_ctx* c, Species& sp, const SpeciesDev& d, int tpr, unsigned ntiles) {
+    hipLaunchKernelGGL((k_push_tile<L, GENB>), dim3(ntiles), dim3(kBlock), 0, c->stream,
+                       sp.buf[sp.cur], c->g, d, tpr, c->fld, sp.start, sp.counts, sp.rank, sp.absorbed);
+}
+
+template <int L, int GENB>
+static void scatter_kernel(pic_ctx* c, Species& sp, const SpeciesDev& d, int src, int dst) {
+    hipLaunchKernelGGL((k_scatter_push<L, GENB>), dim3(blocks_for(sp.n_upper)), dim3(kBlock), 0, c->stream,
+                       sp.ndev + src, sp.buf[src], sp.buf[dst], c->g, d, c->fld, sp.rank, sp.start);
+}
+
+// Gather + Boris + walls + re-bin of species s: tile push/count, prefix
+// scan, recomputing scatter into the other buffer.
+int launch_move(pic_ctx* c, int s) {
+    Species& sp = c->sp[s];
+    if (!sp.sorted) PIC_TRY(launch_bin(c, s));   // tiles need cell-sorted storage
+    SpeciesDev d = species_dev(c, sp);
+    const int tpr = tiles_per_row(c);
+    const unsigned ntiles = (unsigned)(tpr * c->g.gy);
+    const bool f64 = c->p.layout == PIC_LAYOUT_F64;
+    const int src = sp.cur, dst = sp.cur ^ 1;
+    hipEvent_t ev;
+    PIC_TRY(timer_begin(c, PIC_STAGE_PUSH, &ev));
+    if (sp.n_upper > 0) {
+        if (f64 && !d.general_b) tile_kernel<PIC_LAYOUT_F64, 0>(c, sp, d, tpr, ntiles);
+        else if (f64) tile_kernel<PIC_LAYOUT_F64, 1>(c, sp, d, tpr, ntiles);
+        else if (!d.general_b) tile_kernel<PIC_LAYOUT_CELL32, 0>(c, sp, d, tpr, ntiles);
+        else tile_kernel<PIC_LAYOUT_CELL32, 1>(c, sp, d, tpr, ntiles);
+        PIC_HIP(hipGetLastError());
+    }
+    PIC_TRY(timer_end(c, PIC_STAGE_PUSH, ev));
+    PIC_TRY(timer_begin(c, PIC_STAGE_BIN, &ev));
+    PIC_TRY(scan_counts(c, sp, dst));
+    if (sp.n_upper > 0) {
+        if (f64 && !d.general_b) scatter_kernel<PIC_LAYOUT_F64, 0>(c, sp, d, src, dst);
+        else if (f64) scatter_kernel<PIC_LAYOUT_F64, 1>(c, sp, d, src, dst);
+        else if (!d.general_b) scatter_kernel<PIC_LAYOUT_CELL32, 0>(c, sp, d, src, dst);
+        else scatter_kernel<PIC_LAYOUT_CELL32, 1>(c, sp, d, src, dst);
+        PIC_HIP(hipGetLastError());
+    }
+    sp.cur = dst;
+    sp.deposit_pending = 0;
+    sp.sorted = 1;
+    PIC_TRY(timer_end(c, PIC_STAGE_BIN, ev));
     return PIC_OK;
 }
 

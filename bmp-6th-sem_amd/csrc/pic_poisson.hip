@@ -10,6 +10,15 @@
 // which is the reference's CSR row times phi = rho after overwriteRho negates
 // rho; Dirichlet nodes hold phi = V, which is what the reference's -1 diagonal
 // rows give.  Eliminating them leaves a symmetric positive definite system.
+//
+// Launch structure: dot products leave one partial sum per block; the kernel
+// that needs the scalar re-adds the partials in a fixed order (so every block
+// sees the same value, with no fences or atomics).  Convergence is a device
+// flag: once set, every kernel of the remaining enqueued iterations returns at
+// once, and the host reads the flag only between batches of iterations.  The
+// V-cycle fuses the two pre-smoothing sweeps into one stencil pass and the
+// prolongation into the first post-smoothing sweep, and runs all levels of at
+// most 65 x 65 nodes inside one workgroup's LDS.
 #include "pic_internal.h"
 
 #include <algorithm>
@@ -18,58 +27,38 @@
 namespace pic {
 
 static constexpr int kB = 256;
-static constexpr int kRedBlocks = 1024;     // fixed grid for reductions (deterministic order)
+static constexpr int kNP = 512;              // blocks of the reduction kernels (partials)
+static constexpr int kBatch = 2;             // iterations enqueued between host checks
 
-enum Scal { S_RR = 0, S_BB, S_RZ, S_RZOLD, S_PQ, S_RZPREV, S_ALPHA, S_BETA, S_N };
+// scalars: [0] rr, [1] bb, [2] rr0, [3..4] rz of the previous two iterations
+enum Scal { S_RR = 0, S_BB, S_RR0, S_RZP0, S_RZP1, S_N };
+// partial-sum arrays, kNP doubles each
+enum Part { P_RR = 0, P_BB, P_PQ, P_RZ, P_RZO, P_N };
 
 static inline unsigned nblk(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
-static inline unsigned nred(int64_t n) { return (unsigned)std::min<int64_t>(kRedBlocks, (n + kB - 1) / kB); }
 
 // ------------------------------------------------------- block reductions
-// Sum of `v` over the block in a fixed order; result valid in thread 0.
-__device__ __forceinline__ double block_sum(double v, double* sh) {
+// Sum over the block in a fixed order; the result is returned to every thread.
+__device__ __forceinline__ double block_sum_all(double v, double* sh) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) sh[wid] = v;
     __syncthreads();
     double s = 0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
     __syncthreads();
     return s;
 }
 
-// Grid-wide sum of up to two values per thread: block partials go to `part`,
-// the last block to finish adds them in block order and writes scal[slot0/1].
-__device__ void grid_sum2(double a, double b, double* part, unsigned* count,
-                          double* scal, int slot0, int slot1) {
-    __shared__ double sh[16];
-    __shared__ bool last;
-    double sa = block_sum(a, sh);
-    double sb = slot1 >= 0 ? block_sum(b, sh) : 0.0;
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = sa;
-        part[2 * blockIdx.x + 1] = sb;
-        __threadfence();
-        unsigned prev = atomicInc(count, 0xffffffffu);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double ta = 0, tb = 0;
-    for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) {
-        ta += part[2 * k];
-        tb += part[2 * k + 1];
-    }
-    // combine per-thread partials in a fixed order
-    ta = block_sum(ta, sh);
-    tb = block_sum(tb, sh);
-    if (threadIdx.x == 0) {
-        scal[slot0] = ta;
-        if (slot1 >= 0) scal[slot1] = tb;
-        *count = 0;
-    }
+__device__ __forceinline__ double sum_partials(const double* part, double* sh) {
+    double v = 0;
+    for (int j = threadIdx.x; j < kNP; j += blockDim.x) v += part[j];
+    return block_sum_all(v, sh);
+}
+
+__device__ __forceinline__ void write_partial(double v, double* part, double* sh) {
+    double s = block_sum_all(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // ------------------------------------------------------------ fine kernels
@@ -100,12 +89,16 @@ __global__ void k_init_guess(int G, double* x, const double* src, const uint8_t*
     x[n] = dir[n] ? dval[n] : (src ? src[n] : 0.0);
 }
 
-// r = rhs - A x on unknown nodes (0 on Dirichlet), rr = r.r.
-// rhs = rho after overwriteRho.  x holds V on Dirichlet nodes.
+__global__ void k_reset_flags(int* flags) {
+    flags[0] = 0;
+    flags[1] = 0;
+}
+
+// r = rhs - A x on unknown nodes (0 on Dirichlet); partial of r.r
 __global__ __launch_bounds__(kB) void k_residual(Fine f, const double* __restrict__ rhs,
                                                  const double* __restrict__ x, const uint8_t* __restrict__ dir,
-                                                 double* __restrict__ r, double* part, unsigned* count,
-                                                 double* scal, int slot) {
+                                                 double* __restrict__ r, double* part) {
+    __shared__ double sh[kB / 64];
     double acc = 0;
     const int G = f.gx * f.gy;
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
@@ -117,12 +110,35 @@ __global__ __launch_bounds__(kB) void k_residual(Fine f, const double* __restric
         r[n] = v;
         acc += v * v;
     }
-    grid_sum2(acc, 0, part, count, scal, slot, -1);
+    write_partial(acc, part, sh);
 }
 
-// q = A p (p = 0 on Dirichlet nodes), pq = p.q
+// rr = sum of the partials; done when rr <= tol^2 bb (flags[0] = 1), NaN -> 2.
+// Outside the first call, one more iteration is recorded unless already done.
+__global__ __launch_bounds__(kB) void k_check(const double* part_rr, const double* part_bb, double tol2,
+                                              int* flags, double* scal, int first) {
+    __shared__ double sh[kB / 64];
+    if (flags[0]) return;
+    double rr = sum_partials(part_rr, sh);
+    double bb = first ? sum_partials(part_bb, sh) : scal[S_BB];
+    if (threadIdx.x == 0) {
+        if (first) {
+            scal[S_BB] = bb;
+            scal[S_RR0] = rr;
+        } else {
+            flags[1] += 1;
+        }
+        scal[S_RR] = rr;
+        if (!(rr == rr)) flags[0] = 2;
+        else if (rr <= tol2 * bb || bb == 0.0) flags[0] = 1;
+    }
+}
+
+// q = A p (p = 0 on Dirichlet nodes); partial of p.q
 __global__ __launch_bounds__(kB) void k_apply(Fine f, const double* __restrict__ p, const uint8_t* __restrict__ dir,
-                                              double* __restrict__ q, double* part, unsigned* count, double* scal) {
+                                              double* __restrict__ q, double* part_pq, const int* flags) {
+    __shared__ double sh[kB / 64];
+    if (flags[0]) return;
     double acc = 0;
     const int G = f.gx * f.gy;
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
@@ -133,14 +149,19 @@ __global__ __launch_bounds__(kB) void k_apply(Fine f, const double* __restrict__
         }
         q[n] = v;
     }
-    grid_sum2(acc, 0, part, count, scal, S_PQ, -1);
+    write_partial(acc, part_pq, sh);
 }
 
-// alpha = rz / pq; x += alpha p; r -= alpha q; rr = r.r
+// alpha = rz / pq; x += alpha p; r -= alpha q; partial of r.r
 __global__ __launch_bounds__(kB) void k_update_xr(int G, double* __restrict__ x, double* __restrict__ r,
                                                   const double* __restrict__ p, const double* __restrict__ q,
-                                                  double* part, unsigned* count, double* scal) {
-    const double alpha = scal[S_RZ] / scal[S_PQ];
+                                                  const double* part_pq, const double* part_rz, double* part_rr,
+                                                  const int* flags) {
+    __shared__ double sh[kB / 64];
+    if (flags[0]) return;
+    const double rz = sum_partials(part_rz, sh);
+    const double pq = sum_partials(part_pq, sh);
+    const double alpha = rz / pq;
     double acc = 0;
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
         x[n] += alpha * p[n];
@@ -148,14 +169,15 @@ __global__ __launch_bounds__(kB) void k_update_xr(int G, double* __restrict__ x,
         r[n] = rn;
         acc += rn * rn;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_ALPHA] = alpha;
-    grid_sum2(acc, 0, part, count, scal, S_RR, -1);
+    write_partial(acc, part_rr, sh);
 }
 
-// z = (double) e0; rz = r.z; rzold = r.zold (for the flexible beta)
-__global__ __launch_bounds__(kB) void k_precond_out(int G, const float* __restrict__ e0, const double* __restrict__ r,
-                                                    double* __restrict__ z, const double* __restrict__ zold,
-                                                    double* part, unsigned* count, double* scal, int first) {
+// z = (double) e0; partials of r.z and r.zold (flexible beta)
+__global__ __launch_bounds__(kB) void k_zout(int G, const float* __restrict__ e0, const double* __restrict__ r,
+                                             double* __restrict__ z, const double* __restrict__ zold,
+                                             double* part_rz, double* part_rzo, const int* flags, int first) {
+    __shared__ double sh[kB / 64];
+    if (flags[0]) return;
     double a = 0, b = 0;
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
         double zn = (double)e0[n];
@@ -163,21 +185,30 @@ __global__ __launch_bounds__(kB) void k_precond_out(int G, const float* __restri
         a += r[n] * zn;
         if (!first) b += r[n] * zold[n];
     }
-    grid_sum2(a, b, part, count, scal, S_RZ, first ? -1 : S_RZOLD);
+    write_partial(a, part_rz, sh);
+    if (!first) write_partial(b, part_rzo, sh);
 }
 
-// beta = (rz - rzold) / rzprev; p = z + beta p; zold = z; rzprev = rz
+// beta = (r.z - r.zold) / rz_prev; p = z + beta p; zold = z; rz kept for the next iteration
 __global__ __launch_bounds__(kB) void k_update_p(int G, double* __restrict__ p, const double* __restrict__ z,
-                                                 double* __restrict__ zold, const double* scal_in, int first) {
-    const double beta = first ? 0.0 : (scal_in[S_RZ] - scal_in[S_RZOLD]) / scal_in[S_RZPREV];
+                                                 double* __restrict__ zold, const double* part_rz,
+                                                 const double* part_rzo, double* scal, const int* flags,
+                                                 int parity, int first) {
+    __shared__ double sh[kB / 64];
+    if (flags[0]) return;
+    const double rz = sum_partials(part_rz, sh);
+    double beta = 0.0;
+    if (!first) {
+        const double rzo = sum_partials(part_rzo, sh);
+        beta = (rz - rzo) / scal[S_RZP0 + (parity ^ 1)];
+    }
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
         double zn = z[n];
         p[n] = first ? zn : zn + beta * p[n];
         zold[n] = zn;
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_RZP0 + parity] = rz;
 }
-
-__global__ void k_roll_rz(double* scal) { scal[S_RZPREV] = scal[S_RZ]; }
 
 // --------------------------------------------------------------- MG (FP32)
 struct Lvl {
@@ -185,7 +216,8 @@ struct Lvl {
     float cx, cy, diag;
 };
 
-__global__ void k_d2f(int G, const double* __restrict__ r, float* __restrict__ b) {
+__global__ void k_d2f(int G, const double* __restrict__ r, float* __restrict__ b, const int* flags) {
+    if (flags[0]) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n < G) b[n] = (float)r[n];
 }
@@ -194,7 +226,8 @@ __global__ void k_d2f(int G, const double* __restrict__ r, float* __restrict__ b
 // With zero_in the input iterate is taken as 0.
 __global__ __launch_bounds__(kB) void k_mg_smooth(Lvl L, float w, const float* __restrict__ ei,
                                                   float* __restrict__ eo, const float* __restrict__ b,
-                                                  const uint8_t* __restrict__ dir, int zero_in) {
+                                                  const uint8_t* __restrict__ dir, int zero_in, const int* flags) {
+    if (flags[0]) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= L.gx * L.gy) return;
     if (dir[n]) { eo[n] = 0.f; return; }
@@ -202,6 +235,27 @@ __global__ __launch_bounds__(kB) void k_mg_smooth(Lvl L, float w, const float* _
     float e = ei[n];
     float ax = L.diag * e - L.cx * (ei[n - 1] + ei[n + 1]) - L.cy * (ei[n - L.gx] + ei[n + L.gx]);
     eo[n] = e + w * (b[n] - ax) / L.diag;
+}
+
+// Two weighted-Jacobi sweeps from zero in one pass: the first sweep is
+// pointwise (e1 = w b / diag), so the second needs only b on the 5-point
+// stencil.  At the finest level b is the FP64 residual, rounded to FP32 here
+// (and stored for the restriction and post-smoothing).
+template <typename TB>
+__global__ __launch_bounds__(kB) void k_mg_pre2(Lvl L, float w, const TB* __restrict__ bin, float* __restrict__ bout,
+                                                float* __restrict__ eo, const uint8_t* __restrict__ dir,
+                                                const int* flags) {
+    if (flags[0]) return;
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= L.gx * L.gy) return;
+    float bn = (float)bin[n];
+    if (bout) bout[n] = bn;
+    if (dir[n]) { eo[n] = 0.f; return; }
+    const float s = w / L.diag;
+    auto e1 = [&](int m) -> float { return dir[m] ? 0.f : s * (float)bin[m]; };
+    float c = s * bn;
+    float ax = L.diag * c - L.cx * (e1(n - 1) + e1(n + 1)) - L.cy * (e1(n - L.gx) + e1(n + L.gx));
+    eo[n] = c + w * (bn - ax) / L.diag;
 }
 
 __device__ __forceinline__ float mg_res(const Lvl& L, const float* e, const float* b, const uint8_t* dir, int i, int j) {
@@ -214,7 +268,9 @@ __device__ __forceinline__ float mg_res(const Lvl& L, const float* e, const floa
 // Coarse rhs = full weighting of the fine residual; zero on coarse Dirichlet.
 __global__ __launch_bounds__(kB) void k_mg_restrict(Lvl F, Lvl C, const float* __restrict__ e,
                                                     const float* __restrict__ b, const uint8_t* __restrict__ fdir,
-                                                    float* __restrict__ bc, const uint8_t* __restrict__ cdir) {
+                                                    float* __restrict__ bc, const uint8_t* __restrict__ cdir,
+                                                    const int* flags) {
+    if (flags[0]) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= C.gx * C.gy) return;
     if (cdir[n]) { bc[n] = 0.f; return; }
@@ -228,46 +284,163 @@ __global__ __launch_bounds__(kB) void k_mg_restrict(Lvl F, Lvl C, const float* _
     bc[n] = s * 0.0625f;
 }
 
-// e += bilinear interpolation of the coarse correction (unknown nodes only).
+// bilinear interpolation of the coarse correction at fine node (i, j)
+__device__ __forceinline__ float prolong_at(const Lvl& C, const float* ec, int i, int j) {
+    int I = i >> 1, J = j >> 1;
+    bool oi = i & 1, oj = j & 1;
+    const float* c0 = ec + J * C.gx + I;
+    if (!oi && !oj) return c0[0];
+    if (oi && !oj) return 0.5f * (c0[0] + c0[1]);
+    if (!oi && oj) return 0.5f * (c0[0] + c0[C.gx]);
+    return 0.25f * (c0[0] + c0[1] + c0[C.gx] + c0[C.gx + 1]);
+}
+
+// e += P ec (unknown nodes only) — generic path.
 __global__ __launch_bounds__(kB) void k_mg_prolong(Lvl F, Lvl C, float* __restrict__ e,
-                                                   const uint8_t* __restrict__ fdir, const float* __restrict__ ec) {
+                                                   const uint8_t* __restrict__ fdir, const float* __restrict__ ec,
+                                                   const int* flags) {
+    if (flags[0]) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= F.gx * F.gy) return;
     if (fdir[n]) return;
-    int i = n % F.gx, j = n / F.gx;
-    int I = i >> 1, J = j >> 1;
-    bool oi = i & 1, oj = j & 1;
-    float v;
-    const float* c0 = ec + J * C.gx + I;
-    if (!oi && !oj) v = c0[0];
-    else if (oi && !oj) v = 0.5f * (c0[0] + c0[1]);
-    else if (!oi && oj) v = 0.5f * (c0[0] + c0[C.gx]);
-    else v = 0.25f * (c0[0] + c0[1] + c0[C.gx] + c0[C.gx + 1]);
-    e[n] += v;
+    e[n] += prolong_at(C, ec, n % F.gx, n / F.gx);
 }
 
-// Coarsest level: weighted-Jacobi sweeps in one workgroup, in LDS.
-static constexpr int kCoarseMax = 8192;
-__global__ __launch_bounds__(1024) void k_mg_coarse(Lvl L, float w, int sweeps, float* __restrict__ e,
-                                                    const float* __restrict__ b, const uint8_t* __restrict__ dir) {
-    __shared__ float ea[kCoarseMax / 2];
-    __shared__ float eb[kCoarseMax / 2];
-    const int G = L.gx * L.gy;
-    for (int n = threadIdx.x; n < G; n += blockDim.x) { ea[n] = 0.f; eb[n] = 0.f; }
+// Prolongation fused with the first post-smoothing sweep:
+// e1 = e + P ec (computed at the node and its four neighbours), eo = one
+// weighted-Jacobi sweep of e1.
+__global__ __launch_bounds__(kB) void k_mg_prolong_sweep(Lvl F, Lvl C, float w, const float* __restrict__ e,
+                                                         const float* __restrict__ ec, const float* __restrict__ b,
+                                                         const uint8_t* __restrict__ dir, float* __restrict__ eo,
+                                                         const int* flags) {
+    if (flags[0]) return;
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= F.gx * F.gy) return;
+    if (dir[n]) { eo[n] = 0.f; return; }
+    const int i = n % F.gx, j = n / F.gx;
+    auto e1 = [&](int m, int mi, int mj) -> float { return dir[m] ? 0.f : e[m] + prolong_at(C, ec, mi, mj); };
+    float c = e1(n, i, j);
+    float ax = F.diag * c - F.cx * (e1(n - 1, i - 1, j) + e1(n + 1, i + 1, j))
+             - F.cy * (e1(n - F.gx, i, j - 1) + e1(n + F.gx, i, j + 1));
+    eo[n] = c + w * (b[n] - ax) / F.diag;
+}
+
+// ---------------------------------------------- coarse levels in one workgroup
+static constexpr int kLdsMaxLev = 8;
+static constexpr int kLdsNodes = 5800;       // all in-LDS levels together
+static constexpr int kLdsPerThread = 5;      // nodes per thread per level (1024 threads)
+
+struct CoarseLevels {
+    int nl;
+    int gx[kLdsMaxLev], gy[kLdsMaxLev], off[kLdsMaxLev];
+    float cx[kLdsMaxLev], cy[kLdsMaxLev], diag[kLdsMaxLev];
+    const uint8_t* dir[kLdsMaxLev];
+    float w;
+    int pre, post, sweeps;
+};
+
+__device__ __forceinline__ void lds_sweep(float* E, const float* B, const uint8_t* D, int gx, int gy, float cx,
+                                          float cy, float diag, float w, bool zero_in) {
+    const int G = gx * gy;
+    float nv[kLdsPerThread];
+#pragma unroll
+    for (int j = 0; j < kLdsPerThread; ++j) {
+        int n = threadIdx.x + j * 1024;
+        nv[j] = 0.f;
+        if (n < G && !D[n]) {
+            if (zero_in) {
+                nv[j] = w * B[n] / diag;
+            } else {
+                float e = E[n];
+                float ax = diag * e - cx * (E[n - 1] + E[n + 1]) - cy * (E[n - gx] + E[n + gx]);
+                nv[j] = e + w * (B[n] - ax) / diag;
+            }
+        }
+    }
     __syncthreads();
-    float* src = ea;
-    float* dst = eb;
-    for (int s = 0; s < sweeps; ++s) {
-        for (int n = threadIdx.x; n < G; n += blockDim.x) {
-            if (dir[n]) { dst[n] = 0.f; continue; }
-            float x = src[n];
-            float ax = L.diag * x - L.cx * (src[n - 1] + src[n + 1]) - L.cy * (src[n - L.gx] + src[n + L.gx]);
-            dst[n] = x + w * (b[n] - ax) / L.diag;
+#pragma unroll
+    for (int j = 0; j < kLdsPerThread; ++j) {
+        int n = threadIdx.x + j * 1024;
+        if (n < G) E[n] = nv[j];
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const float* __restrict__ bglob,
+                                                          float* __restrict__ eglob, const int* flags) {
+    __shared__ float E[kLdsNodes];
+    __shared__ float B[kLdsNodes];
+    __shared__ uint8_t D[kLdsNodes];
+    if (flags[0]) return;
+    const int tid = threadIdx.x;
+    for (int l = 0; l < cl.nl; ++l) {
+        const int G = cl.gx[l] * cl.gy[l];
+        for (int n = tid; n < G; n += 1024) {
+            D[cl.off[l] + n] = cl.dir[l][n];
+            E[cl.off[l] + n] = 0.f;
+        }
+    }
+    for (int n = tid; n < cl.gx[0] * cl.gy[0]; n += 1024) B[n] = bglob[n];
+    __syncthreads();
+    // down
+    for (int l = 0; l + 1 < cl.nl; ++l) {
+        float* e = E + cl.off[l];
+        const float* b = B + cl.off[l];
+        const uint8_t* d = D + cl.off[l];
+        const int gx = cl.gx[l];
+        for (int s = 0; s < cl.pre; ++s)
+            lds_sweep(e, b, d, gx, cl.gy[l], cl.cx[l], cl.cy[l], cl.diag[l], cl.w, s == 0);
+        // restrict the residual
+        const int cgx = cl.gx[l + 1], cgy = cl.gy[l + 1];
+        float* bc = B + cl.off[l + 1];
+        const uint8_t* dc = D + cl.off[l + 1];
+        const float cx = cl.cx[l], cy = cl.cy[l], dg = cl.diag[l];
+        auto res = [&](int i, int j) -> float {
+            int n = j * gx + i;
+            if (d[n]) return 0.f;
+            float ax = dg * e[n] - cx * (e[n - 1] + e[n + 1]) - cy * (e[n - gx] + e[n + gx]);
+            return b[n] - ax;
+        };
+        for (int n = tid; n < cgx * cgy; n += 1024) {
+            if (dc[n]) { bc[n] = 0.f; continue; }
+            int i = 2 * (n % cgx), j = 2 * (n / cgx);
+            float s = 4.f * res(i, j) + 2.f * (res(i - 1, j) + res(i + 1, j) + res(i, j - 1) + res(i, j + 1))
+                    + (res(i - 1, j - 1) + res(i + 1, j - 1) + res(i - 1, j + 1) + res(i + 1, j + 1));
+            bc[n] = s * 0.0625f;
         }
         __syncthreads();
-        float* t = src; src = dst; dst = t;
     }
-    for (int n = threadIdx.x; n < G; n += blockDim.x) e[n] = src[n];
+    // coarsest
+    {
+        const int l = cl.nl - 1;
+        for (int s = 0; s < cl.sweeps; ++s)
+            lds_sweep(E + cl.off[l], B + cl.off[l], D + cl.off[l], cl.gx[l], cl.gy[l], cl.cx[l], cl.cy[l],
+                      cl.diag[l], cl.w, s == 0);
+    }
+    // up
+    for (int l = cl.nl - 2; l >= 0; --l) {
+        float* e = E + cl.off[l];
+        const uint8_t* d = D + cl.off[l];
+        const float* ec = E + cl.off[l + 1];
+        const int gx = cl.gx[l], cgx = cl.gx[l + 1];
+        for (int n = tid; n < gx * cl.gy[l]; n += 1024) {
+            if (d[n]) continue;
+            int i = n % gx, j = n / gx;
+            int I = i >> 1, J = j >> 1;
+            bool oi = i & 1, oj = j & 1;
+            const float* c0 = ec + J * cgx + I;
+            float v;
+            if (!oi && !oj) v = c0[0];
+            else if (oi && !oj) v = 0.5f * (c0[0] + c0[1]);
+            else if (!oi && oj) v = 0.5f * (c0[0] + c0[cgx]);
+            else v = 0.25f * (c0[0] + c0[1] + c0[cgx] + c0[cgx + 1]);
+            e[n] += v;
+        }
+        __syncthreads();
+        for (int s = 0; s < cl.post; ++s)
+            lds_sweep(e, B + cl.off[l], d, gx, cl.gy[l], cl.cx[l], cl.cy[l], cl.diag[l], cl.w, false);
+    }
+    for (int n = tid; n < cl.gx[0] * cl.gy[0]; n += 1024) eglob[n] = E[n];
 }
 
 // ----------------------------------------------------------------- E field
@@ -314,6 +487,22 @@ void poisson_free(pic_ctx* c) {
         if (m.t) (void)hipFree(m.t);
     }
     c->mg.clear();
+}
+
+// first level whose levels (down to the coarsest) fit the in-LDS cycle
+static int lds_first_level(const pic_ctx* c) {
+    const int L = (int)c->mg.size();
+    for (int l = 0; l < L; ++l) {
+        int total = 0;
+        bool ok = (L - l) <= kLdsMaxLev;
+        for (int m = l; m < L; ++m) {
+            int nn = c->mg[m].gx * c->mg[m].gy;
+            total += nn;
+            if (nn > kLdsPerThread * 1024) ok = false;
+        }
+        if (ok && total <= kLdsNodes) return l;
+    }
+    return L - 1;
 }
 
 int poisson_setup(pic_ctx* c) {
@@ -384,7 +573,8 @@ int poisson_setup(pic_ctx* c) {
         hx *= 2; hy *= 2;
     }
     PIC_HIP(hipDeviceSynchronize());
-    if ((size_t)c->mg.back().gx * c->mg.back().gy > (size_t)kCoarseMax / 2) {
+    const size_t last = (size_t)c->mg.back().gx * c->mg.back().gy;
+    if (last > (size_t)kLdsNodes || last > (size_t)kLdsPerThread * 1024) {
         set_error("coarsest multigrid level %dx%d is too large for the in-LDS coarse solve; "
                   "use NX, NY with more factors of two",
                   c->mg.back().gx, c->mg.back().gy);
@@ -415,64 +605,101 @@ int launch_efield(pic_ctx* c) {
     return PIC_OK;
 }
 
-// One V-cycle: c->mg[0].b holds the rhs, result in c->mg[0].e.
+// One V-cycle on the FP64 residual c->pr; result in c->mg[0].e.
 static int vcycle(pic_ctx* c) {
     const float w = (float)c->p.mg_omega;
     const int L = (int)c->mg.size();
+    const int lc = lds_first_level(c);
     hipStream_t st = c->stream;
-    for (int l = 0; l < L - 1; ++l) {
+    const int* flags = c->flags;
+    const bool fused = c->p.mg_pre == 2 && c->p.mg_post == 2;
+    if (lc == 0) {
+        MgLevel& m = c->mg[0];
+        hipLaunchKernelGGL(k_d2f, dim3(nblk(c->G)), dim3(kB), 0, st, c->G, c->pr, m.b, flags);
+    }
+    // fine levels in global memory
+    for (int l = 0; l < lc; ++l) {
         MgLevel& m = c->mg[l];
         Lvl lv = lvl_of(m);
         unsigned nb = nblk((int64_t)m.gx * m.gy);
-        // pre-smoothing, starting from zero
-        float* a = m.e;
-        float* b = m.t;
-        for (int s = 0; s < c->p.mg_pre; ++s) {
-            hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(kB), 0, st, lv, w, a, b, m.b, m.dir, s == 0 ? 1 : 0);
-            std::swap(a, b);
+        if (fused) {
+            if (l == 0)
+                hipLaunchKernelGGL(k_mg_pre2<double>, dim3(nb), dim3(kB), 0, st, lv, w, c->pr, m.b, m.e, m.dir, flags);
+            else
+                hipLaunchKernelGGL(k_mg_pre2<float>, dim3(nb), dim3(kB), 0, st, lv, w, m.b, (float*)nullptr, m.e,
+                                   m.dir, flags);
+        } else {
+            if (l == 0) hipLaunchKernelGGL(k_d2f, dim3(nb), dim3(kB), 0, st, c->G, c->pr, m.b, flags);
+            float* a = m.e;
+            float* b = m.t;
+            for (int s = 0; s < c->p.mg_pre; ++s) {
+                hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(kB), 0, st, lv, w, a, b, m.b, m.dir, s == 0 ? 1 : 0,
+                                   flags);
+                std::swap(a, b);
+            }
+            if (c->p.mg_pre == 0)
+                hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(kB), 0, st, lv, 0.f, b, a, m.b, m.dir, 1, flags);
+            if (a != m.e) std::swap(m.e, m.t);
         }
-        if (c->p.mg_pre == 0)
-            hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(kB), 0, st, lv, 0.f, b, a, m.b, m.dir, 1);
-        if (a != m.e) std::swap(m.e, m.t);   // keep the iterate in m.e
         MgLevel& n = c->mg[l + 1];
-        hipLaunchKernelGGL(k_mg_restrict, dim3(nblk((int64_t)n.gx * n.gy)), dim3(kB), 0, st, lv, lvl_of(n),
-                           m.e, m.b, m.dir, n.b, n.dir);
+        hipLaunchKernelGGL(k_mg_restrict, dim3(nblk((int64_t)n.gx * n.gy)), dim3(kB), 0, st, lv, lvl_of(n), m.e, m.b,
+                           m.dir, n.b, n.dir, flags);
     }
+    // coarse levels in one workgroup
     {
-        MgLevel& m = c->mg[L - 1];
-        int sweeps = std::max(50, 4 * std::max(m.gx, m.gy));
-        hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(1024), 0, st, lvl_of(m), w, sweeps, m.e, m.b, m.dir);
+        CoarseLevels cl;
+        cl.nl = L - lc;
+        int off = 0;
+        for (int l = lc; l < L; ++l) {
+            const MgLevel& m = c->mg[l];
+            int k = l - lc;
+            cl.gx[k] = m.gx; cl.gy[k] = m.gy; cl.off[k] = off;
+            cl.cx[k] = m.cx; cl.cy[k] = m.cy; cl.diag[k] = m.diag;
+            cl.dir[k] = m.dir;
+            off += m.gx * m.gy;
+        }
+        cl.w = w;
+        cl.pre = c->p.mg_pre;
+        cl.post = c->p.mg_post;
+        const MgLevel& last = c->mg[L - 1];
+        cl.sweeps = std::max(50, 4 * std::max(last.gx, last.gy));
+        hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(1024), 0, st, cl, c->mg[lc].b, c->mg[lc].e, flags);
     }
-    for (int l = L - 2; l >= 0; --l) {
+    for (int l = lc - 1; l >= 0; --l) {
         MgLevel& m = c->mg[l];
         MgLevel& n = c->mg[l + 1];
         Lvl lv = lvl_of(m);
         unsigned nb = nblk((int64_t)m.gx * m.gy);
-        hipLaunchKernelGGL(k_mg_prolong, dim3(nb), dim3(kB), 0, st, lv, lvl_of(n), m.e, m.dir, n.e);
-        float* a = m.e;
-        float* b = m.t;
-        for (int s = 0; s < c->p.mg_post; ++s) {
-            hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(kB), 0, st, lv, w, a, b, m.b, m.dir, 0);
-            std::swap(a, b);
+        if (fused) {
+            hipLaunchKernelGGL(k_mg_prolong_sweep, dim3(nb), dim3(kB), 0, st, lv, lvl_of(n), w, m.e, n.e, m.b, m.dir,
+                               m.t, flags);
+            hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(kB), 0, st, lv, w, m.t, m.e, m.b, m.dir, 0, flags);
+        } else {
+            hipLaunchKernelGGL(k_mg_prolong, dim3(nb), dim3(kB), 0, st, lv, lvl_of(n), m.e, m.dir, n.e, flags);
+            float* a = m.e;
+            float* b = m.t;
+            for (int s = 0; s < c->p.mg_post; ++s) {
+                hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(kB), 0, st, lv, w, a, b, m.b, m.dir, 0, flags);
+                std::swap(a, b);
+            }
+            if (a != m.e) std::swap(m.e, m.t);
         }
-        if (a != m.e) std::swap(m.e, m.t);
     }
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
 
 static int precondition(pic_ctx* c, int first) {
-    MgLevel& m0 = c->mg[0];
-    hipLaunchKernelGGL(k_d2f, dim3(nblk(c->G)), dim3(kB), 0, c->stream, c->G, c->pr, m0.b);
     PIC_TRY(vcycle(c));
-    hipLaunchKernelGGL(k_precond_out, dim3(nred(c->G)), dim3(kB), 0, c->stream, c->G, c->mg[0].e, c->pr, c->pz,
-                       c->pzold, c->red, c->red_count, c->scal, first);
+    hipLaunchKernelGGL(k_zout, dim3(kNP), dim3(kB), 0, c->stream, c->G, c->mg[0].e, c->pr, c->pz, c->pzold,
+                       c->part + P_RZ * kNP, c->part + P_RZO * kNP, c->flags, first);
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
 
-static int read_scal(pic_ctx* c) {
+static int read_state(pic_ctx* c) {
     PIC_HIP(hipMemcpyAsync(c->scal_host, c->scal, S_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipMemcpyAsync(c->flags_host, c->flags, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }
@@ -487,55 +714,57 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     PIC_HIP(hipEventCreate(&t0));
     PIC_HIP(hipEventCreate(&t1));
     PIC_HIP(hipEventRecord(t0, s));
+    double* part = c->part;
+    const double tol2 = c->p.poisson_tol * c->p.poisson_tol;
 
+    hipLaunchKernelGGL(k_reset_flags, dim3(1), dim3(1), 0, s, c->flags);
     PIC_TRY(launch_overwrite_rho(c));
-    // ||b||: residual of the Dirichlet-only field (unknowns at zero) -> pb
+    // ||b||^2: residual of the Dirichlet-only field (unknowns at zero)
     hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->pq, (const double*)nullptr, c->dir, c->dval);
-    hipLaunchKernelGGL(k_residual, dim3(nred(G)), dim3(kB), 0, s, f, c->rho, c->pq, c->dir, c->pb, c->red,
-                       c->red_count, c->scal, (int)S_BB);
+    hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->pq, c->dir, c->pb, part + P_BB * kNP);
     int guess = 0;
     const double* src = nullptr;
     if (c->guess_set) { src = c->guess; guess = 2; }
     else if (iteration > 0 && c->have_prev_phi) { src = c->phi; guess = 1; }
     hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, src, c->dir, c->dval);
-    hipLaunchKernelGGL(k_residual, dim3(nred(G)), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, c->pr, c->red,
-                       c->red_count, c->scal, (int)S_RR);
+    hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, c->pr, part + P_RR * kNP);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(kB), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2, c->flags, c->scal,
+                       1);
+    PIC_TRY(precondition(c, 1));
+    hipLaunchKernelGGL(k_update_p, dim3(kNP), dim3(kB), 0, s, G, c->pp, c->pz, c->pzold, part + P_RZ * kNP,
+                       part + P_RZO * kNP, c->scal, c->flags, 0, 1);
     PIC_HIP(hipGetLastError());
-    PIC_TRY(read_scal(c));
-    const double bb = c->scal_host[S_BB];
-    const double tol = c->p.poisson_tol;
-    double rr = c->scal_host[S_RR];
-    const double bnorm = std::sqrt(bb);
-    double rel0 = bnorm > 0 ? std::sqrt(rr) / bnorm : 0.0;
-    double rel = rel0;
     int it = 0;
-    if (bnorm > 0 && rel > tol) {
-        PIC_TRY(precondition(c, 1));
-        hipLaunchKernelGGL(k_update_p, dim3(nred(G)), dim3(kB), 0, s, G, c->pp, c->pz, c->pzold, c->scal, 1);
-        hipLaunchKernelGGL(k_roll_rz, dim3(1), dim3(1), 0, s, c->scal);
-        for (it = 1; it <= c->p.poisson_max_iter; ++it) {
-            hipLaunchKernelGGL(k_apply, dim3(nred(G)), dim3(kB), 0, s, f, c->pp, c->dir, c->pq, c->red,
-                               c->red_count, c->scal);
-            hipLaunchKernelGGL(k_update_xr, dim3(nred(G)), dim3(kB), 0, s, G, c->phi, c->pr, c->pp, c->pq, c->red,
-                               c->red_count, c->scal);
-            PIC_HIP(hipGetLastError());
-            PIC_TRY(read_scal(c));
-            rr = c->scal_host[S_RR];
-            rel = std::sqrt(rr) / bnorm;
-            if (!(rel == rel)) {
-                set_error("Poisson PCG diverged (NaN residual) at iteration %d", it);
-                return PIC_ERR_SOLVER;
-            }
-            if (rel <= tol) break;
+    while (true) {
+        PIC_TRY(read_state(c));
+        if (c->flags_host[0] || it >= c->p.poisson_max_iter) break;
+        for (int k = 0; k < kBatch && it < c->p.poisson_max_iter; ++k) {
+            ++it;
+            hipLaunchKernelGGL(k_apply, dim3(kNP), dim3(kB), 0, s, f, c->pp, c->dir, c->pq, part + P_PQ * kNP,
+                               c->flags);
+            hipLaunchKernelGGL(k_update_xr, dim3(kNP), dim3(kB), 0, s, G, c->phi, c->pr, c->pp, c->pq,
+                               part + P_PQ * kNP, part + P_RZ * kNP, part + P_RR * kNP, c->flags);
+            hipLaunchKernelGGL(k_check, dim3(1), dim3(kB), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2, c->flags,
+                               c->scal, 0);
             PIC_TRY(precondition(c, 0));
-            hipLaunchKernelGGL(k_update_p, dim3(nred(G)), dim3(kB), 0, s, G, c->pp, c->pz, c->pzold, c->scal, 0);
-            hipLaunchKernelGGL(k_roll_rz, dim3(1), dim3(1), 0, s, c->scal);
+            hipLaunchKernelGGL(k_update_p, dim3(kNP), dim3(kB), 0, s, G, c->pp, c->pz, c->pzold, part + P_RZ * kNP,
+                               part + P_RZO * kNP, c->scal, c->flags, it & 1, 0);
+            PIC_HIP(hipGetLastError());
         }
-        if (it > c->p.poisson_max_iter) {
-            set_error("Poisson PCG did not reach relative residual %.3e in %d iterations (%.3e)", tol,
-                      c->p.poisson_max_iter, rel);
-            return PIC_ERR_SOLVER;
-        }
+    }
+    const double bb = c->scal_host[S_BB];
+    const double bnorm = std::sqrt(bb);
+    const double rel0 = bnorm > 0 ? std::sqrt(c->scal_host[S_RR0]) / bnorm : 0.0;
+    const double rel = bnorm > 0 ? std::sqrt(c->scal_host[S_RR]) / bnorm : 0.0;
+    const int iters = c->flags_host[1];
+    if (c->flags_host[0] == 2) {
+        set_error("Poisson PCG diverged (NaN residual) after %d iterations", iters);
+        return PIC_ERR_SOLVER;
+    }
+    if (c->flags_host[0] == 0) {
+        set_error("Poisson PCG did not reach relative residual %.3e in %d iterations (%.3e)", c->p.poisson_tol,
+                  iters, rel);
+        return PIC_ERR_SOLVER;
     }
     PIC_TRY(launch_overwrite_phi(c));
     PIC_HIP(hipEventRecord(t1, s));
@@ -548,7 +777,7 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     c->have_prev_phi = 1;
     PIC_TRY(timer_end(c, PIC_STAGE_POISSON, ev));
     if (st) {
-        st->iterations = it;
+        st->iterations = iters;
         st->initial_relres = rel0;
         st->final_relres = rel;
         st->ms = ms;

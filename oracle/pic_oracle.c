@@ -221,30 +221,12 @@ void orc_efield(int gxn, int gyn, double dx, double dy, const double* phi, doubl
 /* ---------------------------------------------------------------- push */
 static inline double fld_at(const double* f, int64_t G, int64_t idx) { return idx < G ? f[idx] : 0.0; }
 
-/* deterministic_mover (src/mover.cpp:103-176) + the boundary test and new key
- * of MoverKernel (:304-364).  Gather: gx = (int)(x/dx) (division), lx = x - gx*dx,
- * bilinear sum over (dx*dy) at flat nodes gy*GX+gx, +1, +GX, +GX+1 (reads past
- * the last node see 0).  Boris with Bz (:151-166); with bx or by non-zero the
- * rotation uses the full t = (q/m) B dt / 2 vector.  Absorbed (returns -1)
- * when x >= W, x < 0, y >= H or y < 0 (:325); left[0] counts x < 0 (:344-352). */
-int32_t orc_push_one(double* px_, double* py_, double* pvx_, double* pvy_, double* pvz_, const double* ex,
-                     const double* ey, const double* bz, int gxn, int gyn, double dx, double dy, double width,
-                     double height, double qm, double dt, double bx, double by, int64_t* absorbed) {
-    const int64_t G = (int64_t)gxn * gyn;
-    double px = *px_, py = *py_, pvx = *pvx_, pvy = *pvy_, pvz = *pvz_;
-    const double pctm = qm;
-    int gx = (int)(px / dx);
-    int gy = (int)(py / dy);
-    double lx = px - gx * dx;
-    double ly = py - gy * dy;
-    int64_t i00 = (int64_t)gy * gxn + gx;
-    int64_t i10 = i00 + 1, i01 = i00 + gxn, i11 = i00 + gxn + 1;
-    double Epx = (fld_at(ex, G, i00) * (dx - lx) * (dy - ly) + fld_at(ex, G, i10) * lx * (dy - ly) +
-                  fld_at(ex, G, i01) * (dx - lx) * ly + fld_at(ex, G, i11) * lx * ly) / (dx * dy);
-    double Epy = (fld_at(ey, G, i00) * (dx - lx) * (dy - ly) + fld_at(ey, G, i10) * lx * (dy - ly) +
-                  fld_at(ey, G, i01) * (dx - lx) * ly + fld_at(ey, G, i11) * lx * ly) / (dx * dy);
-    double Bpz = (fld_at(bz, G, i00) * (dx - lx) * (dy - ly) + fld_at(bz, G, i10) * lx * (dy - ly) +
-                  fld_at(bz, G, i01) * (dx - lx) * ly + fld_at(bz, G, i11) * lx * ly) / (dx * dy);
+/* Boris velocity update of deterministic_mover (src/mover.cpp:151-162) for
+ * Bz; with bx or by non-zero the same rotation with the full t = (q/m) B dt/2
+ * vector: v' = v- + v- x t, v+ = v- + v' x s, s = 2t / (1 + |t|^2). */
+static inline void orc_boris(double Epx, double Epy, double Bpz, double pctm, double dt, double bx, double by,
+                             double* pvx_, double* pvy_, double* pvz_) {
+    double pvx = *pvx_, pvy = *pvy_, pvz = *pvz_;
     double Vxminus = pvx + 0.5 * pctm * Epx * dt;
     double Vyminus = pvy + 0.5 * pctm * Epy * dt;
     if (bx == 0.0 && by == 0.0) {
@@ -255,7 +237,6 @@ int32_t orc_push_one(double* px_, double* py_, double* pvx_, double* pvy_, doubl
         pvx = Vxminus + Vyprime * s + 0.5 * pctm * Epx * dt;
         pvy = Vyminus - Vxprime * s + 0.5 * pctm * Epy * dt;
     } else {
-        /* v' = v- + v- x t ; v+ = v- + v' x s, s = 2t / (1 + |t|^2) */
         double tx = 0.5 * pctm * bx * dt;
         double ty = 0.5 * pctm * by * dt;
         double tz = 0.5 * pctm * Bpz * dt;
@@ -269,13 +250,38 @@ int32_t orc_push_one(double* px_, double* py_, double* pvx_, double* pvy_, doubl
         pvy = Vyminus + (vpz * sx - vpx * sz) + 0.5 * pctm * Epy * dt;
         pvz = Vzminus + (vpx * sy - vpy * sx);
     }
-    px = px + dt * pvx;
-    py = py + dt * pvy;
-    *px_ = px;
-    *py_ = py;
     *pvx_ = pvx;
     *pvy_ = pvy;
     *pvz_ = pvz;
+}
+
+/* deterministic_mover (src/mover.cpp:103-176) + the boundary test and new key
+ * of MoverKernel (:304-364).  Gather: gx = (int)(x/dx) (division), lx = x - gx*dx,
+ * bilinear sum over (dx*dy) at flat nodes gy*GX+gx, +1, +GX, +GX+1 (reads past
+ * the last node see 0).  Absorbed (returns -1) when x >= W, x < 0, y >= H or
+ * y < 0 (:325); absorbed[0] counts x < 0 (:344-352), absorbed[1] all. */
+int32_t orc_push_one(double* px_, double* py_, double* pvx_, double* pvy_, double* pvz_, const double* ex,
+                     const double* ey, const double* bz, int gxn, int gyn, double dx, double dy, double width,
+                     double height, double qm, double dt, double bx, double by, int64_t* absorbed) {
+    const int64_t G = (int64_t)gxn * gyn;
+    double px = *px_, py = *py_;
+    int gx = (int)(px / dx);
+    int gy = (int)(py / dy);
+    double lx = px - gx * dx;
+    double ly = py - gy * dy;
+    int64_t i00 = (int64_t)gy * gxn + gx;
+    int64_t i10 = i00 + 1, i01 = i00 + gxn, i11 = i00 + gxn + 1;
+    double Epx = (fld_at(ex, G, i00) * (dx - lx) * (dy - ly) + fld_at(ex, G, i10) * lx * (dy - ly) +
+                  fld_at(ex, G, i01) * (dx - lx) * ly + fld_at(ex, G, i11) * lx * ly) / (dx * dy);
+    double Epy = (fld_at(ey, G, i00) * (dx - lx) * (dy - ly) + fld_at(ey, G, i10) * lx * (dy - ly) +
+                  fld_at(ey, G, i01) * (dx - lx) * ly + fld_at(ey, G, i11) * lx * ly) / (dx * dy);
+    double Bpz = (fld_at(bz, G, i00) * (dx - lx) * (dy - ly) + fld_at(bz, G, i10) * lx * (dy - ly) +
+                  fld_at(bz, G, i01) * (dx - lx) * ly + fld_at(bz, G, i11) * lx * ly) / (dx * dy);
+    orc_boris(Epx, Epy, Bpz, qm, dt, bx, by, pvx_, pvy_, pvz_);
+    px = px + dt * (*pvx_);
+    py = py + dt * (*pvy_);
+    *px_ = px;
+    *py_ = py;
     int out = (px >= width || px < 0.0 || py >= height || py < 0);
     if (out) {
         if (absorbed) {
@@ -306,38 +312,55 @@ void orc_push_f64(int64_t n, double* x, double* y, double* vx, double* vy, doubl
     }
 }
 
-/* CELL32 storage: decode x = (gx + ox) * dx, push in FP64 as above, encode
- * cell = new key, ox = (float)(x * inv_dx - gx) kept below 1. */
+/* CELL32 storage (the build's compact layout): key cell (cgx, cgy), float
+ * offsets (fx, fy) in cell units, float velocities.  Push in FP64 in cell
+ * units: bilinear weights (1-fx)(1-fy), fx(1-fy), (1-fx)fy, fx*fy on nodes
+ * cell, +1, +GX, +GX+1; the reference's Boris step; new position
+ * t = (cgx + fx) + v * (dt * (1.0/dx)); absorbed when t >= NX or t < 0 (same
+ * for y); new cell (int)t and offset (float)(t - (int)t) kept below 1. */
 static inline float clamp_offset(float o) { return o < 1.0f ? o : 0x1.fffffep-1f; }
 
 void orc_push_cell32(int64_t n, int32_t* cell, float* ox, float* oy, float* vx, float* vy, float* vz, int32_t* key,
                      const double* ex, const double* ey, const double* bz, int gxn, int gyn, double dx, double dy,
                      double width, double height, double qm, double dt, double bx, double by, int64_t* absorbed) {
     int64_t a0 = 0, a1 = 0;
-    const double inv_dx = 1.0 / dx, inv_dy = 1.0 / dy;
+    const int64_t G = (int64_t)gxn * gyn;
+    const double dtdx = dt * (1.0 / dx), dtdy = dt * (1.0 / dy);
+    const double NX = gxn - 1, NY = gyn - 1;
+    (void)width;
+    (void)height;
 #pragma omp parallel for schedule(static) reduction(+ : a0, a1)
     for (int64_t i = 0; i < n; ++i) {
         int c = cell[i];
         int cgx = c % gxn, cgy = c / gxn;
-        double px = ((double)cgx + (double)ox[i]) * dx;
-        double py = ((double)cgy + (double)oy[i]) * dy;
+        double fx = (double)ox[i], fy = (double)oy[i];
+        double w00 = (1.0 - fx) * (1.0 - fy), w10 = fx * (1.0 - fy), w01 = (1.0 - fx) * fy, w11 = fx * fy;
+        int64_t n00 = c, n10 = c + 1, n01 = c + gxn, n11 = c + gxn + 1;
+        double Epx = fld_at(ex, G, n00) * w00 + fld_at(ex, G, n10) * w10 + fld_at(ex, G, n01) * w01 +
+                     fld_at(ex, G, n11) * w11;
+        double Epy = fld_at(ey, G, n00) * w00 + fld_at(ey, G, n10) * w10 + fld_at(ey, G, n01) * w01 +
+                     fld_at(ey, G, n11) * w11;
+        double Bpz = fld_at(bz, G, n00) * w00 + fld_at(bz, G, n10) * w10 + fld_at(bz, G, n01) * w01 +
+                     fld_at(bz, G, n11) * w11;
         double pvx = vx[i], pvy = vy[i], pvz = vz[i];
-        int64_t ab[2] = {0, 0};
-        int32_t k = orc_push_one(&px, &py, &pvx, &pvy, &pvz, ex, ey, bz, gxn, gyn, dx, dy, width, height, qm, dt,
-                                 bx, by, ab);
-        a0 += ab[0];
-        a1 += ab[1];
-        key[i] = k;
-        if (k >= 0) {
-            double tx = px * inv_dx, ty = py * inv_dy;
-            int ngx = (int)tx, ngy = (int)ty;
-            cell[i] = k;
-            ox[i] = clamp_offset((float)(tx - (double)ngx));
-            oy[i] = clamp_offset((float)(ty - (double)ngy));
-            vx[i] = (float)pvx;
-            vy[i] = (float)pvy;
-            vz[i] = (float)pvz;
+        orc_boris(Epx, Epy, Bpz, qm, dt, bx, by, &pvx, &pvy, &pvz);
+        double tx = ((double)cgx + fx) + pvx * dtdx;
+        double ty = ((double)cgy + fy) + pvy * dtdy;
+        if (tx >= NX || tx < 0.0 || ty >= NY || ty < 0.0) {
+            if (tx < 0.0) a0++;
+            a1++;
+            key[i] = -1;
+            continue;
         }
+        int ngx = (int)tx, ngy = (int)ty;
+        int32_t k = ngy * gxn + ngx;
+        key[i] = k;
+        cell[i] = k;
+        ox[i] = clamp_offset((float)(tx - (double)ngx));
+        oy[i] = clamp_offset((float)(ty - (double)ngy));
+        vx[i] = (float)pvx;
+        vy[i] = (float)pvy;
+        vz[i] = (float)pvz;
     }
     if (absorbed) {
         absorbed[0] += a0;

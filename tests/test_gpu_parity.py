@@ -209,3 +209,103 @@ def test_poisson_electrode_vs_direct(hip_mod, oracle_mod):
     st2 = c.poisson_solve(1)
     assert st2["iterations"] <= 1 and st2["guess"] == 1
     c.close()
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_move_fused_deposit_parity(hip_mod, oracle_mod, layout):
+    """The deposit accumulated inside the fused move (next step's charge)
+    equals the oracle's fixed-point deposit of the pushed particles."""
+    hip, O = hip_mod, oracle_mod
+    c = _ctx(hip, 64, layout, bx=0.025)
+    rng = np.random.default_rng(21)
+    ex, ey = rng.normal(0, 3e3, (2, c.G))
+    c.set_grid(hip.GRID_EX, ex)
+    c.set_grid(hip.GRID_EY, ey)
+    bz = c.grid(hip.GRID_BZ)
+    q, m = -E, ME
+    s = c.add_species(q, m, 60000, 1e16, 60000)
+    x, y, vx, vy, vz = _parts(hip, c, 60000, seed=12)
+    vx, vy = vx * 20, vy * 20
+    c.upload(s, x, y, vx, vy, vz)
+    g = dict(gx=c.gx, gy=c.gy, dx=c.dx, dy=c.dy, width=c.params.width, height=c.params.height)
+    if layout == 0:
+        X, Y, VX, VY, VZ, key, ab = O.push_f64(x, y, vx, vy, vz, ex, ey, bz, qm=q / m, dt=c.params.dt, bx=0.025, **g)
+        live = key >= 0
+        rqc, rqe = O.deposit_fixed_f64(X[live], Y[live], VX[live], VY[live], VZ[live], key[live], c.gx, c.gy, c.dx,
+                                       c.dy, 40, 32, q / m)
+    else:
+        cell, ox, oy, fvx, fvy, fvz = O.encode_cell32(x, y, vx, vy, vz, c.gx, c.dx, c.dy)
+        rc, rox, roy, rvx, rvy, rvz, key, ab = O.push_cell32(cell, ox, oy, fvx, fvy, fvz, ex, ey, bz, qm=q / m,
+                                                             dt=c.params.dt, bx=0.025, **g)
+        live = key >= 0
+        rqc, rqe = O.deposit_fixed_cell32(rc[live], rox[live], roy[live], rvx[live], rvy[live], rvz[live], c.gx,
+                                          c.gy, 40, 32, q / m)
+    c.push(s, rebin=True)
+    c.zero_rho()
+    c.deposit(s)
+    qc, qe = c.deposit_fixed(s)
+    assert np.array_equal(qc, rqc)
+    assert np.array_equal(qe, rqe)
+    # a second, explicit bin leaves membership and deposit unchanged
+    counts = c.cell_counts(s)
+    c.bin(s)
+    assert np.array_equal(c.cell_counts(s), counts)
+    c.close()
+
+
+def test_inplace_push_then_bin(hip_mod, oracle_mod):
+    """pic_push without re-bin keeps storage order; deposit and bin still agree."""
+    hip, O = hip_mod, oracle_mod
+    c = _ctx(hip, 32, 0)
+    rng = np.random.default_rng(2)
+    ex = rng.normal(0, 1e3, c.G)
+    c.set_grid(hip.GRID_EX, ex)
+    bz = c.grid(hip.GRID_BZ)
+    ey = c.grid(hip.GRID_EY)
+    s = c.add_species(-E, ME, 20000, 1e16, 20000)
+    x, y, vx, vy, vz = _parts(hip, c, 20000, seed=4)
+    vx = vx * 40
+    c.upload(s, x, y, vx, vy, vz)
+    g = dict(gx=c.gx, gy=c.gy, dx=c.dx, dy=c.dy, width=c.params.width, height=c.params.height)
+    X, Y, VX, VY, VZ, key, ab = O.push_f64(x, y, vx, vy, vz, ex, ey, bz, qm=-E / ME, dt=c.params.dt, **g)
+    live = key >= 0
+    assert (~live).sum() > 0
+    c.push(s, rebin=False)
+    c.deposit(s)
+    qc, _ = c.deposit_fixed(s)
+    rqc, _ = O.deposit_fixed_f64(X[live], Y[live], VX[live], VY[live], VZ[live], key[live], c.gx, c.gy, c.dx, c.dy,
+                                 40, 32, -E / ME)
+    assert np.array_equal(qc, rqc)
+    c.bin(s)
+    assert c.count(s) == live.sum()
+    counts, _, _ = O.bin(key, c.G)
+    assert np.array_equal(c.cell_counts(s), counts)
+    c.close()
+
+
+def test_full_steps_vs_cpu_baseline(hip_mod, oracle_mod):
+    """Five full steps (F64 layout, electrode, tilted B) against the oracle's
+    CPU baseline (PARDISO or direct): rho and phi to 1e-9, absorbed counts
+    equal, per-cell counts equal up to particles sitting on a face."""
+    hip, O = hip_mod, oracle_mod
+    import pic_configs as cfg
+
+    w = cfg.WORKLOADS["electrode_512"].scaled(NX=64, n=30000, name="steps_64")
+    parts = [cfg.generate(w, s) for s in range(2)]
+    ctx = cfg.make_context(w, layout=0, particles=parts, poisson_tol=1e-13)
+    sim = O.Sim(w.NX, w.NY, w.width, w.height, w.dt, cfg.EPS, w.bz, w.rects(), bx=w.bx)
+    for s, (q, m, _) in enumerate(w.species()):
+        sim.add_species(q, m, w.n_per_species, cfg.N0, *parts[s])
+    for it in range(5):
+        ctx.step(it)
+        sim.step()
+        for which in (hip.GRID_RHO, hip.GRID_PHI):
+            a, b = ctx.grid(which), sim.grid(which)
+            assert np.abs(a - b).max() / np.abs(b).max() < 1e-9, (it, which)
+    for s in range(2):
+        assert abs(ctx.count(s) - sim.count(s)) <= 2
+        ca = ctx.cell_counts(s)
+        cb = np.bincount(sim.particles(s)[5], minlength=ctx.G)
+        assert np.abs(ca - cb).sum() <= 4
+    ctx.close()
+    sim.close()
