@@ -22,11 +22,18 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "bmp-6th-sem_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic HBM bytes per particle of the particle kernels (CELL32 / F64 layout):
-# push: read state (cell + 2 offsets + 3 velocities), write state + key.
-PUSH_BYTES = {1: 24 + 24, 0: 44 + 44}
-# bin scatter: read state minus cell + key + rank, write state.
-SCATTER_BYTES = {1: 20 + 8 + 24, 0: 40 + 8 + 44}
+# Algorithmic HBM bytes of the particle kernels, per particle (and per cell),
+# for the CELL32 (1) and F64 (0) layouts:
+#   k_push_tile    reads the state (cell, 2 offsets, 3 velocities), writes the slot;
+#   k_scatter_push reads the state and the slot, writes the state to its sorted slot;
+#   k_deposit_tile reads offsets (+ velocities for the energy mesh) per particle and,
+#                  per cell, its start index and four charge + four energy corner sums.
+KERNEL_BYTES = {
+    "push": {1: (24 + 4, 0), 0: (44 + 4, 0)},
+    "scatter": {1: (24 + 4 + 24, 0), 0: (44 + 4 + 44, 0)},
+    "deposit": {1: (8 + 12, 4 + 64), 0: (16 + 24, 4 + 64)},
+}
+KERNEL_NAMES = {"push": "k_push_tile", "scatter": "k_scatter_push", "deposit": "k_deposit_tile"}
 
 
 def log(msg):
@@ -119,22 +126,33 @@ def main():
     ms_step = 1e3 * elapsed / a.steps
     log(f"gpu: {value:.3e} particle-steps/s, {ms_step:.3f} ms/step")
 
-    # roofline of the push kernel (one launch per species per step)
-    push_ms, push_n = stages["push"]
-    avg_push_s = push_ms / max(push_n, 1) * 1e-3
+    # roofline of the dominant particle kernel (one launch per species per step)
+    dom = max(KERNEL_BYTES, key=lambda k: stages[k][0])
+    k_ms, k_n = stages[dom]
+    avg_s = k_ms / max(k_n, 1) * 1e-3
     n_avg = 0.5 * (n_start + n_end) / 2           # particles per species per launch
-    push_bytes = PUSH_BYTES[layout] * n_avg
-    achieved = push_bytes / avg_push_s / 1e9 if avg_push_s > 0 else None
+    per_p, per_c = KERNEL_BYTES[dom][layout]
+    kbytes = per_p * n_avg + per_c * (w.NX + 1) * (w.NY + 1)
+    achieved = kbytes / avg_s / 1e9 if avg_s > 0 else None
     traffic = None
-    tf = ROOT / "profiles" / "push_traffic.json"
+    tf = ROOT / "profiles" / f"{KERNEL_NAMES[dom]}_traffic.json"
     if tf.exists():
         try:
             traffic = json.loads(tf.read_text()).get("bytes_per_launch")
         except Exception:
             traffic = None
-    roof = {"kernel": "k_push", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    roof = {"kernel": KERNEL_NAMES[dom], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
-            "bytes_per_particle": PUSH_BYTES[layout], "avg_launch_us": avg_push_s * 1e6}
+            "bytes_per_launch": kbytes, "bytes_per_particle": per_p, "bytes_per_cell": per_c,
+            "avg_launch_us": avg_s * 1e6}
+    others = {}
+    for k in KERNEL_BYTES:
+        ms_k, n_k = stages[k]
+        if n_k:
+            pp, pc = KERNEL_BYTES[k][layout]
+            b = pp * n_avg + pc * (w.NX + 1) * (w.NY + 1)
+            others[KERNEL_NAMES[k]] = {"avg_launch_us": round(ms_k / n_k * 1e3, 2),
+                                       "GB_s": round(b / (ms_k / n_k * 1e-3) / 1e9, 1)}
 
     cpu = None
     if not a.no_cpu:
@@ -161,8 +179,10 @@ def main():
                    "species": 2, "electrode_V": w.electrode_v if w.electrode else None,
                    "B_T": w.b_total, "B_tilt_deg": w.b_tilt_deg, "poisson_tol": 1e-10},
         "stages_ms_per_step": {k: round(v[0] / a.steps, 4) for k, v in stages.items()},
-        "poisson": {"iterations_mean": float(np.mean(iters)), "ms_mean": float(np.mean([s["ms"] for s in stats]))},
+        "poisson": {"iterations_mean": float(np.mean(iters)), "ms_mean": float(np.mean([s["ms"] for s in stats])),
+                    "initial_relres_mean": float(np.mean([s["initial_relres"] for s in stats]))},
         "roofline": roof,
+        "particle_kernels": others,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
     }

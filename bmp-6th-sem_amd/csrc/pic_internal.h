@@ -129,6 +129,9 @@ struct pic_ctx {
     int guess_set = 0;
     int have_prev_phi = 0;
     std::vector<pic::MgLevel> mg;
+    float* mg_inv = nullptr;        // inverse of the coarsest level's operator (unknowns only)
+    int* mg_unk = nullptr;          // node index of each coarsest-level unknown
+    int mg_nu = 0;
     double* part = nullptr;         // per-block partial sums of the dot products
     double* scal = nullptr;         // device scalars
     int* flags = nullptr;           // [0] converged / failed, [1] iterations

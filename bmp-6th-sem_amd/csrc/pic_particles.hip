@@ -253,90 +253,105 @@ __device__ __forceinline__ double particle_energy(const ParticleBuf& b, int i, c
     return 0.5 * v2 * a.denom;
 }
 
-// One workgroup per tile of kDepC consecutive cells of one key row (the
-// cells' particles are contiguous).  Lanes take consecutive particles
-// (coalesced loads); a wave-level segmented scan over the sorted keys sums
-// each cell's corner weights, and the last lane of each cell segment adds the
-// sums to the tile's LDS cell accumulators.  The tile then writes every
-// cell's four corner sums to the node-indexed corner meshes — the reference's
-// per-cell reduction and corner meshes (:56-148): corner c of cell k goes to
-// node k, k+1, k+GX, k+GX+1.  Every cell is written by exactly one tile, so no
-// global atomics; corner-mesh entries that no cell writes stay zero.
+// One workgroup per tile of kDepC consecutive cells of one key row; thread
+// t owns cell c0 + t — the reference's per-cell reduction and corner meshes
+// (:56-148).  The tile's particles (contiguous in cell-sorted storage) are
+// staged through LDS in chunks with coalesced loads; each thread then adds
+// the fixed-point corner weights of its own cell's particles in registers
+// and writes its four corner sums to the node-indexed corner meshes: corner c
+// of cell k goes to node k, k+1, k+GX, k+GX+1.  Every cell is written by
+// exactly one thread: no atomics; entries no cell writes stay zero.
 static constexpr int kDepC = 256;
+static constexpr int kDepChunk = 1024;        // particles staged per pass
 
-__device__ __forceinline__ long long seg_scan_step(long long v, int d, int lane, int head) {
-    long long t = __shfl_up(v, d, 64);
-    return (lane - d >= head) ? v + t : v;
-}
+template <int L> struct DepStage;
+template <> struct DepStage<PIC_LAYOUT_F64> {
+    double a[kDepChunk / 2], b[kDepChunk / 2], vx[kDepChunk / 2], vy[kDepChunk / 2], vz[kDepChunk / 2];
+    static constexpr int n = kDepChunk / 2;
+};
+template <> struct DepStage<PIC_LAYOUT_CELL32> {
+    float a[kDepChunk], b[kDepChunk], vx[kDepChunk], vy[kDepChunk], vz[kDepChunk];
+    static constexpr int n = kDepChunk;
+};
 
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_deposit_tile(
     ParticleBuf b, GridDev g, DepositArgs a, int tiles_per_row, const int32_t* __restrict__ start,
     long long* __restrict__ cq, long long* __restrict__ ce)
 {
-    __shared__ long long acc[8][kDepC];
-    const int tid = threadIdx.x, lane = tid & 63;
+    using P = typename Lay<L>::P;
+    using V = typename Lay<L>::V;
+    __shared__ DepStage<L> st;
+    constexpr int CH = DepStage<L>::n;
+    const int tid = threadIdx.x;
     const int r = blockIdx.x / tiles_per_row;
     const int c0 = (blockIdx.x % tiles_per_row) * kDepC;
     const int cw = min(kDepC, g.gx - c0);
     const int k0 = r * g.gx + c0;
-    const int nv = a.energy ? 8 : 4;
-    for (int j = tid; j < 8 * kDepC; j += kBlock) acc[j / kDepC][j % kDepC] = 0;
-    __syncthreads();
-    const bool live_row = r + 1 < g.gy;
     const int beg = start[k0], end = start[k0 + cw];
-    const unsigned long long le_mask = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-    if (live_row) {
-        for (int base = beg; base < end; base += kBlock) {
-            const int i = base + tid;
-            int lc = -1;
-            long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            if (i < end) {
-                const int kx = b.cell[i] - r * g.gx;
-                lc = kx - c0;
-                if (kx + 1 < g.gx) {
-                    double w[4];
-                    cic_weights<L>(b, i, kx, r, g, a, w);
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) v[c] = __double2ll_rn(w[c] * a.scale_q);
-                    if (a.energy) {
-                        double en = particle_energy<L>(b, i, a);
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) v[4 + c] = __double2ll_rn((w[c] * en) * a.scale_e);
-                    }
-                }
+    const bool mine = tid < cw;
+    const int kx = c0 + tid;
+    const bool live = mine && (r + 1 < g.gy) && (kx + 1 < g.gx);
+    const int ms = mine ? start[k0 + tid] : 0, me = mine ? start[k0 + tid + 1] : 0;
+    long long s0 = 0, s1 = 0, s2 = 0, s3 = 0, e0 = 0, e1 = 0, e2 = 0, e3 = 0;
+    for (int cb = beg; cb < end; cb += CH) {
+        const int cn = min(CH, end - cb);
+        __syncthreads();
+        for (int j = tid; j < cn; j += kBlock) {
+            st.a[j] = ((const P*)b.pa)[cb + j];
+            st.b[j] = ((const P*)b.pb)[cb + j];
+            if (a.energy) {
+                st.vx[j] = ((const V*)b.vx)[cb + j];
+                st.vy[j] = ((const V*)b.vy)[cb + j];
+                st.vz[j] = ((const V*)b.vz)[cb + j];
             }
-            // segment heads: first lane of each run of equal cells
-            const int prev = __shfl_up(lc, 1, 64);
-            const unsigned long long heads = __ballot(lane == 0 || lc != prev);
-            const int head = 63 - __clzll(heads & le_mask);
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-#pragma unroll
-                for (int c = 0; c < 8; ++c)
-                    if (c < nv) v[c] = seg_scan_step(v[c], d, lane, head);
+        }
+        __syncthreads();
+        if (!live) continue;
+        const int lo = max(ms, cb) - cb, hi = min(me, cb + cn) - cb;
+        for (int j = lo; j < hi; ++j) {
+            double w[4];
+            if (L == PIC_LAYOUT_F64) {
+                double lx = (double)st.a[j] - kx * g.dx;
+                double ly = (double)st.b[j] - r * g.dy;
+                w[0] = ((g.dx - lx) * (g.dy - ly)) * a.inv_area;
+                w[1] = (lx * (g.dy - ly)) * a.inv_area;
+                w[2] = ((g.dx - lx) * ly) * a.inv_area;
+                w[3] = (lx * ly) * a.inv_area;
+            } else {
+                double fx = (double)st.a[j], fy = (double)st.b[j];
+                w[0] = (1.0 - fx) * (1.0 - fy);
+                w[1] = fx * (1.0 - fy);
+                w[2] = (1.0 - fx) * fy;
+                w[3] = fx * fy;
             }
-            const bool tail = (lane == 63) || ((heads >> (lane + 1)) & 1ull);
-            if (tail && lc >= 0) {
-                for (int c = 0; c < nv; ++c)
-                    if (v[c]) atomicAdd((unsigned long long*)&acc[c][lc], (unsigned long long)v[c]);
+            s0 += __double2ll_rn(w[0] * a.scale_q);
+            s1 += __double2ll_rn(w[1] * a.scale_q);
+            s2 += __double2ll_rn(w[2] * a.scale_q);
+            s3 += __double2ll_rn(w[3] * a.scale_q);
+            if (a.energy) {
+                double pvx = (double)st.vx[j], pvy = (double)st.vy[j], pvz = (double)st.vz[j];
+                double v2 = pvx * pvx + pvy * pvy + pvz * pvz;
+                double en = 0.5 * v2 * a.denom;
+                e0 += __double2ll_rn((w[0] * en) * a.scale_e);
+                e1 += __double2ll_rn((w[1] * en) * a.scale_e);
+                e2 += __double2ll_rn((w[2] * en) * a.scale_e);
+                e3 += __double2ll_rn((w[3] * en) * a.scale_e);
             }
         }
     }
-    __syncthreads();
+    if (!mine) return;
     const int gp = g.gp;
-    for (int c = tid; c < cw; c += kBlock) {
-        const int k = k0 + c;
-        cq[k] = acc[0][c];
-        cq[gp + k + 1] = acc[1][c];
-        cq[2 * gp + k + g.gx] = acc[2][c];
-        cq[3 * gp + k + g.gx + 1] = acc[3][c];
-        if (a.energy) {
-            ce[k] = acc[4][c];
-            ce[gp + k + 1] = acc[5][c];
-            ce[2 * gp + k + g.gx] = acc[6][c];
-            ce[3 * gp + k + g.gx + 1] = acc[7][c];
-        }
+    const int k = k0 + tid;
+    cq[k] = s0;
+    cq[gp + k + 1] = s1;
+    cq[2 * gp + k + g.gx] = s2;
+    cq[3 * gp + k + g.gx + 1] = s3;
+    if (a.energy) {
+        ce[k] = e0;
+        ce[gp + k + 1] = e1;
+        ce[2 * gp + k + g.gx] = e2;
+        ce[3 * gp + k + g.gx + 1] = e3;
     }
 }
 
@@ -441,8 +456,8 @@ __device__ __forceinline__ void boris(double Epx, double Epy, double Bpz, const 
         double vpx = Vxminus + (Vyminus * tz - Vzminus * ty);
         double vpy = Vyminus + (Vzminus * tx - Vxminus * tz);
         double vpz = Vzminus + (Vxminus * ty - Vyminus * tx);
-        double den = 1 + (tx * tx + ty * ty + tz * tz);
-        double sx = 2 * tx / den, sy = 2 * ty / den, sz = 2 * tz / den;
+        double rden = 1 / (1 + (tx * tx + ty * ty + tz * tz));
+        double sx = (2 * tx) * rden, sy = (2 * ty) * rden, sz = (2 * tz) * rden;
         pvx = Vxminus + (vpy * sz - vpz * sy) + hx;
         pvy = Vyminus + (vpz * sx - vpx * sz) + hy;
         pvz = Vzminus + (vpx * sy - vpy * sx);
@@ -808,8 +823,10 @@ int launch_deposit(pic_ctx* c, int s) {
         PIC_HIP(hipGetLastError());
         sp.deposit_pending = 1;
     }
-    PIC_TRY(finalize_deposit(c, sp));
     PIC_TRY(timer_end(c, PIC_STAGE_DEPOSIT, ev));
+    PIC_TRY(timer_begin(c, PIC_STAGE_FINALIZE, &ev));
+    PIC_TRY(finalize_deposit(c, sp));
+    PIC_TRY(timer_end(c, PIC_STAGE_FINALIZE, ev));
     return PIC_OK;
 }
 
@@ -877,6 +894,8 @@ int launch_move(pic_ctx* c, int s) {
     PIC_TRY(timer_end(c, PIC_STAGE_PUSH, ev));
     PIC_TRY(timer_begin(c, PIC_STAGE_BIN, &ev));
     PIC_TRY(scan_counts(c, sp, dst));
+    PIC_TRY(timer_end(c, PIC_STAGE_BIN, ev));
+    PIC_TRY(timer_begin(c, PIC_STAGE_SCATTER, &ev));
     if (sp.n_upper > 0) {
         if (f64 && !d.general_b) scatter_kernel<PIC_LAYOUT_F64, 0>(c, sp, d, src, dst);
         else if (f64) scatter_kernel<PIC_LAYOUT_F64, 1>(c, sp, d, src, dst);
@@ -887,7 +906,7 @@ int launch_move(pic_ctx* c, int s) {
     sp.cur = dst;
     sp.deposit_pending = 0;
     sp.sorted = 1;
-    PIC_TRY(timer_end(c, PIC_STAGE_BIN, ev));
+    PIC_TRY(timer_end(c, PIC_STAGE_SCATTER, ev));
     return PIC_OK;
 }
 

@@ -211,10 +211,17 @@ __global__ __launch_bounds__(kB) void k_update_p(int G, double* __restrict__ p, 
 }
 
 // --------------------------------------------------------------- MG (FP32)
+// Level kernels use 2D grids (x: 256 nodes of a row, y: row) so node indices
+// need no division; `wd` is the Jacobi weight over the diagonal.
 struct Lvl {
     int gx, gy;
-    float cx, cy, diag;
+    float cx, cy, diag, wd;
 };
+
+#define MG_IJ(L)                                                              \
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;     \
+    if (i >= (L).gx) return;                                                  \
+    const int n = j * (L).gx + i;
 
 __global__ void k_d2f(int G, const double* __restrict__ r, float* __restrict__ b, const int* flags) {
     if (flags[0]) return;
@@ -222,44 +229,39 @@ __global__ void k_d2f(int G, const double* __restrict__ r, float* __restrict__ b
     if (n < G) b[n] = (float)r[n];
 }
 
-// Weighted Jacobi: eo = ei + w/diag (b - A ei) on unknowns; zero on Dirichlet.
-// With zero_in the input iterate is taken as 0.
-__global__ __launch_bounds__(kB) void k_mg_smooth(Lvl L, float w, const float* __restrict__ ei,
-                                                  float* __restrict__ eo, const float* __restrict__ b,
-                                                  const uint8_t* __restrict__ dir, int zero_in, const int* flags) {
+// Weighted Jacobi: eo = ei + wd (b - A ei) on unknowns; zero on Dirichlet.
+__global__ __launch_bounds__(kB) void k_mg_smooth(Lvl L, const float* __restrict__ ei, float* __restrict__ eo,
+                                                  const float* __restrict__ b, const uint8_t* __restrict__ dir,
+                                                  int zero_in, const int* flags) {
     if (flags[0]) return;
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= L.gx * L.gy) return;
+    MG_IJ(L)
     if (dir[n]) { eo[n] = 0.f; return; }
-    if (zero_in) { eo[n] = w * b[n] / L.diag; return; }
+    if (zero_in) { eo[n] = L.wd * b[n]; return; }
     float e = ei[n];
     float ax = L.diag * e - L.cx * (ei[n - 1] + ei[n + 1]) - L.cy * (ei[n - L.gx] + ei[n + L.gx]);
-    eo[n] = e + w * (b[n] - ax) / L.diag;
+    eo[n] = e + L.wd * (b[n] - ax);
 }
 
 // Two weighted-Jacobi sweeps from zero in one pass: the first sweep is
-// pointwise (e1 = w b / diag), so the second needs only b on the 5-point
-// stencil.  At the finest level b is the FP64 residual, rounded to FP32 here
-// (and stored for the restriction and post-smoothing).
+// pointwise (e1 = wd b), so the second needs only b on the 5-point stencil.
+// At the finest level b is the FP64 residual, rounded to FP32 here (and
+// stored for the restriction and post-smoothing).
 template <typename TB>
-__global__ __launch_bounds__(kB) void k_mg_pre2(Lvl L, float w, const TB* __restrict__ bin, float* __restrict__ bout,
+__global__ __launch_bounds__(kB) void k_mg_pre2(Lvl L, const TB* __restrict__ bin, float* __restrict__ bout,
                                                 float* __restrict__ eo, const uint8_t* __restrict__ dir,
                                                 const int* flags) {
     if (flags[0]) return;
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= L.gx * L.gy) return;
+    MG_IJ(L)
     float bn = (float)bin[n];
     if (bout) bout[n] = bn;
     if (dir[n]) { eo[n] = 0.f; return; }
-    const float s = w / L.diag;
-    auto e1 = [&](int m) -> float { return dir[m] ? 0.f : s * (float)bin[m]; };
-    float c = s * bn;
+    auto e1 = [&](int m) -> float { return dir[m] ? 0.f : L.wd * (float)bin[m]; };
+    float c = L.wd * bn;
     float ax = L.diag * c - L.cx * (e1(n - 1) + e1(n + 1)) - L.cy * (e1(n - L.gx) + e1(n + L.gx));
-    eo[n] = c + w * (bn - ax) / L.diag;
+    eo[n] = c + L.wd * (bn - ax);
 }
 
-__device__ __forceinline__ float mg_res(const Lvl& L, const float* e, const float* b, const uint8_t* dir, int i, int j) {
-    int n = j * L.gx + i;
+__device__ __forceinline__ float mg_res(const Lvl& L, const float* e, const float* b, const uint8_t* dir, int n) {
     if (dir[n]) return 0.f;
     float ax = L.diag * e[n] - L.cx * (e[n - 1] + e[n + 1]) - L.cy * (e[n - L.gx] + e[n + L.gx]);
     return b[n] - ax;
@@ -271,16 +273,15 @@ __global__ __launch_bounds__(kB) void k_mg_restrict(Lvl F, Lvl C, const float* _
                                                     float* __restrict__ bc, const uint8_t* __restrict__ cdir,
                                                     const int* flags) {
     if (flags[0]) return;
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= C.gx * C.gy) return;
+    MG_IJ(C)
     if (cdir[n]) { bc[n] = 0.f; return; }
-    int I = n % C.gx, J = n / C.gx;
-    int i = 2 * I, j = 2 * J;
-    float s = 4.f * mg_res(F, e, b, fdir, i, j)
-            + 2.f * (mg_res(F, e, b, fdir, i - 1, j) + mg_res(F, e, b, fdir, i + 1, j)
-                   + mg_res(F, e, b, fdir, i, j - 1) + mg_res(F, e, b, fdir, i, j + 1))
-            + (mg_res(F, e, b, fdir, i - 1, j - 1) + mg_res(F, e, b, fdir, i + 1, j - 1)
-             + mg_res(F, e, b, fdir, i - 1, j + 1) + mg_res(F, e, b, fdir, i + 1, j + 1));
+    const int f = (2 * j) * F.gx + 2 * i;
+    const int gx = F.gx;
+    float s = 4.f * mg_res(F, e, b, fdir, f)
+            + 2.f * (mg_res(F, e, b, fdir, f - 1) + mg_res(F, e, b, fdir, f + 1)
+                   + mg_res(F, e, b, fdir, f - gx) + mg_res(F, e, b, fdir, f + gx))
+            + (mg_res(F, e, b, fdir, f - gx - 1) + mg_res(F, e, b, fdir, f - gx + 1)
+             + mg_res(F, e, b, fdir, f + gx - 1) + mg_res(F, e, b, fdir, f + gx + 1));
     bc[n] = s * 0.0625f;
 }
 
@@ -300,47 +301,49 @@ __global__ __launch_bounds__(kB) void k_mg_prolong(Lvl F, Lvl C, float* __restri
                                                    const uint8_t* __restrict__ fdir, const float* __restrict__ ec,
                                                    const int* flags) {
     if (flags[0]) return;
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= F.gx * F.gy) return;
+    MG_IJ(F)
     if (fdir[n]) return;
-    e[n] += prolong_at(C, ec, n % F.gx, n / F.gx);
+    e[n] += prolong_at(C, ec, i, j);
 }
 
 // Prolongation fused with the first post-smoothing sweep:
 // e1 = e + P ec (computed at the node and its four neighbours), eo = one
 // weighted-Jacobi sweep of e1.
-__global__ __launch_bounds__(kB) void k_mg_prolong_sweep(Lvl F, Lvl C, float w, const float* __restrict__ e,
+__global__ __launch_bounds__(kB) void k_mg_prolong_sweep(Lvl F, Lvl C, const float* __restrict__ e,
                                                          const float* __restrict__ ec, const float* __restrict__ b,
                                                          const uint8_t* __restrict__ dir, float* __restrict__ eo,
                                                          const int* flags) {
     if (flags[0]) return;
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= F.gx * F.gy) return;
+    MG_IJ(F)
     if (dir[n]) { eo[n] = 0.f; return; }
-    const int i = n % F.gx, j = n / F.gx;
     auto e1 = [&](int m, int mi, int mj) -> float { return dir[m] ? 0.f : e[m] + prolong_at(C, ec, mi, mj); };
     float c = e1(n, i, j);
     float ax = F.diag * c - F.cx * (e1(n - 1, i - 1, j) + e1(n + 1, i + 1, j))
              - F.cy * (e1(n - F.gx, i, j - 1) + e1(n + F.gx, i, j + 1));
-    eo[n] = c + w * (b[n] - ax) / F.diag;
+    eo[n] = c + F.wd * (b[n] - ax);
 }
 
 // ---------------------------------------------- coarse levels in one workgroup
+// Levels of at most 65 x 65 nodes run in LDS; the coarsest level is solved
+// exactly with its inverse (precomputed on the host, unknowns only).
 static constexpr int kLdsMaxLev = 8;
 static constexpr int kLdsNodes = 5800;       // all in-LDS levels together
 static constexpr int kLdsPerThread = 5;      // nodes per thread per level (1024 threads)
+static constexpr int kDirectMax = 64;        // unknowns of a directly solved coarsest level
 
 struct CoarseLevels {
     int nl;
     int gx[kLdsMaxLev], gy[kLdsMaxLev], off[kLdsMaxLev];
-    float cx[kLdsMaxLev], cy[kLdsMaxLev], diag[kLdsMaxLev];
+    float cx[kLdsMaxLev], cy[kLdsMaxLev], diag[kLdsMaxLev], wd[kLdsMaxLev];
     const uint8_t* dir[kLdsMaxLev];
-    float w;
     int pre, post, sweeps;
+    int nu;                       // unknowns of the coarsest level solved directly (0: use sweeps)
+    const float* inv;             // nu x nu inverse
+    const int* unk;               // node index of each unknown
 };
 
 __device__ __forceinline__ void lds_sweep(float* E, const float* B, const uint8_t* D, int gx, int gy, float cx,
-                                          float cy, float diag, float w, bool zero_in) {
+                                          float cy, float diag, float wd, bool zero_in) {
     const int G = gx * gy;
     float nv[kLdsPerThread];
 #pragma unroll
@@ -349,11 +352,11 @@ __device__ __forceinline__ void lds_sweep(float* E, const float* B, const uint8_
         nv[j] = 0.f;
         if (n < G && !D[n]) {
             if (zero_in) {
-                nv[j] = w * B[n] / diag;
+                nv[j] = wd * B[n];
             } else {
                 float e = E[n];
                 float ax = diag * e - cx * (E[n - 1] + E[n + 1]) - cy * (E[n - gx] + E[n + gx]);
-                nv[j] = e + w * (B[n] - ax) / diag;
+                nv[j] = e + wd * (B[n] - ax);
             }
         }
     }
@@ -389,33 +392,42 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const
         const uint8_t* d = D + cl.off[l];
         const int gx = cl.gx[l];
         for (int s = 0; s < cl.pre; ++s)
-            lds_sweep(e, b, d, gx, cl.gy[l], cl.cx[l], cl.cy[l], cl.diag[l], cl.w, s == 0);
+            lds_sweep(e, b, d, gx, cl.gy[l], cl.cx[l], cl.cy[l], cl.diag[l], cl.wd[l], s == 0);
         // restrict the residual
         const int cgx = cl.gx[l + 1], cgy = cl.gy[l + 1];
         float* bc = B + cl.off[l + 1];
         const uint8_t* dc = D + cl.off[l + 1];
         const float cx = cl.cx[l], cy = cl.cy[l], dg = cl.diag[l];
-        auto res = [&](int i, int j) -> float {
-            int n = j * gx + i;
+        auto res = [&](int n) -> float {
             if (d[n]) return 0.f;
             float ax = dg * e[n] - cx * (e[n - 1] + e[n + 1]) - cy * (e[n - gx] + e[n + gx]);
             return b[n] - ax;
         };
         for (int n = tid; n < cgx * cgy; n += 1024) {
             if (dc[n]) { bc[n] = 0.f; continue; }
-            int i = 2 * (n % cgx), j = 2 * (n / cgx);
-            float s = 4.f * res(i, j) + 2.f * (res(i - 1, j) + res(i + 1, j) + res(i, j - 1) + res(i, j + 1))
-                    + (res(i - 1, j - 1) + res(i + 1, j - 1) + res(i - 1, j + 1) + res(i + 1, j + 1));
+            const int f = 2 * (n / cgx) * gx + 2 * (n % cgx);
+            float s = 4.f * res(f) + 2.f * (res(f - 1) + res(f + 1) + res(f - gx) + res(f + gx))
+                    + (res(f - gx - 1) + res(f - gx + 1) + res(f + gx - 1) + res(f + gx + 1));
             bc[n] = s * 0.0625f;
         }
         __syncthreads();
     }
-    // coarsest
+    // coarsest: exact solve with the precomputed inverse, else sweeps
     {
         const int l = cl.nl - 1;
-        for (int s = 0; s < cl.sweeps; ++s)
-            lds_sweep(E + cl.off[l], B + cl.off[l], D + cl.off[l], cl.gx[l], cl.gy[l], cl.cx[l], cl.cy[l],
-                      cl.diag[l], cl.w, s == 0);
+        float* e = E + cl.off[l];
+        const float* b = B + cl.off[l];
+        if (cl.nu > 0) {
+            if (tid < cl.nu) {
+                float v = 0.f;
+                for (int q = 0; q < cl.nu; ++q) v += cl.inv[tid * cl.nu + q] * b[cl.unk[q]];
+                e[cl.unk[tid]] = v;
+            }
+            __syncthreads();
+        } else {
+            for (int s = 0; s < cl.sweeps; ++s)
+                lds_sweep(e, b, D + cl.off[l], cl.gx[l], cl.gy[l], cl.cx[l], cl.cy[l], cl.diag[l], cl.wd[l], s == 0);
+        }
     }
     // up
     for (int l = cl.nl - 2; l >= 0; --l) {
@@ -438,7 +450,7 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const
         }
         __syncthreads();
         for (int s = 0; s < cl.post; ++s)
-            lds_sweep(e, B + cl.off[l], d, gx, cl.gy[l], cl.cx[l], cl.cy[l], cl.diag[l], cl.w, false);
+            lds_sweep(e, B + cl.off[l], d, gx, cl.gy[l], cl.cx[l], cl.cy[l], cl.diag[l], cl.wd[l], false);
     }
     for (int n = tid; n < cl.gx[0] * cl.gy[0]; n += 1024) eglob[n] = E[n];
 }
@@ -473,11 +485,14 @@ static Fine fine_of(const pic_ctx* c) {
     return f;
 }
 
-static Lvl lvl_of(const MgLevel& m) {
+static Lvl lvl_of(const MgLevel& m, float w) {
     Lvl L;
     L.gx = m.gx; L.gy = m.gy; L.cx = m.cx; L.cy = m.cy; L.diag = m.diag;
+    L.wd = w / m.diag;
     return L;
 }
+
+static dim3 grid2d(const MgLevel& m) { return dim3((unsigned)((m.gx + kB - 1) / kB), (unsigned)m.gy); }
 
 void poisson_free(pic_ctx* c) {
     for (auto& m : c->mg) {
@@ -487,6 +502,62 @@ void poisson_free(pic_ctx* c) {
         if (m.t) (void)hipFree(m.t);
     }
     c->mg.clear();
+    if (c->mg_inv) (void)hipFree(c->mg_inv);
+    if (c->mg_unk) (void)hipFree(c->mg_unk);
+    c->mg_inv = nullptr;
+    c->mg_unk = nullptr;
+    c->mg_nu = 0;
+}
+
+// Inverse of the coarsest level's operator on its unknowns (Gauss-Jordan in
+// FP64, stored as FP32) for the exact coarse solve.
+static int coarsest_inverse(pic_ctx* c, const std::vector<uint8_t>& dir) {
+    const MgLevel& m = c->mg.back();
+    std::vector<int> unk;
+    for (int n = 0; n < m.gx * m.gy; ++n)
+        if (!dir[n]) unk.push_back(n);
+    const int nu = (int)unk.size();
+    if (nu == 0 || nu > kDirectMax) return PIC_OK;
+    std::vector<double> A((size_t)nu * 2 * nu, 0.0);
+    const double cx = m.cx, cy = m.cy, dg = m.diag;
+    auto col = [&](int node) { for (int q = 0; q < nu; ++q) if (unk[q] == node) return q; return -1; };
+    for (int p = 0; p < nu; ++p) {
+        double* row = &A[(size_t)p * 2 * nu];
+        int n = unk[p];
+        row[p] = dg;
+        int nb[4] = {n - 1, n + 1, n - m.gx, n + m.gx};
+        double cf[4] = {cx, cx, cy, cy};
+        for (int k = 0; k < 4; ++k) {
+            int q = col(nb[k]);
+            if (q >= 0) row[q] -= cf[k];
+        }
+        row[nu + p] = 1.0;
+    }
+    for (int p = 0; p < nu; ++p) {
+        int piv = p;
+        for (int q = p + 1; q < nu; ++q)
+            if (std::fabs(A[(size_t)q * 2 * nu + p]) > std::fabs(A[(size_t)piv * 2 * nu + p])) piv = q;
+        if (piv != p)
+            for (int k = 0; k < 2 * nu; ++k) std::swap(A[(size_t)p * 2 * nu + k], A[(size_t)piv * 2 * nu + k]);
+        double d = A[(size_t)p * 2 * nu + p];
+        if (!(std::fabs(d) > 0)) return PIC_OK;
+        for (int k = 0; k < 2 * nu; ++k) A[(size_t)p * 2 * nu + k] /= d;
+        for (int q = 0; q < nu; ++q) {
+            if (q == p) continue;
+            double f = A[(size_t)q * 2 * nu + p];
+            if (f != 0)
+                for (int k = 0; k < 2 * nu; ++k) A[(size_t)q * 2 * nu + k] -= f * A[(size_t)p * 2 * nu + k];
+        }
+    }
+    std::vector<float> inv((size_t)nu * nu);
+    for (int p = 0; p < nu; ++p)
+        for (int q = 0; q < nu; ++q) inv[(size_t)p * nu + q] = (float)A[(size_t)p * 2 * nu + nu + q];
+    PIC_HIP(hipMalloc(&c->mg_inv, inv.size() * sizeof(float)));
+    PIC_HIP(hipMalloc(&c->mg_unk, nu * sizeof(int)));
+    PIC_HIP(hipMemcpy(c->mg_inv, inv.data(), inv.size() * sizeof(float), hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(c->mg_unk, unk.data(), nu * sizeof(int), hipMemcpyHostToDevice));
+    c->mg_nu = nu;
+    return PIC_OK;
 }
 
 // first level whose levels (down to the coarsest) fit the in-LDS cycle
@@ -572,6 +643,7 @@ int poisson_setup(pic_ctx* c) {
         lx = nlx; ly = nly;
         hx *= 2; hy *= 2;
     }
+    PIC_TRY(coarsest_inverse(c, cur));
     PIC_HIP(hipDeviceSynchronize());
     const size_t last = (size_t)c->mg.back().gx * c->mg.back().gy;
     if (last > (size_t)kLdsNodes || last > (size_t)kLdsPerThread * 1024) {
@@ -620,30 +692,31 @@ static int vcycle(pic_ctx* c) {
     // fine levels in global memory
     for (int l = 0; l < lc; ++l) {
         MgLevel& m = c->mg[l];
-        Lvl lv = lvl_of(m);
-        unsigned nb = nblk((int64_t)m.gx * m.gy);
+        Lvl lv = lvl_of(m, w);
+        dim3 g2 = grid2d(m);
         if (fused) {
             if (l == 0)
-                hipLaunchKernelGGL(k_mg_pre2<double>, dim3(nb), dim3(kB), 0, st, lv, w, c->pr, m.b, m.e, m.dir, flags);
+                hipLaunchKernelGGL(k_mg_pre2<double>, g2, dim3(kB), 0, st, lv, c->pr, m.b, m.e, m.dir, flags);
             else
-                hipLaunchKernelGGL(k_mg_pre2<float>, dim3(nb), dim3(kB), 0, st, lv, w, m.b, (float*)nullptr, m.e,
-                                   m.dir, flags);
+                hipLaunchKernelGGL(k_mg_pre2<float>, g2, dim3(kB), 0, st, lv, m.b, (float*)nullptr, m.e, m.dir, flags);
         } else {
-            if (l == 0) hipLaunchKernelGGL(k_d2f, dim3(nb), dim3(kB), 0, st, c->G, c->pr, m.b, flags);
+            if (l == 0) hipLaunchKernelGGL(k_d2f, dim3(nblk(c->G)), dim3(kB), 0, st, c->G, c->pr, m.b, flags);
             float* a = m.e;
             float* b = m.t;
             for (int s = 0; s < c->p.mg_pre; ++s) {
-                hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(kB), 0, st, lv, w, a, b, m.b, m.dir, s == 0 ? 1 : 0,
-                                   flags);
+                hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, st, lv, a, b, m.b, m.dir, s == 0 ? 1 : 0, flags);
                 std::swap(a, b);
             }
-            if (c->p.mg_pre == 0)
-                hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(kB), 0, st, lv, 0.f, b, a, m.b, m.dir, 1, flags);
+            if (c->p.mg_pre == 0) {
+                Lvl z = lv;
+                z.wd = 0.f;
+                hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, st, z, b, a, m.b, m.dir, 1, flags);
+            }
             if (a != m.e) std::swap(m.e, m.t);
         }
         MgLevel& n = c->mg[l + 1];
-        hipLaunchKernelGGL(k_mg_restrict, dim3(nblk((int64_t)n.gx * n.gy)), dim3(kB), 0, st, lv, lvl_of(n), m.e, m.b,
-                           m.dir, n.b, n.dir, flags);
+        hipLaunchKernelGGL(k_mg_restrict, grid2d(n), dim3(kB), 0, st, lv, lvl_of(n, w), m.e, m.b, m.dir, n.b, n.dir,
+                           flags);
     }
     // coarse levels in one workgroup
     {
@@ -655,31 +728,34 @@ static int vcycle(pic_ctx* c) {
             int k = l - lc;
             cl.gx[k] = m.gx; cl.gy[k] = m.gy; cl.off[k] = off;
             cl.cx[k] = m.cx; cl.cy[k] = m.cy; cl.diag[k] = m.diag;
+            cl.wd[k] = w / m.diag;
             cl.dir[k] = m.dir;
             off += m.gx * m.gy;
         }
-        cl.w = w;
         cl.pre = c->p.mg_pre;
         cl.post = c->p.mg_post;
         const MgLevel& last = c->mg[L - 1];
         cl.sweeps = std::max(50, 4 * std::max(last.gx, last.gy));
+        cl.nu = c->mg_nu;
+        cl.inv = c->mg_inv;
+        cl.unk = c->mg_unk;
         hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(1024), 0, st, cl, c->mg[lc].b, c->mg[lc].e, flags);
     }
     for (int l = lc - 1; l >= 0; --l) {
         MgLevel& m = c->mg[l];
         MgLevel& n = c->mg[l + 1];
-        Lvl lv = lvl_of(m);
-        unsigned nb = nblk((int64_t)m.gx * m.gy);
+        Lvl lv = lvl_of(m, w);
+        dim3 g2 = grid2d(m);
         if (fused) {
-            hipLaunchKernelGGL(k_mg_prolong_sweep, dim3(nb), dim3(kB), 0, st, lv, lvl_of(n), w, m.e, n.e, m.b, m.dir,
-                               m.t, flags);
-            hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(kB), 0, st, lv, w, m.t, m.e, m.b, m.dir, 0, flags);
+            hipLaunchKernelGGL(k_mg_prolong_sweep, g2, dim3(kB), 0, st, lv, lvl_of(n, w), m.e, n.e, m.b, m.dir, m.t,
+                               flags);
+            hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, st, lv, m.t, m.e, m.b, m.dir, 0, flags);
         } else {
-            hipLaunchKernelGGL(k_mg_prolong, dim3(nb), dim3(kB), 0, st, lv, lvl_of(n), m.e, m.dir, n.e, flags);
+            hipLaunchKernelGGL(k_mg_prolong, g2, dim3(kB), 0, st, lv, lvl_of(n, w), m.e, m.dir, n.e, flags);
             float* a = m.e;
             float* b = m.t;
             for (int s = 0; s < c->p.mg_post; ++s) {
-                hipLaunchKernelGGL(k_mg_smooth, dim3(nb), dim3(kB), 0, st, lv, w, a, b, m.b, m.dir, 0, flags);
+                hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, st, lv, a, b, m.b, m.dir, 0, flags);
                 std::swap(a, b);
             }
             if (a != m.e) std::swap(m.e, m.t);

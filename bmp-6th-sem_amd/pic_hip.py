@@ -22,7 +22,7 @@ PIC_LAYOUT_F64 = 0
 PIC_LAYOUT_CELL32 = 1
 GRID_RHO, GRID_PHI, GRID_EX, GRID_EY, GRID_BZ = 0, 1, 2, 3, 4
 GRID_FINAL0, GRID_ENERGY0 = 8, 16
-STAGES = ("deposit", "poisson", "efield", "push", "bin")
+STAGES = ("deposit", "poisson", "efield", "push", "bin", "scatter", "finalize")
 
 ERRORS = {
     -1: "PIC_ERR_ARG", -2: "PIC_ERR_FILE", -3: "PIC_ERR_FORMAT", -4: "PIC_ERR_ALLOC", -5: "PIC_ERR_SOLVER",

@@ -65,12 +65,14 @@ extern "C" {
 #define PIC_GRID_ENERGY0   16   /* + species: Particle::energy_mesh                 */
 
 /* -------------------------------------------------------------- stages */
-#define PIC_STAGE_DEPOSIT   0
-#define PIC_STAGE_POISSON   1
-#define PIC_STAGE_EFIELD    2
-#define PIC_STAGE_PUSH      3
-#define PIC_STAGE_BIN       4
-#define PIC_NSTAGES         5
+#define PIC_STAGE_DEPOSIT   0   /* per-cell corner sums of the CIC deposit               */
+#define PIC_STAGE_POISSON   1   /* whole Poisson solve                                   */
+#define PIC_STAGE_EFIELD    2   /* E = -grad phi                                         */
+#define PIC_STAGE_PUSH      3   /* gather + Boris + walls + slot count (move)            */
+#define PIC_STAGE_BIN       4   /* prefix scan of the counts (+ count/scatter of a re-bin) */
+#define PIC_STAGE_SCATTER   5   /* recomputed push written to the sorted slots (move)    */
+#define PIC_STAGE_FINALIZE  6   /* corner-mesh node sums into final_mesh, energy, rho    */
+#define PIC_NSTAGES         7
 
 typedef struct pic_ctx pic_ctx;
 

@@ -223,7 +223,7 @@ static inline double fld_at(const double* f, int64_t G, int64_t idx) { return id
 
 /* Boris velocity update of deterministic_mover (src/mover.cpp:151-162) for
  * Bz; with bx or by non-zero the same rotation with the full t = (q/m) B dt/2
- * vector: v' = v- + v- x t, v+ = v- + v' x s, s = 2t / (1 + |t|^2). */
+ * vector: v' = v- + v- x t, v+ = v- + v' x s, s = 2t * (1 / (1 + |t|^2)). */
 static inline void orc_boris(double Epx, double Epy, double Bpz, double pctm, double dt, double bx, double by,
                              double* pvx_, double* pvy_, double* pvz_) {
     double pvx = *pvx_, pvy = *pvy_, pvz = *pvz_;
@@ -244,8 +244,8 @@ static inline void orc_boris(double Epx, double Epy, double Bpz, double pctm, do
         double vpx = Vxminus + (Vyminus * tz - Vzminus * ty);
         double vpy = Vyminus + (Vzminus * tx - Vxminus * tz);
         double vpz = Vzminus + (Vxminus * ty - Vyminus * tx);
-        double den = 1 + (tx * tx + ty * ty + tz * tz);
-        double sx = 2 * tx / den, sy = 2 * ty / den, sz = 2 * tz / den;
+        double rden = 1 / (1 + (tx * tx + ty * ty + tz * tz));
+        double sx = (2 * tx) * rden, sy = (2 * ty) * rden, sz = (2 * tz) * rden;
         pvx = Vxminus + (vpy * sz - vpz * sy) + 0.5 * pctm * Epx * dt;
         pvy = Vyminus + (vpz * sx - vpx * sz) + 0.5 * pctm * Epy * dt;
         pvz = Vzminus + (vpx * sy - vpy * sx);
