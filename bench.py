@@ -40,6 +40,27 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def copy_calibration(mib):
+    """Three device-to-device copies of `mib` MiB through the HIP runtime's
+    copy kernel: the known-size reference that scales the HBM byte counters
+    (FETCH_SIZE, WRITE_SIZE) read for the PIC kernels."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, C.c_size_t(n), 3):          # hipMemcpyDeviceToDevice
+            raise SystemExit("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
+
+
 def cpu_baseline(w, parts, steps=3):
     """Oracle (C, OpenMP, FP64 AoS, MKL PARDISO) on the same workload."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -82,7 +103,12 @@ def main():
     ap.add_argument("--layout", type=int, default=None, help="0 F64, 1 CELL32 (default: the workload's)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run three MIB-MiB device copies (counter calibration) and exit")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1", "n_gpus": a.gpus}))
@@ -162,7 +188,7 @@ def main():
             log(f"cpu baseline failed: {e}")
     iters = [s["iterations"] for s in stats]
     out = {
-        "metric": "particle-steps/s (16M ptcl, 1024^2)",
+        "metric": "particle-steps/s (16M ptcl, 1024\u00b2); Poisson ms learned vs plain @1e-8 res",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,

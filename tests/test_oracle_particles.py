@@ -254,3 +254,70 @@ def test_cell32_encode_decode(oracle_mod):
     assert np.all((ox >= 0) & (ox < 1) & (oy >= 0) & (oy < 1))
     xd = ((cell % (NX + 1)) + ox.astype(np.float64)) * dx
     assert np.abs(xd - x).max() < 1e-7 * dx
+
+
+def test_cell32_push_matches_f64_push(oracle_mod):
+    """The CELL32 push (cell units, offsets from the key cell) against the
+    FP64 reference push on the decoded particles: positions within a few
+    float ulps of a cell, velocities to float rounding, same keys away from
+    cell faces, same absorbed counts."""
+    O = oracle_mod
+    g = PUSH_GRID
+    G = g["gx"] * g["gy"]
+    rng = np.random.default_rng(11)
+    n = 4000
+    x = rng.uniform(0, 32e-4, n)
+    y = rng.uniform(0, 32e-4, n)
+    vx, vy, vz = rng.normal(0, 3e5, (3, n))
+    ex, ey = rng.normal(0, 2e3, (2, G))
+    bz = np.full(G, 0.0433)
+    for bx in (0.0, 0.025):
+        cell, ox, oy, fvx, fvy, fvz = O.encode_cell32(x, y, vx, vy, vz, g["gx"], g["dx"], g["dy"])
+        xd = ((cell % g["gx"]) + ox.astype(np.float64)) * g["dx"]
+        yd = ((cell // g["gx"]) + oy.astype(np.float64)) * g["dy"]
+        vd = [v.astype(np.float64) for v in (fvx, fvy, fvz)]
+        ref = O.push_f64(xd, yd, *vd, ex, ey, bz, qm=-1.7588e11, dt=1.77e-11, bx=bx, **g)
+        out = O.push_cell32(cell, ox, oy, fvx, fvy, fvz, ex, ey, bz, qm=-1.7588e11, dt=1.77e-11, bx=bx, **g)
+        rkey, rab = ref[5], ref[6]
+        c2, ox2, oy2, vx2, vy2, vz2, key, ab = out
+        assert list(ab) == list(rab)
+        live = key >= 0
+        assert np.array_equal(live, rkey >= 0)
+        X = ((key[live] % g["gx"]) + ox2[live].astype(np.float64)) * g["dx"]
+        Y = ((key[live] // g["gx"]) + oy2[live].astype(np.float64)) * g["dy"]
+        assert np.abs(X - ref[0][live]).max() < 4e-7 * g["dx"]
+        assert np.abs(Y - ref[1][live]).max() < 4e-7 * g["dy"]
+        for a, b in ((vx2, ref[2]), (vy2, ref[3]), (vz2, ref[4])):
+            assert np.allclose(a[live].astype(np.float64), b[live], rtol=2e-7, atol=1e-3)
+        # keys differ only where the FP64 position sits within float rounding of a face
+        fx = ref[0][live] / g["dx"]
+        fy = ref[1][live] / g["dy"]
+        near = (np.abs(fx - np.round(fx)) < 1e-5) | (np.abs(fy - np.round(fy)) < 1e-5)
+        assert np.array_equal(key[live][~near], rkey[live][~near])
+
+
+def test_cell32_push_rotation_and_walls(oracle_mod):
+    """E = 0: the cell-unit Boris step conserves |v| (to float storage) and
+    vz; a particle stepping below x = 0 is absorbed and counted as leaving."""
+    O = oracle_mod
+    g = PUSH_GRID
+    G = g["gx"] * g["gy"]
+    rng = np.random.default_rng(12)
+    n = 1000
+    x = rng.uniform(5e-4, 27e-4, n)
+    y = rng.uniform(5e-4, 27e-4, n)
+    vx, vy, vz = rng.normal(0, 1e3, (3, n))
+    ex, ey, bz = _uniform_fields(G, bz=0.05)
+    cell, ox, oy, fvx, fvy, fvz = O.encode_cell32(x, y, vx, vy, vz, g["gx"], g["dx"], g["dy"])
+    c2, ox2, oy2, vx2, vy2, vz2, key, ab = O.push_cell32(cell, ox, oy, fvx, fvy, fvz, ex, ey, bz, qm=-1.7588e11,
+                                                        dt=1e-12, **g)
+    s0 = np.hypot(fvx.astype(np.float64), fvy.astype(np.float64))
+    s1 = np.hypot(vx2.astype(np.float64), vy2.astype(np.float64))
+    assert np.allclose(s1, s0, rtol=3e-7)
+    assert np.array_equal(vz2, fvz)
+    # leaving through the left wall, and through the top wall
+    xs, ys = np.array([0.2e-4, 10e-4]), np.array([10e-4, 31.9e-4])
+    vxs, vys, zz = np.array([-1e7, 0.0]), np.array([0.0, 1e7]), np.zeros(2)
+    cell, ox, oy, a, b, c = O.encode_cell32(xs, ys, vxs, vys, zz, g["gx"], g["dx"], g["dy"])
+    out = O.push_cell32(cell, ox, oy, a, b, c, *_uniform_fields(G), qm=1.0, dt=1e-11, **g)
+    assert list(out[6]) == [-1, -1] and list(out[7]) == [1, 2]
