@@ -364,7 +364,9 @@ extern "C" int pic_add_species(pic_ctx* c, double charge, double mass, double si
     const double dx = c->g.dx, dy = c->g.dy;
     s.W = actual_count * c->p.surf_area / (sim_count * dx * dy * dx * dy);   // updateWeighingFactor
     s.capacity = capacity;
-    const size_t n = (size_t)std::max<int64_t>(capacity, 1);
+    // particle arrays rounded up to 64 entries: kernels load 4 consecutive
+    // particles at once from 16-byte aligned groups
+    const size_t n = ((size_t)std::max<int64_t>(capacity, 1) + 63) & ~(size_t)63;
     const size_t G = c->G;
     const bool f64 = c->p.layout == PIC_LAYOUT_F64;
     const size_t ps = f64 ? sizeof(double) : sizeof(float);

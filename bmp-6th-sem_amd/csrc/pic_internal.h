@@ -132,6 +132,9 @@ struct pic_ctx {
     float* mg_inv = nullptr;        // inverse of the coarsest level's operator (unknowns only)
     int* mg_unk = nullptr;          // node index of each coarsest-level unknown
     int mg_nu = 0;
+    uint8_t* mg_cdir = nullptr;     // Dirichlet masks of the in-LDS levels, concatenated
+    int mg_lc = 0;                  // first level of the in-LDS coarse cycle
+    int mg_ctotal = 0;              // nodes of the in-LDS levels
     double* part = nullptr;         // per-block partial sums of the dot products
     double* scal = nullptr;         // device scalars
     int* flags = nullptr;           // [0] converged / failed, [1] iterations

@@ -332,10 +332,10 @@ static constexpr int kLdsPerThread = 5;      // nodes per thread per level (1024
 static constexpr int kDirectMax = 64;        // unknowns of a directly solved coarsest level
 
 struct CoarseLevels {
-    int nl;
+    int nl, total;
     int gx[kLdsMaxLev], gy[kLdsMaxLev], off[kLdsMaxLev];
     float cx[kLdsMaxLev], cy[kLdsMaxLev], diag[kLdsMaxLev], wd[kLdsMaxLev];
-    const uint8_t* dir[kLdsMaxLev];
+    const uint8_t* cdir;          // masks of all levels, concatenated at off[]
     int pre, post, sweeps;
     int nu;                       // unknowns of the coarsest level solved directly (0: use sweeps)
     const float* inv;             // nu x nu inverse
@@ -374,16 +374,53 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const
     __shared__ float E[kLdsNodes];
     __shared__ float B[kLdsNodes];
     __shared__ uint8_t D[kLdsNodes];
+    __shared__ float INV[kDirectMax * kDirectMax];
+    __shared__ int UNK[kDirectMax];
     if (flags[0]) return;
     const int tid = threadIdx.x;
-    for (int l = 0; l < cl.nl; ++l) {
-        const int G = cl.gx[l] * cl.gy[l];
-        for (int n = tid; n < G; n += 1024) {
-            D[cl.off[l] + n] = cl.dir[l][n];
-            E[cl.off[l] + n] = 0.f;
+    const int G0 = cl.gx[0] * cl.gy[0];
+    const int nn = cl.nu * cl.nu;
+    // every global load of the set-up is issued before the first wait
+    constexpr int QD = (kLdsNodes + 1023) / 1024;
+    constexpr int QI = (kDirectMax * kDirectMax + 1023) / 1024;
+    uint8_t dv[QD];
+    float bv[kLdsPerThread], iv[QI];
+    int uv = 0;
+#pragma unroll
+    for (int q = 0; q < QD; ++q) {
+        const int n = tid + q * 1024;
+        dv[q] = n < cl.total ? cl.cdir[n] : (uint8_t)1;
+    }
+#pragma unroll
+    for (int q = 0; q < kLdsPerThread; ++q) {
+        const int n = tid + q * 1024;
+        bv[q] = n < G0 ? bglob[n] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < QI; ++q) {
+        const int n = tid + q * 1024;
+        iv[q] = n < nn ? cl.inv[n] : 0.f;
+    }
+    if (tid < cl.nu) uv = cl.unk[tid];
+#pragma unroll
+    for (int q = 0; q < QD; ++q) {
+        const int n = tid + q * 1024;
+        if (n < cl.total) {
+            D[n] = dv[q];
+            E[n] = 0.f;
         }
     }
-    for (int n = tid; n < cl.gx[0] * cl.gy[0]; n += 1024) B[n] = bglob[n];
+#pragma unroll
+    for (int q = 0; q < kLdsPerThread; ++q) {
+        const int n = tid + q * 1024;
+        if (n < G0) B[n] = bv[q];
+    }
+#pragma unroll
+    for (int q = 0; q < QI; ++q) {
+        const int n = tid + q * 1024;
+        if (n < nn) INV[n] = iv[q];
+    }
+    if (tid < cl.nu) UNK[tid] = uv;
     __syncthreads();
     // down
     for (int l = 0; l + 1 < cl.nl; ++l) {
@@ -420,8 +457,8 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const
         if (cl.nu > 0) {
             if (tid < cl.nu) {
                 float v = 0.f;
-                for (int q = 0; q < cl.nu; ++q) v += cl.inv[tid * cl.nu + q] * b[cl.unk[q]];
-                e[cl.unk[tid]] = v;
+                for (int q = 0; q < cl.nu; ++q) v += INV[tid * cl.nu + q] * b[UNK[q]];
+                e[UNK[tid]] = v;
             }
             __syncthreads();
         } else {
@@ -504,9 +541,13 @@ void poisson_free(pic_ctx* c) {
     c->mg.clear();
     if (c->mg_inv) (void)hipFree(c->mg_inv);
     if (c->mg_unk) (void)hipFree(c->mg_unk);
+    if (c->mg_cdir) (void)hipFree(c->mg_cdir);
     c->mg_inv = nullptr;
     c->mg_unk = nullptr;
+    c->mg_cdir = nullptr;
     c->mg_nu = 0;
+    c->mg_lc = 0;
+    c->mg_ctotal = 0;
 }
 
 // Inverse of the coarsest level's operator on its unknowns (Gauss-Jordan in
@@ -612,6 +653,7 @@ int poisson_setup(pic_ctx* c) {
     int lx = gx, ly = gy;
     double hx = c->g.dx, hy = c->g.dy;
     std::vector<uint8_t> cur = dir;
+    std::vector<std::vector<uint8_t>> masks;
     while (true) {
         MgLevel m;
         m.gx = lx; m.gy = ly;
@@ -624,6 +666,7 @@ int poisson_setup(pic_ctx* c) {
         PIC_HIP(hipMalloc(&m.b, nn * sizeof(float)));
         PIC_HIP(hipMalloc(&m.t, nn * sizeof(float)));
         PIC_HIP(hipMemcpy(m.dir, cur.data(), nn, hipMemcpyHostToDevice));
+        masks.push_back(cur);
         PIC_HIP(hipMemset(m.e, 0, nn * sizeof(float)));
         PIC_HIP(hipMemset(m.b, 0, nn * sizeof(float)));
         PIC_HIP(hipMemset(m.t, 0, nn * sizeof(float)));
@@ -644,7 +687,6 @@ int poisson_setup(pic_ctx* c) {
         hx *= 2; hy *= 2;
     }
     PIC_TRY(coarsest_inverse(c, cur));
-    PIC_HIP(hipDeviceSynchronize());
     const size_t last = (size_t)c->mg.back().gx * c->mg.back().gy;
     if (last > (size_t)kLdsNodes || last > (size_t)kLdsPerThread * 1024) {
         set_error("coarsest multigrid level %dx%d is too large for the in-LDS coarse solve; "
@@ -652,6 +694,14 @@ int poisson_setup(pic_ctx* c) {
                   c->mg.back().gx, c->mg.back().gy);
         return PIC_ERR_UNSUPPORTED;
     }
+    // masks of the in-LDS levels, concatenated in level order
+    c->mg_lc = lds_first_level(c);
+    std::vector<uint8_t> cd;
+    for (size_t l = (size_t)c->mg_lc; l < c->mg.size(); ++l) cd.insert(cd.end(), masks[l].begin(), masks[l].end());
+    c->mg_ctotal = (int)cd.size();
+    PIC_HIP(hipMalloc(&c->mg_cdir, cd.size()));
+    PIC_HIP(hipMemcpy(c->mg_cdir, cd.data(), cd.size(), hipMemcpyHostToDevice));
+    PIC_HIP(hipDeviceSynchronize());
     return PIC_OK;
 }
 
@@ -681,7 +731,7 @@ int launch_efield(pic_ctx* c) {
 static int vcycle(pic_ctx* c) {
     const float w = (float)c->p.mg_omega;
     const int L = (int)c->mg.size();
-    const int lc = lds_first_level(c);
+    const int lc = c->mg_lc;
     hipStream_t st = c->stream;
     const int* flags = c->flags;
     const bool fused = c->p.mg_pre == 2 && c->p.mg_post == 2;
@@ -722,6 +772,8 @@ static int vcycle(pic_ctx* c) {
     {
         CoarseLevels cl;
         cl.nl = L - lc;
+        cl.total = c->mg_ctotal;
+        cl.cdir = c->mg_cdir;
         int off = 0;
         for (int l = lc; l < L; ++l) {
             const MgLevel& m = c->mg[l];
@@ -729,7 +781,6 @@ static int vcycle(pic_ctx* c) {
             cl.gx[k] = m.gx; cl.gy[k] = m.gy; cl.off[k] = off;
             cl.cx[k] = m.cx; cl.cy[k] = m.cy; cl.diag[k] = m.diag;
             cl.wd[k] = w / m.diag;
-            cl.dir[k] = m.dir;
             off += m.gx * m.gy;
         }
         cl.pre = c->p.mg_pre;
