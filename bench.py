@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--workload", default="electrode_1024")
     ap.add_argument("--layout", type=int, default=None, help="0 F64, 1 CELL32 (default: the workload's)")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--guess", choices=("prev", "extrap"), default="prev",
+                    help="Poisson initial guess: previous phi, or extrapolated from the last two")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run three MIB-MiB device copies (counter calibration) and exit")
@@ -124,7 +126,8 @@ def main():
     t0 = time.perf_counter()
     parts = [cfg.generate(w, s) for s in range(2)]
     log(f"generated {2 * w.n_per_species} particles in {time.perf_counter() - t0:.1f} s")
-    ctx = cfg.make_context(w, layout=layout, particles=parts, poisson_tol=1e-10)
+    ctx = cfg.make_context(w, layout=layout, particles=parts, poisson_tol=1e-10,
+                           poisson_guess=1 if a.guess == "extrap" else 0)
     log(f"context ready ({time.perf_counter() - t0:.1f} s)")
 
     it = 0
@@ -203,7 +206,8 @@ def main():
         "data": "synthetic (project generator, seeded)",
         "config": {"workload": w.name, "cells": f"{w.NX}x{w.NY}", "particles_per_species": w.n_per_species,
                    "species": 2, "electrode_V": w.electrode_v if w.electrode else None,
-                   "B_T": w.b_total, "B_tilt_deg": w.b_tilt_deg, "poisson_tol": 1e-10},
+                   "B_T": w.b_total, "B_tilt_deg": w.b_tilt_deg, "poisson_tol": 1e-10,
+                   "poisson_guess": a.guess},
         "stages_ms_per_step": {k: round(v[0] / a.steps, 4) for k, v in stages.items()},
         "poisson": {"iterations_mean": float(np.mean(iters)), "ms_mean": float(np.mean([s["ms"] for s in stats])),
                     "initial_relres_mean": float(np.mean([s["initial_relres"] for s in stats]))},

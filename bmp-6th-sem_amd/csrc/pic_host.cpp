@@ -67,6 +67,7 @@ extern "C" void pic_params_default(pic_params* p) {
     p->mg_pre = 2;
     p->mg_post = 2;
     p->mg_omega = 0.8;
+    p->poisson_guess = 0;
 }
 
 static void free_species(pic::Species& s) {
@@ -90,8 +91,8 @@ extern "C" void pic_destroy(pic_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& s : c->sp) free_species(s);
     poisson_free(c);
-    void* ptrs[] = {c->rho, c->phi, c->ex, c->ey, c->bz, c->fld, c->dir, c->dval, c->pr, c->pp, c->pq,
-                    c->pz, c->pzold, c->pb, c->guess, c->part, c->scal, c->flags, c->scan_bsum};
+    void* ptrs[] = {c->rho, c->phi, c->ex, c->ey, c->bz, c->fld, c->dir, c->dval, c->pr, c->pr2, c->pp, c->pp2,
+                    c->pq, c->pb, c->zf0, c->zf1, c->guess, c->phi_prev, c->part, c->scal, c->flags, c->scan_bsum};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c->scal_host) (void)hipHostFree(c->scal_host);
@@ -214,13 +215,16 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
     ALLOC(c->dir, G);
     ALLOC(c->dval, G * sizeof(double));
     ALLOC(c->pr, G * sizeof(double));
+    ALLOC(c->pr2, G * sizeof(double));
     ALLOC(c->pp, G * sizeof(double));
+    ALLOC(c->pp2, G * sizeof(double));
     ALLOC(c->pq, G * sizeof(double));
-    ALLOC(c->pz, G * sizeof(double));
-    ALLOC(c->pzold, G * sizeof(double));
     ALLOC(c->pb, G * sizeof(double));
+    ALLOC(c->zf0, G * sizeof(float));
+    ALLOC(c->zf1, G * sizeof(float));
     ALLOC(c->guess, G * sizeof(double));
-    ALLOC(c->part, 8 * 1024 * sizeof(double));
+    ALLOC(c->phi_prev, G * sizeof(double));
+    ALLOC(c->part, PIC_PART_DOUBLES * sizeof(double));
     ALLOC(c->scal, 16 * sizeof(double));
     ALLOC(c->flags, 4 * sizeof(int));
     c->scan_nb = (int)((G + 2047) / 2048);
@@ -241,7 +245,9 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
     (void)hipMemset(c->ey, 0, G * sizeof(double));
     (void)hipMemset(c->fld, 0, pad * sizeof(double4));
     (void)hipMemset(c->flags, 0, 4 * sizeof(int));
-    (void)hipMemset(c->part, 0, 8 * 1024 * sizeof(double));
+    (void)hipMemset(c->part, 0, PIC_PART_DOUBLES * sizeof(double));
+    (void)hipMemset(c->pp, 0, G * sizeof(double));
+    (void)hipMemset(c->pp2, 0, G * sizeof(double));
     (void)hipMemset(c->scal, 0, 16 * sizeof(double));
     {
         std::vector<double> b(G, p->b0);   // initMagneticField: uniform B0 (particles.cpp:709-719)

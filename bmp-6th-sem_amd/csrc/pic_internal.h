@@ -10,6 +10,10 @@
 
 #include "../../include/bmp_pic.h"
 
+// per-block partial sums of the Poisson dot products: 8 arrays of 2048
+#define PIC_PART_BLOCKS 2048
+#define PIC_PART_DOUBLES (8 * PIC_PART_BLOCKS)
+
 namespace pic {
 
 // ------------------------------------------------------------------ errors
@@ -124,10 +128,15 @@ struct pic_ctx {
     uint8_t* dir = nullptr;    // Dirichlet mask (fine grid)
     double* dval = nullptr;    // Dirichlet value per node (0 elsewhere)
     // Poisson work vectors (FP64, fine grid)
-    double *pr = nullptr, *pp = nullptr, *pq = nullptr, *pz = nullptr, *pzold = nullptr, *pb = nullptr;
+    double *pr = nullptr, *pr2 = nullptr;     // residual, alternating between iterations
+    double *pp = nullptr, *pp2 = nullptr;     // search direction, alternating
+    double *pq = nullptr, *pb = nullptr;      // A p; residual of the Dirichlet-only field
+    float *zf0 = nullptr, *zf1 = nullptr;     // preconditioned residual z (FP32), alternating
     double* guess = nullptr;
+    double* phi_prev = nullptr;               // phi of the solve before the last (extrapolated guess)
     int guess_set = 0;
-    int have_prev_phi = 0;
+    int have_prev_phi = 0;                    // solutions kept: 0, 1 (phi) or 2 (phi, phi_prev)
+    int last_iters = 0;
     std::vector<pic::MgLevel> mg;
     float* mg_inv = nullptr;        // inverse of the coarsest level's operator (unknowns only)
     int* mg_unk = nullptr;          // node index of each coarsest-level unknown

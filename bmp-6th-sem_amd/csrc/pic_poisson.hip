@@ -11,14 +11,26 @@
 // rho; Dirichlet nodes hold phi = V, which is what the reference's -1 diagonal
 // rows give.  Eliminating them leaves a symmetric positive definite system.
 //
-// Launch structure: dot products leave one partial sum per block; the kernel
-// that needs the scalar re-adds the partials in a fixed order (so every block
-// sees the same value, with no fences or atomics).  Convergence is a device
-// flag: once set, every kernel of the remaining enqueued iterations returns at
-// once, and the host reads the flag only between batches of iterations.  The
-// V-cycle fuses the two pre-smoothing sweeps into one stencil pass and the
-// prolongation into the first post-smoothing sweep, and runs all levels of at
-// most 65 x 65 nodes inside one workgroup's LDS.
+// One PCG iteration k is, on the fine grid:
+//   k_apply_p     p_k = z_{k-1} + beta p_{k-1} (formed at the node and its
+//                 neighbours), q = A p_k, partials of p.q
+//   k_alpha       alpha = (r.z)_{k-1} / (p.q)
+//   k_update_pre2 x += alpha p; r_k = r_{k-1} - alpha q (node and neighbours);
+//                 the V-cycle's two pre-smoothing sweeps from zero on r_k in
+//                 the same pass; partials of r.r
+//   k_check       convergence flag
+//   ...           the rest of the V-cycle (coarser levels, prolongation and
+//                 the first post-smoothing sweep fused)
+//   k_smooth_dots last post-smoothing sweep -> z_k (FP32), partials of r.z
+//                 and r.z_{k-1}
+//   k_beta        Polak-Ribiere beta = (r.z_k - r.z_{k-1}) / (r.z)_{k-1}
+// r, p and z alternate between two buffers so that no kernel overwrites what a
+// neighbouring workgroup still reads.  Dot products leave one partial sum per
+// block; a one-block kernel re-adds them in a fixed order (deterministic, no
+// atomics).  Convergence is a device flag: once set, every kernel of the
+// remaining enqueued iterations returns at once, and the host reads the flag
+// only between batches of iterations.  Levels of at most 65 x 65 nodes run
+// inside one workgroup's LDS.
 #include "pic_internal.h"
 
 #include <algorithm>
@@ -27,11 +39,10 @@
 namespace pic {
 
 static constexpr int kB = 256;
-static constexpr int kNP = 512;              // blocks of the reduction kernels (partials)
-static constexpr int kBatch = 2;             // iterations enqueued between host checks
+static constexpr int kNP = PIC_PART_BLOCKS;  // blocks of the reduction kernels (partials)
 
-// scalars: [0] rr, [1] bb, [2] rr0, [3..4] rz of the previous two iterations
-enum Scal { S_RR = 0, S_BB, S_RR0, S_RZP0, S_RZP1, S_N };
+// device scalars
+enum Scal { S_RR = 0, S_BB, S_RR0, S_RZ, S_ALPHA, S_BETA, S_N };
 // partial-sum arrays, kNP doubles each
 enum Part { P_RR = 0, P_BB, P_PQ, P_RZ, P_RZO, P_N };
 
@@ -50,16 +61,17 @@ __device__ __forceinline__ double block_sum_all(double v, double* sh) {
     return s;
 }
 
-__device__ __forceinline__ double sum_partials(const double* part, double* sh) {
-    double v = 0;
-    for (int j = threadIdx.x; j < kNP; j += blockDim.x) v += part[j];
-    return block_sum_all(v, sh);
-}
-
 __device__ __forceinline__ void write_partial(double v, double* part, double* sh) {
     double s = block_sum_all(v, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
+
+// Sum of the kNP partials by a 1024-thread block (two per thread, fixed order).
+__device__ __forceinline__ double sum_partials(const double* part, double* sh) {
+    double v = part[threadIdx.x] + part[threadIdx.x + 1024];
+    return block_sum_all(v, sh);
+}
+static_assert(kNP == 2048, "sum_partials reads two partials per thread of a 1024-thread block");
 
 // ------------------------------------------------------------ fine kernels
 struct Fine {
@@ -82,11 +94,19 @@ __global__ void k_overwrite_phi(int G, double* phi, const uint8_t* dir, const do
     if (dir[n]) phi[n] = dval[n];
 }
 
-// x <- guess (or 0) with Dirichlet values imposed
-__global__ void k_init_guess(int G, double* x, const double* src, const uint8_t* dir, const double* dval) {
+// x <- guess (or 0) with Dirichlet values imposed; `keep` (optional) receives
+// the previous x.  With `prev` the guess is 2 x - prev (x, prev: the last two
+// solutions).
+__global__ void k_init_guess(int G, double* x, const double* src, const double* prev, double* keep,
+                             const uint8_t* dir, const double* dval) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= G) return;
-    x[n] = dir[n] ? dval[n] : (src ? src[n] : 0.0);
+    const double cur = x[n];
+    double v = 0.0;
+    if (prev) v = 2.0 * cur - prev[n];
+    else if (src) v = src[n];
+    if (keep) keep[n] = cur;
+    x[n] = dir[n] ? dval[n] : v;
 }
 
 __global__ void k_reset_flags(int* flags) {
@@ -115,9 +135,9 @@ __global__ __launch_bounds__(kB) void k_residual(Fine f, const double* __restric
 
 // rr = sum of the partials; done when rr <= tol^2 bb (flags[0] = 1), NaN -> 2.
 // Outside the first call, one more iteration is recorded unless already done.
-__global__ __launch_bounds__(kB) void k_check(const double* part_rr, const double* part_bb, double tol2,
-                                              int* flags, double* scal, int first) {
-    __shared__ double sh[kB / 64];
+__global__ __launch_bounds__(1024) void k_check(const double* part_rr, const double* part_bb, double tol2,
+                                                int* flags, double* scal, int first) {
+    __shared__ double sh[16];
     if (flags[0]) return;
     double rr = sum_partials(part_rr, sh);
     double bb = first ? sum_partials(part_bb, sh) : scal[S_BB];
@@ -134,90 +154,128 @@ __global__ __launch_bounds__(kB) void k_check(const double* part_rr, const doubl
     }
 }
 
-// q = A p (p = 0 on Dirichlet nodes); partial of p.q
-__global__ __launch_bounds__(kB) void k_apply(Fine f, const double* __restrict__ p, const uint8_t* __restrict__ dir,
-                                              double* __restrict__ q, double* part_pq, const int* flags) {
+// p = z + beta pold (first: p = z) at the node and its neighbours; q = A p;
+// partial of p.q.  z, pold and p are zero on Dirichlet nodes.
+__global__ __launch_bounds__(kB) void k_apply_p(Fine f, const float* __restrict__ z, const double* __restrict__ pold,
+                                                double* __restrict__ p, const uint8_t* __restrict__ dir,
+                                                double* __restrict__ q, double* part_pq, const double* scal,
+                                                const int* flags, int first) {
     __shared__ double sh[kB / 64];
     if (flags[0]) return;
+    const double beta = first ? 0.0 : scal[S_BETA];
     double acc = 0;
     const int G = f.gx * f.gy;
+    auto P = [&](int m) -> double { return first ? (double)z[m] : (double)z[m] + beta * pold[m]; };
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
-        double v = 0;
+        double v = 0, pn = 0;
         if (!dir[n]) {
-            v = f.diag * p[n] - f.cx * (p[n - 1] + p[n + 1]) - f.cy * (p[n - f.gx] + p[n + f.gx]);
-            acc += p[n] * v;
+            pn = P(n);
+            v = f.diag * pn - f.cx * (P(n - 1) + P(n + 1)) - f.cy * (P(n - f.gx) + P(n + f.gx));
+            acc += pn * v;
         }
+        p[n] = pn;
         q[n] = v;
     }
     write_partial(acc, part_pq, sh);
 }
 
-// alpha = rz / pq; x += alpha p; r -= alpha q; partial of r.r
-__global__ __launch_bounds__(kB) void k_update_xr(int G, double* __restrict__ x, double* __restrict__ r,
-                                                  const double* __restrict__ p, const double* __restrict__ q,
-                                                  const double* part_pq, const double* part_rz, double* part_rr,
-                                                  const int* flags) {
-    __shared__ double sh[kB / 64];
+// alpha = (r.z) / (p.q)
+__global__ __launch_bounds__(1024) void k_alpha(const double* part_pq, double* scal, const int* flags) {
+    __shared__ double sh[16];
     if (flags[0]) return;
-    const double rz = sum_partials(part_rz, sh);
     const double pq = sum_partials(part_pq, sh);
-    const double alpha = rz / pq;
-    double acc = 0;
-    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
-        x[n] += alpha * p[n];
-        double rn = r[n] - alpha * q[n];
-        r[n] = rn;
-        acc += rn * rn;
-    }
-    write_partial(acc, part_rr, sh);
+    if (threadIdx.x == 0) scal[S_ALPHA] = scal[S_RZ] / pq;
 }
 
-// z = (double) e0; partials of r.z and r.zold (flexible beta)
-__global__ __launch_bounds__(kB) void k_zout(int G, const float* __restrict__ e0, const double* __restrict__ r,
-                                             double* __restrict__ z, const double* __restrict__ zold,
-                                             double* part_rz, double* part_rzo, const int* flags, int first) {
-    __shared__ double sh[kB / 64];
-    if (flags[0]) return;
-    double a = 0, b = 0;
-    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
-        double zn = (double)e0[n];
-        z[n] = zn;
-        a += r[n] * zn;
-        if (!first) b += r[n] * zold[n];
-    }
-    write_partial(a, part_rz, sh);
-    if (!first) write_partial(b, part_rzo, sh);
-}
-
-// beta = (r.z - r.zold) / rz_prev; p = z + beta p; zold = z; rz kept for the next iteration
-__global__ __launch_bounds__(kB) void k_update_p(int G, double* __restrict__ p, const double* __restrict__ z,
-                                                 double* __restrict__ zold, const double* part_rz,
-                                                 const double* part_rzo, double* scal, const int* flags,
-                                                 int parity, int first) {
-    __shared__ double sh[kB / 64];
+// beta = (r.z - r.zold) / (r.z)_prev (Polak-Ribiere; first: 0); keeps r.z.
+__global__ __launch_bounds__(1024) void k_beta(const double* part_rz, const double* part_rzo, double* scal,
+                                               const int* flags, int first) {
+    __shared__ double sh[16];
     if (flags[0]) return;
     const double rz = sum_partials(part_rz, sh);
-    double beta = 0.0;
-    if (!first) {
-        const double rzo = sum_partials(part_rzo, sh);
-        beta = (rz - rzo) / scal[S_RZP0 + (parity ^ 1)];
+    const double rzo = first ? 0.0 : sum_partials(part_rzo, sh);
+    if (threadIdx.x == 0) {
+        scal[S_BETA] = first ? 0.0 : (rz - rzo) / scal[S_RZ];
+        scal[S_RZ] = rz;
     }
-    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
-        double zn = z[n];
-        p[n] = first ? zn : zn + beta * p[n];
-        zold[n] = zn;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_RZP0 + parity] = rz;
 }
 
-// --------------------------------------------------------------- MG (FP32)
-// Level kernels use 2D grids (x: 256 nodes of a row, y: row) so node indices
-// need no division; `wd` is the Jacobi weight over the diagonal.
 struct Lvl {
     int gx, gy;
     float cx, cy, diag, wd;
 };
 
+// x += alpha p; rnew = rold - alpha q (also at the neighbours, for the
+// stencil); partial of rnew.rnew.  PRE2: the finest level's two weighted-Jacobi
+// sweeps from zero on b = (float) rnew in the same pass (the first sweep is
+// pointwise, e1 = wd b, so the second needs only b on the 5-point stencil);
+// b and e are stored for the restriction and the post-smoothing.  Without
+// PRE2 only b is stored.
+template <int PRE2>
+__global__ __launch_bounds__(kB) void k_update_pre2(Fine f, Lvl L, double* __restrict__ x,
+                                                    const double* __restrict__ p, const double* __restrict__ rold,
+                                                    double* __restrict__ rnew, const double* __restrict__ q,
+                                                    const uint8_t* __restrict__ dir, float* __restrict__ b0,
+                                                    float* __restrict__ e0, double* part_rr, const double* scal,
+                                                    const int* flags) {
+    __shared__ double sh[kB / 64];
+    if (flags[0]) return;
+    const double alpha = scal[S_ALPHA];
+    double acc = 0;
+    const int G = f.gx * f.gy;
+    auto R = [&](int m) -> double { return rold[m] - alpha * q[m]; };
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
+        if (dir[n]) {
+            rnew[n] = 0.0;
+            b0[n] = 0.f;
+            if (PRE2) e0[n] = 0.f;
+            continue;
+        }
+        x[n] += alpha * p[n];
+        const double rn = R(n);
+        rnew[n] = rn;
+        acc += rn * rn;
+        const float bn = (float)rn;
+        b0[n] = bn;
+        if (PRE2) {
+            auto e1 = [&](int m) -> float { return dir[m] ? 0.f : L.wd * (float)R(m); };
+            const float c = L.wd * bn;
+            const float ax = L.diag * c - L.cx * (e1(n - 1) + e1(n + 1)) - L.cy * (e1(n - f.gx) + e1(n + f.gx));
+            e0[n] = c + L.wd * (bn - ax);
+        }
+    }
+    write_partial(acc, part_rr, sh);
+}
+
+// Last post-smoothing sweep of the finest level: z = ei + wd (b - A ei) (zero
+// on Dirichlet), stored in FP32; partials of r.z and r.zold.
+__global__ __launch_bounds__(kB) void k_smooth_dots(Lvl L, const float* __restrict__ ei, float* __restrict__ z,
+                                                    const float* __restrict__ b, const uint8_t* __restrict__ dir,
+                                                    const double* __restrict__ r, const float* __restrict__ zold,
+                                                    double* part_rz, double* part_rzo, const int* flags, int first) {
+    __shared__ double sh[kB / 64];
+    if (flags[0]) return;
+    double a = 0, c = 0;
+    const int G = L.gx * L.gy;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
+        float v = 0.f;
+        if (!dir[n]) {
+            const float e = ei[n];
+            const float ax = L.diag * e - L.cx * (ei[n - 1] + ei[n + 1]) - L.cy * (ei[n - L.gx] + ei[n + L.gx]);
+            v = e + L.wd * (b[n] - ax);
+        }
+        z[n] = v;
+        const double rn = r[n];
+        a += rn * (double)v;
+        if (!first) c += rn * (double)zold[n];
+    }
+    write_partial(a, part_rz, sh);
+    if (!first) write_partial(c, part_rzo, sh);
+}
+
+// --------------------------------------------------------------- MG (FP32)
+// Level kernels use 2D grids (x: 256 nodes of a row, y: row) so node indices
+// need no division; `wd` is the Jacobi weight over the diagonal.
 #define MG_IJ(L)                                                              \
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;     \
     if (i >= (L).gx) return;                                                  \
@@ -323,9 +381,13 @@ __global__ __launch_bounds__(kB) void k_mg_prolong_sweep(Lvl F, Lvl C, const flo
     eo[n] = c + F.wd * (b[n] - ax);
 }
 
+
 // ---------------------------------------------- coarse levels in one workgroup
 // Levels of at most 65 x 65 nodes run in LDS; the coarsest level is solved
-// exactly with its inverse (precomputed on the host, unknowns only).
+// exactly with its inverse (precomputed on the host, unknowns only).  With
+// two pre- and two post-smoothing sweeps each level costs two barriers on the
+// way down (fused pre-smoothing, restriction) and two on the way up
+// (prolongation fused with the first sweep, second sweep).
 static constexpr int kLdsMaxLev = 8;
 static constexpr int kLdsNodes = 5800;       // all in-LDS levels together
 static constexpr int kLdsPerThread = 5;      // nodes per thread per level (1024 threads)
@@ -342,6 +404,7 @@ struct CoarseLevels {
     const int* unk;               // node index of each unknown
 };
 
+// One weighted-Jacobi sweep in place (two barriers: read all, then write).
 __device__ __forceinline__ void lds_sweep(float* E, const float* B, const uint8_t* D, int gx, int gy, float cx,
                                           float cy, float diag, float wd, bool zero_in) {
     const int G = gx * gy;
@@ -369,9 +432,20 @@ __device__ __forceinline__ void lds_sweep(float* E, const float* B, const uint8_
     __syncthreads();
 }
 
+__device__ __forceinline__ float lds_prolong(const float* ec, int cgx, int i, int j) {
+    const int I = i >> 1, J = j >> 1;
+    const bool oi = i & 1, oj = j & 1;
+    const float* c0 = ec + J * cgx + I;
+    if (!oi && !oj) return c0[0];
+    if (oi && !oj) return 0.5f * (c0[0] + c0[1]);
+    if (!oi && oj) return 0.5f * (c0[0] + c0[cgx]);
+    return 0.25f * (c0[0] + c0[1] + c0[cgx] + c0[cgx + 1]);
+}
+
 __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const float* __restrict__ bglob,
                                                           float* __restrict__ eglob, const int* flags) {
     __shared__ float E[kLdsNodes];
+    __shared__ float T[kLdsNodes];
     __shared__ float B[kLdsNodes];
     __shared__ uint8_t D[kLdsNodes];
     __shared__ float INV[kDirectMax * kDirectMax];
@@ -427,14 +501,28 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const
         float* e = E + cl.off[l];
         const float* b = B + cl.off[l];
         const uint8_t* d = D + cl.off[l];
-        const int gx = cl.gx[l];
-        for (int s = 0; s < cl.pre; ++s)
-            lds_sweep(e, b, d, gx, cl.gy[l], cl.cx[l], cl.cy[l], cl.diag[l], cl.wd[l], s == 0);
+        const int gx = cl.gx[l], G = gx * cl.gy[l];
+        const float cx = cl.cx[l], cy = cl.cy[l], dg = cl.diag[l], wd = cl.wd[l];
+        if (cl.pre == 2) {
+            // two sweeps from zero: e1 = wd b is pointwise
+#pragma unroll
+            for (int q = 0; q < kLdsPerThread; ++q) {
+                const int n = tid + q * 1024;
+                if (n >= G) continue;
+                if (d[n]) { e[n] = 0.f; continue; }
+                auto e1 = [&](int m) -> float { return d[m] ? 0.f : wd * b[m]; };
+                const float c = wd * b[n];
+                const float ax = dg * c - cx * (e1(n - 1) + e1(n + 1)) - cy * (e1(n - gx) + e1(n + gx));
+                e[n] = c + wd * (b[n] - ax);
+            }
+            __syncthreads();
+        } else {
+            for (int s = 0; s < cl.pre; ++s) lds_sweep(e, b, d, gx, cl.gy[l], cx, cy, dg, wd, s == 0);
+        }
         // restrict the residual
         const int cgx = cl.gx[l + 1], cgy = cl.gy[l + 1];
         float* bc = B + cl.off[l + 1];
         const uint8_t* dc = D + cl.off[l + 1];
-        const float cx = cl.cx[l], cy = cl.cy[l], dg = cl.diag[l];
         auto res = [&](int n) -> float {
             if (d[n]) return 0.f;
             float ax = dg * e[n] - cx * (e[n - 1] + e[n + 1]) - cy * (e[n - gx] + e[n + gx]);
@@ -469,27 +557,48 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const
     // up
     for (int l = cl.nl - 2; l >= 0; --l) {
         float* e = E + cl.off[l];
+        float* t = T + cl.off[l];
+        const float* b = B + cl.off[l];
         const uint8_t* d = D + cl.off[l];
         const float* ec = E + cl.off[l + 1];
-        const int gx = cl.gx[l], cgx = cl.gx[l + 1];
-        for (int n = tid; n < gx * cl.gy[l]; n += 1024) {
-            if (d[n]) continue;
-            int i = n % gx, j = n / gx;
-            int I = i >> 1, J = j >> 1;
-            bool oi = i & 1, oj = j & 1;
-            const float* c0 = ec + J * cgx + I;
-            float v;
-            if (!oi && !oj) v = c0[0];
-            else if (oi && !oj) v = 0.5f * (c0[0] + c0[1]);
-            else if (!oi && oj) v = 0.5f * (c0[0] + c0[cgx]);
-            else v = 0.25f * (c0[0] + c0[1] + c0[cgx] + c0[cgx + 1]);
-            e[n] += v;
+        const int gx = cl.gx[l], G = gx * cl.gy[l], cgx = cl.gx[l + 1];
+        const float cx = cl.cx[l], cy = cl.cy[l], dg = cl.diag[l], wd = cl.wd[l];
+        if (cl.post == 2) {
+            // prolongation fused with the first sweep (into t), second sweep back into e
+#pragma unroll
+            for (int q = 0; q < kLdsPerThread; ++q) {
+                const int n = tid + q * 1024;
+                if (n >= G) continue;
+                if (d[n]) { t[n] = 0.f; continue; }
+                const int j = n / gx, i = n - j * gx;
+                auto e1 = [&](int m, int mi, int mj) -> float { return d[m] ? 0.f : e[m] + lds_prolong(ec, cgx, mi, mj); };
+                const float c = e1(n, i, j);
+                const float ax = dg * c - cx * (e1(n - 1, i - 1, j) + e1(n + 1, i + 1, j))
+                               - cy * (e1(n - gx, i, j - 1) + e1(n + gx, i, j + 1));
+                t[n] = c + wd * (b[n] - ax);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < kLdsPerThread; ++q) {
+                const int n = tid + q * 1024;
+                if (n >= G) continue;
+                if (d[n]) { e[n] = 0.f; continue; }
+                const float c = t[n];
+                const float ax = dg * c - cx * (t[n - 1] + t[n + 1]) - cy * (t[n - gx] + t[n + gx]);
+                e[n] = c + wd * (b[n] - ax);
+            }
+            __syncthreads();
+        } else {
+            for (int n = tid; n < G; n += 1024) {
+                if (d[n]) continue;
+                const int j = n / gx, i = n - j * gx;
+                e[n] += lds_prolong(ec, cgx, i, j);
+            }
+            __syncthreads();
+            for (int s = 0; s < cl.post; ++s) lds_sweep(e, b, d, gx, cl.gy[l], cx, cy, dg, wd, false);
         }
-        __syncthreads();
-        for (int s = 0; s < cl.post; ++s)
-            lds_sweep(e, B + cl.off[l], d, gx, cl.gy[l], cl.cx[l], cl.cy[l], cl.diag[l], cl.wd[l], false);
     }
-    for (int n = tid; n < cl.gx[0] * cl.gy[0]; n += 1024) eglob[n] = E[n];
+    for (int n = tid; n < G0; n += 1024) eglob[n] = E[n];
 }
 
 // ----------------------------------------------------------------- E field
@@ -510,6 +619,7 @@ __global__ __launch_bounds__(kB) void k_efield(int gx, int gy, double dx, double
     ey[n] = fy;
     fld[n] = make_double4(fx, fy, bz[n], 0.0);
 }
+
 
 // ============================================================= host side
 static Fine fine_of(const pic_ctx* c) {
@@ -705,6 +815,7 @@ int poisson_setup(pic_ctx* c) {
     return PIC_OK;
 }
 
+
 int launch_overwrite_rho(pic_ctx* c) {
     hipLaunchKernelGGL(k_overwrite_rho, dim3(nblk(c->G)), dim3(kB), 0, c->stream, c->G, c->rho, c->dir, c->dval);
     PIC_HIP(hipGetLastError());
@@ -727,48 +838,63 @@ int launch_efield(pic_ctx* c) {
     return PIC_OK;
 }
 
-// One V-cycle on the FP64 residual c->pr; result in c->mg[0].e.
-static int vcycle(pic_ctx* c) {
+// Pre-smoothing of one level from zero into m.e with the configured sweep count.
+static void presmooth(pic_ctx* c, MgLevel& m) {
+    const float w = (float)c->p.mg_omega;
+    Lvl lv = lvl_of(m, w);
+    dim3 g2 = grid2d(m);
+    float* a = m.e;
+    float* b = m.t;
+    for (int s = 0; s < c->p.mg_pre; ++s) {
+        hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, c->stream, lv, a, b, m.b, m.dir, s == 0 ? 1 : 0, c->flags);
+        std::swap(a, b);
+    }
+    if (c->p.mg_pre == 0) {
+        Lvl zl = lv;
+        zl.wd = 0.f;
+        hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, c->stream, zl, b, a, m.b, m.dir, 1, c->flags);
+    }
+    if (a != m.e) std::swap(m.e, m.t);
+}
+
+// Finest-level pre-smoothing for sweep counts other than two.
+static int vcycle_pre_generic(pic_ctx* c, const double* r) {
+    MgLevel& m = c->mg[0];
+    hipLaunchKernelGGL(k_d2f, dim3(nblk(c->G)), dim3(kB), 0, c->stream, c->G, r, m.b, c->flags);
+    presmooth(c, m);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+static int vcycle_pre_generic_fine(pic_ctx* c) {
+    presmooth(c, c->mg[0]);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+// The V-cycle below the finest level's pre-smoothing (already in m0.e, with
+// its right-hand side in m0.b), ending with the last post-smoothing sweep of
+// the finest level into z and the partials of r.z and r.zold.
+static int vcycle_rest(pic_ctx* c, const double* r, float* z, const float* zold, int first) {
     const float w = (float)c->p.mg_omega;
     const int L = (int)c->mg.size();
     const int lc = c->mg_lc;
     hipStream_t st = c->stream;
     const int* flags = c->flags;
-    const bool fused = c->p.mg_pre == 2 && c->p.mg_post == 2;
-    if (lc == 0) {
-        MgLevel& m = c->mg[0];
-        hipLaunchKernelGGL(k_d2f, dim3(nblk(c->G)), dim3(kB), 0, st, c->G, c->pr, m.b, flags);
-    }
-    // fine levels in global memory
+    const bool fpre = c->p.mg_pre == 2, fpost = c->p.mg_post == 2;
     for (int l = 0; l < lc; ++l) {
         MgLevel& m = c->mg[l];
-        Lvl lv = lvl_of(m, w);
-        dim3 g2 = grid2d(m);
-        if (fused) {
-            if (l == 0)
-                hipLaunchKernelGGL(k_mg_pre2<double>, g2, dim3(kB), 0, st, lv, c->pr, m.b, m.e, m.dir, flags);
+        if (l > 0) {
+            if (fpre)
+                hipLaunchKernelGGL(k_mg_pre2<float>, grid2d(m), dim3(kB), 0, st, lvl_of(m, w), m.b, (float*)nullptr,
+                                   m.e, m.dir, flags);
             else
-                hipLaunchKernelGGL(k_mg_pre2<float>, g2, dim3(kB), 0, st, lv, m.b, (float*)nullptr, m.e, m.dir, flags);
-        } else {
-            if (l == 0) hipLaunchKernelGGL(k_d2f, dim3(nblk(c->G)), dim3(kB), 0, st, c->G, c->pr, m.b, flags);
-            float* a = m.e;
-            float* b = m.t;
-            for (int s = 0; s < c->p.mg_pre; ++s) {
-                hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, st, lv, a, b, m.b, m.dir, s == 0 ? 1 : 0, flags);
-                std::swap(a, b);
-            }
-            if (c->p.mg_pre == 0) {
-                Lvl z = lv;
-                z.wd = 0.f;
-                hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, st, z, b, a, m.b, m.dir, 1, flags);
-            }
-            if (a != m.e) std::swap(m.e, m.t);
+                presmooth(c, m);
         }
         MgLevel& n = c->mg[l + 1];
-        hipLaunchKernelGGL(k_mg_restrict, grid2d(n), dim3(kB), 0, st, lv, lvl_of(n, w), m.e, m.b, m.dir, n.b, n.dir,
-                           flags);
+        hipLaunchKernelGGL(k_mg_restrict, grid2d(n), dim3(kB), 0, st, lvl_of(m, w), lvl_of(n, w), m.e, m.b, m.dir,
+                           n.b, n.dir, flags);
     }
-    // coarse levels in one workgroup
     {
         CoarseLevels cl;
         cl.nl = L - lc;
@@ -792,34 +918,42 @@ static int vcycle(pic_ctx* c) {
         cl.unk = c->mg_unk;
         hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(1024), 0, st, cl, c->mg[lc].b, c->mg[lc].e, flags);
     }
+    if (lc == 0) {
+        // the whole cycle ran in LDS: z is its output, copied with the dot products
+        MgLevel& m = c->mg[0];
+        Lvl zl = lvl_of(m, w);
+        zl.wd = 0.f;
+        hipLaunchKernelGGL(k_smooth_dots, dim3(kNP), dim3(kB), 0, st, zl, m.e, z, m.b, m.dir, r, zold,
+                           c->part + P_RZ * kNP, c->part + P_RZO * kNP, flags, first);
+    }
     for (int l = lc - 1; l >= 0; --l) {
         MgLevel& m = c->mg[l];
         MgLevel& n = c->mg[l + 1];
         Lvl lv = lvl_of(m, w);
         dim3 g2 = grid2d(m);
-        if (fused) {
+        // post-smoothing: all sweeps but the last leave the correction in m.e
+        int rest;
+        if (fpost) {
             hipLaunchKernelGGL(k_mg_prolong_sweep, g2, dim3(kB), 0, st, lv, lvl_of(n, w), m.e, n.e, m.b, m.dir, m.t,
                                flags);
-            hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, st, lv, m.t, m.e, m.b, m.dir, 0, flags);
+            std::swap(m.e, m.t);
+            rest = 1;
         } else {
             hipLaunchKernelGGL(k_mg_prolong, g2, dim3(kB), 0, st, lv, lvl_of(n, w), m.e, m.dir, n.e, flags);
-            float* a = m.e;
-            float* b = m.t;
-            for (int s = 0; s < c->p.mg_post; ++s) {
-                hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, st, lv, a, b, m.b, m.dir, 0, flags);
-                std::swap(a, b);
-            }
-            if (a != m.e) std::swap(m.e, m.t);
+            rest = c->p.mg_post;
+        }
+        const int last_here = l == 0 ? 1 : 0;         // the finest level's last sweep is k_smooth_dots
+        for (int s = 0; s < rest - last_here; ++s) {
+            hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, st, lv, m.e, m.t, m.b, m.dir, 0, flags);
+            std::swap(m.e, m.t);
+        }
+        if (l == 0) {
+            Lvl zl = lv;
+            if (rest == 0) zl.wd = 0.f;                 // no post-smoothing: z = e
+            hipLaunchKernelGGL(k_smooth_dots, dim3(kNP), dim3(kB), 0, st, zl, m.e, z, m.b, m.dir, r, zold,
+                               c->part + P_RZ * kNP, c->part + P_RZO * kNP, flags, first);
         }
     }
-    PIC_HIP(hipGetLastError());
-    return PIC_OK;
-}
-
-static int precondition(pic_ctx* c, int first) {
-    PIC_TRY(vcycle(c));
-    hipLaunchKernelGGL(k_zout, dim3(kNP), dim3(kB), 0, c->stream, c->G, c->mg[0].e, c->pr, c->pz, c->pzold,
-                       c->part + P_RZ * kNP, c->part + P_RZO * kNP, c->flags, first);
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
@@ -843,41 +977,78 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     PIC_HIP(hipEventRecord(t0, s));
     double* part = c->part;
     const double tol2 = c->p.poisson_tol * c->p.poisson_tol;
+    const float w = (float)c->p.mg_omega;
+    MgLevel& m0 = c->mg[0];
+    const Lvl l0 = lvl_of(m0, w);
+    double* R[2] = {c->pr, c->pr2};
+    double* P[2] = {c->pp, c->pp2};
+    float* Z[2] = {c->zf0, c->zf1};
 
     hipLaunchKernelGGL(k_reset_flags, dim3(1), dim3(1), 0, s, c->flags);
     PIC_TRY(launch_overwrite_rho(c));
     // ||b||^2: residual of the Dirichlet-only field (unknowns at zero)
-    hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->pq, (const double*)nullptr, c->dir, c->dval);
+    hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->pq, (const double*)nullptr,
+                       (const double*)nullptr, (double*)nullptr, c->dir, c->dval);
     hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->pq, c->dir, c->pb, part + P_BB * kNP);
+    // initial guess
     int guess = 0;
     const double* src = nullptr;
-    if (c->guess_set) { src = c->guess; guess = 2; }
-    else if (iteration > 0 && c->have_prev_phi) { src = c->phi; guess = 1; }
-    hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, src, c->dir, c->dval);
-    hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, c->pr, part + P_RR * kNP);
-    hipLaunchKernelGGL(k_check, dim3(1), dim3(kB), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2, c->flags, c->scal,
+    const double* prev = nullptr;
+    double* keep = nullptr;
+    const bool extrap = c->p.poisson_guess == 1;
+    if (c->guess_set) {
+        src = c->guess;
+        guess = 2;
+    } else if (iteration > 0 && c->have_prev_phi) {
+        if (extrap && c->have_prev_phi >= 2) {
+            prev = c->phi_prev;
+            guess = 3;
+        } else {
+            src = c->phi;
+            guess = 1;
+        }
+        if (extrap) keep = c->phi_prev;
+    }
+    hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, src, prev, keep, c->dir, c->dval);
+    hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, R[0], part + P_RR * kNP);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2, c->flags,
+                       c->scal, 1);
+    // z_0 = M r_0
+    if (c->p.mg_pre == 2)
+        hipLaunchKernelGGL(k_mg_pre2<double>, grid2d(m0), dim3(kB), 0, s, l0, R[0], m0.b, m0.e, m0.dir, c->flags);
+    else
+        PIC_TRY(vcycle_pre_generic(c, R[0]));
+    PIC_TRY(vcycle_rest(c, R[0], Z[0], Z[1], 1));
+    hipLaunchKernelGGL(k_beta, dim3(1), dim3(1024), 0, s, part + P_RZ * kNP, part + P_RZO * kNP, c->scal, c->flags,
                        1);
-    PIC_TRY(precondition(c, 1));
-    hipLaunchKernelGGL(k_update_p, dim3(kNP), dim3(kB), 0, s, G, c->pp, c->pz, c->pzold, part + P_RZ * kNP,
-                       part + P_RZO * kNP, c->scal, c->flags, 0, 1);
     PIC_HIP(hipGetLastError());
     int it = 0;
+    // the first batch is as long as the previous solve; then two at a time
+    int batch = c->last_iters > 0 ? c->last_iters : 2;
     while (true) {
         PIC_TRY(read_state(c));
         if (c->flags_host[0] || it >= c->p.poisson_max_iter) break;
-        for (int k = 0; k < kBatch && it < c->p.poisson_max_iter; ++k) {
+        for (int k = 0; k < batch && it < c->p.poisson_max_iter; ++k) {
             ++it;
-            hipLaunchKernelGGL(k_apply, dim3(kNP), dim3(kB), 0, s, f, c->pp, c->dir, c->pq, part + P_PQ * kNP,
-                               c->flags);
-            hipLaunchKernelGGL(k_update_xr, dim3(kNP), dim3(kB), 0, s, G, c->phi, c->pr, c->pp, c->pq,
-                               part + P_PQ * kNP, part + P_RZ * kNP, part + P_RR * kNP, c->flags);
-            hipLaunchKernelGGL(k_check, dim3(1), dim3(kB), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2, c->flags,
-                               c->scal, 0);
-            PIC_TRY(precondition(c, 0));
-            hipLaunchKernelGGL(k_update_p, dim3(kNP), dim3(kB), 0, s, G, c->pp, c->pz, c->pzold, part + P_RZ * kNP,
-                               part + P_RZO * kNP, c->scal, c->flags, it & 1, 0);
+            const int cur = it & 1, old = cur ^ 1;
+            hipLaunchKernelGGL(k_apply_p, dim3(kNP), dim3(kB), 0, s, f, Z[old], P[old], P[cur], c->dir, c->pq,
+                               part + P_PQ * kNP, c->scal, c->flags, it == 1 ? 1 : 0);
+            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1024), 0, s, part + P_PQ * kNP, c->scal, c->flags);
+            if (c->p.mg_pre == 2)
+                hipLaunchKernelGGL(k_update_pre2<1>, dim3(kNP), dim3(kB), 0, s, f, l0, c->phi, P[cur], R[old], R[cur],
+                                   c->pq, c->dir, m0.b, m0.e, part + P_RR * kNP, c->scal, c->flags);
+            else
+                hipLaunchKernelGGL(k_update_pre2<0>, dim3(kNP), dim3(kB), 0, s, f, l0, c->phi, P[cur], R[old], R[cur],
+                                   c->pq, c->dir, m0.b, m0.e, part + P_RR * kNP, c->scal, c->flags);
+            hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2,
+                               c->flags, c->scal, 0);
+            if (c->p.mg_pre != 2) PIC_TRY(vcycle_pre_generic_fine(c));
+            PIC_TRY(vcycle_rest(c, R[cur], Z[cur], Z[old], 0));
+            hipLaunchKernelGGL(k_beta, dim3(1), dim3(1024), 0, s, part + P_RZ * kNP, part + P_RZO * kNP, c->scal,
+                               c->flags, 0);
             PIC_HIP(hipGetLastError());
         }
+        batch = 2;
     }
     const double bb = c->scal_host[S_BB];
     const double bnorm = std::sqrt(bb);
@@ -901,7 +1072,8 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     c->guess_set = 0;
-    c->have_prev_phi = 1;
+    c->have_prev_phi = (extrap && keep) ? 2 : 1;
+    c->last_iters = iters;
     PIC_TRY(timer_end(c, PIC_STAGE_POISSON, ev));
     if (st) {
         st->iterations = iters;

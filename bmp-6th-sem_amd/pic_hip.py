@@ -48,6 +48,7 @@ class Params(C.Structure):
         ("charge_shift", C.c_int), ("energy_shift", C.c_int), ("deposit_energy", C.c_int),
         ("poisson_tol", C.c_double), ("poisson_max_iter", C.c_int),
         ("mg_pre", C.c_int), ("mg_post", C.c_int), ("mg_omega", C.c_double),
+        ("poisson_guess", C.c_int),
     ]
 
 

@@ -95,6 +95,8 @@ typedef struct pic_params {
     int    poisson_max_iter;  /* cap on outer (FP64) iterations                                */
     int    mg_pre, mg_post;   /* weighted-Jacobi sweeps before / after coarse correction      */
     double mg_omega;          /* Jacobi weight                                                 */
+    int    poisson_guess;     /* initial guess: 0 previous phi (the reference's warm start),
+                                 1 linear extrapolation 2 phi_n-1 - phi_n-2 of the last two solves */
 } pic_params;
 
 /* Per-solve statistics of pic_poisson_solve. */
@@ -103,7 +105,8 @@ typedef struct pic_solve_stats {
     double initial_relres;    /* ||b - A phi0|| / ||b|| of the initial guess       */
     double final_relres;      /* same after the solve                              */
     double ms;                /* device time of the solve (HIP events)             */
-    int    guess;             /* 0 zero guess, 1 previous phi, 2 caller-set guess  */
+    int    guess;             /* 0 zero guess, 1 previous phi, 2 caller-set guess,
+                                 3 extrapolated from the last two solves           */
 } pic_solve_stats;
 
 /* ====================================================== stage group */
