@@ -38,7 +38,7 @@ EXTRA = {
     "pic_particles.hip": ["-ffp-contract=off"],
 }
 
-SOURCES = ["pic_particles.hip", "pic_poisson.hip", "pic_host.cpp", "pic_config.cpp", "pic_dropin.cpp"]
+SOURCES = ["pic_particles.hip", "pic_poisson.hip", "pic_unet.hip", "pic_host.cpp", "pic_config.cpp", "pic_dropin.cpp"]
 
 
 def _compile(src: str) -> Path:

@@ -91,6 +91,7 @@ extern "C" void pic_destroy(pic_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& s : c->sp) free_species(s);
     poisson_free(c);
+    if (c->unet) pic_unet_destroy(c->unet);
     void* ptrs[] = {c->rho, c->phi, c->ex, c->ey, c->bz, c->fld, c->dir, c->dval, c->pr, c->pr2, c->pp, c->pp2,
                     c->pq, c->pb, c->zf0, c->zf1, c->guess, c->phi_prev, c->part, c->scal, c->flags, c->scan_bsum};
     for (void* q : ptrs)
@@ -802,5 +803,40 @@ extern "C" int pic_grid_shape(pic_ctx* c, int* gx, int* gy) {
     }
     if (gx) *gx = c->g.gx;
     if (gy) *gy = c->g.gy;
+    return PIC_OK;
+}
+
+extern "C" int pic_learned_enable(pic_ctx* c, int train_steps, int min_trained, float lr, uint64_t seed) {
+    if (!c || train_steps < 0 || min_trained < 0 || !(lr > 0.f)) {
+        set_error("pic_learned_enable: invalid argument");
+        return PIC_ERR_ARG;
+    }
+    const int n = (std::min(c->p.NX, c->p.NY) / 128) * 128;
+    if (n < 128) {
+        set_error("pic_learned_enable: the grid needs at least 128 x 128 cells (got %d x %d)", c->p.NX, c->p.NY);
+        return PIC_ERR_UNSUPPORTED;
+    }
+    if (c->unet) {
+        pic_unet_destroy(c->unet);
+        c->unet = nullptr;
+    }
+    PIC_TRY(pic_unet_create(n, seed, &c->unet));
+    c->learn_train = train_steps;
+    c->learn_min = min_trained;
+    c->learn_lr = lr;
+    c->learn_trained = c->learn_tried = c->learn_used = 0;
+    c->learn_loss = 0.f;
+    return PIC_OK;
+}
+
+extern "C" int pic_learned_stats(pic_ctx* c, int* trained, int* tried, int* used, float* last_loss) {
+    if (!c) {
+        set_error("pic_learned_stats: null context");
+        return PIC_ERR_ARG;
+    }
+    if (trained) *trained = c->learn_trained;
+    if (tried) *tried = c->learn_tried;
+    if (used) *used = c->learn_used;
+    if (last_loss) *last_loss = c->learn_loss;
     return PIC_OK;
 }

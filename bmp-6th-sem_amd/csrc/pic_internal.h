@@ -137,6 +137,12 @@ struct pic_ctx {
     int guess_set = 0;
     int have_prev_phi = 0;                    // solutions kept: 0, 1 (phi) or 2 (phi, phi_prev)
     int last_iters = 0;
+    // learned initial guess
+    pic_unet* unet = nullptr;
+    int learn_train = 0, learn_min = 0;       // pairs to train on; pairs before the guess is offered
+    float learn_lr = 1e-3f;
+    int learn_trained = 0, learn_tried = 0, learn_used = 0;
+    float learn_loss = 0.f;
     std::vector<pic::MgLevel> mg;
     float* mg_inv = nullptr;        // inverse of the coarsest level's operator (unknowns only)
     int* mg_unk = nullptr;          // node index of each coarsest-level unknown

@@ -32,6 +32,7 @@
 // only between batches of iterations.  Levels of at most 65 x 65 nodes run
 // inside one workgroup's LDS.
 #include "pic_internal.h"
+#include "pic_unet.h"
 
 #include <algorithm>
 #include <cmath>
@@ -42,7 +43,7 @@ static constexpr int kB = 256;
 static constexpr int kNP = PIC_PART_BLOCKS;  // blocks of the reduction kernels (partials)
 
 // device scalars
-enum Scal { S_RR = 0, S_BB, S_RR0, S_RZ, S_ALPHA, S_BETA, S_N };
+enum Scal { S_RR = 0, S_BB, S_RR0, S_RZ, S_ALPHA, S_BETA, S_G0, S_G1, S_N };
 // partial-sum arrays, kNP doubles each
 enum Part { P_RR = 0, P_BB, P_PQ, P_RZ, P_RZO, P_N };
 
@@ -197,6 +198,17 @@ __global__ __launch_bounds__(1024) void k_beta(const double* part_rz, const doub
     if (threadIdx.x == 0) {
         scal[S_BETA] = first ? 0.0 : (rz - rzo) / scal[S_RZ];
         scal[S_RZ] = rz;
+    }
+}
+
+// scal[S_G0], scal[S_G1] = sums of two partial arrays (residuals of two guesses)
+__global__ __launch_bounds__(1024) void k_sum2(const double* pa, const double* pb, double* scal) {
+    __shared__ double sh[16];
+    const double a = sum_partials(pa, sh);
+    const double b = sum_partials(pb, sh);
+    if (threadIdx.x == 0) {
+        scal[S_G0] = a;
+        scal[S_G1] = b;
     }
 }
 
@@ -1009,6 +1021,26 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
         }
         if (extrap) keep = c->phi_prev;
     }
+    // learned guess: keep phi_{n-1} for training; try the prediction against it
+    const bool learned = c->unet && !c->guess_set && iteration > 0 && c->have_prev_phi;
+    if (learned) {
+        PIC_HIP(hipMemcpyAsync(c->phi_prev, c->phi, (size_t)G * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (c->learn_trained >= c->learn_min) {
+            PIC_TRY(unet_guess(c, c->unet, c->guess));
+            hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, R[0],
+                               part + P_RR * kNP);
+            hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->guess, c->dir, R[1],
+                               part + P_RZO * kNP);
+            hipLaunchKernelGGL(k_sum2, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_RZO * kNP, c->scal);
+            PIC_TRY(read_state(c));
+            c->learn_tried += 1;
+            if (c->scal_host[S_G1] < c->scal_host[S_G0]) {
+                src = c->guess;
+                guess = 4;
+                c->learn_used += 1;
+            }
+        }
+    }
     hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, src, prev, keep, c->dir, c->dval);
     hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, R[0], part + P_RR * kNP);
     hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2, c->flags,
@@ -1074,6 +1106,12 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     c->guess_set = 0;
     c->have_prev_phi = (extrap && keep) ? 2 : 1;
     c->last_iters = iters;
+    if (learned && c->learn_trained < c->learn_train) {
+        float loss = 0.f;
+        PIC_TRY(unet_train_pair(c, c->unet, c->phi_prev, c->learn_lr, &loss));
+        c->learn_trained += 1;
+        c->learn_loss = loss;
+    }
     PIC_TRY(timer_end(c, PIC_STAGE_POISSON, ev));
     if (st) {
         st->iterations = iters;

@@ -1,0 +1,994 @@
+// pic_unet.hip — learned initial guess for the Poisson solve.
+//
+// A 4-level U-Net over the N x N node image (3x3 convolutions, 16/32/64/64
+// channels, ReLU, 2x2 average pooling, nearest upsampling with skip
+// concatenation, a 1x1 head) maps (rho_n, phi_{n-1}) to the change
+// phi_n - phi_{n-1}.  Weights and activations are BF16, every convolution
+// accumulates in FP32 on the MFMA units (v_mfma_f32_16x16x16_bf16 as an
+// implicit GEMM: M = 16 pixels of a row, N = 16 output channels, K = 16 input
+// channels of one tap).  Training is online: forward, backward (input
+// gradients by the same kernel with flipped, transposed weights; weight
+// gradients by MFMA over pixels) and Adam with FP32 master weights.
+//
+// Activations are NHWC with one zero pixel of padding on every side, so the
+// 3x3 taps need no bounds checks; gradients are FP32 in the same layout.
+#include "pic_internal.h"
+#include "pic_unet.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace pic {
+
+typedef uint16_t bf16;
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(bf16 v) { return __uint_as_float((uint32_t)v << 16); }
+// round to nearest even (finite inputs)
+__device__ __forceinline__ bf16 f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (bf16)(u >> 16);
+}
+static inline bf16 f2bf_host(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (bf16)(u >> 16);
+}
+
+static constexpr int kT = 256;
+static inline unsigned nb(int64_t n) { return (unsigned)((n + kT - 1) / kT); }
+
+// ------------------------------------------------------------- MFMA probe
+// D = A B for 16 x 16 operands given row-major in `a`, `b` (as float, exact in
+// BF16); D row-major in `d`.  Lane l holds A[l%16][4(l/16)+e], B[4(l/16)+e][l%16]
+// and receives D[4(l/16)+e][l%16].
+__global__ void k_mfma_probe(const float* a, const float* b, float* d) {
+    const int l = threadIdx.x, i = l & 15, kq = l >> 4;
+    s4v av, bv;
+    for (int e = 0; e < 4; ++e) {
+        av[e] = (short)f2bf(a[i * 16 + 4 * kq + e]);
+        bv[e] = (short)f2bf(b[(4 * kq + e) * 16 + i]);
+    }
+    f4v acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    for (int e = 0; e < 4; ++e) d[(4 * kq + e) * 16 + i] = acc[e];
+}
+
+// ----------------------------------------------------------- convolution
+// Y[p][co] = act(bias[co] + sum_{tap, ci} Wt[tap][co][ci] X[p + off(tap)][ci])
+// X: padded NHWC, channel stride xC, first channel xo, cin channels (multiple
+// of 16).  Wt: [9][NT*16][cin].  Y: padded NHWC (stride yC, first channel yo),
+// BF16 (forward) or FP32 (input gradients).  One wave computes 64 pixels of a
+// row (4 M tiles) by NT*16 output channels.
+template <int NT, int RELU, typename TO>
+__global__ __launch_bounds__(kT) void k_conv3x3(const bf16* __restrict__ X, int xC, int xo, int cin,
+                                                const bf16* __restrict__ Wt, const float* __restrict__ bias,
+                                                TO* __restrict__ Y, int yC, int yo, int H, int W) {
+    const int lane = threadIdx.x & 63;
+    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nseg = (W + 63) >> 6;
+    const int y = gw / nseg;
+    if (y >= H) return;
+    const int x0 = (gw - y * nseg) * 64;
+    const int i = lane & 15, kq = lane >> 4;
+    constexpr int cout = NT * 16;
+    const int Wp = W + 2;
+    f4v acc[4][NT];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][t] = (f4v){0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < 9; ++tap) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const bf16* xrow = X + (size_t)(y + 1 + dy) * Wp * xC + xo;
+        const bf16* wtap = Wt + (size_t)tap * cout * cin;
+        for (int cc = 0; cc < cin; cc += 16) {
+            s4v a[4], b[NT];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int x = x0 + m * 16 + i;
+                if (x < W) a[m] = *(const s4v*)(xrow + (size_t)(x + 1 + dx) * xC + cc + 4 * kq);
+                else a[m] = (s4v){0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int t = 0; t < NT; ++t) b[t] = *(const s4v*)(wtap + (size_t)(t * 16 + i) * cin + cc + 4 * kq);
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b[t], acc[m][t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + i;
+        const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int x = x0 + m * 16 + 4 * kq + e;
+                if (x >= W) continue;
+                float v = acc[m][t][e] + bb;
+                if (RELU) v = fmaxf(v, 0.f);
+                TO* dst = Y + ((size_t)(y + 1) * Wp + x + 1) * yC + yo + co;
+                if constexpr (sizeof(TO) == 2) *dst = f2bf(v);
+                else *dst = v;
+            }
+    }
+}
+
+// dW[tap][co][ci] += sum_{y,x} dZT[co][y][x] XT[ci][y+1+dy][x+1+dx]
+// dZT: [cout][H][W] (unpadded), XT: [cin][H+2][W+2] (padded).  One wave: one
+// tap, one 16-channel co tile, CT ci tiles, rows [r0, r0 + rows); FP32 atomics
+// add the wave's tile into dW.
+template <int CT>
+__global__ __launch_bounds__(kT) void k_conv_wgrad(const bf16* __restrict__ dZT, const bf16* __restrict__ XT,
+                                                   int cout, int cin, int H, int W, int rows,
+                                                   float* __restrict__ dW) {
+    const int lane = threadIdx.x & 63;
+    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nchunk = (H + rows - 1) / rows;
+    const int cot_n = cout >> 4;
+    const int chunk = gw % nchunk;
+    const int t2 = gw / nchunk;
+    const int cot = t2 % cot_n, tap = t2 / cot_n;
+    if (tap >= 9) return;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int i = lane & 15, kq = lane >> 4;
+    const int Wp = W + 2;
+    f4v acc[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[c] = (f4v){0.f, 0.f, 0.f, 0.f};
+    const int r0 = chunk * rows, r1 = min(H, r0 + rows);
+    for (int y = r0; y < r1; ++y) {
+        const bf16* arow = dZT + ((size_t)(cot * 16 + i) * H + y) * W;
+        for (int x0 = 0; x0 < W; x0 += 16) {
+            const s4v a = *(const s4v*)(arow + x0 + 4 * kq);
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                const bf16* src = XT + ((size_t)(c * 16 + i) * (H + 2) + y + 1 + dy) * Wp + x0 + 4 * kq + 1 + dx;
+                s4v b;
+                b[0] = (short)src[0];
+                b[1] = (short)src[1];
+                b[2] = (short)src[2];
+                b[3] = (short)src[3];
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[c], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int co = cot * 16 + 4 * kq + e, ci = c * 16 + i;
+            atomicAdd(dW + ((size_t)tap * cout + co) * cin + ci, acc[c][e]);
+        }
+}
+
+// db[co] += sum_p dZ[p][co]; dZ padded NHWC with cout channels.
+__global__ __launch_bounds__(kT) void k_bias_grad(const bf16* __restrict__ dZ, int cout, int H, int W, int chunk,
+                                                  float* __restrict__ db) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int co = t % cout, part = t / cout;
+    const int HW = H * W;
+    const int p0 = part * chunk;
+    if (p0 >= HW) return;
+    const int p1 = min(HW, p0 + chunk);
+    float s = 0.f;
+    for (int p = p0; p < p1; ++p) {
+        const int y = p / W, x = p - y * W;
+        s += bf2f(dZ[((size_t)(y + 1) * (W + 2) + x + 1) * cout + co]);
+    }
+    atomicAdd(db + co, s);
+}
+
+// dZ = bf16(dY * (Y > 0)); dY FP32 (stride gC, first channel go), Y BF16
+// (stride yC, first channel yo), dZ padded NHWC with c channels.  The whole
+// padded grid is written (zero halo): the buffer is shared by layers of
+// different shapes and the input-gradient convolution reads the halo.
+__global__ void k_relu_bwd(const float* __restrict__ dY, int gC, int go, const bf16* __restrict__ Y, int yC, int yo,
+                           bf16* __restrict__ dZ, int c, int H, int W) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t tot = (int64_t)(H + 2) * (W + 2) * c;
+    if (n >= tot) return;
+    const int ch = (int)(n % c);
+    const int64_t q = n / c;
+    const int yy = (int)(q / (W + 2)), xx = (int)(q - (int64_t)yy * (W + 2));
+    bf16 r = 0;
+    if (yy >= 1 && yy <= H && xx >= 1 && xx <= W) {
+        const float g = dY[q * gC + go + ch];
+        const float v = bf2f(Y[q * yC + yo + ch]);
+        r = f2bf(v > 0.f ? g : 0.f);
+    }
+    dZ[n] = r;
+}
+
+// out[c][y][x] = in[(y * (W + 2) + x) * C + o + c] over the padded grid (pad = 1)
+// or the interior only (pad = 0, out [c][H][W]).
+__global__ void k_to_chw(const bf16* __restrict__ in, int C, int o, int c, int H, int W, int pad,
+                         bf16* __restrict__ out) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ho = H + 2 * pad, Wo = W + 2 * pad;
+    const int64_t tot = (int64_t)c * Ho * Wo;
+    if (n >= tot) return;
+    const int x = (int)(n % Wo);
+    const int64_t r = n / Wo;
+    const int y = (int)(r % Ho), ch = (int)(r / Ho);
+    const int yy = y + 1 - pad, xx = x + 1 - pad;
+    out[n] = in[((size_t)yy * (W + 2) + xx) * C + o + ch];
+}
+
+// 2x2 average pooling: out (H/2 x W/2) <- in (H x W), c channels.
+__global__ void k_pool2(const bf16* __restrict__ in, int iC, int io, bf16* __restrict__ out, int oC, int oo, int c,
+                        int H, int W) {
+    const int h = H / 2, w = W / 2;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (int64_t)h * w * c) return;
+    const int ch = (int)(n % c);
+    const int64_t p = n / c;
+    const int y = (int)(p / w), x = (int)(p - (int64_t)y * w);
+    auto at = [&](int yy, int xx) { return bf2f(in[((size_t)(yy + 1) * (W + 2) + xx + 1) * iC + io + ch]); };
+    const float s = (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1)) + (at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1));
+    out[((size_t)(y + 1) * (w + 2) + x + 1) * oC + oo + ch] = f2bf(0.25f * s);
+}
+
+// gin (H x W) += 0.25 gout (H/2 x W/2) at each of the four children.
+__global__ void k_pool2_bwd(const float* __restrict__ gout, int oC, int oo, float* __restrict__ gin, int iC, int io,
+                            int c, int H, int W) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (int64_t)H * W * c) return;
+    const int ch = (int)(n % c);
+    const int64_t p = n / c;
+    const int y = (int)(p / W), x = (int)(p - (int64_t)y * W);
+    const int w = W / 2;
+    const float g = gout[((size_t)(y / 2 + 1) * (w + 2) + x / 2 + 1) * oC + oo + ch];
+    gin[((size_t)(y + 1) * (W + 2) + x + 1) * iC + io + ch] += 0.25f * g;
+}
+
+// nearest upsampling: out (2h x 2w) <- in (h x w)
+__global__ void k_up2(const bf16* __restrict__ in, int iC, int io, bf16* __restrict__ out, int oC, int oo, int c,
+                      int h, int w) {
+    const int H = 2 * h, W = 2 * w;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (int64_t)H * W * c) return;
+    const int ch = (int)(n % c);
+    const int64_t p = n / c;
+    const int y = (int)(p / W), x = (int)(p - (int64_t)y * W);
+    out[((size_t)(y + 1) * (W + 2) + x + 1) * oC + oo + ch] = in[((size_t)(y / 2 + 1) * (w + 2) + x / 2 + 1) * iC + io + ch];
+}
+
+// gin (h x w) = sum of gout over the four children (2h x 2w)
+__global__ void k_up2_bwd(const float* __restrict__ gout, int oC, int oo, float* __restrict__ gin, int iC, int io,
+                          int c, int h, int w) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (int64_t)h * w * c) return;
+    const int ch = (int)(n % c);
+    const int64_t p = n / c;
+    const int y = (int)(p / w), x = (int)(p - (int64_t)y * w);
+    const int W = 2 * w;
+    auto at = [&](int yy, int xx) { return gout[((size_t)(yy + 1) * (W + 2) + xx + 1) * oC + oo + ch]; };
+    gin[((size_t)(y + 1) * (w + 2) + x + 1) * iC + io + ch] =
+        (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1)) + (at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1));
+}
+
+// out[y][x] = bh + sum_c wh[c] d0[y][x][c]  (FP32)
+__global__ void k_head(const bf16* __restrict__ d0, int c, const float* __restrict__ wh, const float* __restrict__ bh,
+                       float* __restrict__ out, int H, int W) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= H * W) return;
+    const int y = p / W, x = p - y * W;
+    const bf16* v = d0 + ((size_t)(y + 1) * (W + 2) + x + 1) * c;
+    float s = bh[0];
+    for (int k = 0; k < c; ++k) s += wh[k] * bf2f(v[k]);
+    out[p] = s;
+}
+
+// loss = sum over masked pixels of (out - t)^2 / count; g = dloss/dout;
+// gd0[p][c] = g wh[c]; dwh[c] += sum g d0[c]; dbh += sum g; loss += sum.
+// count: number of masked pixels (device).  c = 16 channels.
+__global__ __launch_bounds__(kT) void k_head_bwd(const float* __restrict__ out, const float* __restrict__ tgt,
+                                                 const uint8_t* __restrict__ mask, const int* __restrict__ count,
+                                                 const bf16* __restrict__ d0, const float* __restrict__ wh,
+                                                 float* __restrict__ gd0, float* __restrict__ dwh,
+                                                 float* __restrict__ dbh, float* __restrict__ loss, int H, int W) {
+    __shared__ float sh[18][kT / 64];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const float inv = *count > 0 ? 1.f / (float)*count : 0.f;
+    float red[18];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) red[k] = 0.f;
+    if (p < H * W) {
+        const int y = p / W, x = p - y * W;
+        const size_t q = (size_t)(y + 1) * (W + 2) + x + 1;
+        float g = 0.f;
+        if (mask[p]) {
+            const float d = out[p] - tgt[p];
+            red[17] = d * d * inv;
+            g = 2.f * d * inv;
+        }
+        red[16] = g;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            gd0[q * 16 + k] = g * wh[k];
+            red[k] = g * bf2f(d0[q * 16 + k]);
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) {
+        float v = red[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) sh[k][wid] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 18) {
+        float s = 0.f;
+        for (int w = 0; w < kT / 64; ++w) s += sh[threadIdx.x][w];
+        if (threadIdx.x < 16) atomicAdd(dwh + threadIdx.x, s);
+        else if (threadIdx.x == 16) atomicAdd(dbh, s);
+        else atomicAdd(loss, s);
+    }
+}
+
+__global__ void k_zero_f32(float* __restrict__ p, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0.f;
+}
+
+// count[0] = number of non-zero mask entries (count zeroed by the caller's kernel)
+__global__ __launch_bounds__(kT) void k_mask_count(const uint8_t* __restrict__ mask, int n, int* __restrict__ count) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int v = (p < n && mask[p]) ? 1 : 0;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(count, v);
+}
+
+__global__ void k_zero_i32(int* p) { *p = 0; }
+
+// Adam on FP32 masters; refreshes the BF16 copy (and for conv weights the
+// flipped, transposed copy used by the input-gradient convolution).
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int n, float lr_t, float b1, float b2, float eps,
+                       bf16* __restrict__ wb, bf16* __restrict__ wflip, int cout, int cin) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float gk = g[k];
+    const float mk = b1 * m[k] + (1.f - b1) * gk;
+    const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    const float wk = w[k] - lr_t * mk / (sqrtf(vk) + eps);
+    w[k] = wk;
+    if (wb) {
+        const bf16 b = f2bf(wk);
+        wb[k] = b;
+        if (wflip) {
+            // k = (tap * cout + co) * cin + ci  ->  wflip[(8 - tap) * cin + ci][co]
+            const int ci = k % cin, r = k / cin, co = r % cout, tap = r / cout;
+            wflip[((size_t)(8 - tap) * cin + ci) * cout + co] = b;
+        }
+    }
+}
+
+// inputs from the PIC grids: x0[y][x] = (rho s_rho, phi s_phi, 0...)
+__global__ void k_unet_input(const double* __restrict__ rho, const double* __restrict__ phi, int gx, int N,
+                             double s_rho, double s_phi, bf16* __restrict__ x0) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N * N) return;
+    const int y = p / N, x = p - y * N;
+    bf16* d = x0 + ((size_t)(y + 1) * (N + 2) + x + 1) * 16;
+    d[0] = f2bf((float)(rho[y * gx + x] * s_rho));
+    d[1] = f2bf((float)(phi[y * gx + x] * s_phi));
+}
+
+// target (phi_new - phi_old) s_delta on the image; mask = not Dirichlet
+__global__ void k_unet_target(const double* __restrict__ phi_new, const double* __restrict__ phi_old,
+                              const uint8_t* __restrict__ dir, int gx, int N, double s_delta, float* __restrict__ t,
+                              uint8_t* __restrict__ mask) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N * N) return;
+    const int y = p / N, x = p - y * N;
+    const int n = y * gx + x;
+    t[p] = (float)((phi_new[n] - phi_old[n]) * s_delta);
+    mask[p] = dir[n] ? 0 : 1;
+}
+
+// guess = phi_old + out / s_delta on unknown nodes of the image; the rest of
+// the grid keeps phi_old (Dirichlet values are imposed by the solver).
+__global__ void k_unet_guess(const double* __restrict__ phi_old, const float* __restrict__ out,
+                             const uint8_t* __restrict__ dir, int gx, int gy, int N, double inv_s_delta,
+                             double* __restrict__ guess) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= gx * gy) return;
+    const int y = n / gx, x = n - y * gx;
+    double v = phi_old[n];
+    if (x < N && y < N && !dir[n]) v += (double)out[y * N + x] * inv_s_delta;
+    guess[n] = v;
+}
+
+// ================================================================= host side
+namespace {
+
+struct Tensor {
+    bf16* p = nullptr;   // padded NHWC
+    int H = 0, W = 0, C = 0;
+    float* g = nullptr;  // gradient (same layout, FP32), when needed
+};
+
+struct Conv {
+    int in, io, cin;     // input tensor index, first channel, channels (multiple of 16)
+    int cin_real;        // channels carrying data (fan-in of the initialisation)
+    int out, oo, cout;   // output tensor index, first channel, channels
+    size_t woff, boff;   // offsets of W [9][cout][cin] and b [cout] in the parameter vector
+};
+
+struct Resample {        // pool (down) or upsample (up)
+    bool up;
+    int in, io, out, oo, c;
+};
+
+} // namespace
+} // namespace pic
+
+struct pic_unet {
+    int N = 0;
+    std::vector<pic::Tensor> t;           // activations
+    std::vector<pic::Conv> conv;
+    // parameter vector: all conv W and b, then head wh[16], bh
+    size_t nparam = 0, head_w = 0, head_b = 0;
+    float* w = nullptr;                   // FP32 masters
+    float* g = nullptr;                   // gradients
+    float* m = nullptr;
+    float* v = nullptr;
+    pic::bf16* wb = nullptr;              // BF16 copies (conv weights; biases/head unused)
+    pic::bf16* wflip = nullptr;           // [9][cin][cout] flipped copies, same offsets as W
+    float* out = nullptr;                 // head output N x N
+    float* tgt = nullptr;
+    uint8_t* mask = nullptr;
+    float* loss = nullptr;
+    int* count = nullptr;
+    float* loss_host = nullptr;
+    pic::bf16* dz = nullptr;              // relu'ed output gradient (padded NHWC)
+    pic::bf16* dzt = nullptr;             // its CHW transpose
+    pic::bf16* xt = nullptr;              // CHW padded transpose of a conv input
+    int step = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double s_rho = 0, s_phi = 0, s_delta = 0;   // input / target scales (set from the first pair)
+};
+
+namespace pic {
+
+static int unet_fail(pic_unet* u, const char* what) {
+    set_error("pic_unet: %s", what);
+    (void)u;
+    return PIC_ERR_ALLOC;
+}
+
+// layer table, built for an N x N image (N a multiple of 64)
+static int unet_build(pic_unet* u) {
+    const int N = u->N;
+    auto add = [&](int H, int C) {
+        Tensor x;
+        x.H = H;
+        x.W = H;
+        x.C = C;
+        u->t.push_back(x);
+        return (int)u->t.size() - 1;
+    };
+    const int X0 = add(N, 16), A0 = add(N, 16), CAT0 = add(N, 48);
+    const int P0 = add(N / 2, 16), A1 = add(N / 2, 32), CAT1 = add(N / 2, 96);
+    const int P1 = add(N / 4, 32), A2 = add(N / 4, 64), CAT2 = add(N / 4, 128);
+    const int P2 = add(N / 8, 64), A3 = add(N / 8, 64), B3 = add(N / 8, 64);
+    const int A4 = add(N / 4, 64), D2 = add(N / 4, 64);
+    const int A5 = add(N / 2, 32), D1 = add(N / 2, 32);
+    const int A6 = add(N, 16), D0 = add(N, 16);
+    size_t off = 0;
+    auto conv = [&](int in, int io, int cin, int cin_real, int out, int oo, int cout) {
+        Conv c{in, io, cin, cin_real, out, oo, cout, 0, 0};
+        c.woff = off;
+        off += (size_t)9 * cout * cin;
+        c.boff = off;
+        off += cout;
+        u->conv.push_back(c);
+    };
+    conv(X0, 0, 16, 2, A0, 0, 16);          // 0
+    conv(A0, 0, 16, 16, CAT0, 32, 16);      // 1  s0
+    conv(P0, 0, 16, 16, A1, 0, 32);         // 2
+    conv(A1, 0, 32, 32, CAT1, 64, 32);      // 3  s1
+    conv(P1, 0, 32, 32, A2, 0, 64);         // 4
+    conv(A2, 0, 64, 64, CAT2, 64, 64);      // 5  s2
+    conv(P2, 0, 64, 64, A3, 0, 64);         // 6
+    conv(A3, 0, 64, 64, B3, 0, 64);         // 7
+    conv(CAT2, 0, 128, 128, A4, 0, 64);     // 8
+    conv(A4, 0, 64, 64, D2, 0, 64);         // 9
+    conv(CAT1, 0, 96, 96, A5, 0, 32);       // 10
+    conv(A5, 0, 32, 32, D1, 0, 32);         // 11
+    conv(CAT0, 0, 48, 48, A6, 0, 16);       // 12
+    conv(A6, 0, 16, 16, D0, 0, 16);         // 13
+    u->head_w = off;
+    off += 16;
+    u->head_b = off;
+    off += 1;
+    u->nparam = off;
+    (void)B3;
+    (void)D2;
+    (void)D1;
+    return PIC_OK;
+}
+
+// tensor indices used by the fixed schedule below
+enum { X0 = 0, A0, CAT0, P0, A1, CAT1, P1, A2, CAT2, P2, A3, B3, A4, D2, A5, D1, A6, D0 };
+
+template <int NT, int RELU, typename TO>
+static void conv_launch(hipStream_t s, const bf16* X, int xC, int xo, int cin, const bf16* Wt, const float* bias,
+                        TO* Y, int yC, int yo, int H, int W) {
+    const int waves = H * ((W + 63) / 64);
+    hipLaunchKernelGGL((k_conv3x3<NT, RELU, TO>), dim3((unsigned)((waves + 3) / 4)), dim3(kT), 0, s, X, xC, xo, cin,
+                       Wt, bias, Y, yC, yo, H, W);
+}
+
+template <int RELU, typename TO>
+static int conv_any(hipStream_t s, int nt, const bf16* X, int xC, int xo, int cin, const bf16* Wt, const float* bias,
+                    TO* Y, int yC, int yo, int H, int W) {
+    switch (nt) {
+    case 1: conv_launch<1, RELU, TO>(s, X, xC, xo, cin, Wt, bias, Y, yC, yo, H, W); break;
+    case 2: conv_launch<2, RELU, TO>(s, X, xC, xo, cin, Wt, bias, Y, yC, yo, H, W); break;
+    case 3: conv_launch<3, RELU, TO>(s, X, xC, xo, cin, Wt, bias, Y, yC, yo, H, W); break;
+    case 4: conv_launch<4, RELU, TO>(s, X, xC, xo, cin, Wt, bias, Y, yC, yo, H, W); break;
+    case 6: conv_launch<6, RELU, TO>(s, X, xC, xo, cin, Wt, bias, Y, yC, yo, H, W); break;
+    case 8: conv_launch<8, RELU, TO>(s, X, xC, xo, cin, Wt, bias, Y, yC, yo, H, W); break;
+    default: set_error("pic_unet: unsupported channel count %d", nt * 16); return PIC_ERR_UNSUPPORTED;
+    }
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+static int wgrad_any(hipStream_t s, int ct, const bf16* dZT, const bf16* XT, int cout, int cin, int H, int W,
+                     float* dW) {
+    const int rows = H >= 64 ? 8 : 1;
+    const int waves = 9 * (cout / 16) * ((H + rows - 1) / rows);
+    const dim3 grid((unsigned)((waves + 3) / 4));
+    switch (ct) {
+    case 1: hipLaunchKernelGGL(k_conv_wgrad<1>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, dW); break;
+    case 2: hipLaunchKernelGGL(k_conv_wgrad<2>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, dW); break;
+    case 3: hipLaunchKernelGGL(k_conv_wgrad<3>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, dW); break;
+    case 4: hipLaunchKernelGGL(k_conv_wgrad<4>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, dW); break;
+    case 6: hipLaunchKernelGGL(k_conv_wgrad<6>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, dW); break;
+    case 8: hipLaunchKernelGGL(k_conv_wgrad<8>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, dW); break;
+    default: set_error("pic_unet: unsupported channel count %d", ct * 16); return PIC_ERR_UNSUPPORTED;
+    }
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+static size_t tsize(const Tensor& x) { return (size_t)(x.H + 2) * (x.W + 2) * x.C; }
+
+static int resample(pic_unet* u, bool up, int in, int io, int out, int oo, int c) {
+    const Tensor& a = u->t[in];
+    const Tensor& b = u->t[out];
+    if (!up) {
+        const int64_t n = (int64_t)b.H * b.W * c;
+        hipLaunchKernelGGL(k_pool2, dim3(nb(n)), dim3(kT), 0, u->stream, a.p, a.C, io, b.p, b.C, oo, c, a.H, a.W);
+    } else {
+        const int64_t n = (int64_t)b.H * b.W * c;
+        hipLaunchKernelGGL(k_up2, dim3(nb(n)), dim3(kT), 0, u->stream, a.p, a.C, io, b.p, b.C, oo, c, a.H, a.W);
+    }
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+static int conv_fwd(pic_unet* u, int k) {
+    const Conv& c = u->conv[k];
+    const Tensor& a = u->t[c.in];
+    Tensor& b = u->t[c.out];
+    return conv_any<1, bf16>(u->stream, c.cout / 16, a.p, a.C, c.io, c.cin, u->wb + c.woff, u->w + c.boff, b.p, b.C,
+                             c.oo, a.H, a.W);
+}
+
+static int unet_forward(pic_unet* u) {
+    PIC_TRY(conv_fwd(u, 0));
+    PIC_TRY(conv_fwd(u, 1));
+    PIC_TRY(resample(u, false, CAT0, 32, P0, 0, 16));
+    PIC_TRY(conv_fwd(u, 2));
+    PIC_TRY(conv_fwd(u, 3));
+    PIC_TRY(resample(u, false, CAT1, 64, P1, 0, 32));
+    PIC_TRY(conv_fwd(u, 4));
+    PIC_TRY(conv_fwd(u, 5));
+    PIC_TRY(resample(u, false, CAT2, 64, P2, 0, 64));
+    PIC_TRY(conv_fwd(u, 6));
+    PIC_TRY(conv_fwd(u, 7));
+    PIC_TRY(resample(u, true, B3, 0, CAT2, 0, 64));
+    PIC_TRY(conv_fwd(u, 8));
+    PIC_TRY(conv_fwd(u, 9));
+    PIC_TRY(resample(u, true, D2, 0, CAT1, 0, 64));
+    PIC_TRY(conv_fwd(u, 10));
+    PIC_TRY(conv_fwd(u, 11));
+    PIC_TRY(resample(u, true, D1, 0, CAT0, 0, 32));
+    PIC_TRY(conv_fwd(u, 12));
+    PIC_TRY(conv_fwd(u, 13));
+    const int N = u->N;
+    hipLaunchKernelGGL(k_head, dim3(nb((int64_t)N * N)), dim3(kT), 0, u->stream, u->t[D0].p, 16, u->w + u->head_w,
+                       u->w + u->head_b, u->out, N, N);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+// Backward of conv k given the FP32 gradient of its output (in the output
+// tensor's gradient buffer); writes the input gradient (unless k == 0) and
+// adds the weight and bias gradients.
+static int conv_bwd(pic_unet* u, int k) {
+    const Conv& c = u->conv[k];
+    const Tensor& a = u->t[c.in];
+    const Tensor& b = u->t[c.out];
+    const int H = a.H, W = a.W;
+    hipStream_t s = u->stream;
+    const int64_t n = (int64_t)(H + 2) * (W + 2) * c.cout;
+    hipLaunchKernelGGL(k_relu_bwd, dim3(nb(n)), dim3(kT), 0, s, b.g, b.C, c.oo, b.p, b.C, c.oo, u->dz, c.cout, H, W);
+    const int64_t nt = (int64_t)c.cout * H * W;
+    hipLaunchKernelGGL(k_to_chw, dim3(nb(nt)), dim3(kT), 0, s, u->dz, c.cout, 0, c.cout, H, W, 0, u->dzt);
+    const int64_t nx = (int64_t)c.cin * (H + 2) * (W + 2);
+    hipLaunchKernelGGL(k_to_chw, dim3(nb(nx)), dim3(kT), 0, s, a.p, a.C, c.io, c.cin, H, W, 1, u->xt);
+    PIC_HIP(hipGetLastError());
+    PIC_TRY(wgrad_any(s, c.cin / 16, u->dzt, u->xt, c.cout, c.cin, H, W, u->g + c.woff));
+    const int chunk = 4096;
+    const int parts = (H * W + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_bias_grad, dim3(nb((int64_t)parts * c.cout)), dim3(kT), 0, s, u->dz, c.cout, H, W, chunk,
+                       u->g + c.boff);
+    PIC_HIP(hipGetLastError());
+    if (k > 0)
+        PIC_TRY((conv_any<0, float>(s, c.cin / 16, u->dz, c.cout, 0, c.cout, u->wflip + c.woff, nullptr, a.g, a.C,
+                                    c.io, H, W)));
+    return PIC_OK;
+}
+
+static int unet_backward(pic_unet* u) {
+    hipStream_t s = u->stream;
+    const int N = u->N;
+    hipLaunchKernelGGL(k_zero_f32, dim3(nb((int64_t)u->nparam)), dim3(kT), 0, s, u->g, (int64_t)u->nparam);
+    hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(kT), 0, s, u->loss, (int64_t)1);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, u->count);
+    hipLaunchKernelGGL(k_mask_count, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, u->mask, N * N, u->count);
+    hipLaunchKernelGGL(k_head_bwd, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, u->out, u->tgt, u->mask, u->count,
+                       u->t[D0].p, u->w + u->head_w, u->t[D0].g, u->g + u->head_w, u->g + u->head_b, u->loss, N, N);
+    PIC_HIP(hipGetLastError());
+    auto up_bwd = [&](int from, int fo, int to, int c) -> int {
+        const Tensor& a = u->t[from];
+        const Tensor& b = u->t[to];
+        const int64_t n = (int64_t)b.H * b.W * c;
+        hipLaunchKernelGGL(k_up2_bwd, dim3(nb(n)), dim3(kT), 0, s, a.g, a.C, fo, b.g, b.C, 0, c, b.H, b.W);
+        PIC_HIP(hipGetLastError());
+        return PIC_OK;
+    };
+    auto pool_bwd = [&](int from, int to, int to_o, int c) -> int {
+        const Tensor& a = u->t[from];
+        const Tensor& b = u->t[to];
+        const int64_t n = (int64_t)b.H * b.W * c;
+        hipLaunchKernelGGL(k_pool2_bwd, dim3(nb(n)), dim3(kT), 0, s, a.g, a.C, 0, b.g, b.C, to_o, c, b.H, b.W);
+        PIC_HIP(hipGetLastError());
+        return PIC_OK;
+    };
+    PIC_TRY(conv_bwd(u, 13));
+    PIC_TRY(conv_bwd(u, 12));
+    PIC_TRY(up_bwd(CAT0, 0, D1, 32));
+    PIC_TRY(conv_bwd(u, 11));
+    PIC_TRY(conv_bwd(u, 10));
+    PIC_TRY(up_bwd(CAT1, 0, D2, 64));
+    PIC_TRY(conv_bwd(u, 9));
+    PIC_TRY(conv_bwd(u, 8));
+    PIC_TRY(up_bwd(CAT2, 0, B3, 64));
+    PIC_TRY(conv_bwd(u, 7));
+    PIC_TRY(conv_bwd(u, 6));
+    PIC_TRY(pool_bwd(P2, CAT2, 64, 64));
+    PIC_TRY(conv_bwd(u, 5));
+    PIC_TRY(conv_bwd(u, 4));
+    PIC_TRY(pool_bwd(P1, CAT1, 64, 32));
+    PIC_TRY(conv_bwd(u, 3));
+    PIC_TRY(conv_bwd(u, 2));
+    PIC_TRY(pool_bwd(P0, CAT0, 32, 16));
+    PIC_TRY(conv_bwd(u, 1));
+    PIC_TRY(conv_bwd(u, 0));
+    return PIC_OK;
+}
+
+// BF16 and flipped copies of the conv weights from the FP32 masters.
+__global__ void k_refresh_bf16(const float* __restrict__ w, int n, bf16* __restrict__ wb, bf16* __restrict__ wflip,
+                               int cout, int cin) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const bf16 b = f2bf(w[k]);
+    wb[k] = b;
+    const int ci = k % cin, r = k / cin, co = r % cout, tap = r / cout;
+    wflip[((size_t)(8 - tap) * cin + ci) * cout + co] = b;
+}
+
+static int refresh_copies(pic_unet* u) {
+    for (const Conv& c : u->conv) {
+        const int n = 9 * c.cout * c.cin;
+        hipLaunchKernelGGL(k_refresh_bf16, dim3(nb(n)), dim3(kT), 0, u->stream, u->w + c.woff, n, u->wb + c.woff,
+                           u->wflip + c.woff, c.cout, c.cin);
+    }
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+static int adam_step(pic_unet* u, float lr) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    u->step += 1;
+    const float lr_t = lr * std::sqrt(1.f - std::pow(b2, (float)u->step)) / (1.f - std::pow(b1, (float)u->step));
+    for (const Conv& c : u->conv) {
+        const int n = 9 * c.cout * c.cin;
+        hipLaunchKernelGGL(k_adam, dim3(nb(n)), dim3(kT), 0, u->stream, u->w + c.woff, u->g + c.woff, u->m + c.woff,
+                           u->v + c.woff, n, lr_t, b1, b2, eps, u->wb + c.woff, u->wflip + c.woff, c.cout, c.cin);
+        hipLaunchKernelGGL(k_adam, dim3(nb(c.cout)), dim3(kT), 0, u->stream, u->w + c.boff, u->g + c.boff,
+                           u->m + c.boff, u->v + c.boff, c.cout, lr_t, b1, b2, eps, (bf16*)nullptr, (bf16*)nullptr, 1,
+                           1);
+    }
+    hipLaunchKernelGGL(k_adam, dim3(1), dim3(kT), 0, u->stream, u->w + u->head_w, u->g + u->head_w, u->m + u->head_w,
+                       u->v + u->head_w, 17, lr_t, b1, b2, eps, (bf16*)nullptr, (bf16*)nullptr, 1, 1);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+// deterministic initialisation (He normal for convs, small head)
+static uint64_t sm64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static double unif(uint64_t& s) { return ((sm64(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+static double gauss(uint64_t& s) {
+    double a = unif(s), b = unif(s);
+    return std::sqrt(-2.0 * std::log(a)) * std::cos(6.283185307179586 * b);
+}
+
+} // namespace pic
+
+using namespace pic;
+
+extern "C" {
+
+int pic_unet_mfma_probe(const float* a, const float* b, float* d) {
+    if (!a || !b || !d) {
+        set_error("pic_unet_mfma_probe: null argument");
+        return PIC_ERR_ARG;
+    }
+    float *da, *db, *dd;
+    PIC_HIP(hipMalloc(&da, 256 * sizeof(float)));
+    PIC_HIP(hipMalloc(&db, 256 * sizeof(float)));
+    PIC_HIP(hipMalloc(&dd, 256 * sizeof(float)));
+    PIC_HIP(hipMemcpy(da, a, 256 * sizeof(float), hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(db, b, 256 * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, da, db, dd);
+    PIC_HIP(hipGetLastError());
+    PIC_HIP(hipMemcpy(d, dd, 256 * sizeof(float), hipMemcpyDeviceToHost));
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dd);
+    return PIC_OK;
+}
+
+int pic_unet_create(int n, uint64_t seed, pic_unet** out) {
+    if (!out || n < 64 || n % 64 != 0) {
+        set_error("pic_unet_create: image size must be a positive multiple of 64 (got %d)", n);
+        return PIC_ERR_ARG;
+    }
+    int dev = 0;
+    if (hipGetDeviceCount(&dev) != hipSuccess || dev < 1) {
+        set_error("pic_unet_create: no HIP device");
+        return PIC_ERR_HIP;
+    }
+    pic_unet* u = new pic_unet();
+    u->N = n;
+    unet_build(u);
+    auto cleanup = [&](int rc) {
+        pic_unet_destroy(u);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(unet_fail(u, "stream"));
+    u->own_stream = true;
+    size_t maxz = 0, maxx = 0;
+    for (Tensor& x : u->t) {
+        const size_t sz = tsize(x);
+        if (hipMalloc(&x.p, sz * sizeof(bf16)) != hipSuccess) return cleanup(unet_fail(u, "activation allocation"));
+        if (hipMalloc(&x.g, sz * sizeof(float)) != hipSuccess) return cleanup(unet_fail(u, "gradient allocation"));
+        if (hipMemset(x.p, 0, sz * sizeof(bf16)) != hipSuccess) return cleanup(unet_fail(u, "memset"));
+        if (hipMemset(x.g, 0, sz * sizeof(float)) != hipSuccess) return cleanup(unet_fail(u, "memset"));
+    }
+    for (const Conv& c : u->conv) {
+        const Tensor& a = u->t[c.in];
+        maxz = std::max(maxz, (size_t)(a.H + 2) * (a.W + 2) * c.cout);
+        maxx = std::max(maxx, (size_t)(a.H + 2) * (a.W + 2) * c.cin);
+    }
+    const size_t P = u->nparam;
+    bool ok = hipMalloc(&u->w, P * 4) == hipSuccess && hipMalloc(&u->g, P * 4) == hipSuccess &&
+              hipMalloc(&u->m, P * 4) == hipSuccess && hipMalloc(&u->v, P * 4) == hipSuccess &&
+              hipMalloc(&u->wb, P * 2) == hipSuccess && hipMalloc(&u->wflip, P * 2) == hipSuccess &&
+              hipMalloc(&u->out, (size_t)n * n * 4) == hipSuccess && hipMalloc(&u->tgt, (size_t)n * n * 4) == hipSuccess &&
+              hipMalloc(&u->mask, (size_t)n * n) == hipSuccess && hipMalloc(&u->loss, 4) == hipSuccess &&
+              hipMalloc(&u->count, 4) == hipSuccess &&
+              hipMalloc(&u->dz, maxz * 2) == hipSuccess && hipMalloc(&u->dzt, maxz * 2) == hipSuccess &&
+              hipMalloc(&u->xt, maxx * 2) == hipSuccess &&
+              hipHostMalloc((void**)&u->loss_host, 4, hipHostMallocDefault) == hipSuccess;
+    if (!ok) return cleanup(unet_fail(u, "parameter allocation"));
+    (void)hipMemset(u->dz, 0, maxz * 2);
+    (void)hipMemset(u->m, 0, P * 4);
+    (void)hipMemset(u->v, 0, P * 4);
+    (void)hipMemset(u->g, 0, P * 4);
+    std::vector<float> w(P, 0.f);
+    uint64_t s = seed * 0x2545F4914F6CDD1Dull + 1;
+    for (const Conv& c : u->conv) {
+        const double std_ = std::sqrt(2.0 / (9.0 * c.cin_real));
+        for (int tap = 0; tap < 9; ++tap)
+            for (int co = 0; co < c.cout; ++co)
+                for (int ci = 0; ci < c.cin; ++ci)
+                    w[c.woff + ((size_t)tap * c.cout + co) * c.cin + ci] = ci < c.cin_real ? (float)(std_ * gauss(s)) : 0.f;
+    }
+    for (int k = 0; k < 16; ++k) w[u->head_w + k] = (float)(1e-3 * gauss(s));
+    if (hipMemcpy(u->w, w.data(), P * 4, hipMemcpyHostToDevice) != hipSuccess) return cleanup(unet_fail(u, "copy"));
+    if (refresh_copies(u) != PIC_OK || hipStreamSynchronize(u->stream) != hipSuccess)
+        return cleanup(unet_fail(u, "weight copies"));
+    *out = u;
+    return PIC_OK;
+}
+
+void pic_unet_destroy(pic_unet* u) {
+    if (!u) return;
+    if (u->stream) (void)hipStreamSynchronize(u->stream);
+    for (Tensor& x : u->t) {
+        if (x.p) (void)hipFree(x.p);
+        if (x.g) (void)hipFree(x.g);
+    }
+    void* ptrs[] = {u->w, u->g, u->m, u->v, u->wb, u->wflip, u->out, u->tgt, u->mask, u->loss, u->count, u->dz,
+                    u->dzt, u->xt};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (u->loss_host) (void)hipHostFree(u->loss_host);
+    if (u->own_stream && u->stream) (void)hipStreamDestroy(u->stream);
+    delete u;
+}
+
+int64_t pic_unet_num_params(pic_unet* u) { return u ? (int64_t)u->nparam : 0; }
+
+int pic_unet_get_params(pic_unet* u, float* w) {
+    if (!u || !w) {
+        set_error("pic_unet_get_params: null argument");
+        return PIC_ERR_ARG;
+    }
+    PIC_HIP(hipStreamSynchronize(u->stream));
+    PIC_HIP(hipMemcpy(w, u->w, u->nparam * 4, hipMemcpyDeviceToHost));
+    return PIC_OK;
+}
+
+int pic_unet_set_params(pic_unet* u, const float* w) {
+    if (!u || !w) {
+        set_error("pic_unet_set_params: null argument");
+        return PIC_ERR_ARG;
+    }
+    PIC_HIP(hipStreamSynchronize(u->stream));
+    PIC_HIP(hipMemcpy(u->w, w, u->nparam * 4, hipMemcpyHostToDevice));
+    PIC_TRY(refresh_copies(u));
+    PIC_HIP(hipStreamSynchronize(u->stream));
+    return PIC_OK;
+}
+
+int pic_unet_get_grads(pic_unet* u, float* g) {
+    if (!u || !g) {
+        set_error("pic_unet_get_grads: null argument");
+        return PIC_ERR_ARG;
+    }
+    PIC_HIP(hipStreamSynchronize(u->stream));
+    PIC_HIP(hipMemcpy(g, u->g, u->nparam * 4, hipMemcpyDeviceToHost));
+    return PIC_OK;
+}
+
+// x: host [N][N][2] inputs (already scaled); out: host [N][N]
+int pic_unet_forward(pic_unet* u, const float* x, float* out) {
+    if (!u || !x || !out) {
+        set_error("pic_unet_forward: null argument");
+        return PIC_ERR_ARG;
+    }
+    const int N = u->N;
+    std::vector<bf16> h((size_t)(N + 2) * (N + 2) * 16, 0);
+    for (int y = 0; y < N; ++y)
+        for (int xx = 0; xx < N; ++xx)
+            for (int c = 0; c < 2; ++c)
+                h[((size_t)(y + 1) * (N + 2) + xx + 1) * 16 + c] = f2bf_host(x[((size_t)y * N + xx) * 2 + c]);
+    PIC_HIP(hipMemcpyAsync(u->t[X0].p, h.data(), h.size() * 2, hipMemcpyHostToDevice, u->stream));
+    PIC_TRY(unet_forward(u));
+    PIC_HIP(hipMemcpyAsync(out, u->out, (size_t)N * N * 4, hipMemcpyDeviceToHost, u->stream));
+    PIC_HIP(hipStreamSynchronize(u->stream));
+    return PIC_OK;
+}
+
+// One training step on host data: forward, backward, and (lr > 0) Adam.
+// tgt: [N][N]; mask: [N][N] (1 = counted).  *loss receives the loss before the update.
+int pic_unet_train_step(pic_unet* u, const float* x, const float* tgt, const uint8_t* mask, float lr, float* loss) {
+    if (!u || !x || !tgt || !mask) {
+        set_error("pic_unet_train_step: null argument");
+        return PIC_ERR_ARG;
+    }
+    const int N = u->N;
+    std::vector<float> o((size_t)N * N);
+    PIC_TRY(pic_unet_forward(u, x, o.data()));
+    PIC_HIP(hipMemcpyAsync(u->tgt, tgt, (size_t)N * N * 4, hipMemcpyHostToDevice, u->stream));
+    PIC_HIP(hipMemcpyAsync(u->mask, mask, (size_t)N * N, hipMemcpyHostToDevice, u->stream));
+    PIC_TRY(unet_backward(u));
+    PIC_HIP(hipMemcpyAsync(u->loss_host, u->loss, 4, hipMemcpyDeviceToHost, u->stream));
+    if (lr > 0.f) PIC_TRY(adam_step(u, lr));
+    PIC_HIP(hipStreamSynchronize(u->stream));
+    if (loss) *loss = *u->loss_host;
+    return PIC_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------- use inside the solver
+namespace pic {
+
+int unet_guess(pic_ctx* c, pic_unet* u, double* guess) {
+    const int N = u->N, gx = c->g.gx;
+    if (u->s_rho == 0) return PIC_ERR_STATE;
+    hipStream_t s = c->stream;
+    hipLaunchKernelGGL(k_unet_input, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, c->rho, c->phi, gx, N, u->s_rho,
+                       u->s_phi, u->t[X0].p);
+    PIC_HIP(hipGetLastError());
+    hipStream_t keep = u->stream;
+    u->stream = s;
+    const int rc = unet_forward(u);
+    u->stream = keep;
+    PIC_TRY(rc);
+    hipLaunchKernelGGL(k_unet_guess, dim3(nb(c->G)), dim3(kT), 0, s, c->phi, u->out, c->dir, gx, c->g.gy, N,
+                       1.0 / u->s_delta, guess);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+int unet_train_pair(pic_ctx* c, pic_unet* u, const double* phi_old, float lr, float* loss) {
+    const int N = u->N, gx = c->g.gx;
+    hipStream_t s = c->stream;
+    if (u->s_rho == 0) {
+        // scales from the first pair: 1 / max |rho|, 1 / max |phi|, 1 / rms(delta phi)
+        std::vector<double> r(c->G), p(c->G), q(c->G);
+        PIC_HIP(hipMemcpyAsync(r.data(), c->rho, c->G * 8, hipMemcpyDeviceToHost, s));
+        PIC_HIP(hipMemcpyAsync(p.data(), phi_old, c->G * 8, hipMemcpyDeviceToHost, s));
+        PIC_HIP(hipMemcpyAsync(q.data(), c->phi, c->G * 8, hipMemcpyDeviceToHost, s));
+        PIC_HIP(hipStreamSynchronize(s));
+        double mr = 0, mp = 0, sd = 0;
+        for (int k = 0; k < c->G; ++k) {
+            mr = std::max(mr, std::fabs(r[k]));
+            mp = std::max(mp, std::fabs(p[k]));
+            sd += (q[k] - p[k]) * (q[k] - p[k]);
+        }
+        sd = std::sqrt(sd / c->G);
+        u->s_rho = mr > 0 ? 1.0 / mr : 1.0;
+        u->s_phi = mp > 0 ? 1.0 / mp : 1.0;
+        u->s_delta = sd > 0 ? 1.0 / sd : 1.0;
+    }
+    hipLaunchKernelGGL(k_unet_input, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, c->rho, phi_old, gx, N, u->s_rho,
+                       u->s_phi, u->t[X0].p);
+    hipLaunchKernelGGL(k_unet_target, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, c->phi, phi_old, c->dir, gx, N,
+                       u->s_delta, u->tgt, u->mask);
+    PIC_HIP(hipGetLastError());
+    hipStream_t keep = u->stream;
+    u->stream = s;
+    int rc = unet_forward(u);
+    if (rc == PIC_OK) rc = unet_backward(u);
+    if (rc == PIC_OK) rc = adam_step(u, lr);
+    u->stream = keep;
+    PIC_TRY(rc);
+    if (loss) {
+        PIC_HIP(hipMemcpyAsync(u->loss_host, u->loss, 4, hipMemcpyDeviceToHost, s));
+        PIC_HIP(hipStreamSynchronize(s));
+        *loss = *u->loss_host;
+    }
+    return PIC_OK;
+}
+
+} // namespace pic

@@ -112,6 +112,17 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "pic_stage_times": (i32, [vp, dp, lp]),
         "pic_grid_shape": (i32, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "pic_last_error": (C.c_char_p, []),
+        "pic_unet_create": (i32, [i32, C.c_uint64, C.POINTER(vp)]),
+        "pic_unet_destroy": (None, [vp]),
+        "pic_unet_num_params": (i64, [vp]),
+        "pic_unet_get_params": (i32, [vp, fp]),
+        "pic_unet_set_params": (i32, [vp, fp]),
+        "pic_unet_get_grads": (i32, [vp, fp]),
+        "pic_unet_forward": (i32, [vp, fp, fp]),
+        "pic_unet_train_step": (i32, [vp, fp, fp, C.POINTER(C.c_uint8), C.c_float, fp]),
+        "pic_unet_mfma_probe": (i32, [fp, fp, fp]),
+        "pic_learned_enable": (i32, [vp, i32, i32, C.c_float, C.c_uint64]),
+        "pic_learned_stats": (i32, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), fp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -327,8 +338,83 @@ class Context:
     def set_timing(self, on=True):
         _chk(self.lib.pic_set_timing(self.h, 1 if on else 0))
 
+    def learned_enable(self, train_steps, min_trained=0, lr=1e-3, seed=1):
+        _chk(self.lib.pic_learned_enable(self.h, int(train_steps), int(min_trained), float(lr), int(seed)))
+
+    def learned_stats(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        l = np.zeros(1, np.float32)
+        _chk(self.lib.pic_learned_stats(self.h, C.byref(a), C.byref(b), C.byref(c), _fp(l)))
+        return {"trained": a.value, "tried": b.value, "used": c.value, "loss": float(l[0])}
+
     def stage_times(self):
         ms = np.zeros(len(STAGES), np.float64)
         n = np.zeros(len(STAGES), np.int64)
         _chk(self.lib.pic_stage_times(self.h, _dp(ms), _lp(n)))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(STAGES)}
+
+
+def mfma_probe(a, b):
+    """16x16x16 BF16 MFMA of row-major 16x16 float inputs (exact in BF16)."""
+    lib = load()
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    d = np.empty((16, 16), np.float32)
+    _chk(lib.pic_unet_mfma_probe(_fp(a), _fp(b), _fp(d)))
+    return d
+
+
+class UNet:
+    """The learned-guess network on its own (tests, experiments)."""
+
+    def __init__(self, n: int, seed: int = 1):
+        lib = load()
+        self.lib = lib
+        self.n = n
+        h = C.c_void_p()
+        _chk(lib.pic_unet_create(n, seed, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.pic_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def num_params(self) -> int:
+        return int(self.lib.pic_unet_num_params(self.h))
+
+    def params(self):
+        w = np.empty(self.num_params, np.float32)
+        _chk(self.lib.pic_unet_get_params(self.h, _fp(w)))
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        _chk(self.lib.pic_unet_set_params(self.h, _fp(w)))
+
+    def grads(self):
+        g = np.empty(self.num_params, np.float32)
+        _chk(self.lib.pic_unet_get_grads(self.h, _fp(g)))
+        return g
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.empty((self.n, self.n), np.float32)
+        _chk(self.lib.pic_unet_forward(self.h, _fp(x), _fp(out)))
+        return out
+
+    def train_step(self, x, target, mask, lr):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        m = np.ascontiguousarray(mask, np.uint8)
+        loss = np.zeros(1, np.float32)
+        _chk(self.lib.pic_unet_train_step(self.h, _fp(x), _fp(t), m.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                          float(lr), _fp(loss)))
+        return float(loss[0])

@@ -106,7 +106,8 @@ typedef struct pic_solve_stats {
     double final_relres;      /* same after the solve                              */
     double ms;                /* device time of the solve (HIP events)             */
     int    guess;             /* 0 zero guess, 1 previous phi, 2 caller-set guess,
-                                 3 extrapolated from the last two solves           */
+                                 3 extrapolated from the last two solves,
+                                 4 learned (U-Net) prediction                      */
 } pic_solve_stats;
 
 /* ====================================================== stage group */
@@ -230,6 +231,43 @@ int  pic_stage_times(pic_ctx* ctx, double* ms, int64_t* launches);
 int  pic_grid_shape(pic_ctx* ctx, int* grid_x, int* grid_y);
 /* Text of the last error (thread-local). */
 const char* pic_last_error(void);
+
+/* ============================================ learned initial guess */
+/* A 4-level U-Net (3x3 convolutions, 16-64 channels, BF16 weights and
+ * activations, FP32 accumulation on MFMA) trained online with Adam on the
+ * solver's own (rho_n, phi_{n-1}) -> phi_n pairs; its prediction is tried as
+ * the PCG initial guess and kept only when its starting residual is lower
+ * than the previous solution's.  No counterpart in the reference, whose ML
+ * path replaces the solver by a surrogate (poisson_ml_bridge.py:55-106); the
+ * PCG here always runs to the stated residual. */
+typedef struct pic_unet pic_unet;
+
+/* Network for an n x n node image (n a multiple of 128); weights from `seed`. */
+int     pic_unet_create(int n, uint64_t seed, pic_unet** out);
+void    pic_unet_destroy(pic_unet* u);
+int64_t pic_unet_num_params(pic_unet* u);
+/* FP32 master parameters, flattened: per convolution W[9][cout][cin] then
+ * b[cout] (14 convolutions), then the head's 16 weights and bias. */
+int     pic_unet_get_params(pic_unet* u, float* w);
+int     pic_unet_set_params(pic_unet* u, const float* w);
+int     pic_unet_get_grads(pic_unet* u, float* g);
+/* x: [n][n][2] scaled inputs; out: [n][n] predicted scaled change. */
+int     pic_unet_forward(pic_unet* u, const float* x, float* out);
+/* Forward, backward and (lr > 0) one Adam step; *loss = masked mean squared
+ * error before the step; gradients stay readable by pic_unet_get_grads. */
+int     pic_unet_train_step(pic_unet* u, const float* x, const float* target, const uint8_t* mask, float lr,
+                            float* loss);
+/* 16x16x16 BF16 MFMA on row-major 16x16 inputs (layout check). */
+int     pic_unet_mfma_probe(const float* a, const float* b, float* d);
+
+/* Use a learned guess in pic_poisson_solve: train on the first `train_steps`
+ * solves (Adam, learning rate lr), offer the prediction from solve
+ * `min_trained` on.  The image is the first n x n nodes (n = NX rounded down
+ * to a multiple of 128); the rest of the grid keeps the previous solution. */
+int     pic_learned_enable(pic_ctx* ctx, int train_steps, int min_trained, float lr, uint64_t seed);
+/* Counters: pairs trained on, solves where the prediction was tried, where it
+ * was used (lower starting residual), and the last training loss. */
+int     pic_learned_stats(pic_ctx* ctx, int* trained, int* tried, int* used, float* last_loss);
 
 /* =================================================== drop-in group */
 /* Same names/signatures/semantics as SYCL-PIC-poissonsolver/src/sycl_pic_ctypes.cpp.

@@ -1,0 +1,213 @@
+"""NumPy restatement of the learned-guess U-Net (bmp-6th-sem_amd/csrc/pic_unet.hip).
+
+TEST INFRASTRUCTURE ONLY: used by tests/ as the checker of the GPU network's
+forward pass, gradients and training step.  The reference has no such network
+(its ML path, SYCL-PIC-poissonsolver/src/poisson_ml_bridge.py, calls a
+surrogate whose weights are not in the repository), so this file restates the
+network this build defines:
+
+  x0 (2 channels, padded to 16) -> conv 16 -> conv 16 = s0
+  pool -> conv 32 -> conv 32 = s1 ; pool -> conv 64 -> conv 64 = s2
+  pool -> conv 64 -> conv 64 = b3
+  up(b3) | s2 -> conv 64 -> conv 64 = d2 ; up(d2) | s1 -> conv 32 -> conv 32 = d1
+  up(d1) | s0 -> conv 16 -> conv 16 = d0 ; head: 1x1 conv 16 -> 1
+every 3x3 convolution with bias and ReLU, 2x2 average pooling, nearest
+upsampling, concatenation as [upsampled, skip].
+
+With `bf16=True` values are rounded to BF16 (round to nearest even) exactly
+where the GPU stores BF16: inputs, conv weights, conv outputs, pooled values,
+and the ReLU-masked output gradients fed to the weight and input gradients.
+Sums are FP32 on the GPU and FP64 here.  With `bf16=False` everything is FP64
+(used for finite-difference checks of the backward pass).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# (input tensor, first input channel, input channels, real input channels,
+#  output tensor, first output channel, output channels), in parameter order
+CONVS = [
+    ("x0", 0, 16, 2, "a0", 0, 16),
+    ("a0", 0, 16, 16, "cat0", 32, 16),
+    ("p0", 0, 16, 16, "a1", 0, 32),
+    ("a1", 0, 32, 32, "cat1", 64, 32),
+    ("p1", 0, 32, 32, "a2", 0, 64),
+    ("a2", 0, 64, 64, "cat2", 64, 64),
+    ("p2", 0, 64, 64, "a3", 0, 64),
+    ("a3", 0, 64, 64, "b3", 0, 64),
+    ("cat2", 0, 128, 128, "a4", 0, 64),
+    ("a4", 0, 64, 64, "d2", 0, 64),
+    ("cat1", 0, 96, 96, "a5", 0, 32),
+    ("a5", 0, 32, 32, "d1", 0, 32),
+    ("cat0", 0, 48, 48, "a6", 0, 16),
+    ("a6", 0, 16, 16, "d0", 0, 16),
+]
+# tensors: name -> (level, channels)
+TENSORS = {"x0": (0, 16), "a0": (0, 16), "cat0": (0, 48), "p0": (1, 16), "a1": (1, 32), "cat1": (1, 96),
+           "p1": (2, 32), "a2": (2, 64), "cat2": (2, 128), "p2": (3, 64), "a3": (3, 64), "b3": (3, 64),
+           "a4": (2, 64), "d2": (2, 64), "a5": (1, 32), "d1": (1, 32), "a6": (0, 16), "d0": (0, 16)}
+
+
+def bf16(a):
+    """Round float32 values to BF16 (nearest, ties to even), returned as float64."""
+    f = np.asarray(a, np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def layout():
+    """Offsets of every parameter block in the flattened vector."""
+    off = 0
+    blocks = []
+    for (_, _, cin, _, _, _, cout) in CONVS:
+        w = off
+        off += 9 * cout * cin
+        b = off
+        off += cout
+        blocks.append((w, b, cin, cout))
+    head_w = off
+    off += 16
+    head_b = off
+    off += 1
+    return blocks, head_w, head_b, off
+
+
+def unpack(params):
+    blocks, hw, hb, n = layout()
+    assert params.size == n
+    ws = [(params[w:w + 9 * co * ci].reshape(9, co, ci), params[b:b + co]) for (w, b, ci, co) in blocks]
+    return ws, params[hw:hw + 16], params[hb]
+
+
+def conv3x3(x, w, b):
+    """y[y, x, co] = b[co] + sum_{tap, ci} w[tap, co, ci] xpad[y + dy, x + dx, ci]; x: (H, W, C)."""
+    H, W, C = x.shape
+    xp = np.zeros((H + 2, W + 2, C))
+    xp[1:-1, 1:-1] = x
+    out = np.zeros((H, W, w.shape[1]))
+    for tap in range(9):
+        dy, dx = tap // 3 - 1, tap % 3 - 1
+        out += xp[1 + dy:1 + dy + H, 1 + dx:1 + dx + W] @ w[tap].T
+    return out + b
+
+
+def conv3x3_dx(dz, w):
+    """Input gradient of conv3x3 (no bias): dx[p, ci] = sum_tap w[tap, :, ci] . dz[p - off(tap)]."""
+    H, W, _ = dz.shape
+    dp = np.zeros((H + 2, W + 2, dz.shape[2]))
+    dp[1:-1, 1:-1] = dz
+    out = np.zeros((H, W, w.shape[2]))
+    for tap in range(9):
+        dy, dx = tap // 3 - 1, tap % 3 - 1
+        out += dp[1 - dy:1 - dy + H, 1 - dx:1 - dx + W] @ w[tap]
+    return out
+
+
+def conv3x3_dw(x, dz):
+    """Weight gradient: dw[tap, co, ci] = sum_p dz[p, co] xpad[p + off(tap), ci]."""
+    H, W, C = x.shape
+    xp = np.zeros((H + 2, W + 2, C))
+    xp[1:-1, 1:-1] = x
+    out = np.zeros((9, dz.shape[2], C))
+    dzf = dz.reshape(-1, dz.shape[2])
+    for tap in range(9):
+        dy, dx = tap // 3 - 1, tap % 3 - 1
+        out[tap] = dzf.T @ xp[1 + dy:1 + dy + H, 1 + dx:1 + dx + W].reshape(-1, C)
+    return out
+
+
+def pool2(x):
+    H, W, C = x.shape
+    return 0.25 * ((x[0::2, 0::2] + x[0::2, 1::2]) + (x[1::2, 0::2] + x[1::2, 1::2]))
+
+
+def up2(x):
+    return np.repeat(np.repeat(x, 2, axis=0), 2, axis=1)
+
+
+def forward(params, x, bf=True):
+    """x: (N, N, 2) scaled inputs (N a multiple of 8).  Returns (output (N, N), activations)."""
+    N = x.shape[0]
+    rnd = bf16 if bf else (lambda v: np.asarray(v, np.float64))
+    ws, wh, bh = unpack(np.asarray(params, np.float64))
+    wsb = [(rnd(w), b) for (w, b) in ws]
+    act = {}
+    for name, (lvl, c) in TENSORS.items():
+        act[name] = np.zeros((N >> lvl, N >> lvl, c))
+    act["x0"][:, :, :2] = rnd(x)
+
+    def conv(k):
+        src, io, ci, _, dst, oo, co = CONVS[k]
+        w, b = wsb[k]
+        y = conv3x3(act[src][:, :, io:io + ci], w, b)
+        act[dst][:, :, oo:oo + co] = rnd(np.maximum(y, 0.0))
+
+    conv(0); conv(1)
+    act["p0"][:] = rnd(pool2(act["cat0"][:, :, 32:48]))
+    conv(2); conv(3)
+    act["p1"][:] = rnd(pool2(act["cat1"][:, :, 64:96]))
+    conv(4); conv(5)
+    act["p2"][:] = rnd(pool2(act["cat2"][:, :, 64:128]))
+    conv(6); conv(7)
+    act["cat2"][:, :, 0:64] = up2(act["b3"])
+    conv(8); conv(9)
+    act["cat1"][:, :, 0:64] = up2(act["d2"])
+    conv(10); conv(11)
+    act["cat0"][:, :, 0:32] = up2(act["d1"])
+    conv(12); conv(13)
+    out = act["d0"] @ wh + bh
+    return out, act
+
+
+def loss_and_grads(params, x, target, mask, bf=True):
+    """Masked mean squared error of forward(x) against target and its gradient
+    with respect to the flattened parameters."""
+    rnd = bf16 if bf else (lambda v: np.asarray(v, np.float64))
+    out, act = forward(params, x, bf)
+    ws, wh, bh = unpack(np.asarray(params, np.float64))
+    wsb = [rnd(w) for (w, _) in ws]
+    cnt = max(int(mask.sum()), 1)
+    d = (out - target) * mask
+    loss = float((d * d).sum() / cnt)
+    g_out = 2.0 * d / cnt
+    blocks, hw, hb, n = layout()
+    grads = np.zeros(n)
+    grads[hw:hw + 16] = np.einsum("yx,yxc->c", g_out, act["d0"])
+    grads[hb] = g_out.sum()
+    gact = {name: np.zeros_like(a) for name, a in act.items()}
+    gact["d0"][:] = g_out[:, :, None] * wh[None, None, :]
+
+    def conv_bwd(k):
+        src, io, ci, _, dst, oo, co = CONVS[k]
+        y = act[dst][:, :, oo:oo + co]
+        dz = rnd(gact[dst][:, :, oo:oo + co] * (y > 0))
+        w_off, b_off, _, _ = blocks[k]
+        grads[w_off:w_off + 9 * co * ci] = conv3x3_dw(act[src][:, :, io:io + ci], dz).ravel()
+        grads[b_off:b_off + co] = dz.sum(axis=(0, 1))
+        if k > 0:
+            gact[src][:, :, io:io + ci] = conv3x3_dx(dz, wsb[k])
+
+    def up_bwd(src, so, dst, c):
+        g = gact[src][:, :, so:so + c]
+        gact[dst][:] = (g[0::2, 0::2] + g[0::2, 1::2]) + (g[1::2, 0::2] + g[1::2, 1::2])
+
+    def pool_bwd(src, dst, do, c):
+        gact[dst][:, :, do:do + c] += 0.25 * up2(gact[src])
+
+    conv_bwd(13); conv_bwd(12); up_bwd("cat0", 0, "d1", 32)
+    conv_bwd(11); conv_bwd(10); up_bwd("cat1", 0, "d2", 64)
+    conv_bwd(9); conv_bwd(8); up_bwd("cat2", 0, "b3", 64)
+    conv_bwd(7); conv_bwd(6); pool_bwd("p2", "cat2", 64, 64)
+    conv_bwd(5); conv_bwd(4); pool_bwd("p1", "cat1", 64, 32)
+    conv_bwd(3); conv_bwd(2); pool_bwd("p0", "cat0", 32, 16)
+    conv_bwd(1); conv_bwd(0)
+    return loss, grads
+
+
+def adam(params, grads, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step (bias-corrected step size), in place on float32 arrays."""
+    m[:] = b1 * m + (1 - b1) * grads
+    v[:] = b2 * v + (1 - b2) * grads * grads
+    lr_t = lr * np.sqrt(1 - b2 ** t) / (1 - b1 ** t)
+    params[:] = params - lr_t * m / (np.sqrt(v) + eps)

@@ -1,0 +1,134 @@
+"""The learned-guess network on the GPU (hand-written BF16 MFMA kernels) against
+the NumPy oracle (oracle/unet_oracle.py): MFMA lane layout, forward pass,
+gradients, Adam step, training, and its use as the Poisson initial guess.
+
+Tolerances: the GPU sums in FP32 in another order than the oracle's FP64 and
+both round to BF16 at the same points, so an activation can differ by one BF16
+unit where a sum lands near a rounding tie; outputs are compared normwise to
+2e-2, gradients per parameter block normwise to 5e-2."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 128
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _inputs(seed=0):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:N, 0:N] / N
+    x = np.stack([np.sin(6 * xx) * np.cos(4 * yy) + 0.3 * rng.normal(size=(N, N)),
+                  np.cos(3 * xx + yy) + 0.1 * rng.normal(size=(N, N))], axis=-1)
+    return x.astype(np.float32)
+
+
+def _params(net, seed=0):
+    import unet_oracle as U
+    p = net.params().astype(np.float64)
+    _, hw, hb, _ = U.layout()
+    rng = np.random.default_rng(seed)
+    p[hw:hw + 16] = rng.normal(0, 0.3, 16)        # a head large enough to pass gradients
+    p[hb] = 0.1
+    net.set_params(p.astype(np.float32))
+    return net.params().astype(np.float64)
+
+
+def test_mfma_lane_layout(hip_mod):
+    rng = np.random.default_rng(1)
+    a = rng.integers(-8, 8, (16, 16)).astype(np.float32)
+    b = rng.integers(-8, 8, (16, 16)).astype(np.float32)
+    d = hip_mod.mfma_probe(a, b)
+    assert np.array_equal(d, a @ b)
+
+
+def test_unet_forward_matches_oracle(hip_mod):
+    import unet_oracle as U
+    net = hip_mod.UNet(N, seed=7)
+    p = _params(net, 1)
+    x = _inputs(1)
+    out = net.forward(x)
+    ref, _ = U.forward(p, x.astype(np.float64))
+    assert _rel(out, ref) < 2e-2
+    net.close()
+
+
+def test_unet_gradients_match_oracle(hip_mod):
+    import unet_oracle as U
+    net = hip_mod.UNet(N, seed=8)
+    p = _params(net, 2)
+    x = _inputs(2)
+    rng = np.random.default_rng(3)
+    tgt = (0.5 * x[:, :, 0] + 0.2 * rng.normal(size=(N, N))).astype(np.float32)
+    mask = np.ones((N, N), np.uint8)
+    mask[0, :] = mask[:, 0] = 0
+    loss = net.train_step(x, tgt, mask, 0.0)
+    g = net.grads().astype(np.float64)
+    rloss, rg = U.loss_and_grads(p, x.astype(np.float64), tgt.astype(np.float64), mask.astype(np.float64))
+    assert loss == pytest.approx(rloss, rel=1e-2)
+    blocks, hw, hb, n = U.layout()
+    errs = []
+    for k, (w, b, ci, co) in enumerate(blocks):
+        errs.append((k, _rel(g[w:w + 9 * co * ci], rg[w:w + 9 * co * ci]), _rel(g[b:b + co], rg[b:b + co])))
+    errs.append(("head", _rel(g[hw:hw + 17], rg[hw:hw + 17]), 0.0))
+    print("gradient errors (block, W, b):", errs)
+    assert all(e[1] < 5e-2 and e[2] < 5e-2 for e in errs), errs
+    net.close()
+
+
+def test_unet_adam_step_and_training(hip_mod):
+    import unet_oracle as U
+    net = hip_mod.UNet(N, seed=9)
+    p0 = _params(net, 3).astype(np.float32)
+    x = _inputs(4)
+    tgt = (0.25 * (np.roll(x[:, :, 0], 1, 0) + np.roll(x[:, :, 0], -1, 1)) - 0.3 * x[:, :, 1]).astype(np.float32)
+    mask = np.ones((N, N), np.uint8)
+    # one step against the oracle's Adam on the GPU's own gradients
+    net.train_step(x, tgt, mask, 0.0)
+    g = net.grads()
+    net.set_params(p0)
+    net.train_step(x, tgt, mask, 1e-3)
+    p1 = net.params()
+    pr, m, v = p0.copy(), np.zeros_like(p0), np.zeros_like(p0)
+    U.adam(pr, g, m, v, 1, 1e-3)
+    assert np.abs(p1 - pr).max() <= 1e-6 * max(1.0, np.abs(pr).max())
+    # training lowers the loss
+    losses = [net.train_step(x, tgt, mask, 1e-3) for _ in range(60)]
+    print("loss first/last:", losses[0], losses[-1])
+    assert losses[-1] < 0.5 * losses[0]
+    net.close()
+
+
+def test_learned_guess_in_the_solver(hip_mod):
+    """256 x 256 cells, electrode geometry, tolerance 1e-8: the PCG with the
+    learned guess converges like the plain one and reaches the same potential;
+    the network's guess replaces the previous solution only when its starting
+    residual is lower.  Prints what the trained network gains."""
+    import pic_configs as cfg
+    w = cfg.WORKLOADS["electrode_512"].scaled(NX=256, n=200_000, name="learned_256")
+    parts = [cfg.generate(w, s) for s in range(2)]
+    a = cfg.make_context(w, particles=parts, poisson_tol=1e-8)
+    b = cfg.make_context(w, particles=parts, poisson_tol=1e-8)
+    b.learned_enable(train_steps=40, min_trained=20, lr=1e-3, seed=1)
+    sa, sb = [], []
+    for it in range(60):
+        sa.append(a.step(it))
+        sb.append(b.step(it))
+        pa, pb = a.grid(hip_mod.GRID_PHI), b.grid(hip_mod.GRID_PHI)
+        assert np.abs(pa - pb).max() <= 1e-6 * np.abs(pa).max(), it
+    st = b.learned_stats()
+    print("learned stats:", st)
+    tail = slice(40, 60)
+    ia = np.mean([s["iterations"] for s in sa[tail]])
+    ib = np.mean([s["iterations"] for s in sb[tail]])
+    ra = np.mean([s["initial_relres"] for s in sa[tail]])
+    rb = np.mean([s["initial_relres"] for s in sb[tail]])
+    print(f"iterations plain {ia:.2f} learned {ib:.2f}; initial relres plain {ra:.3e} learned {rb:.3e}")
+    assert st["trained"] == 40 and st["tried"] == 60 - 1 - 20 and 0 <= st["used"] <= st["tried"]
+    assert all(s["final_relres"] <= 1e-8 for s in sb)
+    assert rb <= ra * (1 + 1e-9)       # the guess is used only when it starts lower
+    a.close()
+    b.close()
