@@ -61,6 +61,67 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
 
 
+def unet_flops(n):
+    """Multiply-add FLOPs of one forward pass of the learned-guess network on an
+    n x n image (padded input channels counted: that is the MFMA work done)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import unet_oracle as U
+    f = 0
+    for (src, _, cin, _, _, _, cout) in U.CONVS:
+        lvl = U.TENSORS[src][0]
+        f += 2 * 9 * cin * cout * (n >> lvl) ** 2
+    return f + 2 * 16 * n * n
+
+
+def learned_vs_plain(w, parts, layout, train, evaluate, tol=1e-8):
+    """Poisson solve time at relative residual `tol` with the learned initial
+    guess against the plain previous-solution warm start, on two contexts
+    stepping the same particles: `train` steps of online training (the guess is
+    not offered yet), then `evaluate` steps with it offered."""
+    import numpy as np
+    import pic_configs as cfg
+
+    plain = cfg.make_context(w, layout=layout, particles=parts, poisson_tol=tol)
+    learn = cfg.make_context(w, layout=layout, particles=parts, poisson_tol=tol)
+    learn.learned_enable(train_steps=train, min_trained=train, lr=1e-3, seed=1)
+    t0 = time.perf_counter()
+    sp, sl = [], []
+    for it in range(train + 1 + evaluate):
+        a = plain.step(it)
+        b = learn.step(it)
+        if it > train:
+            sp.append(a)
+            sl.append(b)
+    elapsed = time.perf_counter() - t0
+    st = learn.learned_stats()
+    fwd_ms, train_ms = learn.learned_benchmark(5)
+    n = (min(w.NX, w.NY) // 128) * 128
+    fl = unet_flops(n)
+    out = {
+        "image": n, "tol": tol, "train_steps": train, "eval_steps": evaluate,
+        "poisson_ms_plain": float(np.mean([s["ms"] for s in sp])),
+        "poisson_ms_learned": float(np.mean([s["ms"] for s in sl])),
+        "iterations_plain": float(np.mean([s["iterations"] for s in sp])),
+        "iterations_learned": float(np.mean([s["iterations"] for s in sl])),
+        "initial_relres_plain": float(np.mean([s["initial_relres"] for s in sp])),
+        "initial_relres_learned": float(np.mean([s["initial_relres"] for s in sl])),
+        "guess_used": st["used"], "guess_tried": st["tried"],
+        "fallback_rate": (1 - st["used"] / st["tried"]) if st["tried"] else None,
+        "last_train_loss": st["loss"],
+        "unet_forward_ms": fwd_ms, "unet_train_ms": train_ms,
+        "unet_forward_gflop": fl / 1e9,
+        "unet_mfma_tflops": fl / (fwd_ms * 1e-3) / 1e12 if fwd_ms > 0 else None,
+        "unet_mfma_util": (fl / (fwd_ms * 1e-3)) / 2.5e15 if fwd_ms > 0 else None,
+        "wall_s": elapsed,
+    }
+    out["speedup"] = out["poisson_ms_plain"] / out["poisson_ms_learned"] if out["poisson_ms_learned"] else None
+    plain.close()
+    learn.close()
+    log(f"learned vs plain @ {tol:g}: {out['poisson_ms_learned']:.3f} vs {out['poisson_ms_plain']:.3f} ms, "
+        f"used {st['used']}/{st['tried']}, U-Net forward {fwd_ms:.3f} ms")
+    return out
+
+
 def cpu_baseline(w, parts, steps=3):
     """Oracle (C, OpenMP, FP64 AoS, MKL PARDISO) on the same workload."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -102,6 +163,9 @@ def main():
     ap.add_argument("--workload", default="electrode_1024")
     ap.add_argument("--layout", type=int, default=None, help="0 F64, 1 CELL32 (default: the workload's)")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--learned-train", type=int, default=30,
+                    help="online training steps of the learned-guess comparison (0 skips it)")
+    ap.add_argument("--learned-eval", type=int, default=20)
     ap.add_argument("--guess", choices=("prev", "extrap"), default="prev",
                     help="Poisson initial guess: previous phi, or extrapolated from the last two")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -183,6 +247,14 @@ def main():
             others[KERNEL_NAMES[k]] = {"avg_launch_us": round(ms_k / n_k * 1e3, 2),
                                        "GB_s": round(b / (ms_k / n_k * 1e-3) / 1e9, 1)}
 
+    learned = None
+    if a.learned_train > 0:
+        try:
+            learned = learned_vs_plain(w, parts, layout, a.learned_train, a.learned_eval)
+        except Exception as e:  # reported, the main measurement stands
+            log(f"learned-guess comparison failed: {e}")
+            learned = {"error": str(e)}
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -213,6 +285,7 @@ def main():
                     "initial_relres_mean": float(np.mean([s["initial_relres"] for s in stats]))},
         "roofline": roof,
         "particle_kernels": others,
+        "poisson_learned_vs_plain": learned,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
     }

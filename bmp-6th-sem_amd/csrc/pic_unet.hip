@@ -375,39 +375,83 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
     }
 }
 
-// inputs from the PIC grids: x0[y][x] = (rho s_rho, phi s_phi, 0...)
-__global__ void k_unet_input(const double* __restrict__ rho, const double* __restrict__ phi, int gx, int N,
-                             double s_rho, double s_phi, bf16* __restrict__ x0) {
+// Inputs from the PIC grids, scaled per solve so the network sees O(1) values
+// whatever the amplitude: channel 0 is the starting residual
+// r0 = rhs - A phi_{n-1} (computed in FP64, a function of rho_n and
+// phi_{n-1}) over max|r0|, channel 1 is phi_{n-1} over max|phi_{n-1}|.  The
+// target and the output are the change of phi in units of max|r0| / diag,
+// diag = 2(1/dx^2 + 1/dy^2), so the map the network learns does not depend on
+// the solve's amplitude (the change is linear in r0).
+// scl[0] = max|r0| (as ordered bits), scl[1] = max|phi|.
+__global__ __launch_bounds__(kT) void k_unet_resid(const double* __restrict__ rhs, const double* __restrict__ phi,
+                                                   const uint8_t* __restrict__ dir, int gx, int N, double cx,
+                                                   double cy, double diag, double* __restrict__ r0,
+                                                   unsigned long long* __restrict__ scl) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0, a = 0.0;
+    if (p < N * N) {
+        const int y = p / N, x = p - y * N;
+        const int n = y * gx + x;
+        if (!dir[n])
+            v = rhs[n] - (diag * phi[n] - cx * (phi[n - 1] + phi[n + 1]) - cy * (phi[n - gx] + phi[n + gx]));
+        r0[p] = v;
+        a = fabs(phi[n]);
+    }
+    double m = fabs(v);
+    for (int off = 32; off > 0; off >>= 1) {
+        m = fmax(m, __shfl_down(m, off, 64));
+        a = fmax(a, __shfl_down(a, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(scl, (unsigned long long)__double_as_longlong(m));
+        atomicMax(scl + 1, (unsigned long long)__double_as_longlong(a));
+    }
+}
+
+__global__ void k_zero_u64x2(unsigned long long* p) {
+    p[0] = 0ull;
+    p[1] = 0ull;
+}
+
+__device__ __forceinline__ double scl_get(const unsigned long long* scl, int k) {
+    const double v = __longlong_as_double((long long)scl[k]);
+    return v > 0.0 ? v : 1.0;
+}
+
+__global__ void k_unet_input(const double* __restrict__ r0, const double* __restrict__ phi, int gx, int N,
+                             const unsigned long long* __restrict__ scl, bf16* __restrict__ x0) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N * N) return;
     const int y = p / N, x = p - y * N;
+    const double sr = 1.0 / scl_get(scl, 0), sp = 1.0 / scl_get(scl, 1);
     bf16* d = x0 + ((size_t)(y + 1) * (N + 2) + x + 1) * 16;
-    d[0] = f2bf((float)(rho[y * gx + x] * s_rho));
-    d[1] = f2bf((float)(phi[y * gx + x] * s_phi));
+    d[0] = f2bf((float)(r0[p] * sr));
+    d[1] = f2bf((float)(phi[y * gx + x] * sp));
 }
 
-// target (phi_new - phi_old) s_delta on the image; mask = not Dirichlet
+// target (phi_new - phi_old) diag / max|r0| on the image; mask = not Dirichlet
 __global__ void k_unet_target(const double* __restrict__ phi_new, const double* __restrict__ phi_old,
-                              const uint8_t* __restrict__ dir, int gx, int N, double s_delta, float* __restrict__ t,
+                              const uint8_t* __restrict__ dir, int gx, int N, double diag,
+                              const unsigned long long* __restrict__ scl, float* __restrict__ t,
                               uint8_t* __restrict__ mask) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N * N) return;
     const int y = p / N, x = p - y * N;
     const int n = y * gx + x;
-    t[p] = (float)((phi_new[n] - phi_old[n]) * s_delta);
+    t[p] = (float)((phi_new[n] - phi_old[n]) * (diag / scl_get(scl, 0)));
     mask[p] = dir[n] ? 0 : 1;
 }
 
-// guess = phi_old + out / s_delta on unknown nodes of the image; the rest of
-// the grid keeps phi_old (Dirichlet values are imposed by the solver).
+// guess = phi_old + out max|r0| / diag on unknown nodes of the image; the rest
+// of the grid keeps phi_old (Dirichlet values are imposed by the solver).
 __global__ void k_unet_guess(const double* __restrict__ phi_old, const float* __restrict__ out,
-                             const uint8_t* __restrict__ dir, int gx, int gy, int N, double inv_s_delta,
-                             double* __restrict__ guess) {
+                             const uint8_t* __restrict__ dir, int gx, int gy, int N, double diag,
+                             const unsigned long long* __restrict__ scl, double* __restrict__ guess) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= gx * gy) return;
     const int y = n / gx, x = n - y * gx;
     double v = phi_old[n];
-    if (x < N && y < N && !dir[n]) v += (double)out[y * N + x] * inv_s_delta;
+    if (x < N && y < N && !dir[n]) v += (double)out[y * N + x] * (scl_get(scl, 0) / diag);
     guess[n] = v;
 }
 
@@ -459,7 +503,8 @@ struct pic_unet {
     int step = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    double s_rho = 0, s_phi = 0, s_delta = 0;   // input / target scales (set from the first pair)
+    double* r0 = nullptr;                 // starting residual on the image (FP64)
+    unsigned long long* scl = nullptr;    // per-solve scales (max|r0|, max|phi|)
 };
 
 namespace pic {
@@ -812,7 +857,8 @@ int pic_unet_create(int n, uint64_t seed, pic_unet** out) {
               hipMalloc(&u->wb, P * 2) == hipSuccess && hipMalloc(&u->wflip, P * 2) == hipSuccess &&
               hipMalloc(&u->out, (size_t)n * n * 4) == hipSuccess && hipMalloc(&u->tgt, (size_t)n * n * 4) == hipSuccess &&
               hipMalloc(&u->mask, (size_t)n * n) == hipSuccess && hipMalloc(&u->loss, 4) == hipSuccess &&
-              hipMalloc(&u->count, 4) == hipSuccess &&
+              hipMalloc(&u->count, 4) == hipSuccess && hipMalloc(&u->r0, (size_t)n * n * 8) == hipSuccess &&
+              hipMalloc(&u->scl, 16) == hipSuccess &&
               hipMalloc(&u->dz, maxz * 2) == hipSuccess && hipMalloc(&u->dzt, maxz * 2) == hipSuccess &&
               hipMalloc(&u->xt, maxx * 2) == hipSuccess &&
               hipHostMalloc((void**)&u->loss_host, 4, hipHostMallocDefault) == hipSuccess;
@@ -846,7 +892,7 @@ void pic_unet_destroy(pic_unet* u) {
         if (x.g) (void)hipFree(x.g);
     }
     void* ptrs[] = {u->w, u->g, u->m, u->v, u->wb, u->wflip, u->out, u->tgt, u->mask, u->loss, u->count, u->dz,
-                    u->dzt, u->xt};
+                    u->dzt, u->xt, u->r0, u->scl};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (u->loss_host) (void)hipHostFree(u->loss_host);
@@ -927,25 +973,73 @@ int pic_unet_train_step(pic_unet* u, const float* x, const float* tgt, const uin
     return PIC_OK;
 }
 
+// Average device time of the forward pass and of a full training step
+// (forward, backward, Adam at lr = 0 so the weights stay) over `reps` runs on
+// the inputs and target currently on the device.
+int pic_unet_benchmark(pic_unet* u, int reps, float* fwd_ms, float* train_ms) {
+    if (!u || reps < 1) {
+        set_error("pic_unet_benchmark: invalid argument");
+        return PIC_ERR_ARG;
+    }
+    hipEvent_t e0, e1, e2;
+    PIC_HIP(hipEventCreate(&e0));
+    PIC_HIP(hipEventCreate(&e1));
+    PIC_HIP(hipEventCreate(&e2));
+    PIC_TRY(unet_forward(u));                      // warm-up
+    PIC_HIP(hipEventRecord(e0, u->stream));
+    for (int k = 0; k < reps; ++k) PIC_TRY(unet_forward(u));
+    PIC_HIP(hipEventRecord(e1, u->stream));
+    for (int k = 0; k < reps; ++k) {
+        PIC_TRY(unet_forward(u));
+        PIC_TRY(unet_backward(u));
+    }
+    PIC_HIP(hipEventRecord(e2, u->stream));
+    PIC_HIP(hipEventSynchronize(e2));
+    float a = 0, b = 0;
+    PIC_HIP(hipEventElapsedTime(&a, e0, e1));
+    PIC_HIP(hipEventElapsedTime(&b, e1, e2));
+    if (fwd_ms) *fwd_ms = a / reps;
+    if (train_ms) *train_ms = b / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    return PIC_OK;
+}
+
+// the context's network (null when the learned guess is off)
+pic_unet* pic_learned_unet(pic_ctx* c) { return c ? c->unet : nullptr; }
+
 } // extern "C"
 
 // ---------------------------------------------------- use inside the solver
 namespace pic {
 
+// r0 and the per-solve scales for (rhs = c->rho after the Dirichlet overwrite, phi_old)
+static int unet_prepare(pic_ctx* c, pic_unet* u, const double* phi_old) {
+    const int N = u->N, gx = c->g.gx;
+    hipStream_t s = c->stream;
+    const double cx = 1.0 / (c->g.dx * c->g.dx), cy = 1.0 / (c->g.dy * c->g.dy);
+    hipLaunchKernelGGL(k_zero_u64x2, dim3(1), dim3(1), 0, s, u->scl);
+    hipLaunchKernelGGL(k_unet_resid, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, c->rho, phi_old, c->dir, gx, N, cx, cy,
+                       2 * (cx + cy), u->r0, u->scl);
+    hipLaunchKernelGGL(k_unet_input, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, u->r0, phi_old, gx, N, u->scl,
+                       u->t[X0].p);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
 int unet_guess(pic_ctx* c, pic_unet* u, double* guess) {
     const int N = u->N, gx = c->g.gx;
-    if (u->s_rho == 0) return PIC_ERR_STATE;
     hipStream_t s = c->stream;
-    hipLaunchKernelGGL(k_unet_input, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, c->rho, c->phi, gx, N, u->s_rho,
-                       u->s_phi, u->t[X0].p);
-    PIC_HIP(hipGetLastError());
+    PIC_TRY(unet_prepare(c, u, c->phi));
     hipStream_t keep = u->stream;
     u->stream = s;
     const int rc = unet_forward(u);
     u->stream = keep;
     PIC_TRY(rc);
-    hipLaunchKernelGGL(k_unet_guess, dim3(nb(c->G)), dim3(kT), 0, s, c->phi, u->out, c->dir, gx, c->g.gy, N,
-                       1.0 / u->s_delta, guess);
+    const double diag = 2.0 / (c->g.dx * c->g.dx) + 2.0 / (c->g.dy * c->g.dy);
+    hipLaunchKernelGGL(k_unet_guess, dim3(nb(c->G)), dim3(kT), 0, s, c->phi, u->out, c->dir, gx, c->g.gy, N, diag,
+                       u->scl, guess);
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
@@ -953,28 +1047,10 @@ int unet_guess(pic_ctx* c, pic_unet* u, double* guess) {
 int unet_train_pair(pic_ctx* c, pic_unet* u, const double* phi_old, float lr, float* loss) {
     const int N = u->N, gx = c->g.gx;
     hipStream_t s = c->stream;
-    if (u->s_rho == 0) {
-        // scales from the first pair: 1 / max |rho|, 1 / max |phi|, 1 / rms(delta phi)
-        std::vector<double> r(c->G), p(c->G), q(c->G);
-        PIC_HIP(hipMemcpyAsync(r.data(), c->rho, c->G * 8, hipMemcpyDeviceToHost, s));
-        PIC_HIP(hipMemcpyAsync(p.data(), phi_old, c->G * 8, hipMemcpyDeviceToHost, s));
-        PIC_HIP(hipMemcpyAsync(q.data(), c->phi, c->G * 8, hipMemcpyDeviceToHost, s));
-        PIC_HIP(hipStreamSynchronize(s));
-        double mr = 0, mp = 0, sd = 0;
-        for (int k = 0; k < c->G; ++k) {
-            mr = std::max(mr, std::fabs(r[k]));
-            mp = std::max(mp, std::fabs(p[k]));
-            sd += (q[k] - p[k]) * (q[k] - p[k]);
-        }
-        sd = std::sqrt(sd / c->G);
-        u->s_rho = mr > 0 ? 1.0 / mr : 1.0;
-        u->s_phi = mp > 0 ? 1.0 / mp : 1.0;
-        u->s_delta = sd > 0 ? 1.0 / sd : 1.0;
-    }
-    hipLaunchKernelGGL(k_unet_input, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, c->rho, phi_old, gx, N, u->s_rho,
-                       u->s_phi, u->t[X0].p);
-    hipLaunchKernelGGL(k_unet_target, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, c->phi, phi_old, c->dir, gx, N,
-                       u->s_delta, u->tgt, u->mask);
+    PIC_TRY(unet_prepare(c, u, phi_old));
+    const double diag = 2.0 / (c->g.dx * c->g.dx) + 2.0 / (c->g.dy * c->g.dy);
+    hipLaunchKernelGGL(k_unet_target, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, c->phi, phi_old, c->dir, gx, N, diag,
+                       u->scl, u->tgt, u->mask);
     PIC_HIP(hipGetLastError());
     hipStream_t keep = u->stream;
     u->stream = s;

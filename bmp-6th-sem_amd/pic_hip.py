@@ -18,6 +18,7 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "lib" / "libbmp_pic.so"
 
 PIC_OK = 0
+PIC_ERR_STATE = -13
 PIC_LAYOUT_F64 = 0
 PIC_LAYOUT_CELL32 = 1
 GRID_RHO, GRID_PHI, GRID_EX, GRID_EY, GRID_BZ = 0, 1, 2, 3, 4
@@ -121,6 +122,8 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "pic_unet_forward": (i32, [vp, fp, fp]),
         "pic_unet_train_step": (i32, [vp, fp, fp, C.POINTER(C.c_uint8), C.c_float, fp]),
         "pic_unet_mfma_probe": (i32, [fp, fp, fp]),
+        "pic_unet_benchmark": (i32, [vp, i32, fp, fp]),
+        "pic_learned_unet": (vp, [vp]),
         "pic_learned_enable": (i32, [vp, i32, i32, C.c_float, C.c_uint64]),
         "pic_learned_stats": (i32, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), fp]),
     }
@@ -340,6 +343,16 @@ class Context:
 
     def learned_enable(self, train_steps, min_trained=0, lr=1e-3, seed=1):
         _chk(self.lib.pic_learned_enable(self.h, int(train_steps), int(min_trained), float(lr), int(seed)))
+
+    def learned_benchmark(self, reps=5):
+        """(forward ms, forward + backward ms) of the context's network."""
+        h = self.lib.pic_learned_unet(self.h)
+        if not h:
+            raise PicError(PIC_ERR_STATE, "learned guess not enabled")
+        a = np.zeros(1, np.float32)
+        b = np.zeros(1, np.float32)
+        _chk(self.lib.pic_unet_benchmark(h, int(reps), _fp(a), _fp(b)))
+        return float(a[0]), float(b[0])
 
     def learned_stats(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()

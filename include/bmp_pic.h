@@ -257,6 +257,9 @@ int     pic_unet_forward(pic_unet* u, const float* x, float* out);
  * error before the step; gradients stay readable by pic_unet_get_grads. */
 int     pic_unet_train_step(pic_unet* u, const float* x, const float* target, const uint8_t* mask, float lr,
                             float* loss);
+/* Average device ms of a forward pass and of forward + backward over `reps`
+ * runs on the data currently on the device (no weight update). */
+int     pic_unet_benchmark(pic_unet* u, int reps, float* fwd_ms, float* train_ms);
 /* 16x16x16 BF16 MFMA on row-major 16x16 inputs (layout check). */
 int     pic_unet_mfma_probe(const float* a, const float* b, float* d);
 
@@ -265,6 +268,8 @@ int     pic_unet_mfma_probe(const float* a, const float* b, float* d);
  * `min_trained` on.  The image is the first n x n nodes (n = NX rounded down
  * to a multiple of 128); the rest of the grid keeps the previous solution. */
 int     pic_learned_enable(pic_ctx* ctx, int train_steps, int min_trained, float lr, uint64_t seed);
+/* The context's network, null when the learned guess is off. */
+pic_unet* pic_learned_unet(pic_ctx* ctx);
 /* Counters: pairs trained on, solves where the prediction was tried, where it
  * was used (lower starting residual), and the last training loss. */
 int     pic_learned_stats(pic_ctx* ctx, int* trained, int* tried, int* used, float* last_loss);
