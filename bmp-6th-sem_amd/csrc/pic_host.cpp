@@ -68,6 +68,8 @@ extern "C" void pic_params_default(pic_params* p) {
     p->mg_post = 2;
     p->mg_omega = 0.8;
     p->poisson_guess = 0;
+    p->poisson_precision = 0;
+    p->poisson_inner_tol = 1e-4;
 }
 
 static void free_species(pic::Species& s) {
@@ -93,7 +95,8 @@ extern "C" void pic_destroy(pic_ctx* c) {
     poisson_free(c);
     if (c->unet) pic_unet_destroy(c->unet);
     void* ptrs[] = {c->rho, c->phi, c->ex, c->ey, c->bz, c->fld, c->dir, c->dval, c->pr, c->pr2, c->pp, c->pp2,
-                    c->pq, c->pb, c->zf0, c->zf1, c->guess, c->phi_prev, c->part, c->scal, c->flags, c->scan_bsum};
+                    c->pq, c->pb, c->zf0, c->zf1, c->guess, c->phi_prev, c->part, c->scal, c->flags, c->scan_bsum,
+                    c->rf0, c->rf1, c->pf0, c->pf1, c->qf, c->df};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c->scal_host) (void)hipHostFree(c->scal_host);

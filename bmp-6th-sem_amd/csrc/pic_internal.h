@@ -132,6 +132,9 @@ struct pic_ctx {
     double *pp = nullptr, *pp2 = nullptr;     // search direction, alternating
     double *pq = nullptr, *pb = nullptr;      // A p; residual of the Dirichlet-only field
     float *zf0 = nullptr, *zf1 = nullptr;     // preconditioned residual z (FP32), alternating
+    // FP32 correction solve of the mixed-precision path (allocated on first use)
+    float *rf0 = nullptr, *rf1 = nullptr, *pf0 = nullptr, *pf1 = nullptr, *qf = nullptr, *df = nullptr;
+    int last_inner = 0;
     double* guess = nullptr;
     double* phi_prev = nullptr;               // phi of the solve before the last (extrapolated guess)
     int guess_set = 0;

@@ -43,7 +43,7 @@ static constexpr int kB = 256;
 static constexpr int kNP = PIC_PART_BLOCKS;  // blocks of the reduction kernels (partials)
 
 // device scalars
-enum Scal { S_RR = 0, S_BB, S_RR0, S_RZ, S_ALPHA, S_BETA, S_G0, S_G1, S_N };
+enum Scal { S_RR = 0, S_BB, S_RR0, S_RZ, S_ALPHA, S_BETA, S_G0, S_G1, S_IRR, S_IBB, S_IRR0, S_N };
 // partial-sum arrays, kNP doubles each
 enum Part { P_RR = 0, P_BB, P_PQ, P_RZ, P_RZO, P_N };
 
@@ -111,8 +111,10 @@ __global__ void k_init_guess(int G, double* x, const double* src, const double* 
 }
 
 __global__ void k_reset_flags(int* flags) {
-    flags[0] = 0;
-    flags[1] = 0;
+    flags[0] = 0;   // outer: converged (1) / failed (2)
+    flags[1] = 0;   // outer: iterations
+    flags[2] = 0;   // inner (mixed precision): converged / failed
+    flags[3] = 0;   // inner: iterations, all corrections together
 }
 
 // r = rhs - A x on unknown nodes (0 on Dirichlet); partial of r.r
@@ -135,21 +137,23 @@ __global__ __launch_bounds__(kB) void k_residual(Fine f, const double* __restric
 }
 
 // rr = sum of the partials; done when rr <= tol^2 bb (flags[0] = 1), NaN -> 2.
-// Outside the first call, one more iteration is recorded unless already done.
+// The first call also records bb (from part_bb, or rr itself when part_bb is
+// null) and rr0; later calls count one more iteration unless already done.
+// sbb, srr, srr0: the scalar slots (outer solve or inner correction solve).
 __global__ __launch_bounds__(1024) void k_check(const double* part_rr, const double* part_bb, double tol2,
-                                                int* flags, double* scal, int first) {
+                                                int* flags, double* scal, int first, int sbb, int srr, int srr0) {
     __shared__ double sh[16];
     if (flags[0]) return;
     double rr = sum_partials(part_rr, sh);
-    double bb = first ? sum_partials(part_bb, sh) : scal[S_BB];
+    double bb = first ? (part_bb ? sum_partials(part_bb, sh) : rr) : scal[sbb];
     if (threadIdx.x == 0) {
         if (first) {
-            scal[S_BB] = bb;
-            scal[S_RR0] = rr;
+            scal[sbb] = bb;
+            scal[srr0] = rr;
         } else {
             flags[1] += 1;
         }
-        scal[S_RR] = rr;
+        scal[srr] = rr;
         if (!(rr == rr)) flags[0] = 2;
         else if (rr <= tol2 * bb || bb == 0.0) flags[0] = 1;
     }
@@ -157,16 +161,17 @@ __global__ __launch_bounds__(1024) void k_check(const double* part_rr, const dou
 
 // p = z + beta pold (first: p = z) at the node and its neighbours; q = A p;
 // partial of p.q.  z, pold and p are zero on Dirichlet nodes.
-__global__ __launch_bounds__(kB) void k_apply_p(Fine f, const float* __restrict__ z, const double* __restrict__ pold,
-                                                double* __restrict__ p, const uint8_t* __restrict__ dir,
-                                                double* __restrict__ q, double* part_pq, const double* scal,
+template <typename T>
+__global__ __launch_bounds__(kB) void k_apply_p(Fine f, const float* __restrict__ z, const T* __restrict__ pold,
+                                                T* __restrict__ p, const uint8_t* __restrict__ dir,
+                                                T* __restrict__ q, double* part_pq, const double* scal,
                                                 const int* flags, int first) {
     __shared__ double sh[kB / 64];
     if (flags[0]) return;
     const double beta = first ? 0.0 : scal[S_BETA];
     double acc = 0;
     const int G = f.gx * f.gy;
-    auto P = [&](int m) -> double { return first ? (double)z[m] : (double)z[m] + beta * pold[m]; };
+    auto P = [&](int m) -> double { return first ? (double)z[m] : (double)z[m] + beta * (double)pold[m]; };
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
         double v = 0, pn = 0;
         if (!dir[n]) {
@@ -174,8 +179,8 @@ __global__ __launch_bounds__(kB) void k_apply_p(Fine f, const float* __restrict_
             v = f.diag * pn - f.cx * (P(n - 1) + P(n + 1)) - f.cy * (P(n - f.gx) + P(n + f.gx));
             acc += pn * v;
         }
-        p[n] = pn;
-        q[n] = v;
+        p[n] = (T)pn;
+        q[n] = (T)v;
     }
     write_partial(acc, part_pq, sh);
 }
@@ -223,10 +228,10 @@ struct Lvl {
 // pointwise, e1 = wd b, so the second needs only b on the 5-point stencil);
 // b and e are stored for the restriction and the post-smoothing.  Without
 // PRE2 only b is stored.
-template <int PRE2>
-__global__ __launch_bounds__(kB) void k_update_pre2(Fine f, Lvl L, double* __restrict__ x,
-                                                    const double* __restrict__ p, const double* __restrict__ rold,
-                                                    double* __restrict__ rnew, const double* __restrict__ q,
+template <int PRE2, typename T>
+__global__ __launch_bounds__(kB) void k_update_pre2(Fine f, Lvl L, T* __restrict__ x,
+                                                    const T* __restrict__ p, const T* __restrict__ rold,
+                                                    T* __restrict__ rnew, const T* __restrict__ q,
                                                     const uint8_t* __restrict__ dir, float* __restrict__ b0,
                                                     float* __restrict__ e0, double* part_rr, const double* scal,
                                                     const int* flags) {
@@ -235,22 +240,24 @@ __global__ __launch_bounds__(kB) void k_update_pre2(Fine f, Lvl L, double* __res
     const double alpha = scal[S_ALPHA];
     double acc = 0;
     const int G = f.gx * f.gy;
-    auto R = [&](int m) -> double { return rold[m] - alpha * q[m]; };
+    auto R = [&](int m) -> double { return (double)rold[m] - alpha * (double)q[m]; };
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
         if (dir[n]) {
-            rnew[n] = 0.0;
+            rnew[n] = (T)0;
             b0[n] = 0.f;
             if (PRE2) e0[n] = 0.f;
             continue;
         }
-        x[n] += alpha * p[n];
-        const double rn = R(n);
-        rnew[n] = rn;
+        x[n] = (T)((double)x[n] + alpha * (double)p[n]);
+        const T rt = (T)R(n);
+        const double rn = (double)rt;
+        rnew[n] = rt;
         acc += rn * rn;
         const float bn = (float)rn;
         b0[n] = bn;
         if (PRE2) {
-            auto e1 = [&](int m) -> float { return dir[m] ? 0.f : L.wd * (float)R(m); };
+            // neighbours' residuals rounded like the stored ones
+            auto e1 = [&](int m) -> float { return dir[m] ? 0.f : L.wd * (float)(T)R(m); };
             const float c = L.wd * bn;
             const float ax = L.diag * c - L.cx * (e1(n - 1) + e1(n + 1)) - L.cy * (e1(n - f.gx) + e1(n + f.gx));
             e0[n] = c + L.wd * (bn - ax);
@@ -261,9 +268,10 @@ __global__ __launch_bounds__(kB) void k_update_pre2(Fine f, Lvl L, double* __res
 
 // Last post-smoothing sweep of the finest level: z = ei + wd (b - A ei) (zero
 // on Dirichlet), stored in FP32; partials of r.z and r.zold.
+template <typename T>
 __global__ __launch_bounds__(kB) void k_smooth_dots(Lvl L, const float* __restrict__ ei, float* __restrict__ z,
                                                     const float* __restrict__ b, const uint8_t* __restrict__ dir,
-                                                    const double* __restrict__ r, const float* __restrict__ zold,
+                                                    const T* __restrict__ r, const float* __restrict__ zold,
                                                     double* part_rz, double* part_rzo, const int* flags, int first) {
     __shared__ double sh[kB / 64];
     if (flags[0]) return;
@@ -277,13 +285,39 @@ __global__ __launch_bounds__(kB) void k_smooth_dots(Lvl L, const float* __restri
             v = e + L.wd * (b[n] - ax);
         }
         z[n] = v;
-        const double rn = r[n];
+        const double rn = (double)r[n];
         a += rn * (double)v;
         if (!first) c += rn * (double)zold[n];
     }
     write_partial(a, part_rz, sh);
     if (!first) write_partial(c, part_rzo, sh);
 }
+
+// rf = (float) r; partial of rf.rf (the inner solve's starting residual)
+__global__ __launch_bounds__(kB) void k_r_to_f32(int G, const double* __restrict__ r, float* __restrict__ rf,
+                                                 double* part) {
+    __shared__ double sh[kB / 64];
+    double acc = 0;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
+        const float v = (float)r[n];
+        rf[n] = v;
+        acc += (double)v * (double)v;
+    }
+    write_partial(acc, part, sh);
+}
+
+__global__ void k_zero_f32v(int G, float* __restrict__ d) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < G) d[n] = 0.f;
+}
+
+// iterative-refinement correction in FP64: x += d
+__global__ void k_ir_update(int G, double* __restrict__ x, const float* __restrict__ d) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < G) x[n] += (double)d[n];
+}
+
+__global__ void k_reset_inner(int* flags) { flags[2] = 0; }
 
 // --------------------------------------------------------------- MG (FP32)
 // Level kernels use 2D grids (x: 256 nodes of a row, y: row) so node indices
@@ -293,7 +327,8 @@ __global__ __launch_bounds__(kB) void k_smooth_dots(Lvl L, const float* __restri
     if (i >= (L).gx) return;                                                  \
     const int n = j * (L).gx + i;
 
-__global__ void k_d2f(int G, const double* __restrict__ r, float* __restrict__ b, const int* flags) {
+template <typename T>
+__global__ void k_d2f(int G, const T* __restrict__ r, float* __restrict__ b, const int* flags) {
     if (flags[0]) return;
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n < G) b[n] = (float)r[n];
@@ -851,48 +886,33 @@ int launch_efield(pic_ctx* c) {
 }
 
 // Pre-smoothing of one level from zero into m.e with the configured sweep count.
-static void presmooth(pic_ctx* c, MgLevel& m) {
+static void presmooth(pic_ctx* c, MgLevel& m, const int* flags) {
     const float w = (float)c->p.mg_omega;
     Lvl lv = lvl_of(m, w);
     dim3 g2 = grid2d(m);
     float* a = m.e;
     float* b = m.t;
     for (int s = 0; s < c->p.mg_pre; ++s) {
-        hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, c->stream, lv, a, b, m.b, m.dir, s == 0 ? 1 : 0, c->flags);
+        hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, c->stream, lv, a, b, m.b, m.dir, s == 0 ? 1 : 0, flags);
         std::swap(a, b);
     }
     if (c->p.mg_pre == 0) {
         Lvl zl = lv;
         zl.wd = 0.f;
-        hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, c->stream, zl, b, a, m.b, m.dir, 1, c->flags);
+        hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, c->stream, zl, b, a, m.b, m.dir, 1, flags);
     }
     if (a != m.e) std::swap(m.e, m.t);
-}
-
-// Finest-level pre-smoothing for sweep counts other than two.
-static int vcycle_pre_generic(pic_ctx* c, const double* r) {
-    MgLevel& m = c->mg[0];
-    hipLaunchKernelGGL(k_d2f, dim3(nblk(c->G)), dim3(kB), 0, c->stream, c->G, r, m.b, c->flags);
-    presmooth(c, m);
-    PIC_HIP(hipGetLastError());
-    return PIC_OK;
-}
-
-static int vcycle_pre_generic_fine(pic_ctx* c) {
-    presmooth(c, c->mg[0]);
-    PIC_HIP(hipGetLastError());
-    return PIC_OK;
 }
 
 // The V-cycle below the finest level's pre-smoothing (already in m0.e, with
 // its right-hand side in m0.b), ending with the last post-smoothing sweep of
 // the finest level into z and the partials of r.z and r.zold.
-static int vcycle_rest(pic_ctx* c, const double* r, float* z, const float* zold, int first) {
+template <typename T>
+static int vcycle_rest(pic_ctx* c, const T* r, float* z, const float* zold, int first, const int* flags) {
     const float w = (float)c->p.mg_omega;
     const int L = (int)c->mg.size();
     const int lc = c->mg_lc;
     hipStream_t st = c->stream;
-    const int* flags = c->flags;
     const bool fpre = c->p.mg_pre == 2, fpost = c->p.mg_post == 2;
     for (int l = 0; l < lc; ++l) {
         MgLevel& m = c->mg[l];
@@ -901,7 +921,7 @@ static int vcycle_rest(pic_ctx* c, const double* r, float* z, const float* zold,
                 hipLaunchKernelGGL(k_mg_pre2<float>, grid2d(m), dim3(kB), 0, st, lvl_of(m, w), m.b, (float*)nullptr,
                                    m.e, m.dir, flags);
             else
-                presmooth(c, m);
+                presmooth(c, m, flags);
         }
         MgLevel& n = c->mg[l + 1];
         hipLaunchKernelGGL(k_mg_restrict, grid2d(n), dim3(kB), 0, st, lvl_of(m, w), lvl_of(n, w), m.e, m.b, m.dir,
@@ -935,7 +955,7 @@ static int vcycle_rest(pic_ctx* c, const double* r, float* z, const float* zold,
         MgLevel& m = c->mg[0];
         Lvl zl = lvl_of(m, w);
         zl.wd = 0.f;
-        hipLaunchKernelGGL(k_smooth_dots, dim3(kNP), dim3(kB), 0, st, zl, m.e, z, m.b, m.dir, r, zold,
+        hipLaunchKernelGGL(k_smooth_dots<T>, dim3(kNP), dim3(kB), 0, st, zl, m.e, z, m.b, m.dir, r, zold,
                            c->part + P_RZ * kNP, c->part + P_RZO * kNP, flags, first);
     }
     for (int l = lc - 1; l >= 0; --l) {
@@ -962,7 +982,7 @@ static int vcycle_rest(pic_ctx* c, const double* r, float* z, const float* zold,
         if (l == 0) {
             Lvl zl = lv;
             if (rest == 0) zl.wd = 0.f;                 // no post-smoothing: z = e
-            hipLaunchKernelGGL(k_smooth_dots, dim3(kNP), dim3(kB), 0, st, zl, m.e, z, m.b, m.dir, r, zold,
+            hipLaunchKernelGGL(k_smooth_dots<T>, dim3(kNP), dim3(kB), 0, st, zl, m.e, z, m.b, m.dir, r, zold,
                                c->part + P_RZ * kNP, c->part + P_RZO * kNP, flags, first);
         }
     }
@@ -972,8 +992,75 @@ static int vcycle_rest(pic_ctx* c, const double* r, float* z, const float* zold,
 
 static int read_state(pic_ctx* c) {
     PIC_HIP(hipMemcpyAsync(c->scal_host, c->scal, S_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    PIC_HIP(hipMemcpyAsync(c->flags_host, c->flags, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipMemcpyAsync(c->flags_host, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+// Flexible PCG from the residual already in R[0] (type T; its check already
+// done): z0 = M r0, then iterations until flags[0] is set or max_it.  x, R, P,
+// q are T; z is FP32 (the V-cycle output); dots and scalars are FP64.  The
+// host reads the flag between batches (the first as long as *batch_hint).
+template <typename T>
+static int run_pcg(pic_ctx* c, T* x, T* const R[2], T* const P[2], T* q, float* const Z[2], int* flags,
+                   double tol2, int sbb, int srr, int srr0, int max_it, int* batch_hint) {
+    const Fine f = fine_of(c);
+    hipStream_t s = c->stream;
+    double* part = c->part;
+    const float w = (float)c->p.mg_omega;
+    MgLevel& m0 = c->mg[0];
+    const Lvl l0 = lvl_of(m0, w);
+    const int flag_off = (int)(flags - c->flags);
+    if (c->p.mg_pre == 2) {
+        hipLaunchKernelGGL(k_mg_pre2<T>, grid2d(m0), dim3(kB), 0, s, l0, R[0], m0.b, m0.e, m0.dir, flags);
+    } else {
+        hipLaunchKernelGGL(k_d2f<T>, dim3(nblk(c->G)), dim3(kB), 0, s, c->G, R[0], m0.b, flags);
+        presmooth(c, m0, flags);
+    }
+    PIC_TRY(vcycle_rest<T>(c, R[0], Z[0], Z[1], 1, flags));
+    hipLaunchKernelGGL(k_beta, dim3(1), dim3(1024), 0, s, part + P_RZ * kNP, part + P_RZO * kNP, c->scal, flags, 1);
+    PIC_HIP(hipGetLastError());
+    int it = 0, count0 = -1;
+    int batch = *batch_hint > 0 ? *batch_hint : 2;
+    while (true) {
+        PIC_TRY(read_state(c));
+        if (count0 < 0) count0 = c->flags_host[flag_off + 1];
+        if (c->flags_host[flag_off] || it >= max_it) break;
+        for (int k = 0; k < batch && it < max_it; ++k) {
+            ++it;
+            const int cur = it & 1, old = cur ^ 1;
+            hipLaunchKernelGGL(k_apply_p<T>, dim3(kNP), dim3(kB), 0, s, f, Z[old], P[old], P[cur], c->dir, q,
+                               part + P_PQ * kNP, c->scal, flags, it == 1 ? 1 : 0);
+            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1024), 0, s, part + P_PQ * kNP, c->scal, flags);
+            if (c->p.mg_pre == 2)
+                hipLaunchKernelGGL((k_update_pre2<1, T>), dim3(kNP), dim3(kB), 0, s, f, l0, x, P[cur], R[old], R[cur],
+                                   q, c->dir, m0.b, m0.e, part + P_RR * kNP, c->scal, flags);
+            else
+                hipLaunchKernelGGL((k_update_pre2<0, T>), dim3(kNP), dim3(kB), 0, s, f, l0, x, P[cur], R[old], R[cur],
+                                   q, c->dir, m0.b, m0.e, part + P_RR * kNP, c->scal, flags);
+            hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, (const double*)nullptr, tol2,
+                               flags, c->scal, 0, sbb, srr, srr0);
+            if (c->p.mg_pre != 2) presmooth(c, m0, flags);
+            PIC_TRY(vcycle_rest<T>(c, R[cur], Z[cur], Z[old], 0, flags));
+            hipLaunchKernelGGL(k_beta, dim3(1), dim3(1024), 0, s, part + P_RZ * kNP, part + P_RZO * kNP, c->scal,
+                               flags, 0);
+            PIC_HIP(hipGetLastError());
+        }
+        batch = 2;
+    }
+    *batch_hint = c->flags_host[flag_off + 1] - count0;
+    return PIC_OK;
+}
+
+// FP32 work vectors of the mixed-precision solve (allocated on first use)
+static int ensure_f32(pic_ctx* c) {
+    if (c->rf0) return PIC_OK;
+    const size_t b = (size_t)c->G * sizeof(float);
+    float** bufs[] = {&c->rf0, &c->rf1, &c->pf0, &c->pf1, &c->qf, &c->df};
+    for (float** q : bufs) {
+        PIC_HIP(hipMalloc((void**)q, b));
+        PIC_HIP(hipMemsetAsync(*q, 0, b, c->stream));
+    }
     return PIC_OK;
 }
 
@@ -989,12 +1076,11 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     PIC_HIP(hipEventRecord(t0, s));
     double* part = c->part;
     const double tol2 = c->p.poisson_tol * c->p.poisson_tol;
-    const float w = (float)c->p.mg_omega;
-    MgLevel& m0 = c->mg[0];
-    const Lvl l0 = lvl_of(m0, w);
-    double* R[2] = {c->pr, c->pr2};
-    double* P[2] = {c->pp, c->pp2};
-    float* Z[2] = {c->zf0, c->zf1};
+    double* const R[2] = {c->pr, c->pr2};
+    double* const P[2] = {c->pp, c->pp2};
+    float* const Z[2] = {c->zf0, c->zf1};
+    const bool mixed = c->p.poisson_precision == 1;
+    if (mixed) PIC_TRY(ensure_f32(c));
 
     hipLaunchKernelGGL(k_reset_flags, dim3(1), dim3(1), 0, s, c->flags);
     PIC_TRY(launch_overwrite_rho(c));
@@ -1044,49 +1130,51 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, src, prev, keep, c->dir, c->dval);
     hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, R[0], part + P_RR * kNP);
     hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2, c->flags,
-                       c->scal, 1);
-    // z_0 = M r_0
-    if (c->p.mg_pre == 2)
-        hipLaunchKernelGGL(k_mg_pre2<double>, grid2d(m0), dim3(kB), 0, s, l0, R[0], m0.b, m0.e, m0.dir, c->flags);
-    else
-        PIC_TRY(vcycle_pre_generic(c, R[0]));
-    PIC_TRY(vcycle_rest(c, R[0], Z[0], Z[1], 1));
-    hipLaunchKernelGGL(k_beta, dim3(1), dim3(1024), 0, s, part + P_RZ * kNP, part + P_RZO * kNP, c->scal, c->flags,
-                       1);
+                       c->scal, 1, S_BB, S_RR, S_RR0);
     PIC_HIP(hipGetLastError());
-    int it = 0;
-    // the first batch is as long as the previous solve; then two at a time
-    int batch = c->last_iters > 0 ? c->last_iters : 2;
-    while (true) {
-        PIC_TRY(read_state(c));
-        if (c->flags_host[0] || it >= c->p.poisson_max_iter) break;
-        for (int k = 0; k < batch && it < c->p.poisson_max_iter; ++k) {
-            ++it;
-            const int cur = it & 1, old = cur ^ 1;
-            hipLaunchKernelGGL(k_apply_p, dim3(kNP), dim3(kB), 0, s, f, Z[old], P[old], P[cur], c->dir, c->pq,
-                               part + P_PQ * kNP, c->scal, c->flags, it == 1 ? 1 : 0);
-            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1024), 0, s, part + P_PQ * kNP, c->scal, c->flags);
-            if (c->p.mg_pre == 2)
-                hipLaunchKernelGGL(k_update_pre2<1>, dim3(kNP), dim3(kB), 0, s, f, l0, c->phi, P[cur], R[old], R[cur],
-                                   c->pq, c->dir, m0.b, m0.e, part + P_RR * kNP, c->scal, c->flags);
-            else
-                hipLaunchKernelGGL(k_update_pre2<0>, dim3(kNP), dim3(kB), 0, s, f, l0, c->phi, P[cur], R[old], R[cur],
-                                   c->pq, c->dir, m0.b, m0.e, part + P_RR * kNP, c->scal, c->flags);
-            hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2,
-                               c->flags, c->scal, 0);
-            if (c->p.mg_pre != 2) PIC_TRY(vcycle_pre_generic_fine(c));
-            PIC_TRY(vcycle_rest(c, R[cur], Z[cur], Z[old], 0));
-            hipLaunchKernelGGL(k_beta, dim3(1), dim3(1024), 0, s, part + P_RZ * kNP, part + P_RZO * kNP, c->scal,
-                               c->flags, 0);
+    int iters = 0, inner_total = 0;
+    if (!mixed) {
+        PIC_TRY(run_pcg<double>(c, c->phi, R, P, c->pq, Z, c->flags, tol2, S_BB, S_RR, S_RR0, c->p.poisson_max_iter,
+                                &c->last_iters));
+        iters = c->flags_host[1];
+        inner_total = iters;
+    } else {
+        // iterative refinement: FP64 residual, FP32 PCG for the correction to
+        // the inner relative residual, FP64 update of phi
+        float* const RF[2] = {c->rf0, c->rf1};
+        float* const PF[2] = {c->pf0, c->pf1};
+        const double eta = c->p.poisson_inner_tol > 0 ? c->p.poisson_inner_tol : 1e-4;
+        int outer = 0;
+        while (true) {
+            PIC_TRY(read_state(c));
+            if (c->flags_host[0] || outer >= c->p.poisson_max_iter) break;
+            hipLaunchKernelGGL(k_r_to_f32, dim3(kNP), dim3(kB), 0, s, G, R[0], RF[0], part + P_RR * kNP);
+            hipLaunchKernelGGL(k_reset_inner, dim3(1), dim3(1), 0, s, c->flags);
+            hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, (const double*)nullptr,
+                               eta * eta, c->flags + 2, c->scal, 1, S_IBB, S_IRR, S_IRR0);
+            hipLaunchKernelGGL(k_zero_f32v, dim3(nblk(G)), dim3(kB), 0, s, G, c->df);
             PIC_HIP(hipGetLastError());
+            PIC_TRY(run_pcg<float>(c, c->df, RF, PF, c->qf, Z, c->flags + 2, eta * eta, S_IBB, S_IRR, S_IRR0,
+                                   c->p.poisson_max_iter, &c->last_inner));
+            if (c->flags_host[2] == 2) {
+                set_error("Poisson inner FP32 PCG diverged (NaN residual)");
+                return PIC_ERR_SOLVER;
+            }
+            hipLaunchKernelGGL(k_ir_update, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, c->df);
+            hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, R[0],
+                               part + P_RR * kNP);
+            hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, (const double*)nullptr, tol2,
+                               c->flags, c->scal, 0, S_BB, S_RR, S_RR0);
+            PIC_HIP(hipGetLastError());
+            ++outer;
         }
-        batch = 2;
+        iters = outer;
+        inner_total = c->flags_host[3];
     }
     const double bb = c->scal_host[S_BB];
     const double bnorm = std::sqrt(bb);
     const double rel0 = bnorm > 0 ? std::sqrt(c->scal_host[S_RR0]) / bnorm : 0.0;
     const double rel = bnorm > 0 ? std::sqrt(c->scal_host[S_RR]) / bnorm : 0.0;
-    const int iters = c->flags_host[1];
     if (c->flags_host[0] == 2) {
         set_error("Poisson PCG diverged (NaN residual) after %d iterations", iters);
         return PIC_ERR_SOLVER;
@@ -1105,7 +1193,6 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     (void)hipEventDestroy(t1);
     c->guess_set = 0;
     c->have_prev_phi = (extrap && keep) ? 2 : 1;
-    c->last_iters = iters;
     if (learned && c->learn_trained < c->learn_train) {
         float loss = 0.f;
         PIC_TRY(unet_train_pair(c, c->unet, c->phi_prev, c->learn_lr, &loss));
@@ -1119,6 +1206,7 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
         st->final_relres = rel;
         st->ms = ms;
         st->guess = guess;
+        st->inner_iterations = inner_total;
     }
     return PIC_OK;
 }

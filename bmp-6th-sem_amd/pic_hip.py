@@ -50,13 +50,14 @@ class Params(C.Structure):
         ("poisson_tol", C.c_double), ("poisson_max_iter", C.c_int),
         ("mg_pre", C.c_int), ("mg_post", C.c_int), ("mg_omega", C.c_double),
         ("poisson_guess", C.c_int),
+        ("poisson_precision", C.c_int), ("poisson_inner_tol", C.c_double),
     ]
 
 
 class SolveStats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int), ("initial_relres", C.c_double), ("final_relres", C.c_double),
-        ("ms", C.c_double), ("guess", C.c_int),
+        ("ms", C.c_double), ("guess", C.c_int), ("inner_iterations", C.c_int),
     ]
 
     def as_dict(self):

@@ -97,17 +97,22 @@ typedef struct pic_params {
     double mg_omega;          /* Jacobi weight                                                 */
     int    poisson_guess;     /* initial guess: 0 previous phi (the reference's warm start),
                                  1 linear extrapolation 2 phi_n-1 - phi_n-2 of the last two solves */
+    int    poisson_precision; /* 0: FP64 PCG (FP32 multigrid preconditioner);
+                                 1: iterative refinement — FP64 residual, dot products and update
+                                 of phi, FP32 PCG vectors for each correction solve            */
+    double poisson_inner_tol; /* relative residual of each FP32 correction solve (precision 1)  */
 } pic_params;
 
 /* Per-solve statistics of pic_poisson_solve. */
 typedef struct pic_solve_stats {
-    int    iterations;        /* PCG iterations (FP64 outer loop)                 */
+    int    iterations;        /* PCG iterations (precision 0) or refinement steps (precision 1) */
     double initial_relres;    /* ||b - A phi0|| / ||b|| of the initial guess       */
     double final_relres;      /* same after the solve                              */
     double ms;                /* device time of the solve (HIP events)             */
     int    guess;             /* 0 zero guess, 1 previous phi, 2 caller-set guess,
                                  3 extrapolated from the last two solves,
                                  4 learned (U-Net) prediction                      */
+    int    inner_iterations;  /* PCG iterations in all (precision 1: FP32 ones)   */
 } pic_solve_stats;
 
 /* ====================================================== stage group */

@@ -309,3 +309,53 @@ def test_full_steps_vs_cpu_baseline(hip_mod, oracle_mod):
         assert np.abs(ca - cb).sum() <= 4
     ctx.close()
     sim.close()
+
+
+@pytest.mark.parametrize("it", [0, 2])
+def test_poisson_mixed_precision_matches_reference_pardiso(hip_mod, it):
+    """Iterative refinement (FP32 correction solves, FP64 residual, dots and
+    update) on the PARDISO dumps: phi to 1e-10 at tolerance 1e-12."""
+    from conftest import GOLDEN
+
+    hip = hip_mod
+    p = hip.default_params(NX=256, NY=256, width=1.0, height=1.0, dt=2.5e-9, poisson_tol=1e-12, poisson_precision=1)
+    c = hip.Context(p)
+    c.set_geometry(WALLS)
+    rho = np.load(GOLDEN / f"iter{it:04d}_rho_input.npy")
+    ref = np.load(GOLDEN / f"iter{it:04d}_phi_cpp.npy")
+    c.set_grid(hip.GRID_RHO, rho)
+    st = c.poisson_solve(0)
+    phi = c.grid(hip.GRID_PHI)
+    print("refinement steps", st["iterations"], "FP32 iterations", st["inner_iterations"])
+    assert st["final_relres"] <= 1e-12 and st["inner_iterations"] >= st["iterations"] >= 1
+    assert np.abs(phi - ref).max() / np.abs(ref).max() < 1e-10
+    c.close()
+
+
+def test_precision_form_field_errors(hip_mod, oracle_mod):
+    """Error of each field against the FP64 oracle (direct solve, FP64 E) for
+    the FP64 PCG and the mixed-precision refinement at 1e-10, electrode
+    geometry, 256^2: phi, Ex, Ey relative (max-norm) errors all below 1e-8."""
+    hip, O = hip_mod, oracle_mod
+    NX = 256
+    rng = np.random.default_rng(5)
+    errs = {}
+    for prec in (0, 1):
+        c = _ctx(hip, NX, 1, geom=ELEC, poisson_tol=1e-10, poisson_precision=prec)
+        rho = rng.normal(0, 1e6, c.G) if prec == 0 else rho_keep
+        rho_keep = rho
+        c.set_grid(hip.GRID_RHO, rho)
+        st = c.poisson_solve(0)
+        rhs, phi_ref, _, _ = O.poisson_solver(rho, c.gx, c.gy, c.dx, c.dy, ELEC)
+        ex_ref, ey_ref = O.efield(phi_ref, c.gx, c.gy, c.dx, c.dy)
+        e = {}
+        for name, which, ref in (("phi", hip.GRID_PHI, phi_ref), ("Ex", hip.GRID_EX, ex_ref),
+                                 ("Ey", hip.GRID_EY, ey_ref)):
+            e[name] = float(np.abs(c.grid(which) - ref).max() / np.abs(ref).max())
+        e["iterations"] = st["iterations"]
+        e["inner_iterations"] = st["inner_iterations"]
+        errs["fp64" if prec == 0 else "mixed"] = e
+        c.close()
+    print("field errors vs the FP64 oracle:", errs)
+    for e in errs.values():
+        assert e["phi"] < 1e-8 and e["Ex"] < 1e-8 and e["Ey"] < 1e-8
