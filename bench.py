@@ -166,6 +166,8 @@ def main():
     ap.add_argument("--learned-train", type=int, default=30,
                     help="online training steps of the learned-guess comparison (0 skips it)")
     ap.add_argument("--learned-eval", type=int, default=20)
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="Poisson: FP64 PCG, or FP32 correction solves refined in FP64")
     ap.add_argument("--guess", choices=("prev", "extrap"), default="prev",
                     help="Poisson initial guess: previous phi, or extrapolated from the last two")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -191,7 +193,8 @@ def main():
     parts = [cfg.generate(w, s) for s in range(2)]
     log(f"generated {2 * w.n_per_species} particles in {time.perf_counter() - t0:.1f} s")
     ctx = cfg.make_context(w, layout=layout, particles=parts, poisson_tol=1e-10,
-                           poisson_guess=1 if a.guess == "extrap" else 0)
+                           poisson_guess=1 if a.guess == "extrap" else 0,
+                           poisson_precision=1 if a.precision == "mixed" else 0)
     log(f"context ready ({time.perf_counter() - t0:.1f} s)")
 
     it = 0
@@ -279,9 +282,10 @@ def main():
         "config": {"workload": w.name, "cells": f"{w.NX}x{w.NY}", "particles_per_species": w.n_per_species,
                    "species": 2, "electrode_V": w.electrode_v if w.electrode else None,
                    "B_T": w.b_total, "B_tilt_deg": w.b_tilt_deg, "poisson_tol": 1e-10,
-                   "poisson_guess": a.guess},
+                   "poisson_guess": a.guess, "poisson_precision": a.precision},
         "stages_ms_per_step": {k: round(v[0] / a.steps, 4) for k, v in stages.items()},
         "poisson": {"iterations_mean": float(np.mean(iters)), "ms_mean": float(np.mean([s["ms"] for s in stats])),
+                    "inner_iterations_mean": float(np.mean([s["inner_iterations"] for s in stats])),
                     "initial_relres_mean": float(np.mean([s["initial_relres"] for s in stats]))},
         "roofline": roof,
         "particle_kernels": others,

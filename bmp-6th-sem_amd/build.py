@@ -36,9 +36,11 @@ COMMON = [
 # per-file extra flags
 EXTRA = {
     "pic_particles.hip": ["-ffp-contract=off"],
+    "pic_ensemble.hip": ["-ffp-contract=off"],
 }
 
-SOURCES = ["pic_particles.hip", "pic_poisson.hip", "pic_unet.hip", "pic_host.cpp", "pic_config.cpp", "pic_dropin.cpp"]
+SOURCES = ["pic_particles.hip", "pic_poisson.hip", "pic_unet.hip", "pic_ensemble.hip", "pic_host.cpp",
+           "pic_config.cpp", "pic_dropin.cpp"]
 
 
 def _compile(src: str) -> Path:

@@ -113,6 +113,24 @@ WORKLOADS = {
 }
 
 
+def ensemble(nruns: int = 64, seed: int = 4, grids=(64, 128, 256), n_min: float = 5e4, n_max: float = 1e6):
+    """Uneven ensemble: grids drawn from `grids`, particles per species
+    log-uniform in [n_min, n_max], electrode voltage uniform in 10-200 V, Bz
+    uniform in 0-0.1 T (no tilt), all drawn from `seed`; run k's particles use
+    seed 1000 * seed + k."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    runs = []
+    for k in range(nruns):
+        NX = int(rng.choice(grids))
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        v = float(rng.uniform(10.0, 200.0))
+        bz = float(rng.uniform(0.0, 0.1))
+        w = Workload(f"ens{k}", NX, NX, n, 1000 * seed + k, bz, 0.0, ELECTRODE, electrode_v=v, layout=1)
+        runs.append(w)
+    return runs
+
+
 def generate(w: Workload, s: int):
     """Particles of species s of workload w from the library's generator."""
     import pic_hip

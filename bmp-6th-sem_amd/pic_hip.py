@@ -125,6 +125,12 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "pic_unet_mfma_probe": (i32, [fp, fp, fp]),
         "pic_unet_benchmark": (i32, [vp, i32, fp, fp]),
         "pic_learned_unet": (vp, [vp]),
+        "pic_ensemble_create": (i32, [C.POINTER(vp), i32, C.POINTER(vp)]),
+        "pic_ensemble_destroy": (None, [vp]),
+        "pic_ensemble_step": (i32, [vp, i32, i32]),
+        "pic_ensemble_poisson_stats": (i32, [vp, ip, dp]),
+        "pic_ensemble_tune": (i32, [vp, i32, C.POINTER(C.c_int), C.POINTER(C.c_int), dp]),
+        "pic_ensemble_steps_done": (i32, [vp, lp]),
         "pic_learned_enable": (i32, [vp, i32, i32, C.c_float, C.c_uint64]),
         "pic_learned_stats": (i32, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), fp]),
     }
@@ -432,3 +438,42 @@ class UNet:
         _chk(self.lib.pic_unet_train_step(self.h, _fp(x), _fp(t), m.ctypes.data_as(C.POINTER(C.c_uint8)),
                                           float(lr), _fp(loss)))
         return float(loss[0])
+
+
+class Ensemble:
+    """Batched steps of several contexts (persistent kernels, one HIP graph)."""
+
+    def __init__(self, contexts):
+        self.lib = load()
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = C.c_void_p()
+        _chk(self.lib.pic_ensemble_create(arr, len(self.contexts), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pic_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1, graph=True):
+        _chk(self.lib.pic_ensemble_step(self.h, int(n), 1 if graph else 0))
+
+    def poisson_stats(self):
+        R = len(self.contexts)
+        it = np.zeros(R, np.int32)
+        rr = np.zeros(2 * R)
+        _chk(self.lib.pic_ensemble_poisson_stats(self.h, _ip(it), _dp(rr)))
+        return it, rr.reshape(R, 2)
+
+    def tune(self, steps=2):
+        t, b = C.c_int(), C.c_int()
+        ms = np.zeros(1)
+        _chk(self.lib.pic_ensemble_tune(self.h, int(steps), C.byref(t), C.byref(b), _dp(ms)))
+        return {"tile": t.value, "blocks": b.value, "ms_per_step": float(ms[0])}

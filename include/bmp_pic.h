@@ -279,6 +279,28 @@ pic_unet* pic_learned_unet(pic_ctx* ctx);
  * was used (lower starting residual), and the last training loss. */
 int     pic_learned_stats(pic_ctx* ctx, int* trained, int* tried, int* used, float* last_loss);
 
+/* ======================================================= ensembles */
+/* Batched execution of independent runs of uneven sizes: every stage is one
+ * launch for all runs (persistent work-queue kernels for deposit, E, push and
+ * scatter; one workgroup per run for the Poisson solve and the scan), the
+ * step is captured once in a HIP graph and replayed.  The runs are contexts
+ * already set up (geometry, two species, particles uploaded), CELL32 storage,
+ * Bz-only field, FP64 Poisson; after the ensemble's steps they are in the
+ * state a serial pic_step would leave.  No counterpart in the reference (one
+ * simulation per process, sycl_pic_ctypes.cpp:39-41). */
+typedef struct pic_ensemble pic_ensemble;
+
+int  pic_ensemble_create(pic_ctx** runs, int nruns, pic_ensemble** out);
+void pic_ensemble_destroy(pic_ensemble* e);
+/* nsteps steps of every run; graph = 1 replays the captured step. */
+int  pic_ensemble_step(pic_ensemble* e, int nsteps, int graph);
+/* Iterations and (initial, final) relative residual of each run's last solve. */
+int  pic_ensemble_poisson_stats(pic_ensemble* e, int* iters, double* relres);
+/* Times `steps` steps per candidate (push tile width 64/128/256 cells x
+ * persistent workgroups 2/4/8 per CU) and keeps the fastest; advances the runs. */
+int  pic_ensemble_tune(pic_ensemble* e, int steps, int* tile, int* blocks, double* ms_per_step);
+int  pic_ensemble_steps_done(pic_ensemble* e, int64_t* steps);
+
 /* =================================================== drop-in group */
 /* Same names/signatures/semantics as SYCL-PIC-poissonsolver/src/sycl_pic_ctypes.cpp.
  * init reads ./M_PICDAT.DAT and ./geometry.conf from the working directory;
