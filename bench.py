@@ -122,6 +122,84 @@ def learned_vs_plain(w, parts, layout, train, evaluate, tol=1e-8):
     return out
 
 
+def ensemble_form(nruns, steps, seed=4):
+    """Aggregate particle-steps/s of an uneven ensemble stepped as one batch
+    (persistent kernels, one HIP graph per step, autotuned) against the same
+    runs stepped one after another with the single-run path; plus a check of
+    the first runs against serial copies after one step."""
+    import numpy as np
+    import pic_configs as cfg
+    import pic_hip
+
+    runs = cfg.ensemble(nruns=nruns, seed=seed)
+    t0 = time.perf_counter()
+    parts = [[cfg.generate(w, s) for s in range(2)] for w in runs]
+    log(f"ensemble: {nruns} runs, {sum(2 * w.n_per_species for w in runs)} particles generated in "
+        f"{time.perf_counter() - t0:.1f} s")
+
+    def contexts():
+        return [cfg.make_context(w, layout=1, particles=p, poisson_tol=1e-10) for w, p in zip(runs, parts)]
+
+    def live(ctxs):
+        return sum(c.count(s) for c in ctxs for s in range(2))
+
+    # serial: each run stepped by the single-run path
+    ser = contexts()
+    for k, c in enumerate(ser):
+        c.step(0)
+    for c in ser:
+        c.sync()
+    n0 = live(ser)
+    t1 = time.perf_counter()
+    for it in range(1, 1 + steps):
+        for c in ser:
+            c.step(it)
+    for c in ser:
+        c.sync()
+    t_ser = time.perf_counter() - t1
+    rate_ser = steps * 0.5 * (n0 + live(ser)) / t_ser
+    # batched
+    ens_ctx = contexts()
+    ens = pic_hip.Ensemble(ens_ctx)
+    ens.step(1)
+    tuned = ens.tune(2)
+    n0 = live(ens_ctx)
+    t1 = time.perf_counter()
+    ens.step(steps)
+    t_ens = time.perf_counter() - t1
+    rate_ens = steps * 0.5 * (n0 + live(ens_ctx)) / t_ens
+    it, rr = ens.poisson_stats()
+    ens.close()
+    for c in ens_ctx + ser:
+        c.close()
+    # per-run check: a fresh ensemble of the first runs against serial copies, one step
+    k = min(8, nruns)
+    a = [cfg.make_context(w, layout=1, particles=p, poisson_tol=1e-12) for w, p in zip(runs[:k], parts[:k])]
+    b = [cfg.make_context(w, layout=1, particles=p, poisson_tol=1e-12) for w, p in zip(runs[:k], parts[:k])]
+    e2 = pic_hip.Ensemble(a)
+    e2.step(1)
+    for c in b:
+        c.step(0)
+    rho_exact = all(np.array_equal(x.grid(pic_hip.GRID_RHO), y.grid(pic_hip.GRID_RHO)) for x, y in zip(a, b))
+    phi_err = max(float(np.abs(x.grid(pic_hip.GRID_PHI) - y.grid(pic_hip.GRID_PHI)).max()
+                        / np.abs(y.grid(pic_hip.GRID_PHI)).max()) for x, y in zip(a, b))
+    e2.close()
+    for c in a + b:
+        c.close()
+    out = {
+        "runs": nruns, "steps": steps, "grids": sorted({w.NX for w in runs}),
+        "particles": int(sum(2 * w.n_per_species for w in runs)),
+        "value": rate_ens, "unit": "particle-steps/s",
+        "serial_value": rate_ser, "speedup_vs_serial": rate_ens / rate_ser,
+        "ms_per_step": 1e3 * t_ens / steps, "serial_ms_per_step": 1e3 * t_ser / steps,
+        "autotuner": tuned, "poisson_iterations_max": int(it.max()), "poisson_final_relres_max": float(rr[:, 1].max()),
+        "check_runs": k, "check_rho_bit_identical": bool(rho_exact), "check_phi_max_rel_err": phi_err,
+    }
+    log(f"ensemble: {rate_ens:.3e} particle-steps/s batched vs {rate_ser:.3e} serial "
+        f"({out['speedup_vs_serial']:.1f}x), tuned {tuned}")
+    return out
+
+
 def cpu_baseline(w, parts, steps=3):
     """Oracle (C, OpenMP, FP64 AoS, MKL PARDISO) on the same workload."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -166,6 +244,9 @@ def main():
     ap.add_argument("--learned-train", type=int, default=30,
                     help="online training steps of the learned-guess comparison (0 skips it)")
     ap.add_argument("--learned-eval", type=int, default=20)
+    ap.add_argument("--ensemble-runs", type=int, default=64,
+                    help="runs of the batched-ensemble comparison (0 skips it)")
+    ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="Poisson: FP64 PCG, or FP32 correction solves refined in FP64")
     ap.add_argument("--guess", choices=("prev", "extrap"), default="prev",
@@ -258,6 +339,14 @@ def main():
             log(f"learned-guess comparison failed: {e}")
             learned = {"error": str(e)}
 
+    ensemble = None
+    if a.ensemble_runs > 0:
+        try:
+            ensemble = ensemble_form(a.ensemble_runs, a.ensemble_steps)
+        except Exception as e:  # reported, the main measurement stands
+            log(f"ensemble comparison failed: {e}")
+            ensemble = {"error": str(e)}
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -290,6 +379,7 @@ def main():
         "roofline": roof,
         "particle_kernels": others,
         "poisson_learned_vs_plain": learned,
+        "ensemble": ensemble,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
     }
