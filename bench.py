@@ -242,9 +242,9 @@ def main():
     ap.add_argument("--workload", default="electrode_1024")
     ap.add_argument("--layout", type=int, default=None, help="0 F64, 1 CELL32 (default: the workload's)")
     ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--learned-train", type=int, default=30,
+    ap.add_argument("--learned-train", type=int, default=200,
                     help="online training steps of the learned-guess comparison (0 skips it)")
-    ap.add_argument("--learned-eval", type=int, default=20)
+    ap.add_argument("--learned-eval", type=int, default=50)
     ap.add_argument("--ensemble-runs", type=int, default=64,
                     help="runs of the batched-ensemble comparison (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=10)

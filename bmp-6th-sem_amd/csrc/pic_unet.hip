@@ -170,21 +170,28 @@ __global__ __launch_bounds__(kT) void k_conv_wgrad(const bf16* __restrict__ dZT,
         }
 }
 
-// db[co] += sum_p dZ[p][co]; dZ padded NHWC with cout channels.
+// db[co] += sum_p dZ[p][co]; dZ padded NHWC with cout channels (cout divides
+// 256).  A block takes `chunk` pixels; thread t reads channel t % cout of
+// pixels t / cout, t / cout + 256 / cout, ... (each group of cout threads reads
+// one pixel's channels, contiguous); per-channel block sums go to one atomic each.
 __global__ __launch_bounds__(kT) void k_bias_grad(const bf16* __restrict__ dZ, int cout, int H, int W, int chunk,
                                                   float* __restrict__ db) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int co = t % cout, part = t / cout;
+    __shared__ float sh[kT];
+    const int co = threadIdx.x % cout, po = threadIdx.x / cout, step = kT / cout;
     const int HW = H * W;
-    const int p0 = part * chunk;
-    if (p0 >= HW) return;
-    const int p1 = min(HW, p0 + chunk);
+    const int p0 = blockIdx.x * chunk, p1 = min(HW, p0 + chunk);
     float s = 0.f;
-    for (int p = p0; p < p1; ++p) {
+    for (int p = p0 + po; p < p1; p += step) {
         const int y = p / W, x = p - y * W;
         s += bf2f(dZ[((size_t)(y + 1) * (W + 2) + x + 1) * cout + co]);
     }
-    atomicAdd(db + co, s);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if ((int)threadIdx.x < cout) {
+        float t = 0.f;
+        for (int k = 0; k < step; ++k) t += sh[k * cout + threadIdx.x];
+        atomicAdd(db + threadIdx.x, t);
+    }
 }
 
 // dZ = bf16(dY * (Y > 0)); dY FP32 (stride gC, first channel go), Y BF16
@@ -288,29 +295,97 @@ __global__ void k_head(const bf16* __restrict__ d0, int c, const float* __restri
     out[p] = s;
 }
 
-// loss = sum over masked pixels of (out - t)^2 / count; g = dloss/dout;
-// gd0[p][c] = g wh[c]; dwh[c] += sum g d0[c]; dbh += sum g; loss += sum.
-// count: number of masked pixels (device).  c = 16 channels.
-__global__ __launch_bounds__(kT) void k_head_bwd(const float* __restrict__ out, const float* __restrict__ tgt,
+// Output gradient of the masked mean squared error against `tgt`; loss added.
+__global__ __launch_bounds__(kT) void k_loss_mse(const float* __restrict__ out, const float* __restrict__ tgt,
                                                  const uint8_t* __restrict__ mask, const int* __restrict__ count,
-                                                 const bf16* __restrict__ d0, const float* __restrict__ wh,
-                                                 float* __restrict__ gd0, float* __restrict__ dwh,
-                                                 float* __restrict__ dbh, float* __restrict__ loss, int H, int W) {
-    __shared__ float sh[18][kT / 64];
+                                                 float* __restrict__ gout, float* __restrict__ loss, int n) {
+    __shared__ float sh[kT / 64];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const float inv = *count > 0 ? 1.f / (float)*count : 0.f;
-    float red[18];
-#pragma unroll
-    for (int k = 0; k < 18; ++k) red[k] = 0.f;
-    if (p < H * W) {
-        const int y = p / W, x = p - y * W;
-        const size_t q = (size_t)(y + 1) * (W + 2) + x + 1;
+    float l = 0.f;
+    if (p < n) {
         float g = 0.f;
         if (mask[p]) {
             const float d = out[p] - tgt[p];
-            red[17] = d * d * inv;
+            l = d * d * inv;
             g = 2.f * d * inv;
         }
+        gout[p] = g;
+    }
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, (sh[0] + sh[1]) + (sh[2] + sh[3]));
+}
+
+// Residual loss: the prediction d (= out on unknown nodes, 0 elsewhere) should
+// cancel the scaled starting residual, res = r0 / max|r0| - (A / diag) d on
+// unknown nodes; loss = mean res^2.  (A / diag has unit diagonal and
+// off-diagonals -cxd, -cyd.)
+__global__ __launch_bounds__(kT) void k_loss_res(const float* __restrict__ out, const double* __restrict__ r0,
+                                                 const unsigned long long* __restrict__ scl,
+                                                 const uint8_t* __restrict__ mask, const int* __restrict__ count,
+                                                 float cxd, float cyd, int N, float* __restrict__ res,
+                                                 float* __restrict__ loss) {
+    __shared__ float sh[kT / 64];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const float inv = *count > 0 ? 1.f / (float)*count : 0.f;
+    float l = 0.f;
+    if (p < N * N) {
+        float v = 0.f;
+        if (mask[p]) {
+            const int y = p / N, x = p - y * N;
+            auto d = [&](int yy, int xx) -> float {
+                if (yy < 0 || yy >= N || xx < 0 || xx >= N) return 0.f;
+                const int q = yy * N + xx;
+                return mask[q] ? out[q] : 0.f;
+            };
+            const double s0 = __longlong_as_double((long long)scl[0]);
+            const float rs = (float)(r0[p] / (s0 > 0.0 ? s0 : 1.0));
+            const float ad = d(y, x) - cxd * (d(y, x - 1) + d(y, x + 1)) - cyd * (d(y - 1, x) + d(y + 1, x));
+            v = rs - ad;
+            l = v * v * inv;
+        }
+        res[p] = v;
+    }
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, (sh[0] + sh[1]) + (sh[2] + sh[3]));
+}
+
+// d loss / d out = -2 (A / diag) res / count on unknown nodes (A symmetric)
+__global__ void k_loss_res_grad(const float* __restrict__ res, const uint8_t* __restrict__ mask,
+                                const int* __restrict__ count, float cxd, float cyd, int N, float* __restrict__ gout) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N * N) return;
+    const float inv = *count > 0 ? 1.f / (float)*count : 0.f;
+    float g = 0.f;
+    if (mask[p]) {
+        const int y = p / N, x = p - y * N;
+        auto r = [&](int yy, int xx) -> float {
+            if (yy < 0 || yy >= N || xx < 0 || xx >= N) return 0.f;
+            return res[yy * N + xx];
+        };
+        g = -2.f * inv * (r(y, x) - cxd * (r(y, x - 1) + r(y, x + 1)) - cyd * (r(y - 1, x) + r(y + 1, x)));
+    }
+    gout[p] = g;
+}
+
+// Head backward from the output gradient gout: gd0[p][c] = gout wh[c];
+// dwh[c] += sum gout d0[c]; dbh += sum gout.  c = 16 channels.
+__global__ __launch_bounds__(kT) void k_head_bwd(const float* __restrict__ gout, const bf16* __restrict__ d0,
+                                                 const float* __restrict__ wh, float* __restrict__ gd0,
+                                                 float* __restrict__ dwh, float* __restrict__ dbh, int H, int W) {
+    __shared__ float sh[17][kT / 64];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    float red[17];
+#pragma unroll
+    for (int k = 0; k < 17; ++k) red[k] = 0.f;
+    if (p < H * W) {
+        const int y = p / W, x = p - y * W;
+        const size_t q = (size_t)(y + 1) * (W + 2) + x + 1;
+        const float g = gout[p];
         red[16] = g;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
@@ -320,18 +395,17 @@ __global__ __launch_bounds__(kT) void k_head_bwd(const float* __restrict__ out, 
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < 18; ++k) {
+    for (int k = 0; k < 17; ++k) {
         float v = red[k];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
         if (lane == 0) sh[k][wid] = v;
     }
     __syncthreads();
-    if (threadIdx.x < 18) {
+    if (threadIdx.x < 17) {
         float s = 0.f;
         for (int w = 0; w < kT / 64; ++w) s += sh[threadIdx.x][w];
         if (threadIdx.x < 16) atomicAdd(dwh + threadIdx.x, s);
-        else if (threadIdx.x == 16) atomicAdd(dbh, s);
-        else atomicAdd(loss, s);
+        else atomicAdd(dbh, s);
     }
 }
 
@@ -402,7 +476,17 @@ __global__ __launch_bounds__(kT) void k_unet_resid(const double* __restrict__ rh
         m = fmax(m, __shfl_down(m, off, 64));
         a = fmax(a, __shfl_down(a, off, 64));
     }
+    __shared__ double sm[kT / 64], sa[kT / 64];
     if ((threadIdx.x & 63) == 0) {
+        sm[threadIdx.x >> 6] = m;
+        sa[threadIdx.x >> 6] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kT / 64; ++w) {
+            m = fmax(m, sm[w]);
+            a = fmax(a, sa[w]);
+        }
         atomicMax(scl, (unsigned long long)__double_as_longlong(m));
         atomicMax(scl + 1, (unsigned long long)__double_as_longlong(a));
     }
@@ -492,7 +576,8 @@ struct pic_unet {
     pic::bf16* wb = nullptr;              // BF16 copies (conv weights; biases/head unused)
     pic::bf16* wflip = nullptr;           // [9][cin][cout] flipped copies, same offsets as W
     float* out = nullptr;                 // head output N x N
-    float* tgt = nullptr;
+    float* tgt = nullptr;                 // target (MSE) or residual buffer (residual loss)
+    float* gout = nullptr;                // d loss / d output
     uint8_t* mask = nullptr;
     float* loss = nullptr;
     int* count = nullptr;
@@ -596,7 +681,7 @@ static int conv_any(hipStream_t s, int nt, const bf16* X, int xC, int xo, int ci
 
 static int wgrad_any(hipStream_t s, int ct, const bf16* dZT, const bf16* XT, int cout, int cin, int H, int W,
                      float* dW) {
-    const int rows = H >= 64 ? 8 : 1;
+    const int rows = H >= 256 ? 2 : 1;
     const int waves = 9 * (cout / 16) * ((H + rows - 1) / rows);
     const dim3 grid((unsigned)((waves + 3) / 4));
     switch (ct) {
@@ -683,8 +768,7 @@ static int conv_bwd(pic_unet* u, int k) {
     PIC_TRY(wgrad_any(s, c.cin / 16, u->dzt, u->xt, c.cout, c.cin, H, W, u->g + c.woff));
     const int chunk = 4096;
     const int parts = (H * W + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_bias_grad, dim3(nb((int64_t)parts * c.cout)), dim3(kT), 0, s, u->dz, c.cout, H, W, chunk,
-                       u->g + c.boff);
+    hipLaunchKernelGGL(k_bias_grad, dim3(parts), dim3(kT), 0, s, u->dz, c.cout, H, W, chunk, u->g + c.boff);
     PIC_HIP(hipGetLastError());
     if (k > 0)
         PIC_TRY((conv_any<0, float>(s, c.cin / 16, u->dz, c.cout, 0, c.cout, u->wflip + c.woff, nullptr, a.g, a.C,
@@ -692,15 +776,26 @@ static int conv_bwd(pic_unet* u, int k) {
     return PIC_OK;
 }
 
-static int unet_backward(pic_unet* u) {
+// loss 0: masked mean squared error against u->tgt; 1: the residual loss
+// (needs u->r0, u->scl from unet_prepare and the grid constants cxd, cyd).
+static int unet_backward(pic_unet* u, int loss_kind = 0, float cxd = 0.f, float cyd = 0.f) {
     hipStream_t s = u->stream;
     const int N = u->N;
+    const unsigned gN = nb((int64_t)N * N);
     hipLaunchKernelGGL(k_zero_f32, dim3(nb((int64_t)u->nparam)), dim3(kT), 0, s, u->g, (int64_t)u->nparam);
     hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(kT), 0, s, u->loss, (int64_t)1);
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, u->count);
-    hipLaunchKernelGGL(k_mask_count, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, u->mask, N * N, u->count);
-    hipLaunchKernelGGL(k_head_bwd, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, u->out, u->tgt, u->mask, u->count,
-                       u->t[D0].p, u->w + u->head_w, u->t[D0].g, u->g + u->head_w, u->g + u->head_b, u->loss, N, N);
+    hipLaunchKernelGGL(k_mask_count, dim3(gN), dim3(kT), 0, s, u->mask, N * N, u->count);
+    if (loss_kind == 0) {
+        hipLaunchKernelGGL(k_loss_mse, dim3(gN), dim3(kT), 0, s, u->out, u->tgt, u->mask, u->count, u->gout, u->loss,
+                           N * N);
+    } else {
+        hipLaunchKernelGGL(k_loss_res, dim3(gN), dim3(kT), 0, s, u->out, u->r0, u->scl, u->mask, u->count, cxd, cyd,
+                           N, u->tgt, u->loss);
+        hipLaunchKernelGGL(k_loss_res_grad, dim3(gN), dim3(kT), 0, s, u->tgt, u->mask, u->count, cxd, cyd, N, u->gout);
+    }
+    hipLaunchKernelGGL(k_head_bwd, dim3(gN), dim3(kT), 0, s, u->gout, u->t[D0].p, u->w + u->head_w, u->t[D0].g,
+                       u->g + u->head_w, u->g + u->head_b, N, N);
     PIC_HIP(hipGetLastError());
     auto up_bwd = [&](int from, int fo, int to, int c) -> int {
         const Tensor& a = u->t[from];
@@ -857,6 +952,7 @@ int pic_unet_create(int n, uint64_t seed, pic_unet** out) {
               hipMalloc(&u->wb, P * 2) == hipSuccess && hipMalloc(&u->wflip, P * 2) == hipSuccess &&
               hipMalloc(&u->out, (size_t)n * n * 4) == hipSuccess && hipMalloc(&u->tgt, (size_t)n * n * 4) == hipSuccess &&
               hipMalloc(&u->mask, (size_t)n * n) == hipSuccess && hipMalloc(&u->loss, 4) == hipSuccess &&
+              hipMalloc(&u->gout, (size_t)n * n * 4) == hipSuccess &&
               hipMalloc(&u->count, 4) == hipSuccess && hipMalloc(&u->r0, (size_t)n * n * 8) == hipSuccess &&
               hipMalloc(&u->scl, 16) == hipSuccess &&
               hipMalloc(&u->dz, maxz * 2) == hipSuccess && hipMalloc(&u->dzt, maxz * 2) == hipSuccess &&
@@ -891,8 +987,8 @@ void pic_unet_destroy(pic_unet* u) {
         if (x.p) (void)hipFree(x.p);
         if (x.g) (void)hipFree(x.g);
     }
-    void* ptrs[] = {u->w, u->g, u->m, u->v, u->wb, u->wflip, u->out, u->tgt, u->mask, u->loss, u->count, u->dz,
-                    u->dzt, u->xt, u->r0, u->scl};
+    void* ptrs[] = {u->w, u->g, u->m, u->v, u->wb, u->wflip, u->out, u->tgt, u->gout, u->mask, u->loss, u->count,
+                    u->dz, u->dzt, u->xt, u->r0, u->scl};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (u->loss_host) (void)hipHostFree(u->loss_host);
@@ -966,6 +1062,33 @@ int pic_unet_train_step(pic_unet* u, const float* x, const float* tgt, const uin
     PIC_HIP(hipMemcpyAsync(u->tgt, tgt, (size_t)N * N * 4, hipMemcpyHostToDevice, u->stream));
     PIC_HIP(hipMemcpyAsync(u->mask, mask, (size_t)N * N, hipMemcpyHostToDevice, u->stream));
     PIC_TRY(unet_backward(u));
+    PIC_HIP(hipMemcpyAsync(u->loss_host, u->loss, 4, hipMemcpyDeviceToHost, u->stream));
+    if (lr > 0.f) PIC_TRY(adam_step(u, lr));
+    PIC_HIP(hipStreamSynchronize(u->stream));
+    if (loss) *loss = *u->loss_host;
+    return PIC_OK;
+}
+
+// Training step with the residual loss on host data: r: [n][n] scaled
+// residual (max |r| = 1 or less), mask: [n][n] unknown nodes, cxd, cyd: the
+// operator's off-diagonals over its diagonal.
+int pic_unet_train_step_residual(pic_unet* u, const float* x, const double* r, const uint8_t* mask, float cxd,
+                                 float cyd, float lr, float* loss) {
+    if (!u || !x || !r || !mask) {
+        set_error("pic_unet_train_step_residual: null argument");
+        return PIC_ERR_ARG;
+    }
+    const int N = u->N;
+    std::vector<float> o((size_t)N * N);
+    PIC_TRY(pic_unet_forward(u, x, o.data()));
+    const double one = 1.0;
+    unsigned long long sc[2];
+    std::memcpy(&sc[0], &one, 8);
+    sc[1] = sc[0];
+    PIC_HIP(hipMemcpyAsync(u->r0, r, (size_t)N * N * 8, hipMemcpyHostToDevice, u->stream));
+    PIC_HIP(hipMemcpyAsync(u->scl, sc, 16, hipMemcpyHostToDevice, u->stream));
+    PIC_HIP(hipMemcpyAsync(u->mask, mask, (size_t)N * N, hipMemcpyHostToDevice, u->stream));
+    PIC_TRY(unet_backward(u, 1, cxd, cyd));
     PIC_HIP(hipMemcpyAsync(u->loss_host, u->loss, 4, hipMemcpyDeviceToHost, u->stream));
     if (lr > 0.f) PIC_TRY(adam_step(u, lr));
     PIC_HIP(hipStreamSynchronize(u->stream));
@@ -1048,14 +1171,16 @@ int unet_train_pair(pic_ctx* c, pic_unet* u, const double* phi_old, float lr, fl
     const int N = u->N, gx = c->g.gx;
     hipStream_t s = c->stream;
     PIC_TRY(unet_prepare(c, u, phi_old));
-    const double diag = 2.0 / (c->g.dx * c->g.dx) + 2.0 / (c->g.dy * c->g.dy);
+    const double cx = 1.0 / (c->g.dx * c->g.dx), cy = 1.0 / (c->g.dy * c->g.dy);
+    const double diag = 2 * (cx + cy);
+    // the unknown-node mask (the MSE target it also writes is not used)
     hipLaunchKernelGGL(k_unet_target, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, c->phi, phi_old, c->dir, gx, N, diag,
                        u->scl, u->tgt, u->mask);
     PIC_HIP(hipGetLastError());
     hipStream_t keep = u->stream;
     u->stream = s;
     int rc = unet_forward(u);
-    if (rc == PIC_OK) rc = unet_backward(u);
+    if (rc == PIC_OK) rc = unet_backward(u, 1, (float)(cx / diag), (float)(cy / diag));
     if (rc == PIC_OK) rc = adam_step(u, lr);
     u->stream = keep;
     PIC_TRY(rc);

@@ -124,6 +124,8 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "pic_unet_train_step": (i32, [vp, fp, fp, C.POINTER(C.c_uint8), C.c_float, fp]),
         "pic_unet_mfma_probe": (i32, [fp, fp, fp]),
         "pic_unet_benchmark": (i32, [vp, i32, fp, fp]),
+        "pic_unet_train_step_residual": (i32, [vp, fp, dp, C.POINTER(C.c_uint8), C.c_float, C.c_float, C.c_float,
+                                               fp]),
         "pic_learned_unet": (vp, [vp]),
         "pic_ensemble_create": (i32, [C.POINTER(vp), i32, C.POINTER(vp)]),
         "pic_ensemble_destroy": (None, [vp]),
@@ -429,6 +431,15 @@ class UNet:
         out = np.empty((self.n, self.n), np.float32)
         _chk(self.lib.pic_unet_forward(self.h, _fp(x), _fp(out)))
         return out
+
+    def train_step_residual(self, x, r, mask, cxd, cyd, lr):
+        x = np.ascontiguousarray(x, np.float32)
+        r = np.ascontiguousarray(r, np.float64)
+        m = np.ascontiguousarray(mask, np.uint8)
+        loss = np.zeros(1, np.float32)
+        _chk(self.lib.pic_unet_train_step_residual(self.h, _fp(x), _dp(r), m.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                   float(cxd), float(cyd), float(lr), _fp(loss)))
+        return float(loss[0])
 
     def train_step(self, x, target, mask, lr):
         x = np.ascontiguousarray(x, np.float32)

@@ -262,6 +262,11 @@ int     pic_unet_forward(pic_unet* u, const float* x, float* out);
  * error before the step; gradients stay readable by pic_unet_get_grads. */
 int     pic_unet_train_step(pic_unet* u, const float* x, const float* target, const uint8_t* mask, float lr,
                             float* loss);
+/* The same with the loss used online: mean over mask of
+ * (r - (A/diag) d)^2, d = output on masked nodes (0 elsewhere), A/diag with unit
+ * diagonal and off-diagonals -cxd (x neighbours), -cyd (y neighbours). */
+int     pic_unet_train_step_residual(pic_unet* u, const float* x, const double* r, const uint8_t* mask, float cxd,
+                                     float cyd, float lr, float* loss);
 /* Average device ms of a forward pass and of forward + backward over `reps`
  * runs on the data currently on the device (no weight update). */
 int     pic_unet_benchmark(pic_unet* u, int reps, float* fwd_ms, float* train_ms);

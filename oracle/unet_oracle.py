@@ -205,6 +205,29 @@ def loss_and_grads(params, x, target, mask, bf=True):
     return loss, grads
 
 
+def residual_loss_and_grads(params, x, r, mask, cxd, cyd, bf=True):
+    """The online loss: mean over mask of (r - (A/diag) d)^2 with d = output on
+    masked nodes (zero elsewhere and outside the image), A/diag = unit diagonal,
+    -cxd on x neighbours, -cyd on y neighbours.  Returns (loss, gradient)."""
+    out, _ = forward(params, x, bf)
+
+    def A(v):
+        p = np.pad(v, 1)
+        return v - cxd * (p[1:-1, :-2] + p[1:-1, 2:]) - cyd * (p[:-2, 1:-1] + p[2:, 1:-1])
+
+    m = mask.astype(np.float64)
+    d = out * m
+    res = (r - A(d)) * m
+    cnt = max(int(m.sum()), 1)
+    loss = float((res * res).sum() / cnt)
+    g_out = -2.0 * A(res) * m / cnt
+    # reuse the MSE backward with a target whose gradient equals g_out:
+    # 2 (out - t) / cnt = g_out on the mask  ->  t = out - g_out cnt / 2
+    t = out - 0.5 * cnt * g_out
+    _, grads = loss_and_grads(params, x, t, mask, bf)
+    return loss, grads
+
+
 def adam(params, grads, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
     """One Adam step (bias-corrected step size), in place on float32 arrays."""
     m[:] = b1 * m + (1 - b1) * grads

@@ -132,3 +132,20 @@ def test_learned_guess_in_the_solver(hip_mod):
     assert rb <= ra * (1 + 1e-9)       # the guess is used only when it starts lower
     a.close()
     b.close()
+
+
+def test_unet_residual_loss_gradients_match_oracle(hip_mod):
+    import unet_oracle as U
+    net = hip_mod.UNet(N, seed=10)
+    p = _params(net, 5)
+    x = _inputs(5)
+    rng = np.random.default_rng(6)
+    r = rng.uniform(-1, 1, (N, N))
+    mask = np.ones((N, N), np.uint8)
+    mask[0, :] = mask[:, 0] = 0
+    loss = net.train_step_residual(x, r, mask, 0.25, 0.25, 0.0)
+    g = net.grads().astype(np.float64)
+    rloss, rg = U.residual_loss_and_grads(p, x.astype(np.float64), r, mask.astype(np.float64), 0.25, 0.25)
+    assert loss == pytest.approx(rloss, rel=1e-2)
+    assert _rel(g, rg) < 5e-2
+    net.close()

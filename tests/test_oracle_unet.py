@@ -88,3 +88,27 @@ def test_adam_first_step_is_signed_lr():
     v = np.zeros(3, np.float32)
     U.adam(p, g, m, v, 1, 1e-2)
     assert np.allclose(p, [1.0 - 1e-2, -2.0 + 1e-2, 0.5 - 1e-2], atol=1e-6)
+
+
+@pytest.mark.slow
+def test_residual_loss_gradient_matches_finite_differences():
+    rng = np.random.default_rng(6)
+    N = 16
+    _, hw, hb, n = U.layout()
+    params = rng.normal(0, 0.15, n)
+    params[hw:hw + 16] = rng.normal(0, 0.5, 16)
+    x = rng.normal(size=(N, N, 2))
+    r = rng.normal(size=(N, N))
+    mask = np.ones((N, N))
+    mask[0, :] = mask[:, 0] = mask[-1, :] = 0
+    cxd = cyd = 0.25
+    loss, g = U.residual_loss_and_grads(params, x, r, mask, cxd, cyd, bf=False)
+    bad = []
+    for k in [hw + 1, hb, 5, 200, n - 100]:
+        p1 = params.copy(); p1[k] += 1e-6
+        p2 = params.copy(); p2[k] -= 1e-6
+        fd = (U.residual_loss_and_grads(p1, x, r, mask, cxd, cyd, bf=False)[0]
+              - U.residual_loss_and_grads(p2, x, r, mask, cxd, cyd, bf=False)[0]) / 2e-6
+        if abs(fd - g[k]) > 1e-5 * max(1.0, abs(fd)) + 1e-8:
+            bad.append((k, fd, g[k]))
+    assert not bad, bad
