@@ -37,9 +37,11 @@ COMMON = [
 EXTRA = {
     "pic_particles.hip": ["-ffp-contract=off"],
     "pic_ensemble.hip": ["-ffp-contract=off"],
+    "pic_inject.hip": ["-ffp-contract=off"],
 }
 
-SOURCES = ["pic_particles.hip", "pic_poisson.hip", "pic_unet.hip", "pic_ensemble.hip", "pic_host.cpp",
+SOURCES = ["pic_particles.hip", "pic_poisson.hip", "pic_unet.hip", "pic_ensemble.hip", "pic_inject.hip",
+           "pic_host.cpp",
            "pic_config.cpp", "pic_dropin.cpp"]
 
 

@@ -148,7 +148,8 @@ int sycl_pic_init(const char* dev_type, int wg, int ps, const char* input1, cons
         const pic::PicdatSpecies& row = d.species[s];
         int64_t n = (int64_t)row.sim;
         int id = 0;
-        rc = pic_add_species(g.ctx, row.charge, row.mass, row.sim, row.actual, n, &id);
+        // storage for five times the initial particles (space, particles.cpp:183)
+        rc = pic_add_species(g.ctx, row.charge, row.mass, row.sim, row.actual, 5 * n, &id);
         if (rc != PIC_OK) {
             report(rc, "sycl_pic_init: species");
             sycl_pic_finalize();
@@ -168,6 +169,22 @@ int sycl_pic_init(const char* dev_type, int wg, int ps, const char* input1, cons
         rc = pic_upload_particles(g.ctx, id, n, x.data(), y.data(), vx.data(), vy.data(), vz.data());
         if (rc != PIC_OK) {
             report(rc, "sycl_pic_init: upload");
+            sycl_pic_finalize();
+            return rc;
+        }
+    }
+    if (d.num_type >= 2) {
+        // generate_new_electrons gets electronTemp as an int (utils.cpp:381);
+        // generate_pairs uses the double globals (utils.cpp:646,664)
+        pic_injection inj;
+        pic_injection_default(&inj);
+        inj.te_reinject_eV = (double)(int)d.Te;
+        inj.te_eV = d.Te;
+        inj.ti_eV = d.Ti;
+        inj.seed = seed;
+        rc = pic_set_injection(g.ctx, &inj);
+        if (rc != PIC_OK) {
+            report(rc, "sycl_pic_init: injection");
             sycl_pic_finalize();
             return rc;
         }
@@ -268,13 +285,14 @@ double sycl_pic_cleanup_map(int species) {
 }
 
 double sycl_pic_generate_new_particles(void) {
-    // Injection (generate_new_electrons / generate_pairs) is not part of this build.
-    static bool warned = false;
-    if (!warned) {
-        std::fprintf(stderr, "sycl_pic_generate_new_particles: particle injection is not implemented; no particles added\n");
-        warned = true;
-    }
-    return 0.0;
+    // sycl_pic_ctypes.cpp:212-236: re-inject electrons for the x < 0 exits
+    // since the last call, add this step's ionization pairs
+    if (!g.ctx || g.dat.num_type < 2) return 0.0;
+    auto t0 = std::chrono::high_resolution_clock::now();
+    int rc = pic_inject(g.ctx);
+    if (rc != PIC_OK) report(rc, "sycl_pic_generate_new_particles");
+    auto t1 = std::chrono::high_resolution_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
 }
 
 void sycl_pic_print_diagnostics(const char* dir, int iteration) {

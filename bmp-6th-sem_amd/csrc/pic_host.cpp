@@ -91,6 +91,7 @@ static void free_species(pic::Species& s) {
 extern "C" void pic_destroy(pic_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    inject_free(c);
     for (auto& s : c->sp) free_species(s);
     poisson_free(c);
     if (c->unet) pic_unet_destroy(c->unet);
@@ -685,8 +686,30 @@ extern "C" int pic_step(pic_ctx* c, int iteration, pic_solve_stats* st) {
     for (int s = 0; s < (int)c->sp.size(); ++s) PIC_TRY(launch_deposit(c, s));
     PIC_TRY(poisson_solve(c, iteration, st));
     PIC_TRY(launch_efield(c));
-    for (int s = 0; s < (int)c->sp.size(); ++s) PIC_TRY(launch_move(c, s));
-    return PIC_OK;
+    if (!c->inj.on) {
+        for (int s = 0; s < (int)c->sp.size(); ++s) PIC_TRY(launch_move(c, s));
+        return PIC_OK;
+    }
+    // with injection (main.cpp:232-245): push every species, stage the new
+    // particles (their count depends on both species' exits), count them into
+    // the cells, then scan and scatter pushed and new particles together
+    const int ns = (int)c->sp.size();
+    for (int s = 0; s < ns; ++s) PIC_TRY(move_push(c, s));
+    hipEvent_t ev;
+    PIC_TRY(timer_begin(c, PIC_STAGE_BIN, &ev));
+    PIC_TRY(inject_generate(c));
+    PIC_TRY(inject_count(c, 0));
+    PIC_TRY(inject_count(c, 1));
+    PIC_TRY(timer_end(c, PIC_STAGE_BIN, ev));
+    for (int s = 0; s < ns; ++s) {
+        PIC_TRY(move_finish(c, s));
+        if (s < 2) {
+            PIC_TRY(timer_begin(c, PIC_STAGE_SCATTER, &ev));
+            PIC_TRY(inject_scatter(c, s, c->sp[s].cur));
+            PIC_TRY(timer_end(c, PIC_STAGE_SCATTER, ev));
+        }
+    }
+    return inject_publish(c);
 }
 
 static double* grid_ptr(pic_ctx* c, int which) {
@@ -754,6 +777,64 @@ extern "C" int pic_get_absorbed(pic_ctx* c, int s, int64_t* left, int64_t* total
     PIC_HIP(hipMemcpy(v, c->sp[s].absorbed, sizeof(v), hipMemcpyDeviceToHost));
     if (left) *left = (int64_t)v[0];
     if (total) *total = (int64_t)v[1];
+    return PIC_OK;
+}
+
+extern "C" void pic_injection_default(pic_injection* q) {
+    if (!q) return;
+    q->reinject = 1;
+    q->pairs = 1;
+    q->x_inject = 0.024;          // utils.cpp:400
+    q->te_reinject_eV = 10;       // electronTemp (types.cpp:89) as the int argument of utils.cpp:381
+    q->te_eV = 10;
+    q->ti_eV = 0.026;             // ionTemp (types.cpp:90)
+    q->x11 = 0.0025;              // types.cpp:103-105
+    q->x22 = 0.01;
+    q->xm = 0.00625;
+    q->ionimax = 5.23e23;         // utils.cpp:619
+    q->max_reinject = 1 << 20;
+    q->seed = 1;
+}
+
+extern "C" int pic_set_injection(pic_ctx* c, const pic_injection* cfg) {
+    if (!c) {
+        set_error("pic_set_injection: null context");
+        return PIC_ERR_ARG;
+    }
+    return inject_setup(c, cfg);
+}
+
+extern "C" int pic_inject(pic_ctx* c) {
+    if (!c) {
+        set_error("pic_inject: null context");
+        return PIC_ERR_ARG;
+    }
+    PIC_TRY(inject_append_rebin(c));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+extern "C" int pic_injection_target(pic_ctx* c, int64_t* target, int64_t* attempts) {
+    if (!c) {
+        set_error("pic_injection_target: null context");
+        return PIC_ERR_ARG;
+    }
+    if (target) *target = c->inj.on ? c->inj.target : 0;
+    if (attempts) *attempts = c->inj.on ? c->inj.attempts : 0;
+    return PIC_OK;
+}
+
+extern "C" int pic_injection_stats(pic_ctx* c, int64_t* injections, int64_t* reinjected, int64_t* pairs,
+                                   int64_t* dropped, int64_t* last_electrons, int64_t* last_ions) {
+    if (!c) {
+        set_error("pic_injection_stats: null context");
+        return PIC_ERR_ARG;
+    }
+    int64_t v[6];
+    PIC_TRY(inject_stats(c, v));
+    int64_t* outs[6] = {injections, reinjected, pairs, dropped, last_electrons, last_ions};
+    for (int i = 0; i < 6; ++i)
+        if (outs[i]) *outs[i] = v[i];
     return PIC_OK;
 }
 

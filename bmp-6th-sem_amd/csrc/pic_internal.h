@@ -109,6 +109,21 @@ struct Geometry {
     double potential;
 };
 
+// Particle injection state (pic_inject.hip).
+struct Injection {
+    int on = 0;
+    pic_injection cfg{};
+    int target = 0, attempts = 0;   // pairs per step; attempts cap
+    int ecap = 0;                   // re-injected electrons staged per step
+    ParticleBuf stage[2];           // staged new particles per species (storage layout)
+    int32_t* key[2] = {nullptr, nullptr};
+    int32_t* rank[2] = {nullptr, nullptr};
+    long long* state = nullptr;     // device counters
+    long long* host = nullptr;      // pinned copy of the counters and live counts
+    hipEvent_t ev = nullptr;        // recorded after the copy
+    int pending = 0;                // host copy in flight: n_upper not yet raised
+};
+
 } // namespace pic
 
 struct pic_ctx {
@@ -160,6 +175,7 @@ struct pic_ctx {
     int* flags_host = nullptr;      // pinned
     int scan_nb = 0;
     int32_t* scan_bsum = nullptr;
+    pic::Injection inj;
     // timing
     int timing = 0;
     pic::StageTimer timers[PIC_NSTAGES];
@@ -174,7 +190,20 @@ int launch_bin(pic_ctx* c, int s);
 int launch_deposit(pic_ctx* c, int s);
 int launch_push(pic_ctx* c, int s);
 int launch_move(pic_ctx* c, int s);
+int move_push(pic_ctx* c, int s);                 // launch_move in phases, for injection between them
+int move_finish(pic_ctx* c, int s);
 int launch_zero_f64(pic_ctx* c, double* p, int64_t n);
+
+// injection (pic_inject.hip)
+int inject_setup(pic_ctx* c, const pic_injection* cfg);
+void inject_free(pic_ctx* c);
+int inject_settle(pic_ctx* c);                    // raise n_upper by the last injection
+int inject_generate(pic_ctx* c);                  // staged particles of this step
+int inject_count(pic_ctx* c, int s);              // their slots in the cell counts
+int inject_scatter(pic_ctx* c, int s, int dst);   // and into the sorted buffer
+int inject_publish(pic_ctx* c);                   // counters to the host, asynchronously
+int inject_append_rebin(pic_ctx* c);              // standalone injection into binned storage
+int inject_stats(pic_ctx* c, int64_t* out);       // injections, reinjected, pairs, dropped, last e, last i
 int launch_decode(pic_ctx* c, int s, double* x, double* y, double* vx, double* vy, double* vz);
 
 // Poisson (pic_poisson.hip)

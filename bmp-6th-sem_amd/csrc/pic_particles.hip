@@ -308,6 +308,7 @@ int launch_zero_f64(pic_ctx* c, double* p, int64_t n) {
 
 int launch_encode_f64(pic_ctx* c, int s, int64_t n, const double* x, const double* y,
                       const double* vx, const double* vy, const double* vz) {
+    PIC_TRY(inject_settle(c));
     Species& sp = c->sp[s];
     if (n > 0) {
         if (c->p.layout == PIC_LAYOUT_F64)
@@ -326,6 +327,7 @@ int launch_encode_f64(pic_ctx* c, int s, int64_t n, const double* x, const doubl
 }
 
 int launch_key_from_cell(pic_ctx* c, int s, int64_t n) {
+    PIC_TRY(inject_settle(c));
     Species& sp = c->sp[s];
     if (n > 0) {
         hipLaunchKernelGGL(k_copy_i32, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream,
@@ -352,6 +354,7 @@ static int scan_counts(pic_ctx* c, Species& sp, int dst) {
 // Counting sort of the current storage by sp.key (upload / pic_bin / after an
 // in-place push).
 int launch_bin(pic_ctx* c, int s) {
+    PIC_TRY(inject_settle(c));
     Species& sp = c->sp[s];
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_BIN, &ev));
@@ -441,6 +444,7 @@ int launch_deposit(pic_ctx* c, int s) {
                                    c->stream, sp.buf[sp.cur], c->g, a, tpr, (int)ntiles, sp.start,
                                    (long long*)sp.q_charge, (long long*)sp.q_energy);
         } else {
+            PIC_TRY(inject_settle(c));   // the per-particle grid needs the live-count bound
             const int64_t nacc = 4 * (int64_t)c->g.gp;
             hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(nacc)), dim3(kBlock), 0, c->stream,
                                (unsigned long long*)sp.q_charge, nacc);
@@ -469,6 +473,7 @@ int launch_deposit(pic_ctx* c, int s) {
 
 // In-place push, no re-bin (keys left in sp.key).
 int launch_push(pic_ctx* c, int s) {
+    PIC_TRY(inject_settle(c));
     Species& sp = c->sp[s];
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_PUSH, &ev));
@@ -509,16 +514,18 @@ static void scatter_kernel(pic_ctx* c, Species& sp, const SpeciesDev& d, int src
                        sp.ndev + src, sp.buf[src], sp.buf[dst], c->g, d, c->fld, sp.rank, sp.start);
 }
 
-// Gather + Boris + walls + re-bin of species s: tile push/count, prefix
-// scan, recomputing scatter into the other buffer.
-int launch_move(pic_ctx* c, int s) {
+// Gather + Boris + walls + re-bin of species s: tile push/count (move_push),
+// prefix scan and recomputing scatter into the other buffer (move_finish).
+// Injection runs between the two phases: its particles are counted into the
+// same cells before the scan.
+int move_push(pic_ctx* c, int s) {
+    PIC_TRY(inject_settle(c));
     Species& sp = c->sp[s];
     if (!sp.sorted) PIC_TRY(launch_bin(c, s));   // tiles need cell-sorted storage
     SpeciesDev d = species_dev(c, sp);
     const int tpr = tiles_per_row(c);
     const unsigned ntiles = (unsigned)(tpr * c->g.gy);
     const bool f64 = c->p.layout == PIC_LAYOUT_F64;
-    const int src = sp.cur, dst = sp.cur ^ 1;
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_PUSH, &ev));
     if (sp.n_upper > 0) {
@@ -529,6 +536,15 @@ int launch_move(pic_ctx* c, int s) {
         PIC_HIP(hipGetLastError());
     }
     PIC_TRY(timer_end(c, PIC_STAGE_PUSH, ev));
+    return PIC_OK;
+}
+
+int move_finish(pic_ctx* c, int s) {
+    Species& sp = c->sp[s];
+    SpeciesDev d = species_dev(c, sp);
+    const bool f64 = c->p.layout == PIC_LAYOUT_F64;
+    const int src = sp.cur, dst = sp.cur ^ 1;
+    hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_BIN, &ev));
     PIC_TRY(scan_counts(c, sp, dst));
     PIC_TRY(timer_end(c, PIC_STAGE_BIN, ev));
@@ -545,6 +561,11 @@ int launch_move(pic_ctx* c, int s) {
     sp.sorted = 1;
     PIC_TRY(timer_end(c, PIC_STAGE_SCATTER, ev));
     return PIC_OK;
+}
+
+int launch_move(pic_ctx* c, int s) {
+    PIC_TRY(move_push(c, s));
+    return move_finish(c, s);
 }
 
 } // namespace pic

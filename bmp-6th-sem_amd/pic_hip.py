@@ -64,6 +64,16 @@ class SolveStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Injection(C.Structure):
+    _fields_ = [
+        ("reinject", C.c_int), ("pairs", C.c_int),
+        ("x_inject", C.c_double), ("te_reinject_eV", C.c_double),
+        ("te_eV", C.c_double), ("ti_eV", C.c_double),
+        ("x11", C.c_double), ("x22", C.c_double), ("xm", C.c_double),
+        ("ionimax", C.c_double), ("max_reinject", C.c_int64), ("seed", C.c_uint64),
+    ]
+
+
 _lib = None
 
 
@@ -135,6 +145,11 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "pic_ensemble_steps_done": (i32, [vp, lp]),
         "pic_learned_enable": (i32, [vp, i32, i32, C.c_float, C.c_uint64]),
         "pic_learned_stats": (i32, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), fp]),
+        "pic_injection_default": (None, [C.POINTER(Injection)]),
+        "pic_set_injection": (i32, [vp, C.POINTER(Injection)]),
+        "pic_inject": (i32, [vp]),
+        "pic_injection_target": (i32, [vp, lp, lp]),
+        "pic_injection_stats": (i32, [vp, lp, lp, lp, lp, lp, lp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -368,6 +383,36 @@ class Context:
         l = np.zeros(1, np.float32)
         _chk(self.lib.pic_learned_stats(self.h, C.byref(a), C.byref(b), C.byref(c), _fp(l)))
         return {"trained": a.value, "tried": b.value, "used": c.value, "loss": float(l[0])}
+
+    # -- injection
+    def set_injection(self, enable=True, **kw):
+        """Turn injection in step() on with the reference values overridden
+        by `kw` (fields of pic_injection), or off."""
+        if not enable:
+            _chk(self.lib.pic_set_injection(self.h, None))
+            return None
+        cfg = Injection()
+        self.lib.pic_injection_default(C.byref(cfg))
+        for k, v in kw.items():
+            if not hasattr(cfg, k):
+                raise KeyError(k)
+            setattr(cfg, k, v)
+        _chk(self.lib.pic_set_injection(self.h, C.byref(cfg)))
+        return {k: getattr(cfg, k) for k, _ in cfg._fields_}
+
+    def inject(self):
+        _chk(self.lib.pic_inject(self.h))
+
+    def injection_target(self):
+        t, a = C.c_int64(), C.c_int64()
+        _chk(self.lib.pic_injection_target(self.h, C.byref(t), C.byref(a)))
+        return t.value, a.value
+
+    def injection_stats(self):
+        v = [C.c_int64() for _ in range(6)]
+        _chk(self.lib.pic_injection_stats(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(("injections", "reinjected", "pairs", "dropped", "last_electrons", "last_ions"),
+                        (x.value for x in v)))
 
     def stage_times(self):
         ms = np.zeros(len(STAGES), np.float64)

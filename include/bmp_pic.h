@@ -224,6 +224,55 @@ int  pic_set_grid(pic_ctx* ctx, int which, const double* in);
  * absorbed particles.  Either pointer may be NULL. */
 int  pic_get_absorbed(pic_ctx* ctx, int s, int64_t* left, int64_t* total);
 
+/* ------------------------------------------------------------ injection */
+/* New particles after the push of every species (main.cpp:232-245), species 0
+ * the electrons and species 1 the ions:
+ *   re-injection (generate_new_electrons, utils.cpp:374-499): as many
+ *     electrons as left through x < 0 minus the ions that did (never
+ *     negative), at x = x_inject, y uniform in [0, height), Maxwellian at
+ *     te_reinject_eV (the reference passes its electron temperature as an int,
+ *     utils.cpp:381);
+ *   pairs (generate_pairs, utils.cpp:610-703): target = ceil(height dt (2/PI)
+ *     (x22 - x11) ionimax / 1.67e6) per step with PI = 3.141592654; attempt a
+ *     (a < max(10 target, 100)) draws x = xm + asin(2u - 1)(x22 - x11)/PI and is
+ *     accepted when 0 < x < width; the first `target` accepted attempts give an
+ *     electron (te_eV) and an ion (ti_eV) at the same (x, y).
+ * Each velocity component is normal with variance |q/m| T.  Draws come from a
+ * counter-based generator keyed by (seed, injection count, stream, index)
+ * instead of the reference's time-seeded std::rand / ran2: reruns give the
+ * same particles.  The reference exits when the storage would overflow
+ * (utils.cpp:388-393,685-688); here the excess is not added and is counted
+ * in `dropped`, as are re-injected electrons beyond max_reinject per step. */
+typedef struct pic_injection {
+    int      reinject;        /* 1: re-inject electrons that left through x < 0  */
+    int      pairs;           /* 1: generate ionization pairs                    */
+    double   x_inject;        /* re-injection plane, m (utils.cpp:400: 0.024)    */
+    double   te_reinject_eV;  /* temperature of re-injected electrons            */
+    double   te_eV, ti_eV;    /* pair temperatures (electronTemp, ionTemp)       */
+    double   x11, x22, xm;    /* ionization zone, m (types.cpp:103-105)          */
+    double   ionimax;         /* peak ionization rate (utils.cpp:619: 5.23e23)   */
+    int64_t  max_reinject;    /* re-injected electrons staged per step           */
+    uint64_t seed;
+} pic_injection;
+
+/* Reference values: x_inject 0.024, te 10, ti 0.026, x11 0.0025, x22 0.01,
+ * xm 0.00625, ionimax 5.23e23, both kinds on, max_reinject 1 << 20, seed 1. */
+void pic_injection_default(pic_injection* cfg);
+/* Enable injection in pic_step (cfg != NULL) or turn it off (NULL).  Needs two
+ * species.  Counters of exits start from the current absorption counters. */
+int  pic_set_injection(pic_ctx* ctx, const pic_injection* cfg);
+/* One injection now into the binned storage of both species (the drop-in's
+ * generate_new_particles, sycl_pic_ctypes.cpp:212-236): the new particles are
+ * appended and both species re-binned. */
+int  pic_inject(pic_ctx* ctx);
+/* Pairs per step of the configuration (the ceil above) and the attempts cap. */
+int  pic_injection_target(pic_ctx* ctx, int64_t* target, int64_t* attempts);
+/* Injections so far, re-injected electrons, pairs, dropped particles in all,
+ * and the particles added by the last injection to species 0 and 1.  Any
+ * pointer may be NULL. */
+int  pic_injection_stats(pic_ctx* ctx, int64_t* injections, int64_t* reinjected, int64_t* pairs,
+                         int64_t* dropped, int64_t* last_electrons, int64_t* last_ions);
+
 /* Block until all work of the context is done. */
 int  pic_sync(pic_ctx* ctx);
 /* Enable per-stage HIP-event timing (0/1), and read the accumulated device

@@ -110,6 +110,36 @@ def test_python_driver_loop_matches_oracle(hip_mod, oracle_mod, tmp_path, monkey
         sim.close()
 
 
+@pytest.mark.gpu
+def test_driver_loop_with_generate_new_particles(hip_mod, tmp_path, monkeypatch):
+    """main.cpp's loop order (movers, then generate_new_particles): each call
+    adds this step's ionization pairs to both species; the re-injection plane
+    x = 0.024 m lies outside this 6.4 mm domain, so re-injected electrons are
+    dropped as the reference's out-of-grid insert drops them (utils.cpp:456-461)."""
+    import inject_oracle as IO
+
+    (tmp_path / "M_PICDAT.DAT").write_text(_picdat())
+    (tmp_path / "geometry.conf").write_text(_geometry())
+    monkeypatch.chdir(tmp_path)
+    import sycl_pic
+
+    target, _ = IO.pair_target(NX * DX, DT)
+    sycl_pic.init("gpu", 256, 8, "electronParticle.inp", "ionParticle.inp")
+    try:
+        for it in range(3):
+            sycl_pic.zero_rho()
+            for sp in range(2):
+                sycl_pic.charge_deposition(sp)
+            sycl_pic.poisson_solve(it)
+            for sp in range(2):
+                sycl_pic.move_particles(sp, it)
+            ne, ni = sycl_pic.get_particle_counts()
+            assert sycl_pic.generate_new_particles() >= 0
+            assert sycl_pic.get_particle_counts() == (ne + target, ni + target)
+    finally:
+        sycl_pic.finalize()
+
+
 def test_module_surface_matches_reference_table(hip_mod):
     """Every method of the reference module table (sycl_pic_module.cpp:386-410)."""
     import sycl_pic
