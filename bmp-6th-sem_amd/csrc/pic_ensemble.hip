@@ -689,9 +689,10 @@ int pic_ensemble_create(pic_ctx** runs, int nruns, pic_ensemble** out) {
     for (int r = 0; r < nruns; ++r) {
         pic_ctx* c = runs[r];
         if (!c || c->sp.size() != 2 || c->p.layout != PIC_LAYOUT_CELL32 || c->p.bx != 0.0 || c->p.by != 0.0 ||
-            c->p.poisson_precision != 0 || c->unet || c->inj.on || (int)c->mg.size() > kEnsMaxLev) {
+            c->p.poisson_precision != 0 || c->unet || c->inj.on || c->p.periodic ||
+            (int)c->mg.size() > kEnsMaxLev) {
             set_error("pic_ensemble_create: run %d must have two species, CELL32 storage, Bz-only field, "
-                      "FP64 Poisson, no learned guess, no injection", r);
+                      "FP64 Poisson, no learned guess, no injection, no periodic y", r);
             return PIC_ERR_UNSUPPORTED;
         }
         if (c->p.deposit_energy != runs[0]->p.deposit_energy) {

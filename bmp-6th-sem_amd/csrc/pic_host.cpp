@@ -164,9 +164,9 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
         set_error("pic_create: grid of %d x %d cells exceeds the 16.7M-node limit", p->NX, p->NY);
         return PIC_ERR_ARG;
     }
-    if (p->periodic) {
-        set_error("pic_create: periodic-y boundaries are not implemented in this build");
-        return PIC_ERR_UNSUPPORTED;
+    if (p->periodic != 0 && p->periodic != 1) {
+        set_error("pic_create: periodic must be 0 or 1");
+        return PIC_ERR_ARG;
     }
     if (p->layout != PIC_LAYOUT_F64 && p->layout != PIC_LAYOUT_CELL32) {
         set_error("pic_create: unknown layout %d", p->layout);
@@ -194,6 +194,7 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
     c->g.inv_dy = 1.0 / c->g.dy;
     c->g.nx = p->NX;
     c->g.ny = p->NY;
+    c->g.periodic = p->periodic;
     c->g.rcp_gx = 1.0f / (float)c->g.gx;
     c->G = c->g.gx * c->g.gy;
     c->g.gp = c->G + c->g.gx + 2;
@@ -261,9 +262,10 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
             return fail(PIC_ERR_HIP);
         }
     }
-    // default geometry: four grounded walls (M_PICDAT/geometry.conf default)
+    // default geometry: four grounded walls (M_PICDAT/geometry.conf default);
+    // periodic y keeps the two x walls only
     const double walls[20] = {0, 0, 0, 1, 0, 1, 0, 1, 1, 0, 0, 1, 1, 1, 0, 0, 0, 1, 0, 0};
-    int rc = apply_geometry(c, walls, 4);
+    int rc = apply_geometry(c, walls, p->periodic ? 2 : 4);
     if (rc != PIC_OK) return fail(rc);
     // E = 0 initially (the reference computes E only after the first solve)
     std::vector<double4> f(G);
@@ -780,6 +782,15 @@ extern "C" int pic_get_absorbed(pic_ctx* c, int s, int64_t* left, int64_t* total
     return PIC_OK;
 }
 
+extern "C" int pic_kinetic_energy(pic_ctx* c, int s, double* energy) {
+    PIC_TRY(check_species(c, s, "pic_kinetic_energy"));
+    if (!energy) {
+        set_error("pic_kinetic_energy: null output");
+        return PIC_ERR_ARG;
+    }
+    return launch_kinetic_energy(c, s, energy);
+}
+
 extern "C" void pic_injection_default(pic_injection* q) {
     if (!q) return;
     q->reinject = 1;
@@ -894,6 +905,10 @@ extern "C" int pic_learned_enable(pic_ctx* c, int train_steps, int min_trained, 
     if (!c || train_steps < 0 || min_trained < 0 || !(lr > 0.f)) {
         set_error("pic_learned_enable: invalid argument");
         return PIC_ERR_ARG;
+    }
+    if (c->p.periodic) {
+        set_error("pic_learned_enable: the learned guess is trained on the non-periodic operator");
+        return PIC_ERR_UNSUPPORTED;
     }
     const int n = (std::min(c->p.NX, c->p.NY) / 128) * 128;
     if (n < 128) {

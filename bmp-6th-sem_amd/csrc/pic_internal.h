@@ -45,6 +45,7 @@ struct GridDev {
     double width, height;   // GRID_WIDTH, GRID_HEIGHT
     double inv_dx, inv_dy;  // 1.0 / dx, 1.0 / dy as the reference computes them
     double nx, ny;          // NX, NY (cells): the walls in cell units
+    int    periodic;        // periodic y (mover.cpp:317-320)
 };
 
 // Per-species constants for the particle kernels.
@@ -92,7 +93,8 @@ struct Species {
 struct MgLevel {
     int gx = 0, gy = 0;
     float cx = 0, cy = 0, diag = 0;   // 1/hx^2, 1/hy^2, 2(cx+cy)
-    uint8_t* dir = nullptr;           // 1 = Dirichlet node
+    int per = 0;                      // periodic y: rows 0 .. gy-2 wrap, row gy-1 is the image
+    uint8_t* dir = nullptr;           // 1 = Dirichlet node, 2 = periodic image row (not an unknown)
     float* e = nullptr;               // correction
     float* b = nullptr;               // right-hand side
     float* t = nullptr;               // scratch
@@ -204,7 +206,7 @@ int inject_scatter(pic_ctx* c, int s, int dst);   // and into the sorted buffer
 int inject_publish(pic_ctx* c);                   // counters to the host, asynchronously
 int inject_append_rebin(pic_ctx* c);              // standalone injection into binned storage
 int inject_stats(pic_ctx* c, int64_t* out);       // injections, reinjected, pairs, dropped, last e, last i
-int launch_decode(pic_ctx* c, int s, double* x, double* y, double* vx, double* vy, double* vz);
+int launch_kinetic_energy(pic_ctx* c, int s, double* out);   // host result, synchronous
 
 // Poisson (pic_poisson.hip)
 int poisson_setup(pic_ctx* c);

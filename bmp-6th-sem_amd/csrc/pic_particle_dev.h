@@ -271,7 +271,18 @@ __device__ __forceinline__ void deposit_tile_body(
 // rotation is done with the full vector t = (q/m) B dt / 2.
 // Boundary (mover.cpp:325,331-352): absorbed when x >= W, x < 0, y >= H or
 // y < 0 (CELL32: the same walls in cell units); particles leaving through
-// x < 0 are counted separately.
+// x < 0 are counted separately.  With periodic y the new y is wrapped first.
+
+// Periodic y (mover.cpp:13-22,317-320): modulo(x, y) = fmod for x >= 0; for
+// x < 0, 0 when x / y is an integer, else y - fmod(-x, y).  Applied to the new
+// y before the wall test (metres; CELL32: cell units, period NY).
+__device__ __forceinline__ double modulo_ref(double x, double y) {
+    if (x < 0) {
+        if ((double)(int)(x / y) == x / y) return 0;
+        return y - fmod(-x, y);
+    }
+    return fmod(x, y);
+}
 
 // Particle state after the push, in storage form.
 template <int L> struct Pushed {
@@ -385,6 +396,7 @@ __device__ __forceinline__ Pushed<L> push_state(const PState<L>& p, const Corner
         boris<GENB>(Epx, Epy, Bpz, s, pvx, pvy, pvz);
         px = px + s.dt * pvx;
         py = py + s.dt * pvy;
+        if (g.periodic) py = modulo_ref(py, g.height);
         bool out = (px >= g.width || px < 0.0 || py >= g.height || py < 0);
         o.left = px < 0.0;
         if (out) {
@@ -406,6 +418,7 @@ __device__ __forceinline__ Pushed<L> push_state(const PState<L>& p, const Corner
         boris<GENB>(Epx, Epy, Bpz, s, pvx, pvy, pvz);
         double tx = ((double)c.cgx + fx) + pvx * s.dtdx;
         double ty = ((double)c.cgy + fy) + pvy * s.dtdy;
+        if (g.periodic) ty = modulo_ref(ty, g.ny);
         bool out = (tx >= g.nx || tx < 0.0 || ty >= g.ny || ty < 0.0);
         o.left = tx < 0.0;
         if (out) {

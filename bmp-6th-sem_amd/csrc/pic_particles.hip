@@ -568,4 +568,63 @@ int launch_move(pic_ctx* c, int s) {
     return move_finish(c, s);
 }
 
+// ====================================================== kinetic energy
+// Sum of 0.5 |v|^2 / |q/m| over the live particles (calculate_energy,
+// utils.cpp:77-118).  A fixed grid walks the particles grid-stride, each
+// workgroup reduces in a fixed tree and writes one partial; one workgroup adds
+// the partials in order: the FP64 result does not depend on scheduling.
+static constexpr int kKeBlocks = 1024;
+
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_kinetic_partial(const int32_t* __restrict__ ndev, ParticleBuf b,
+                                                            double denom, double* __restrict__ part) {
+    using V = typename Lay<L>::V;
+    __shared__ double red[kBlock];
+    const int n = *ndev;
+    double acc = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += kKeBlocks * kBlock) {
+        const double vx = (double)((const V*)b.vx)[i], vy = (double)((const V*)b.vy)[i];
+        const double vz = (double)((const V*)b.vz)[i];
+        const double v2 = vx * vx + vy * vy + vz * vz;
+        acc += 0.5 * v2 / denom;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kKeBlocks) void k_kinetic_total(const double* __restrict__ part, double* out) {
+    __shared__ double red[kKeBlocks];
+    red[threadIdx.x] = part[threadIdx.x];
+    __syncthreads();
+    for (int w = kKeBlocks / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+int launch_kinetic_energy(pic_ctx* c, int s, double* out) {
+    PIC_TRY(inject_settle(c));
+    Species& sp = c->sp[s];
+    static_assert(kKeBlocks + 1 <= PIC_PART_DOUBLES, "partials fit the dot-product scratch");
+    double* part = c->part;   // dot-product partials: free between Poisson solves
+    const double denom = std::fabs(sp.qm);
+    if (c->p.layout == PIC_LAYOUT_F64)
+        hipLaunchKernelGGL(k_kinetic_partial<PIC_LAYOUT_F64>, dim3(kKeBlocks), dim3(kBlock), 0, c->stream,
+                           sp.ndev + sp.cur, sp.buf[sp.cur], denom, part);
+    else
+        hipLaunchKernelGGL(k_kinetic_partial<PIC_LAYOUT_CELL32>, dim3(kKeBlocks), dim3(kBlock), 0, c->stream,
+                           sp.ndev + sp.cur, sp.buf[sp.cur], denom, part);
+    hipLaunchKernelGGL(k_kinetic_total, dim3(1), dim3(kKeBlocks), 0, c->stream, part, part + kKeBlocks);
+    PIC_HIP(hipGetLastError());
+    PIC_HIP(hipMemcpyAsync(out, part + kKeBlocks, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
 } // namespace pic

@@ -31,6 +31,12 @@
 // remaining enqueued iterations returns at once, and the host reads the flag
 // only between batches of iterations.  Levels of at most 65 x 65 nodes run
 // inside one workgroup's LDS.
+//
+// Periodic y (generateCoeffsPeriodic, poissonSolver.cpp:82-180): the unknowns
+// are rows 0 .. GRID_Y-2 and the south neighbour of row 0 is row GRID_Y-2
+// (and the reverse); row GRID_Y-1 is the image of row 0, masked in the solve
+// (mask value 2) and copied from row 0 afterwards (overwritePhi :364-382).
+// Every level keeps the same form, so the multigrid transfers wrap too.
 #include "pic_internal.h"
 #include "pic_unet.h"
 
@@ -78,21 +84,49 @@ static_assert(kNP == 2048, "sum_partials reads two partials per thread of a 1024
 struct Fine {
     int gx, gy;
     double cx, cy, diag;   // 1/dx^2, 1/dy^2, 2(1/dx^2 + 1/dy^2)
+    int per;               // periodic y
 };
+
+// Offsets of the south / north neighbour of node n (row j) on a gx x gy grid;
+// periodic grids wrap rows 0 and gy-2 onto each other (row gy-1 is the image).
+__device__ __forceinline__ int south_of(int per, int j, int gx, int gy) {
+    return (per && j == 0) ? (gy - 2) * gx : -gx;
+}
+__device__ __forceinline__ int north_of(int per, int j, int gx, int gy) {
+    return (per && j == gy - 2) ? -(gy - 2) * gx : gx;
+}
+// The same from the flat index (no division): row 0 is n < gx, row gy-2 is
+// (gy-2) gx <= n < (gy-1) gx, and the image row is never a stencil centre.
+__device__ __forceinline__ int south_n(int per, int n, int gx, int gy) {
+    return (per && n < gx) ? (gy - 2) * gx : -gx;
+}
+__device__ __forceinline__ int north_n(int per, int n, int gx, int gy) {
+    return (per && n >= (gy - 2) * gx) ? -(gy - 2) * gx : gx;
+}
+// row j - 1 / j + 1 with the periodic wrap
+__device__ __forceinline__ int row_s(int per, int j, int gy) { return (per && j == 0) ? gy - 2 : j - 1; }
+__device__ __forceinline__ int row_n(int per, int j, int gy) { return (per && j == gy - 2) ? 0 : j + 1; }
 
 // rho[geom] = V; rho = -rho  (overwriteRho, poissonSolver.cpp:384-402)
 __global__ void k_overwrite_rho(int G, double* rho, const uint8_t* dir, const double* dval) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= G) return;
-    double r = dir[n] ? dval[n] : rho[n];
+    double r = dir[n] == 1 ? dval[n] : rho[n];
     rho[n] = -r;
 }
 
-// phi[geom] = V (overwritePhi :364-382)
-__global__ void k_overwrite_phi(int G, double* phi, const uint8_t* dir, const double* dval) {
+// phi[geom] = V (overwritePhi :364-382).  Mask value 1 marks geometry nodes;
+// 2 marks the periodic image row, which takes row 0 first (the reference's
+// memcpy) unless a geometry covers it.
+__global__ void k_overwrite_phi(int G, int gx, double* phi, const uint8_t* dir, const double* dval) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= G) return;
-    if (dir[n]) phi[n] = dval[n];
+    const uint8_t d = dir[n];
+    if (d == 1) phi[n] = dval[n];
+    else if (d == 2) {
+        const int m = n % gx;
+        phi[n] = dir[m] == 1 ? dval[m] : phi[m];
+    }
 }
 
 // x <- guess (or 0) with Dirichlet values imposed; `keep` (optional) receives
@@ -127,7 +161,8 @@ __global__ __launch_bounds__(kB) void k_residual(Fine f, const double* __restric
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
         double v = 0;
         if (!dir[n]) {
-            double ax = f.diag * x[n] - f.cx * (x[n - 1] + x[n + 1]) - f.cy * (x[n - f.gx] + x[n + f.gx]);
+            const int so = south_n(f.per, n, f.gx, f.gy), no = north_n(f.per, n, f.gx, f.gy);
+            double ax = f.diag * x[n] - f.cx * (x[n - 1] + x[n + 1]) - f.cy * (x[n + so] + x[n + no]);
             v = rhs[n] - ax;
         }
         r[n] = v;
@@ -176,7 +211,8 @@ __global__ __launch_bounds__(kB) void k_apply_p(Fine f, const float* __restrict_
         double v = 0, pn = 0;
         if (!dir[n]) {
             pn = P(n);
-            v = f.diag * pn - f.cx * (P(n - 1) + P(n + 1)) - f.cy * (P(n - f.gx) + P(n + f.gx));
+            const int so = south_n(f.per, n, f.gx, f.gy), no = north_n(f.per, n, f.gx, f.gy);
+            v = f.diag * pn - f.cx * (P(n - 1) + P(n + 1)) - f.cy * (P(n + so) + P(n + no));
             acc += pn * v;
         }
         p[n] = (T)pn;
@@ -220,6 +256,7 @@ __global__ __launch_bounds__(1024) void k_sum2(const double* pa, const double* p
 struct Lvl {
     int gx, gy;
     float cx, cy, diag, wd;
+    int per;               // periodic y
 };
 
 // x += alpha p; rnew = rold - alpha q (also at the neighbours, for the
@@ -259,7 +296,8 @@ __global__ __launch_bounds__(kB) void k_update_pre2(Fine f, Lvl L, T* __restrict
             // neighbours' residuals rounded like the stored ones
             auto e1 = [&](int m) -> float { return dir[m] ? 0.f : L.wd * (float)(T)R(m); };
             const float c = L.wd * bn;
-            const float ax = L.diag * c - L.cx * (e1(n - 1) + e1(n + 1)) - L.cy * (e1(n - f.gx) + e1(n + f.gx));
+            const int so = south_n(f.per, n, f.gx, f.gy), no = north_n(f.per, n, f.gx, f.gy);
+            const float ax = L.diag * c - L.cx * (e1(n - 1) + e1(n + 1)) - L.cy * (e1(n + so) + e1(n + no));
             e0[n] = c + L.wd * (bn - ax);
         }
     }
@@ -281,7 +319,8 @@ __global__ __launch_bounds__(kB) void k_smooth_dots(Lvl L, const float* __restri
         float v = 0.f;
         if (!dir[n]) {
             const float e = ei[n];
-            const float ax = L.diag * e - L.cx * (ei[n - 1] + ei[n + 1]) - L.cy * (ei[n - L.gx] + ei[n + L.gx]);
+            const int so = south_n(L.per, n, L.gx, L.gy), no = north_n(L.per, n, L.gx, L.gy);
+            const float ax = L.diag * e - L.cx * (ei[n - 1] + ei[n + 1]) - L.cy * (ei[n + so] + ei[n + no]);
             v = e + L.wd * (b[n] - ax);
         }
         z[n] = v;
@@ -342,8 +381,9 @@ __global__ __launch_bounds__(kB) void k_mg_smooth(Lvl L, const float* __restrict
     MG_IJ(L)
     if (dir[n]) { eo[n] = 0.f; return; }
     if (zero_in) { eo[n] = L.wd * b[n]; return; }
+    const int so = south_of(L.per, j, L.gx, L.gy), no = north_of(L.per, j, L.gx, L.gy);
     float e = ei[n];
-    float ax = L.diag * e - L.cx * (ei[n - 1] + ei[n + 1]) - L.cy * (ei[n - L.gx] + ei[n + L.gx]);
+    float ax = L.diag * e - L.cx * (ei[n - 1] + ei[n + 1]) - L.cy * (ei[n + so] + ei[n + no]);
     eo[n] = e + L.wd * (b[n] - ax);
 }
 
@@ -361,14 +401,18 @@ __global__ __launch_bounds__(kB) void k_mg_pre2(Lvl L, const TB* __restrict__ bi
     if (bout) bout[n] = bn;
     if (dir[n]) { eo[n] = 0.f; return; }
     auto e1 = [&](int m) -> float { return dir[m] ? 0.f : L.wd * (float)bin[m]; };
+    const int so = south_of(L.per, j, L.gx, L.gy), no = north_of(L.per, j, L.gx, L.gy);
     float c = L.wd * bn;
-    float ax = L.diag * c - L.cx * (e1(n - 1) + e1(n + 1)) - L.cy * (e1(n - L.gx) + e1(n + L.gx));
+    float ax = L.diag * c - L.cx * (e1(n - 1) + e1(n + 1)) - L.cy * (e1(n + so) + e1(n + no));
     eo[n] = c + L.wd * (bn - ax);
 }
 
-__device__ __forceinline__ float mg_res(const Lvl& L, const float* e, const float* b, const uint8_t* dir, int n) {
+// residual at node n of row j
+__device__ __forceinline__ float mg_res(const Lvl& L, const float* e, const float* b, const uint8_t* dir, int n,
+                                        int j) {
     if (dir[n]) return 0.f;
-    float ax = L.diag * e[n] - L.cx * (e[n - 1] + e[n + 1]) - L.cy * (e[n - L.gx] + e[n + L.gx]);
+    const int so = south_of(L.per, j, L.gx, L.gy), no = north_of(L.per, j, L.gx, L.gy);
+    float ax = L.diag * e[n] - L.cx * (e[n - 1] + e[n + 1]) - L.cy * (e[n + so] + e[n + no]);
     return b[n] - ax;
 }
 
@@ -380,25 +424,29 @@ __global__ __launch_bounds__(kB) void k_mg_restrict(Lvl F, Lvl C, const float* _
     if (flags[0]) return;
     MG_IJ(C)
     if (cdir[n]) { bc[n] = 0.f; return; }
-    const int f = (2 * j) * F.gx + 2 * i;
-    const int gx = F.gx;
-    float s = 4.f * mg_res(F, e, b, fdir, f)
-            + 2.f * (mg_res(F, e, b, fdir, f - 1) + mg_res(F, e, b, fdir, f + 1)
-                   + mg_res(F, e, b, fdir, f - gx) + mg_res(F, e, b, fdir, f + gx))
-            + (mg_res(F, e, b, fdir, f - gx - 1) + mg_res(F, e, b, fdir, f - gx + 1)
-             + mg_res(F, e, b, fdir, f + gx - 1) + mg_res(F, e, b, fdir, f + gx + 1));
+    const int fj = 2 * j;
+    const int f = fj * F.gx + 2 * i;
+    const int js = row_s(F.per, fj, F.gy), jn = row_n(F.per, fj, F.gy);
+    const int fs = f + south_of(F.per, fj, F.gx, F.gy), fn = f + north_of(F.per, fj, F.gx, F.gy);
+    float s = 4.f * mg_res(F, e, b, fdir, f, fj)
+            + 2.f * (mg_res(F, e, b, fdir, f - 1, fj) + mg_res(F, e, b, fdir, f + 1, fj)
+                   + mg_res(F, e, b, fdir, fs, js) + mg_res(F, e, b, fdir, fn, jn))
+            + (mg_res(F, e, b, fdir, fs - 1, js) + mg_res(F, e, b, fdir, fs + 1, js)
+             + mg_res(F, e, b, fdir, fn - 1, jn) + mg_res(F, e, b, fdir, fn + 1, jn));
     bc[n] = s * 0.0625f;
 }
 
-// bilinear interpolation of the coarse correction at fine node (i, j)
+// bilinear interpolation of the coarse correction at fine node (i, j); on a
+// periodic grid the coarse row after the last real one is row 0
 __device__ __forceinline__ float prolong_at(const Lvl& C, const float* ec, int i, int j) {
     int I = i >> 1, J = j >> 1;
     bool oi = i & 1, oj = j & 1;
     const float* c0 = ec + J * C.gx + I;
     if (!oi && !oj) return c0[0];
     if (oi && !oj) return 0.5f * (c0[0] + c0[1]);
-    if (!oi && oj) return 0.5f * (c0[0] + c0[C.gx]);
-    return 0.25f * (c0[0] + c0[1] + c0[C.gx] + c0[C.gx + 1]);
+    const int up = north_of(C.per, J, C.gx, C.gy);
+    if (!oi && oj) return 0.5f * (c0[0] + c0[up]);
+    return 0.25f * (c0[0] + c0[1] + c0[up] + c0[up + 1]);
 }
 
 // e += P ec (unknown nodes only) — generic path.
@@ -422,9 +470,10 @@ __global__ __launch_bounds__(kB) void k_mg_prolong_sweep(Lvl F, Lvl C, const flo
     MG_IJ(F)
     if (dir[n]) { eo[n] = 0.f; return; }
     auto e1 = [&](int m, int mi, int mj) -> float { return dir[m] ? 0.f : e[m] + prolong_at(C, ec, mi, mj); };
+    const int so = south_of(F.per, j, F.gx, F.gy), no = north_of(F.per, j, F.gx, F.gy);
     float c = e1(n, i, j);
     float ax = F.diag * c - F.cx * (e1(n - 1, i - 1, j) + e1(n + 1, i + 1, j))
-             - F.cy * (e1(n - F.gx, i, j - 1) + e1(n + F.gx, i, j + 1));
+             - F.cy * (e1(n + so, i, row_s(F.per, j, F.gy)) + e1(n + no, i, row_n(F.per, j, F.gy)));
     eo[n] = c + F.wd * (b[n] - ax);
 }
 
@@ -446,14 +495,26 @@ struct CoarseLevels {
     float cx[kLdsMaxLev], cy[kLdsMaxLev], diag[kLdsMaxLev], wd[kLdsMaxLev];
     const uint8_t* cdir;          // masks of all levels, concatenated at off[]
     int pre, post, sweeps;
+    int per;                      // periodic y
     int nu;                       // unknowns of the coarsest level solved directly (0: use sweeps)
     const float* inv;             // nu x nu inverse
     const int* unk;               // node index of each unknown
 };
 
+// south / north offsets of node n (row n / gx) of an in-LDS level
+__device__ __forceinline__ void lds_nb(int per, int n, int gx, int gy, int& so, int& no) {
+    so = -gx;
+    no = gx;
+    if (per) {
+        const int j = n / gx;
+        so = south_of(per, j, gx, gy);
+        no = north_of(per, j, gx, gy);
+    }
+}
+
 // One weighted-Jacobi sweep in place (two barriers: read all, then write).
 __device__ __forceinline__ void lds_sweep(float* E, const float* B, const uint8_t* D, int gx, int gy, float cx,
-                                          float cy, float diag, float wd, bool zero_in) {
+                                          float cy, float diag, float wd, bool zero_in, int per) {
     const int G = gx * gy;
     float nv[kLdsPerThread];
 #pragma unroll
@@ -464,8 +525,10 @@ __device__ __forceinline__ void lds_sweep(float* E, const float* B, const uint8_
             if (zero_in) {
                 nv[j] = wd * B[n];
             } else {
+                int so, no;
+                lds_nb(per, n, gx, gy, so, no);
                 float e = E[n];
-                float ax = diag * e - cx * (E[n - 1] + E[n + 1]) - cy * (E[n - gx] + E[n + gx]);
+                float ax = diag * e - cx * (E[n - 1] + E[n + 1]) - cy * (E[n + so] + E[n + no]);
                 nv[j] = e + wd * (B[n] - ax);
             }
         }
@@ -479,14 +542,15 @@ __device__ __forceinline__ void lds_sweep(float* E, const float* B, const uint8_
     __syncthreads();
 }
 
-__device__ __forceinline__ float lds_prolong(const float* ec, int cgx, int i, int j) {
+__device__ __forceinline__ float lds_prolong(const float* ec, int cgx, int cgy, int per, int i, int j) {
     const int I = i >> 1, J = j >> 1;
     const bool oi = i & 1, oj = j & 1;
     const float* c0 = ec + J * cgx + I;
     if (!oi && !oj) return c0[0];
     if (oi && !oj) return 0.5f * (c0[0] + c0[1]);
-    if (!oi && oj) return 0.5f * (c0[0] + c0[cgx]);
-    return 0.25f * (c0[0] + c0[1] + c0[cgx] + c0[cgx + 1]);
+    const int up = north_of(per, J, cgx, cgy);
+    if (!oi && oj) return 0.5f * (c0[0] + c0[up]);
+    return 0.25f * (c0[0] + c0[1] + c0[up] + c0[up + 1]);
 }
 
 __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const float* __restrict__ bglob,
@@ -558,28 +622,35 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const
                 if (n >= G) continue;
                 if (d[n]) { e[n] = 0.f; continue; }
                 auto e1 = [&](int m) -> float { return d[m] ? 0.f : wd * b[m]; };
+                int so, no;
+                lds_nb(cl.per, n, gx, cl.gy[l], so, no);
                 const float c = wd * b[n];
-                const float ax = dg * c - cx * (e1(n - 1) + e1(n + 1)) - cy * (e1(n - gx) + e1(n + gx));
+                const float ax = dg * c - cx * (e1(n - 1) + e1(n + 1)) - cy * (e1(n + so) + e1(n + no));
                 e[n] = c + wd * (b[n] - ax);
             }
             __syncthreads();
         } else {
-            for (int s = 0; s < cl.pre; ++s) lds_sweep(e, b, d, gx, cl.gy[l], cx, cy, dg, wd, s == 0);
+            for (int s = 0; s < cl.pre; ++s) lds_sweep(e, b, d, gx, cl.gy[l], cx, cy, dg, wd, s == 0, cl.per);
         }
         // restrict the residual
         const int cgx = cl.gx[l + 1], cgy = cl.gy[l + 1];
         float* bc = B + cl.off[l + 1];
         const uint8_t* dc = D + cl.off[l + 1];
-        auto res = [&](int n) -> float {
+        const int gy = cl.gy[l], per = cl.per;
+        auto res = [&](int n, int j) -> float {
             if (d[n]) return 0.f;
-            float ax = dg * e[n] - cx * (e[n - 1] + e[n + 1]) - cy * (e[n - gx] + e[n + gx]);
+            float ax = dg * e[n] - cx * (e[n - 1] + e[n + 1])
+                     - cy * (e[n + south_of(per, j, gx, gy)] + e[n + north_of(per, j, gx, gy)]);
             return b[n] - ax;
         };
         for (int n = tid; n < cgx * cgy; n += 1024) {
             if (dc[n]) { bc[n] = 0.f; continue; }
-            const int f = 2 * (n / cgx) * gx + 2 * (n % cgx);
-            float s = 4.f * res(f) + 2.f * (res(f - 1) + res(f + 1) + res(f - gx) + res(f + gx))
-                    + (res(f - gx - 1) + res(f - gx + 1) + res(f + gx - 1) + res(f + gx + 1));
+            const int fj = 2 * (n / cgx);
+            const int f = fj * gx + 2 * (n % cgx);
+            const int js = row_s(per, fj, gy), jn = row_n(per, fj, gy);
+            const int fs = f + south_of(per, fj, gx, gy), fn = f + north_of(per, fj, gx, gy);
+            float s = 4.f * res(f, fj) + 2.f * (res(f - 1, fj) + res(f + 1, fj) + res(fs, js) + res(fn, jn))
+                    + (res(fs - 1, js) + res(fs + 1, js) + res(fn - 1, jn) + res(fn + 1, jn));
             bc[n] = s * 0.0625f;
         }
         __syncthreads();
@@ -598,7 +669,8 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const
             __syncthreads();
         } else {
             for (int s = 0; s < cl.sweeps; ++s)
-                lds_sweep(e, b, D + cl.off[l], cl.gx[l], cl.gy[l], cl.cx[l], cl.cy[l], cl.diag[l], cl.wd[l], s == 0);
+                lds_sweep(e, b, D + cl.off[l], cl.gx[l], cl.gy[l], cl.cx[l], cl.cy[l], cl.diag[l], cl.wd[l], s == 0,
+                          cl.per);
         }
     }
     // up
@@ -608,7 +680,8 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const
         const float* b = B + cl.off[l];
         const uint8_t* d = D + cl.off[l];
         const float* ec = E + cl.off[l + 1];
-        const int gx = cl.gx[l], G = gx * cl.gy[l], cgx = cl.gx[l + 1];
+        const int gx = cl.gx[l], gy = cl.gy[l], G = gx * gy, cgx = cl.gx[l + 1], cgy = cl.gy[l + 1];
+        const int per = cl.per;
         const float cx = cl.cx[l], cy = cl.cy[l], dg = cl.diag[l], wd = cl.wd[l];
         if (cl.post == 2) {
             // prolongation fused with the first sweep (into t), second sweep back into e
@@ -618,10 +691,13 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const
                 if (n >= G) continue;
                 if (d[n]) { t[n] = 0.f; continue; }
                 const int j = n / gx, i = n - j * gx;
-                auto e1 = [&](int m, int mi, int mj) -> float { return d[m] ? 0.f : e[m] + lds_prolong(ec, cgx, mi, mj); };
+                auto e1 = [&](int m, int mi, int mj) -> float {
+                    return d[m] ? 0.f : e[m] + lds_prolong(ec, cgx, cgy, per, mi, mj);
+                };
                 const float c = e1(n, i, j);
                 const float ax = dg * c - cx * (e1(n - 1, i - 1, j) + e1(n + 1, i + 1, j))
-                               - cy * (e1(n - gx, i, j - 1) + e1(n + gx, i, j + 1));
+                               - cy * (e1(n + south_of(per, j, gx, gy), i, row_s(per, j, gy))
+                                     + e1(n + north_of(per, j, gx, gy), i, row_n(per, j, gy)));
                 t[n] = c + wd * (b[n] - ax);
             }
             __syncthreads();
@@ -630,8 +706,10 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const
                 const int n = tid + q * 1024;
                 if (n >= G) continue;
                 if (d[n]) { e[n] = 0.f; continue; }
+                int so, no;
+                lds_nb(per, n, gx, gy, so, no);
                 const float c = t[n];
-                const float ax = dg * c - cx * (t[n - 1] + t[n + 1]) - cy * (t[n - gx] + t[n + gx]);
+                const float ax = dg * c - cx * (t[n - 1] + t[n + 1]) - cy * (t[n + so] + t[n + no]);
                 e[n] = c + wd * (b[n] - ax);
             }
             __syncthreads();
@@ -639,10 +717,10 @@ __global__ __launch_bounds__(1024) void k_mg_coarse_cycle(CoarseLevels cl, const
             for (int n = tid; n < G; n += 1024) {
                 if (d[n]) continue;
                 const int j = n / gx, i = n - j * gx;
-                e[n] += lds_prolong(ec, cgx, i, j);
+                e[n] += lds_prolong(ec, cgx, cgy, per, i, j);
             }
             __syncthreads();
-            for (int s = 0; s < cl.post; ++s) lds_sweep(e, b, d, gx, cl.gy[l], cx, cy, dg, wd, false);
+            for (int s = 0; s < cl.post; ++s) lds_sweep(e, b, d, gx, gy, cx, cy, dg, wd, false, per);
         }
     }
     for (int n = tid; n < G0; n += 1024) eglob[n] = E[n];
@@ -667,6 +745,28 @@ __global__ __launch_bounds__(kB) void k_efield(int gx, int gy, double dx, double
     fld[n] = make_double4(fx, fy, bz[n], 0.0);
 }
 
+// calculateElectricFieldPeriodic (poissonSolver.cpp:411-430): Ex one-sided in
+// the first and last column, central inside; Ey central with rows taken
+// modulo GRID_Y-1 (row GRID_Y-1 is the image of row 0).
+__global__ __launch_bounds__(kB) void k_efield_periodic(int gx, int gy, double dx, double dy,
+                                                        const double* __restrict__ phi,
+                                                        const double* __restrict__ bz, double* __restrict__ ex,
+                                                        double* __restrict__ ey, double4* __restrict__ fld) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= gx * gy) return;
+    const int j = n % gx, i = n / gx;
+    double fx;
+    if (j == 0) fx = -((phi[n + 1] - phi[n]) / dx);
+    else if (j == gx - 1) fx = -((phi[n] - phi[n - 1]) / dx);
+    else fx = -((phi[n + 1] - phi[n - 1]) / (2 * dx));
+    const int p = gy - 1;
+    const int up = (i + 1) % p, dn = ((i - 1) + p) % p;
+    const double fy = -((phi[up * gx + j] - phi[dn * gx + j]) / (2 * dy));
+    ex[n] = fx;
+    ey[n] = fy;
+    fld[n] = make_double4(fx, fy, bz[n], 0.0);
+}
+
 
 // ============================================================= host side
 static Fine fine_of(const pic_ctx* c) {
@@ -676,6 +776,7 @@ static Fine fine_of(const pic_ctx* c) {
     f.cx = 1.0 / (c->g.dx * c->g.dx);
     f.cy = 1.0 / (c->g.dy * c->g.dy);
     f.diag = 2 * (f.cx + f.cy);
+    f.per = c->p.periodic ? 1 : 0;
     return f;
 }
 
@@ -683,6 +784,7 @@ static Lvl lvl_of(const MgLevel& m, float w) {
     Lvl L;
     L.gx = m.gx; L.gy = m.gy; L.cx = m.cx; L.cy = m.cy; L.diag = m.diag;
     L.wd = w / m.diag;
+    L.per = m.per;
     return L;
 }
 
@@ -723,7 +825,11 @@ static int coarsest_inverse(pic_ctx* c, const std::vector<uint8_t>& dir) {
         double* row = &A[(size_t)p * 2 * nu];
         int n = unk[p];
         row[p] = dg;
-        int nb[4] = {n - 1, n + 1, n - m.gx, n + m.gx};
+        const int j = n / m.gx;
+        int so = -m.gx, no = m.gx;
+        if (m.per && j == 0) so = (m.gy - 2) * m.gx;
+        if (m.per && j == m.gy - 2) no = -(m.gy - 2) * m.gx;
+        int nb[4] = {n - 1, n + 1, n + so, n + no};
         double cf[4] = {cx, cx, cy, cy};
         for (int k = 0; k < 4; ++k) {
             int q = col(nb[k]);
@@ -778,6 +884,7 @@ int poisson_setup(pic_ctx* c) {
     const int gx = c->g.gx, gy = c->g.gy, G = c->G;
     // Dirichlet mask and values from the geometry rectangles, in file order
     // (later rectangles overwrite earlier ones, as the reference loops do).
+    const int per = c->p.periodic ? 1 : 0;
     std::vector<uint8_t> dir(G, 0);
     std::vector<double> dval(G, 0.0);
     for (const auto& r : c->geom) {
@@ -788,18 +895,26 @@ int poisson_setup(pic_ctx* c) {
                 dval[k * gx + j] = r.potential;
             }
     }
-    // A non-Dirichlet node on the outer frame would make the reference's rows
-    // reach outside the grid; require the walls to be covered.
-    for (int j = 0; j < gx; ++j)
-        if (!dir[j] || !dir[(gy - 1) * gx + j]) {
-            set_error("geometry leaves wall node (%d, %d) or (%d, %d) free; the non-periodic solver needs Dirichlet walls",
-                      j, 0, j, gy - 1);
-            return PIC_ERR_FORMAT;
-        }
+    if (per) {
+        // the image row is not an unknown (the reference's system has
+        // GRID_X * (GRID_Y-1) rows, poissonSolver.cpp:86)
+        for (int j = 0; j < gx; ++j)
+            if (!dir[(gy - 1) * gx + j]) dir[(gy - 1) * gx + j] = 2;
+    } else {
+        // A non-Dirichlet node on the outer frame would make the reference's
+        // rows reach outside the grid; require the walls to be covered.
+        for (int j = 0; j < gx; ++j)
+            if (!dir[j] || !dir[(gy - 1) * gx + j]) {
+                set_error("geometry leaves wall node (%d, %d) or (%d, %d) free; the non-periodic solver needs "
+                          "Dirichlet walls", j, 0, j, gy - 1);
+                return PIC_ERR_FORMAT;
+            }
+    }
+    // x walls: the reference's rows never wrap in x
     for (int k = 0; k < gy; ++k)
         if (!dir[k * gx] || !dir[k * gx + gx - 1]) {
-            set_error("geometry leaves wall node (0, %d) or (%d, %d) free; the non-periodic solver needs Dirichlet walls",
-                      k, gx - 1, k);
+            set_error("geometry leaves wall node (0, %d) or (%d, %d) free; the solver needs Dirichlet walls "
+                      "at x = 0 and x = width", k, gx - 1, k);
             return PIC_ERR_FORMAT;
         }
     PIC_HIP(hipMemcpy(c->dir, dir.data(), G, hipMemcpyHostToDevice));
@@ -817,6 +932,7 @@ int poisson_setup(pic_ctx* c) {
         m.cx = (float)(1.0 / (hx * hx));
         m.cy = (float)(1.0 / (hy * hy));
         m.diag = 2.f * (m.cx + m.cy);
+        m.per = per;
         size_t nn = (size_t)lx * ly;
         PIC_HIP(hipMalloc(&m.dir, nn));
         PIC_HIP(hipMalloc(&m.e, nn * sizeof(float)));
@@ -835,8 +951,9 @@ int poisson_setup(pic_ctx* c) {
         std::vector<uint8_t> nxt((size_t)nlx * nly, 0);
         for (int J = 0; J < nly; ++J)
             for (int I = 0; I < nlx; ++I) {
-                bool d = cur[(size_t)(2 * J) * lx + 2 * I] != 0;
-                if (I == 0 || J == 0 || I == nlx - 1 || J == nly - 1) d = true;
+                uint8_t d = cur[(size_t)(2 * J) * lx + 2 * I] != 0 ? 1 : 0;
+                if (I == 0 || I == nlx - 1 || (!per && (J == 0 || J == nly - 1))) d = 1;
+                if (per && J == nly - 1 && !d) d = 2;
                 nxt[(size_t)J * nlx + I] = d;
             }
         cur.swap(nxt);
@@ -870,7 +987,8 @@ int launch_overwrite_rho(pic_ctx* c) {
 }
 
 int launch_overwrite_phi(pic_ctx* c) {
-    hipLaunchKernelGGL(k_overwrite_phi, dim3(nblk(c->G)), dim3(kB), 0, c->stream, c->G, c->phi, c->dir, c->dval);
+    hipLaunchKernelGGL(k_overwrite_phi, dim3(nblk(c->G)), dim3(kB), 0, c->stream, c->G, c->g.gx, c->phi, c->dir,
+                       c->dval);
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
@@ -878,8 +996,12 @@ int launch_overwrite_phi(pic_ctx* c) {
 int launch_efield(pic_ctx* c) {
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_EFIELD, &ev));
-    hipLaunchKernelGGL(k_efield, dim3(nblk(c->G)), dim3(kB), 0, c->stream, c->g.gx, c->g.gy, c->g.dx, c->g.dy,
-                       c->phi, c->bz, c->ex, c->ey, c->fld);
+    if (c->p.periodic)
+        hipLaunchKernelGGL(k_efield_periodic, dim3(nblk(c->G)), dim3(kB), 0, c->stream, c->g.gx, c->g.gy, c->g.dx,
+                           c->g.dy, c->phi, c->bz, c->ex, c->ey, c->fld);
+    else
+        hipLaunchKernelGGL(k_efield, dim3(nblk(c->G)), dim3(kB), 0, c->stream, c->g.gx, c->g.gy, c->g.dx, c->g.dy,
+                           c->phi, c->bz, c->ex, c->ey, c->fld);
     PIC_HIP(hipGetLastError());
     PIC_TRY(timer_end(c, PIC_STAGE_EFIELD, ev));
     return PIC_OK;
@@ -943,6 +1065,7 @@ static int vcycle_rest(pic_ctx* c, const T* r, float* z, const float* zold, int 
         }
         cl.pre = c->p.mg_pre;
         cl.post = c->p.mg_post;
+        cl.per = c->p.periodic ? 1 : 0;
         const MgLevel& last = c->mg[L - 1];
         cl.sweeps = std::max(50, 4 * std::max(last.gx, last.gy));
         cl.nu = c->mg_nu;

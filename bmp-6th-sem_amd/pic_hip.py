@@ -145,6 +145,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "pic_ensemble_steps_done": (i32, [vp, lp]),
         "pic_learned_enable": (i32, [vp, i32, i32, C.c_float, C.c_uint64]),
         "pic_learned_stats": (i32, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), fp]),
+        "pic_kinetic_energy": (i32, [vp, i32, dp]),
         "pic_injection_default": (None, [C.POINTER(Injection)]),
         "pic_set_injection": (i32, [vp, C.POINTER(Injection)]),
         "pic_inject": (i32, [vp]),
@@ -383,6 +384,12 @@ class Context:
         l = np.zeros(1, np.float32)
         _chk(self.lib.pic_learned_stats(self.h, C.byref(a), C.byref(b), C.byref(c), _fp(l)))
         return {"trained": a.value, "tried": b.value, "used": c.value, "loss": float(l[0])}
+
+    def kinetic_energy(self, s) -> float:
+        """Sum of 0.5 |v|^2 / |q/m| over the live particles of species s."""
+        e = np.zeros(1)
+        _chk(self.lib.pic_kinetic_energy(self.h, s, _dp(e)))
+        return float(e[0])
 
     # -- injection
     def set_injection(self, enable=True, **kw):

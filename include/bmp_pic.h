@@ -43,7 +43,7 @@ extern "C" {
 #define PIC_ERR_PERIODIC_GEOM -9   /* periodic flag with > 3 geometries (reference exit(9), particles.cpp:651-654) */
 #define PIC_ERR_HIP          -10   /* HIP runtime error */
 #define PIC_ERR_RANGE        -11   /* capacity / fixed-point headroom exceeded (reference exit(1) on map overflow, mover.cpp:412-419) */
-#define PIC_ERR_UNSUPPORTED  -12   /* configuration this build does not implement (e.g. periodic-y) */
+#define PIC_ERR_UNSUPPORTED  -12   /* configuration this build does not implement (e.g. a learned guess with periodic y) */
 #define PIC_ERR_STATE        -13   /* call made in the wrong order (e.g. before init) */
 
 /* ------------------------------------------------------- storage layouts */
@@ -86,7 +86,8 @@ typedef struct pic_params {
     double b0;                /* uniform Bz filled into magneticField, T (initMagneticField) */
     double bx, by;            /* uniform in-plane B, T; 0 = the reference's Bz-only Boris     */
     double surf_area;         /* surfArea for the weighting factor (particles.cpp:431-432)   */
-    int    periodic;          /* periodicBoundary; only 0 is implemented                     */
+    int    periodic;          /* periodicBoundary: 1 = periodic y (poissonSolver.cpp:82-180,411-430,
+                                 mover.cpp:317-320); needs Dirichlet walls at x = 0 and x = width  */
     int    layout;            /* PIC_LAYOUT_*                                                 */
     int    charge_shift;      /* fixed-point scale 2^-charge_shift of normalised CIC weights  */
     int    energy_shift;      /* fixed-point scale 2^-energy_shift eV of energy weights        */
@@ -223,6 +224,11 @@ int  pic_set_grid(pic_ctx* ctx, int which, const double* in);
  * through x < 0 (the reference `new_additions`, mover.cpp:344-352) and all
  * absorbed particles.  Either pointer may be NULL. */
 int  pic_get_absorbed(pic_ctx* ctx, int s, int64_t* left, int64_t* total);
+
+/* Total kinetic energy of species s, sum of 0.5 |v|^2 / |q/m| over the live
+ * particles (calculate_energy, utils.cpp:77-118); summed in a fixed order, so
+ * repeated calls on the same state give the same double. */
+int  pic_kinetic_energy(pic_ctx* ctx, int s, double* energy);
 
 /* ------------------------------------------------------------ injection */
 /* New particles after the push of every species (main.cpp:232-245), species 0

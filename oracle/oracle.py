@@ -49,6 +49,12 @@ def load():
                                 dbl, dbl, lp]),
         "orc_push_cell32": (None, [i64, ip, fp, fp, fp, fp, fp, ip, dp, dp, dp, i32, i32, dbl, dbl, dbl, dbl, dbl,
                                    dbl, dbl, dbl, lp]),
+        "orc_push_f64_p": (None, [i64, dp, dp, dp, dp, dp, ip, dp, dp, dp, i32, i32, dbl, dbl, dbl, dbl, dbl, dbl,
+                                  dbl, dbl, i32, lp]),
+        "orc_push_cell32_p": (None, [i64, ip, fp, fp, fp, fp, fp, ip, dp, dp, dp, i32, i32, dbl, dbl, dbl, dbl,
+                                     dbl, dbl, dbl, dbl, i32, lp]),
+        "orc_efield_periodic": (None, [i32, i32, dbl, dbl, dp, dp, dp]),
+        "orc_modulo": (dbl, [dbl, dbl]),
         "orc_encode_cell32": (None, [i64, dp, dp, dp, dp, dp, i32, dbl, dbl, ip, fp, fp, fp, fp, fp]),
         "orc_geometry_nodes": (None, [i32, i32, i32, dp, C.POINTER(C.c_int), dp]),
         "orc_dirichlet": (None, [i32, i32, i32, C.POINTER(C.c_int), dp, u8, dp]),
@@ -163,26 +169,40 @@ def efield(phi, gx, gy, dx, dy):
     return ex, ey
 
 
-def push_f64(x, y, vx, vy, vz, ex, ey, bz, gx, gy, dx, dy, width, height, qm, dt, bx=0.0, by=0.0):
+def push_f64(x, y, vx, vy, vz, ex, ey, bz, gx, gy, dx, dy, width, height, qm, dt, bx=0.0, by=0.0,
+             periodic=False):
     """Returns the pushed copies (x, y, vx, vy, vz), keys (-1 = absorbed), absorbed [left, total]."""
     a = [np.array(v, np.float64, copy=True) for v in (x, y, vx, vy, vz)]
     f = [np.ascontiguousarray(v, np.float64) for v in (ex, ey, bz)]
     key = np.empty(a[0].size, np.int32)
     ab = np.zeros(2, np.int64)
-    load().orc_push_f64(a[0].size, *[dp(v) for v in a], ip(key), *[dp(v) for v in f], gx, gy, dx, dy, width,
-                        height, qm, dt, bx, by, lp(ab))
+    load().orc_push_f64_p(a[0].size, *[dp(v) for v in a], ip(key), *[dp(v) for v in f], gx, gy, dx, dy, width,
+                          height, qm, dt, bx, by, 1 if periodic else 0, lp(ab))
     return (*a, key, ab)
 
 
-def push_cell32(cell, ox, oy, vx, vy, vz, ex, ey, bz, gx, gy, dx, dy, width, height, qm, dt, bx=0.0, by=0.0):
+def push_cell32(cell, ox, oy, vx, vy, vz, ex, ey, bz, gx, gy, dx, dy, width, height, qm, dt, bx=0.0, by=0.0,
+                periodic=False):
     c = np.array(cell, np.int32, copy=True)
     a = [np.array(v, np.float32, copy=True) for v in (ox, oy, vx, vy, vz)]
     f = [np.ascontiguousarray(v, np.float64) for v in (ex, ey, bz)]
     key = np.empty(c.size, np.int32)
     ab = np.zeros(2, np.int64)
-    load().orc_push_cell32(c.size, ip(c), *[fp(v) for v in a], ip(key), *[dp(v) for v in f], gx, gy, dx, dy,
-                           width, height, qm, dt, bx, by, lp(ab))
+    load().orc_push_cell32_p(c.size, ip(c), *[fp(v) for v in a], ip(key), *[dp(v) for v in f], gx, gy, dx, dy,
+                             width, height, qm, dt, bx, by, 1 if periodic else 0, lp(ab))
     return (c, *a, key, ab)
+
+
+def modulo(x, y):
+    return load().orc_modulo(float(x), float(y))
+
+
+def efield_periodic(phi, gx, gy, dx, dy):
+    phi = np.ascontiguousarray(phi, np.float64)
+    ex = np.zeros(gx * gy)
+    ey = np.zeros(gx * gy)
+    load().orc_efield_periodic(gx, gy, dx, dy, dp(phi), dp(ex), dp(ey))
+    return ex, ey
 
 
 def encode_cell32(x, y, vx, vy, vz, gx, dx, dy):
@@ -245,6 +265,50 @@ def poisson_solver(rho, gx, gy, dx, dy, rects):
     rhs = overwrite_rho(rho, mask, dval)
     phi = poisson_direct(gx, gy, dx, dy, mask, dval, rhs)
     ex, ey = efield(phi, gx, gy, dx, dy)
+    return rhs, phi, ex, ey
+
+
+def assemble_periodic(gx, gy, dx, dy, mask):
+    """generateCoeffsPeriodic (src/poissonSolver.cpp:82-180) as a scipy CSR
+    matrix of N = gx (gy - 1) rows: Dirichlet rows -1 on the diagonal; other
+    rows 2(1/dx^2 + 1/dy^2) on the diagonal, -1/dx^2 at n -+ 1 and -1/dy^2 at
+    the rows above and below taken modulo gy - 1.  mask: the first N nodes of
+    the geometry mask (nonzero = Dirichlet)."""
+    import scipy.sparse as sp
+
+    N = gx * (gy - 1)
+    n = np.arange(N)
+    d = np.asarray(mask, np.uint8)[:N] != 0
+    cx, cy = 1.0 / (dx * dx), 1.0 / (dy * dy)
+    u = n[~d]
+    rows = [n[d], u, u, u, u, u]
+    cols = [n[d], u, u - 1, u + 1, (u - gx) % N, (u + gx) % N]
+    vals = [np.full(d.sum(), -1.0), np.full(u.size, 2 * (cx + cy)), np.full(u.size, -cx), np.full(u.size, -cx),
+            np.full(u.size, -cy), np.full(u.size, -cy)]
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+
+
+def poisson_periodic(rho, gx, gy, dx, dy, rects):
+    """poissonSolver with periodicBoundary = 1, solved by SuperLU (FP64):
+    returns (rho after overwriteRho, phi after overwritePhi, ex, ey)."""
+    import scipy.sparse.linalg as sla
+
+    _, _, mask, dval = geometry(gx, gy, rects)
+    rhs = overwrite_rho(rho, mask, dval)
+    N = gx * (gy - 1)
+    A = assemble_periodic(gx, gy, dx, dy, mask).tocsr()
+    # The Dirichlet rows (-x = rhs) fix x there; eliminating them leaves the
+    # SPD system on the unknowns, solved without the 1 : 1/dx^2 row scaling
+    # that limits a direct solve of the full matrix.
+    d = np.asarray(mask, np.uint8)[:N] != 0
+    b = rhs[:N]
+    x = np.empty(N)
+    x[d] = -b[d]
+    AU = A[~d]
+    x[~d] = sla.spsolve(AU[:, ~d].tocsc(), b[~d] - AU[:, d] @ x[d])
+    phi = np.concatenate([x, x[:gx]])            # overwritePhi: row 0 -> image row
+    phi = np.where(mask != 0, dval, phi)         # then the geometry values
+    ex, ey = efield_periodic(phi, gx, gy, dx, dy)
     return rhs, phi, ex, ey
 
 

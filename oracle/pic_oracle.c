@@ -218,6 +218,31 @@ void orc_efield(int gxn, int gyn, double dx, double dy, const double* phi, doubl
     }
 }
 
+/* calculateElectricFieldPeriodic (src/poissonSolver.cpp:411-430): Ex one-sided
+ * at the first and last column, central inside; Ey central with row indices
+ * taken modulo GRID_Y-1, for every row including the image row. */
+void orc_efield_periodic(int gxn, int gyn, double dx, double dy, const double* phi, double* ex, double* ey) {
+    for (int i = 0; i < gyn; i++) {
+        ex[i * gxn] = -((phi[i * gxn + 1] - phi[i * gxn]) / dx);
+        for (int j = 1; j < gxn - 1; j++)
+            ex[i * gxn + j] = -((phi[i * gxn + (j + 1)] - phi[i * gxn + (j - 1)]) / (2 * dx));
+        ex[(i + 1) * gxn - 1] = -((phi[(i + 1) * gxn - 1] - phi[(i + 1) * gxn - 2]) / dx);
+    }
+    for (int j = 0; j < gxn; j++)
+        for (int i = 0; i < gyn; i++)
+            ey[i * gxn + j] = -((phi[((i + 1) % (gyn - 1)) * gxn + j] -
+                                 phi[(((i - 1) + gyn - 1) % (gyn - 1)) * gxn + j]) / (2 * dy));
+}
+
+/* modulo (src/mover.cpp:13-22): the periodic-y wrap of the mover (:317-320). */
+double orc_modulo(double x, double y) {
+    if (x < 0) {
+        if ((int)(x / y) == x / y) return 0;
+        return y - fmod(-x, y);
+    }
+    return fmod(x, y);
+}
+
 /* ---------------------------------------------------------------- push */
 static inline double fld_at(const double* f, int64_t G, int64_t idx) { return idx < G ? f[idx] : 0.0; }
 
@@ -260,9 +285,10 @@ static inline void orc_boris(double Epx, double Epy, double Bpz, double pctm, do
  * bilinear sum over (dx*dy) at flat nodes gy*GX+gx, +1, +GX, +GX+1 (reads past
  * the last node see 0).  Absorbed (returns -1) when x >= W, x < 0, y >= H or
  * y < 0 (:325); absorbed[0] counts x < 0 (:344-352), absorbed[1] all. */
-int32_t orc_push_one(double* px_, double* py_, double* pvx_, double* pvy_, double* pvz_, const double* ex,
-                     const double* ey, const double* bz, int gxn, int gyn, double dx, double dy, double width,
-                     double height, double qm, double dt, double bx, double by, int64_t* absorbed) {
+static int32_t push_one(double* px_, double* py_, double* pvx_, double* pvy_, double* pvz_, const double* ex,
+                        const double* ey, const double* bz, int gxn, int gyn, double dx, double dy, double width,
+                        double height, double qm, double dt, double bx, double by, int periodic,
+                        int64_t* absorbed) {
     const int64_t G = (int64_t)gxn * gyn;
     double px = *px_, py = *py_;
     int gx = (int)(px / dx);
@@ -280,6 +306,7 @@ int32_t orc_push_one(double* px_, double* py_, double* pvx_, double* pvy_, doubl
     orc_boris(Epx, Epy, Bpz, qm, dt, bx, by, pvx_, pvy_, pvz_);
     px = px + dt * (*pvx_);
     py = py + dt * (*pvy_);
+    if (periodic) py = orc_modulo(py, height);   /* mover.cpp:317-320 */
     *px_ = px;
     *py_ = py;
     int out = (px >= width || px < 0.0 || py >= height || py < 0);
@@ -294,15 +321,23 @@ int32_t orc_push_one(double* px_, double* py_, double* pvx_, double* pvy_, doubl
     return orc_key(px, py, gxn, dx, dy);
 }
 
-void orc_push_f64(int64_t n, double* x, double* y, double* vx, double* vy, double* vz, int32_t* key,
-                  const double* ex, const double* ey, const double* bz, int gxn, int gyn, double dx, double dy,
-                  double width, double height, double qm, double dt, double bx, double by, int64_t* absorbed) {
+int32_t orc_push_one(double* px_, double* py_, double* pvx_, double* pvy_, double* pvz_, const double* ex,
+                     const double* ey, const double* bz, int gxn, int gyn, double dx, double dy, double width,
+                     double height, double qm, double dt, double bx, double by, int64_t* absorbed) {
+    return push_one(px_, py_, pvx_, pvy_, pvz_, ex, ey, bz, gxn, gyn, dx, dy, width, height, qm, dt, bx, by, 0,
+                    absorbed);
+}
+
+void orc_push_f64_p(int64_t n, double* x, double* y, double* vx, double* vy, double* vz, int32_t* key,
+                    const double* ex, const double* ey, const double* bz, int gxn, int gyn, double dx, double dy,
+                    double width, double height, double qm, double dt, double bx, double by, int periodic,
+                    int64_t* absorbed) {
     int64_t a0 = 0, a1 = 0;
 #pragma omp parallel for schedule(static) reduction(+ : a0, a1)
     for (int64_t i = 0; i < n; ++i) {
         int64_t ab[2] = {0, 0};
-        key[i] = orc_push_one(&x[i], &y[i], &vx[i], &vy[i], &vz[i], ex, ey, bz, gxn, gyn, dx, dy, width, height,
-                              qm, dt, bx, by, ab);
+        key[i] = push_one(&x[i], &y[i], &vx[i], &vy[i], &vz[i], ex, ey, bz, gxn, gyn, dx, dy, width, height,
+                          qm, dt, bx, by, periodic, ab);
         a0 += ab[0];
         a1 += ab[1];
     }
@@ -310,6 +345,13 @@ void orc_push_f64(int64_t n, double* x, double* y, double* vx, double* vy, doubl
         absorbed[0] += a0;
         absorbed[1] += a1;
     }
+}
+
+void orc_push_f64(int64_t n, double* x, double* y, double* vx, double* vy, double* vz, int32_t* key,
+                  const double* ex, const double* ey, const double* bz, int gxn, int gyn, double dx, double dy,
+                  double width, double height, double qm, double dt, double bx, double by, int64_t* absorbed) {
+    orc_push_f64_p(n, x, y, vx, vy, vz, key, ex, ey, bz, gxn, gyn, dx, dy, width, height, qm, dt, bx, by, 0,
+                   absorbed);
 }
 
 /* CELL32 storage (the build's compact layout): key cell (cgx, cgy), float
@@ -320,9 +362,10 @@ void orc_push_f64(int64_t n, double* x, double* y, double* vx, double* vy, doubl
  * for y); new cell (int)t and offset (float)(t - (int)t) kept below 1. */
 static inline float clamp_offset(float o) { return o < 1.0f ? o : 0x1.fffffep-1f; }
 
-void orc_push_cell32(int64_t n, int32_t* cell, float* ox, float* oy, float* vx, float* vy, float* vz, int32_t* key,
-                     const double* ex, const double* ey, const double* bz, int gxn, int gyn, double dx, double dy,
-                     double width, double height, double qm, double dt, double bx, double by, int64_t* absorbed) {
+void orc_push_cell32_p(int64_t n, int32_t* cell, float* ox, float* oy, float* vx, float* vy, float* vz,
+                       int32_t* key, const double* ex, const double* ey, const double* bz, int gxn, int gyn,
+                       double dx, double dy, double width, double height, double qm, double dt, double bx,
+                       double by, int periodic, int64_t* absorbed) {
     int64_t a0 = 0, a1 = 0;
     const int64_t G = (int64_t)gxn * gyn;
     const double dtdx = dt * (1.0 / dx), dtdy = dt * (1.0 / dy);
@@ -346,6 +389,7 @@ void orc_push_cell32(int64_t n, int32_t* cell, float* ox, float* oy, float* vx, 
         orc_boris(Epx, Epy, Bpz, qm, dt, bx, by, &pvx, &pvy, &pvz);
         double tx = ((double)cgx + fx) + pvx * dtdx;
         double ty = ((double)cgy + fy) + pvy * dtdy;
+        if (periodic) ty = orc_modulo(ty, NY);   /* the wrap in cell units, period NY */
         if (tx >= NX || tx < 0.0 || ty >= NY || ty < 0.0) {
             if (tx < 0.0) a0++;
             a1++;
@@ -366,6 +410,13 @@ void orc_push_cell32(int64_t n, int32_t* cell, float* ox, float* oy, float* vx, 
         absorbed[0] += a0;
         absorbed[1] += a1;
     }
+}
+
+void orc_push_cell32(int64_t n, int32_t* cell, float* ox, float* oy, float* vx, float* vy, float* vz, int32_t* key,
+                     const double* ex, const double* ey, const double* bz, int gxn, int gyn, double dx, double dy,
+                     double width, double height, double qm, double dt, double bx, double by, int64_t* absorbed) {
+    orc_push_cell32_p(n, cell, ox, oy, vx, vy, vz, key, ex, ey, bz, gxn, gyn, dx, dy, width, height, qm, dt, bx,
+                      by, 0, absorbed);
 }
 
 /* Encode absolute positions into CELL32 storage (the GPU upload path). */

@@ -31,6 +31,16 @@ void orc_push_f64(int64_t n, double* x, double* y, double* vx, double* vy, doubl
 void orc_push_cell32(int64_t n, int32_t* cell, float* ox, float* oy, float* vx, float* vy, float* vz, int32_t* key,
                      const double* ex, const double* ey, const double* bz, int gxn, int gyn, double dx, double dy,
                      double width, double height, double qm, double dt, double bx, double by, int64_t* absorbed);
+void orc_push_f64_p(int64_t n, double* x, double* y, double* vx, double* vy, double* vz, int32_t* key,
+                    const double* ex, const double* ey, const double* bz, int gxn, int gyn, double dx, double dy,
+                    double width, double height, double qm, double dt, double bx, double by, int periodic,
+                    int64_t* absorbed);
+void orc_push_cell32_p(int64_t n, int32_t* cell, float* ox, float* oy, float* vx, float* vy, float* vz,
+                       int32_t* key, const double* ex, const double* ey, const double* bz, int gxn, int gyn,
+                       double dx, double dy, double width, double height, double qm, double dt, double bx,
+                       double by, int periodic, int64_t* absorbed);
+void orc_efield_periodic(int gxn, int gyn, double dx, double dy, const double* phi, double* ex, double* ey);
+double orc_modulo(double x, double y);
 void orc_encode_cell32(int64_t n, const double* x, const double* y, const double* vx, const double* vy,
                        const double* vz, int gxn, double dx, double dy, int32_t* cell, float* ox, float* oy,
                        float* fvx, float* fvy, float* fvz);

@@ -359,3 +359,24 @@ def test_precision_form_field_errors(hip_mod, oracle_mod):
     print("field errors vs the FP64 oracle:", errs)
     for e in errs.values():
         assert e["phi"] < 1e-8 and e["Ex"] < 1e-8 and e["Ey"] < 1e-8
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_kinetic_energy_reduction(hip_mod, layout):
+    """calculate_energy (utils.cpp:77-118): sum of 0.5 |v|^2 / |q/m| over the
+    live particles, against an FP64 numpy sum of the downloaded velocities
+    (1e-12 relative: the summation orders differ); repeated calls agree bit for bit."""
+    hip = hip_mod
+    c = _ctx(hip, 64, layout)
+    for sp, (q, m, T, n) in enumerate([(-E, ME, 2.0, 70001), (E, MI, 0.03, 5)]):
+        s = c.add_species(q, m, n, 1e16, n)
+        c.upload(s, *_parts(hip, c, n, seed=4, stream=sp, qm=abs(q / m), T=T))
+        _, _, vx, vy, vz, _ = c.download(s)
+        want = np.sum(0.5 * (vx * vx + vy * vy + vz * vz) / abs(q / m))
+        got = c.kinetic_energy(s)
+        assert abs(got - want) <= 1e-12 * want
+        assert c.kinetic_energy(s) == got
+        # mean energy per particle ~ 1.5 T (eV) for a Maxwellian
+        if n > 1000:
+            assert abs(got / n / (1.5 * T) - 1) < 0.02
+    c.close()
