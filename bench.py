@@ -201,6 +201,35 @@ def ensemble_form(nruns, steps, seed=4):
     return out
 
 
+def injection_form(ctx, w, it, steps):
+    """The step with the reference's particle injection after the push
+    (main.cpp:232-245): re-injected electrons for the x < 0 exits and the
+    ionization pairs of generate_pairs, at the workload's temperatures."""
+    (_, _, te), (_, _, ti) = w.species()
+    cfg = ctx.set_injection(te_reinject_eV=float(int(te)), te_eV=te, ti_eV=ti, seed=7)
+    target, _ = ctx.injection_target()
+    ctx.step(it)                                  # warm-up (allocations settle)
+    it += 1
+    ctx.sync()
+    n0 = sum(ctx.count(s) for s in range(2))
+    st0 = ctx.injection_stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.step(it)
+        it += 1
+    ctx.sync()
+    t1 = time.perf_counter()
+    n1 = sum(ctx.count(s) for s in range(2))
+    st1 = ctx.injection_stats()
+    ctx.set_injection(False)
+    ms = 1e3 * (t1 - t0) / steps
+    return {"steps": steps, "ms_per_step": ms,
+            "particle_steps_per_s": steps * 0.5 * (n0 + n1) / (t1 - t0),
+            "pairs_per_step": target, "x_inject_m": cfg["x_inject"],
+            "reinjected_per_step": (st1["reinjected"] - st0["reinjected"]) / steps,
+            "dropped": st1["dropped"] - st0["dropped"]}, it
+
+
 def cpu_baseline(w, parts, steps=3):
     """Oracle (C, OpenMP, FP64 AoS, MKL PARDISO) on the same workload."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -252,6 +281,10 @@ def main():
                     help="Poisson: FP64 PCG, or FP32 correction solves refined in FP64")
     ap.add_argument("--guess", choices=("prev", "extrap"), default="prev",
                     help="Poisson initial guess: previous phi, or extrapolated from the last two")
+    ap.add_argument("--smoother", choices=("chebyshev", "jacobi"), default="chebyshev",
+                    help="multigrid smoother of the Poisson preconditioner")
+    ap.add_argument("--injection-steps", type=int, default=10,
+                    help="steps timed with particle injection on (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run three MIB-MiB device copies (counter calibration) and exit")
@@ -276,7 +309,8 @@ def main():
     log(f"generated {2 * w.n_per_species} particles in {time.perf_counter() - t0:.1f} s")
     ctx = cfg.make_context(w, layout=layout, particles=parts, poisson_tol=1e-10,
                            poisson_guess=1 if a.guess == "extrap" else 0,
-                           poisson_precision=1 if a.precision == "mixed" else 0)
+                           poisson_precision=1 if a.precision == "mixed" else 0,
+                           mg_smoother=1 if a.smoother == "chebyshev" else 0, capacity_factor=1.05)
     log(f"context ready ({time.perf_counter() - t0:.1f} s)")
 
     it = 0
@@ -332,6 +366,15 @@ def main():
             others[KERNEL_NAMES[k]] = {"avg_launch_us": round(ms_k / n_k * 1e3, 2),
                                        "GB_s": round(b / (ms_k / n_k * 1e-3) / 1e9, 1)}
 
+    injection = None
+    if a.injection_steps > 0:
+        try:
+            injection, it = injection_form(ctx, w, it, a.injection_steps)
+            log(f"with injection: {injection['ms_per_step']:.3f} ms/step")
+        except Exception as e:  # reported, the main measurement stands
+            log(f"injection run failed: {e}")
+            injection = {"error": str(e)}
+
     learned = None
     if a.learned_train > 0:
         try:
@@ -372,13 +415,14 @@ def main():
         "config": {"workload": w.name, "cells": f"{w.NX}x{w.NY}", "particles_per_species": w.n_per_species,
                    "species": 2, "electrode_V": w.electrode_v if w.electrode else None,
                    "B_T": w.b_total, "B_tilt_deg": w.b_tilt_deg, "poisson_tol": 1e-10,
-                   "poisson_guess": a.guess, "poisson_precision": a.precision},
+                   "poisson_guess": a.guess, "poisson_precision": a.precision, "mg_smoother": a.smoother},
         "stages_ms_per_step": {k: round(v[0] / a.steps, 4) for k, v in stages.items()},
         "poisson": {"iterations_mean": float(np.mean(iters)), "ms_mean": float(np.mean([s["ms"] for s in stats])),
                     "inner_iterations_mean": float(np.mean([s["inner_iterations"] for s in stats])),
                     "initial_relres_mean": float(np.mean([s["initial_relres"] for s in stats]))},
         "roofline": roof,
         "particle_kernels": others,
+        "with_injection": injection,
         "poisson_learned_vs_plain": learned,
         "ensemble": ensemble,
         "cpu_baseline": cpu,

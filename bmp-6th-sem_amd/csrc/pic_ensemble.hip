@@ -33,6 +33,7 @@ static constexpr int kEnsMaxLev = 12;
 struct EnsLevel {
     int gx, gy;
     float cx, cy, diag, wd;
+    float wd2;                    // second pre-smoothing sweep (first post-smoothing sweep)
     const uint8_t* dir;
     float *e, *b, *t;
 };
@@ -162,19 +163,19 @@ __device__ __forceinline__ double wg_sum(double v, double* sh) {
 
 #define WG_FOR(n, N) for (int n = threadIdx.x; n < (N); n += blockDim.x)
 
-// out = in + wd (b - A in) on unknowns (zero_in: out = wd b); 0 on Dirichlet
-__device__ void wg_jacobi(const EnsLevel& L, const float* in, float* out, bool zero_in) {
+// out = in + w (b - A in) on unknowns (zero_in: out = w b); 0 on Dirichlet
+__device__ void wg_jacobi(const EnsLevel& L, const float* in, float* out, bool zero_in, float w) {
     const int G = L.gx * L.gy;
 #pragma unroll 4
     WG_FOR(n, G) {
         float v = 0.f;
         if (!L.dir[n]) {
             if (zero_in) {
-                v = L.wd * L.b[n];
+                v = w * L.b[n];
             } else {
                 const float e = in[n];
                 const float ax = L.diag * e - L.cx * (in[n - 1] + in[n + 1]) - L.cy * (in[n - L.gx] + in[n + L.gx]);
-                v = e + L.wd * (L.b[n] - ax);
+                v = e + w * (L.b[n] - ax);
             }
         }
         out[n] = v;
@@ -209,7 +210,7 @@ __device__ float* wg_vcycle(const EnsRun& run, const double* r, float** ecur) {
                 __syncthreads();
             } else {
                 for (int s = 0; s < run.sweeps; ++s) {
-                    wg_jacobi(L, e, t, s == 0);
+                    wg_jacobi(L, e, t, s == 0, L.wd);
                     float* x = e;
                     e = t;
                     t = x;
@@ -220,7 +221,7 @@ __device__ float* wg_vcycle(const EnsRun& run, const double* r, float** ecur) {
         }
         // pre-smoothing from zero
         for (int s = 0; s < run.pre; ++s) {
-            wg_jacobi(L, e, t, s == 0);
+            wg_jacobi(L, e, t, s == 0, s == 1 ? L.wd2 : L.wd);
             float* x = e;
             e = t;
             t = x;
@@ -273,7 +274,7 @@ __device__ float* wg_vcycle(const EnsRun& run, const double* r, float** ecur) {
         }
         __syncthreads();
         for (int s = 0; s < run.post; ++s) {
-            wg_jacobi(L, e, t, false);
+            wg_jacobi(L, e, t, false, (s == 0 && run.post == 2) ? L.wd2 : L.wd);
             float* x = e;
             e = t;
             t = x;
@@ -589,7 +590,7 @@ int fill_tables(pic_ensemble* e) {
                 v.cx = m.cx;
                 v.cy = m.cy;
                 v.diag = m.diag;
-                v.wd = (float)c->p.mg_omega / m.diag;
+                mg_sweep_weights(c, m, &v.wd, &v.wd2);
                 v.dir = m.dir;
                 v.e = m.e;
                 v.b = m.b;

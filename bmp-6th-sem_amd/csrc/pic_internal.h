@@ -212,6 +212,9 @@ int launch_kinetic_energy(pic_ctx* c, int s, double* out);   // host result, syn
 int poisson_setup(pic_ctx* c);
 void poisson_free(pic_ctx* c);
 int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st);
+// weights over the diagonal of the two pre-smoothing sweeps of a level (the
+// post-smoothing uses them in reverse order)
+void mg_sweep_weights(const pic_ctx* c, const MgLevel& m, float* w0, float* w1);
 int launch_overwrite_phi(pic_ctx* c);
 int launch_overwrite_rho(pic_ctx* c);
 int launch_efield(pic_ctx* c);

@@ -139,10 +139,13 @@ def generate(w: Workload, s: int):
                                       excl=w.electrode_nodes())
 
 
-def make_context(w: Workload, layout: int | None = None, particles=None, **params):
+def make_context(w: Workload, layout: int | None = None, particles=None, capacity_factor: float = 1.0,
+                 **params):
     """Create a GPU context for workload w and upload its particles.
 
-    `particles` may hold pre-generated (x, y, vx, vy, vz) per species."""
+    `particles` may hold pre-generated (x, y, vx, vy, vz) per species;
+    capacity_factor sizes the storage above the initial count (room for
+    injected particles)."""
     import pic_hip
     lay = w.layout if layout is None else layout
     p = pic_hip.default_params(NX=w.NX, NY=w.NY, width=w.width, height=w.height, dt=w.dt, epsilon=EPS,
@@ -150,7 +153,7 @@ def make_context(w: Workload, layout: int | None = None, particles=None, **param
     ctx = pic_hip.Context(p)
     ctx.set_geometry(w.rects())
     for s, (q, m, _) in enumerate(w.species()):
-        sid = ctx.add_species(q, m, w.n_per_species, N0, w.n_per_species)
+        sid = ctx.add_species(q, m, w.n_per_species, N0, int(w.n_per_species * capacity_factor))
         parts = particles[s] if particles is not None else generate(w, s)
         ctx.upload(sid, *parts)
     return ctx

@@ -102,6 +102,8 @@ typedef struct pic_params {
                                  1: iterative refinement — FP64 residual, dot products and update
                                  of phi, FP32 PCG vectors for each correction solve            */
     double poisson_inner_tol; /* relative residual of each FP32 correction solve (precision 1)  */
+    int    mg_smoother;       /* 0: weighted Jacobi (mg_omega); 1: Chebyshev steps on the upper
+                                 spectrum of D^-1 A (with mg_pre = mg_post = 2)                */
 } pic_params;
 
 /* Per-solve statistics of pic_poisson_solve. */
