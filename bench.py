@@ -82,8 +82,8 @@ def learned_vs_plain(w, parts, layout, train, evaluate, tol=1e-8):
     import numpy as np
     import pic_configs as cfg
 
-    plain = cfg.make_context(w, layout=layout, particles=parts, poisson_tol=tol)
-    learn = cfg.make_context(w, layout=layout, particles=parts, poisson_tol=tol)
+    plain = cfg.make_context(w, layout=layout, particles=parts, poisson_tol=tol, poisson_direct=0)
+    learn = cfg.make_context(w, layout=layout, particles=parts, poisson_tol=tol, poisson_direct=0)
     learn.learned_enable(train_steps=train, min_trained=train, lr=1e-3, seed=1)
     t0 = time.perf_counter()
     sp, sl = [], []

@@ -3,7 +3,8 @@
 Usage: python bmp-6th-sem_amd/build.py [--jobs N]
 
 Every source under csrc/ is compiled for --offload-arch=gfx950 and linked into
-bmp-6th-sem_amd/lib/libbmp_pic.so.  Packed FP32 instructions are disabled for
+bmp-6th-sem_amd/lib/libbmp_pic.so (with rocBLAS for the FP64 transform GEMMs
+of the direct Poisson solve).  Packed FP32 instructions are disabled for
 all device code; the particle kernels are also built with -ffp-contract=off so
 their FP64 expressions round exactly as written.
 """
@@ -40,7 +41,8 @@ EXTRA = {
     "pic_inject.hip": ["-ffp-contract=off"],
 }
 
-SOURCES = ["pic_particles.hip", "pic_poisson.hip", "pic_unet.hip", "pic_ensemble.hip", "pic_inject.hip",
+SOURCES = ["pic_particles.hip", "pic_poisson.hip", "pic_fastpoisson.hip", "pic_unet.hip", "pic_ensemble.hip",
+           "pic_inject.hip",
            "pic_host.cpp",
            "pic_config.cpp", "pic_dropin.cpp"]
 
@@ -61,7 +63,8 @@ def build(jobs: int = 8, verbose: bool = True) -> Path:
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIBDIR / "libbmp_pic.so.tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp), "-lpthread"]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp),
+           "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocblas", "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

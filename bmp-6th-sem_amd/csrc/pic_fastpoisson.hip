@@ -1,0 +1,423 @@
+// pic_fastpoisson.hip — direct solve of the reference's Poisson system
+// (poissonSolver.cpp:183-241,459-544: the 5-point rows of the unknown nodes,
+// Dirichlet rows phi = V) for walls held at fixed potentials plus internal
+// Dirichlet nodes (electrodes).  The reference factors the system once with
+// PARDISO; here the same system is solved exactly by
+//
+//   * the discrete sine transform: on the interior of the rectangle (m x n
+//     nodes, walls eliminated into the right-hand side) the operator
+//     T = cx L_m (x) I + cy I (x) L_n, L = tridiag(-1, 2, -1), is diagonal in
+//     the orthonormal DST-I basis S[j][k] = sqrt(2/(N+1)) sin(pi (j+1)(k+1)/(N+1)):
+//     T^-1 f = S_n ((S_n F S_m) ./ Lambda) S_m, Lambda[k][l] =
+//     cy (2 - 2 cos(pi (k+1)/(n+1))) + cx (2 - 2 cos(pi (l+1)/(m+1)));
+//     each product with S is one FP64 GEMM (MFMA, rocBLAS);
+//   * a capacitance correction for the internal Dirichlet nodes: with Q the
+//     internal Dirichlet nodes next to an unknown (P of them), sources c on Q
+//     are chosen so that u = T^-1 (f + B c) equals V on Q:
+//     c = C^-1 (V_Q - (T^-1 f)_Q), C = (T^-1)_QQ (P x P, symmetric positive
+//     definite, inverted once at set-up).  The unknown rows only see Q among
+//     the internal Dirichlet nodes, so u solves the reference's system there.
+//
+// Per solve: F from rho (4 GEMMs of n x n x m / m x m x n with the transforms,
+// one n x P x m GEMM for the correction, a few small kernels).  The PCG of
+// pic_poisson.hip then starts from this solution and only checks (or, if
+// rounding left the residual above the tolerance, corrects) it.
+#include "pic_internal.h"
+
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace pic {
+
+static constexpr int kFdMaxP = 1024;      // internal Dirichlet nodes next to unknowns
+static constexpr int kFdMaxN = 2047;      // interior nodes per direction
+static constexpr int kFdB = 256;
+
+struct FastPoisson {
+    int m = 0, n = 0, P = 0, nr = 0;
+    rocblas_handle blas = nullptr;
+    double *Sm = nullptr, *Sn = nullptr;     // m x m, n x n
+    double* Dinv = nullptr;                  // n x m
+    double *F = nullptr, *W = nullptr, *U = nullptr, *G = nullptr;   // n x m work
+    double* SnR = nullptr;                   // nr x n: rows of S_n at the rows of Q
+    double* V1 = nullptr;                    // nr x m
+    double* SnQ = nullptr;                   // n x P: S_n[k][row(q)]
+    double* SmQ = nullptr;                   // P x m: S_m[col(q)][l]
+    double* Z = nullptr;                     // P x m: c_q S_m[col(q)][l]
+    double* Cinv = nullptr;                  // P x P
+    double* VQ = nullptr;                    // P: potentials on Q
+    double* dq = nullptr;                    // P: V_Q - u0_Q
+    int *qri = nullptr, *qcol = nullptr;     // P: index into the row list, column of q
+};
+
+static void* dmalloc(size_t bytes, bool& ok) {
+    void* p = nullptr;
+    if (ok && hipMalloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess) {
+        ok = false;
+        p = nullptr;
+    }
+    return p;
+}
+
+void fast_free(pic_ctx* c) {
+    FastPoisson* f = c->fast;
+    if (!f) return;
+    void* ptrs[] = {f->Sm, f->Sn, f->Dinv, f->F, f->W, f->U, f->G, f->SnR, f->V1, f->SnQ, f->SmQ, f->Z,
+                    f->Cinv, f->VQ, f->dq, f->qri, f->qcol};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (f->blas) (void)rocblas_destroy_handle(f->blas);
+    delete f;
+    c->fast = nullptr;
+}
+
+// ------------------------------------------------------------------ kernels
+// F (n x m, interior node (r+1, c+1)): rhs on unknowns plus the wall values
+// moved to the right-hand side; 0 on internal Dirichlet nodes.
+__global__ __launch_bounds__(kFdB) void k_fd_rhs(int gx, int gy, double cx, double cy, const double* __restrict__ rhs,
+                                                 const uint8_t* __restrict__ dir, const double* __restrict__ dval,
+                                                 double* __restrict__ F) {
+    const int m = gx - 2, n = gy - 2;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m * n) return;
+    const int r = k / m, q = k - r * m;
+    const int node = (r + 1) * gx + q + 1;
+    double v = 0.0;
+    if (!dir[node]) {
+        v = rhs[node];
+        if (q == 0) v += cx * dval[node - 1];
+        if (q == m - 1) v += cx * dval[node + 1];
+        if (r == 0) v += cy * dval[node - gx];
+        if (r == n - 1) v += cy * dval[node + gx];
+    }
+    F[k] = v;
+}
+
+__global__ __launch_bounds__(kFdB) void k_fd_scale(int N, double* __restrict__ X, const double* __restrict__ Dinv) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < N) X[k] *= Dinv[k];
+}
+
+// X += G .* Dinv
+__global__ __launch_bounds__(kFdB) void k_fd_addscaled(int N, double* __restrict__ X, const double* __restrict__ G,
+                                                       const double* __restrict__ Dinv) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < N) X[k] += G[k] * Dinv[k];
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// d_q = V_q - u0(row q, col q), u0(r, c) = sum_l V1[r][l] S_m[c][l]; one wave per q
+__global__ __launch_bounds__(kFdB) void k_fd_u0q(int P, int m, const double* __restrict__ V1,
+                                                 const double* __restrict__ Sm, const int* __restrict__ qri,
+                                                 const int* __restrict__ qcol, const double* __restrict__ VQ,
+                                                 double* __restrict__ dq) {
+    const int q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (q >= P) return;
+    const double* a = V1 + (size_t)qri[q] * m;
+    const double* b = Sm + (size_t)qcol[q] * m;
+    double s = 0.0;
+    for (int l = lane; l < m; l += 64) s += a[l] * b[l];
+    s = wave_sum(s);
+    if (lane == 0) dq[q] = VQ[q] - s;
+}
+
+// c_q = (C^-1 d)_q, then Z[q][l] = c_q S_m[col q][l]; one workgroup per q
+__global__ __launch_bounds__(kFdB) void k_fd_sources(int P, int m, const double* __restrict__ Cinv,
+                                                     const double* __restrict__ dq, const double* __restrict__ SmQ,
+                                                     double* __restrict__ Z) {
+    __shared__ double sh[kFdB / 64];
+    const int q = blockIdx.x;
+    double s = 0.0;
+    for (int p = threadIdx.x; p < P; p += kFdB) s += Cinv[(size_t)q * P + p] * dq[p];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    const double cq = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    for (int l = threadIdx.x; l < m; l += kFdB) Z[(size_t)q * m + l] = cq * SmQ[(size_t)q * m + l];
+}
+
+// full-grid solution: interior unknowns from U, every Dirichlet node its value
+__global__ __launch_bounds__(kFdB) void k_fd_out(int gx, int gy, const double* __restrict__ U,
+                                                 const uint8_t* __restrict__ dir, const double* __restrict__ dval,
+                                                 double* __restrict__ out) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= gx * gy) return;
+    const int j = node / gx, i = node - j * gx;
+    double v = dval[node];
+    if (!dir[node] && i > 0 && j > 0 && i < gx - 1 && j < gy - 1) v = U[(size_t)(j - 1) * (gx - 2) + (i - 1)];
+    out[node] = v;
+}
+
+// Rows [k0, k0 + kc) of the factor A with C = A A^T:
+// A[q][(k - k0) m + l] = S_n[row q][k] S_m[col q][l] sqrt(Dinv[k][l]).
+__global__ __launch_bounds__(kFdB) void k_fd_amat(int P, int n, int m, int k0, int kc, const double* __restrict__ Sn,
+                                                  const double* __restrict__ Sm, const double* __restrict__ Dinv,
+                                                  const int* __restrict__ qrow, const int* __restrict__ qcol,
+                                                  double* __restrict__ A) {
+    const size_t K = (size_t)kc * m;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)P * K) return;
+    const int q = (int)(idx / K);
+    const size_t rem = idx - (size_t)q * K;
+    const int kk = (int)(rem / m), l = (int)(rem - (size_t)kk * m);
+    const int k = k0 + kk;
+    double v = 0.0;
+    if (k < n) v = Sn[(size_t)qrow[q] * n + k] * Sm[(size_t)qcol[q] * m + l] * sqrt(Dinv[(size_t)k * m + l]);
+    A[idx] = v;
+}
+
+// ------------------------------------------------------------------- host
+static std::vector<double> dst_matrix(int N) {
+    std::vector<double> S((size_t)N * N);
+    const double a = std::sqrt(2.0 / (N + 1));
+    const double w = M_PI / (N + 1);
+    for (int j = 0; j < N; ++j)
+        for (int k = 0; k < N; ++k) {
+            // reduce (j+1)(k+1) modulo 2(N+1) so the sine argument stays below 2 pi
+            const long long t = ((long long)(j + 1) * (k + 1)) % (2LL * (N + 1));
+            S[(size_t)j * N + k] = a * std::sin(w * (double)t);
+        }
+    return S;
+}
+
+#define FD_BLAS(call)                                                                            \
+    do {                                                                                         \
+        rocblas_status s_ = (call);                                                              \
+        if (s_ != rocblas_status_success) {                                                      \
+            set_error("%s:%d rocBLAS %s: %s", __FILE__, __LINE__, #call, rocblas_status_to_string(s_)); \
+            return PIC_ERR_HIP;                                                                  \
+        }                                                                                        \
+    } while (0)
+
+// Column-major C (mr x nc) = A (mr x k) B (k x nc); row-major arrays are their
+// transposes, so row-major products are issued with the operands swapped.
+static int gemm(rocblas_handle h, int mr, int nc, int k, const double* A, int lda, const double* B, int ldb,
+                double beta, double* C, int ldc) {
+    const double one = 1.0;
+    FD_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, mr, nc, k, &one, A, lda, B, ldb, &beta,
+                          C, ldc));
+    return PIC_OK;
+}
+
+// Explicit inverse of the SPD matrix C (P x P, row-major) by Cholesky.
+static bool spd_inverse(std::vector<double>& C, int P) {
+    std::vector<double> L((size_t)P * P, 0.0);
+    for (int j = 0; j < P; ++j) {
+        double d = C[(size_t)j * P + j];
+        for (int k = 0; k < j; ++k) d -= L[(size_t)j * P + k] * L[(size_t)j * P + k];
+        if (!(d > 0)) return false;
+        const double ljj = std::sqrt(d);
+        L[(size_t)j * P + j] = ljj;
+        for (int i = j + 1; i < P; ++i) {
+            double s = C[(size_t)i * P + j];
+            for (int k = 0; k < j; ++k) s -= L[(size_t)i * P + k] * L[(size_t)j * P + k];
+            L[(size_t)i * P + j] = s / ljj;
+        }
+    }
+    // Linv (lower), then C^-1 = Linv^T Linv
+    std::vector<double> Li((size_t)P * P, 0.0);
+    for (int i = 0; i < P; ++i) {
+        Li[(size_t)i * P + i] = 1.0 / L[(size_t)i * P + i];
+        for (int j = 0; j < i; ++j) {
+            double s = 0.0;
+            for (int k = j; k < i; ++k) s += L[(size_t)i * P + k] * Li[(size_t)k * P + j];
+            Li[(size_t)i * P + j] = -s / L[(size_t)i * P + i];
+        }
+    }
+    for (int i = 0; i < P; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double s = 0.0;
+            for (int k = i; k < P; ++k) s += Li[(size_t)k * P + i] * Li[(size_t)k * P + j];
+            C[(size_t)i * P + j] = s;
+            C[(size_t)j * P + i] = s;
+        }
+    return true;
+}
+
+// Set up the direct solve for the current geometry (`dir`: 1 = Dirichlet,
+// the walls included).  Leaves c->fast null when the configuration is not
+// covered (periodic y, too many internal Dirichlet nodes next to unknowns,
+// large grids, or turned off); the PCG then solves alone.
+int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<double>& dval) {
+    fast_free(c);
+    const int gx = c->g.gx, gy = c->g.gy;
+    const int m = gx - 2, n = gy - 2;
+    if (!c->p.poisson_direct || c->p.periodic || m < 1 || n < 1 || m > kFdMaxN || n > kFdMaxN) return PIC_OK;
+    std::vector<int> qrow, qcol;
+    std::vector<double> vq;
+    for (int r = 1; r < gy - 1; ++r)
+        for (int q = 1; q < gx - 1; ++q) {
+            const int node = r * gx + q;
+            if (!dir[node]) continue;
+            if (!dir[node - 1] || !dir[node + 1] || !dir[node - gx] || !dir[node + gx]) {
+                qrow.push_back(r - 1);
+                qcol.push_back(q - 1);
+                vq.push_back(dval[node]);
+            }
+        }
+    const int P = (int)qrow.size();
+    if (P > kFdMaxP) return PIC_OK;
+    FastPoisson* f = new FastPoisson();
+    c->fast = f;
+    f->m = m;
+    f->n = n;
+    f->P = P;
+    if (rocblas_create_handle(&f->blas) != rocblas_status_success) {
+        fast_free(c);
+        set_error("rocBLAS handle creation failed");
+        return PIC_ERR_HIP;
+    }
+    if (rocblas_set_stream(f->blas, c->stream) != rocblas_status_success) {
+        fast_free(c);
+        set_error("rocBLAS set stream failed");
+        return PIC_ERR_HIP;
+    }
+    // distinct rows of Q and each q's index among them
+    std::vector<int> rows(qrow);
+    std::sort(rows.begin(), rows.end());
+    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+    std::vector<int> qri(P);
+    for (int q = 0; q < P; ++q) qri[q] = (int)(std::lower_bound(rows.begin(), rows.end(), qrow[q]) - rows.begin());
+    f->nr = (int)rows.size();
+    const int nr = f->nr;
+
+    const std::vector<double> Sm = dst_matrix(m);
+    const std::vector<double> Sn = (n == m) ? Sm : dst_matrix(n);
+    const double cx = 1.0 / (c->g.dx * c->g.dx), cy = 1.0 / (c->g.dy * c->g.dy);
+    std::vector<double> Dinv((size_t)n * m);
+    for (int k = 0; k < n; ++k) {
+        const double ly = cy * (2.0 - 2.0 * std::cos(M_PI * (k + 1) / (n + 1)));
+        for (int l = 0; l < m; ++l) {
+            const double lx = cx * (2.0 - 2.0 * std::cos(M_PI * (l + 1) / (m + 1)));
+            Dinv[(size_t)k * m + l] = 1.0 / (ly + lx);
+        }
+    }
+    std::vector<double> SnR((size_t)std::max(nr, 1) * n), SnQ((size_t)n * std::max(P, 1)), SmQ((size_t)std::max(P, 1) * m);
+    for (int a = 0; a < nr; ++a)
+        for (int k = 0; k < n; ++k) SnR[(size_t)a * n + k] = Sn[(size_t)rows[a] * n + k];
+    for (int k = 0; k < n; ++k)
+        for (int q = 0; q < P; ++q) SnQ[(size_t)k * P + q] = Sn[(size_t)qrow[q] * n + k];
+    for (int q = 0; q < P; ++q)
+        for (int l = 0; l < m; ++l) SmQ[(size_t)q * m + l] = Sm[(size_t)qcol[q] * m + l];
+
+    bool ok = true;
+    const size_t nm = (size_t)n * m;
+    f->Sm = (double*)dmalloc((size_t)m * m * 8, ok);
+    f->Sn = (double*)dmalloc((size_t)n * n * 8, ok);
+    f->Dinv = (double*)dmalloc(nm * 8, ok);
+    f->F = (double*)dmalloc(nm * 8, ok);
+    f->W = (double*)dmalloc(nm * 8, ok);
+    f->U = (double*)dmalloc(nm * 8, ok);
+    f->G = (double*)dmalloc(nm * 8, ok);
+    f->SnR = (double*)dmalloc(SnR.size() * 8, ok);
+    f->V1 = (double*)dmalloc((size_t)std::max(nr, 1) * m * 8, ok);
+    f->SnQ = (double*)dmalloc(SnQ.size() * 8, ok);
+    f->SmQ = (double*)dmalloc(SmQ.size() * 8, ok);
+    f->Z = (double*)dmalloc(SmQ.size() * 8, ok);
+    f->Cinv = (double*)dmalloc((size_t)std::max(P, 1) * std::max(P, 1) * 8, ok);
+    f->VQ = (double*)dmalloc((size_t)std::max(P, 1) * 8, ok);
+    f->dq = (double*)dmalloc((size_t)std::max(P, 1) * 8, ok);
+    f->qri = (int*)dmalloc((size_t)std::max(P, 1) * 4, ok);
+    f->qcol = (int*)dmalloc((size_t)std::max(P, 1) * 4, ok);
+    int* qrow_d = (int*)dmalloc((size_t)std::max(P, 1) * 4, ok);
+    if (!ok) {
+        if (qrow_d) (void)hipFree(qrow_d);
+        fast_free(c);
+        set_error("direct Poisson solve: device allocation failed");
+        return PIC_ERR_ALLOC;
+    }
+    PIC_HIP(hipMemcpy(f->Sm, Sm.data(), (size_t)m * m * 8, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(f->Sn, Sn.data(), (size_t)n * n * 8, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(f->Dinv, Dinv.data(), nm * 8, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(f->SnR, SnR.data(), SnR.size() * 8, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(f->SnQ, SnQ.data(), SnQ.size() * 8, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(f->SmQ, SmQ.data(), SmQ.size() * 8, hipMemcpyHostToDevice));
+    if (P > 0) {
+        PIC_HIP(hipMemcpy(f->VQ, vq.data(), (size_t)P * 8, hipMemcpyHostToDevice));
+        PIC_HIP(hipMemcpy(f->qri, qri.data(), (size_t)P * 4, hipMemcpyHostToDevice));
+        PIC_HIP(hipMemcpy(f->qcol, qcol.data(), (size_t)P * 4, hipMemcpyHostToDevice));
+        PIC_HIP(hipMemcpy(qrow_d, qrow.data(), (size_t)P * 4, hipMemcpyHostToDevice));
+        // C = A A^T over the spectral index, in chunks of kc rows of k
+        const int kc = std::max(1, std::min(n, (int)((size_t)(256u << 20) / ((size_t)P * m * 8))));
+        double* A = nullptr;
+        double* Cd = nullptr;
+        PIC_HIP(hipMalloc((void**)&A, (size_t)P * kc * m * 8));
+        PIC_HIP(hipMalloc((void**)&Cd, (size_t)P * P * 8));
+        int rc = PIC_OK;
+        for (int k0 = 0; k0 < n && rc == PIC_OK; k0 += kc) {
+            const size_t tot = (size_t)P * kc * m;
+            hipLaunchKernelGGL(k_fd_amat, dim3((unsigned)((tot + kFdB - 1) / kFdB)), dim3(kFdB), 0, c->stream, P, n, m,
+                               k0, kc, f->Sn, f->Sm, f->Dinv, qrow_d, f->qcol, A);
+            // column-major view: A^T is (kc m) x P with ld kc m; C = (A^T)^T A^T
+            const double one = 1.0, beta = k0 == 0 ? 0.0 : 1.0;
+            if (rocblas_dgemm(f->blas, rocblas_operation_transpose, rocblas_operation_none, P, P, kc * m, &one, A,
+                              kc * m, A, kc * m, &beta, Cd, P) != rocblas_status_success)
+                rc = PIC_ERR_HIP;
+        }
+        std::vector<double> C((size_t)P * P);
+        if (rc == PIC_OK && hipMemcpyAsync(C.data(), Cd, (size_t)P * P * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            rc = PIC_ERR_HIP;
+        if (rc == PIC_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = PIC_ERR_HIP;
+        (void)hipFree(A);
+        (void)hipFree(Cd);
+        (void)hipFree(qrow_d);
+        qrow_d = nullptr;
+        if (rc != PIC_OK) {
+            fast_free(c);
+            set_error("direct Poisson solve: capacitance matrix assembly failed");
+            return rc;
+        }
+        if (!spd_inverse(C, P)) {
+            // not positive definite in floating point: leave the PCG alone
+            fast_free(c);
+            return PIC_OK;
+        }
+        PIC_HIP(hipMemcpy(f->Cinv, C.data(), (size_t)P * P * 8, hipMemcpyHostToDevice));
+    }
+    if (qrow_d) (void)hipFree(qrow_d);
+    PIC_HIP(hipDeviceSynchronize());
+    return PIC_OK;
+}
+
+// The solution of the current right-hand side (c->rho after overwriteRho)
+// into `out` (all nodes, Dirichlet values included).
+int fast_solve(pic_ctx* c, double* out) {
+    FastPoisson* f = c->fast;
+    const int gx = c->g.gx, gy = c->g.gy, m = f->m, n = f->n, P = f->P, nr = f->nr;
+    const int nm = m * n;
+    const double cx = 1.0 / (c->g.dx * c->g.dx), cy = 1.0 / (c->g.dy * c->g.dy);
+    hipStream_t s = c->stream;
+    const unsigned nb = (unsigned)((nm + kFdB - 1) / kFdB);
+    hipLaunchKernelGGL(k_fd_rhs, dim3(nb), dim3(kFdB), 0, s, gx, gy, cx, cy, c->rho, c->dir, c->dval, f->F);
+    // row-major X (n x m) is the column-major m x n matrix X^T:
+    // W^T = S_m F^T, then Fhat^T = W^T S_n
+    PIC_TRY(gemm(f->blas, m, n, m, f->Sm, m, f->F, m, 0.0, f->W, m));
+    PIC_TRY(gemm(f->blas, m, n, n, f->W, m, f->Sn, n, 0.0, f->U, m));
+    hipLaunchKernelGGL(k_fd_scale, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->Dinv);
+    if (P > 0) {
+        // u0 on Q: V1 = S_n[rows] Uhat (nr x m) -> V1^T = Uhat^T S_n[rows]^T
+        PIC_TRY(gemm(f->blas, m, nr, n, f->U, m, f->SnR, n, 0.0, f->V1, m));
+        hipLaunchKernelGGL(k_fd_u0q, dim3((unsigned)((P * 64 + kFdB - 1) / kFdB)), dim3(kFdB), 0, s, P, m, f->V1,
+                           f->Sm, f->qri, f->qcol, f->VQ, f->dq);
+        hipLaunchKernelGGL(k_fd_sources, dim3((unsigned)P), dim3(kFdB), 0, s, P, m, f->Cinv, f->dq, f->SmQ, f->Z);
+        // G = S_n[:, rows(Q)] Z (n x P x m) -> G^T = Z^T SnQ^T
+        PIC_TRY(gemm(f->blas, m, n, P, f->Z, m, f->SnQ, P, 0.0, f->G, m));
+        hipLaunchKernelGGL(k_fd_addscaled, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->G, f->Dinv);
+    }
+    // U = S_n Uhat S_m -> W^T = S_m Uhat^T, U^T = W^T S_n
+    PIC_TRY(gemm(f->blas, m, n, m, f->Sm, m, f->U, m, 0.0, f->W, m));
+    PIC_TRY(gemm(f->blas, m, n, n, f->W, m, f->Sn, n, 0.0, f->G, m));
+    hipLaunchKernelGGL(k_fd_out, dim3((unsigned)((gx * gy + kFdB - 1) / kFdB)), dim3(kFdB), 0, s, gx, gy, f->G,
+                       c->dir, c->dval, out);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+} // namespace pic

@@ -68,6 +68,7 @@ extern "C" void pic_params_default(pic_params* p) {
     p->mg_post = 2;
     p->mg_omega = 0.8;
     p->mg_smoother = 1;
+    p->poisson_direct = 1;
     p->poisson_guess = 0;
     p->poisson_precision = 0;
     p->poisson_inner_tol = 1e-4;

@@ -111,6 +111,8 @@ struct Geometry {
     double potential;
 };
 
+struct FastPoisson;   // direct Poisson solve (pic_fastpoisson.hip)
+
 // Particle injection state (pic_inject.hip).
 struct Injection {
     int on = 0;
@@ -178,6 +180,7 @@ struct pic_ctx {
     int scan_nb = 0;
     int32_t* scan_bsum = nullptr;
     pic::Injection inj;
+    pic::FastPoisson* fast = nullptr;   // direct solve of the current geometry (null: PCG only)
     // timing
     int timing = 0;
     pic::StageTimer timers[PIC_NSTAGES];
@@ -212,6 +215,11 @@ int launch_kinetic_energy(pic_ctx* c, int s, double* out);   // host result, syn
 int poisson_setup(pic_ctx* c);
 void poisson_free(pic_ctx* c);
 int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st);
+// direct solve (pic_fastpoisson.hip): set-up for the geometry masks, solve of
+// the current right-hand side into a full grid
+int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<double>& dval);
+void fast_free(pic_ctx* c);
+int fast_solve(pic_ctx* c, double* out);
 // weights over the diagonal of the two pre-smoothing sweeps of a level (the
 // post-smoothing uses them in reverse order)
 void mg_sweep_weights(const pic_ctx* c, const MgLevel& m, float* w0, float* w1);

@@ -360,11 +360,13 @@ __global__ void k_ir_update(int G, double* __restrict__ x, const float* __restri
 __global__ void k_reset_inner(int* flags) { flags[2] = 0; }
 
 // --------------------------------------------------------------- MG (FP32)
-// Level kernels use 2D grids (x: 256 nodes of a row, y: row) so node indices
-// need no division; `wd` is the Jacobi weight over the diagonal.
+// Level kernels use 2D grids of 64 x 4 node blocks so node indices need no
+// division and narrow coarse rows waste few lanes; `wd` is the Jacobi weight
+// over the diagonal.
 #define MG_IJ(L)                                                              \
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;     \
-    if (i >= (L).gx) return;                                                  \
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;                      \
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;                      \
+    if (i >= (L).gx || j >= (L).gy) return;                                   \
     const int n = j * (L).gx + i;
 
 template <typename T>
@@ -408,32 +410,63 @@ __global__ __launch_bounds__(kB) void k_mg_pre2(Lvl L, const TB* __restrict__ bi
     eo[n] = c + L.wd2 * (bn - ax);
 }
 
-// residual at node n of row j
-__device__ __forceinline__ float mg_res(const Lvl& L, const float* e, const float* b, const uint8_t* dir, int n,
-                                        int j) {
-    if (dir[n]) return 0.f;
-    const int so = south_of(L.per, j, L.gx, L.gy), no = north_of(L.per, j, L.gx, L.gy);
-    float ax = L.diag * e[n] - L.cx * (e[n - 1] + e[n + 1]) - L.cy * (e[n + so] + e[n + no]);
-    return b[n] - ax;
+// Row of the stored grid that region row rj stands for: periodic grids wrap
+// rows modulo gy - 1 (the image row gy - 1 reads as row 0), others give -1
+// outside [0, gy).
+__device__ __forceinline__ int grid_row(const Lvl& L, int rj) {
+    if (L.per) {
+        const int p = L.gy - 1;
+        return ((rj % p) + p) % p;
+    }
+    return (rj >= 0 && rj < L.gy) ? rj : -1;
 }
 
 // Coarse rhs = full weighting of the fine residual; zero on coarse Dirichlet.
-__global__ __launch_bounds__(kB) void k_mg_restrict(Lvl F, Lvl C, const float* __restrict__ e,
-                                                    const float* __restrict__ b, const uint8_t* __restrict__ fdir,
-                                                    float* __restrict__ bc, const uint8_t* __restrict__ cdir,
-                                                    const int* flags) {
+// One workgroup per kRt x kRt coarse nodes: the fine e of the covering region
+// (+1 halo) is staged in LDS, each fine residual is computed once into LDS,
+// then each thread sums the 3 x 3 weights of its coarse node.
+static constexpr int kRt = 16;
+static constexpr int kRe = 2 * kRt + 3;     // fine e region edge
+static constexpr int kRr = 2 * kRt + 1;     // fine residual region edge
+__global__ __launch_bounds__(kRt * kRt) void k_mg_restrict(Lvl F, Lvl C, const float* __restrict__ e,
+                                                           const float* __restrict__ b,
+                                                           const uint8_t* __restrict__ fdir, float* __restrict__ bc,
+                                                           const uint8_t* __restrict__ cdir, const int* flags) {
+    __shared__ float se[kRe * kRe];
+    __shared__ float sr[kRr * kRr];
     if (flags[0]) return;
-    MG_IJ(C)
+    const int tid = threadIdx.x;
+    const int I0 = blockIdx.x * kRt, J0 = blockIdx.y * kRt;
+    const int fi0 = 2 * I0 - 2, fj0 = 2 * J0 - 2;   // fine origin of the e region
+    for (int k = tid; k < kRe * kRe; k += kRt * kRt) {
+        const int rj = k / kRe, ri = k - rj * kRe;
+        const int gj = grid_row(F, fj0 + rj), fi = fi0 + ri;
+        se[k] = (gj >= 0 && fi >= 0 && fi < F.gx) ? e[gj * F.gx + fi] : 0.f;
+    }
+    __syncthreads();
+    for (int k = tid; k < kRr * kRr; k += kRt * kRt) {
+        const int rj = k / kRr, ri = k - rj * kRr;
+        const int gj = grid_row(F, fj0 + 1 + rj), fi = fi0 + 1 + ri;
+        float v = 0.f;
+        if (gj >= 0 && fi >= 0 && fi < F.gx) {
+            const int n = gj * F.gx + fi;
+            if (!fdir[n]) {
+                const float* c = &se[(rj + 1) * kRe + ri + 1];
+                const float ax = F.diag * c[0] - F.cx * (c[-1] + c[1]) - F.cy * (c[-kRe] + c[kRe]);
+                v = b[n] - ax;
+            }
+        }
+        sr[k] = v;
+    }
+    __syncthreads();
+    const int tx = tid % kRt, ty = tid / kRt;
+    const int I = I0 + tx, J = J0 + ty;
+    if (I >= C.gx || J >= C.gy) return;
+    const int n = J * C.gx + I;
     if (cdir[n]) { bc[n] = 0.f; return; }
-    const int fj = 2 * j;
-    const int f = fj * F.gx + 2 * i;
-    const int js = row_s(F.per, fj, F.gy), jn = row_n(F.per, fj, F.gy);
-    const int fs = f + south_of(F.per, fj, F.gx, F.gy), fn = f + north_of(F.per, fj, F.gx, F.gy);
-    float s = 4.f * mg_res(F, e, b, fdir, f, fj)
-            + 2.f * (mg_res(F, e, b, fdir, f - 1, fj) + mg_res(F, e, b, fdir, f + 1, fj)
-                   + mg_res(F, e, b, fdir, fs, js) + mg_res(F, e, b, fdir, fn, jn))
-            + (mg_res(F, e, b, fdir, fs - 1, js) + mg_res(F, e, b, fdir, fs + 1, js)
-             + mg_res(F, e, b, fdir, fn - 1, jn) + mg_res(F, e, b, fdir, fn + 1, jn));
+    const float* c = &sr[(2 * ty + 1) * kRr + 2 * tx + 1];
+    const float s = 4.f * c[0] + 2.f * (c[-1] + c[1] + c[-kRr] + c[kRr])
+                  + (c[-kRr - 1] + c[-kRr + 1] + c[kRr - 1] + c[kRr + 1]);
     bc[n] = s * 0.0625f;
 }
 
@@ -460,22 +493,38 @@ __global__ __launch_bounds__(kB) void k_mg_prolong(Lvl F, Lvl C, float* __restri
     e[n] += prolong_at(C, ec, i, j);
 }
 
-// Prolongation fused with the first post-smoothing sweep:
-// e1 = e + P ec (computed at the node and its four neighbours), eo = one
-// weighted-Jacobi sweep of e1.
-__global__ __launch_bounds__(kB) void k_mg_prolong_sweep(Lvl F, Lvl C, const float* __restrict__ e,
-                                                         const float* __restrict__ ec, const float* __restrict__ b,
-                                                         const uint8_t* __restrict__ dir, float* __restrict__ eo,
-                                                         const int* flags) {
+// Prolongation fused with the first post-smoothing sweep: e1 = e + P ec on a
+// kPx x kPy fine tile and its one-node halo, computed once per node into LDS,
+// then eo = one weighted-Jacobi sweep of e1 on the tile.
+static constexpr int kPx = 64, kPy = 4;
+__global__ __launch_bounds__(kPx * kPy) void k_mg_prolong_sweep(Lvl F, Lvl C, const float* __restrict__ e,
+                                                                const float* __restrict__ ec,
+                                                                const float* __restrict__ b,
+                                                                const uint8_t* __restrict__ dir,
+                                                                float* __restrict__ eo, const int* flags) {
+    constexpr int W = kPx + 2, H = kPy + 2;
+    __shared__ float s1[W * H];
     if (flags[0]) return;
-    MG_IJ(F)
+    const int tid = threadIdx.y * kPx + threadIdx.x;
+    const int i0 = blockIdx.x * kPx, j0 = blockIdx.y * kPy;
+    for (int k = tid; k < W * H; k += kPx * kPy) {
+        const int rj = k / W, ri = k - rj * W;
+        const int gj = grid_row(F, j0 - 1 + rj), fi = i0 - 1 + ri;
+        float v = 0.f;
+        if (gj >= 0 && fi >= 0 && fi < F.gx) {
+            const int m = gj * F.gx + fi;
+            if (!dir[m]) v = e[m] + prolong_at(C, ec, fi, gj);
+        }
+        s1[k] = v;
+    }
+    __syncthreads();
+    const int i = i0 + (int)threadIdx.x, j = j0 + (int)threadIdx.y;
+    if (i >= F.gx || j >= F.gy) return;
+    const int n = j * F.gx + i;
     if (dir[n]) { eo[n] = 0.f; return; }
-    auto e1 = [&](int m, int mi, int mj) -> float { return dir[m] ? 0.f : e[m] + prolong_at(C, ec, mi, mj); };
-    const int so = south_of(F.per, j, F.gx, F.gy), no = north_of(F.per, j, F.gx, F.gy);
-    float c = e1(n, i, j);
-    float ax = F.diag * c - F.cx * (e1(n - 1, i - 1, j) + e1(n + 1, i + 1, j))
-             - F.cy * (e1(n + so, i, row_s(F.per, j, F.gy)) + e1(n + no, i, row_n(F.per, j, F.gy)));
-    eo[n] = c + F.wd * (b[n] - ax);
+    const float* c = &s1[(threadIdx.y + 1) * W + threadIdx.x + 1];
+    const float ax = F.diag * c[0] - F.cx * (c[-1] + c[1]) - F.cy * (c[-W] + c[W]);
+    eo[n] = c[0] + F.wd * (b[n] - ax);
 }
 
 
@@ -825,7 +874,11 @@ static Lvl lvl_post(const pic_ctx* c, const MgLevel& m, int k) {
     return L;
 }
 
-static dim3 grid2d(const MgLevel& m) { return dim3((unsigned)((m.gx + kB - 1) / kB), (unsigned)m.gy); }
+static const dim3 kMgBlock(64, 4);
+static dim3 grid2d(const MgLevel& m) { return dim3((unsigned)((m.gx + 63) / 64), (unsigned)((m.gy + 3) / 4)); }
+static dim3 grid_restrict(const MgLevel& coarse) {
+    return dim3((unsigned)((coarse.gx + kRt - 1) / kRt), (unsigned)((coarse.gy + kRt - 1) / kRt));
+}
 
 void poisson_free(pic_ctx* c) {
     for (auto& m : c->mg) {
@@ -841,6 +894,7 @@ void poisson_free(pic_ctx* c) {
     c->mg_inv = nullptr;
     c->mg_unk = nullptr;
     c->mg_cdir = nullptr;
+    fast_free(c);
     c->mg_nu = 0;
     c->mg_lc = 0;
     c->mg_ctotal = 0;
@@ -1013,6 +1067,7 @@ int poisson_setup(pic_ctx* c) {
     PIC_HIP(hipMalloc(&c->mg_cdir, cd.size()));
     PIC_HIP(hipMemcpy(c->mg_cdir, cd.data(), cd.size(), hipMemcpyHostToDevice));
     PIC_HIP(hipDeviceSynchronize());
+    PIC_TRY(fast_setup(c, dir, dval));
     return PIC_OK;
 }
 
@@ -1052,13 +1107,13 @@ static void presmooth(pic_ctx* c, MgLevel& m, const int* flags) {
     float* a = m.e;
     float* b = m.t;
     for (int s = 0; s < c->p.mg_pre; ++s) {
-        hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, c->stream, lv, a, b, m.b, m.dir, s == 0 ? 1 : 0, flags);
+        hipLaunchKernelGGL(k_mg_smooth, g2, kMgBlock, 0, c->stream, lv, a, b, m.b, m.dir, s == 0 ? 1 : 0, flags);
         std::swap(a, b);
     }
     if (c->p.mg_pre == 0) {
         Lvl zl = lv;
         zl.wd = 0.f;
-        hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, c->stream, zl, b, a, m.b, m.dir, 1, flags);
+        hipLaunchKernelGGL(k_mg_smooth, g2, kMgBlock, 0, c->stream, zl, b, a, m.b, m.dir, 1, flags);
     }
     if (a != m.e) std::swap(m.e, m.t);
 }
@@ -1077,13 +1132,14 @@ static int vcycle_rest(pic_ctx* c, const T* r, float* z, const float* zold, int 
         MgLevel& m = c->mg[l];
         if (l > 0) {
             if (fpre)
-                hipLaunchKernelGGL(k_mg_pre2<float>, grid2d(m), dim3(kB), 0, st, lvl_pre(c, m), m.b, (float*)nullptr,
+                hipLaunchKernelGGL(k_mg_pre2<float>, grid2d(m), kMgBlock, 0, st, lvl_pre(c, m), m.b, (float*)nullptr,
                                    m.e, m.dir, flags);
             else
                 presmooth(c, m, flags);
         }
         MgLevel& n = c->mg[l + 1];
-        hipLaunchKernelGGL(k_mg_restrict, grid2d(n), dim3(kB), 0, st, lvl_of(m, w), lvl_of(n, w), m.e, m.b, m.dir,
+        hipLaunchKernelGGL(k_mg_restrict, grid_restrict(n), dim3(kRt * kRt), 0, st, lvl_of(m, w), lvl_of(n, w), m.e,
+                           m.b, m.dir,
                            n.b, n.dir, flags);
     }
     {
@@ -1126,18 +1182,18 @@ static int vcycle_rest(pic_ctx* c, const T* r, float* z, const float* zold, int 
         // post-smoothing: all sweeps but the last leave the correction in m.e
         int rest;
         if (fpost) {
-            hipLaunchKernelGGL(k_mg_prolong_sweep, g2, dim3(kB), 0, st, lvl_post(c, m, 0), lvl_of(n, w), m.e, n.e,
+            hipLaunchKernelGGL(k_mg_prolong_sweep, g2, dim3(kPx, kPy), 0, st, lvl_post(c, m, 0), lvl_of(n, w), m.e, n.e,
                                m.b, m.dir, m.t, flags);
             std::swap(m.e, m.t);
             rest = 1;
             lv = lvl_post(c, m, 1);
         } else {
-            hipLaunchKernelGGL(k_mg_prolong, g2, dim3(kB), 0, st, lv, lvl_of(n, w), m.e, m.dir, n.e, flags);
+            hipLaunchKernelGGL(k_mg_prolong, g2, kMgBlock, 0, st, lv, lvl_of(n, w), m.e, m.dir, n.e, flags);
             rest = c->p.mg_post;
         }
         const int last_here = l == 0 ? 1 : 0;         // the finest level's last sweep is k_smooth_dots
         for (int s = 0; s < rest - last_here; ++s) {
-            hipLaunchKernelGGL(k_mg_smooth, g2, dim3(kB), 0, st, lv, m.e, m.t, m.b, m.dir, 0, flags);
+            hipLaunchKernelGGL(k_mg_smooth, g2, kMgBlock, 0, st, lv, m.e, m.t, m.b, m.dir, 0, flags);
             std::swap(m.e, m.t);
         }
         if (l == 0) {
@@ -1172,7 +1228,7 @@ static int run_pcg(pic_ctx* c, T* x, T* const R[2], T* const P[2], T* q, float* 
     const Lvl l0 = lvl_pre(c, m0);
     const int flag_off = (int)(flags - c->flags);
     if (c->p.mg_pre == 2) {
-        hipLaunchKernelGGL(k_mg_pre2<T>, grid2d(m0), dim3(kB), 0, s, l0, R[0], m0.b, m0.e, m0.dir, flags);
+        hipLaunchKernelGGL(k_mg_pre2<T>, grid2d(m0), kMgBlock, 0, s, l0, R[0], m0.b, m0.e, m0.dir, flags);
     } else {
         hipLaunchKernelGGL(k_d2f<T>, dim3(nblk(c->G)), dim3(kB), 0, s, c->G, R[0], m0.b, flags);
         presmooth(c, m0, flags);
@@ -1286,6 +1342,14 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
                 c->learn_used += 1;
             }
         }
+    }
+    // direct solve: the PCG below starts from it and only checks (or
+    // corrects) the residual
+    if (c->fast && !mixed && !learned && !c->guess_set) {
+        PIC_TRY(fast_solve(c, c->guess));
+        src = c->guess;
+        prev = nullptr;
+        guess = 5;
     }
     hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, src, prev, keep, c->dir, c->dval);
     hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, R[0], part + P_RR * kNP);

@@ -51,7 +51,7 @@ class Params(C.Structure):
         ("mg_pre", C.c_int), ("mg_post", C.c_int), ("mg_omega", C.c_double),
         ("poisson_guess", C.c_int),
         ("poisson_precision", C.c_int), ("poisson_inner_tol", C.c_double),
-        ("mg_smoother", C.c_int),
+        ("mg_smoother", C.c_int), ("poisson_direct", C.c_int),
     ]
 
 

@@ -104,6 +104,11 @@ typedef struct pic_params {
     double poisson_inner_tol; /* relative residual of each FP32 correction solve (precision 1)  */
     int    mg_smoother;       /* 0: weighted Jacobi (mg_omega); 1: Chebyshev steps on the upper
                                  spectrum of D^-1 A (with mg_pre = mg_post = 2)                */
+    int    poisson_direct;    /* 1: solve exactly (sine transforms as FP64 GEMMs + capacitance
+                                 correction for internal Dirichlet nodes) and let the PCG only
+                                 check the result; used for non-periodic grids of at most 2047
+                                 interior nodes per direction and 1024 internal Dirichlet nodes
+                                 next to unknowns, with precision 0 and no learned guess        */
 } pic_params;
 
 /* Per-solve statistics of pic_poisson_solve. */
@@ -114,7 +119,7 @@ typedef struct pic_solve_stats {
     double ms;                /* device time of the solve (HIP events)             */
     int    guess;             /* 0 zero guess, 1 previous phi, 2 caller-set guess,
                                  3 extrapolated from the last two solves,
-                                 4 learned (U-Net) prediction                      */
+                                 4 learned (U-Net) prediction, 5 direct solve      */
     int    inner_iterations;  /* PCG iterations in all (precision 1: FP32 ones)   */
 } pic_solve_stats;
 

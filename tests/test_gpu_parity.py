@@ -204,10 +204,68 @@ def test_poisson_electrode_vs_direct(hip_mod, oracle_mod):
     rhs, phi_ref, _, _ = O.poisson_solver(rho, c.gx, c.gy, c.dx, c.dy, ELEC)
     phi = c.grid(hip.GRID_PHI)
     assert np.abs(phi - phi_ref).max() / np.abs(phi_ref).max() < 1e-10
-    # warm start from the solution converges immediately
+    # the direct solve leaves the PCG nothing to do
+    assert st["guess"] == 5 and st["iterations"] == 0 and st["final_relres"] <= 1e-12
+    c.close()
+    # the PCG alone: same potential; warm start from the solution converges at once
+    c = _ctx(hip, NX, 0, geom=ELEC, poisson_tol=1e-12, poisson_direct=0)
+    c.set_grid(hip.GRID_RHO, rho)
+    st = c.poisson_solve(0)
+    assert st["guess"] == 0 and st["iterations"] > 2
+    assert np.abs(c.grid(hip.GRID_PHI) - phi_ref).max() / np.abs(phi_ref).max() < 1e-10
     c.set_grid(hip.GRID_RHO, rho)
     st2 = c.poisson_solve(1)
     assert st2["iterations"] <= 1 and st2["guess"] == 1
+    c.close()
+
+
+@pytest.mark.parametrize("case", ["walls_only", "biased_walls_rect", "two_electrodes", "too_many_for_direct"])
+def test_direct_poisson_solve(hip_mod, oracle_mod, case):
+    """The direct solve (sine transforms + capacitance correction) against the
+    oracle's direct solve of the reference's system, to 1e-12 relative: no
+    internal Dirichlet nodes, walls at different potentials on a non-square
+    grid, two electrodes, and a geometry with more internal Dirichlet nodes
+    next to unknowns than the direct path takes (the PCG solves alone)."""
+    hip, O = hip_mod, oracle_mod
+    NX, NY = 128, 128
+    geom = WALLS
+    if case == "biased_walls_rect":
+        NX, NY = 96, 160
+        geom = [[0, 0, 0, 1, 5.0], [1, 0, 1, 1, -3.0], [0, 1, 1, 1, 1.0], [0, 0, 1, 0, 0.0],
+                [0.25, 0.5, 0.375, 0.625, 40.0]]
+    elif case == "two_electrodes":
+        geom = WALLS + [[0.2, 0.2, 0.3, 0.6, 100.0], [0.6, 0.5, 0.8, 0.55, -50.0]]
+    elif case == "too_many_for_direct":
+        # three electrodes with 1588 nodes next to unknowns (the direct path takes 1024)
+        geom = WALLS + [[0.1, 0.1, 0.35, 0.9, 10.0], [0.4, 0.1, 0.6, 0.9, -10.0], [0.65, 0.1, 0.9, 0.9, 5.0]]
+        NX = NY = 256
+    dx = 1e-4
+    p = hip.default_params(NX=NX, NY=NY, width=NX * dx, height=NY * dx, dt=1.77e-11, layout=0, poisson_tol=1e-12)
+    c = hip.Context(p)
+    c.set_geometry(geom)
+    rng = np.random.default_rng(11)
+    rho = rng.normal(0, 1e6, c.G)
+    c.set_grid(hip.GRID_RHO, rho)
+    st = c.poisson_solve(0)
+    rhs, phi_ref, ex_ref, ey_ref = O.poisson_solver(rho, c.gx, c.gy, c.dx, c.dy, geom)
+    phi = c.grid(hip.GRID_PHI)
+    # the direct solve to 1e-12; the PCG alone stops at relative residual 1e-12
+    tol = 1e-10 if case == "too_many_for_direct" else 1e-12
+    assert np.abs(phi - phi_ref).max() / np.abs(phi_ref).max() < tol, case
+    assert np.array_equal(c.grid(hip.GRID_RHO), rhs)
+    ex, ey = O.efield(phi, c.gx, c.gy, c.dx, c.dy)
+    assert np.array_equal(c.grid(hip.GRID_EX), ex) and np.array_equal(c.grid(hip.GRID_EY), ey)
+    if case == "too_many_for_direct":
+        assert st["guess"] == 0 and st["iterations"] > 0
+    else:
+        assert st["guess"] == 5 and st["iterations"] == 0, st
+        # a second solve with a new right-hand side: the same set-up serves it
+        rho2 = rng.normal(0, 1e6, c.G)
+        c.set_grid(hip.GRID_RHO, rho2)
+        st2 = c.poisson_solve(1)
+        _, phi2_ref, _, _ = O.poisson_solver(rho2, c.gx, c.gy, c.dx, c.dy, geom)
+        assert np.abs(c.grid(hip.GRID_PHI) - phi2_ref).max() / np.abs(phi2_ref).max() < 1e-12
+        assert st2["guess"] == 5 and st2["iterations"] == 0
     c.close()
 
 

@@ -110,8 +110,9 @@ def test_learned_guess_in_the_solver(hip_mod):
     import pic_configs as cfg
     w = cfg.WORKLOADS["electrode_512"].scaled(NX=256, n=200_000, name="learned_256")
     parts = [cfg.generate(w, s) for s in range(2)]
-    a = cfg.make_context(w, particles=parts, poisson_tol=1e-8)
-    b = cfg.make_context(w, particles=parts, poisson_tol=1e-8)
+    # both on the PCG (the direct solve would leave the plain run nothing to iterate)
+    a = cfg.make_context(w, particles=parts, poisson_tol=1e-8, poisson_direct=0)
+    b = cfg.make_context(w, particles=parts, poisson_tol=1e-8, poisson_direct=0)
     b.learned_enable(train_steps=40, min_trained=20, lr=1e-3, seed=1)
     sa, sb = [], []
     for it in range(60):
