@@ -35,22 +35,28 @@ namespace pic {
 static constexpr int kFdMaxP = 1024;      // internal Dirichlet nodes next to unknowns
 static constexpr int kFdMaxN = 2047;      // interior nodes per direction
 static constexpr int kFdB = 256;
+static constexpr int kLines = 16;         // covering lines handled by the line kernels (more: GEMMs)
+static constexpr int kSeg = 16;           // row segments of the row-line products
 
 struct FastPoisson {
-    int m = 0, n = 0, P = 0, nr = 0;
+    int m = 0, n = 0, P = 0, nr = 0, nc = 0;
     rocblas_handle blas = nullptr;
     double *Sm = nullptr, *Sn = nullptr;     // m x m, n x n
     double* Dinv = nullptr;                  // n x m
     double *F = nullptr, *W = nullptr, *U = nullptr, *G = nullptr;   // n x m work
-    double* SnR = nullptr;                   // nr x n: rows of S_n at the rows of Q
-    double* V1 = nullptr;                    // nr x m
+    double* SnR = nullptr;                   // nr x n: rows of S_n at the covering rows
+    double* V1 = nullptr;                    // nr x m: S_n[rows] Uhat
+    double* SmC = nullptr;                   // nc x m: rows of S_m at the covering columns
+    double* V2 = nullptr;                    // n x nc (column-major): Uhat S_m[:, cols]
+    double* part = nullptr;                  // kSeg x nr x m partial row-line products
     double* SnQ = nullptr;                   // n x P: S_n[k][row(q)]
     double* SmQ = nullptr;                   // P x m: S_m[col(q)][l]
     double* Z = nullptr;                     // P x m: c_q S_m[col(q)][l]
     double* Cinv = nullptr;                  // P x P
     double* VQ = nullptr;                    // P: potentials on Q
     double* dq = nullptr;                    // P: V_Q - u0_Q
-    int *qri = nullptr, *qcol = nullptr;     // P: index into the row list, column of q
+    int *qri = nullptr, *qcol = nullptr;     // P: covering line of q (row a >= 0, column j as -1 - j); column of q
+    int* qrow = nullptr;                     // P: row of q
 };
 
 static void* dmalloc(size_t bytes, bool& ok) {
@@ -65,8 +71,8 @@ static void* dmalloc(size_t bytes, bool& ok) {
 void fast_free(pic_ctx* c) {
     FastPoisson* f = c->fast;
     if (!f) return;
-    void* ptrs[] = {f->Sm, f->Sn, f->Dinv, f->F, f->W, f->U, f->G, f->SnR, f->V1, f->SnQ, f->SmQ, f->Z,
-                    f->Cinv, f->VQ, f->dq, f->qri, f->qcol};
+    void* ptrs[] = {f->Sm, f->Sn, f->Dinv, f->F, f->W, f->U, f->G, f->SnR, f->V1, f->SmC, f->V2, f->SnQ, f->SmQ,
+                    f->Z, f->Cinv, f->VQ, f->dq, f->qri, f->qcol, f->qrow, f->part};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (f->blas) (void)rocblas_destroy_handle(f->blas);
@@ -113,19 +119,95 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// d_q = V_q - u0(row q, col q), u0(r, c) = sum_l V1[r][l] S_m[c][l]; one wave per q
-__global__ __launch_bounds__(kFdB) void k_fd_u0q(int P, int m, const double* __restrict__ V1,
+// d_q = V_q - u0(r, c) at q = (r, c), u0 = S_n Uhat S_m.  A q covered by row a
+// of the line cover: u0 = V1[a] . S_m[c]; by column j: u0 = S_n[r] . V2[:, j].
+// One wave per q.
+__global__ __launch_bounds__(kFdB) void k_fd_u0q(int P, int n, int m, const double* __restrict__ V1,
+                                                 const double* __restrict__ V2, const double* __restrict__ Sn,
                                                  const double* __restrict__ Sm, const int* __restrict__ qri,
-                                                 const int* __restrict__ qcol, const double* __restrict__ VQ,
-                                                 double* __restrict__ dq) {
+                                                 const int* __restrict__ qrow, const int* __restrict__ qcol,
+                                                 const double* __restrict__ VQ, double* __restrict__ dq) {
     const int q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
     if (q >= P) return;
-    const double* a = V1 + (size_t)qri[q] * m;
-    const double* b = Sm + (size_t)qcol[q] * m;
+    const int line = qri[q];
+    const double *a, *b;
+    int len;
+    if (line >= 0) {
+        a = V1 + (size_t)line * m;
+        b = Sm + (size_t)qcol[q] * m;
+        len = m;
+    } else {
+        a = V2 + (size_t)(-1 - line) * n;
+        b = Sn + (size_t)qrow[q] * n;
+        len = n;
+    }
     double s = 0.0;
-    for (int l = lane; l < m; l += 64) s += a[l] * b[l];
+    for (int l = lane; l < len; l += 64) s += a[l] * b[l];
     s = wave_sum(s);
     if (lane == 0) dq[q] = VQ[q] - s;
+}
+
+// Row lines: V1[a][l] = sum_k S_n[row a][k] Uhat[k][l] for nr <= kLines rows,
+// one pass over Uhat: workgroup (64 columns, one of kSeg row segments), four
+// waves over interleaved rows, partial sums per segment (added in a fixed
+// order by k_fd_rowsum).
+__global__ __launch_bounds__(kFdB) void k_fd_rowlines(int nr, int n, int m, const double* __restrict__ SnR,
+                                                      const double* __restrict__ U, double* __restrict__ part) {
+    __shared__ double sh[4][kLines][64];
+    const int lx = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const int l = blockIdx.x * 64 + lx;
+    const int per = (n + kSeg - 1) / kSeg;
+    const int k0 = blockIdx.y * per, k1 = min(n, k0 + per);
+    double acc[kLines];
+#pragma unroll
+    for (int a = 0; a < kLines; ++a) acc[a] = 0.0;
+    if (l < m)
+        for (int k = k0 + ph; k < k1; k += 4) {
+            const double u = U[(size_t)k * m + l];
+#pragma unroll
+            for (int a = 0; a < kLines; ++a)
+                if (a < nr) acc[a] += SnR[(size_t)a * n + k] * u;
+        }
+#pragma unroll
+    for (int a = 0; a < kLines; ++a) sh[ph][a][lx] = acc[a];
+    __syncthreads();
+    if (ph == 0 && l < m)
+        for (int a = 0; a < nr; ++a)
+            part[((size_t)blockIdx.y * nr + a) * m + l] = (sh[0][a][lx] + sh[1][a][lx]) + (sh[2][a][lx] + sh[3][a][lx]);
+}
+
+__global__ __launch_bounds__(kFdB) void k_fd_rowsum(int nr, int m, const double* __restrict__ part,
+                                                    double* __restrict__ V1) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nr * m) return;
+    double s = 0.0;
+    for (int seg = 0; seg < kSeg; ++seg) s += part[(size_t)seg * nr * m + k];
+    V1[k] = s;
+}
+
+// Column lines: V2[k][j] = Uhat[k] . S_m[col j] for nc <= kLines columns; one
+// workgroup per row k (the row read once).
+__global__ __launch_bounds__(kFdB) void k_fd_collines(int nc, int n, int m, const double* __restrict__ U,
+                                                      const double* __restrict__ SmC, double* __restrict__ V2) {
+    __shared__ double sh[kLines][kFdB / 64];
+    const int k = blockIdx.x;
+    double acc[kLines];
+#pragma unroll
+    for (int a = 0; a < kLines; ++a) acc[a] = 0.0;
+    for (int l = threadIdx.x; l < m; l += kFdB) {
+        const double u = U[(size_t)k * m + l];
+#pragma unroll
+        for (int a = 0; a < kLines; ++a)
+            if (a < nc) acc[a] += u * SmC[(size_t)a * m + l];
+    }
+#pragma unroll
+    for (int a = 0; a < kLines; ++a) {
+        const double v = wave_sum(acc[a]);
+        if ((threadIdx.x & 63) == 0) sh[a][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < nc) V2[(size_t)t * n + k] = (sh[t][0] + sh[t][1]) + (sh[t][2] + sh[t][3]);
 }
 
 // c_q = (C^-1 d)_q, then Z[q][l] = c_q S_m[col q][l]; one workgroup per q
@@ -279,14 +361,36 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
         set_error("rocBLAS set stream failed");
         return PIC_ERR_HIP;
     }
-    // distinct rows of Q and each q's index among them
-    std::vector<int> rows(qrow);
-    std::sort(rows.begin(), rows.end());
-    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
-    std::vector<int> qri(P);
-    for (int q = 0; q < P; ++q) qri[q] = (int)(std::lower_bound(rows.begin(), rows.end(), qrow[q]) - rows.begin());
+    // Cover Q greedily with rows and columns (the line holding most uncovered
+    // nodes first): u0 on Q then needs one product per covering line, e.g. two
+    // rows and two columns for a rectangular electrode.
+    std::vector<int> rows, cols, qri(P, 0);
+    {
+        std::vector<char> done(P, 0);
+        int left = P;
+        while (left > 0) {
+            std::vector<int> rc(n, 0), cc(m, 0);
+            for (int q = 0; q < P; ++q)
+                if (!done[q]) {
+                    ++rc[qrow[q]];
+                    ++cc[qcol[q]];
+                }
+            const int br = (int)(std::max_element(rc.begin(), rc.end()) - rc.begin());
+            const int bc = (int)(std::max_element(cc.begin(), cc.end()) - cc.begin());
+            const bool by_row = rc[br] >= cc[bc];
+            for (int q = 0; q < P; ++q) {
+                if (done[q] || (by_row ? qrow[q] != br : qcol[q] != bc)) continue;
+                done[q] = 1;
+                --left;
+                qri[q] = by_row ? (int)rows.size() : -1 - (int)cols.size();
+            }
+            if (by_row) rows.push_back(br);
+            else cols.push_back(bc);
+        }
+    }
     f->nr = (int)rows.size();
-    const int nr = f->nr;
+    f->nc = (int)cols.size();
+    const int nr = f->nr, nc = f->nc;
 
     const std::vector<double> Sm = dst_matrix(m);
     const std::vector<double> Sn = (n == m) ? Sm : dst_matrix(n);
@@ -299,9 +403,12 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
             Dinv[(size_t)k * m + l] = 1.0 / (ly + lx);
         }
     }
-    std::vector<double> SnR((size_t)std::max(nr, 1) * n), SnQ((size_t)n * std::max(P, 1)), SmQ((size_t)std::max(P, 1) * m);
+    std::vector<double> SnR((size_t)std::max(nr, 1) * n), SmC((size_t)std::max(nc, 1) * m);
+    std::vector<double> SnQ((size_t)n * std::max(P, 1)), SmQ((size_t)std::max(P, 1) * m);
     for (int a = 0; a < nr; ++a)
         for (int k = 0; k < n; ++k) SnR[(size_t)a * n + k] = Sn[(size_t)rows[a] * n + k];
+    for (int j = 0; j < nc; ++j)
+        for (int l = 0; l < m; ++l) SmC[(size_t)j * m + l] = Sm[(size_t)cols[j] * m + l];
     for (int k = 0; k < n; ++k)
         for (int q = 0; q < P; ++q) SnQ[(size_t)k * P + q] = Sn[(size_t)qrow[q] * n + k];
     for (int q = 0; q < P; ++q)
@@ -318,6 +425,10 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
     f->G = (double*)dmalloc(nm * 8, ok);
     f->SnR = (double*)dmalloc(SnR.size() * 8, ok);
     f->V1 = (double*)dmalloc((size_t)std::max(nr, 1) * m * 8, ok);
+    f->SmC = (double*)dmalloc(SmC.size() * 8, ok);
+    f->V2 = (double*)dmalloc((size_t)std::max(nc, 1) * n * 8, ok);
+    f->qrow = (int*)dmalloc((size_t)std::max(P, 1) * 4, ok);
+    if (nr > 0 && nr <= kLines) f->part = (double*)dmalloc((size_t)kSeg * nr * m * 8, ok);
     f->SnQ = (double*)dmalloc(SnQ.size() * 8, ok);
     f->SmQ = (double*)dmalloc(SmQ.size() * 8, ok);
     f->Z = (double*)dmalloc(SmQ.size() * 8, ok);
@@ -337,6 +448,7 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
     PIC_HIP(hipMemcpy(f->Sn, Sn.data(), (size_t)n * n * 8, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(f->Dinv, Dinv.data(), nm * 8, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(f->SnR, SnR.data(), SnR.size() * 8, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(f->SmC, SmC.data(), SmC.size() * 8, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(f->SnQ, SnQ.data(), SnQ.size() * 8, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(f->SmQ, SmQ.data(), SmQ.size() * 8, hipMemcpyHostToDevice));
     if (P > 0) {
@@ -344,6 +456,7 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
         PIC_HIP(hipMemcpy(f->qri, qri.data(), (size_t)P * 4, hipMemcpyHostToDevice));
         PIC_HIP(hipMemcpy(f->qcol, qcol.data(), (size_t)P * 4, hipMemcpyHostToDevice));
         PIC_HIP(hipMemcpy(qrow_d, qrow.data(), (size_t)P * 4, hipMemcpyHostToDevice));
+        PIC_HIP(hipMemcpy(f->qrow, qrow.data(), (size_t)P * 4, hipMemcpyHostToDevice));
         // C = A A^T over the spectral index, in chunks of kc rows of k
         const int kc = std::max(1, std::min(n, (int)((size_t)(256u << 20) / ((size_t)P * m * 8))));
         double* A = nullptr;
@@ -390,7 +503,7 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
 // into `out` (all nodes, Dirichlet values included).
 int fast_solve(pic_ctx* c, double* out) {
     FastPoisson* f = c->fast;
-    const int gx = c->g.gx, gy = c->g.gy, m = f->m, n = f->n, P = f->P, nr = f->nr;
+    const int gx = c->g.gx, gy = c->g.gy, m = f->m, n = f->n, P = f->P, nr = f->nr, nc = f->nc;
     const int nm = m * n;
     const double cx = 1.0 / (c->g.dx * c->g.dx), cy = 1.0 / (c->g.dy * c->g.dy);
     hipStream_t s = c->stream;
@@ -402,10 +515,26 @@ int fast_solve(pic_ctx* c, double* out) {
     PIC_TRY(gemm(f->blas, m, n, n, f->W, m, f->Sn, n, 0.0, f->U, m));
     hipLaunchKernelGGL(k_fd_scale, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->Dinv);
     if (P > 0) {
-        // u0 on Q: V1 = S_n[rows] Uhat (nr x m) -> V1^T = Uhat^T S_n[rows]^T
-        PIC_TRY(gemm(f->blas, m, nr, n, f->U, m, f->SnR, n, 0.0, f->V1, m));
-        hipLaunchKernelGGL(k_fd_u0q, dim3((unsigned)((P * 64 + kFdB - 1) / kFdB)), dim3(kFdB), 0, s, P, m, f->V1,
-                           f->Sm, f->qri, f->qcol, f->VQ, f->dq);
+        // u0 on Q through the covering lines: V1 = S_n[rows] Uhat (nr x m),
+        // V1^T = Uhat^T S_n[rows]^T; V2 = Uhat S_m[cols]^T (n x nc, column-major)
+        // (a few lines: one pass over Uhat each; many lines: GEMMs)
+        if (nr > 0 && nr <= kLines) {
+            hipLaunchKernelGGL(k_fd_rowlines, dim3((unsigned)((m + 63) / 64), kSeg), dim3(kFdB), 0, s, nr, n, m,
+                               f->SnR, f->U, f->part);
+            hipLaunchKernelGGL(k_fd_rowsum, dim3((unsigned)((nr * m + kFdB - 1) / kFdB)), dim3(kFdB), 0, s, nr, m,
+                               f->part, f->V1);
+        } else if (nr > 0) {
+            PIC_TRY(gemm(f->blas, m, nr, n, f->U, m, f->SnR, n, 0.0, f->V1, m));
+        }
+        if (nc > 0 && nc <= kLines) {
+            hipLaunchKernelGGL(k_fd_collines, dim3((unsigned)n), dim3(kFdB), 0, s, nc, n, m, f->U, f->SmC, f->V2);
+        } else if (nc > 0) {
+            const double one = 1.0, zero = 0.0;
+            FD_BLAS(rocblas_dgemm(f->blas, rocblas_operation_transpose, rocblas_operation_none, n, nc, m, &one, f->U,
+                                  m, f->SmC, m, &zero, f->V2, n));
+        }
+        hipLaunchKernelGGL(k_fd_u0q, dim3((unsigned)((P * 64 + kFdB - 1) / kFdB)), dim3(kFdB), 0, s, P, n, m, f->V1,
+                           f->V2, f->Sn, f->Sm, f->qri, f->qrow, f->qcol, f->VQ, f->dq);
         hipLaunchKernelGGL(k_fd_sources, dim3((unsigned)P), dim3(kFdB), 0, s, P, m, f->Cinv, f->dq, f->SmQ, f->Z);
         // G = S_n[:, rows(Q)] Z (n x P x m) -> G^T = Z^T SnQ^T
         PIC_TRY(gemm(f->blas, m, n, P, f->Z, m, f->SnQ, P, 0.0, f->G, m));

@@ -19,10 +19,22 @@ void set_error(const char* fmt, ...) {
     va_end(ap);
 }
 
+// Stage events come from a pool (returned by pic_stage_times) so the timed
+// loop does not create events.
+static int timer_event(pic_ctx* c, hipEvent_t* ev) {
+    if (!c->event_pool.empty()) {
+        *ev = c->event_pool.back();
+        c->event_pool.pop_back();
+        return PIC_OK;
+    }
+    PIC_HIP(hipEventCreate(ev));
+    return PIC_OK;
+}
+
 int timer_begin(pic_ctx* c, int stage, hipEvent_t* ev) {
     *ev = nullptr;
     if (!c->timing) return PIC_OK;
-    PIC_HIP(hipEventCreate(ev));
+    PIC_TRY(timer_event(c, ev));
     PIC_HIP(hipEventRecord(*ev, c->stream));
     return PIC_OK;
 }
@@ -30,7 +42,7 @@ int timer_begin(pic_ctx* c, int stage, hipEvent_t* ev) {
 int timer_end(pic_ctx* c, int stage, hipEvent_t ev0) {
     if (!c->timing || !ev0) return PIC_OK;
     hipEvent_t ev1;
-    PIC_HIP(hipEventCreate(&ev1));
+    PIC_TRY(timer_event(c, &ev1));
     PIC_HIP(hipEventRecord(ev1, c->stream));
     c->timers[stage].pending.emplace_back(ev0, ev1);
     c->timers[stage].launches++;
@@ -109,6 +121,9 @@ extern "C" void pic_destroy(pic_ctx* c) {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
         }
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->solve_ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -643,8 +658,7 @@ extern "C" int pic_poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st)
         set_error("pic_poisson_solve: null context");
         return PIC_ERR_ARG;
     }
-    PIC_TRY(poisson_solve(c, iteration, st));
-    PIC_TRY(launch_efield(c));
+    PIC_TRY(poisson_solve(c, iteration, st));   // also computes E
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }
@@ -688,8 +702,7 @@ extern "C" int pic_step(pic_ctx* c, int iteration, pic_solve_stats* st) {
     }
     PIC_TRY(launch_zero_f64(c, c->rho, c->G));
     for (int s = 0; s < (int)c->sp.size(); ++s) PIC_TRY(launch_deposit(c, s));
-    PIC_TRY(poisson_solve(c, iteration, st));
-    PIC_TRY(launch_efield(c));
+    PIC_TRY(poisson_solve(c, iteration, st));   // also computes E
     if (!c->inj.on) {
         for (int s = 0; s < (int)c->sp.size(); ++s) PIC_TRY(launch_move(c, s));
         return PIC_OK;
@@ -881,8 +894,8 @@ extern "C" int pic_stage_times(pic_ctx* c, double* ms, int64_t* launches) {
             float m = 0;
             PIC_HIP(hipEventElapsedTime(&m, e.first, e.second));
             t.ms += m;
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
+            c->event_pool.push_back(e.first);
+            c->event_pool.push_back(e.second);
         }
         t.pending.clear();
         if (ms) ms[k] = t.ms;

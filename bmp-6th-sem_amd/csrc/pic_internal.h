@@ -184,6 +184,8 @@ struct pic_ctx {
     // timing
     int timing = 0;
     pic::StageTimer timers[PIC_NSTAGES];
+    std::vector<hipEvent_t> event_pool;   // recorded-and-read stage events, reused
+    hipEvent_t solve_ev[2] = {nullptr, nullptr};   // start / end of a Poisson solve
 };
 
 namespace pic {
@@ -214,7 +216,7 @@ int launch_kinetic_energy(pic_ctx* c, int s, double* out);   // host result, syn
 // Poisson (pic_poisson.hip)
 int poisson_setup(pic_ctx* c);
 void poisson_free(pic_ctx* c);
-int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st);
+int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st);   // phi and E
 // direct solve (pic_fastpoisson.hip): set-up for the geometry masks, solve of
 // the current right-hand side into a full grid
 int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<double>& dval);

@@ -1227,6 +1227,14 @@ static int run_pcg(pic_ctx* c, T* x, T* const R[2], T* const P[2], T* q, float* 
     MgLevel& m0 = c->mg[0];
     const Lvl l0 = lvl_pre(c, m0);
     const int flag_off = (int)(flags - c->flags);
+    // the starting residual may already satisfy the tolerance (a direct solve,
+    // a converged warm start): then nothing more is enqueued
+    PIC_TRY(read_state(c));
+    const int count0 = c->flags_host[flag_off + 1];
+    if (c->flags_host[flag_off] || max_it <= 0) {
+        *batch_hint = 0;
+        return PIC_OK;
+    }
     if (c->p.mg_pre == 2) {
         hipLaunchKernelGGL(k_mg_pre2<T>, grid2d(m0), kMgBlock, 0, s, l0, R[0], m0.b, m0.e, m0.dir, flags);
     } else {
@@ -1236,12 +1244,15 @@ static int run_pcg(pic_ctx* c, T* x, T* const R[2], T* const P[2], T* q, float* 
     PIC_TRY(vcycle_rest<T>(c, R[0], Z[0], Z[1], 1, flags));
     hipLaunchKernelGGL(k_beta, dim3(1), dim3(1024), 0, s, part + P_RZ * kNP, part + P_RZO * kNP, c->scal, flags, 1);
     PIC_HIP(hipGetLastError());
-    int it = 0, count0 = -1;
+    int it = 0;
     int batch = *batch_hint > 0 ? *batch_hint : 2;
+    bool first = true;
     while (true) {
-        PIC_TRY(read_state(c));
-        if (count0 < 0) count0 = c->flags_host[flag_off + 1];
-        if (c->flags_host[flag_off] || it >= max_it) break;
+        if (!first) {
+            PIC_TRY(read_state(c));
+            if (c->flags_host[flag_off] || it >= max_it) break;
+        }
+        first = false;
         for (int k = 0; k < batch && it < max_it; ++k) {
             ++it;
             const int cur = it & 1, old = cur ^ 1;
@@ -1286,9 +1297,11 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     hipStream_t s = c->stream;
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_POISSON, &ev));
-    hipEvent_t t0, t1;
-    PIC_HIP(hipEventCreate(&t0));
-    PIC_HIP(hipEventCreate(&t1));
+    if (!c->solve_ev[0]) {
+        PIC_HIP(hipEventCreate(&c->solve_ev[0]));
+        PIC_HIP(hipEventCreate(&c->solve_ev[1]));
+    }
+    hipEvent_t t0 = c->solve_ev[0], t1 = c->solve_ev[1];
     PIC_HIP(hipEventRecord(t0, s));
     double* part = c->part;
     const double tol2 = c->p.poisson_tol * c->p.poisson_tol;
@@ -1345,19 +1358,37 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     }
     // direct solve: the PCG below starts from it and only checks (or
     // corrects) the residual
-    if (c->fast && !mixed && !learned && !c->guess_set) {
-        PIC_TRY(fast_solve(c, c->guess));
-        src = c->guess;
-        prev = nullptr;
+    const bool direct = c->fast && !mixed && !learned && !c->guess_set;
+    if (direct) {
+        if (keep) PIC_HIP(hipMemcpyAsync(keep, c->phi, (size_t)G * sizeof(double), hipMemcpyDeviceToDevice, s));
+        PIC_TRY(fast_solve(c, c->phi));   // every node, Dirichlet values included
         guess = 5;
+    } else {
+        hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, src, prev, keep, c->dir,
+                           c->dval);
     }
-    hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, src, prev, keep, c->dir, c->dval);
     hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, R[0], part + P_RR * kNP);
     hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2, c->flags,
                        c->scal, 1, S_BB, S_RR, S_RR0);
     PIC_HIP(hipGetLastError());
+    bool fields_done = false;
+    if (direct) {
+        // The check's flags go to the host while the device already writes phi
+        // and E: the host waits for the flags only, not for the field kernels.
+        PIC_HIP(hipMemcpyAsync(c->scal_host, c->scal, S_N * sizeof(double), hipMemcpyDeviceToHost, s));
+        PIC_HIP(hipMemcpyAsync(c->flags_host, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+        PIC_HIP(hipEventRecord(t1, s));
+        PIC_TRY(launch_overwrite_phi(c));
+        PIC_TRY(timer_end(c, PIC_STAGE_POISSON, ev));
+        PIC_TRY(launch_efield(c));
+        PIC_HIP(hipEventSynchronize(t1));
+        fields_done = c->flags_host[0] == 1;
+        if (!fields_done) PIC_TRY(timer_begin(c, PIC_STAGE_POISSON, &ev));
+    }
     int iters = 0, inner_total = 0;
-    if (!mixed) {
+    if (fields_done) {
+        // the direct solution met the tolerance: no PCG iteration
+    } else if (!mixed) {
         PIC_TRY(run_pcg<double>(c, c->phi, R, P, c->pq, Z, c->flags, tol2, S_BB, S_RR, S_RR0, c->p.poisson_max_iter,
                                 &c->last_iters));
         iters = c->flags_host[1];
@@ -1408,13 +1439,13 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
                   iters, rel);
         return PIC_ERR_SOLVER;
     }
-    PIC_TRY(launch_overwrite_phi(c));
-    PIC_HIP(hipEventRecord(t1, s));
-    PIC_HIP(hipEventSynchronize(t1));
+    if (!fields_done) {
+        PIC_TRY(launch_overwrite_phi(c));
+        PIC_HIP(hipEventRecord(t1, s));
+        PIC_HIP(hipEventSynchronize(t1));
+    }
     float ms = 0;
     PIC_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     c->guess_set = 0;
     c->have_prev_phi = (extrap && keep) ? 2 : 1;
     if (learned && c->learn_trained < c->learn_train) {
@@ -1423,7 +1454,10 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
         c->learn_trained += 1;
         c->learn_loss = loss;
     }
-    PIC_TRY(timer_end(c, PIC_STAGE_POISSON, ev));
+    if (!fields_done) {
+        PIC_TRY(timer_end(c, PIC_STAGE_POISSON, ev));
+        PIC_TRY(launch_efield(c));
+    }
     if (st) {
         st->iterations = iters;
         st->initial_relres = rel0;
