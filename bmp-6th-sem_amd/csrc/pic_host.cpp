@@ -33,20 +33,36 @@ static int timer_event(pic_ctx* c, hipEvent_t* ev) {
 
 int timer_begin(pic_ctx* c, int stage, hipEvent_t* ev) {
     *ev = nullptr;
-    if (!c->timing) return PIC_OK;
+    if (c->timing != 1) return PIC_OK;
     PIC_TRY(timer_event(c, ev));
     PIC_HIP(hipEventRecord(*ev, c->stream));
     return PIC_OK;
 }
 
 int timer_end(pic_ctx* c, int stage, hipEvent_t ev0) {
-    if (!c->timing || !ev0) return PIC_OK;
+    if (c->timing != 1 || !ev0) return PIC_OK;
     hipEvent_t ev1;
     PIC_TRY(timer_event(c, &ev1));
     PIC_HIP(hipEventRecord(ev1, c->stream));
     c->timers[stage].pending.emplace_back(ev0, ev1);
     c->timers[stage].launches++;
     return PIC_OK;
+}
+
+// Kernel events (timing mode 2): start / stop timestamps written by the
+// dispatch of one kernel, no marker in the stream.
+int kt_begin(pic_ctx* c, hipEvent_t ev[2]) {
+    ev[0] = ev[1] = nullptr;
+    if (c->timing != 2) return PIC_OK;
+    PIC_TRY(timer_event(c, &ev[0]));
+    PIC_TRY(timer_event(c, &ev[1]));
+    return PIC_OK;
+}
+
+void kt_end(pic_ctx* c, int stage, const hipEvent_t ev[2]) {
+    if (!ev[0]) return;
+    c->timers[stage].pending.emplace_back(ev[0], ev[1]);
+    c->timers[stage].launches++;
 }
 
 int launch_key_from_cell(pic_ctx* c, int s, int64_t n);   // pic_particles.hip
@@ -878,7 +894,11 @@ extern "C" int pic_set_timing(pic_ctx* c, int on) {
         set_error("pic_set_timing: null context");
         return PIC_ERR_ARG;
     }
-    c->timing = on ? 1 : 0;
+    if (on < 0 || on > 2) {
+        set_error("pic_set_timing: mode %d (0 off, 1 stages, 2 particle kernels)", on);
+        return PIC_ERR_ARG;
+    }
+    c->timing = on;
     return PIC_OK;
 }
 

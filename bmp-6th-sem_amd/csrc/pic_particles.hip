@@ -16,9 +16,26 @@
 //            t = (cell + offset) + v * dt / dx.
 #include "pic_particle_dev.h"
 
+#include <hip/hip_ext.h>
+
 #include <cmath>
+#include <tuple>
 
 namespace pic {
+
+// Launch with optional start / stop events taken by the dispatch itself
+// (timing mode 2); the arguments are converted to the kernel's parameter types.
+template <typename... KArgs, typename... Args>
+static void launch_timed(void (*kernel)(KArgs...), dim3 grid, dim3 block, hipStream_t stream, const hipEvent_t ev[2],
+                         Args... args) {
+    std::tuple<KArgs...> t{args...};
+    void* ptrs[sizeof...(KArgs)];
+    std::apply([&](auto&... x) {
+        int k = 0;
+        ((ptrs[k++] = (void*)&x), ...);
+    }, t);
+    (void)hipExtLaunchKernel((void*)kernel, grid, block, ptrs, 0, stream, ev[0], ev[1], 0);
+}
 
 // ================================================================ encode
 // Upload path: absolute FP64 particles -> storage layout + reference key
@@ -435,14 +452,17 @@ int launch_deposit(pic_ctx* c, int s) {
         if (sp.sorted) {
             const int tpr = (c->g.gx + kDepC - 1) / kDepC;
             const unsigned ntiles = (unsigned)(tpr * c->g.gy);
+            hipEvent_t kt[2];
+            PIC_TRY(kt_begin(c, kt));
             if (c->p.layout == PIC_LAYOUT_F64)
-                hipLaunchKernelGGL(k_deposit_tile<PIC_LAYOUT_F64>, dim3(ntiles), dim3(kBlock), 0,
-                                   c->stream, sp.buf[sp.cur], c->g, a, tpr, (int)ntiles, sp.start,
-                                   (long long*)sp.q_charge, (long long*)sp.q_energy);
+                launch_timed(k_deposit_tile<PIC_LAYOUT_F64>, dim3(ntiles), dim3(kBlock), c->stream, kt,
+                             sp.buf[sp.cur], c->g, a, tpr, (int)ntiles, (const int32_t*)sp.start,
+                             (long long*)sp.q_charge, (long long*)sp.q_energy);
             else
-                hipLaunchKernelGGL(k_deposit_tile<PIC_LAYOUT_CELL32>, dim3(ntiles), dim3(kBlock), 0,
-                                   c->stream, sp.buf[sp.cur], c->g, a, tpr, (int)ntiles, sp.start,
-                                   (long long*)sp.q_charge, (long long*)sp.q_energy);
+                launch_timed(k_deposit_tile<PIC_LAYOUT_CELL32>, dim3(ntiles), dim3(kBlock), c->stream, kt,
+                             sp.buf[sp.cur], c->g, a, tpr, (int)ntiles, (const int32_t*)sp.start,
+                             (long long*)sp.q_charge, (long long*)sp.q_energy);
+            kt_end(c, PIC_STAGE_DEPOSIT, kt);
         } else {
             PIC_TRY(inject_settle(c));   // the per-particle grid needs the live-count bound
             const int64_t nacc = 4 * (int64_t)c->g.gp;
@@ -503,15 +523,17 @@ int launch_push(pic_ctx* c, int s) {
 }
 
 template <int L, int GENB>
-static void tile_kernel(pic_ctx* c, Species& sp, const SpeciesDev& d, int tpr, unsigned ntiles) {
-    hipLaunchKernelGGL((k_push_tile<L, GENB>), dim3(ntiles), dim3(kBlock), 0, c->stream,
-                       sp.buf[sp.cur], c->g, d, tpr, (int)ntiles, c->fld, sp.start, sp.counts, sp.rank, sp.absorbed);
+static void tile_kernel(pic_ctx* c, Species& sp, const SpeciesDev& d, int tpr, unsigned ntiles,
+                        const hipEvent_t kt[2]) {
+    launch_timed(k_push_tile<L, GENB>, dim3(ntiles), dim3(kBlock), c->stream, kt, sp.buf[sp.cur], c->g, d, tpr,
+                 (int)ntiles, (const double4*)c->fld, (const int32_t*)sp.start, sp.counts, sp.rank, sp.absorbed);
 }
 
 template <int L, int GENB>
-static void scatter_kernel(pic_ctx* c, Species& sp, const SpeciesDev& d, int src, int dst) {
-    hipLaunchKernelGGL((k_scatter_push<L, GENB>), dim3(blocks_for(sp.n_upper)), dim3(kBlock), 0, c->stream,
-                       sp.ndev + src, sp.buf[src], sp.buf[dst], c->g, d, c->fld, sp.rank, sp.start);
+static void scatter_kernel(pic_ctx* c, Species& sp, const SpeciesDev& d, int src, int dst, const hipEvent_t kt[2]) {
+    launch_timed(k_scatter_push<L, GENB>, dim3(blocks_for(sp.n_upper)), dim3(kBlock), c->stream, kt,
+                 (const int32_t*)(sp.ndev + src), sp.buf[src], sp.buf[dst], c->g, d, (const double4*)c->fld,
+                 (const int32_t*)sp.rank, (const int32_t*)sp.start);
 }
 
 // Gather + Boris + walls + re-bin of species s: tile push/count (move_push),
@@ -529,11 +551,14 @@ int move_push(pic_ctx* c, int s) {
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_PUSH, &ev));
     if (sp.n_upper > 0) {
-        if (f64 && !d.general_b) tile_kernel<PIC_LAYOUT_F64, 0>(c, sp, d, tpr, ntiles);
-        else if (f64) tile_kernel<PIC_LAYOUT_F64, 1>(c, sp, d, tpr, ntiles);
-        else if (!d.general_b) tile_kernel<PIC_LAYOUT_CELL32, 0>(c, sp, d, tpr, ntiles);
-        else tile_kernel<PIC_LAYOUT_CELL32, 1>(c, sp, d, tpr, ntiles);
+        hipEvent_t kt[2];
+        PIC_TRY(kt_begin(c, kt));
+        if (f64 && !d.general_b) tile_kernel<PIC_LAYOUT_F64, 0>(c, sp, d, tpr, ntiles, kt);
+        else if (f64) tile_kernel<PIC_LAYOUT_F64, 1>(c, sp, d, tpr, ntiles, kt);
+        else if (!d.general_b) tile_kernel<PIC_LAYOUT_CELL32, 0>(c, sp, d, tpr, ntiles, kt);
+        else tile_kernel<PIC_LAYOUT_CELL32, 1>(c, sp, d, tpr, ntiles, kt);
         PIC_HIP(hipGetLastError());
+        kt_end(c, PIC_STAGE_PUSH, kt);
     }
     PIC_TRY(timer_end(c, PIC_STAGE_PUSH, ev));
     return PIC_OK;
@@ -550,11 +575,14 @@ int move_finish(pic_ctx* c, int s) {
     PIC_TRY(timer_end(c, PIC_STAGE_BIN, ev));
     PIC_TRY(timer_begin(c, PIC_STAGE_SCATTER, &ev));
     if (sp.n_upper > 0) {
-        if (f64 && !d.general_b) scatter_kernel<PIC_LAYOUT_F64, 0>(c, sp, d, src, dst);
-        else if (f64) scatter_kernel<PIC_LAYOUT_F64, 1>(c, sp, d, src, dst);
-        else if (!d.general_b) scatter_kernel<PIC_LAYOUT_CELL32, 0>(c, sp, d, src, dst);
-        else scatter_kernel<PIC_LAYOUT_CELL32, 1>(c, sp, d, src, dst);
+        hipEvent_t kt[2];
+        PIC_TRY(kt_begin(c, kt));
+        if (f64 && !d.general_b) scatter_kernel<PIC_LAYOUT_F64, 0>(c, sp, d, src, dst, kt);
+        else if (f64) scatter_kernel<PIC_LAYOUT_F64, 1>(c, sp, d, src, dst, kt);
+        else if (!d.general_b) scatter_kernel<PIC_LAYOUT_CELL32, 0>(c, sp, d, src, dst, kt);
+        else scatter_kernel<PIC_LAYOUT_CELL32, 1>(c, sp, d, src, dst, kt);
         PIC_HIP(hipGetLastError());
+        kt_end(c, PIC_STAGE_SCATTER, kt);
     }
     sp.cur = dst;
     sp.deposit_pending = 0;

@@ -365,7 +365,8 @@ class Context:
 
     # -- timing
     def set_timing(self, on=True):
-        _chk(self.lib.pic_set_timing(self.h, 1 if on else 0))
+        """0/False off, 1/True per stage, 2 per particle kernel (no markers in the stream)."""
+        _chk(self.lib.pic_set_timing(self.h, int(on)))
 
     def learned_enable(self, train_steps, min_trained=0, lr=1e-3, seed=1):
         _chk(self.lib.pic_learned_enable(self.h, int(train_steps), int(min_trained), float(lr), int(seed)))

@@ -288,9 +288,13 @@ int  pic_injection_stats(pic_ctx* ctx, int64_t* injections, int64_t* reinjected,
 
 /* Block until all work of the context is done. */
 int  pic_sync(pic_ctx* ctx);
-/* Enable per-stage HIP-event timing (0/1), and read the accumulated device
+/* Timing: 0 off; 1 per stage (HIP event markers around each stage's
+ * kernels); 2 per kernel for the three particle kernels only (the tile push,
+ * the scatter and the tile deposit, in the PUSH, SCATTER and DEPOSIT slots),
+ * timestamps taken by the kernel dispatches themselves, so a measured loop
+ * carries no markers.  pic_stage_times reads the accumulated device
  * milliseconds and launch counts per stage (PIC_NSTAGES entries each; either
- * pointer may be NULL).  Reading resets the accumulators. */
+ * pointer may be NULL) and resets them. */
 int  pic_set_timing(pic_ctx* ctx, int on);
 int  pic_stage_times(pic_ctx* ctx, double* ms, int64_t* launches);
 

@@ -219,7 +219,8 @@ def test_poisson_electrode_vs_direct(hip_mod, oracle_mod):
     c.close()
 
 
-@pytest.mark.parametrize("case", ["walls_only", "biased_walls_rect", "two_electrodes", "too_many_for_direct"])
+@pytest.mark.parametrize("case", ["walls_only", "biased_walls_rect", "two_electrodes", "many_small_electrodes",
+                                  "too_many_for_direct"])
 def test_direct_poisson_solve(hip_mod, oracle_mod, case):
     """The direct solve (sine transforms + capacitance correction) against the
     oracle's direct solve of the reference's system, to 1e-12 relative: no
@@ -235,6 +236,11 @@ def test_direct_poisson_solve(hip_mod, oracle_mod, case):
                 [0.25, 0.5, 0.375, 0.625, 40.0]]
     elif case == "two_electrodes":
         geom = WALLS + [[0.2, 0.2, 0.3, 0.6, 100.0], [0.6, 0.5, 0.8, 0.55, -50.0]]
+    elif case == "many_small_electrodes":
+        # 24 electrodes of 3 x 3 nodes on distinct rows and columns: the line
+        # cover needs more lines than the one-pass kernels take (GEMM path)
+        geom = WALLS + [[(8 + 5 * k) / 128, (6 + 5 * k) / 128, (10 + 5 * k) / 128, (8 + 5 * k) / 128,
+                         float(10 * k - 100)] for k in range(24)]
     elif case == "too_many_for_direct":
         # three electrodes with 1588 nodes next to unknowns (the direct path takes 1024)
         geom = WALLS + [[0.1, 0.1, 0.35, 0.9, 10.0], [0.4, 0.1, 0.6, 0.9, -10.0], [0.65, 0.1, 0.9, 0.9, 5.0]]
