@@ -10,7 +10,9 @@
 //     the orthonormal DST-I basis S[j][k] = sqrt(2/(N+1)) sin(pi (j+1)(k+1)/(N+1)):
 //     T^-1 f = S_n ((S_n F S_m) ./ Lambda) S_m, Lambda[k][l] =
 //     cy (2 - 2 cos(pi (k+1)/(n+1))) + cx (2 - 2 cos(pi (l+1)/(m+1)));
-//     each product with S is one FP64 GEMM (MFMA, rocBLAS);
+//     the products with S run as FFTs (one LDS-resident complex FFT of length
+//     2(N+1) per pair of rows) when N + 1 is a power of two, else as FP64
+//     GEMMs (MFMA, rocBLAS);
 //   * a capacitance correction for the internal Dirichlet nodes: with Q the
 //     internal Dirichlet nodes next to an unknown (P of them), sources c on Q
 //     are chosen so that u = T^-1 (f + B c) equals V on Q:
@@ -18,8 +20,8 @@
 //     definite, inverted once at set-up).  The unknown rows only see Q among
 //     the internal Dirichlet nodes, so u solves the reference's system there.
 //
-// Per solve: F from rho (4 GEMMs of n x n x m / m x m x n with the transforms,
-// one n x P x m GEMM for the correction, a few small kernels).  The PCG of
+// Per solve: four transform passes (FFT passes, or GEMMs of n x n x m /
+// m x m x n), one n x P x m GEMM for the correction, a few small kernels.  The PCG of
 // pic_poisson.hip then starts from this solution and only checks (or, if
 // rounding left the residual above the tolerance, corrects) it.
 #include "pic_internal.h"
@@ -57,6 +59,10 @@ struct FastPoisson {
     double* dq = nullptr;                    // P: V_Q - u0_Q
     int *qri = nullptr, *qcol = nullptr;     // P: covering line of q (row a >= 0, column j as -1 - j); column of q
     int* qrow = nullptr;                     // P: row of q
+    // sine transforms by FFT when m + 1 and n + 1 are powers of two (else GEMMs)
+    int fft = 0, lm = 0, ln = 0;             // log2(2 (m + 1)), log2(2 (n + 1))
+    double2 *twm = nullptr, *twn = nullptr;  // exp(-2 pi i t / M), t < M, for both lengths
+    double* Fb = nullptr;                    // n x m: wall values moved to the right-hand side
 };
 
 static void* dmalloc(size_t bytes, bool& ok) {
@@ -72,7 +78,7 @@ void fast_free(pic_ctx* c) {
     FastPoisson* f = c->fast;
     if (!f) return;
     void* ptrs[] = {f->Sm, f->Sn, f->Dinv, f->F, f->W, f->U, f->G, f->SnR, f->V1, f->SmC, f->V2, f->SnQ, f->SmQ,
-                    f->Z, f->Cinv, f->VQ, f->dq, f->qri, f->qcol, f->qrow, f->part};
+                    f->Z, f->Cinv, f->VQ, f->dq, f->qri, f->qcol, f->qrow, f->part, f->twm, f->twn, f->Fb};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (f->blas) (void)rocblas_destroy_handle(f->blas);
@@ -255,6 +261,272 @@ __global__ __launch_bounds__(kFdB) void k_fd_amat(int P, int n, int m, int k0, i
     A[idx] = v;
 }
 
+// ------------------------------------------------------- sine transform by FFT
+// DST-I of length N = M/2 - 1 (M = 2^L) through a complex FFT of length M:
+// the odd extension z = (0, x_0 .. x_{N-1}, 0, -x_{N-1} .. -x_0) has
+// Z[k+1] = -2i sum_j x_j sin(pi (j+1)(k+1)/(N+1)).  Two real rows a, b go into
+// one FFT as z_a + i z_b; both transforms are odd and real, so
+// (S a)_k = -Im Z[k+1] s/2 and (S b)_k = Re Z[k+1] s/2, s = sqrt(2/(N+1)).
+// The FFT is a Stockham radix-4 sequence (a final radix-2 stage for odd L) in
+// LDS: every thread reads its butterflies' inputs, the workgroup synchronises,
+// every thread writes its outputs.  Each pass reads rows and writes the result
+// transposed, so two passes are the 2-D transform back in the input layout;
+// the loaders and storers fold the right-hand side, the spectral scaling, the
+// correction and the full-grid output into the passes.  Every global load of
+// a pass is issued before the first wait (rows, twiddles, then the gathers of
+// the storer), so a workgroup waits for memory twice.
+static constexpr int kDstT = 256;          // threads per workgroup
+static constexpr int kDstPts = 2048;       // complex points per workgroup (at least)
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// plain rows: X[row][col], row length ld
+struct DstLoadRows {
+    const double* X;
+    int ld;
+    __device__ double operator()(int row, int col) const { return X[(size_t)row * ld + col]; }
+};
+
+// rows of U + G .* Dinv (the corrected spectrum)
+struct DstLoadCorrected {
+    const double *U, *G, *Dinv;
+    int ld;
+    __device__ double operator()(int row, int col) const {
+        const size_t i = (size_t)row * ld + col;
+        return U[i] + G[i] * Dinv[i];
+    }
+};
+
+// the right-hand side F at interior node (r + 1, c + 1): rhs on unknowns plus
+// the wall terms Fb (fixed by the geometry), 0 on internal Dirichlet nodes
+struct DstLoadRhs {
+    int gx, m;
+    const double* rhs;
+    const uint8_t* dir;
+    const double* Fb;
+    __device__ double operator()(int r, int q) const {
+        const int node = (r + 1) * gx + q + 1;
+        const double v = rhs[node] + Fb[(size_t)r * m + q];
+        return dir[node] ? 0.0 : v;
+    }
+};
+
+// transposed: Y[t][row], row length R
+struct DstStoreT {
+    double* Y;
+    int R;
+    struct Aux {};
+    static constexpr bool kGrid = false;
+    __device__ Aux fetch(int, int) const { return {}; }
+    __device__ void put(int t, int row, double v, Aux) const { Y[(size_t)t * R + row] = v; }
+};
+
+// transposed and scaled by Dinv (the same layout as Y)
+struct DstStoreTScaled {
+    double* Y;
+    const double* Dinv;
+    int R;
+    using Aux = double;
+    static constexpr bool kGrid = false;
+    __device__ Aux fetch(int t, int row) const { return Dinv[(size_t)t * R + row]; }
+    __device__ void put(int t, int row, double v, Aux d) const { Y[(size_t)t * R + row] = v * d; }
+};
+
+// the full grid: interior node (t + 1, row + 1); Dirichlet nodes keep their value
+struct DstStoreGrid {
+    double* out;
+    const uint8_t* dir;
+    const double* dval;
+    int gx, gy;
+    struct Aux {
+        double val;
+        int fixed;
+    };
+    static constexpr bool kGrid = true;
+    __device__ Aux fetch(int t, int row) const {
+        const int node = (t + 1) * gx + row + 1;
+        return {dval[node], (int)dir[node]};
+    }
+    __device__ void put(int t, int row, double v, Aux a) const { out[(t + 1) * gx + row + 1] = a.fixed ? a.val : v; }
+};
+
+// One pass: R rows of N = 2^(L-1) - 1 values in, N rows of R values out.
+// Workgroup w (after the XCD-aware remap: the workgroups of one XCD take
+// consecutive row pairs, so the transposed stores of one cache line meet in
+// one L2) transforms NB = max(1, 2048 / M) row pairs.
+template <int L, class Load, class Store>
+__global__ __launch_bounds__(kDstT) void k_fd_dst(int R, int nwg, Load ld, Store st, const double2* __restrict__ tw) {
+    constexpr int M = 1 << L;
+    constexpr int N = M / 2 - 1;
+    constexpr int NB = M >= kDstPts ? 1 : kDstPts / M;
+    constexpr int PTS = NB * M;
+    constexpr int NQ = PTS / 4 / kDstT;                  // radix-4 butterflies per thread
+    constexpr int NE = NB * N;                           // values per row of the workgroup's pairs
+    constexpr int IT = (NE + kDstT - 1) / kDstT;
+    constexpr bool kTwLds = L <= 11;                     // twiddles staged in LDS (64 KB in all)
+    __shared__ double2 lds[PTS + (kTwLds ? M : 0)];
+    double2* z = lds;
+    double2* tws = lds + PTS;
+    const double2* twp = kTwLds ? tws : tw;
+    const int per = (int)(gridDim.x >> 3);
+    const int w = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (w >= nwg) return;
+    const int tid = threadIdx.x;
+    __builtin_assume(tid < kDstT);
+    const int r0 = w * 2 * NB;
+    // loads: indices clamped into the matrix so that none waits behind a branch
+    double va[IT], vb[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int e = min(tid + kDstT * it, NE - 1);
+        const int b = e / N, j = e - b * N;
+        const int ra = r0 + 2 * b;
+        va[it] = ld(min(ra, R - 1), j);
+        vb[it] = ld(min(ra + 1, R - 1), j);
+    }
+    if constexpr (kTwLds) {
+        constexpr int TI = (M + kDstT - 1) / kDstT;
+        double2 t[TI];
+#pragma unroll
+        for (int it = 0; it < TI; ++it) t[it] = tw[min(tid + kDstT * it, M - 1)];
+#pragma unroll
+        for (int it = 0; it < TI; ++it)
+            if (M % kDstT == 0 || tid + kDstT * it < M) tws[tid + kDstT * it] = t[it];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int e = tid + kDstT * it;
+        if (e < NE) {
+            const int b = e / N, j = e - b * N;
+            const int ra = r0 + 2 * b;
+            const double a = ra < R ? va[it] : 0.0;
+            const double c = ra + 1 < R ? vb[it] : 0.0;
+            z[b * M + j + 1] = make_double2(a, c);
+            z[b * M + M - 1 - j] = make_double2(-a, -c);
+        }
+    }
+    if (tid < NB) {
+        z[tid * M] = make_double2(0.0, 0.0);
+        z[tid * M + M / 2] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < L / 2; ++s) {
+        const int p = 1 << (2 * s);
+        double2 u[NQ][4];
+        int dst[NQ];
+#pragma unroll
+        for (int t = 0; t < NQ; ++t) {
+            const int ig = tid + kDstT * t;
+            const int base = (ig >> (L - 2)) << L;
+            const int i = ig & (M / 4 - 1);
+            const int k = i & (p - 1);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) u[t][r] = z[base + i + r * (M / 4)];
+            if (s > 0) {
+#pragma unroll
+                for (int r = 1; r < 4; ++r) u[t][r] = cmul(u[t][r], twp[(r * k) << (L - 2 - 2 * s)]);
+            }
+            const double2 v0 = make_double2(u[t][0].x + u[t][2].x, u[t][0].y + u[t][2].y);
+            const double2 v1 = make_double2(u[t][0].x - u[t][2].x, u[t][0].y - u[t][2].y);
+            const double2 v2 = make_double2(u[t][1].x + u[t][3].x, u[t][1].y + u[t][3].y);
+            const double2 v3 = make_double2(u[t][1].y - u[t][3].y, u[t][3].x - u[t][1].x);   // (u1 - u3)(-i)
+            u[t][0] = make_double2(v0.x + v2.x, v0.y + v2.y);
+            u[t][1] = make_double2(v1.x + v3.x, v1.y + v3.y);
+            u[t][2] = make_double2(v0.x - v2.x, v0.y - v2.y);
+            u[t][3] = make_double2(v1.x - v3.x, v1.y - v3.y);
+            dst[t] = base + ((i - k) << 2) + k;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < NQ; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) z[dst[t] + r * p] = u[t][r];
+        __syncthreads();
+    }
+    if (L & 1) {
+        // radix 2 with p = M/2: butterfly i pairs i and i + M/2 in place
+#pragma unroll
+        for (int t = 0; t < 2 * NQ; ++t) {
+            const int ig = tid + kDstT * t;
+            const int base = (ig >> (L - 1)) << L;
+            const int i = ig & (M / 2 - 1);
+            const double2 a = z[base + i];
+            const double2 b = cmul(z[base + i + M / 2], twp[i]);
+            z[base + i] = make_double2(a.x + b.x, a.y + b.y);
+            z[base + i + M / 2] = make_double2(a.x - b.x, a.y - b.y);
+        }
+        __syncthreads();
+    }
+    // the storer's gathers first, then the stores
+    typename Store::Aux xa[IT], xb[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int e = min(tid + kDstT * it, NE - 1);
+        const int b = e / N, t = e - b * N;
+        const int ra = r0 + 2 * b;
+        xa[it] = st.fetch(t, min(ra, R - 1));
+        xb[it] = st.fetch(t, min(ra + 1, R - 1));
+    }
+    const double sc = 0.5 * sqrt(2.0 / (double)(N + 1));
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int e = tid + kDstT * it;
+        if (e < NE) {
+            const int b = e / N, t = e - b * N;
+            const int ra = r0 + 2 * b;
+            const double2 v = z[b * M + t + 1];
+            if (ra < R) st.put(t, ra, -v.y * sc, xa[it]);
+            if (ra + 1 < R) st.put(t, ra + 1, v.x * sc, xb[it]);
+        }
+    }
+    if constexpr (Store::kGrid) {
+        // the wall nodes: rows 0 and GY-1 above and below these columns; the
+        // first workgroup also writes columns 0 and GX-1
+        for (int e = tid; e < 2 * NB; e += kDstT) {
+            const int row = r0 + e;
+            if (row < R) {
+                st.out[row + 1] = st.dval[row + 1];
+                st.out[(size_t)(st.gy - 1) * st.gx + row + 1] = st.dval[(size_t)(st.gy - 1) * st.gx + row + 1];
+            }
+        }
+        if (w == 0)
+            for (int y = tid; y < st.gy; y += kDstT) {
+                st.out[(size_t)y * st.gx] = st.dval[(size_t)y * st.gx];
+                st.out[(size_t)y * st.gx + st.gx - 1] = st.dval[(size_t)y * st.gx + st.gx - 1];
+            }
+    }
+}
+
+template <int L, class Load, class Store>
+static void dst_pass_l(int R, const Load& ld, const Store& st, const double2* tw, hipStream_t s) {
+    constexpr int M = 1 << L;
+    constexpr int NB = M >= kDstPts ? 1 : kDstPts / M;
+    const int nwg = (R + 2 * NB - 1) / (2 * NB);
+    const int grid = (nwg + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_fd_dst<L, Load, Store>), dim3((unsigned)grid), dim3(kDstT), 0, s, R, nwg, ld, st, tw);
+}
+
+// DST-I of each of R rows of length 2^(L-1) - 1, transposed into `st`
+template <class Load, class Store>
+static void dst_pass(int L, int R, const Load& ld, const Store& st, const double2* tw, hipStream_t s) {
+    switch (L) {
+    case 3: dst_pass_l<3>(R, ld, st, tw, s); break;
+    case 4: dst_pass_l<4>(R, ld, st, tw, s); break;
+    case 5: dst_pass_l<5>(R, ld, st, tw, s); break;
+    case 6: dst_pass_l<6>(R, ld, st, tw, s); break;
+    case 7: dst_pass_l<7>(R, ld, st, tw, s); break;
+    case 8: dst_pass_l<8>(R, ld, st, tw, s); break;
+    case 9: dst_pass_l<9>(R, ld, st, tw, s); break;
+    case 10: dst_pass_l<10>(R, ld, st, tw, s); break;
+    case 11: dst_pass_l<11>(R, ld, st, tw, s); break;
+    case 12: dst_pass_l<12>(R, ld, st, tw, s); break;
+    default: break;   // fast_setup admits 3 <= L <= 12 only
+    }
+}
+
 // ------------------------------------------------------------------- host
 static std::vector<double> dst_matrix(int N) {
     std::vector<double> S((size_t)N * N);
@@ -428,6 +700,34 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
     f->SmC = (double*)dmalloc(SmC.size() * 8, ok);
     f->V2 = (double*)dmalloc((size_t)std::max(nc, 1) * n * 8, ok);
     f->qrow = (int*)dmalloc((size_t)std::max(P, 1) * 4, ok);
+    // FFT transforms for N + 1 = 2^(L-1), 3 <= L <= 12 (N = m, n)
+    auto fft_log = [](int N) {
+        int L = 0;
+        while ((2 << L) < 2 * (N + 1)) ++L;
+        L += 1;
+        return ((1 << L) == 2 * (N + 1) && L >= 3 && L <= 12) ? L : 0;
+    };
+    f->lm = fft_log(m);
+    f->ln = fft_log(n);
+    f->fft = c->p.poisson_direct == 1 && f->lm > 0 && f->ln > 0;
+    std::vector<double2> twm, twn;
+    if (f->fft) {
+        auto table = [](int L) {
+            const int M = 1 << L;
+            const long double pi = 3.141592653589793238462643383279502884L;
+            std::vector<double2> t(M);
+            for (int i = 0; i < M; ++i) {
+                const long double a = 2.0L * pi * (long double)i / (long double)M;
+                t[i] = make_double2((double)cosl(a), (double)-sinl(a));
+            }
+            return t;
+        };
+        twm = table(f->lm);
+        twn = table(f->ln);
+        f->twm = (double2*)dmalloc(twm.size() * sizeof(double2), ok);
+        f->twn = (double2*)dmalloc(twn.size() * sizeof(double2), ok);
+        f->Fb = (double*)dmalloc(nm * 8, ok);
+    }
     if (nr > 0 && nr <= kLines) f->part = (double*)dmalloc((size_t)kSeg * nr * m * 8, ok);
     f->SnQ = (double*)dmalloc(SnQ.size() * 8, ok);
     f->SmQ = (double*)dmalloc(SmQ.size() * 8, ok);
@@ -451,6 +751,23 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
     PIC_HIP(hipMemcpy(f->SmC, SmC.data(), SmC.size() * 8, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(f->SnQ, SnQ.data(), SnQ.size() * 8, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(f->SmQ, SmQ.data(), SmQ.size() * 8, hipMemcpyHostToDevice));
+    if (f->fft) {
+        PIC_HIP(hipMemcpy(f->twm, twm.data(), twm.size() * sizeof(double2), hipMemcpyHostToDevice));
+        PIC_HIP(hipMemcpy(f->twn, twn.data(), twn.size() * sizeof(double2), hipMemcpyHostToDevice));
+        // wall terms of F: the Dirichlet values of the wall nodes next to the interior
+        std::vector<double> Fb(nm, 0.0);
+        for (int r = 0; r < n; ++r)
+            for (int q = 0; q < m; ++q) {
+                const int node = (r + 1) * gx + q + 1;
+                double v = 0.0;
+                if (q == 0) v += cx * dval[node - 1];
+                if (q == m - 1) v += cx * dval[node + 1];
+                if (r == 0) v += cy * dval[node - gx];
+                if (r == n - 1) v += cy * dval[node + gx];
+                Fb[(size_t)r * m + q] = v;
+            }
+        PIC_HIP(hipMemcpy(f->Fb, Fb.data(), nm * 8, hipMemcpyHostToDevice));
+    }
     if (P > 0) {
         PIC_HIP(hipMemcpy(f->VQ, vq.data(), (size_t)P * 8, hipMemcpyHostToDevice));
         PIC_HIP(hipMemcpy(f->qri, qri.data(), (size_t)P * 4, hipMemcpyHostToDevice));
@@ -508,12 +825,18 @@ int fast_solve(pic_ctx* c, double* out) {
     const double cx = 1.0 / (c->g.dx * c->g.dx), cy = 1.0 / (c->g.dy * c->g.dy);
     hipStream_t s = c->stream;
     const unsigned nb = (unsigned)((nm + kFdB - 1) / kFdB);
-    hipLaunchKernelGGL(k_fd_rhs, dim3(nb), dim3(kFdB), 0, s, gx, gy, cx, cy, c->rho, c->dir, c->dval, f->F);
-    // row-major X (n x m) is the column-major m x n matrix X^T:
-    // W^T = S_m F^T, then Fhat^T = W^T S_n
-    PIC_TRY(gemm(f->blas, m, n, m, f->Sm, m, f->F, m, 0.0, f->W, m));
-    PIC_TRY(gemm(f->blas, m, n, n, f->W, m, f->Sn, n, 0.0, f->U, m));
-    hipLaunchKernelGGL(k_fd_scale, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->Dinv);
+    if (f->fft) {
+        // rows of F (n x m) -> W (m x n) -> Uhat = (S_n F S_m) ./ Lambda (n x m)
+        dst_pass(f->lm, n, DstLoadRhs{gx, m, c->rho, c->dir, f->Fb}, DstStoreT{f->W, n}, f->twm, s);
+        dst_pass(f->ln, m, DstLoadRows{f->W, n}, DstStoreTScaled{f->U, f->Dinv, m}, f->twn, s);
+    } else {
+        hipLaunchKernelGGL(k_fd_rhs, dim3(nb), dim3(kFdB), 0, s, gx, gy, cx, cy, c->rho, c->dir, c->dval, f->F);
+        // row-major X (n x m) is the column-major m x n matrix X^T:
+        // W^T = S_m F^T, then Fhat^T = W^T S_n
+        PIC_TRY(gemm(f->blas, m, n, m, f->Sm, m, f->F, m, 0.0, f->W, m));
+        PIC_TRY(gemm(f->blas, m, n, n, f->W, m, f->Sn, n, 0.0, f->U, m));
+        hipLaunchKernelGGL(k_fd_scale, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->Dinv);
+    }
     if (P > 0) {
         // u0 on Q through the covering lines: V1 = S_n[rows] Uhat (nr x m),
         // V1^T = Uhat^T S_n[rows]^T; V2 = Uhat S_m[cols]^T (n x nc, column-major)
@@ -538,7 +861,17 @@ int fast_solve(pic_ctx* c, double* out) {
         hipLaunchKernelGGL(k_fd_sources, dim3((unsigned)P), dim3(kFdB), 0, s, P, m, f->Cinv, f->dq, f->SmQ, f->Z);
         // G = S_n[:, rows(Q)] Z (n x P x m) -> G^T = Z^T SnQ^T
         PIC_TRY(gemm(f->blas, m, n, P, f->Z, m, f->SnQ, P, 0.0, f->G, m));
-        hipLaunchKernelGGL(k_fd_addscaled, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->G, f->Dinv);
+        if (!f->fft) hipLaunchKernelGGL(k_fd_addscaled, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->G, f->Dinv);
+    }
+    if (f->fft) {
+        // rows of Uhat (+ G ./ Lambda) -> W (m x n) -> the grid (interior + Dirichlet values)
+        if (P > 0)
+            dst_pass(f->lm, n, DstLoadCorrected{f->U, f->G, f->Dinv, m}, DstStoreT{f->W, n}, f->twm, s);
+        else
+            dst_pass(f->lm, n, DstLoadRows{f->U, m}, DstStoreT{f->W, n}, f->twm, s);
+        dst_pass(f->ln, m, DstLoadRows{f->W, n}, DstStoreGrid{out, c->dir, c->dval, gx, gy}, f->twn, s);
+        PIC_HIP(hipGetLastError());
+        return PIC_OK;
     }
     // U = S_n Uhat S_m -> W^T = S_m Uhat^T, U^T = W^T S_n
     PIC_TRY(gemm(f->blas, m, n, m, f->Sm, m, f->U, m, 0.0, f->W, m));

@@ -201,6 +201,10 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
         set_error("pic_create: periodic must be 0 or 1");
         return PIC_ERR_ARG;
     }
+    if (p->poisson_direct < 0 || p->poisson_direct > 2) {
+        set_error("pic_create: poisson_direct must be 0, 1 or 2");
+        return PIC_ERR_ARG;
+    }
     if (p->layout != PIC_LAYOUT_F64 && p->layout != PIC_LAYOUT_CELL32) {
         set_error("pic_create: unknown layout %d", p->layout);
         return PIC_ERR_ARG;

@@ -373,6 +373,34 @@ __device__ __forceinline__ Corners gather(const PState<L>& p, const GridDev& g, 
     return c;
 }
 
+// Field window of a CELL32 tile (cells c0 .. c0+cw-1 of key row r, first key
+// k0) in LDS: nodes (r, c0 .. c0+cw) then (r+1, c0 .. c0+cw), row stride TC+1;
+// the four corners of every cell of the tile (fld is padded past the last
+// node row, so row r+1 exists for the last key row too).
+template <int TC>
+__device__ __forceinline__ void load_field_window(const double4* __restrict__ fld, const GridDev& g, int k0, int cw,
+                                                  double4* wf) {
+    for (int j = threadIdx.x; j < 2 * (TC + 1); j += kBlock) {
+        const int row = j >= TC + 1 ? 1 : 0;
+        const int col = j - row * (TC + 1);
+        if (col <= cw) wf[j] = fld[k0 + row * g.gx + col];
+    }
+}
+
+// The corners of a particle in cell `cell` of the tile, from the window.
+template <int TC>
+__device__ __forceinline__ Corners gather_window(int cell, int k0, int c0, int r, const double4* wf) {
+    Corners c;
+    const int lc = cell - k0;
+    c.cgx = c0 + lc;
+    c.cgy = r;
+    c.f00 = wf[lc];
+    c.f10 = wf[lc + 1];
+    c.f01 = wf[TC + 1 + lc];
+    c.f11 = wf[TC + 2 + lc];
+    return c;
+}
+
 template <int L, int GENB>
 __device__ __forceinline__ Pushed<L> push_state(const PState<L>& p, const Corners& c, const GridDev& g,
                                                const SpeciesDev& s) {
@@ -478,20 +506,23 @@ static constexpr int kFallback = (int)0x80000000;
 static constexpr int kCodeCap = 4096;
 
 // Push + slot count of one tile of TC cells; LDS: hist[kHR * (TC + 2)],
-// codes[kCodeCap], nabs[2].
-template <int L, int GENB, int TC>
+// codes[kCodeCap], nabs[2], and with WF (CELL32 only) the tile's field window
+// wf[2 * (TC + 1)], from which the gathers read.
+template <int L, int GENB, int TC, bool WF = false>
 __device__ __forceinline__ void push_tile_body(
     const ParticleBuf& b, const GridDev& g, const SpeciesDev& s, int tiles_per_row, int tile,
     const double4* __restrict__ fld, const int32_t* __restrict__ start, int32_t* __restrict__ counts,
     int32_t* __restrict__ rank, unsigned long long* __restrict__ absorbed, int* hist, int* codes,
-    unsigned long long* nabs)
+    unsigned long long* nabs, double4* wf = nullptr)
 {
+    static_assert(!WF || L == PIC_LAYOUT_CELL32, "the field window serves the CELL32 layout");
     constexpr int kHW = TC + 2;
     const int tid = threadIdx.x;
     const int r = tile / tiles_per_row;
     const int c0 = (tile % tiles_per_row) * TC;
     const int cw = min(TC, g.gx - c0);
     const int k0 = r * g.gx + c0;
+    if constexpr (WF) load_field_window<TC>(fld, g, k0, cw, wf);
     for (int j = tid; j < kHR * kHW; j += kBlock) hist[j] = 0;
     if (tid < 2) nabs[tid] = 0ull;
     __syncthreads();
@@ -502,7 +533,10 @@ __device__ __forceinline__ void push_tile_body(
         const int i = base + tid;
         const bool have = i < end;
         Corners cn;
-        if (have) cn = gather<L>(cur, g, fld);
+        if (have) {
+            if constexpr (WF) cn = gather_window<TC>(cur.cell, k0, c0, r, wf);
+            else cn = gather<L>(cur, g, fld);
+        }
         PState<L> nxt;
         if (i + kBlock < end) nxt = load_state<L>(b, i + kBlock);
         int bin = -1, code = -1;

@@ -284,8 +284,10 @@ __global__ __launch_bounds__(kBlock) void k_push_tile(
     __shared__ int hist[kHR * (kTileC + 2)];
     __shared__ int codes[kCodeCap];
     __shared__ unsigned long long nabs[2];
-    push_tile_body<L, GENB, kTileC>(b, g, s, tiles_per_row, xcd_tile(blockIdx.x, ntiles), fld, start, counts, rank,
-                                    absorbed, hist, codes, nabs);
+    constexpr bool WF = L == PIC_LAYOUT_CELL32;
+    __shared__ double4 wf[WF ? 2 * (kTileC + 1) : 1];
+    push_tile_body<L, GENB, kTileC, WF>(b, g, s, tiles_per_row, xcd_tile(blockIdx.x, ntiles), fld, start, counts,
+                                        rank, absorbed, hist, codes, nabs, wf);
 }
 
 template <int L>

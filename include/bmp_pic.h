@@ -104,11 +104,13 @@ typedef struct pic_params {
     double poisson_inner_tol; /* relative residual of each FP32 correction solve (precision 1)  */
     int    mg_smoother;       /* 0: weighted Jacobi (mg_omega); 1: Chebyshev steps on the upper
                                  spectrum of D^-1 A (with mg_pre = mg_post = 2)                */
-    int    poisson_direct;    /* 1: solve exactly (sine transforms as FP64 GEMMs + capacitance
-                                 correction for internal Dirichlet nodes) and let the PCG only
-                                 check the result; used for non-periodic grids of at most 2047
-                                 interior nodes per direction and 1024 internal Dirichlet nodes
-                                 next to unknowns, with precision 0 and no learned guess        */
+    int    poisson_direct;    /* 1: solve exactly (FP64 sine transforms + capacitance correction
+                                 for internal Dirichlet nodes) and let the PCG only check the
+                                 result; used for non-periodic grids of at most 2047 interior
+                                 nodes per direction and 1024 internal Dirichlet nodes next to
+                                 unknowns, with precision 0 and no learned guess.  The
+                                 transforms run as FFTs when NX and NY are powers of two, else
+                                 as GEMMs; 2: GEMMs always.  0: PCG alone                     */
 } pic_params;
 
 /* Per-solve statistics of pic_poisson_solve. */
