@@ -300,16 +300,52 @@ __global__ __launch_bounds__(kBlock) void k_deposit_tile(
 }
 
 // Recompute the push and write each surviving particle to its sorted slot.
+// CELL32: the workgroup's particles are cell-sorted, so their cells are the
+// flat key range [k_first, k_last]; when it is short (the usual case: ~16
+// cells for 256 particles) the corner nodes k_first .. k_last+1 and the same
+// range one node row up are staged in LDS and the gathers read them there.
+static constexpr int kScatterWin = 130;
+
 template <int L, int GENB>
 __global__ __launch_bounds__(kBlock) void k_scatter_push(
     const int32_t* __restrict__ nsrc, ParticleBuf src, ParticleBuf dst, GridDev g, SpeciesDev s,
     const double4* __restrict__ fld, const int32_t* __restrict__ rank, const int32_t* __restrict__ start)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *nsrc) return;
-    const PState<L> p = load_state<L>(src, i);
-    const int rk = rank[i];
-    const Corners c = gather<L>(p, g, fld);
+    const int n = *nsrc;
+    const int i0 = blockIdx.x * blockDim.x;
+    if (i0 >= n) return;
+    const int i = i0 + threadIdx.x;
+    const bool have = i < n;
+    PState<L> p;
+    if (have) p = load_state<L>(src, i);
+    const int rk = have ? rank[i] : 0;
+    Corners c;
+    if constexpr (L == PIC_LAYOUT_CELL32) {
+        __shared__ double4 wf[2 * kScatterWin];
+        const int kf = src.cell[i0], kl = src.cell[min(i0 + kBlock, n) - 1];
+        const int span = kl - kf + 2;                // nodes kf .. kl + 1
+        const bool win = span >= 2 && span <= kScatterWin;
+        if (win)
+            for (int j = threadIdx.x; j < 2 * span; j += kBlock) {
+                const int row = j >= span ? 1 : 0;
+                wf[j] = fld[kf + row * g.gx + (j - row * span)];
+            }
+        __syncthreads();
+        if (!have) return;
+        if (win) {
+            split_key(p.cell, g, c.cgx, c.cgy);
+            const int lc = p.cell - kf;
+            c.f00 = wf[lc];
+            c.f10 = wf[lc + 1];
+            c.f01 = wf[span + lc];
+            c.f11 = wf[span + lc + 1];
+        } else {
+            c = gather<L>(p, g, fld);
+        }
+    } else {
+        if (!have) return;
+        c = gather<L>(p, g, fld);
+    }
     const Pushed<L> o = push_state<L, GENB>(p, c, g, s);
     if (o.key < 0) return;
     const int d = start[o.key] + rk;
