@@ -374,30 +374,39 @@ __device__ __forceinline__ Corners gather(const PState<L>& p, const GridDev& g, 
 }
 
 // Field window of a CELL32 tile (cells c0 .. c0+cw-1 of key row r, first key
-// k0) in LDS: nodes (r, c0 .. c0+cw) then (r+1, c0 .. c0+cw), row stride TC+1;
-// the four corners of every cell of the tile (fld is padded past the last
-// node row, so row r+1 exists for the last key row too).
+// k0) in LDS: nodes (r, c0 .. c0+cw) then (r+1, c0 .. c0+cw), as three
+// component arrays (Ex, Ey, Bz) of kFW = 2 (TC + 1) doubles each; the four
+// corners of every cell of the tile (fld is padded past the last node row, so
+// row r+1 exists for the last key row too).
 template <int TC>
 __device__ __forceinline__ void load_field_window(const double4* __restrict__ fld, const GridDev& g, int k0, int cw,
-                                                  double4* wf) {
-    for (int j = threadIdx.x; j < 2 * (TC + 1); j += kBlock) {
+                                                  double* wf) {
+    constexpr int kFW = 2 * (TC + 1);
+    for (int j = threadIdx.x; j < kFW; j += kBlock) {
         const int row = j >= TC + 1 ? 1 : 0;
         const int col = j - row * (TC + 1);
-        if (col <= cw) wf[j] = fld[k0 + row * g.gx + col];
+        if (col <= cw) {
+            const double4 v = fld[k0 + row * g.gx + col];
+            wf[j] = v.x;
+            wf[kFW + j] = v.y;
+            wf[2 * kFW + j] = v.z;
+        }
     }
 }
 
 // The corners of a particle in cell `cell` of the tile, from the window.
 template <int TC>
-__device__ __forceinline__ Corners gather_window(int cell, int k0, int c0, int r, const double4* wf) {
+__device__ __forceinline__ Corners gather_window(int cell, int k0, int c0, int r, const double* wf) {
+    constexpr int kFW = 2 * (TC + 1);
     Corners c;
     const int lc = cell - k0;
     c.cgx = c0 + lc;
     c.cgy = r;
-    c.f00 = wf[lc];
-    c.f10 = wf[lc + 1];
-    c.f01 = wf[TC + 1 + lc];
-    c.f11 = wf[TC + 2 + lc];
+    const int u = TC + 1 + lc;
+    c.f00 = make_double4(wf[lc], wf[kFW + lc], wf[2 * kFW + lc], 0.0);
+    c.f10 = make_double4(wf[lc + 1], wf[kFW + lc + 1], wf[2 * kFW + lc + 1], 0.0);
+    c.f01 = make_double4(wf[u], wf[kFW + u], wf[2 * kFW + u], 0.0);
+    c.f11 = make_double4(wf[u + 1], wf[kFW + u + 1], wf[2 * kFW + u + 1], 0.0);
     return c;
 }
 
@@ -507,13 +516,13 @@ static constexpr int kCodeCap = 4096;
 
 // Push + slot count of one tile of TC cells; LDS: hist[kHR * (TC + 2)],
 // codes[kCodeCap], nabs[2], and with WF (CELL32 only) the tile's field window
-// wf[2 * (TC + 1)], from which the gathers read.
+// wf[6 * (TC + 1)], from which the gathers read.
 template <int L, int GENB, int TC, bool WF = false>
 __device__ __forceinline__ void push_tile_body(
     const ParticleBuf& b, const GridDev& g, const SpeciesDev& s, int tiles_per_row, int tile,
     const double4* __restrict__ fld, const int32_t* __restrict__ start, int32_t* __restrict__ counts,
     int32_t* __restrict__ rank, unsigned long long* __restrict__ absorbed, int* hist, int* codes,
-    unsigned long long* nabs, double4* wf = nullptr)
+    unsigned long long* nabs, double* wf = nullptr)
 {
     static_assert(!WF || L == PIC_LAYOUT_CELL32, "the field window serves the CELL32 layout");
     constexpr int kHW = TC + 2;

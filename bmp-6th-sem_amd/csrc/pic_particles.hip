@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_push_tile(
     __shared__ int codes[kCodeCap];
     __shared__ unsigned long long nabs[2];
     constexpr bool WF = L == PIC_LAYOUT_CELL32;
-    __shared__ double4 wf[WF ? 2 * (kTileC + 1) : 1];
+    __shared__ double wf[WF ? 6 * (kTileC + 1) : 1];
     push_tile_body<L, GENB, kTileC, WF>(b, g, s, tiles_per_row, xcd_tile(blockIdx.x, ntiles), fld, start, counts,
                                         rank, absorbed, hist, codes, nabs, wf);
 }
