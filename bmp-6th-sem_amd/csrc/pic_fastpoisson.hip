@@ -21,7 +21,8 @@
 //     the internal Dirichlet nodes, so u solves the reference's system there.
 //
 // Per solve: four transform passes (FFT passes, or GEMMs of n x n x m /
-// m x m x n), one n x P x m GEMM for the correction, a few small kernels.  The PCG of
+// m x m x n), one n x nq x m GEMM for the correction (nq: the rows holding
+// Q), a few small kernels.  The PCG of
 // pic_poisson.hip then starts from this solution and only checks (or, if
 // rounding left the residual above the tolerance, corrects) it.
 #include "pic_internal.h"
@@ -38,7 +39,7 @@ static constexpr int kFdMaxP = 1024;      // internal Dirichlet nodes next to un
 static constexpr int kFdMaxN = 2047;      // interior nodes per direction
 static constexpr int kFdB = 256;
 static constexpr int kLines = 16;         // covering lines handled by the line kernels (more: GEMMs)
-static constexpr int kSeg = 16;           // row segments of the row-line products
+static constexpr int kSeg = 64;           // row segments of the line products
 
 struct FastPoisson {
     int m = 0, n = 0, P = 0, nr = 0, nc = 0;
@@ -51,9 +52,13 @@ struct FastPoisson {
     double* SmC = nullptr;                   // nc x m: rows of S_m at the covering columns
     double* V2 = nullptr;                    // n x nc (column-major): Uhat S_m[:, cols]
     double* part = nullptr;                  // kSeg x nr x m partial row-line products
-    double* SnQ = nullptr;                   // n x P: S_n[k][row(q)]
-    double* SmQ = nullptr;                   // P x m: S_m[col(q)][l]
-    double* Z = nullptr;                     // P x m: c_q S_m[col(q)][l]
+    double* part2 = nullptr;                 // ceil(m/64) x nc x n partial column-line products
+    int nq = 0;                              // distinct rows of Q
+    double* SnQ = nullptr;                   // n x nq: S_n[k][row a]
+    double* Z = nullptr;                     // nq x m: sum over q on row a of c_q S_m[col(q)][l]
+    double* cq = nullptr;                    // P: the sources c = C^-1 (V_Q - u0_Q), in row order
+    int *rq_ptr = nullptr, *rq_idx = nullptr;   // the q of distinct row a: rq_idx[rq_ptr[a] .. rq_ptr[a+1])
+    int* rq_col = nullptr;                   // P: column of rq_idx[t]
     double* Cinv = nullptr;                  // P x P
     double* VQ = nullptr;                    // P: potentials on Q
     double* dq = nullptr;                    // P: V_Q - u0_Q
@@ -77,8 +82,8 @@ static void* dmalloc(size_t bytes, bool& ok) {
 void fast_free(pic_ctx* c) {
     FastPoisson* f = c->fast;
     if (!f) return;
-    void* ptrs[] = {f->Sm, f->Sn, f->Dinv, f->F, f->W, f->U, f->G, f->SnR, f->V1, f->SmC, f->V2, f->SnQ, f->SmQ,
-                    f->Z, f->Cinv, f->VQ, f->dq, f->qri, f->qcol, f->qrow, f->part, f->twm, f->twn, f->Fb};
+    void* ptrs[] = {f->Sm, f->Sn, f->Dinv, f->F, f->W, f->U, f->G, f->SnR, f->V1, f->SmC, f->V2, f->SnQ, f->cq,
+                    f->rq_ptr, f->rq_idx, f->rq_col, f->Z, f->Cinv, f->VQ, f->dq, f->qri, f->qcol, f->qrow, f->part, f->part2, f->twm, f->twn, f->Fb};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (f->blas) (void)rocblas_destroy_handle(f->blas);
@@ -153,27 +158,40 @@ __global__ __launch_bounds__(kFdB) void k_fd_u0q(int P, int n, int m, const doub
     if (lane == 0) dq[q] = VQ[q] - s;
 }
 
-// Row lines: V1[a][l] = sum_k S_n[row a][k] Uhat[k][l] for nr <= kLines rows,
-// one pass over Uhat: workgroup (64 columns, one of kSeg row segments), four
-// waves over interleaved rows, partial sums per segment (added in a fixed
-// order by k_fd_rowsum).
-__global__ __launch_bounds__(kFdB) void k_fd_rowlines(int nr, int n, int m, const double* __restrict__ SnR,
-                                                      const double* __restrict__ U, double* __restrict__ part) {
+// Covering lines in one pass over Uhat (nr, nc <= kLines; either may be 0):
+//   rows:    V1[a][l] = sum_k S_n[row a][k] Uhat[k][l]
+//   columns: V2[j][k] = sum_l Uhat[k][l] S_m[col j][l]
+// Workgroup (64 columns, one of kSeg row segments), four waves over
+// interleaved rows.  Row lines: per-segment partials of each column (part);
+// column lines: per-column-block partials of each row, one wave sum per row
+// (part2).  k_fd_linesum adds both in a fixed order.
+__global__ __launch_bounds__(kFdB) void k_fd_lines(int nr, int nc, int n, int m, const double* __restrict__ SnR,
+                                                   const double* __restrict__ SmC, const double* __restrict__ U,
+                                                   double* __restrict__ part, double* __restrict__ part2) {
     __shared__ double sh[4][kLines][64];
     const int lx = threadIdx.x & 63, ph = threadIdx.x >> 6;
     const int l = blockIdx.x * 64 + lx;
     const int per = (n + kSeg - 1) / kSeg;
     const int k0 = blockIdx.y * per, k1 = min(n, k0 + per);
-    double acc[kLines];
+    double acc[kLines], smc[kLines];
 #pragma unroll
-    for (int a = 0; a < kLines; ++a) acc[a] = 0.0;
-    if (l < m)
-        for (int k = k0 + ph; k < k1; k += 4) {
-            const double u = U[(size_t)k * m + l];
+    for (int a = 0; a < kLines; ++a) {
+        acc[a] = 0.0;
+        smc[a] = (a < nc && l < m) ? SmC[(size_t)a * m + l] : 0.0;
+    }
+    for (int k = k0 + ph; k < k1; k += 4) {
+        const double u = l < m ? U[(size_t)k * m + l] : 0.0;
 #pragma unroll
-            for (int a = 0; a < kLines; ++a)
-                if (a < nr) acc[a] += SnR[(size_t)a * n + k] * u;
-        }
+        for (int a = 0; a < kLines; ++a)
+            if (a < nr) acc[a] += SnR[(size_t)a * n + k] * u;
+#pragma unroll
+        for (int j = 0; j < kLines; ++j)
+            if (j < nc) {
+                const double v = wave_sum(u * smc[j]);
+                if (lx == 0) part2[((size_t)blockIdx.x * nc + j) * n + k] = v;
+            }
+    }
+    if (nr == 0) return;
 #pragma unroll
     for (int a = 0; a < kLines; ++a) sh[ph][a][lx] = acc[a];
     __syncthreads();
@@ -182,53 +200,63 @@ __global__ __launch_bounds__(kFdB) void k_fd_rowlines(int nr, int n, int m, cons
             part[((size_t)blockIdx.y * nr + a) * m + l] = (sh[0][a][lx] + sh[1][a][lx]) + (sh[2][a][lx] + sh[3][a][lx]);
 }
 
-__global__ __launch_bounds__(kFdB) void k_fd_rowsum(int nr, int m, const double* __restrict__ part,
-                                                    double* __restrict__ V1) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nr * m) return;
-    double s = 0.0;
-    for (int seg = 0; seg < kSeg; ++seg) s += part[(size_t)seg * nr * m + k];
-    V1[k] = s;
+// V1 (nr x m) from the kSeg segment partials; V2 (nc x n) from the ncb
+// column-block partials
+__global__ __launch_bounds__(kFdB) void k_fd_linesum(int nr, int nc, int n, int m, int ncb,
+                                                     const double* __restrict__ part, const double* __restrict__ part2,
+                                                     double* __restrict__ V1, double* __restrict__ V2) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nr * m) {
+        double s = 0.0;
+        for (int seg = 0; seg < kSeg; ++seg) s += part[(size_t)seg * nr * m + t];
+        V1[t] = s;
+    } else if (t < nr * m + nc * n) {
+        const int u = t - nr * m;
+        const int j = u / n, k = u - j * n;
+        double s = 0.0;
+        for (int cb = 0; cb < ncb; ++cb) s += part2[((size_t)cb * nc + j) * n + k];
+        V2[(size_t)j * n + k] = s;
+    }
 }
 
-// Column lines: V2[k][j] = Uhat[k] . S_m[col j] for nc <= kLines columns; one
-// workgroup per row k (the row read once).
-__global__ __launch_bounds__(kFdB) void k_fd_collines(int nc, int n, int m, const double* __restrict__ U,
-                                                      const double* __restrict__ SmC, double* __restrict__ V2) {
-    __shared__ double sh[kLines][kFdB / 64];
-    const int k = blockIdx.x;
-    double acc[kLines];
-#pragma unroll
-    for (int a = 0; a < kLines; ++a) acc[a] = 0.0;
-    for (int l = threadIdx.x; l < m; l += kFdB) {
-        const double u = U[(size_t)k * m + l];
-#pragma unroll
-        for (int a = 0; a < kLines; ++a)
-            if (a < nc) acc[a] += u * SmC[(size_t)a * m + l];
-    }
-#pragma unroll
-    for (int a = 0; a < kLines; ++a) {
-        const double v = wave_sum(acc[a]);
-        if ((threadIdx.x & 63) == 0) sh[a][threadIdx.x >> 6] = v;
-    }
-    __syncthreads();
-    const int t = threadIdx.x;
-    if (t < nc) V2[(size_t)t * n + k] = (sh[t][0] + sh[t][1]) + (sh[t][2] + sh[t][3]);
-}
-
-// c_q = (C^-1 d)_q, then Z[q][l] = c_q S_m[col q][l]; one workgroup per q
-__global__ __launch_bounds__(kFdB) void k_fd_sources(int P, int m, const double* __restrict__ Cinv,
-                                                     const double* __restrict__ dq, const double* __restrict__ SmQ,
-                                                     double* __restrict__ Z) {
-    __shared__ double sh[kFdB / 64];
-    const int q = blockIdx.x;
+// The sources in row order: c_t = (C^-1 d)_q for q = rq_idx[t]; one wave per t
+__global__ __launch_bounds__(kFdB) void k_fd_sources(int P, const double* __restrict__ Cinv,
+                                                     const double* __restrict__ dq, const int* __restrict__ idx,
+                                                     double* __restrict__ ct) {
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (t >= P) return;
+    const int q = idx[t];
     double s = 0.0;
-    for (int p = threadIdx.x; p < P; p += kFdB) s += Cinv[(size_t)q * P + p] * dq[p];
+    for (int p = lane; p < P; p += 64) s += Cinv[(size_t)q * P + p] * dq[p];
     s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    if (lane == 0) ct[t] = s;
+}
+
+// Z[a][l] = sum over the sources t of distinct row a of c_t S_m[col t][l].
+// Workgroup (64 columns, row a): the row's (c_t, col t) staged in LDS, wave w
+// sums t = w, w + 4, ..., the four wave sums added in a fixed order.
+__global__ __launch_bounds__(kFdB) void k_fd_zrows(int m, const double* __restrict__ ct, const int* __restrict__ colt,
+                                                   const int* __restrict__ ptr, const double* __restrict__ Sm,
+                                                   double* __restrict__ Z) {
+    __shared__ double cs[kFdMaxP];
+    __shared__ int cl[kFdMaxP];
+    __shared__ double part[4][64];
+    const int a = blockIdx.y;
+    const int t0 = ptr[a], nt = ptr[a + 1] - t0;
+    for (int t = threadIdx.x; t < nt; t += kFdB) {
+        cs[t] = ct[t0 + t];
+        cl[t] = colt[t0 + t];
+    }
     __syncthreads();
-    const double cq = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-    for (int l = threadIdx.x; l < m; l += kFdB) Z[(size_t)q * m + l] = cq * SmQ[(size_t)q * m + l];
+    const int lx = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int l = min((int)blockIdx.x * 64 + lx, m - 1);
+    double sum = 0.0;
+#pragma unroll 8
+    for (int t = w; t < nt; t += 4) sum += cs[t] * Sm[(size_t)cl[t] * m + l];
+    part[w][lx] = sum;
+    __syncthreads();
+    if (w == 0 && (int)blockIdx.x * 64 + lx < m)
+        Z[(size_t)a * m + l] = (part[0][lx] + part[1][lx]) + (part[2][lx] + part[3][lx]);
 }
 
 // full-grid solution: interior unknowns from U, every Dirichlet node its value
@@ -676,15 +704,27 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
         }
     }
     std::vector<double> SnR((size_t)std::max(nr, 1) * n), SmC((size_t)std::max(nc, 1) * m);
-    std::vector<double> SnQ((size_t)n * std::max(P, 1)), SmQ((size_t)std::max(P, 1) * m);
+    // distinct rows of Q and the q on each (ascending q)
+    std::vector<int> qr_list, rq_ptr(1, 0), rq_idx;
+    {
+        std::vector<std::vector<int>> by_row(n);
+        for (int q = 0; q < P; ++q) by_row[qrow[q]].push_back(q);
+        for (int r = 0; r < n; ++r)
+            if (!by_row[r].empty()) {
+                qr_list.push_back(r);
+                rq_idx.insert(rq_idx.end(), by_row[r].begin(), by_row[r].end());
+                rq_ptr.push_back((int)rq_idx.size());
+            }
+    }
+    const int nq = (int)qr_list.size();
+    f->nq = nq;
+    std::vector<double> SnQ((size_t)n * std::max(nq, 1));
     for (int a = 0; a < nr; ++a)
         for (int k = 0; k < n; ++k) SnR[(size_t)a * n + k] = Sn[(size_t)rows[a] * n + k];
     for (int j = 0; j < nc; ++j)
         for (int l = 0; l < m; ++l) SmC[(size_t)j * m + l] = Sm[(size_t)cols[j] * m + l];
     for (int k = 0; k < n; ++k)
-        for (int q = 0; q < P; ++q) SnQ[(size_t)k * P + q] = Sn[(size_t)qrow[q] * n + k];
-    for (int q = 0; q < P; ++q)
-        for (int l = 0; l < m; ++l) SmQ[(size_t)q * m + l] = Sm[(size_t)qcol[q] * m + l];
+        for (int a = 0; a < nq; ++a) SnQ[(size_t)k * nq + a] = Sn[(size_t)qr_list[a] * n + k];
 
     bool ok = true;
     const size_t nm = (size_t)n * m;
@@ -729,9 +769,13 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
         f->Fb = (double*)dmalloc(nm * 8, ok);
     }
     if (nr > 0 && nr <= kLines) f->part = (double*)dmalloc((size_t)kSeg * nr * m * 8, ok);
+    if (nc > 0 && nc <= kLines) f->part2 = (double*)dmalloc((size_t)((m + 63) / 64) * nc * n * 8, ok);
     f->SnQ = (double*)dmalloc(SnQ.size() * 8, ok);
-    f->SmQ = (double*)dmalloc(SmQ.size() * 8, ok);
-    f->Z = (double*)dmalloc(SmQ.size() * 8, ok);
+    f->Z = (double*)dmalloc((size_t)std::max(nq, 1) * m * 8, ok);
+    f->cq = (double*)dmalloc((size_t)std::max(P, 1) * 8, ok);
+    f->rq_ptr = (int*)dmalloc(rq_ptr.size() * 4, ok);
+    f->rq_idx = (int*)dmalloc((size_t)std::max(P, 1) * 4, ok);
+    f->rq_col = (int*)dmalloc((size_t)std::max(P, 1) * 4, ok);
     f->Cinv = (double*)dmalloc((size_t)std::max(P, 1) * std::max(P, 1) * 8, ok);
     f->VQ = (double*)dmalloc((size_t)std::max(P, 1) * 8, ok);
     f->dq = (double*)dmalloc((size_t)std::max(P, 1) * 8, ok);
@@ -750,7 +794,13 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
     PIC_HIP(hipMemcpy(f->SnR, SnR.data(), SnR.size() * 8, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(f->SmC, SmC.data(), SmC.size() * 8, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(f->SnQ, SnQ.data(), SnQ.size() * 8, hipMemcpyHostToDevice));
-    PIC_HIP(hipMemcpy(f->SmQ, SmQ.data(), SmQ.size() * 8, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(f->rq_ptr, rq_ptr.data(), rq_ptr.size() * 4, hipMemcpyHostToDevice));
+    if (P > 0) {
+        std::vector<int> rq_col(P);
+        for (int t = 0; t < P; ++t) rq_col[t] = qcol[rq_idx[t]];
+        PIC_HIP(hipMemcpy(f->rq_idx, rq_idx.data(), (size_t)P * 4, hipMemcpyHostToDevice));
+        PIC_HIP(hipMemcpy(f->rq_col, rq_col.data(), (size_t)P * 4, hipMemcpyHostToDevice));
+    }
     if (f->fft) {
         PIC_HIP(hipMemcpy(f->twm, twm.data(), twm.size() * sizeof(double2), hipMemcpyHostToDevice));
         PIC_HIP(hipMemcpy(f->twn, twn.data(), twn.size() * sizeof(double2), hipMemcpyHostToDevice));
@@ -841,26 +891,28 @@ int fast_solve(pic_ctx* c, double* out) {
         // u0 on Q through the covering lines: V1 = S_n[rows] Uhat (nr x m),
         // V1^T = Uhat^T S_n[rows]^T; V2 = Uhat S_m[cols]^T (n x nc, column-major)
         // (a few lines: one pass over Uhat each; many lines: GEMMs)
-        if (nr > 0 && nr <= kLines) {
-            hipLaunchKernelGGL(k_fd_rowlines, dim3((unsigned)((m + 63) / 64), kSeg), dim3(kFdB), 0, s, nr, n, m,
-                               f->SnR, f->U, f->part);
-            hipLaunchKernelGGL(k_fd_rowsum, dim3((unsigned)((nr * m + kFdB - 1) / kFdB)), dim3(kFdB), 0, s, nr, m,
-                               f->part, f->V1);
-        } else if (nr > 0) {
-            PIC_TRY(gemm(f->blas, m, nr, n, f->U, m, f->SnR, n, 0.0, f->V1, m));
+        const int lr = nr <= kLines ? nr : 0, lc = nc <= kLines ? nc : 0;
+        if (lr + lc > 0) {
+            const int ncb = (m + 63) / 64;
+            hipLaunchKernelGGL(k_fd_lines, dim3((unsigned)ncb, kSeg), dim3(kFdB), 0, s, lr, lc, n, m, f->SnR, f->SmC,
+                               f->U, f->part, f->part2);
+            hipLaunchKernelGGL(k_fd_linesum, dim3((unsigned)((lr * m + lc * n + kFdB - 1) / kFdB)), dim3(kFdB), 0, s,
+                               lr, lc, n, m, ncb, f->part, f->part2, f->V1, f->V2);
         }
-        if (nc > 0 && nc <= kLines) {
-            hipLaunchKernelGGL(k_fd_collines, dim3((unsigned)n), dim3(kFdB), 0, s, nc, n, m, f->U, f->SmC, f->V2);
-        } else if (nc > 0) {
+        if (nr > kLines) PIC_TRY(gemm(f->blas, m, nr, n, f->U, m, f->SnR, n, 0.0, f->V1, m));
+        if (nc > kLines) {
             const double one = 1.0, zero = 0.0;
             FD_BLAS(rocblas_dgemm(f->blas, rocblas_operation_transpose, rocblas_operation_none, n, nc, m, &one, f->U,
                                   m, f->SmC, m, &zero, f->V2, n));
         }
         hipLaunchKernelGGL(k_fd_u0q, dim3((unsigned)((P * 64 + kFdB - 1) / kFdB)), dim3(kFdB), 0, s, P, n, m, f->V1,
                            f->V2, f->Sn, f->Sm, f->qri, f->qrow, f->qcol, f->VQ, f->dq);
-        hipLaunchKernelGGL(k_fd_sources, dim3((unsigned)P), dim3(kFdB), 0, s, P, m, f->Cinv, f->dq, f->SmQ, f->Z);
-        // G = S_n[:, rows(Q)] Z (n x P x m) -> G^T = Z^T SnQ^T
-        PIC_TRY(gemm(f->blas, m, n, P, f->Z, m, f->SnQ, P, 0.0, f->G, m));
+        hipLaunchKernelGGL(k_fd_sources, dim3((unsigned)((P * 64 + kFdB - 1) / kFdB)), dim3(kFdB), 0, s, P, f->Cinv,
+                           f->dq, f->rq_idx, f->cq);
+        hipLaunchKernelGGL(k_fd_zrows, dim3((unsigned)((m + 63) / 64), (unsigned)f->nq), dim3(kFdB), 0, s, m, f->cq,
+                           f->rq_col, f->rq_ptr, f->Sm, f->Z);
+        // G = S_n[:, rows of Q] Z (n x nq x m) -> G^T = Z^T SnQ^T
+        PIC_TRY(gemm(f->blas, m, n, f->nq, f->Z, m, f->SnQ, f->nq, 0.0, f->G, m));
         if (!f->fft) hipLaunchKernelGGL(k_fd_addscaled, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->G, f->Dinv);
     }
     if (f->fft) {

@@ -126,7 +126,7 @@ extern "C" void pic_destroy(pic_ctx* c) {
     poisson_free(c);
     if (c->unet) pic_unet_destroy(c->unet);
     void* ptrs[] = {c->rho, c->phi, c->ex, c->ey, c->bz, c->fld, c->dir, c->dval, c->pr, c->pr2, c->pp, c->pp2,
-                    c->pq, c->pb, c->zf0, c->zf1, c->guess, c->phi_prev, c->part, c->scal, c->flags, c->scan_bsum,
+                    c->pq, c->zf0, c->zf1, c->guess, c->phi_prev, c->part, c->scal, c->flags, c->scan_bsum,
                     c->rf0, c->rf1, c->pf0, c->pf1, c->qf, c->df};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -262,7 +262,6 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
     ALLOC(c->pp, G * sizeof(double));
     ALLOC(c->pp2, G * sizeof(double));
     ALLOC(c->pq, G * sizeof(double));
-    ALLOC(c->pb, G * sizeof(double));
     ALLOC(c->zf0, G * sizeof(float));
     ALLOC(c->zf1, G * sizeof(float));
     ALLOC(c->guess, G * sizeof(double));

@@ -149,7 +149,7 @@ struct pic_ctx {
     // Poisson work vectors (FP64, fine grid)
     double *pr = nullptr, *pr2 = nullptr;     // residual, alternating between iterations
     double *pp = nullptr, *pp2 = nullptr;     // search direction, alternating
-    double *pq = nullptr, *pb = nullptr;      // A p; residual of the Dirichlet-only field
+    double* pq = nullptr;                     // A p
     float *zf0 = nullptr, *zf1 = nullptr;     // preconditioned residual z (FP32), alternating
     // FP32 correction solve of the mixed-precision path (allocated on first use)
     float *rf0 = nullptr, *rf1 = nullptr, *pf0 = nullptr, *pf1 = nullptr, *qf = nullptr, *df = nullptr;

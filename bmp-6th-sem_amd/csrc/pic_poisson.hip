@@ -144,13 +144,6 @@ __global__ void k_init_guess(int G, double* x, const double* src, const double* 
     x[n] = dir[n] ? dval[n] : v;
 }
 
-__global__ void k_reset_flags(int* flags) {
-    flags[0] = 0;   // outer: converged (1) / failed (2)
-    flags[1] = 0;   // outer: iterations
-    flags[2] = 0;   // inner (mixed precision): converged / failed
-    flags[3] = 0;   // inner: iterations, all corrections together
-}
-
 // r = rhs - A x on unknown nodes (0 on Dirichlet); partial of r.r
 __global__ __launch_bounds__(kB) void k_residual(Fine f, const double* __restrict__ rhs,
                                                  const double* __restrict__ x, const uint8_t* __restrict__ dir,
@@ -169,6 +162,32 @@ __global__ __launch_bounds__(kB) void k_residual(Fine f, const double* __restric
         acc += v * v;
     }
     write_partial(acc, part, sh);
+}
+
+// The start of every solve in one pass: flags cleared; overwriteRho
+// (rho[geom] = V, rho = -rho); the partial of b.b, b = the residual of the
+// Dirichlet-only field (unknowns at zero) on the unknown nodes.
+__global__ __launch_bounds__(kB) void k_prep_rhs(Fine f, double* __restrict__ rho, const uint8_t* __restrict__ dir,
+                                                 const double* __restrict__ dval, double* part_bb, int* flags) {
+    __shared__ double sh[kB / 64];
+    // flags: [0] outer converged (1) / failed (2), [1] outer iterations,
+    // [2] inner (mixed precision) converged / failed, [3] inner iterations
+    if (blockIdx.x == 0 && threadIdx.x < 4) flags[threadIdx.x] = 0;
+    double acc = 0;
+    const int G = f.gx * f.gy;
+    auto x0 = [&](int m) -> double { return dir[m] ? dval[m] : 0.0; };
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
+        const uint8_t d = dir[n];
+        const double r = d == 1 ? dval[n] : rho[n];
+        rho[n] = -r;
+        if (!d) {
+            const int so = south_n(f.per, n, f.gx, f.gy), no = north_n(f.per, n, f.gx, f.gy);
+            const double ax = f.diag * 0.0 - f.cx * (x0(n - 1) + x0(n + 1)) - f.cy * (x0(n + so) + x0(n + no));
+            const double v = -r - ax;
+            acc += v * v;
+        }
+    }
+    write_partial(acc, part_bb, sh);
 }
 
 // rr = sum of the partials; done when rr <= tol^2 bb (flags[0] = 1), NaN -> 2.
@@ -1311,12 +1330,8 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     const bool mixed = c->p.poisson_precision == 1;
     if (mixed) PIC_TRY(ensure_f32(c));
 
-    hipLaunchKernelGGL(k_reset_flags, dim3(1), dim3(1), 0, s, c->flags);
-    PIC_TRY(launch_overwrite_rho(c));
-    // ||b||^2: residual of the Dirichlet-only field (unknowns at zero)
-    hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->pq, (const double*)nullptr,
-                       (const double*)nullptr, (double*)nullptr, c->dir, c->dval);
-    hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->pq, c->dir, c->pb, part + P_BB * kNP);
+    // flags cleared, overwriteRho, ||b||^2 (b: residual of the Dirichlet-only field)
+    hipLaunchKernelGGL(k_prep_rhs, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->dir, c->dval, part + P_BB * kNP, c->flags);
     // initial guess
     int guess = 0;
     const double* src = nullptr;
