@@ -346,7 +346,6 @@ struct DstStoreT {
     double* Y;
     int R;
     struct Aux {};
-    static constexpr bool kGrid = false;
     __device__ Aux fetch(int, int) const { return {}; }
     __device__ void put(int t, int row, double v, Aux) const { Y[(size_t)t * R + row] = v; }
 };
@@ -357,27 +356,18 @@ struct DstStoreTScaled {
     const double* Dinv;
     int R;
     using Aux = double;
-    static constexpr bool kGrid = false;
     __device__ Aux fetch(int t, int row) const { return Dinv[(size_t)t * R + row]; }
     __device__ void put(int t, int row, double v, Aux d) const { Y[(size_t)t * R + row] = v * d; }
 };
 
-// the full grid: interior node (t + 1, row + 1); Dirichlet nodes keep their value
+// the grid's interior node (t + 1, row + 1) (the caller imposes the
+// Dirichlet values afterwards)
 struct DstStoreGrid {
     double* out;
-    const uint8_t* dir;
-    const double* dval;
-    int gx, gy;
-    struct Aux {
-        double val;
-        int fixed;
-    };
-    static constexpr bool kGrid = true;
-    __device__ Aux fetch(int t, int row) const {
-        const int node = (t + 1) * gx + row + 1;
-        return {dval[node], (int)dir[node]};
-    }
-    __device__ void put(int t, int row, double v, Aux a) const { out[(t + 1) * gx + row + 1] = a.fixed ? a.val : v; }
+    int gx;
+    struct Aux {};
+    __device__ Aux fetch(int, int) const { return {}; }
+    __device__ void put(int t, int row, double v, Aux) const { out[(t + 1) * gx + row + 1] = v; }
 };
 
 // One pass: R rows of N = 2^(L-1) - 1 values in, N rows of R values out.
@@ -509,22 +499,6 @@ __global__ __launch_bounds__(kDstT) void k_fd_dst(int R, int nwg, Load ld, Store
             if (ra < R) st.put(t, ra, -v.y * sc, xa[it]);
             if (ra + 1 < R) st.put(t, ra + 1, v.x * sc, xb[it]);
         }
-    }
-    if constexpr (Store::kGrid) {
-        // the wall nodes: rows 0 and GY-1 above and below these columns; the
-        // first workgroup also writes columns 0 and GX-1
-        for (int e = tid; e < 2 * NB; e += kDstT) {
-            const int row = r0 + e;
-            if (row < R) {
-                st.out[row + 1] = st.dval[row + 1];
-                st.out[(size_t)(st.gy - 1) * st.gx + row + 1] = st.dval[(size_t)(st.gy - 1) * st.gx + row + 1];
-            }
-        }
-        if (w == 0)
-            for (int y = tid; y < st.gy; y += kDstT) {
-                st.out[(size_t)y * st.gx] = st.dval[(size_t)y * st.gx];
-                st.out[(size_t)y * st.gx + st.gx - 1] = st.dval[(size_t)y * st.gx + st.gx - 1];
-            }
     }
 }
 
@@ -867,7 +841,9 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
 }
 
 // The solution of the current right-hand side (c->rho after overwriteRho)
-// into `out` (all nodes, Dirichlet values included).
+// into the interior nodes of `out`; the caller imposes the Dirichlet values
+// (internal Dirichlet nodes hold values of the unconstrained solve until then;
+// the GEMM path writes them already).
 int fast_solve(pic_ctx* c, double* out) {
     FastPoisson* f = c->fast;
     const int gx = c->g.gx, gy = c->g.gy, m = f->m, n = f->n, P = f->P, nr = f->nr, nc = f->nc;
@@ -916,12 +892,12 @@ int fast_solve(pic_ctx* c, double* out) {
         if (!f->fft) hipLaunchKernelGGL(k_fd_addscaled, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->G, f->Dinv);
     }
     if (f->fft) {
-        // rows of Uhat (+ G ./ Lambda) -> W (m x n) -> the grid (interior + Dirichlet values)
+        // rows of Uhat (+ G ./ Lambda) -> W (m x n) -> the grid's interior nodes
         if (P > 0)
             dst_pass(f->lm, n, DstLoadCorrected{f->U, f->G, f->Dinv, m}, DstStoreT{f->W, n}, f->twm, s);
         else
             dst_pass(f->lm, n, DstLoadRows{f->U, m}, DstStoreT{f->W, n}, f->twm, s);
-        dst_pass(f->ln, m, DstLoadRows{f->W, n}, DstStoreGrid{out, c->dir, c->dval, gx, gy}, f->twn, s);
+        dst_pass(f->ln, m, DstLoadRows{f->W, n}, DstStoreGrid{out, gx}, f->twn, s);
         PIC_HIP(hipGetLastError());
         return PIC_OK;
     }

@@ -126,12 +126,11 @@ extern "C" void pic_destroy(pic_ctx* c) {
     poisson_free(c);
     if (c->unet) pic_unet_destroy(c->unet);
     void* ptrs[] = {c->rho, c->phi, c->ex, c->ey, c->bz, c->fld, c->dir, c->dval, c->pr, c->pr2, c->pp, c->pp2,
-                    c->pq, c->zf0, c->zf1, c->guess, c->phi_prev, c->part, c->scal, c->flags, c->scan_bsum,
+                    c->pq, c->zf0, c->zf1, c->guess, c->phi_prev, c->part, c->scal, c->scan_bsum,
                     c->rf0, c->rf1, c->pf0, c->pf1, c->qf, c->df};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c->scal_host) (void)hipHostFree(c->scal_host);
-    if (c->flags_host) (void)hipHostFree(c->flags_host);
     for (auto& t : c->timers)
         for (auto& e : t.pending) {
             (void)hipEventDestroy(e.first);
@@ -267,8 +266,9 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
     ALLOC(c->guess, G * sizeof(double));
     ALLOC(c->phi_prev, G * sizeof(double));
     ALLOC(c->part, PIC_PART_DOUBLES * sizeof(double));
-    ALLOC(c->scal, 16 * sizeof(double));
-    ALLOC(c->flags, 4 * sizeof(int));
+    // solver scalars and flags in one block (one copy brings both to the host)
+    ALLOC(c->scal, PIC_STATE_BYTES);
+    c->flags = (int*)(c->scal + PIC_SCAL_DOUBLES);
     c->scan_nb = (int)((G + 2047) / 2048);
     if (c->scan_nb > 8192) {
         set_error("pic_create: grid too large for the two-level scan");
@@ -276,17 +276,17 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
     }
     ALLOC(c->scan_bsum, c->scan_nb * sizeof(int32_t));
 #undef ALLOC
-    if (hipHostMalloc((void**)&c->scal_host, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&c->flags_host, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&c->scal_host, PIC_STATE_BYTES, hipHostMallocDefault) != hipSuccess) {
         set_error("pic_create: hipHostMalloc failed");
         return fail(PIC_ERR_ALLOC);
     }
+    c->flags_host = (int*)(c->scal_host + PIC_SCAL_DOUBLES);
     (void)hipMemset(c->rho, 0, G * sizeof(double));
     (void)hipMemset(c->phi, 0, G * sizeof(double));
     (void)hipMemset(c->ex, 0, G * sizeof(double));
     (void)hipMemset(c->ey, 0, G * sizeof(double));
     (void)hipMemset(c->fld, 0, pad * sizeof(double4));
-    (void)hipMemset(c->flags, 0, 4 * sizeof(int));
+    (void)hipMemset(c->scal, 0, PIC_STATE_BYTES);
     (void)hipMemset(c->part, 0, PIC_PART_DOUBLES * sizeof(double));
     (void)hipMemset(c->pp, 0, G * sizeof(double));
     (void)hipMemset(c->pp2, 0, G * sizeof(double));
@@ -719,8 +719,7 @@ extern "C" int pic_step(pic_ctx* c, int iteration, pic_solve_stats* st) {
         set_error("pic_step: null context");
         return PIC_ERR_ARG;
     }
-    PIC_TRY(launch_zero_f64(c, c->rho, c->G));
-    for (int s = 0; s < (int)c->sp.size(); ++s) PIC_TRY(launch_deposit(c, s));
+    PIC_TRY(launch_deposit_all(c));
     PIC_TRY(poisson_solve(c, iteration, st));   // also computes E
     if (!c->inj.on) {
         for (int s = 0; s < (int)c->sp.size(); ++s) PIC_TRY(launch_move(c, s));

@@ -13,6 +13,9 @@
 // per-block partial sums of the Poisson dot products: 8 arrays of 2048
 #define PIC_PART_BLOCKS 2048
 #define PIC_PART_DOUBLES (8 * PIC_PART_BLOCKS)
+// solver state block: 16 double scalars, then 4 int flags
+#define PIC_SCAL_DOUBLES 16
+#define PIC_STATE_BYTES (PIC_SCAL_DOUBLES * 8 + 4 * 4)
 
 namespace pic {
 
@@ -173,10 +176,11 @@ struct pic_ctx {
     int mg_lc = 0;                  // first level of the in-LDS coarse cycle
     int mg_ctotal = 0;              // nodes of the in-LDS levels
     double* part = nullptr;         // per-block partial sums of the dot products
-    double* scal = nullptr;         // device scalars
-    int* flags = nullptr;           // [0] converged / failed, [1] iterations
-    double* scal_host = nullptr;    // pinned
-    int* flags_host = nullptr;      // pinned
+    double* scal = nullptr;         // device scalars (the state block)
+    int* flags = nullptr;           // in the state block: [0] converged / failed, [1] iterations, [2], [3] inner
+                                    // solve
+    double* scal_host = nullptr;    // pinned copy of the state block
+    int* flags_host = nullptr;      // in it
     int scan_nb = 0;
     int32_t* scan_bsum = nullptr;
     pic::Injection inj;
@@ -194,7 +198,8 @@ namespace pic {
 int launch_encode_f64(pic_ctx* c, int s, int64_t n, const double* x, const double* y,
                       const double* vx, const double* vy, const double* vz);
 int launch_bin(pic_ctx* c, int s);
-int launch_deposit(pic_ctx* c, int s);
+int launch_deposit(pic_ctx* c, int s);            // adds species s to rho
+int launch_deposit_all(pic_ctx* c);               // every species; writes rho
 int launch_push(pic_ctx* c, int s);
 int launch_move(pic_ctx* c, int s);
 int move_push(pic_ctx* c, int s);                 // launch_move in phases, for injection between them
@@ -218,7 +223,8 @@ int poisson_setup(pic_ctx* c);
 void poisson_free(pic_ctx* c);
 int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st);   // phi and E
 // direct solve (pic_fastpoisson.hip): set-up for the geometry masks, solve of
-// the current right-hand side into a full grid
+// the current right-hand side into the interior nodes of a grid (Dirichlet
+// values imposed by the caller)
 int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<double>& dval);
 void fast_free(pic_ctx* c);
 int fast_solve(pic_ctx* c, double* out);

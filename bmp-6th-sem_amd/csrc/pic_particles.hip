@@ -255,6 +255,41 @@ __global__ __launch_bounds__(kBlock) void k_deposit_finalize(
     }
 }
 
+// All species at once (at most kFinMax): rho = sum_s final_s * chrg_s in
+// species order (the sum a zeroed rho accumulates species by species).
+static constexpr int kFinMax = 4;
+struct FinalizeAll {
+    int ns;
+    const long long* cq[kFinMax];
+    const long long* ce[kFinMax];
+    double cqf[kFinMax], cef[kFinMax], chrg[kFinMax];
+    double* fm[kFinMax];
+    double* em[kFinMax];
+};
+
+__global__ __launch_bounds__(kBlock) void k_deposit_finalize_all(int G, int gp, FinalizeAll a, int energy,
+                                                                 double* __restrict__ rho)
+{
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= G) return;
+    double r = 0.0;
+#pragma unroll
+    for (int s = 0; s < kFinMax; ++s) {
+        if (s >= a.ns) break;
+        const long long* cq = a.cq[s];
+        long long q = cq[n] + cq[gp + n] + cq[2 * gp + n] + cq[3 * gp + n];
+        double f = (double)q * a.cqf[s];
+        a.fm[s][n] = f;
+        r += f * a.chrg[s];
+        if (energy) {
+            const long long* ce = a.ce[s];
+            long long e = ce[n] + ce[gp + n] + ce[2 * gp + n] + ce[3 * gp + n];
+            a.em[s][n] = (double)e * a.cef[s];
+        }
+    }
+    rho[n] = r;
+}
+
 // ================================================================= push
 template <int L, int GENB>
 __global__ __launch_bounds__(kBlock) void k_push(
@@ -481,7 +516,50 @@ static int finalize_deposit(pic_ctx* c, Species& sp) {
 // Charge (and energy) deposit of species s for its current particles:
 // per-cell corner sums on cell-sorted storage, per-particle atomics
 // otherwise, then node sums into final_mesh / energy_mesh and rho.
+static int deposit_corners(pic_ctx* c, int s);
+
 int launch_deposit(pic_ctx* c, int s) {
+    PIC_TRY(deposit_corners(c, s));
+    hipEvent_t ev;
+    PIC_TRY(timer_begin(c, PIC_STAGE_FINALIZE, &ev));
+    PIC_TRY(finalize_deposit(c, c->sp[s]));
+    PIC_TRY(timer_end(c, PIC_STAGE_FINALIZE, ev));
+    return PIC_OK;
+}
+
+// Every species' deposit, and rho written (not accumulated) by one
+// node pass over all of them.
+int launch_deposit_all(pic_ctx* c) {
+    const int ns = (int)c->sp.size();
+    if (ns == 0 || ns > kFinMax) {
+        PIC_TRY(launch_zero_f64(c, c->rho, c->G));
+        for (int s = 0; s < ns; ++s) PIC_TRY(launch_deposit(c, s));
+        return PIC_OK;
+    }
+    for (int s = 0; s < ns; ++s) PIC_TRY(deposit_corners(c, s));
+    hipEvent_t ev;
+    PIC_TRY(timer_begin(c, PIC_STAGE_FINALIZE, &ev));
+    FinalizeAll a{};
+    a.ns = ns;
+    const double dx = c->g.dx, dy = c->g.dy;
+    for (int s = 0; s < ns; ++s) {
+        Species& sp = c->sp[s];
+        a.cq[s] = (const long long*)sp.q_charge;
+        a.ce[s] = (const long long*)sp.q_energy;
+        a.cqf[s] = sp.W * (dx * dy) * std::ldexp(1.0, -c->p.charge_shift);
+        a.cef[s] = sp.W * (dx * dy) * std::ldexp(1.0, -c->p.energy_shift);
+        a.chrg[s] = -sp.charge / c->p.epsilon;
+        a.fm[s] = sp.final_mesh;
+        a.em[s] = sp.energy_mesh;
+    }
+    hipLaunchKernelGGL(k_deposit_finalize_all, dim3(blocks_for(c->G)), dim3(kBlock), 0, c->stream, c->G, c->g.gp, a,
+                       c->p.deposit_energy, c->rho);
+    PIC_HIP(hipGetLastError());
+    PIC_TRY(timer_end(c, PIC_STAGE_FINALIZE, ev));
+    return PIC_OK;
+}
+
+static int deposit_corners(pic_ctx* c, int s) {
     Species& sp = c->sp[s];
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_DEPOSIT, &ev));
@@ -523,9 +601,6 @@ int launch_deposit(pic_ctx* c, int s) {
         sp.deposit_pending = 1;
     }
     PIC_TRY(timer_end(c, PIC_STAGE_DEPOSIT, ev));
-    PIC_TRY(timer_begin(c, PIC_STAGE_FINALIZE, &ev));
-    PIC_TRY(finalize_deposit(c, sp));
-    PIC_TRY(timer_end(c, PIC_STAGE_FINALIZE, ev));
     return PIC_OK;
 }
 

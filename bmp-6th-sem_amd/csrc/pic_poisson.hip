@@ -1227,8 +1227,7 @@ static int vcycle_rest(pic_ctx* c, const T* r, float* z, const float* zold, int 
 }
 
 static int read_state(pic_ctx* c) {
-    PIC_HIP(hipMemcpyAsync(c->scal_host, c->scal, S_N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    PIC_HIP(hipMemcpyAsync(c->flags_host, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipMemcpyAsync(c->scal_host, c->scal, PIC_STATE_BYTES, hipMemcpyDeviceToHost, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }
@@ -1376,7 +1375,8 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     const bool direct = c->fast && !mixed && !learned && !c->guess_set;
     if (direct) {
         if (keep) PIC_HIP(hipMemcpyAsync(keep, c->phi, (size_t)G * sizeof(double), hipMemcpyDeviceToDevice, s));
-        PIC_TRY(fast_solve(c, c->phi));   // every node, Dirichlet values included
+        PIC_TRY(fast_solve(c, c->phi));   // the interior nodes
+        PIC_TRY(launch_overwrite_phi(c)); // the Dirichlet values
         guess = 5;
     } else {
         hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, src, prev, keep, c->dir,
@@ -1388,12 +1388,10 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     PIC_HIP(hipGetLastError());
     bool fields_done = false;
     if (direct) {
-        // The check's flags go to the host while the device already writes phi
-        // and E: the host waits for the flags only, not for the field kernels.
-        PIC_HIP(hipMemcpyAsync(c->scal_host, c->scal, S_N * sizeof(double), hipMemcpyDeviceToHost, s));
-        PIC_HIP(hipMemcpyAsync(c->flags_host, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+        // The check's flags go to the host while the device already writes E:
+        // the host waits for the flags only, not for the field kernel.
+        PIC_HIP(hipMemcpyAsync(c->scal_host, c->scal, PIC_STATE_BYTES, hipMemcpyDeviceToHost, s));
         PIC_HIP(hipEventRecord(t1, s));
-        PIC_TRY(launch_overwrite_phi(c));
         PIC_TRY(timer_end(c, PIC_STAGE_POISSON, ev));
         PIC_TRY(launch_efield(c));
         PIC_HIP(hipEventSynchronize(t1));
