@@ -71,6 +71,7 @@ struct EnsRun {
     double tol2;
     int max_it, pre, post, sweeps;
     int* warm;                    // 1 once phi holds a solution (device state)
+    int direct;                   // phi holds this step's direct solve (the PCG only checks it)
     int* iters;                   // per-run stats of the last solve
     double* relres;
 };
@@ -296,7 +297,7 @@ __global__ __launch_bounds__(1024) void k_ens_poisson(const EnsRun* __restrict__
     double* q = run.q;
     const uint8_t* dir = run.dir;
     const double cx = run.cx, cy = run.cy, dg = run.diag;
-    const int warm = *run.warm;
+    const int warm = *run.warm || run.direct;
     // rho[geom] = V; rho = -rho; x0 = warm ? phi : 0 with Dirichlet values
 #pragma unroll 4
     WG_FOR(n, G) {
@@ -516,6 +517,8 @@ struct pic_ensemble {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int energy = 1;
     int64_t steps = 0;
+    FdBatch* fdb = nullptr;                   // batched direct solves (null: PCG only)
+    std::vector<unsigned char> direct;        // per run: solved by fdb
 };
 
 namespace {
@@ -612,6 +615,7 @@ int fill_tables(pic_ensemble* e) {
             const MgLevel& last = c->mg.back();
             d.sweeps = std::max(50, 4 * std::max(last.gx, last.gy));
             d.warm = e->warm + r;
+            d.direct = e->direct.empty() ? 0 : e->direct[r];
             d.iters = e->iters + r;
             d.relres = e->relres + 2 * r;
         }
@@ -634,6 +638,7 @@ int enqueue_step(pic_ensemble* e, int par, hipStream_t s) {
                        e->queues + Q_DEP);
     hipLaunchKernelGGL(k_ens_finalize, dim3(e->blocks), dim3(kBlock), 0, s, tab, e->R, e->node_prefix,
                        e->queues + Q_FIN, e->energy);
+    if (e->fdb) PIC_TRY(fast_batch_enqueue(e->fdb, s));
     hipLaunchKernelGGL(k_ens_poisson, dim3(e->R), dim3(1024), 0, s, tab);
     hipLaunchKernelGGL(k_ens_efield, dim3(e->blocks), dim3(kBlock), 0, s, tab, e->R, e->node_prefix,
                        e->queues + Q_EF);
@@ -766,7 +771,10 @@ int pic_ensemble_create(pic_ctx** runs, int nruns, pic_ensemble** out) {
     }
     e->blocks = 4 * ncu;
     e->tile = 256;
-    int rc = build_prefix(e, e->tile);
+    e->direct.assign(R, 0);
+    int rc = fast_batch_create(runs, nruns, &e->fdb, e->direct.data());
+    if (rc != PIC_OK) return fail(rc);
+    rc = build_prefix(e, e->tile);
     if (rc == PIC_OK) rc = fill_tables(e);
     if (rc == PIC_OK) rc = capture(e);
     if (rc != PIC_OK) return fail(rc);
@@ -778,6 +786,7 @@ void pic_ensemble_destroy(pic_ensemble* e) {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     destroy_graphs(e);
+    fast_batch_free(e->fdb);
     void* ptrs[] = {e->table[0], e->table[1], e->dep_prefix, e->node_prefix, e->push_prefix, e->part_prefix,
                     e->queues, e->warm, e->iters, e->relres};
     for (void* p : ptrs)

@@ -30,6 +30,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <vector>
 
@@ -40,6 +41,8 @@ static constexpr int kFdMaxN = 2047;      // interior nodes per direction
 static constexpr int kFdB = 256;
 static constexpr int kLines = 16;         // covering lines handled by the line kernels (more: GEMMs)
 static constexpr int kSeg = 64;           // row segments of the line products
+
+struct FdRun;
 
 struct FastPoisson {
     int m = 0, n = 0, P = 0, nr = 0, nc = 0;
@@ -68,6 +71,7 @@ struct FastPoisson {
     int fft = 0, lm = 0, ln = 0;             // log2(2 (m + 1)), log2(2 (n + 1))
     double2 *twm = nullptr, *twn = nullptr;  // exp(-2 pi i t / M), t < M, for both lengths
     double* Fb = nullptr;                    // n x m: wall values moved to the right-hand side
+    FdRun* self = nullptr;                   // device: this run's descriptor (a table of one)
 };
 
 static void* dmalloc(size_t bytes, bool& ok) {
@@ -83,7 +87,7 @@ void fast_free(pic_ctx* c) {
     FastPoisson* f = c->fast;
     if (!f) return;
     void* ptrs[] = {f->Sm, f->Sn, f->Dinv, f->F, f->W, f->U, f->G, f->SnR, f->V1, f->SmC, f->V2, f->SnQ, f->cq,
-                    f->rq_ptr, f->rq_idx, f->rq_col, f->Z, f->Cinv, f->VQ, f->dq, f->qri, f->qcol, f->qrow, f->part, f->part2, f->twm, f->twn, f->Fb};
+                    f->rq_ptr, f->rq_idx, f->rq_col, f->Z, f->Cinv, f->VQ, f->dq, f->qri, f->qcol, f->qrow, f->part, f->part2, f->twm, f->twn, f->Fb, f->self};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (f->blas) (void)rocblas_destroy_handle(f->blas);
@@ -130,45 +134,69 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// ------------------------------------------------------ per-run descriptor
+// Everything the correction kernels and the FFT passes read of one run's
+// direct solve.  Kernels take a table of these and pick their run by the last
+// grid dimension, so one launch serves a single run (a table of one) or a
+// batch of runs with the same grid (an ensemble).
+struct FdRun {
+    int gx, m, n, P, nq;
+    int lr, lc;                              // covering rows / columns taken by k_fd_lines (0: GEMM or none)
+    double sgn;                              // +1: rho holds overwriteRho's result; -1: the raw deposit
+    const double* rho;
+    const uint8_t* dir;
+    double* phi;                             // output: the interior nodes
+    const double *Fb, *Dinv;
+    double *W, *U, *G;
+    const double *SnR, *SmC;
+    double *V1, *V2, *part, *part2;
+    const double *Sn, *Sm;
+    const int *qri, *qrow, *qcol;
+    const double* VQ;
+    double* dq;
+    const double* Cinv;
+    const int *rq_idx, *rq_col, *rq_ptr;
+    double *cq, *Z;
+    const double* SnQ;
+    const double2 *twm, *twn;
+};
+
 // d_q = V_q - u0(r, c) at q = (r, c), u0 = S_n Uhat S_m.  A q covered by row a
 // of the line cover: u0 = V1[a] . S_m[c]; by column j: u0 = S_n[r] . V2[:, j].
 // One wave per q.
-__global__ __launch_bounds__(kFdB) void k_fd_u0q(int P, int n, int m, const double* __restrict__ V1,
-                                                 const double* __restrict__ V2, const double* __restrict__ Sn,
-                                                 const double* __restrict__ Sm, const int* __restrict__ qri,
-                                                 const int* __restrict__ qrow, const int* __restrict__ qcol,
-                                                 const double* __restrict__ VQ, double* __restrict__ dq) {
+__global__ __launch_bounds__(kFdB) void k_fd_u0q(const FdRun* __restrict__ runs) {
+    const FdRun& f = runs[blockIdx.z];
     const int q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-    if (q >= P) return;
-    const int line = qri[q];
+    if (q >= f.P) return;
+    const int line = f.qri[q];
     const double *a, *b;
     int len;
     if (line >= 0) {
-        a = V1 + (size_t)line * m;
-        b = Sm + (size_t)qcol[q] * m;
-        len = m;
+        a = f.V1 + (size_t)line * f.m;
+        b = f.Sm + (size_t)f.qcol[q] * f.m;
+        len = f.m;
     } else {
-        a = V2 + (size_t)(-1 - line) * n;
-        b = Sn + (size_t)qrow[q] * n;
-        len = n;
+        a = f.V2 + (size_t)(-1 - line) * f.n;
+        b = f.Sn + (size_t)f.qrow[q] * f.n;
+        len = f.n;
     }
     double s = 0.0;
     for (int l = lane; l < len; l += 64) s += a[l] * b[l];
     s = wave_sum(s);
-    if (lane == 0) dq[q] = VQ[q] - s;
+    if (lane == 0) f.dq[q] = f.VQ[q] - s;
 }
 
-// Covering lines in one pass over Uhat (nr, nc <= kLines; either may be 0):
+// Covering lines in one pass over Uhat (lr, lc <= kLines; either may be 0):
 //   rows:    V1[a][l] = sum_k S_n[row a][k] Uhat[k][l]
 //   columns: V2[j][k] = sum_l Uhat[k][l] S_m[col j][l]
 // Workgroup (64 columns, one of kSeg row segments), four waves over
 // interleaved rows.  Row lines: per-segment partials of each column (part);
 // column lines: per-column-block partials of each row, one wave sum per row
 // (part2).  k_fd_linesum adds both in a fixed order.
-__global__ __launch_bounds__(kFdB) void k_fd_lines(int nr, int nc, int n, int m, const double* __restrict__ SnR,
-                                                   const double* __restrict__ SmC, const double* __restrict__ U,
-                                                   double* __restrict__ part, double* __restrict__ part2) {
+__global__ __launch_bounds__(kFdB) void k_fd_lines(const FdRun* __restrict__ runs) {
     __shared__ double sh[4][kLines][64];
+    const FdRun& f = runs[blockIdx.z];
+    const int nr = f.lr, nc = f.lc, n = f.n, m = f.m;
     const int lx = threadIdx.x & 63, ph = threadIdx.x >> 6;
     const int l = blockIdx.x * 64 + lx;
     const int per = (n + kSeg - 1) / kSeg;
@@ -177,18 +205,18 @@ __global__ __launch_bounds__(kFdB) void k_fd_lines(int nr, int nc, int n, int m,
 #pragma unroll
     for (int a = 0; a < kLines; ++a) {
         acc[a] = 0.0;
-        smc[a] = (a < nc && l < m) ? SmC[(size_t)a * m + l] : 0.0;
+        smc[a] = (a < nc && l < m) ? f.SmC[(size_t)a * m + l] : 0.0;
     }
     for (int k = k0 + ph; k < k1; k += 4) {
-        const double u = l < m ? U[(size_t)k * m + l] : 0.0;
+        const double u = l < m ? f.U[(size_t)k * m + l] : 0.0;
 #pragma unroll
         for (int a = 0; a < kLines; ++a)
-            if (a < nr) acc[a] += SnR[(size_t)a * n + k] * u;
+            if (a < nr) acc[a] += f.SnR[(size_t)a * n + k] * u;
 #pragma unroll
         for (int j = 0; j < kLines; ++j)
             if (j < nc) {
                 const double v = wave_sum(u * smc[j]);
-                if (lx == 0) part2[((size_t)blockIdx.x * nc + j) * n + k] = v;
+                if (lx == 0) f.part2[((size_t)blockIdx.x * nc + j) * n + k] = v;
             }
     }
     if (nr == 0) return;
@@ -197,66 +225,122 @@ __global__ __launch_bounds__(kFdB) void k_fd_lines(int nr, int nc, int n, int m,
     __syncthreads();
     if (ph == 0 && l < m)
         for (int a = 0; a < nr; ++a)
-            part[((size_t)blockIdx.y * nr + a) * m + l] = (sh[0][a][lx] + sh[1][a][lx]) + (sh[2][a][lx] + sh[3][a][lx]);
+            f.part[((size_t)blockIdx.y * nr + a) * m + l] = (sh[0][a][lx] + sh[1][a][lx]) + (sh[2][a][lx] + sh[3][a][lx]);
 }
 
-// V1 (nr x m) from the kSeg segment partials; V2 (nc x n) from the ncb
-// column-block partials
-__global__ __launch_bounds__(kFdB) void k_fd_linesum(int nr, int nc, int n, int m, int ncb,
-                                                     const double* __restrict__ part, const double* __restrict__ part2,
-                                                     double* __restrict__ V1, double* __restrict__ V2) {
+// V1 (lr x m) from the kSeg segment partials; V2 (lc x n) from the
+// ceil(m/64) column-block partials
+__global__ __launch_bounds__(kFdB) void k_fd_linesum(const FdRun* __restrict__ runs) {
+    const FdRun& f = runs[blockIdx.z];
+    const int nr = f.lr, nc = f.lc, n = f.n, m = f.m, ncb = (m + 63) / 64;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nr * m) {
         double s = 0.0;
-        for (int seg = 0; seg < kSeg; ++seg) s += part[(size_t)seg * nr * m + t];
-        V1[t] = s;
+        for (int seg = 0; seg < kSeg; ++seg) s += f.part[(size_t)seg * nr * m + t];
+        f.V1[t] = s;
     } else if (t < nr * m + nc * n) {
         const int u = t - nr * m;
         const int j = u / n, k = u - j * n;
         double s = 0.0;
-        for (int cb = 0; cb < ncb; ++cb) s += part2[((size_t)cb * nc + j) * n + k];
-        V2[(size_t)j * n + k] = s;
+        for (int cb = 0; cb < ncb; ++cb) s += f.part2[((size_t)cb * nc + j) * n + k];
+        f.V2[(size_t)j * n + k] = s;
     }
 }
 
 // The sources in row order: c_t = (C^-1 d)_q for q = rq_idx[t]; one wave per t
-__global__ __launch_bounds__(kFdB) void k_fd_sources(int P, const double* __restrict__ Cinv,
-                                                     const double* __restrict__ dq, const int* __restrict__ idx,
-                                                     double* __restrict__ ct) {
+__global__ __launch_bounds__(kFdB) void k_fd_sources(const FdRun* __restrict__ runs) {
+    const FdRun& f = runs[blockIdx.z];
+    const int P = f.P;
     const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
     if (t >= P) return;
-    const int q = idx[t];
+    const int q = f.rq_idx[t];
     double s = 0.0;
-    for (int p = lane; p < P; p += 64) s += Cinv[(size_t)q * P + p] * dq[p];
+    for (int p = lane; p < P; p += 64) s += f.Cinv[(size_t)q * P + p] * f.dq[p];
     s = wave_sum(s);
-    if (lane == 0) ct[t] = s;
+    if (lane == 0) f.cq[t] = s;
 }
 
 // Z[a][l] = sum over the sources t of distinct row a of c_t S_m[col t][l].
 // Workgroup (64 columns, row a): the row's (c_t, col t) staged in LDS, wave w
 // sums t = w, w + 4, ..., the four wave sums added in a fixed order.
-__global__ __launch_bounds__(kFdB) void k_fd_zrows(int m, const double* __restrict__ ct, const int* __restrict__ colt,
-                                                   const int* __restrict__ ptr, const double* __restrict__ Sm,
-                                                   double* __restrict__ Z) {
+__global__ __launch_bounds__(kFdB) void k_fd_zrows(const FdRun* __restrict__ runs) {
     __shared__ double cs[kFdMaxP];
     __shared__ int cl[kFdMaxP];
     __shared__ double part[4][64];
+    const FdRun& f = runs[blockIdx.z];
+    const int m = f.m;
     const int a = blockIdx.y;
-    const int t0 = ptr[a], nt = ptr[a + 1] - t0;
+    const int t0 = f.rq_ptr[a], nt = f.rq_ptr[a + 1] - t0;
     for (int t = threadIdx.x; t < nt; t += kFdB) {
-        cs[t] = ct[t0 + t];
-        cl[t] = colt[t0 + t];
+        cs[t] = f.cq[t0 + t];
+        cl[t] = f.rq_col[t0 + t];
     }
     __syncthreads();
     const int lx = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int l = min((int)blockIdx.x * 64 + lx, m - 1);
     double sum = 0.0;
 #pragma unroll 8
-    for (int t = w; t < nt; t += 4) sum += cs[t] * Sm[(size_t)cl[t] * m + l];
+    for (int t = w; t < nt; t += 4) sum += cs[t] * f.Sm[(size_t)cl[t] * m + l];
     part[w][lx] = sum;
     __syncthreads();
     if (w == 0 && (int)blockIdx.x * 64 + lx < m)
-        Z[(size_t)a * m + l] = (part[0][lx] + part[1][lx]) + (part[2][lx] + part[3][lx]);
+        f.Z[(size_t)a * m + l] = (part[0][lx] + part[1][lx]) + (part[2][lx] + part[3][lx]);
+}
+
+// The correction's spectrum G = SnQ Z (n x nq times nq x m, FP64): 64 x 64
+// output tiles, k in chunks of 16 staged in LDS, 4 x 4 outputs per thread
+// (rows ty + 16 i, columns tx + 16 j).
+static constexpr int kCT = 64, kCK = 16;
+__global__ __launch_bounds__(kFdB) void k_fd_corr(const FdRun* __restrict__ runs) {
+    __shared__ double As[kCK][kCT + 1];
+    __shared__ double Bs[kCK][kCT];
+    const FdRun& f = runs[blockIdx.z];
+    const int n = f.n, m = f.m, K = f.nq;
+    const int row0 = blockIdx.y * kCT, col0 = blockIdx.x * kCT;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    for (int k0 = 0; k0 < K; k0 += kCK) {
+        // A tile: rows row0 .. +63, k k0 .. +15 (transposed into As[k][row]);
+        // B tile: k k0 .. +15, columns col0 .. +63
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int idx = threadIdx.x + kFdB * e;
+            const int ar = idx >> 4, ak = idx & 15;
+            const int gr = row0 + ar, gk = k0 + ak;
+            As[ak][ar] = (gr < n && gk < K) ? f.SnQ[(size_t)gr * K + gk] : 0.0;
+            const int bk = idx >> 6, bc = idx & 63;
+            const int gk2 = k0 + bk, gc = col0 + bc;
+            Bs[bk][bc] = (gk2 < K && gc < m) ? f.Z[(size_t)gk2 * m + gc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kCK; ++kk) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) av[i] = As[kk][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = row0 + ty + 16 * i;
+        if (r >= n) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int cidx = col0 + tx + 16 * j;
+            if (cidx < m) f.G[(size_t)r * m + cidx] = acc[i][j];
+        }
+    }
 }
 
 // full-grid solution: interior unknowns from U, every Dirichlet node its value
@@ -317,26 +401,29 @@ struct DstLoadRows {
     __device__ double operator()(int row, int col) const { return X[(size_t)row * ld + col]; }
 };
 
-// rows of U + G .* Dinv (the corrected spectrum)
+// rows of U + G .* Dinv (the corrected spectrum; U alone without sources)
 struct DstLoadCorrected {
     const double *U, *G, *Dinv;
-    int ld;
+    int ld, corr;
     __device__ double operator()(int row, int col) const {
         const size_t i = (size_t)row * ld + col;
-        return U[i] + G[i] * Dinv[i];
+        return corr ? U[i] + G[i] * Dinv[i] : U[i];
     }
 };
 
 // the right-hand side F at interior node (r + 1, c + 1): rhs on unknowns plus
-// the wall terms Fb (fixed by the geometry), 0 on internal Dirichlet nodes
+// the wall terms Fb (fixed by the geometry), 0 on internal Dirichlet nodes.
+// rhs = sgn * rho: rho after overwriteRho (sgn 1) or the raw deposit (sgn -1;
+// overwriteRho negates the unknown nodes).
 struct DstLoadRhs {
     int gx, m;
-    const double* rhs;
+    double sgn;
+    const double* rho;
     const uint8_t* dir;
     const double* Fb;
     __device__ double operator()(int r, int q) const {
         const int node = (r + 1) * gx + q + 1;
-        const double v = rhs[node] + Fb[(size_t)r * m + q];
+        const double v = sgn * rho[node] + Fb[(size_t)r * m + q];
         return dir[node] ? 0.0 : v;
     }
 };
@@ -375,7 +462,8 @@ struct DstStoreGrid {
 // consecutive row pairs, so the transposed stores of one cache line meet in
 // one L2) transforms NB = max(1, 2048 / M) row pairs.
 template <int L, class Load, class Store>
-__global__ __launch_bounds__(kDstT) void k_fd_dst(int R, int nwg, Load ld, Store st, const double2* __restrict__ tw) {
+__device__ __forceinline__ void dst_body(int R, int nwg, const Load& ld, const Store& st,
+                                         const double2* __restrict__ tw) {
     constexpr int M = 1 << L;
     constexpr int N = M / 2 - 1;
     constexpr int NB = M >= kDstPts ? 1 : kDstPts / M;
@@ -502,29 +590,46 @@ __global__ __launch_bounds__(kDstT) void k_fd_dst(int R, int nwg, Load ld, Store
     }
 }
 
-template <int L, class Load, class Store>
-static void dst_pass_l(int R, const Load& ld, const Store& st, const double2* tw, hipStream_t s) {
+// The four passes of a direct solve, run blockIdx.y of the table:
+//   1: rows of F (rhs) -> W (m x n)          2: rows of W -> Uhat = . / Lambda
+//   3: rows of Uhat (+ G ./ Lambda) -> W      4: rows of W -> the grid's interior
+template <int L, int KIND>
+__global__ __launch_bounds__(kDstT) void k_fd_dst(const FdRun* __restrict__ runs, int R, int nwg) {
+    const FdRun& f = runs[blockIdx.y];
+    if constexpr (KIND == 1)
+        dst_body<L>(R, nwg, DstLoadRhs{f.gx, f.m, f.sgn, f.rho, f.dir, f.Fb}, DstStoreT{f.W, f.n}, f.twm);
+    else if constexpr (KIND == 2)
+        dst_body<L>(R, nwg, DstLoadRows{f.W, f.n}, DstStoreTScaled{f.U, f.Dinv, f.m}, f.twn);
+    else if constexpr (KIND == 3)
+        dst_body<L>(R, nwg, DstLoadCorrected{f.U, f.G, f.Dinv, f.m, f.P > 0}, DstStoreT{f.W, f.n}, f.twm);
+    else
+        dst_body<L>(R, nwg, DstLoadRows{f.W, f.n}, DstStoreGrid{f.phi, f.gx}, f.twn);
+}
+
+template <int L, int KIND>
+static void dst_pass_l(const FdRun* runs, int nruns, int R, hipStream_t s) {
     constexpr int M = 1 << L;
     constexpr int NB = M >= kDstPts ? 1 : kDstPts / M;
     const int nwg = (R + 2 * NB - 1) / (2 * NB);
     const int grid = (nwg + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_fd_dst<L, Load, Store>), dim3((unsigned)grid), dim3(kDstT), 0, s, R, nwg, ld, st, tw);
+    hipLaunchKernelGGL((k_fd_dst<L, KIND>), dim3((unsigned)grid, (unsigned)nruns), dim3(kDstT), 0, s, runs, R, nwg);
 }
 
-// DST-I of each of R rows of length 2^(L-1) - 1, transposed into `st`
-template <class Load, class Store>
-static void dst_pass(int L, int R, const Load& ld, const Store& st, const double2* tw, hipStream_t s) {
+// One pass over `nruns` runs of the table (same grid): DST-I of R rows of
+// length 2^(L-1) - 1 each
+template <int KIND>
+static void dst_pass(int L, const FdRun* runs, int nruns, int R, hipStream_t s) {
     switch (L) {
-    case 3: dst_pass_l<3>(R, ld, st, tw, s); break;
-    case 4: dst_pass_l<4>(R, ld, st, tw, s); break;
-    case 5: dst_pass_l<5>(R, ld, st, tw, s); break;
-    case 6: dst_pass_l<6>(R, ld, st, tw, s); break;
-    case 7: dst_pass_l<7>(R, ld, st, tw, s); break;
-    case 8: dst_pass_l<8>(R, ld, st, tw, s); break;
-    case 9: dst_pass_l<9>(R, ld, st, tw, s); break;
-    case 10: dst_pass_l<10>(R, ld, st, tw, s); break;
-    case 11: dst_pass_l<11>(R, ld, st, tw, s); break;
-    case 12: dst_pass_l<12>(R, ld, st, tw, s); break;
+    case 3: dst_pass_l<3, KIND>(runs, nruns, R, s); break;
+    case 4: dst_pass_l<4, KIND>(runs, nruns, R, s); break;
+    case 5: dst_pass_l<5, KIND>(runs, nruns, R, s); break;
+    case 6: dst_pass_l<6, KIND>(runs, nruns, R, s); break;
+    case 7: dst_pass_l<7, KIND>(runs, nruns, R, s); break;
+    case 8: dst_pass_l<8, KIND>(runs, nruns, R, s); break;
+    case 9: dst_pass_l<9, KIND>(runs, nruns, R, s); break;
+    case 10: dst_pass_l<10, KIND>(runs, nruns, R, s); break;
+    case 11: dst_pass_l<11, KIND>(runs, nruns, R, s); break;
+    case 12: dst_pass_l<12, KIND>(runs, nruns, R, s); break;
     default: break;   // fast_setup admits 3 <= L <= 12 only
     }
 }
@@ -595,6 +700,71 @@ static bool spd_inverse(std::vector<double>& C, int P) {
             C[(size_t)j * P + i] = s;
         }
     return true;
+}
+
+// The descriptor of context c's direct solve; sgn as FdRun::sgn.
+static FdRun run_of(const pic_ctx* c, double sgn) {
+    const FastPoisson* f = c->fast;
+    FdRun r{};
+    r.gx = c->g.gx;
+    r.m = f->m;
+    r.n = f->n;
+    r.P = f->P;
+    r.nq = f->nq;
+    r.lr = f->nr <= kLines ? f->nr : 0;
+    r.lc = f->nc <= kLines ? f->nc : 0;
+    r.sgn = sgn;
+    r.rho = c->rho;
+    r.dir = c->dir;
+    r.phi = c->phi;
+    r.Fb = f->Fb;
+    r.Dinv = f->Dinv;
+    r.W = f->W;
+    r.U = f->U;
+    r.G = f->G;
+    r.SnR = f->SnR;
+    r.SmC = f->SmC;
+    r.V1 = f->V1;
+    r.V2 = f->V2;
+    r.part = f->part;
+    r.part2 = f->part2;
+    r.Sn = f->Sn;
+    r.Sm = f->Sm;
+    r.qri = f->qri;
+    r.qrow = f->qrow;
+    r.qcol = f->qcol;
+    r.VQ = f->VQ;
+    r.dq = f->dq;
+    r.Cinv = f->Cinv;
+    r.rq_idx = f->rq_idx;
+    r.rq_col = f->rq_col;
+    r.rq_ptr = f->rq_ptr;
+    r.cq = f->cq;
+    r.Z = f->Z;
+    r.SnQ = f->SnQ;
+    r.twm = f->twm;
+    r.twn = f->twn;
+    return r;
+}
+
+// The electrode correction of `nruns` runs of the table with the dimensions
+// of f (V1/V2 lines beyond kLines are left to the caller): covering lines,
+// u0 on Q, sources, their rows and the spectrum G.
+static void enqueue_correction(const FastPoisson* f, const FdRun* tab, int nruns, hipStream_t s) {
+    const int m = f->m, n = f->n, P = f->P;
+    const int lr = f->nr <= kLines ? f->nr : 0, lc = f->nc <= kLines ? f->nc : 0;
+    const unsigned R = (unsigned)nruns;
+    if (lr + lc > 0) {
+        hipLaunchKernelGGL(k_fd_lines, dim3((unsigned)((m + 63) / 64), kSeg, R), dim3(kFdB), 0, s, tab);
+        hipLaunchKernelGGL(k_fd_linesum, dim3((unsigned)((lr * m + lc * n + kFdB - 1) / kFdB), 1, R), dim3(kFdB), 0,
+                           s, tab);
+    }
+    const unsigned wq = (unsigned)((P * 64 + kFdB - 1) / kFdB);
+    hipLaunchKernelGGL(k_fd_u0q, dim3(wq, 1, R), dim3(kFdB), 0, s, tab);
+    hipLaunchKernelGGL(k_fd_sources, dim3(wq, 1, R), dim3(kFdB), 0, s, tab);
+    hipLaunchKernelGGL(k_fd_zrows, dim3((unsigned)((m + 63) / 64), (unsigned)f->nq, R), dim3(kFdB), 0, s, tab);
+    hipLaunchKernelGGL(k_fd_corr, dim3((unsigned)((m + kCT - 1) / kCT), (unsigned)((n + kCT - 1) / kCT), R),
+                       dim3(kFdB), 0, s, tab);
 }
 
 // Set up the direct solve for the current geometry (`dir`: 1 = Dirichlet,
@@ -836,6 +1006,13 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
         PIC_HIP(hipMemcpy(f->Cinv, C.data(), (size_t)P * P * 8, hipMemcpyHostToDevice));
     }
     if (qrow_d) (void)hipFree(qrow_d);
+    const FdRun self = run_of(c, 1.0);
+    if (hipMalloc((void**)&f->self, sizeof(FdRun)) != hipSuccess) {
+        fast_free(c);
+        set_error("direct Poisson solve: device allocation failed");
+        return PIC_ERR_ALLOC;
+    }
+    PIC_HIP(hipMemcpy(f->self, &self, sizeof(FdRun), hipMemcpyHostToDevice));
     PIC_HIP(hipDeviceSynchronize());
     return PIC_OK;
 }
@@ -851,10 +1028,14 @@ int fast_solve(pic_ctx* c, double* out) {
     const double cx = 1.0 / (c->g.dx * c->g.dx), cy = 1.0 / (c->g.dy * c->g.dy);
     hipStream_t s = c->stream;
     const unsigned nb = (unsigned)((nm + kFdB - 1) / kFdB);
+    if (out != c->phi) {
+        set_error("direct Poisson solve: the output is the context's phi");
+        return PIC_ERR_STATE;
+    }
     if (f->fft) {
         // rows of F (n x m) -> W (m x n) -> Uhat = (S_n F S_m) ./ Lambda (n x m)
-        dst_pass(f->lm, n, DstLoadRhs{gx, m, c->rho, c->dir, f->Fb}, DstStoreT{f->W, n}, f->twm, s);
-        dst_pass(f->ln, m, DstLoadRows{f->W, n}, DstStoreTScaled{f->U, f->Dinv, m}, f->twn, s);
+        dst_pass<1>(f->lm, f->self, 1, n, s);
+        dst_pass<2>(f->ln, f->self, 1, m, s);
     } else {
         hipLaunchKernelGGL(k_fd_rhs, dim3(nb), dim3(kFdB), 0, s, gx, gy, cx, cy, c->rho, c->dir, c->dval, f->F);
         // row-major X (n x m) is the column-major m x n matrix X^T:
@@ -866,38 +1047,20 @@ int fast_solve(pic_ctx* c, double* out) {
     if (P > 0) {
         // u0 on Q through the covering lines: V1 = S_n[rows] Uhat (nr x m),
         // V1^T = Uhat^T S_n[rows]^T; V2 = Uhat S_m[cols]^T (n x nc, column-major)
-        // (a few lines: one pass over Uhat each; many lines: GEMMs)
-        const int lr = nr <= kLines ? nr : 0, lc = nc <= kLines ? nc : 0;
-        if (lr + lc > 0) {
-            const int ncb = (m + 63) / 64;
-            hipLaunchKernelGGL(k_fd_lines, dim3((unsigned)ncb, kSeg), dim3(kFdB), 0, s, lr, lc, n, m, f->SnR, f->SmC,
-                               f->U, f->part, f->part2);
-            hipLaunchKernelGGL(k_fd_linesum, dim3((unsigned)((lr * m + lc * n + kFdB - 1) / kFdB)), dim3(kFdB), 0, s,
-                               lr, lc, n, m, ncb, f->part, f->part2, f->V1, f->V2);
-        }
+        // (a few lines: one pass over Uhat; many lines: GEMMs)
         if (nr > kLines) PIC_TRY(gemm(f->blas, m, nr, n, f->U, m, f->SnR, n, 0.0, f->V1, m));
         if (nc > kLines) {
             const double one = 1.0, zero = 0.0;
             FD_BLAS(rocblas_dgemm(f->blas, rocblas_operation_transpose, rocblas_operation_none, n, nc, m, &one, f->U,
                                   m, f->SmC, m, &zero, f->V2, n));
         }
-        hipLaunchKernelGGL(k_fd_u0q, dim3((unsigned)((P * 64 + kFdB - 1) / kFdB)), dim3(kFdB), 0, s, P, n, m, f->V1,
-                           f->V2, f->Sn, f->Sm, f->qri, f->qrow, f->qcol, f->VQ, f->dq);
-        hipLaunchKernelGGL(k_fd_sources, dim3((unsigned)((P * 64 + kFdB - 1) / kFdB)), dim3(kFdB), 0, s, P, f->Cinv,
-                           f->dq, f->rq_idx, f->cq);
-        hipLaunchKernelGGL(k_fd_zrows, dim3((unsigned)((m + 63) / 64), (unsigned)f->nq), dim3(kFdB), 0, s, m, f->cq,
-                           f->rq_col, f->rq_ptr, f->Sm, f->Z);
-        // G = S_n[:, rows of Q] Z (n x nq x m) -> G^T = Z^T SnQ^T
-        PIC_TRY(gemm(f->blas, m, n, f->nq, f->Z, m, f->SnQ, f->nq, 0.0, f->G, m));
+        enqueue_correction(f, f->self, 1, s);
         if (!f->fft) hipLaunchKernelGGL(k_fd_addscaled, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->G, f->Dinv);
     }
     if (f->fft) {
         // rows of Uhat (+ G ./ Lambda) -> W (m x n) -> the grid's interior nodes
-        if (P > 0)
-            dst_pass(f->lm, n, DstLoadCorrected{f->U, f->G, f->Dinv, m}, DstStoreT{f->W, n}, f->twm, s);
-        else
-            dst_pass(f->lm, n, DstLoadRows{f->U, m}, DstStoreT{f->W, n}, f->twm, s);
-        dst_pass(f->ln, m, DstLoadRows{f->W, n}, DstStoreGrid{out, gx}, f->twn, s);
+        dst_pass<3>(f->lm, f->self, 1, n, s);
+        dst_pass<4>(f->ln, f->self, 1, m, s);
         PIC_HIP(hipGetLastError());
         return PIC_OK;
     }
@@ -906,6 +1069,78 @@ int fast_solve(pic_ctx* c, double* out) {
     PIC_TRY(gemm(f->blas, m, n, n, f->W, m, f->Sn, n, 0.0, f->G, m));
     hipLaunchKernelGGL(k_fd_out, dim3((unsigned)((gx * gy + kFdB - 1) / kFdB)), dim3(kFdB), 0, s, gx, gy, f->G,
                        c->dir, c->dval, out);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+// ------------------------------------------------------------ batched solves
+// Direct solves of many runs at once (an ensemble): runs whose solves have the
+// same shape (grid, |Q|, covering lines, rows of Q) form a group with one
+// device table; each pass is one launch per group.  The runs read their raw
+// deposit (sgn -1), so the batch goes before the ensemble's overwriteRho.
+struct FdBatch {
+    struct Group {
+        const FastPoisson* f;                // dimensions (any member)
+        FdRun* tab;                          // device
+        int nruns;
+    };
+    std::vector<Group> groups;
+};
+
+void fast_batch_free(FdBatch* b) {
+    if (!b) return;
+    for (auto& g : b->groups)
+        if (g.tab) (void)hipFree(g.tab);
+    delete b;
+}
+
+int fast_batch_create(pic_ctx* const* runs, int n, FdBatch** out, unsigned char* used) {
+    *out = nullptr;
+    FdBatch* b = new FdBatch();
+    std::vector<std::vector<int>> members;
+    std::vector<std::array<int, 10>> keys;
+    for (int r = 0; r < n; ++r) {
+        used[r] = 0;
+        const pic_ctx* c = runs[r];
+        const FastPoisson* f = c->fast;
+        if (!f || !f->fft || f->nr > kLines || f->nc > kLines) continue;
+        const std::array<int, 10> k{c->g.gx, c->g.gy, f->m, f->n, f->P, f->nq, f->nr, f->nc, f->lm, f->ln};
+        size_t gi = 0;
+        while (gi < keys.size() && keys[gi] != k) ++gi;
+        if (gi == keys.size()) {
+            keys.push_back(k);
+            members.emplace_back();
+        }
+        members[gi].push_back(r);
+        used[r] = 1;
+    }
+    for (size_t gi = 0; gi < keys.size(); ++gi) {
+        std::vector<FdRun> t;
+        for (int r : members[gi]) t.push_back(run_of(runs[r], -1.0));
+        FdBatch::Group g{runs[members[gi][0]]->fast, nullptr, (int)t.size()};
+        if (hipMalloc((void**)&g.tab, t.size() * sizeof(FdRun)) != hipSuccess ||
+            hipMemcpy(g.tab, t.data(), t.size() * sizeof(FdRun), hipMemcpyHostToDevice) != hipSuccess) {
+            if (g.tab) (void)hipFree(g.tab);
+            fast_batch_free(b);
+            set_error("batched direct Poisson solve: table allocation failed");
+            return PIC_ERR_ALLOC;
+        }
+        b->groups.push_back(g);
+    }
+    *out = b;
+    return PIC_OK;
+}
+
+// Every run of the batch: phi's interior nodes from its raw deposit.
+int fast_batch_enqueue(const FdBatch* b, hipStream_t s) {
+    for (const auto& g : b->groups) {
+        const FastPoisson* f = g.f;
+        dst_pass<1>(f->lm, g.tab, g.nruns, f->n, s);
+        dst_pass<2>(f->ln, g.tab, g.nruns, f->m, s);
+        if (f->P > 0) enqueue_correction(f, g.tab, g.nruns, s);
+        dst_pass<3>(f->lm, g.tab, g.nruns, f->n, s);
+        dst_pass<4>(f->ln, g.tab, g.nruns, f->m, s);
+    }
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }

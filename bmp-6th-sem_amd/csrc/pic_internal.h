@@ -228,6 +228,12 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st);   // phi and 
 int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<double>& dval);
 void fast_free(pic_ctx* c);
 int fast_solve(pic_ctx* c, double* out);
+// direct solves of many contexts at once (from their raw deposits); used[r] = 1
+// for the contexts the batch solves
+struct FdBatch;
+int fast_batch_create(pic_ctx* const* runs, int n, FdBatch** out, unsigned char* used);
+int fast_batch_enqueue(const FdBatch* b, hipStream_t s);
+void fast_batch_free(FdBatch* b);
 // weights over the diagonal of the two pre-smoothing sweeps of a level (the
 // post-smoothing uses them in reverse order)
 void mg_sweep_weights(const pic_ctx* c, const MgLevel& m, float* w0, float* w1);
