@@ -29,6 +29,7 @@
 namespace pic {
 
 static constexpr int kEnsMaxLev = 12;
+static constexpr int kEnsChunk = 8;          // scatter item: kEnsChunk x 256 particles
 
 struct EnsLevel {
     int gx, gy;
@@ -415,6 +416,7 @@ __global__ __launch_bounds__(kBlock) void k_ens_push(const EnsRun* __restrict__ 
     __shared__ int hist[kHR * (TC + 2)];
     __shared__ int codes[kCodeCap];
     __shared__ unsigned long long nabs[2];
+    __shared__ double wf[6 * (TC + 1)];
     __shared__ int slot;
     const int per_species = prefix[R];
     while (true) {
@@ -425,8 +427,8 @@ __global__ __launch_bounds__(kBlock) void k_ens_push(const EnsRun* __restrict__ 
         const EnsRun& run = runs[r];
         const EnsSpecies& sp = run.sp[s];
         const int tpr = (run.g.gx + TC - 1) / TC;
-        push_tile_body<PIC_LAYOUT_CELL32, 0, TC>(sp.src, run.g, sp.sd, tpr, k - prefix[r], run.fld, sp.start,
-                                                 sp.counts, sp.rank, sp.absorbed, hist, codes, nabs);
+        push_tile_body<PIC_LAYOUT_CELL32, 0, TC, true>(sp.src, run.g, sp.sd, tpr, k - prefix[r], run.fld, sp.start,
+                                                       sp.counts, sp.rank, sp.absorbed, hist, codes, nabs, wf);
     }
 }
 
@@ -468,7 +470,8 @@ __global__ __launch_bounds__(1024) void k_ens_scan(const EnsRun* __restrict__ ru
     }
 }
 
-// items: (species, run, chunk of 256 particles of the source storage)
+// items: (species, run, chunk of kEnsChunk x 256 particles of the source
+// storage); each thread takes every 256th particle of the chunk
 __global__ __launch_bounds__(kBlock) void k_ens_scatter(const EnsRun* __restrict__ runs, int R,
                                                         const int* __restrict__ prefix, unsigned* queue) {
     __shared__ int slot;
@@ -480,16 +483,21 @@ __global__ __launch_bounds__(kBlock) void k_ens_scatter(const EnsRun* __restrict
         const int r = find_run(prefix, R, k);
         const EnsRun& run = runs[r];
         const EnsSpecies& sp = run.sp[s];
-        const int i = (k - prefix[r]) * kBlock + threadIdx.x;
-        if (i >= *sp.nsrc) continue;
-        const PState<PIC_LAYOUT_CELL32> p = load_state<PIC_LAYOUT_CELL32>(sp.src, i);
-        const int rk = sp.rank[i];
-        const Corners c = gather<PIC_LAYOUT_CELL32>(p, run.g, run.fld);
-        const Pushed<PIC_LAYOUT_CELL32> o = push_state<PIC_LAYOUT_CELL32, 0>(p, c, run.g, sp.sd);
-        if (o.key < 0) continue;
-        const int d = sp.start[o.key] + rk;
-        store_state<PIC_LAYOUT_CELL32>(sp.dst, d, o);
-        sp.dst.cell[d] = o.key;
+        const int n = *sp.nsrc;
+        const int i0 = (k - prefix[r]) * (kBlock * kEnsChunk) + threadIdx.x;
+#pragma unroll 2
+        for (int j = 0; j < kEnsChunk; ++j) {
+            const int i = i0 + j * kBlock;
+            if (i >= n) break;
+            const PState<PIC_LAYOUT_CELL32> p = load_state<PIC_LAYOUT_CELL32>(sp.src, i);
+            const int rk = sp.rank[i];
+            const Corners c = gather<PIC_LAYOUT_CELL32>(p, run.g, run.fld);
+            const Pushed<PIC_LAYOUT_CELL32> o = push_state<PIC_LAYOUT_CELL32, 0>(p, c, run.g, sp.sd);
+            if (o.key < 0) continue;
+            const int d = sp.start[o.key] + rk;
+            store_state<PIC_LAYOUT_CELL32>(sp.dst, d, o);
+            sp.dst.cell[d] = o.key;
+        }
     }
 }
 
@@ -746,7 +754,7 @@ int pic_ensemble_create(pic_ctx** runs, int nruns, pic_ensemble** out) {
         np[r + 1] = np[r] + (c->G + kBlock - 1) / kBlock;
         int64_t cap = 0;
         for (auto& sp : c->sp) cap = std::max(cap, sp.n_upper);
-        pp[r + 1] = pp[r] + (int)((cap + kBlock - 1) / kBlock);
+        pp[r + 1] = pp[r] + (int)((cap + kBlock * kEnsChunk - 1) / (kBlock * kEnsChunk));
         warm[r] = c->have_prev_phi ? 1 : 0;
         for (int k = 0; k < 3; ++k) {
             double* w = nullptr;

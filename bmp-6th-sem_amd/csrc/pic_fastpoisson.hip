@@ -72,6 +72,7 @@ struct FastPoisson {
     double2 *twm = nullptr, *twn = nullptr;  // exp(-2 pi i t / M), t < M, for both lengths
     double* Fb = nullptr;                    // n x m: wall values moved to the right-hand side
     FdRun* self = nullptr;                   // device: this run's descriptor (a table of one)
+    FdRun* hself = nullptr;                  // host copy
 };
 
 static void* dmalloc(size_t bytes, bool& ok) {
@@ -91,6 +92,7 @@ void fast_free(pic_ctx* c) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (f->blas) (void)rocblas_destroy_handle(f->blas);
+    delete f->hself;
     delete f;
     c->fast = nullptr;
 }
@@ -164,8 +166,9 @@ struct FdRun {
 // d_q = V_q - u0(r, c) at q = (r, c), u0 = S_n Uhat S_m.  A q covered by row a
 // of the line cover: u0 = V1[a] . S_m[c]; by column j: u0 = S_n[r] . V2[:, j].
 // One wave per q.
-__global__ __launch_bounds__(kFdB) void k_fd_u0q(const FdRun* __restrict__ runs) {
-    const FdRun& f = runs[blockIdx.z];
+template <bool B>
+__global__ __launch_bounds__(kFdB) void k_fd_u0q(const FdRun one, const FdRun* __restrict__ runs) {
+    const FdRun& f = B ? runs[blockIdx.z] : one;
     const int q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
     if (q >= f.P) return;
     const int line = f.qri[q];
@@ -193,9 +196,10 @@ __global__ __launch_bounds__(kFdB) void k_fd_u0q(const FdRun* __restrict__ runs)
 // interleaved rows.  Row lines: per-segment partials of each column (part);
 // column lines: per-column-block partials of each row, one wave sum per row
 // (part2).  k_fd_linesum adds both in a fixed order.
-__global__ __launch_bounds__(kFdB) void k_fd_lines(const FdRun* __restrict__ runs) {
+template <bool B>
+__global__ __launch_bounds__(kFdB) void k_fd_lines(const FdRun one, const FdRun* __restrict__ runs) {
     __shared__ double sh[4][kLines][64];
-    const FdRun& f = runs[blockIdx.z];
+    const FdRun& f = B ? runs[blockIdx.z] : one;
     const int nr = f.lr, nc = f.lc, n = f.n, m = f.m;
     const int lx = threadIdx.x & 63, ph = threadIdx.x >> 6;
     const int l = blockIdx.x * 64 + lx;
@@ -230,8 +234,9 @@ __global__ __launch_bounds__(kFdB) void k_fd_lines(const FdRun* __restrict__ run
 
 // V1 (lr x m) from the kSeg segment partials; V2 (lc x n) from the
 // ceil(m/64) column-block partials
-__global__ __launch_bounds__(kFdB) void k_fd_linesum(const FdRun* __restrict__ runs) {
-    const FdRun& f = runs[blockIdx.z];
+template <bool B>
+__global__ __launch_bounds__(kFdB) void k_fd_linesum(const FdRun one, const FdRun* __restrict__ runs) {
+    const FdRun& f = B ? runs[blockIdx.z] : one;
     const int nr = f.lr, nc = f.lc, n = f.n, m = f.m, ncb = (m + 63) / 64;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nr * m) {
@@ -248,8 +253,9 @@ __global__ __launch_bounds__(kFdB) void k_fd_linesum(const FdRun* __restrict__ r
 }
 
 // The sources in row order: c_t = (C^-1 d)_q for q = rq_idx[t]; one wave per t
-__global__ __launch_bounds__(kFdB) void k_fd_sources(const FdRun* __restrict__ runs) {
-    const FdRun& f = runs[blockIdx.z];
+template <bool B>
+__global__ __launch_bounds__(kFdB) void k_fd_sources(const FdRun one, const FdRun* __restrict__ runs) {
+    const FdRun& f = B ? runs[blockIdx.z] : one;
     const int P = f.P;
     const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
     if (t >= P) return;
@@ -263,11 +269,12 @@ __global__ __launch_bounds__(kFdB) void k_fd_sources(const FdRun* __restrict__ r
 // Z[a][l] = sum over the sources t of distinct row a of c_t S_m[col t][l].
 // Workgroup (64 columns, row a): the row's (c_t, col t) staged in LDS, wave w
 // sums t = w, w + 4, ..., the four wave sums added in a fixed order.
-__global__ __launch_bounds__(kFdB) void k_fd_zrows(const FdRun* __restrict__ runs) {
+template <bool B>
+__global__ __launch_bounds__(kFdB) void k_fd_zrows(const FdRun one, const FdRun* __restrict__ runs) {
     __shared__ double cs[kFdMaxP];
     __shared__ int cl[kFdMaxP];
     __shared__ double part[4][64];
-    const FdRun& f = runs[blockIdx.z];
+    const FdRun& f = B ? runs[blockIdx.z] : one;
     const int m = f.m;
     const int a = blockIdx.y;
     const int t0 = f.rq_ptr[a], nt = f.rq_ptr[a + 1] - t0;
@@ -291,10 +298,11 @@ __global__ __launch_bounds__(kFdB) void k_fd_zrows(const FdRun* __restrict__ run
 // output tiles, k in chunks of 16 staged in LDS, 4 x 4 outputs per thread
 // (rows ty + 16 i, columns tx + 16 j).
 static constexpr int kCT = 64, kCK = 16;
-__global__ __launch_bounds__(kFdB) void k_fd_corr(const FdRun* __restrict__ runs) {
+template <bool B>
+__global__ __launch_bounds__(kFdB) void k_fd_corr(const FdRun one, const FdRun* __restrict__ runs) {
     __shared__ double As[kCK][kCT + 1];
     __shared__ double Bs[kCK][kCT];
-    const FdRun& f = runs[blockIdx.z];
+    const FdRun& f = B ? runs[blockIdx.z] : one;
     const int n = f.n, m = f.m, K = f.nq;
     const int row0 = blockIdx.y * kCT, col0 = blockIdx.x * kCT;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
@@ -593,9 +601,9 @@ __device__ __forceinline__ void dst_body(int R, int nwg, const Load& ld, const S
 // The four passes of a direct solve, run blockIdx.y of the table:
 //   1: rows of F (rhs) -> W (m x n)          2: rows of W -> Uhat = . / Lambda
 //   3: rows of Uhat (+ G ./ Lambda) -> W      4: rows of W -> the grid's interior
-template <int L, int KIND>
-__global__ __launch_bounds__(kDstT) void k_fd_dst(const FdRun* __restrict__ runs, int R, int nwg) {
-    const FdRun& f = runs[blockIdx.y];
+template <int L, int KIND, bool B>
+__global__ __launch_bounds__(kDstT) void k_fd_dst(const FdRun one, const FdRun* __restrict__ runs, int R, int nwg) {
+    const FdRun& f = B ? runs[blockIdx.y] : one;
     if constexpr (KIND == 1)
         dst_body<L>(R, nwg, DstLoadRhs{f.gx, f.m, f.sgn, f.rho, f.dir, f.Fb}, DstStoreT{f.W, f.n}, f.twm);
     else if constexpr (KIND == 2)
@@ -606,30 +614,42 @@ __global__ __launch_bounds__(kDstT) void k_fd_dst(const FdRun* __restrict__ runs
         dst_body<L>(R, nwg, DstLoadRows{f.W, f.n}, DstStoreGrid{f.phi, f.gx}, f.twn);
 }
 
+// Runs of a launch: one run's descriptor by value (kernel argument), or a
+// device table of `n` runs indexed by the last grid dimension.
+struct FdRuns {
+    const FdRun* one;
+    const FdRun* tab;
+    int n;
+};
+
 template <int L, int KIND>
-static void dst_pass_l(const FdRun* runs, int nruns, int R, hipStream_t s) {
+static void dst_pass_l(const FdRuns& r, int R, hipStream_t s) {
     constexpr int M = 1 << L;
     constexpr int NB = M >= kDstPts ? 1 : kDstPts / M;
     const int nwg = (R + 2 * NB - 1) / (2 * NB);
     const int grid = (nwg + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_fd_dst<L, KIND>), dim3((unsigned)grid, (unsigned)nruns), dim3(kDstT), 0, s, runs, R, nwg);
+    if (r.tab)
+        hipLaunchKernelGGL((k_fd_dst<L, KIND, true>), dim3((unsigned)grid, (unsigned)r.n), dim3(kDstT), 0, s, FdRun{},
+                           r.tab, R, nwg);
+    else
+        hipLaunchKernelGGL((k_fd_dst<L, KIND, false>), dim3((unsigned)grid), dim3(kDstT), 0, s, *r.one,
+                           (const FdRun*)nullptr, R, nwg);
 }
 
-// One pass over `nruns` runs of the table (same grid): DST-I of R rows of
-// length 2^(L-1) - 1 each
+// One pass over the runs (same grid): DST-I of R rows of length 2^(L-1) - 1 each
 template <int KIND>
-static void dst_pass(int L, const FdRun* runs, int nruns, int R, hipStream_t s) {
+static void dst_pass(int L, const FdRuns& r, int R, hipStream_t s) {
     switch (L) {
-    case 3: dst_pass_l<3, KIND>(runs, nruns, R, s); break;
-    case 4: dst_pass_l<4, KIND>(runs, nruns, R, s); break;
-    case 5: dst_pass_l<5, KIND>(runs, nruns, R, s); break;
-    case 6: dst_pass_l<6, KIND>(runs, nruns, R, s); break;
-    case 7: dst_pass_l<7, KIND>(runs, nruns, R, s); break;
-    case 8: dst_pass_l<8, KIND>(runs, nruns, R, s); break;
-    case 9: dst_pass_l<9, KIND>(runs, nruns, R, s); break;
-    case 10: dst_pass_l<10, KIND>(runs, nruns, R, s); break;
-    case 11: dst_pass_l<11, KIND>(runs, nruns, R, s); break;
-    case 12: dst_pass_l<12, KIND>(runs, nruns, R, s); break;
+    case 3: dst_pass_l<3, KIND>(r, R, s); break;
+    case 4: dst_pass_l<4, KIND>(r, R, s); break;
+    case 5: dst_pass_l<5, KIND>(r, R, s); break;
+    case 6: dst_pass_l<6, KIND>(r, R, s); break;
+    case 7: dst_pass_l<7, KIND>(r, R, s); break;
+    case 8: dst_pass_l<8, KIND>(r, R, s); break;
+    case 9: dst_pass_l<9, KIND>(r, R, s); break;
+    case 10: dst_pass_l<10, KIND>(r, R, s); break;
+    case 11: dst_pass_l<11, KIND>(r, R, s); break;
+    case 12: dst_pass_l<12, KIND>(r, R, s); break;
     default: break;   // fast_setup admits 3 <= L <= 12 only
     }
 }
@@ -747,24 +767,35 @@ static FdRun run_of(const pic_ctx* c, double sgn) {
     return r;
 }
 
-// The electrode correction of `nruns` runs of the table with the dimensions
-// of f (V1/V2 lines beyond kLines are left to the caller): covering lines,
-// u0 on Q, sources, their rows and the spectrum G.
-static void enqueue_correction(const FastPoisson* f, const FdRun* tab, int nruns, hipStream_t s) {
+// The electrode correction of the runs (dimensions of f; V1/V2 lines beyond
+// kLines are left to the caller): covering lines, u0 on Q, sources, their
+// rows and the spectrum G (a single run: rocBLAS; a batch: k_fd_corr, so a
+// captured graph holds kernels only).
+template <bool B>
+static void launch_fd(void (*k)(const FdRun, const FdRun*), dim3 g, const FdRuns& r, hipStream_t s) {
+    if (B) hipLaunchKernelGGL(k, dim3(g.x, g.y, (unsigned)r.n), dim3(kFdB), 0, s, FdRun{}, r.tab);
+    else hipLaunchKernelGGL(k, g, dim3(kFdB), 0, s, *r.one, (const FdRun*)nullptr);
+}
+
+template <bool B>
+static int enqueue_correction(FastPoisson* f, const FdRuns& r, hipStream_t s) {
     const int m = f->m, n = f->n, P = f->P;
     const int lr = f->nr <= kLines ? f->nr : 0, lc = f->nc <= kLines ? f->nc : 0;
-    const unsigned R = (unsigned)nruns;
     if (lr + lc > 0) {
-        hipLaunchKernelGGL(k_fd_lines, dim3((unsigned)((m + 63) / 64), kSeg, R), dim3(kFdB), 0, s, tab);
-        hipLaunchKernelGGL(k_fd_linesum, dim3((unsigned)((lr * m + lc * n + kFdB - 1) / kFdB), 1, R), dim3(kFdB), 0,
-                           s, tab);
+        launch_fd<B>(k_fd_lines<B>, dim3((unsigned)((m + 63) / 64), kSeg), r, s);
+        launch_fd<B>(k_fd_linesum<B>, dim3((unsigned)((lr * m + lc * n + kFdB - 1) / kFdB)), r, s);
     }
     const unsigned wq = (unsigned)((P * 64 + kFdB - 1) / kFdB);
-    hipLaunchKernelGGL(k_fd_u0q, dim3(wq, 1, R), dim3(kFdB), 0, s, tab);
-    hipLaunchKernelGGL(k_fd_sources, dim3(wq, 1, R), dim3(kFdB), 0, s, tab);
-    hipLaunchKernelGGL(k_fd_zrows, dim3((unsigned)((m + 63) / 64), (unsigned)f->nq, R), dim3(kFdB), 0, s, tab);
-    hipLaunchKernelGGL(k_fd_corr, dim3((unsigned)((m + kCT - 1) / kCT), (unsigned)((n + kCT - 1) / kCT), R),
-                       dim3(kFdB), 0, s, tab);
+    launch_fd<B>(k_fd_u0q<B>, dim3(wq), r, s);
+    launch_fd<B>(k_fd_sources<B>, dim3(wq), r, s);
+    launch_fd<B>(k_fd_zrows<B>, dim3((unsigned)((m + 63) / 64), (unsigned)f->nq), r, s);
+    if (B) {
+        launch_fd<B>(k_fd_corr<B>, dim3((unsigned)((m + kCT - 1) / kCT), (unsigned)((n + kCT - 1) / kCT)), r, s);
+    } else {
+        // G = S_n[:, rows of Q] Z (n x nq x m) -> G^T = Z^T SnQ^T
+        PIC_TRY(gemm(f->blas, m, n, f->nq, f->Z, m, f->SnQ, f->nq, 0.0, f->G, m));
+    }
+    return PIC_OK;
 }
 
 // Set up the direct solve for the current geometry (`dir`: 1 = Dirichlet,
@@ -1013,6 +1044,7 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
         return PIC_ERR_ALLOC;
     }
     PIC_HIP(hipMemcpy(f->self, &self, sizeof(FdRun), hipMemcpyHostToDevice));
+    f->hself = new FdRun(self);
     PIC_HIP(hipDeviceSynchronize());
     return PIC_OK;
 }
@@ -1032,10 +1064,11 @@ int fast_solve(pic_ctx* c, double* out) {
         set_error("direct Poisson solve: the output is the context's phi");
         return PIC_ERR_STATE;
     }
+    const FdRuns one{f->hself, nullptr, 1};
     if (f->fft) {
         // rows of F (n x m) -> W (m x n) -> Uhat = (S_n F S_m) ./ Lambda (n x m)
-        dst_pass<1>(f->lm, f->self, 1, n, s);
-        dst_pass<2>(f->ln, f->self, 1, m, s);
+        dst_pass<1>(f->lm, one, n, s);
+        dst_pass<2>(f->ln, one, m, s);
     } else {
         hipLaunchKernelGGL(k_fd_rhs, dim3(nb), dim3(kFdB), 0, s, gx, gy, cx, cy, c->rho, c->dir, c->dval, f->F);
         // row-major X (n x m) is the column-major m x n matrix X^T:
@@ -1054,13 +1087,13 @@ int fast_solve(pic_ctx* c, double* out) {
             FD_BLAS(rocblas_dgemm(f->blas, rocblas_operation_transpose, rocblas_operation_none, n, nc, m, &one, f->U,
                                   m, f->SmC, m, &zero, f->V2, n));
         }
-        enqueue_correction(f, f->self, 1, s);
+        PIC_TRY(enqueue_correction<false>(f, one, s));
         if (!f->fft) hipLaunchKernelGGL(k_fd_addscaled, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->G, f->Dinv);
     }
     if (f->fft) {
         // rows of Uhat (+ G ./ Lambda) -> W (m x n) -> the grid's interior nodes
-        dst_pass<3>(f->lm, f->self, 1, n, s);
-        dst_pass<4>(f->ln, f->self, 1, m, s);
+        dst_pass<3>(f->lm, one, n, s);
+        dst_pass<4>(f->ln, one, m, s);
         PIC_HIP(hipGetLastError());
         return PIC_OK;
     }
@@ -1080,7 +1113,7 @@ int fast_solve(pic_ctx* c, double* out) {
 // deposit (sgn -1), so the batch goes before the ensemble's overwriteRho.
 struct FdBatch {
     struct Group {
-        const FastPoisson* f;                // dimensions (any member)
+        FastPoisson* f;                      // dimensions (any member)
         FdRun* tab;                          // device
         int nruns;
     };
@@ -1134,12 +1167,13 @@ int fast_batch_create(pic_ctx* const* runs, int n, FdBatch** out, unsigned char*
 // Every run of the batch: phi's interior nodes from its raw deposit.
 int fast_batch_enqueue(const FdBatch* b, hipStream_t s) {
     for (const auto& g : b->groups) {
-        const FastPoisson* f = g.f;
-        dst_pass<1>(f->lm, g.tab, g.nruns, f->n, s);
-        dst_pass<2>(f->ln, g.tab, g.nruns, f->m, s);
-        if (f->P > 0) enqueue_correction(f, g.tab, g.nruns, s);
-        dst_pass<3>(f->lm, g.tab, g.nruns, f->n, s);
-        dst_pass<4>(f->ln, g.tab, g.nruns, f->m, s);
+        FastPoisson* f = g.f;
+        const FdRuns r{nullptr, g.tab, g.nruns};
+        dst_pass<1>(f->lm, r, f->n, s);
+        dst_pass<2>(f->ln, r, f->m, s);
+        if (f->P > 0) PIC_TRY(enqueue_correction<true>(f, r, s));
+        dst_pass<3>(f->lm, r, f->n, s);
+        dst_pass<4>(f->ln, r, f->m, s);
     }
     PIC_HIP(hipGetLastError());
     return PIC_OK;
