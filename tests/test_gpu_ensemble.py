@@ -1,8 +1,10 @@
 """Batched ensemble of uneven runs (persistent kernels, one captured HIP graph
 per step) against serial runs of the same contexts: rho bit-identical on the
-first step (same particles, integer deposit), phi to 1e-9 relative (two PCG
-implementations converged to 1e-12), live particle counts within 2; graph
-replay identical to direct launches; the autotuner's choice is valid."""
+first step (same particles, integer deposit), phi to 1e-9 relative (batched
+direct solves, or the ensemble's PCG converged to 1e-12, against the serial
+solves), live particle counts within 2; graph replay identical to direct
+launches; the autotuner's choice is valid; mixed ensembles (two batched
+groups, a GEMM-transform grid and a PCG-only run)."""
 import numpy as np
 import pytest
 
@@ -76,4 +78,54 @@ def test_ensemble_graph_equals_direct_launches_and_tuner(hip_mod):
     ea.close()
     eb.close()
     for c in a_ctx + b_ctx:
+        c.close()
+
+
+def test_ensemble_mixed_direct_batches_and_pcg_runs(hip_mod):
+    """Runs of one ensemble solved different ways: batched direct solves in two
+    groups of the same grid (the default electrode and a second geometry), a
+    96-cell grid (sine transforms by GEMM: not batched, the ensemble's PCG
+    solves it) and a run with the direct solve turned off.  Every run against
+    its own serial steps, as above; the batched runs need no PCG iteration."""
+    import pic_configs as cfg
+    hip = hip_mod
+    runs = cfg.ensemble(nruns=6, seed=12, grids=(64, 96), n_min=5e3, n_max=4e4)
+    runs[0] = cfg.Workload("mix0", 64, 64, 20000, 12000, 0.03, 0.0, cfg.ELECTRODE, electrode_v=50.0, layout=1)
+    runs[1] = cfg.Workload("mix1", 64, 64, 15000, 12001, 0.02, 0.0, cfg.ELECTRODE, electrode_v=80.0, layout=1)
+    runs[2] = cfg.Workload("mix2", 64, 64, 12000, 12002, 0.01, 0.0, cfg.ELECTRODE, electrode_v=120.0, layout=1)
+    runs[3] = cfg.Workload("mix3", 96, 96, 12000, 12003, 0.01, 0.0, cfg.ELECTRODE, electrode_v=60.0, layout=1)
+    other = [[0, 0, 0, 1, 0.0], [1, 0, 1, 1, 0.0], [0, 1, 1, 1, 0.0], [0, 0, 1, 0, 0.0],
+             [0.2, 0.6, 0.3, 0.8, -40.0]]
+
+    def build():
+        out = []
+        for k, w in enumerate(runs):
+            parts = [cfg.generate(w, s) for s in range(2)]
+            kw = {"poisson_direct": 0} if k == 2 else {}
+            c = cfg.make_context(w, layout=1, particles=parts, poisson_tol=1e-12, **kw)
+            if k == 1:
+                c.set_geometry(other)
+            out.append(c)
+        return out
+
+    ens_ctx, ser_ctx = build(), build()
+    ens = hip.Ensemble(ens_ctx)
+    for it in range(2):
+        ens.step(1, graph=True)
+        for c in ser_ctx:
+            c.step(it)
+        iters, rr = ens.poisson_stats()
+        assert (rr[:, 1] <= 1e-12).all(), rr
+        assert iters[0] == 0 and iters[1] == 0, iters        # batched direct solves
+        if it == 0:
+            assert iters[2] > 0, iters                        # PCG from zero
+        for k, (a, b) in enumerate(zip(ens_ctx, ser_ctx)):
+            pa, pb = a.grid(hip.GRID_PHI), b.grid(hip.GRID_PHI)
+            if it == 0:
+                assert np.array_equal(a.grid(hip.GRID_RHO), b.grid(hip.GRID_RHO)), (it, k)
+            assert np.abs(pa - pb).max() <= 1e-9 * np.abs(pb).max(), (it, k)
+            for s in range(2):
+                assert abs(a.count(s) - b.count(s)) <= 2, (it, k, s)
+    ens.close()
+    for c in ens_ctx + ser_ctx:
         c.close()
