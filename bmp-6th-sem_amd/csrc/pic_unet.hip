@@ -122,6 +122,90 @@ __global__ __launch_bounds__(kT) void k_conv3x3(const bf16* __restrict__ X, int 
     }
 }
 
+// The same convolution with the operands staged in LDS: a workgroup computes
+// a tile of kCH rows x 64 pixels (wave w: row y0 + w) for all NT*16 output
+// channels.  Per 16-channel input chunk the padded input tile (kCH + 2 rows x
+// 66 pixels) and the chunk's weights [9][cout][16] are copied to LDS with
+// 16-byte loads (each input value leaves HBM once per tile instead of once
+// per tap), then every MFMA operand is an 8-byte LDS read: A = 16 pixels x
+// 4 channels per lane group at [row][pixel][channel], B = 16 output channels
+// x 4 input channels at [tap][co][channel] (both 512 contiguous bytes per
+// wave, no bank conflicts).  H must be a multiple of kCH.  Small images split
+// the output channels over blockIdx.y (coutT in all, NT*16 per workgroup) so
+// that the launch still fills the GPU.
+static constexpr int kCH = 4;
+static constexpr int kCPW = 66;   // tile width with the halo
+
+template <int NT, int RELU, typename TO>
+__global__ __launch_bounds__(kT) void k_conv3x3_lds(const bf16* __restrict__ X, int xC, int xo, int cin,
+                                                    const bf16* __restrict__ Wt, const float* __restrict__ bias,
+                                                    TO* __restrict__ Y, int yC, int yo, int H, int W, int coutT) {
+    constexpr int cout = NT * 16;
+    const int cob = (int)blockIdx.y * cout;
+    __shared__ s4v xin[(kCH + 2) * kCPW * 4];
+    __shared__ s4v wsh[9 * cout * 4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nxt = (W + 63) >> 6;
+    const int y0 = (int)(blockIdx.x / nxt) * kCH, x0 = (int)(blockIdx.x % nxt) * 64;
+    const int i = lane & 15, kq = lane >> 4;
+    const int Wp = W + 2;
+    f4v acc[4][NT];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][t] = (f4v){0.f, 0.f, 0.f, 0.f};
+    for (int cc = 0; cc < cin; cc += 16) {
+        __syncthreads();
+        // padded input rows y0 .. y0 + kCH + 1, padded columns x0 .. x0 + 65
+        for (int e = threadIdx.x; e < (kCH + 2) * kCPW * 2; e += kT) {
+            const int pix = e >> 1, half = e & 1;
+            const int r = pix / kCPW, pc = pix - r * kCPW;
+            const int gxp = x0 + pc;
+            int4 v = make_int4(0, 0, 0, 0);
+            if (gxp <= W + 1) v = *(const int4*)(X + ((size_t)(y0 + r) * Wp + gxp) * xC + xo + cc + 8 * half);
+            ((int4*)xin)[e] = v;
+        }
+        for (int e = threadIdx.x; e < 9 * cout * 2; e += kT) {
+            const int tc = e >> 1, half = e & 1;
+            const int tap = tc / cout, col = tc - tap * cout;
+            ((int4*)wsh)[e] = *(const int4*)(Wt + ((size_t)tap * coutT + cob + col) * cin + cc + 8 * half);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+            s4v a[4], b[NT];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) a[m] = xin[((wv + 1 + dy) * kCPW + 16 * m + i + 1 + dx) * 4 + kq];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) b[t] = wsh[(tap * cout + 16 * t + i) * 4 + kq];
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b[t], acc[m][t], 0, 0, 0);
+        }
+    }
+    const int y = y0 + wv;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = cob + t * 16 + i;
+        const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int x = x0 + m * 16 + 4 * kq + e;
+                if (x >= W) continue;
+                float v = acc[m][t][e] + bb;
+                if (RELU) v = fmaxf(v, 0.f);
+                TO* dst = Y + ((size_t)(y + 1) * Wp + x + 1) * yC + yo + co;
+                if constexpr (sizeof(TO) == 2) *dst = f2bf(v);
+                else *dst = v;
+            }
+    }
+}
+
 // dW[tap][co][ci] += sum_{y,x} dZT[co][y][x] XT[ci][y+1+dy][x+1+dx]
 // dZT: [cout][H][W] (unpadded), XT: [cin][H+2][W+2] (padded).  One wave: one
 // tap, one 16-channel co tile, CT ci tiles, rows [r0, r0 + rows); FP32 atomics
@@ -231,17 +315,26 @@ __global__ void k_to_chw(const bf16* __restrict__ in, int C, int o, int c, int H
 }
 
 // 2x2 average pooling: out (H/2 x W/2) <- in (H x W), c channels.
+// Eight channels (16 bytes) per thread; c, io and oo are multiples of 8.
 __global__ void k_pool2(const bf16* __restrict__ in, int iC, int io, bf16* __restrict__ out, int oC, int oo, int c,
                         int H, int W) {
-    const int h = H / 2, w = W / 2;
+    const int h = H / 2, w = W / 2, c8 = c / 8;
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= (int64_t)h * w * c) return;
-    const int ch = (int)(n % c);
-    const int64_t p = n / c;
+    if (n >= (int64_t)h * w * c8) return;
+    const int ch = (int)(n % c8) * 8;
+    const int64_t p = n / c8;
     const int y = (int)(p / w), x = (int)(p - (int64_t)y * w);
-    auto at = [&](int yy, int xx) { return bf2f(in[((size_t)(yy + 1) * (W + 2) + xx + 1) * iC + io + ch]); };
-    const float s = (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1)) + (at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1));
-    out[((size_t)(y + 1) * (w + 2) + x + 1) * oC + oo + ch] = f2bf(0.25f * s);
+    auto at = [&](int yy, int xx) { return *(const int4*)(in + ((size_t)(yy + 1) * (W + 2) + xx + 1) * iC + io + ch); };
+    const int4 a = at(2 * y, 2 * x), b = at(2 * y, 2 * x + 1), c2 = at(2 * y + 1, 2 * x), d = at(2 * y + 1, 2 * x + 1);
+    const bf16 *pa = (const bf16*)&a, *pb = (const bf16*)&b, *pc = (const bf16*)&c2, *pd = (const bf16*)&d;
+    int4 o;
+    bf16* po = (bf16*)&o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float s = (bf2f(pa[k]) + bf2f(pb[k])) + (bf2f(pc[k]) + bf2f(pd[k]));
+        po[k] = f2bf(0.25f * s);
+    }
+    *(int4*)(out + ((size_t)(y + 1) * (w + 2) + x + 1) * oC + oo + ch) = o;
 }
 
 // gin (H x W) += 0.25 gout (H/2 x W/2) at each of the four children.
@@ -258,15 +351,17 @@ __global__ void k_pool2_bwd(const float* __restrict__ gout, int oC, int oo, floa
 }
 
 // nearest upsampling: out (2h x 2w) <- in (h x w)
+// Eight channels (16 bytes) per thread; c, io and oo are multiples of 8.
 __global__ void k_up2(const bf16* __restrict__ in, int iC, int io, bf16* __restrict__ out, int oC, int oo, int c,
                       int h, int w) {
-    const int H = 2 * h, W = 2 * w;
+    const int H = 2 * h, W = 2 * w, c8 = c / 8;
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= (int64_t)H * W * c) return;
-    const int ch = (int)(n % c);
-    const int64_t p = n / c;
+    if (n >= (int64_t)H * W * c8) return;
+    const int ch = (int)(n % c8) * 8;
+    const int64_t p = n / c8;
     const int y = (int)(p / W), x = (int)(p - (int64_t)y * W);
-    out[((size_t)(y + 1) * (W + 2) + x + 1) * oC + oo + ch] = in[((size_t)(y / 2 + 1) * (w + 2) + x / 2 + 1) * iC + io + ch];
+    *(int4*)(out + ((size_t)(y + 1) * (W + 2) + x + 1) * oC + oo + ch) =
+        *(const int4*)(in + ((size_t)(y / 2 + 1) * (w + 2) + x / 2 + 1) * iC + io + ch);
 }
 
 // gin (h x w) = sum of gout over the four children (2h x 2w)
@@ -289,9 +384,14 @@ __global__ void k_head(const bf16* __restrict__ d0, int c, const float* __restri
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= H * W) return;
     const int y = p / W, x = p - y * W;
-    const bf16* v = d0 + ((size_t)(y + 1) * (W + 2) + x + 1) * c;
+    const int4* v4 = (const int4*)(d0 + ((size_t)(y + 1) * (W + 2) + x + 1) * c);
     float s = bh[0];
-    for (int k = 0; k < c; ++k) s += wh[k] * bf2f(v[k]);
+    for (int k8 = 0; k8 < c / 8; ++k8) {
+        const int4 q = v4[k8];
+        const bf16* v = (const bf16*)&q;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += wh[8 * k8 + k] * bf2f(v[k]);
+    }
     out[p] = s;
 }
 
@@ -658,6 +758,16 @@ enum { X0 = 0, A0, CAT0, P0, A1, CAT1, P1, A2, CAT2, P2, A3, B3, A4, D2, A5, D1,
 template <int NT, int RELU, typename TO>
 static void conv_launch(hipStream_t s, const bf16* X, int xC, int xo, int cin, const bf16* Wt, const float* bias,
                         TO* Y, int yC, int yo, int H, int W) {
+    if (H % kCH == 0) {
+        const unsigned tiles = (unsigned)((H / kCH) * ((W + 63) / 64));
+        if (NT > 1 && tiles < 1024)
+            hipLaunchKernelGGL((k_conv3x3_lds<1, RELU, TO>), dim3(tiles, NT), dim3(kT), 0, s, X, xC, xo, cin, Wt, bias,
+                               Y, yC, yo, H, W, NT * 16);
+        else
+            hipLaunchKernelGGL((k_conv3x3_lds<NT, RELU, TO>), dim3(tiles), dim3(kT), 0, s, X, xC, xo, cin, Wt, bias, Y,
+                               yC, yo, H, W, NT * 16);
+        return;
+    }
     const int waves = H * ((W + 63) / 64);
     hipLaunchKernelGGL((k_conv3x3<NT, RELU, TO>), dim3((unsigned)((waves + 3) / 4)), dim3(kT), 0, s, X, xC, xo, cin,
                        Wt, bias, Y, yC, yo, H, W);
@@ -703,10 +813,10 @@ static int resample(pic_unet* u, bool up, int in, int io, int out, int oo, int c
     const Tensor& a = u->t[in];
     const Tensor& b = u->t[out];
     if (!up) {
-        const int64_t n = (int64_t)b.H * b.W * c;
+        const int64_t n = (int64_t)b.H * b.W * (c / 8);
         hipLaunchKernelGGL(k_pool2, dim3(nb(n)), dim3(kT), 0, u->stream, a.p, a.C, io, b.p, b.C, oo, c, a.H, a.W);
     } else {
-        const int64_t n = (int64_t)b.H * b.W * c;
+        const int64_t n = (int64_t)b.H * b.W * (c / 8);
         hipLaunchKernelGGL(k_up2, dim3(nb(n)), dim3(kT), 0, u->stream, a.p, a.C, io, b.p, b.C, oo, c, a.H, a.W);
     }
     PIC_HIP(hipGetLastError());
