@@ -15,6 +15,7 @@
 #include "pic_internal.h"
 #include "pic_unet.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -254,27 +255,36 @@ __global__ __launch_bounds__(kT) void k_conv_wgrad(const bf16* __restrict__ dZT,
         }
 }
 
-// db[co] += sum_p dZ[p][co]; dZ padded NHWC with cout channels (cout divides
-// 256).  A block takes `chunk` pixels; thread t reads channel t % cout of
-// pixels t / cout, t / cout + 256 / cout, ... (each group of cout threads reads
-// one pixel's channels, contiguous); per-channel block sums go to one atomic each.
-__global__ __launch_bounds__(kT) void k_bias_grad(const bf16* __restrict__ dZ, int cout, int H, int W, int chunk,
+// db[co] += sum_p dZ[p][co]; dZ padded NHWC with cout channels (8 <= cout <=
+// 256, a multiple of 8).  Thread t takes the 8-channel slice t % (cout/8) of
+// pixels t / (cout/8), + kT / (cout/8), ... of a grid-stride walk (16-byte
+// loads); the block adds its per-thread sums per channel through LDS and
+// makes one atomic per channel.
+__global__ __launch_bounds__(kT) void k_bias_grad(const bf16* __restrict__ dZ, int cout, int H, int W,
                                                   float* __restrict__ db) {
-    __shared__ float sh[kT];
-    const int co = threadIdx.x % cout, po = threadIdx.x / cout, step = kT / cout;
+    __shared__ float sh[kT][9];
+    const int c8 = cout / 8, per = kT / c8;
+    const int sl = threadIdx.x % c8, q0 = threadIdx.x / c8;
     const int HW = H * W;
-    const int p0 = blockIdx.x * chunk, p1 = min(HW, p0 + chunk);
-    float s = 0.f;
-    for (int p = p0 + po; p < p1; p += step) {
-        const int y = p / W, x = p - y * W;
-        s += bf2f(dZ[((size_t)(y + 1) * (W + 2) + x + 1) * cout + co]);
-    }
-    sh[threadIdx.x] = s;
+    float acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+    if (q0 < per)
+        for (int p = blockIdx.x * per + q0; p < HW; p += gridDim.x * per) {
+            const int y = p / W, x = p - y * W;
+            const int4 v = *(const int4*)(dZ + ((size_t)(y + 1) * (W + 2) + x + 1) * cout + 8 * sl);
+            const bf16* b = (const bf16*)&v;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc[k] += bf2f(b[k]);
+        }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sh[threadIdx.x][k] = acc[k];
     __syncthreads();
     if ((int)threadIdx.x < cout) {
+        const int ch = threadIdx.x, s8 = ch >> 3, k = ch & 7;
         float t = 0.f;
-        for (int k = 0; k < step; ++k) t += sh[k * cout + threadIdx.x];
-        atomicAdd(db + threadIdx.x, t);
+        for (int q = 0; q < per; ++q) t += sh[q * c8 + s8][k];
+        atomicAdd(db + ch, t);
     }
 }
 
@@ -300,18 +310,34 @@ __global__ void k_relu_bwd(const float* __restrict__ dY, int gC, int go, const b
 }
 
 // out[c][y][x] = in[(y * (W + 2) + x) * C + o + c] over the padded grid (pad = 1)
-// or the interior only (pad = 0, out [c][H][W]).
-__global__ void k_to_chw(const bf16* __restrict__ in, int C, int o, int c, int H, int W, int pad,
-                         bf16* __restrict__ out) {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// or the interior only (pad = 0, out [c][H][W]).  Workgroup: 64 output
+// columns of one output row, 16 channels (blockIdx.z); the tile goes through
+// LDS so both the NHWC reads (16 bytes per pixel) and the CHW writes (64
+// consecutive pixels of a channel) are contiguous.  c, o, C multiples of 8.
+__global__ __launch_bounds__(kT) void k_to_chw(const bf16* __restrict__ in, int C, int o, int c, int H, int W, int pad,
+                                               bf16* __restrict__ out) {
+    __shared__ bf16 tile[16][66];
     const int Ho = H + 2 * pad, Wo = W + 2 * pad;
-    const int64_t tot = (int64_t)c * Ho * Wo;
-    if (n >= tot) return;
-    const int x = (int)(n % Wo);
-    const int64_t r = n / Wo;
-    const int y = (int)(r % Ho), ch = (int)(r / Ho);
-    const int yy = y + 1 - pad, xx = x + 1 - pad;
-    out[n] = in[((size_t)yy * (W + 2) + xx) * C + o + ch];
+    const int x0 = blockIdx.x * 64, y = blockIdx.y, c0 = blockIdx.z * 16;
+    const int t = threadIdx.x;
+    if (t < 128) {
+        const int px = t >> 1, half = t & 1;
+        const int x = x0 + px;
+        int4 v = make_int4(0, 0, 0, 0);
+        if (x < Wo && c0 + 8 * half < c)
+            v = *(const int4*)(in + ((size_t)(y + 1 - pad) * (W + 2) + x + 1 - pad) * C + o + c0 + 8 * half);
+        const bf16* b = (const bf16*)&v;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) tile[8 * half + k][px] = b[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int idx = t + kT * k;
+        const int ch = idx >> 6, px = idx & 63;
+        const int x = x0 + px;
+        if (x < Wo && c0 + ch < c) out[((size_t)(c0 + ch) * Ho + y) * Wo + x] = tile[ch][px];
+    }
 }
 
 // 2x2 average pooling: out (H/2 x W/2) <- in (H x W), c channels.
@@ -870,15 +896,14 @@ static int conv_bwd(pic_unet* u, int k) {
     hipStream_t s = u->stream;
     const int64_t n = (int64_t)(H + 2) * (W + 2) * c.cout;
     hipLaunchKernelGGL(k_relu_bwd, dim3(nb(n)), dim3(kT), 0, s, b.g, b.C, c.oo, b.p, b.C, c.oo, u->dz, c.cout, H, W);
-    const int64_t nt = (int64_t)c.cout * H * W;
-    hipLaunchKernelGGL(k_to_chw, dim3(nb(nt)), dim3(kT), 0, s, u->dz, c.cout, 0, c.cout, H, W, 0, u->dzt);
-    const int64_t nx = (int64_t)c.cin * (H + 2) * (W + 2);
-    hipLaunchKernelGGL(k_to_chw, dim3(nb(nx)), dim3(kT), 0, s, a.p, a.C, c.io, c.cin, H, W, 1, u->xt);
+    hipLaunchKernelGGL(k_to_chw, dim3((unsigned)((W + 63) / 64), (unsigned)H, (unsigned)((c.cout + 15) / 16)), dim3(kT),
+                       0, s, u->dz, c.cout, 0, c.cout, H, W, 0, u->dzt);
+    hipLaunchKernelGGL(k_to_chw, dim3((unsigned)((W + 2 + 63) / 64), (unsigned)(H + 2), (unsigned)((c.cin + 15) / 16)),
+                       dim3(kT), 0, s, a.p, a.C, c.io, c.cin, H, W, 1, u->xt);
     PIC_HIP(hipGetLastError());
     PIC_TRY(wgrad_any(s, c.cin / 16, u->dzt, u->xt, c.cout, c.cin, H, W, u->g + c.woff));
-    const int chunk = 4096;
-    const int parts = (H * W + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_bias_grad, dim3(parts), dim3(kT), 0, s, u->dz, c.cout, H, W, chunk, u->g + c.boff);
+    const int parts = std::min(512, (H * W + 4095) / 4096);
+    hipLaunchKernelGGL(k_bias_grad, dim3(parts), dim3(kT), 0, s, u->dz, c.cout, H, W, u->g + c.boff);
     PIC_HIP(hipGetLastError());
     if (k > 0)
         PIC_TRY((conv_any<0, float>(s, c.cin / 16, u->dz, c.cout, 0, c.cout, u->wflip + c.woff, nullptr, a.g, a.C,
