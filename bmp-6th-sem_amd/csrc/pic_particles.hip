@@ -347,7 +347,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter_push(
     const double4* __restrict__ fld, const int32_t* __restrict__ rank, const int32_t* __restrict__ start)
 {
     const int n = *nsrc;
-    const int i0 = blockIdx.x * blockDim.x;
+    // consecutive particle blocks on one XCD: the partial cache lines two
+    // neighbouring blocks write at their boundary meet in one L2
+    const int i0 = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
     if (i0 >= n) return;
     const int i = i0 + threadIdx.x;
     const bool have = i < n;
