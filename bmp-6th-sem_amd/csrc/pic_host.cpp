@@ -720,16 +720,27 @@ extern "C" int pic_step(pic_ctx* c, int iteration, pic_solve_stats* st) {
         return PIC_ERR_ARG;
     }
     PIC_TRY(launch_deposit_all(c));
-    PIC_TRY(poisson_solve(c, iteration, st));   // also computes E
+    // Poisson and E; a direct solve's check is still in flight: the first
+    // push is enqueued behind it (it does nothing unless the check passed), so
+    // the device has work while the host waits for the check
+    const int ns = (int)c->sp.size();
+    PIC_TRY(poisson_solve(c, iteration, st, 1));
+    const bool early = c->tail.active && ns > 0;
+    if (early) PIC_TRY(move_push(c, 0, c->flags));
+    int redo = 0;
+    PIC_TRY(poisson_finish(c, st, &redo));
+    if (early && redo) PIC_TRY(move_push(c, 0));   // the PCG corrected phi and E: push again
     if (!c->inj.on) {
-        for (int s = 0; s < (int)c->sp.size(); ++s) PIC_TRY(launch_move(c, s));
+        for (int s = 0; s < ns; ++s) {
+            if (s == 0 && early) PIC_TRY(move_finish(c, 0));
+            else PIC_TRY(launch_move(c, s));
+        }
         return PIC_OK;
     }
     // with injection (main.cpp:232-245): push every species, stage the new
     // particles (their count depends on both species' exits), count them into
     // the cells, then scan and scatter pushed and new particles together
-    const int ns = (int)c->sp.size();
-    for (int s = 0; s < ns; ++s) PIC_TRY(move_push(c, s));
+    for (int s = early ? 1 : 0; s < ns; ++s) PIC_TRY(move_push(c, s));
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_BIN, &ev));
     PIC_TRY(inject_generate(c));

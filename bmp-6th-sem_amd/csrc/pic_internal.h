@@ -116,6 +116,15 @@ struct Geometry {
 
 struct FastPoisson;   // direct Poisson solve (pic_fastpoisson.hip)
 
+// A Poisson solve whose end (the wait for its check, statistics) is deferred
+// so the caller can enqueue work behind it first.
+struct SolveTail {
+    int active = 0;
+    bool direct = false, mixed = false, learned = false, extrap_keep = false;
+    int guess = 0;
+    hipEvent_t ev = nullptr;        // stage timer start
+};
+
 // Particle injection state (pic_inject.hip).
 struct Injection {
     int on = 0;
@@ -190,6 +199,7 @@ struct pic_ctx {
     pic::StageTimer timers[PIC_NSTAGES];
     std::vector<hipEvent_t> event_pool;   // recorded-and-read stage events, reused
     hipEvent_t solve_ev[2] = {nullptr, nullptr};   // start / end of a Poisson solve
+    pic::SolveTail tail;                             // a deferred solve end
 };
 
 namespace pic {
@@ -202,7 +212,9 @@ int launch_deposit(pic_ctx* c, int s);            // adds species s to rho
 int launch_deposit_all(pic_ctx* c);               // every species; writes rho
 int launch_push(pic_ctx* c, int s);
 int launch_move(pic_ctx* c, int s);
-int move_push(pic_ctx* c, int s);                 // launch_move in phases, for injection between them
+// launch_move in phases, for injection between them; with `guard`, the push
+// does nothing unless guard[0] == 1 when it runs (a Poisson check passed)
+int move_push(pic_ctx* c, int s, const int* guard = nullptr);
 int move_finish(pic_ctx* c, int s);
 int launch_zero_f64(pic_ctx* c, double* p, int64_t n);
 
@@ -221,7 +233,11 @@ int launch_kinetic_energy(pic_ctx* c, int s, double* out);   // host result, syn
 // Poisson (pic_poisson.hip)
 int poisson_setup(pic_ctx* c);
 void poisson_free(pic_ctx* c);
-int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st);   // phi and E
+// phi and E.  defer: a direct solve returns with its check in flight (E
+// enqueued); poisson_finish waits for it, runs PCG iterations if the check
+// failed (*redo = 1: phi and E were recomputed) and fills st.
+int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer = 0);
+int poisson_finish(pic_ctx* c, pic_solve_stats* st, int* redo);
 // direct solve (pic_fastpoisson.hip): set-up for the geometry masks, solve of
 // the current right-hand side into the interior nodes of a grid (Dirichlet
 // values imposed by the caller)

@@ -314,8 +314,10 @@ template <int L, int GENB>
 __global__ __launch_bounds__(kBlock) void k_push_tile(
     ParticleBuf b, GridDev g, SpeciesDev s, int tiles_per_row, int ntiles, const double4* __restrict__ fld,
     const int32_t* __restrict__ start, int32_t* __restrict__ counts, int32_t* __restrict__ rank,
-    unsigned long long* __restrict__ absorbed)
+    unsigned long long* __restrict__ absorbed, const int* __restrict__ guard)
 {
+    // a push enqueued behind a Poisson check that has not passed does nothing
+    if (guard && guard[0] != 1) return;
     __shared__ int hist[kHR * (kTileC + 2)];
     __shared__ int codes[kCodeCap];
     __shared__ unsigned long long nabs[2];
@@ -639,9 +641,10 @@ int launch_push(pic_ctx* c, int s) {
 
 template <int L, int GENB>
 static void tile_kernel(pic_ctx* c, Species& sp, const SpeciesDev& d, int tpr, unsigned ntiles,
-                        const hipEvent_t kt[2]) {
+                        const hipEvent_t kt[2], const int* guard) {
     launch_timed(k_push_tile<L, GENB>, dim3(ntiles), dim3(kBlock), c->stream, kt, sp.buf[sp.cur], c->g, d, tpr,
-                 (int)ntiles, (const double4*)c->fld, (const int32_t*)sp.start, sp.counts, sp.rank, sp.absorbed);
+                 (int)ntiles, (const double4*)c->fld, (const int32_t*)sp.start, sp.counts, sp.rank, sp.absorbed,
+                 guard);
 }
 
 template <int L, int GENB>
@@ -655,7 +658,7 @@ static void scatter_kernel(pic_ctx* c, Species& sp, const SpeciesDev& d, int src
 // prefix scan and recomputing scatter into the other buffer (move_finish).
 // Injection runs between the two phases: its particles are counted into the
 // same cells before the scan.
-int move_push(pic_ctx* c, int s) {
+int move_push(pic_ctx* c, int s, const int* guard) {
     PIC_TRY(inject_settle(c));
     Species& sp = c->sp[s];
     if (!sp.sorted) PIC_TRY(launch_bin(c, s));   // tiles need cell-sorted storage
@@ -668,10 +671,10 @@ int move_push(pic_ctx* c, int s) {
     if (sp.n_upper > 0) {
         hipEvent_t kt[2];
         PIC_TRY(kt_begin(c, kt));
-        if (f64 && !d.general_b) tile_kernel<PIC_LAYOUT_F64, 0>(c, sp, d, tpr, ntiles, kt);
-        else if (f64) tile_kernel<PIC_LAYOUT_F64, 1>(c, sp, d, tpr, ntiles, kt);
-        else if (!d.general_b) tile_kernel<PIC_LAYOUT_CELL32, 0>(c, sp, d, tpr, ntiles, kt);
-        else tile_kernel<PIC_LAYOUT_CELL32, 1>(c, sp, d, tpr, ntiles, kt);
+        if (f64 && !d.general_b) tile_kernel<PIC_LAYOUT_F64, 0>(c, sp, d, tpr, ntiles, kt, guard);
+        else if (f64) tile_kernel<PIC_LAYOUT_F64, 1>(c, sp, d, tpr, ntiles, kt, guard);
+        else if (!d.general_b) tile_kernel<PIC_LAYOUT_CELL32, 0>(c, sp, d, tpr, ntiles, kt, guard);
+        else tile_kernel<PIC_LAYOUT_CELL32, 1>(c, sp, d, tpr, ntiles, kt, guard);
         PIC_HIP(hipGetLastError());
         kt_end(c, PIC_STAGE_PUSH, kt);
     }

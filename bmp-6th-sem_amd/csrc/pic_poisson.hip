@@ -1309,7 +1309,9 @@ static int ensure_f32(pic_ctx* c) {
     return PIC_OK;
 }
 
-int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
+static int solve_finish(pic_ctx* c, const SolveTail& t, pic_solve_stats* st, int* redo);
+
+int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
     const Fine f = fine_of(c);
     const int G = c->G;
     hipStream_t s = c->stream;
@@ -1386,7 +1388,13 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2, c->flags,
                        c->scal, 1, S_BB, S_RR, S_RR0);
     PIC_HIP(hipGetLastError());
-    bool fields_done = false;
+    SolveTail tail;
+    tail.direct = direct;
+    tail.mixed = mixed;
+    tail.learned = learned;
+    tail.extrap_keep = extrap && keep;
+    tail.guess = guess;
+    tail.ev = ev;
     if (direct) {
         // The check's flags go to the host while the device already writes E:
         // the host waits for the flags only, not for the field kernel.
@@ -1394,9 +1402,47 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
         PIC_HIP(hipEventRecord(t1, s));
         PIC_TRY(timer_end(c, PIC_STAGE_POISSON, ev));
         PIC_TRY(launch_efield(c));
+        if (defer) {
+            // the caller enqueues more work first and calls poisson_finish
+            tail.active = 1;
+            c->tail = tail;
+            return PIC_OK;
+        }
+    }
+    return solve_finish(c, tail, st, nullptr);
+}
+
+int poisson_finish(pic_ctx* c, pic_solve_stats* st, int* redo) {
+    if (redo) *redo = 0;
+    if (!c->tail.active) return PIC_OK;
+    const SolveTail t = c->tail;
+    c->tail.active = 0;
+    return solve_finish(c, t, st, redo);
+}
+
+// The rest of a solve after its first check: the wait for the check of a
+// direct solve, PCG iterations where needed, the statistics.
+static int solve_finish(pic_ctx* c, const SolveTail& t, pic_solve_stats* st, int* redo) {
+    const Fine f = fine_of(c);
+    const int G = c->G;
+    hipStream_t s = c->stream;
+    hipEvent_t ev = t.ev;
+    hipEvent_t t0 = c->solve_ev[0], t1 = c->solve_ev[1];
+    double* part = c->part;
+    const double tol2 = c->p.poisson_tol * c->p.poisson_tol;
+    double* const R[2] = {c->pr, c->pr2};
+    double* const P[2] = {c->pp, c->pp2};
+    float* const Z[2] = {c->zf0, c->zf1};
+    const bool mixed = t.mixed, learned = t.learned;
+    const int guess = t.guess;
+    bool fields_done = false;
+    if (t.direct) {
         PIC_HIP(hipEventSynchronize(t1));
         fields_done = c->flags_host[0] == 1;
-        if (!fields_done) PIC_TRY(timer_begin(c, PIC_STAGE_POISSON, &ev));
+        if (!fields_done) {
+            if (redo) *redo = 1;
+            PIC_TRY(timer_begin(c, PIC_STAGE_POISSON, &ev));
+        }
     }
     int iters = 0, inner_total = 0;
     if (fields_done) {
@@ -1460,7 +1506,7 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st) {
     float ms = 0;
     PIC_HIP(hipEventElapsedTime(&ms, t0, t1));
     c->guess_set = 0;
-    c->have_prev_phi = (extrap && keep) ? 2 : 1;
+    c->have_prev_phi = t.extrap_keep ? 2 : 1;
     if (learned && c->learn_trained < c->learn_train) {
         float loss = 0.f;
         PIC_TRY(unet_train_pair(c, c->unet, c->phi_prev, c->learn_lr, &loss));

@@ -463,3 +463,48 @@ def test_kinetic_energy_reduction(hip_mod, layout):
         if n > 1000:
             assert abs(got / n / (1.5 * T) - 1) < 0.02
     c.close()
+
+
+@pytest.mark.parametrize("tol", [1e-10, 1e-15])
+def test_step_push_behind_the_poisson_check(hip_mod, tol):
+    """pic_step enqueues species 0's push behind the direct solve's check and
+    waits afterwards; the push does nothing unless the check passed.  At
+    1e-15 the direct solution (relative residual ~1e-14) fails the check, the
+    PCG corrects phi and E and the push runs again.  Either way the step
+    equals the stage sequence deposit, Poisson, push (bit for bit, as sets:
+    the order inside a cell is not fixed)."""
+    hip = hip_mod
+    NX = 64
+    ctxs = []
+    for _ in range(2):
+        c = _ctx(hip, NX, 1, geom=ELEC, poisson_tol=tol)
+        for sp, (q, m, T) in enumerate([(-E, ME, 2.0), (E, 40 * 1.66053906660e-27, 0.03)]):
+            s = c.add_species(q, m, 30000, 1e16, 40000)
+            x, y, vx, vy, vz = _parts(hip, c, 30000, seed=31, stream=sp, qm=abs(q / m), T=T)
+            c.upload(s, x, y, vx, vy, vz)
+        ctxs.append(c)
+    a, b = ctxs
+    for it in range(2):
+        st = a.step(it)
+        b.zero_rho()
+        for s in range(2):
+            b.deposit(s)
+        st2 = b.poisson_solve(it)
+        for s in range(2):
+            b.push(s, rebin=True)
+        assert st["guess"] == 5 and st2["guess"] == 5
+        assert st["iterations"] == st2["iterations"]
+        if tol == 1e-15:
+            assert st["iterations"] > 0, st          # the check failed: corrected
+        else:
+            assert st["iterations"] == 0, st
+        assert np.array_equal(a.grid(hip.GRID_PHI), b.grid(hip.GRID_PHI))
+        for s in range(2):
+            assert a.count(s) == b.count(s)
+            # the order inside a cell follows the slot atomics: compare sorted
+            ua, ub = a.download_cell32(s), b.download_cell32(s)
+            oa, ob = np.lexsort(ua[::-1]), np.lexsort(ub[::-1])
+            for u, v in zip(ua, ub):
+                assert np.array_equal(u[oa], v[ob]), (it, s)
+    for c in ctxs:
+        c.close()
