@@ -161,7 +161,7 @@ template <int L>
 __device__ __forceinline__ void deposit_tile_body(
     const ParticleBuf& b, const GridDev& g, const DepositArgs& a, int tiles_per_row, int tile,
     const int32_t* __restrict__ start, long long* __restrict__ cq, long long* __restrict__ ce,
-    unsigned long long (*acc)[kDepC])
+    unsigned long long (*acc)[kDepC], int* sst = nullptr)
 {
     using P = typename Lay<L>::P;
     using V = typename Lay<L>::V;
@@ -172,12 +172,29 @@ __device__ __forceinline__ void deposit_tile_body(
     const int k0 = r * g.gx + c0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) acc[c][tid] = 0ull;
+    // with `sst` the particles' cells come from the tile's cell starts (LDS):
+    // the cell array is not read
+    if (sst)
+        for (int j = tid; j <= cw; j += kBlock) sst[j] = start[k0 + j];
     __syncthreads();
     const int beg = start[k0], end = start[k0 + cw];
     const int first = beg & ~(kVec - 1);
     for (int base = first + kVec * tid; base < end; base += kVec * kBlock) {
         Quad<L> q;
-        load_quad<int>(b.cell, base, q.cell);
+        int sc = 0;
+        if (sst) {
+            // last cell c with sst[c] <= the quad's first particle of the tile
+            const int p0 = max(base, beg);
+            int lo = 0, hi = cw - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (sst[mid] <= p0) lo = mid;
+                else hi = mid - 1;
+            }
+            sc = lo;
+        } else {
+            load_quad<int>(b.cell, base, q.cell);
+        }
         load_quad<P>((const P*)b.pa, base, q.a);
         load_quad<P>((const P*)b.pb, base, q.b);
         if (a.energy) {
@@ -204,7 +221,13 @@ __device__ __forceinline__ void deposit_tile_body(
         for (int i = 0; i < kVec; ++i) {
             const int pi = base + i;
             if (pi < beg || pi >= end) continue;
-            const int cl = q.cell[i];
+            int cl;
+            if (sst) {
+                while (sst[sc + 1] <= pi) ++sc;
+                cl = k0 + sc;
+            } else {
+                cl = q.cell[i];
+            }
             if (cl != cur) {
                 if (cur >= 0) flush();
                 cur = cl;

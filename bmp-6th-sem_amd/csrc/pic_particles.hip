@@ -333,7 +333,8 @@ __global__ __launch_bounds__(kBlock) void k_deposit_tile(
     long long* __restrict__ cq, long long* __restrict__ ce)
 {
     __shared__ unsigned long long acc[8][kDepC];
-    deposit_tile_body<L>(b, g, a, tiles_per_row, xcd_tile(blockIdx.x, ntiles), start, cq, ce, acc);
+    __shared__ int sst[kDepC + 1];
+    deposit_tile_body<L>(b, g, a, tiles_per_row, xcd_tile(blockIdx.x, ntiles), start, cq, ce, acc, sst);
 }
 
 // Recompute the push and write each surviving particle to its sorted slot.
