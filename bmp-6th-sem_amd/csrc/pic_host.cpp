@@ -274,7 +274,7 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
         set_error("pic_create: grid too large for the two-level scan");
         return fail(PIC_ERR_ARG);
     }
-    ALLOC(c->scan_bsum, c->scan_nb * sizeof(int32_t));
+    ALLOC(c->scan_bsum, 2 * (size_t)c->scan_nb * sizeof(int32_t));
 #undef ALLOC
     if (hipHostMalloc((void**)&c->scal_host, PIC_STATE_BYTES, hipHostMallocDefault) != hipSuccess) {
         set_error("pic_create: hipHostMalloc failed");
@@ -731,6 +731,14 @@ extern "C" int pic_step(pic_ctx* c, int iteration, pic_solve_stats* st) {
     PIC_TRY(poisson_finish(c, st, &redo));
     if (early && redo) PIC_TRY(move_push(c, 0));   // the PCG corrected phi and E: push again
     if (!c->inj.on) {
+        if (ns == 2) {
+            // both pushes, both scans in one set of launches, both scatters
+            if (!early) PIC_TRY(move_push(c, 0));
+            PIC_TRY(move_push(c, 1));
+            PIC_TRY(move_scan_pair(c));
+            PIC_TRY(move_scatter(c, 0));
+            return move_scatter(c, 1);
+        }
         for (int s = 0; s < ns; ++s) {
             if (s == 0 && early) PIC_TRY(move_finish(c, 0));
             else PIC_TRY(launch_move(c, s));
@@ -747,8 +755,9 @@ extern "C" int pic_step(pic_ctx* c, int iteration, pic_solve_stats* st) {
     PIC_TRY(inject_count(c, 0));
     PIC_TRY(inject_count(c, 1));
     PIC_TRY(timer_end(c, PIC_STAGE_BIN, ev));
+    if (ns == 2) PIC_TRY(move_scan_pair(c));
     for (int s = 0; s < ns; ++s) {
-        PIC_TRY(move_finish(c, s));
+        PIC_TRY(ns == 2 ? move_scatter(c, s) : move_finish(c, s));
         if (s < 2) {
             PIC_TRY(timer_begin(c, PIC_STAGE_SCATTER, &ev));
             PIC_TRY(inject_scatter(c, s, c->sp[s].cur));

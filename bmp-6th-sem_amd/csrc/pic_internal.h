@@ -191,7 +191,7 @@ struct pic_ctx {
     double* scal_host = nullptr;    // pinned copy of the state block
     int* flags_host = nullptr;      // in it
     int scan_nb = 0;
-    int32_t* scan_bsum = nullptr;
+    int32_t* scan_bsum = nullptr;   // block sums of two scans (2 x scan_nb)
     pic::Injection inj;
     pic::FastPoisson* fast = nullptr;   // direct solve of the current geometry (null: PCG only)
     // timing
@@ -215,7 +215,9 @@ int launch_move(pic_ctx* c, int s);
 // launch_move in phases, for injection between them; with `guard`, the push
 // does nothing unless guard[0] == 1 when it runs (a Poisson check passed)
 int move_push(pic_ctx* c, int s, const int* guard = nullptr);
-int move_finish(pic_ctx* c, int s);
+int move_finish(pic_ctx* c, int s);              // scan + scatter of species s
+int move_scan_pair(pic_ctx* c);                   // the scans of species 0 and 1 in one set of launches
+int move_scatter(pic_ctx* c, int s);              // the scatter after the scan
 int launch_zero_f64(pic_ctx* c, double* p, int64_t n);
 
 // injection (pic_inject.hip)

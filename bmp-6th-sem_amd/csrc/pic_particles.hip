@@ -101,9 +101,19 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 static constexpr int kScanPer = 8;
 static constexpr int kScanChunk = kBlock * kScanPer;   // 2048 counts per block
 
-__global__ __launch_bounds__(kBlock) void k_scan_local(
-    int32_t* __restrict__ counts, int G, int32_t* __restrict__ start, int32_t* __restrict__ bsum)
+// The scan of up to two species at once: species blockIdx.y of the set.
+struct ScanSet {
+    int32_t* counts[2];
+    int32_t* start[2];
+    int32_t* bsum[2];
+    int32_t* nout[2];     // live count after the scan
+};
+
+__global__ __launch_bounds__(kBlock) void k_scan_local(ScanSet ss, int G)
 {
+    int32_t* __restrict__ counts = ss.counts[blockIdx.y];
+    int32_t* __restrict__ start = ss.start[blockIdx.y];
+    int32_t* __restrict__ bsum = ss.bsum[blockIdx.y];
     __shared__ int wsum[kBlock / 64];
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const int base = blockIdx.x * kScanChunk + t * kScanPer;
@@ -136,9 +146,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_local(
 }
 
 // One workgroup of 1024 threads scans up to 8192 block sums in place.
-__global__ __launch_bounds__(1024) void k_scan_top(
-    int32_t* __restrict__ bsum, int nb, int32_t* __restrict__ start, int G, int32_t* __restrict__ ndst)
+__global__ __launch_bounds__(1024) void k_scan_top(ScanSet ss, int nb, int G)
 {
+    int32_t* __restrict__ bsum = ss.bsum[blockIdx.x];
+    int32_t* __restrict__ start = ss.start[blockIdx.x];
+    int32_t* __restrict__ ndst = ss.nout[blockIdx.x];
     __shared__ int wsum[16];
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     int v[8];
@@ -168,9 +180,10 @@ __global__ __launch_bounds__(1024) void k_scan_top(
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_add(
-    int32_t* __restrict__ start, int G, const int32_t* __restrict__ boff)
+__global__ __launch_bounds__(kBlock) void k_scan_add(ScanSet ss, int G)
 {
+    int32_t* __restrict__ start = ss.start[blockIdx.y];
+    const int32_t* __restrict__ boff = ss.bsum[blockIdx.y];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < G) start[i] += boff[i / kScanChunk];
 }
@@ -435,15 +448,29 @@ int launch_key_from_cell(pic_ctx* c, int s, int64_t n) {
     return PIC_OK;
 }
 
-static int scan_counts(pic_ctx* c, Species& sp, int dst) {
-    hipLaunchKernelGGL(k_scan_local, dim3(c->scan_nb), dim3(kBlock), 0, c->stream,
-                       sp.counts, c->G, sp.start, c->scan_bsum);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, c->stream,
-                       c->scan_bsum, c->scan_nb, sp.start, c->G, sp.ndev + dst);
-    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(c->G)), dim3(kBlock), 0, c->stream,
-                       sp.start, c->G, c->scan_bsum);
+// Exclusive scan of the cell counts into start (counts cleared), the live
+// count into ndev[dst]; species s0 and, if s1 >= 0, s1 in the same launches.
+static int scan_counts2(pic_ctx* c, int s0, int s1) {
+    ScanSet ss{};
+    const int n = s1 >= 0 ? 2 : 1;
+    const int ids[2] = {s0, s1 >= 0 ? s1 : s0};
+    for (int k = 0; k < 2; ++k) {
+        Species& sp = c->sp[ids[k]];
+        ss.counts[k] = sp.counts;
+        ss.start[k] = sp.start;
+        ss.bsum[k] = c->scan_bsum + (size_t)k * c->scan_nb;
+        ss.nout[k] = sp.ndev + (sp.cur ^ 1);
+    }
+    hipLaunchKernelGGL(k_scan_local, dim3(c->scan_nb, n), dim3(kBlock), 0, c->stream, ss, c->G);
+    hipLaunchKernelGGL(k_scan_top, dim3(n), dim3(1024), 0, c->stream, ss, c->scan_nb, c->G);
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(c->G), n), dim3(kBlock), 0, c->stream, ss, c->G);
     PIC_HIP(hipGetLastError());
     return PIC_OK;
+}
+
+static int scan_counts(pic_ctx* c, Species& sp, int dst) {
+    (void)dst;   // ndev[cur ^ 1]
+    return scan_counts2(c, (int)(&sp - c->sp.data()), -1);
 }
 
 // Counting sort of the current storage by sp.key (upload / pic_bin / after an
@@ -685,13 +712,27 @@ int move_push(pic_ctx* c, int s, const int* guard) {
 
 int move_finish(pic_ctx* c, int s) {
     Species& sp = c->sp[s];
+    hipEvent_t ev;
+    PIC_TRY(timer_begin(c, PIC_STAGE_BIN, &ev));
+    PIC_TRY(scan_counts(c, sp, sp.cur ^ 1));
+    PIC_TRY(timer_end(c, PIC_STAGE_BIN, ev));
+    return move_scatter(c, s);
+}
+
+int move_scan_pair(pic_ctx* c) {
+    hipEvent_t ev;
+    PIC_TRY(timer_begin(c, PIC_STAGE_BIN, &ev));
+    PIC_TRY(scan_counts2(c, 0, 1));
+    PIC_TRY(timer_end(c, PIC_STAGE_BIN, ev));
+    return PIC_OK;
+}
+
+int move_scatter(pic_ctx* c, int s) {
+    Species& sp = c->sp[s];
     SpeciesDev d = species_dev(c, sp);
     const bool f64 = c->p.layout == PIC_LAYOUT_F64;
     const int src = sp.cur, dst = sp.cur ^ 1;
     hipEvent_t ev;
-    PIC_TRY(timer_begin(c, PIC_STAGE_BIN, &ev));
-    PIC_TRY(scan_counts(c, sp, dst));
-    PIC_TRY(timer_end(c, PIC_STAGE_BIN, ev));
     PIC_TRY(timer_begin(c, PIC_STAGE_SCATTER, &ev));
     if (sp.n_upper > 0) {
         hipEvent_t kt[2];
