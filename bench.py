@@ -26,13 +26,13 @@ HBM_PEAK_GBS = 8000.0
 # for the CELL32 (1) and F64 (0) layouts:
 #   k_push_tile    reads the state (cell, 2 offsets, 3 velocities), writes the slot;
 #   k_scatter_push reads the state and the slot, writes the state to its sorted slot;
-#   k_deposit_tile reads cells and offsets (+ velocities for the energy mesh) per
-#                  particle and, per cell, its start index and four charge + four
-#                  energy corner sums.
+#   k_deposit_tile reads offsets (+ velocities for the energy mesh) per particle
+#                  (the cells come from the cell starts) and, per cell, its start
+#                  index and four charge + four energy corner sums.
 KERNEL_BYTES = {
     "push": {1: (24 + 4, 0), 0: (44 + 4, 0)},
     "scatter": {1: (24 + 4 + 24, 0), 0: (44 + 4 + 44, 0)},
-    "deposit": {1: (4 + 8 + 12, 4 + 64), 0: (4 + 16 + 24, 4 + 64)},
+    "deposit": {1: (8 + 12, 4 + 64), 0: (16 + 24, 4 + 64)},
 }
 KERNEL_NAMES = {"push": "k_push_tile", "scatter": "k_scatter_push", "deposit": "k_deposit_tile"}
 
