@@ -355,12 +355,15 @@ int     pic_learned_stats(pic_ctx* ctx, int* trained, int* tried, int* used, flo
 /* ======================================================= ensembles */
 /* Batched execution of independent runs of uneven sizes: every stage is one
  * launch for all runs (persistent work-queue kernels for deposit, E, push and
- * scatter; one workgroup per run for the Poisson solve and the scan), the
- * step is captured once in a HIP graph and replayed.  The runs are contexts
- * already set up (geometry, two species, particles uploaded), CELL32 storage,
- * Bz-only field, FP64 Poisson; after the ensemble's steps they are in the
- * state a serial pic_step would leave.  No counterpart in the reference (one
- * simulation per process, sycl_pic_ctypes.cpp:39-41). */
+ * scatter; direct Poisson solves batched over the runs of one shape, checked
+ * by one workgroup per run, which runs the PCG for the runs without a direct
+ * solve; one workgroup per run and species for the scan), the step is
+ * captured once in a HIP graph and replayed.  The runs are contexts already
+ * set up (geometry, two species, particles uploaded), CELL32 storage, Bz-only
+ * field, FP64 Poisson; their geometry must not change while the ensemble
+ * exists (it holds their solver buffers).  After the ensemble's steps they are
+ * in the state a serial pic_step would leave.  No counterpart in the
+ * reference (one simulation per process, sycl_pic_ctypes.cpp:39-41). */
 typedef struct pic_ensemble pic_ensemble;
 
 int  pic_ensemble_create(pic_ctx** runs, int nruns, pic_ensemble** out);
