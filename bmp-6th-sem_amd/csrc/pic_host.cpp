@@ -49,11 +49,12 @@ int timer_end(pic_ctx* c, int stage, hipEvent_t ev0) {
     return PIC_OK;
 }
 
-// Kernel events (timing mode 2): start / stop timestamps written by the
-// dispatch of one kernel, no marker in the stream.
-int kt_begin(pic_ctx* c, hipEvent_t ev[2]) {
+// Kernel events (timing mode 2: every particle kernel; mode 3: the scatter
+// kernel only): start / stop timestamps written by the dispatch of one kernel,
+// no marker in the stream.
+int kt_begin(pic_ctx* c, hipEvent_t ev[2], int stage) {
     ev[0] = ev[1] = nullptr;
-    if (c->timing != 2) return PIC_OK;
+    if (c->timing != 2 && !(c->timing == 3 && stage == PIC_STAGE_SCATTER)) return PIC_OK;
     PIC_TRY(timer_event(c, &ev[0]));
     PIC_TRY(timer_event(c, &ev[1]));
     return PIC_OK;
@@ -916,8 +917,8 @@ extern "C" int pic_set_timing(pic_ctx* c, int on) {
         set_error("pic_set_timing: null context");
         return PIC_ERR_ARG;
     }
-    if (on < 0 || on > 2) {
-        set_error("pic_set_timing: mode %d (0 off, 1 stages, 2 particle kernels)", on);
+    if (on < 0 || on > 3) {
+        set_error("pic_set_timing: mode %d (0 off, 1 stages, 2 particle kernels, 3 scatter kernel)", on);
         return PIC_ERR_ARG;
     }
     c->timing = on;

@@ -262,7 +262,7 @@ int launch_efield(pic_ctx* c);
 // timing helpers (pic_host.cpp)
 int timer_begin(pic_ctx* c, int stage, hipEvent_t* ev);
 int timer_end(pic_ctx* c, int stage, hipEvent_t ev0);
-int kt_begin(pic_ctx* c, hipEvent_t ev[2]);          // timing mode 2: events of one kernel dispatch
+int kt_begin(pic_ctx* c, hipEvent_t ev[2], int stage);   // timing modes 2, 3: events of one kernel dispatch
 void kt_end(pic_ctx* c, int stage, const hipEvent_t ev[2]);
 
 } // namespace pic

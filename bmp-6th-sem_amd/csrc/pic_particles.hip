@@ -601,7 +601,7 @@ static int deposit_corners(pic_ctx* c, int s) {
             const int tpr = (c->g.gx + kDepC - 1) / kDepC;
             const unsigned ntiles = (unsigned)(tpr * c->g.gy);
             hipEvent_t kt[2];
-            PIC_TRY(kt_begin(c, kt));
+            PIC_TRY(kt_begin(c, kt, PIC_STAGE_DEPOSIT));
             if (c->p.layout == PIC_LAYOUT_F64)
                 launch_timed(k_deposit_tile<PIC_LAYOUT_F64>, dim3(ntiles), dim3(kBlock), c->stream, kt,
                              sp.buf[sp.cur], c->g, a, tpr, (int)ntiles, (const int32_t*)sp.start,
@@ -698,7 +698,7 @@ int move_push(pic_ctx* c, int s, const int* guard) {
     PIC_TRY(timer_begin(c, PIC_STAGE_PUSH, &ev));
     if (sp.n_upper > 0) {
         hipEvent_t kt[2];
-        PIC_TRY(kt_begin(c, kt));
+        PIC_TRY(kt_begin(c, kt, PIC_STAGE_PUSH));
         if (f64 && !d.general_b) tile_kernel<PIC_LAYOUT_F64, 0>(c, sp, d, tpr, ntiles, kt, guard);
         else if (f64) tile_kernel<PIC_LAYOUT_F64, 1>(c, sp, d, tpr, ntiles, kt, guard);
         else if (!d.general_b) tile_kernel<PIC_LAYOUT_CELL32, 0>(c, sp, d, tpr, ntiles, kt, guard);
@@ -736,7 +736,7 @@ int move_scatter(pic_ctx* c, int s) {
     PIC_TRY(timer_begin(c, PIC_STAGE_SCATTER, &ev));
     if (sp.n_upper > 0) {
         hipEvent_t kt[2];
-        PIC_TRY(kt_begin(c, kt));
+        PIC_TRY(kt_begin(c, kt, PIC_STAGE_SCATTER));
         if (f64 && !d.general_b) scatter_kernel<PIC_LAYOUT_F64, 0>(c, sp, d, src, dst, kt);
         else if (f64) scatter_kernel<PIC_LAYOUT_F64, 1>(c, sp, d, src, dst, kt);
         else if (!d.general_b) scatter_kernel<PIC_LAYOUT_CELL32, 0>(c, sp, d, src, dst, kt);

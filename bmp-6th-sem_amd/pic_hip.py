@@ -365,7 +365,8 @@ class Context:
 
     # -- timing
     def set_timing(self, on=True):
-        """0/False off, 1/True per stage, 2 per particle kernel (no markers in the stream)."""
+        """0/False off, 1/True per stage, 2 per particle kernel (no markers in the
+        stream), 3 the scatter kernel alone."""
         _chk(self.lib.pic_set_timing(self.h, int(on)))
 
     def learned_enable(self, train_steps, min_trained=0, lr=1e-3, seed=1):

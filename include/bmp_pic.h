@@ -294,7 +294,7 @@ int  pic_sync(pic_ctx* ctx);
  * kernels); 2 per kernel for the three particle kernels only (the tile push,
  * the scatter and the tile deposit, in the PUSH, SCATTER and DEPOSIT slots),
  * timestamps taken by the kernel dispatches themselves, so a measured loop
- * carries no markers.  pic_stage_times reads the accumulated device
+ * carries no markers; 3 as 2 for the scatter kernel alone.  pic_stage_times reads the accumulated device
  * milliseconds and launch counts per stage (PIC_NSTAGES entries each; either
  * pointer may be NULL) and resets them. */
 int  pic_set_timing(pic_ctx* ctx, int on);
