@@ -43,6 +43,14 @@ __device__ __forceinline__ long long round_fixed(double v) {
     return __double2ll_rn(v);
 }
 
+// round_fixed for |v| < 2^51, without the range test: one add and one 32-bit
+// subtract.  For such v, v + 1.5 * 2^52 lies in [2^52, 2^53), where the
+// spacing of doubles is 1, so the add rounds v to the nearest integer (ties
+// to even) exactly like __double2ll_rn.
+__device__ __forceinline__ long long round_small(double v) {
+    return __double_as_longlong(v + 0x1.8p52) - 0x4338000000000000LL;
+}
+
 // Largest float below 1: stored offsets stay in [0, 1).
 __device__ __forceinline__ float clamp_offset(float o) {
     return o < 1.0f ? o : 0x1.fffffep-1f;
@@ -75,7 +83,9 @@ struct DepositArgs {
     double half_x;     // 2^(charge_shift / 2)
     double half_y;     // 2^(charge_shift - charge_shift / 2)
     double denom;      // 1 / |q/m|
+    double e_fast;     // 2^(50 - charge_shift): energy sums take round_small below it
     int    energy;
+    int    q_fast;     // charge_shift <= 50: charge weights (< 2 * 2^charge_shift) take round_small
 };
 
 template <int L>
@@ -252,19 +262,35 @@ __device__ __forceinline__ void deposit_tile_body(
                 w2 = ux * vy_;
                 w3 = vx_ * vy_;
             }
-            s0 += round_fixed(w0);
-            s1 += round_fixed(w1);
-            s2 += round_fixed(w2);
-            s3 += round_fixed(w3);
+            // the range tests of round_fixed are taken once per workgroup
+            // (charge) or once per particle (energy): |w| < 2 * 2^S
+            if (a.q_fast) {
+                s0 += round_small(w0);
+                s1 += round_small(w1);
+                s2 += round_small(w2);
+                s3 += round_small(w3);
+            } else {
+                s0 += round_fixed(w0);
+                s1 += round_fixed(w1);
+                s2 += round_fixed(w2);
+                s3 += round_fixed(w3);
+            }
             if (a.energy) {
                 const double pvx = (double)q.vx[i], pvy = (double)q.vy[i], pvz = (double)q.vz[i];
                 const double v2 = pvx * pvx + pvy * pvy + pvz * pvz;
                 const double en = 0.5 * v2 * a.denom;
                 const double es = en * a.scale_eq;    // exact: power of two
-                e0 += round_fixed(w0 * es);
-                e1 += round_fixed(w1 * es);
-                e2 += round_fixed(w2 * es);
-                e3 += round_fixed(w3 * es);
+                if (es < a.e_fast) {
+                    e0 += round_small(w0 * es);
+                    e1 += round_small(w1 * es);
+                    e2 += round_small(w2 * es);
+                    e3 += round_small(w3 * es);
+                } else {
+                    e0 += round_fixed(w0 * es);
+                    e1 += round_fixed(w1 * es);
+                    e2 += round_fixed(w2 * es);
+                    e3 += round_fixed(w3 * es);
+                }
             }
         }
         if (cur >= 0) flush();

@@ -513,7 +513,9 @@ DepositArgs deposit_args(const pic_ctx* c, const Species& sp) {
     a.half_x = std::ldexp(1.0, c->p.charge_shift / 2);
     a.half_y = std::ldexp(1.0, c->p.charge_shift - c->p.charge_shift / 2);
     a.denom = 1 / std::fabs(sp.qm);
+    a.e_fast = std::ldexp(1.0, 50 - c->p.charge_shift);
     a.energy = c->p.deposit_energy;
+    a.q_fast = c->p.charge_shift <= 50;
     return a;
 }
 

@@ -107,6 +107,39 @@ def test_deposit_parity(hip_mod, oracle_mod, layout):
     c.close()
 
 
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("shifts", [(51, 40), (50, 36)])
+def test_deposit_parity_wide_fixed_point(hip_mod, oracle_mod, layout, shifts):
+    """Charge shifts at and above 2^50 and particles whose energy weights pass
+    2^51 (about 1 in 10 carries 10-60 keV): the rounding takes its range-tested
+    path there and still equals the oracle's round-to-nearest bit for bit."""
+    hip, O = hip_mod, oracle_mod
+    S, Es = shifts
+    c = _ctx(hip, 64, layout, charge_shift=S, energy_shift=Es)
+    qm = -E / ME
+    n = 40000
+    s = c.add_species(-E, ME, n, 1e16, n)
+    x, y, vx, vy, vz = _parts(hip, c, n, seed=8)
+    rng = np.random.default_rng(8)
+    hot = rng.random(n) < 0.1
+    boost = rng.uniform(6e7, 1.4e8, n)
+    vx = np.where(hot, boost, vx)
+    c.upload(s, x, y, vx, vy, vz)
+    c.zero_rho()
+    c.deposit(s)
+    qc, qe = c.deposit_fixed(s)
+    if layout == 0:
+        key = O.keys(x, y, c.gx, c.dx, c.dy)
+        rqc, rqe = O.deposit_fixed_f64(x, y, vx, vy, vz, key, c.gx, c.gy, c.dx, c.dy, S, Es, qm)
+    else:
+        cell, ox, oy, fvx, fvy, fvz = O.encode_cell32(x, y, vx, vy, vz, c.gx, c.dx, c.dy)
+        rqc, rqe = O.deposit_fixed_cell32(cell, ox, oy, fvx, fvy, fvz, c.gx, c.gy, S, Es, qm)
+    assert int(np.abs(rqe).max()) > 2 ** 51      # the energy sums left the short path's range
+    assert np.array_equal(qc, rqc), "fixed-point charge differs from the oracle"
+    assert np.array_equal(qe, rqe), "fixed-point energy differs from the oracle"
+    c.close()
+
+
 def test_efield_parity(hip_mod, oracle_mod):
     hip, O = hip_mod, oracle_mod
     c = _ctx(hip, 48, 0, geom=ELEC)
