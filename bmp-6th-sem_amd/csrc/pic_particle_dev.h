@@ -338,6 +338,7 @@ template <int L> struct Pushed {
     using P = typename Lay<L>::P;
     using V = typename Lay<L>::V;
     int key;            // new bin key, -1 when absorbed
+    int kx, ky;         // column and row of the new key (0 <= kx < GRID_X)
     bool left;          // absorbed through x < 0
     P a, b;             // F64: x, y   CELL32: offsets
     V vx, vy, vz;
@@ -491,6 +492,8 @@ __device__ __forceinline__ Pushed<L> push_state(const PState<L>& p, const Corner
         }
         int ngx = (int)(px * g.inv_dx), ngy = (int)(py * g.inv_dy);
         o.key = ngy * g.gx + ngx;
+        o.kx = ngx;
+        o.ky = ngy;
         o.a = (P)px;
         o.b = (P)py;
     } else {
@@ -513,6 +516,8 @@ __device__ __forceinline__ Pushed<L> push_state(const PState<L>& p, const Corner
         }
         int ngx = (int)tx, ngy = (int)ty;
         o.key = ngy * g.gx + ngx;
+        o.kx = ngx;
+        o.ky = ngy;
         o.a = clamp_offset((float)(tx - (double)ngx));
         o.b = clamp_offset((float)(ty - (double)ngy));
     }
@@ -604,9 +609,7 @@ __device__ __forceinline__ void push_tile_body(
                 if (o.left) atomicAdd(&nabs[0], 1ull);
                 atomicAdd(&nabs[1], 1ull);
             } else {
-                int gxn, gyn;
-                split_key(o.key, g, gxn, gyn);
-                const int dr = gyn - (r - 1), dc = gxn - (c0 - 1);
+                const int dr = o.ky - (r - 1), dc = o.kx - (c0 - 1);
                 if (dr >= 0 && dr < kHR && dc >= 0 && dc < kHW) bin = dr * kHW + dc;
                 else code = kFallback | atomicAdd(&counts[o.key], 1);
             }
