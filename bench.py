@@ -23,15 +23,20 @@ sys.path.insert(0, str(ROOT / "bmp-6th-sem_amd"))
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes of the particle kernels, per particle (and per cell),
-# for the CELL32 (1) and F64 (0) layouts:
-#   k_push_tile    reads the state (cell, 2 offsets, 3 velocities), writes the slot;
-#   k_scatter_push reads the state and the slot, writes the state to its sorted slot;
+# for the CELL32 (1) and F64 (0) layouts.  CELL32 storage holds 2 offsets and
+# 3 velocities (20 B); a particle's cell comes from the cell starts, so no
+# kernel of the step reads or writes the cell array.  F64: x, y and 3
+# velocities (40 B) plus the cell array, which its scatter writes.
+#   k_push_tile    reads the state, writes the slot; per cell it reads the
+#                  start and copies it to start_old;
+#   k_scatter_push reads the state and the slot, writes the state to its sorted
+#                  slot; per cell it reads the old start and the new start;
 #   k_deposit_tile reads offsets (+ velocities for the energy mesh) per particle
-#                  (the cells come from the cell starts) and, per cell, its start
-#                  index and four charge + four energy corner sums.
+#                  and, per cell, its start index and four charge + four energy
+#                  corner sums.
 KERNEL_BYTES = {
-    "push": {1: (24 + 4, 0), 0: (44 + 4, 0)},
-    "scatter": {1: (24 + 4 + 24, 0), 0: (44 + 4 + 44, 0)},
+    "push": {1: (20 + 4, 4 + 4), 0: (40 + 4, 0)},
+    "scatter": {1: (20 + 4 + 20, 4 + 4), 0: (40 + 4 + 44, 4)},
     "deposit": {1: (8 + 12, 4 + 64), 0: (16 + 24, 4 + 64)},
 }
 KERNEL_NAMES = {"push": "k_push_tile", "scatter": "k_scatter_push", "deposit": "k_deposit_tile"}

@@ -685,6 +685,7 @@ void advance_host_state(pic_ensemble* e, int n) {
             if (n & 1) sp.cur ^= 1;
             sp.sorted = 1;
             sp.deposit_pending = 0;
+            if (n > 0) sp.cells_stale = 0;   // the ensemble's scatter writes the cell array
         }
         c->have_prev_phi = 1;
         c->guess_set = 0;
@@ -719,6 +720,7 @@ int pic_ensemble_create(pic_ctx** runs, int nruns, pic_ensemble** out) {
                 return PIC_ERR_STATE;
             }
         }
+        for (int s = 0; s < 2; ++s) PIC_TRY(ensure_cells(c, s));   // the ensemble's kernels read the cells
         PIC_HIP(hipStreamSynchronize(c->stream));
     }
     pic_ensemble* e = new pic_ensemble();
@@ -811,6 +813,15 @@ int pic_ensemble_step(pic_ensemble* e, int nsteps, int graph) {
     if (!e || nsteps < 0) {
         set_error("pic_ensemble_step: invalid argument");
         return PIC_ERR_ARG;
+    }
+    // runs stepped on their own since the last ensemble step: cells from the starts
+    for (pic_ctx* c : e->runs) {
+        bool stale = false;
+        for (int s = 0; s < 2; ++s) {
+            stale = stale || c->sp[s].cells_stale;
+            PIC_TRY(ensure_cells(c, s));
+        }
+        if (stale) PIC_HIP(hipStreamSynchronize(c->stream));
     }
     for (int k = 0; k < nsteps; ++k) {
         const int par = (e->parity + k) & 1;

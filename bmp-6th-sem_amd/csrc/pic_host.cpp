@@ -113,7 +113,7 @@ static void free_species(pic::Species& s) {
         if (b.cell) (void)hipFree(b.cell);
         b = pic::ParticleBuf{};
     }
-    void* ptrs[] = {s.ndev, s.key, s.rank, s.counts, s.start, s.q_charge, s.q_energy,
+    void* ptrs[] = {s.ndev, s.key, s.rank, s.counts, s.start, s.start_old, s.blk_first, s.q_charge, s.q_energy,
                     s.final_mesh, s.energy_mesh, s.absorbed};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -434,6 +434,8 @@ extern "C" int pic_add_species(pic_ctx* c, double charge, double mass, double si
     ok = ok && hipMalloc((void**)&s.rank, n * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s.counts, G * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s.start, (G + 1) * sizeof(int32_t)) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s.start_old, (G + 1) * sizeof(int32_t)) == hipSuccess;
+    ok = ok && hipMalloc((void**)&s.blk_first, (n / 256 + 2) * sizeof(int32_t)) == hipSuccess;
     const size_t nacc = 4 * (size_t)c->g.gp;   // four node-indexed corner meshes
     ok = ok && hipMalloc((void**)&s.q_charge, nacc * sizeof(int64_t)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s.q_energy, nacc * sizeof(int64_t)) == hipSuccess;
@@ -448,6 +450,8 @@ extern "C" int pic_add_species(pic_ctx* c, double charge, double mass, double si
     (void)hipMemset(s.ndev, 0, 2 * sizeof(int32_t));
     (void)hipMemset(s.counts, 0, G * sizeof(int32_t));
     (void)hipMemset(s.start, 0, (G + 1) * sizeof(int32_t));
+    (void)hipMemset(s.start_old, 0, (G + 1) * sizeof(int32_t));
+    (void)hipMemset(s.blk_first, 0, (n / 256 + 2) * sizeof(int32_t));
     (void)hipMemset(s.q_charge, 0, nacc * sizeof(int64_t));
     (void)hipMemset(s.q_energy, 0, nacc * sizeof(int64_t));
     (void)hipMemset(s.final_mesh, 0, G * sizeof(double));
@@ -531,6 +535,7 @@ extern "C" int pic_upload_cell32(pic_ctx* c, int s, int64_t n, const int32_t* ce
         PIC_HIP(hipMemcpyAsync(b.vy, vy, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
         PIC_HIP(hipMemcpyAsync(b.vz, vz, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
     }
+    sp.cells_stale = 0;   // the uploaded cells are the storage's
     PIC_TRY(launch_key_from_cell(c, s, n));
     sp.deposit_pending = 0;
     PIC_TRY(launch_bin(c, s));
@@ -559,6 +564,7 @@ extern "C" int pic_download_cell32(pic_ctx* c, int s, int32_t* cell, float* ox, 
         set_error("pic_download_cell32: context layout is not CELL32");
         return PIC_ERR_STATE;
     }
+    if (cell) PIC_TRY(pic::ensure_cells(c, s));
     int32_t n = 0;
     PIC_TRY(live_count(c, s, &n));
     pic::ParticleBuf& b = c->sp[s].buf[c->sp[s].cur];

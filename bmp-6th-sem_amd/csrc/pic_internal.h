@@ -83,6 +83,8 @@ struct Species {
     int32_t* rank = nullptr;  // per-particle slot inside its cell
     int32_t* counts = nullptr;// GRID_X*GRID_Y
     int32_t* start = nullptr; // GRID_X*GRID_Y + 1 (start[G] = live count)
+    int32_t* start_old = nullptr; // CELL32 step: `start` of the order being pushed (the scan overwrites start)
+    int32_t* blk_first = nullptr; // CELL32 step: cell of particle 256 b of the order being pushed
     int64_t* q_charge = nullptr; // fixed-point corner meshes: 4 x (G + GRID_X + 2) int64
     int64_t* q_energy = nullptr;
     double*  final_mesh = nullptr;
@@ -90,6 +92,7 @@ struct Species {
     unsigned long long* absorbed = nullptr; // [0] left through x<0, [1] all absorbed
     int deposit_pending = 0;  // q_charge/q_energy hold the deposit of the stored particles
     int sorted = 0;           // storage is cell-sorted and `start` describes it
+    int cells_stale = 0;      // CELL32: buf[cur].cell not written (the step derives cells from `start`)
 };
 
 // One multigrid level (FP32 storage).
@@ -219,6 +222,8 @@ int move_finish(pic_ctx* c, int s);              // scan + scatter of species s
 int move_scan_pair(pic_ctx* c);                   // the scans of species 0 and 1 in one set of launches
 int move_scatter(pic_ctx* c, int s);              // the scatter after the scan
 int launch_zero_f64(pic_ctx* c, double* p, int64_t n);
+// CELL32: write buf[cur].cell from `start` when the step left it unwritten
+int ensure_cells(pic_ctx* c, int s);
 
 // injection (pic_inject.hip)
 int inject_setup(pic_ctx* c, const pic_injection* cfg);

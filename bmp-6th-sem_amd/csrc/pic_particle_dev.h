@@ -383,12 +383,13 @@ template <int L> struct PState {
     V vx, vy, vz;
 };
 
-template <int L>
+// With CELL = false the cell array is not read (the caller supplies p.cell).
+template <int L, bool CELL = true>
 __device__ __forceinline__ PState<L> load_state(const ParticleBuf& b, int i) {
     using P = typename Lay<L>::P;
     using V = typename Lay<L>::V;
     PState<L> p;
-    p.cell = L == PIC_LAYOUT_CELL32 ? b.cell[i] : 0;
+    p.cell = (L == PIC_LAYOUT_CELL32 && CELL) ? b.cell[i] : 0;
     p.a = ((const P*)b.pa)[i];
     p.b = ((const P*)b.pb)[i];
     p.vx = ((const V*)b.vx)[i];
@@ -546,6 +547,19 @@ __device__ __forceinline__ void store_state(const ParticleBuf& d, int j, const P
     ((V*)d.vz)[j] = o.vz;
 }
 
+// Index of the cell holding particle i among the cells of a sorted range:
+// the last j in [0, hi] with st[j] <= i (st[0] <= i < st[hi + 1]; empty
+// cells have st[j] == st[j + 1] and are passed over).
+__device__ __forceinline__ int cell_of(const int* st, int hi, int i) {
+    int lo = 0;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (st[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // In-place push without re-binning (stage API pic_push(ctx, s, 0)); the
 // stored cell becomes the new key (-1 when absorbed).
 // ====================================================== tiled push + bin
@@ -571,14 +585,22 @@ static constexpr int kCodeCap = 4096;
 // Push + slot count of one tile of TC cells; LDS: hist[kHR * (TC + 2)],
 // codes[kCodeCap], nabs[2], and with WF (CELL32 only) the tile's field window
 // wf[6 * (TC + 1)], from which the gathers read.
-template <int L, int GENB, int TC, bool WF = false>
+//
+// SC (CELL32 with WF): the particles' cells come from the tile's cell starts
+// instead of the cell array; the tile also copies its starts to start_old
+// (the scan that follows overwrites `start`) and records the cell of every
+// particle i with i % 256 == 0 in blk_first[i / 256], from which
+// k_scatter_push finds its particles' cells.  CAP: entries of `codes`.
+template <int L, int GENB, int TC, bool WF = false, bool SC = false, int CAP = kCodeCap>
 __device__ __forceinline__ void push_tile_body(
     const ParticleBuf& b, const GridDev& g, const SpeciesDev& s, int tiles_per_row, int tile,
     const double4* __restrict__ fld, const int32_t* __restrict__ start, int32_t* __restrict__ counts,
     int32_t* __restrict__ rank, unsigned long long* __restrict__ absorbed, int* hist, int* codes,
-    unsigned long long* nabs, double* wf = nullptr)
+    unsigned long long* nabs, double* wf = nullptr, int32_t* __restrict__ start_old = nullptr,
+    int32_t* __restrict__ blk_first = nullptr)
 {
     static_assert(!WF || L == PIC_LAYOUT_CELL32, "the field window serves the CELL32 layout");
+    static_assert(!SC || WF, "cells from the starts serve the windowed CELL32 tile");
     constexpr int kHW = TC + 2;
     const int tid = threadIdx.x;
     const int r = tile / tiles_per_row;
@@ -586,12 +608,31 @@ __device__ __forceinline__ void push_tile_body(
     const int cw = min(TC, g.gx - c0);
     const int k0 = r * g.gx + c0;
     if constexpr (WF) load_field_window<TC>(fld, g, k0, cw, wf);
+    const int beg = start[k0], end = start[k0 + cw];
+    // SC: codes[p - beg] first holds particle p's cell in the tile (thread j
+    // writes cell j's particles); a thread reads the entry of a particle it
+    // owns before replacing it by the particle's code.  Particles past CAP
+    // (tiles far above the mean occupancy) search the starts.
+    if constexpr (SC)
+        for (int j = tid; j < cw; j += kBlock) {
+            const int e = min(start[k0 + j + 1] - beg, CAP);
+            for (int li = start[k0 + j] - beg; li < e; ++li) codes[li] = j;
+        }
     for (int j = tid; j < kHR * kHW; j += kBlock) hist[j] = 0;
     if (tid < 2) nabs[tid] = 0ull;
     __syncthreads();
-    const int beg = start[k0], end = start[k0 + cw];
+    // global stores after the barrier, whose fence would wait for them
+    if constexpr (SC)
+        for (int j = tid; j <= cw; j += kBlock) start_old[k0 + j] = start[k0 + j];
+    auto tile_cell = [&](int i) {
+        const int li = i - beg;
+        return k0 + (li < CAP ? codes[li] : cell_of(start + k0, cw - 1, i));
+    };
     PState<L> cur;
-    if (beg + tid < end) cur = load_state<L>(b, beg + tid);
+    if (beg + tid < end) {
+        cur = load_state<L, !SC>(b, beg + tid);
+        if constexpr (SC) cur.cell = tile_cell(beg + tid);
+    }
     for (int base = beg; base < end; base += kBlock) {
         const int i = base + tid;
         const bool have = i < end;
@@ -599,9 +640,14 @@ __device__ __forceinline__ void push_tile_body(
         if (have) {
             if constexpr (WF) cn = gather_window<TC>(cur.cell, k0, c0, r, wf);
             else cn = gather<L>(cur, g, fld);
+            if constexpr (SC)
+                if ((i & (kBlock - 1)) == 0) blk_first[i / kBlock] = cur.cell;
         }
         PState<L> nxt;
-        if (i + kBlock < end) nxt = load_state<L>(b, i + kBlock);
+        if (i + kBlock < end) {
+            nxt = load_state<L, !SC>(b, i + kBlock);
+            if constexpr (SC) nxt.cell = tile_cell(i + kBlock);
+        }
         int bin = -1, code = -1;
         if (have) {
             Pushed<L> o = push_state<L, GENB>(cur, cn, g, s);
@@ -616,7 +662,7 @@ __device__ __forceinline__ void push_tile_body(
         }
         if (bin >= 0) code = (bin << kRankBits) | atomicAdd(&hist[bin], 1);
         if (have) {
-            if (i - beg < kCodeCap) codes[i - beg] = code;
+            if (i - beg < CAP) codes[i - beg] = code;
             else rank[i] = code;
         }
         cur = nxt;
@@ -636,7 +682,7 @@ __device__ __forceinline__ void push_tile_body(
     __syncthreads();
     for (int i = beg + tid; i < end; i += kBlock) {
         const int li = i - beg;
-        const int v = li < kCodeCap ? codes[li] : rank[i];
+        const int v = li < CAP ? codes[li] : rank[i];
         if (v == -1) continue;                      // absorbed: the scatter skips it
         if (v & kFallback) rank[i] = v & ~kFallback;
         else rank[i] = hist[v >> kRankBits] + (v & ((1 << kRankBits) - 1));

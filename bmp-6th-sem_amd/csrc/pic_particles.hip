@@ -323,21 +323,29 @@ __global__ __launch_bounds__(kBlock) void k_push(
     store_state<L>(b, i, o);
 }
 
+// CELL32: the particles' cells come from the cell starts (the cell array is
+// neither read here nor written by k_scatter_push); the starts are kept in
+// start_old for the scatter, with the cell of every 256th particle in
+// blk_first.  kTileCodeCap fills the LDS left by five workgroups per CU
+// (32 KB each): the mean tile holds 256 cells x 16 particles = 4096.
+static constexpr int kTileCodeCap = 4096;
 template <int L, int GENB>
 __global__ __launch_bounds__(kBlock) void k_push_tile(
     ParticleBuf b, GridDev g, SpeciesDev s, int tiles_per_row, int ntiles, const double4* __restrict__ fld,
     const int32_t* __restrict__ start, int32_t* __restrict__ counts, int32_t* __restrict__ rank,
-    unsigned long long* __restrict__ absorbed, const int* __restrict__ guard)
+    unsigned long long* __restrict__ absorbed, const int* __restrict__ guard, int32_t* __restrict__ start_old,
+    int32_t* __restrict__ blk_first)
 {
     // a push enqueued behind a Poisson check that has not passed does nothing
     if (guard && guard[0] != 1) return;
     __shared__ int hist[kHR * (kTileC + 2)];
-    __shared__ int codes[kCodeCap];
+    __shared__ int codes[kTileCodeCap];
     __shared__ unsigned long long nabs[2];
     constexpr bool WF = L == PIC_LAYOUT_CELL32;
     __shared__ double wf[WF ? 6 * (kTileC + 1) : 1];
-    push_tile_body<L, GENB, kTileC, WF>(b, g, s, tiles_per_row, xcd_tile(blockIdx.x, ntiles), fld, start, counts,
-                                        rank, absorbed, hist, codes, nabs, wf);
+    push_tile_body<L, GENB, kTileC, WF, WF, kTileCodeCap>(b, g, s, tiles_per_row, xcd_tile(blockIdx.x, ntiles), fld,
+                                                          start, counts, rank, absorbed, hist, codes, nabs, wf,
+                                                          start_old, blk_first);
 }
 
 template <int L>
@@ -351,40 +359,52 @@ __global__ __launch_bounds__(kBlock) void k_deposit_tile(
 }
 
 // Recompute the push and write each surviving particle to its sorted slot.
-// CELL32: the workgroup's particles are cell-sorted, so their cells are the
-// flat key range [k_first, k_last]; when it is short (the usual case: ~16
-// cells for 256 particles) the corner nodes k_first .. k_last+1 and the same
-// range one node row up are staged in LDS and the gathers read them there.
+// CELL32: the workgroup's particles are cell-sorted, so their cells are a
+// flat key range from kf = blk_first[b] (the cell of its first particle) to
+// at most blk_first[b + 1] (the cell of the next workgroup's first particle).
+// When that range is short (the usual case: ~16 cells for 256 particles) the
+// corner nodes kf .. kl+1 and the same range one node row up are staged in
+// LDS with the range's cell starts (start_old: the order before this sort),
+// and each particle's cell is found among those starts; otherwise by a search
+// of start_old over [kf, G).  The cell array is neither read nor written.
 static constexpr int kScatterWin = 130;
 
 template <int L, int GENB>
 __global__ __launch_bounds__(kBlock) void k_scatter_push(
     const int32_t* __restrict__ nsrc, ParticleBuf src, ParticleBuf dst, GridDev g, SpeciesDev s,
-    const double4* __restrict__ fld, const int32_t* __restrict__ rank, const int32_t* __restrict__ start)
+    const double4* __restrict__ fld, const int32_t* __restrict__ rank, const int32_t* __restrict__ start,
+    const int32_t* __restrict__ start_old, const int32_t* __restrict__ blk_first)
 {
     const int n = *nsrc;
     // consecutive particle blocks on one XCD: the partial cache lines two
     // neighbouring blocks write at their boundary meet in one L2
-    const int i0 = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const int i0 = b * blockDim.x;
     if (i0 >= n) return;
     const int i = i0 + threadIdx.x;
     const bool have = i < n;
     PState<L> p;
-    if (have) p = load_state<L>(src, i);
+    if (have) p = load_state<L, L != PIC_LAYOUT_CELL32>(src, i);
     const int rk = have ? rank[i] : 0;
     Corners c;
     if constexpr (L == PIC_LAYOUT_CELL32) {
         __shared__ double4 wf[2 * kScatterWin];
-        const int kf = src.cell[i0], kl = src.cell[min(i0 + kBlock, n) - 1];
+        __shared__ int sst[kScatterWin];
+        const int kf = blk_first[b];
+        const int G = g.gx * g.gy;
+        const int kl = i0 + kBlock < n ? blk_first[b + 1] : G - 1;
         const int span = kl - kf + 2;                // nodes kf .. kl + 1
         const bool win = span >= 2 && span <= kScatterWin;
         if (win)
             for (int j = threadIdx.x; j < 2 * span; j += kBlock) {
                 const int row = j >= span ? 1 : 0;
-                wf[j] = fld[kf + row * g.gx + (j - row * span)];
+                const int col = j - row * span;
+                wf[j] = fld[kf + row * g.gx + col];
+                if (row == 0) sst[col] = start_old[kf + col];
             }
         __syncthreads();
         if (!have) return;
+        p.cell = kf + (win ? cell_of(sst, span - 2, i) : cell_of(start_old + kf, G - 1 - kf, i));
         if (win) {
             split_key(p.cell, g, c.cgx, c.cgy);
             const int lc = p.cell - kf;
@@ -403,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter_push(
     if (o.key < 0) return;
     const int d = start[o.key] + rk;
     store_state<L>(dst, d, o);
-    dst.cell[d] = o.key;
+    if constexpr (L != PIC_LAYOUT_CELL32) dst.cell[d] = o.key;
 }
 
 // ============================================================ launchers
@@ -431,11 +451,33 @@ int launch_encode_f64(pic_ctx* c, int s, int64_t n, const double* x, const doubl
     PIC_HIP(hipGetLastError());
     sp.n_upper = n;
     sp.deposit_pending = 0;
+    sp.cells_stale = 0;
+    return PIC_OK;
+}
+
+// Cell array of the current (sorted) storage from the cell starts: the
+// particles start[k] .. start[k+1]-1 are in cell k.
+__global__ __launch_bounds__(kBlock) void k_fill_cells(int G, const int32_t* __restrict__ start,
+                                                       int32_t* __restrict__ cell) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= G) return;
+    const int e = start[k + 1];
+    for (int i = start[k]; i < e; ++i) cell[i] = k;
+}
+
+int ensure_cells(pic_ctx* c, int s) {
+    Species& sp = c->sp[s];
+    if (!sp.cells_stale) return PIC_OK;
+    hipLaunchKernelGGL(k_fill_cells, dim3(blocks_for(c->G)), dim3(kBlock), 0, c->stream, c->G,
+                       (const int32_t*)sp.start, sp.buf[sp.cur].cell);
+    PIC_HIP(hipGetLastError());
+    sp.cells_stale = 0;
     return PIC_OK;
 }
 
 int launch_key_from_cell(pic_ctx* c, int s, int64_t n) {
     PIC_TRY(inject_settle(c));
+    PIC_TRY(ensure_cells(c, s));
     Species& sp = c->sp[s];
     if (n > 0) {
         hipLaunchKernelGGL(k_copy_i32, dim3(blocks_for(n)), dim3(kBlock), 0, c->stream,
@@ -499,6 +541,7 @@ int launch_bin(pic_ctx* c, int s) {
     }
     sp.cur = dst;
     sp.sorted = 1;
+    sp.cells_stale = 0;
     PIC_TRY(timer_end(c, PIC_STAGE_BIN, ev));
     return PIC_OK;
 }
@@ -641,6 +684,7 @@ static int deposit_corners(pic_ctx* c, int s) {
 // In-place push, no re-bin (keys left in sp.key).
 int launch_push(pic_ctx* c, int s) {
     PIC_TRY(inject_settle(c));
+    PIC_TRY(ensure_cells(c, s));
     Species& sp = c->sp[s];
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_PUSH, &ev));
@@ -674,14 +718,15 @@ static void tile_kernel(pic_ctx* c, Species& sp, const SpeciesDev& d, int tpr, u
                         const hipEvent_t kt[2], const int* guard) {
     launch_timed(k_push_tile<L, GENB>, dim3(ntiles), dim3(kBlock), c->stream, kt, sp.buf[sp.cur], c->g, d, tpr,
                  (int)ntiles, (const double4*)c->fld, (const int32_t*)sp.start, sp.counts, sp.rank, sp.absorbed,
-                 guard);
+                 guard, sp.start_old, sp.blk_first);
 }
 
 template <int L, int GENB>
 static void scatter_kernel(pic_ctx* c, Species& sp, const SpeciesDev& d, int src, int dst, const hipEvent_t kt[2]) {
     launch_timed(k_scatter_push<L, GENB>, dim3(blocks_for(sp.n_upper)), dim3(kBlock), c->stream, kt,
                  (const int32_t*)(sp.ndev + src), sp.buf[src], sp.buf[dst], c->g, d, (const double4*)c->fld,
-                 (const int32_t*)sp.rank, (const int32_t*)sp.start);
+                 (const int32_t*)sp.rank, (const int32_t*)sp.start, (const int32_t*)sp.start_old,
+                 (const int32_t*)sp.blk_first);
 }
 
 // Gather + Boris + walls + re-bin of species s: tile push/count (move_push),
@@ -749,6 +794,7 @@ int move_scatter(pic_ctx* c, int s) {
     sp.cur = dst;
     sp.deposit_pending = 0;
     sp.sorted = 1;
+    sp.cells_stale = c->p.layout == PIC_LAYOUT_CELL32 ? 1 : 0;
     PIC_TRY(timer_end(c, PIC_STAGE_SCATTER, ev));
     return PIC_OK;
 }

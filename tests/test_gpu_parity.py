@@ -203,6 +203,49 @@ def test_push_parity(hip_mod, oracle_mod, layout, bx):
     c.close()
 
 
+@pytest.mark.parametrize("n", [2000, 300000])
+def test_cell32_consecutive_pushes_from_cell_starts(hip_mod, oracle_mod, n):
+    """Three re-binning CELL32 pushes in a row with no download between them:
+    after the first, the storage's cell array is not written and the tile push
+    and the scatter take every particle's cell from the cell starts (sparse
+    storage: the scatter's search over the starts in global memory; dense: its
+    LDS range, and tiles of ~4700 particles, past the tile push's LDS table).
+    The result equals three oracle pushes, as sets, and the cell array written
+    on download equals the oracle's keys."""
+    hip, O = hip_mod, oracle_mod
+    NX = 64
+    c = _ctx(hip, NX, 1, bx=0.025)
+    rng = np.random.default_rng(21)
+    ex = rng.normal(0, 2e3, c.G)
+    ey = rng.normal(0, 2e3, c.G)
+    bz = 0.0433 + rng.normal(0, 1e-3, c.G)
+    for which, v in ((hip.GRID_EX, ex), (hip.GRID_EY, ey), (hip.GRID_BZ, bz)):
+        c.set_grid(which, v)
+    dt = c.params.dt
+    g = dict(gx=c.gx, gy=c.gy, dx=c.dx, dy=c.dy, width=c.params.width, height=c.params.height)
+    for sp, (q, m, T) in enumerate([(-E, ME, 2.0), (E, MI, 0.03)]):
+        s = c.add_species(q, m, n, 1e16, n)
+        x, y, vx, vy, vz = _parts(hip, c, n, seed=22, stream=sp, qm=abs(q / m), T=T)
+        vx, vy = vx * 20, vy * 20          # some particles cross walls and several cells
+        c.upload(s, x, y, vx, vy, vz)
+        st = O.encode_cell32(x, y, vx, vy, vz, c.gx, c.dx, c.dy)
+        absorbed = np.zeros(2, np.int64)
+        for _ in range(3):
+            c.push(s, rebin=True)
+            rc, rox, roy, rvx, rvy, rvz, key, ab = O.push_cell32(*st, ex, ey, bz, qm=q / m, dt=dt, bx=0.025, **g)
+            live = key >= 0
+            st = (rc[live], rox[live], roy[live], rvx[live], rvy[live], rvz[live])
+            absorbed += np.asarray(ab[:2], np.int64)
+        assert c.count(s) == len(st[0])
+        assert tuple(c.absorbed(s)) == tuple(absorbed)
+        counts, _, _ = O.bin(st[0], c.G)
+        assert np.array_equal(c.cell_counts(s), counts)
+        got = c.download_cell32(s)
+        assert np.all(np.diff(got[0]) >= 0)            # cell array of the sorted storage
+        assert np.array_equal(_canon(*got), _canon(*st))
+    c.close()
+
+
 @pytest.mark.parametrize("it", [0, 4])
 def test_poisson_matches_reference_pardiso(hip_mod, oracle_mod, it):
     """validation_dumps (PARDISO) at 257^2: phi to 1e-10, rho rewritten as overwriteRho does."""
