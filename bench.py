@@ -358,15 +358,24 @@ def main():
     ms_step = 1e3 * elapsed / a.steps
     log(f"gpu: {value:.3e} particle-steps/s, {ms_step:.3f} ms/step")
 
-    # roofline of the dominant particle kernel (one launch per species per
-    # step), its launch durations taken over the timed loop itself
+    # roofline of the dominant particle kernel (one launch per step serves
+    # both species), its launch durations taken over the timed loop itself
     stages = kernel_times
     dom = max(KERNEL_BYTES, key=lambda k: stages[k][0])
-    k_ms, k_n = live_times[dom] if live_times[dom][1] else stages[dom]
+    live = bool(live_times[dom][1])
+    k_ms, k_n = live_times[dom] if live else stages[dom]
     avg_s = k_ms / max(k_n, 1) * 1e-3
-    n_avg = 0.5 * (n_start + n_end) / 2           # particles per species per launch
+    n_avg = 0.5 * (n_start + n_end)                # particles of both species per step
+    nodes = (w.NX + 1) * (w.NY + 1)
+
+    def launch_bytes(kind, launches, steps):
+        # algorithmic bytes of one launch: the step's particles and both
+        # species' cells, over the launches the step makes
+        pp, pc = KERNEL_BYTES[kind][layout]
+        return (pp * n_avg + pc * 2 * nodes) * steps / max(launches, 1)
+
     per_p, per_c = KERNEL_BYTES[dom][layout]
-    kbytes = per_p * n_avg + per_c * (w.NX + 1) * (w.NY + 1)
+    kbytes = launch_bytes(dom, k_n, a.steps if live else nb)
     achieved = kbytes / avg_s / 1e9 if avg_s > 0 else None
     traffic = None
     tf = ROOT / "profiles" / f"{KERNEL_NAMES[dom]}_traffic.json"
@@ -379,13 +388,13 @@ def main():
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
             "bytes_per_launch": kbytes, "bytes_per_particle": per_p, "bytes_per_cell": per_c,
             "avg_launch_us": avg_s * 1e6, "launches_timed": k_n,
-            "timed_over": "the timed loop" if live_times[dom][1] else "separate steps after the timed loop"}
+            "launches_per_step": k_n / (a.steps if live else nb),
+            "timed_over": "the timed loop" if live else "separate steps after the timed loop"}
     others = {}
     for k in KERNEL_BYTES:
         ms_k, n_k = stages[k]
         if n_k:
-            pp, pc = KERNEL_BYTES[k][layout]
-            b = pp * n_avg + pc * (w.NX + 1) * (w.NY + 1)
+            b = launch_bytes(k, n_k, nb)
             others[KERNEL_NAMES[k]] = {"avg_launch_us": round(ms_k / n_k * 1e3, 2),
                                        "GB_s": round(b / (ms_k / n_k * 1e-3) / 1e9, 1)}
 

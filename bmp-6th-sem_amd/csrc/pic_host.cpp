@@ -726,51 +726,46 @@ extern "C" int pic_step(pic_ctx* c, int iteration, pic_solve_stats* st) {
         set_error("pic_step: null context");
         return PIC_ERR_ARG;
     }
-    PIC_TRY(launch_deposit_all(c));
-    // Poisson and E; a direct solve's check is still in flight: the first
-    // push is enqueued behind it (it does nothing unless the check passed), so
+    PIC_TRY(launch_deposit_all(c));   // both species' tiles in one launch
+    // Poisson and E; a direct solve's check is still in flight: the pushes
+    // are enqueued behind it (they do nothing unless the check passed), so
     // the device has work while the host waits for the check
     const int ns = (int)c->sp.size();
     PIC_TRY(poisson_solve(c, iteration, st, 1));
     const bool early = c->tail.active && ns > 0;
-    if (early) PIC_TRY(move_push(c, 0, c->flags));
+    if (early) PIC_TRY(move_push_all(c, c->flags));
     int redo = 0;
     PIC_TRY(poisson_finish(c, st, &redo));
-    if (early && redo) PIC_TRY(move_push(c, 0));   // the PCG corrected phi and E: push again
+    // the PCG corrected phi and E (push again), or there was no check
+    if (!early || redo) PIC_TRY(move_push_all(c));
     if (!c->inj.on) {
         if (ns == 2) {
-            // both pushes, both scans in one set of launches, both scatters
-            if (!early) PIC_TRY(move_push(c, 0));
-            PIC_TRY(move_push(c, 1));
+            // both scans in one set of launches, both scatters in one launch
             PIC_TRY(move_scan_pair(c));
-            PIC_TRY(move_scatter(c, 0));
-            return move_scatter(c, 1);
+            return move_scatter_pair(c);
         }
-        for (int s = 0; s < ns; ++s) {
-            if (s == 0 && early) PIC_TRY(move_finish(c, 0));
-            else PIC_TRY(launch_move(c, s));
-        }
+        for (int s = 0; s < ns; ++s) PIC_TRY(move_finish(c, s));
         return PIC_OK;
     }
-    // with injection (main.cpp:232-245): push every species, stage the new
-    // particles (their count depends on both species' exits), count them into
-    // the cells, then scan and scatter pushed and new particles together
-    for (int s = early ? 1 : 0; s < ns; ++s) PIC_TRY(move_push(c, s));
+    // with injection (main.cpp:232-245): after the pushes of every species,
+    // stage the new particles (their count depends on both species' exits),
+    // count them into the cells, then scan and scatter pushed and new
+    // particles together
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_BIN, &ev));
     PIC_TRY(inject_generate(c));
     PIC_TRY(inject_count(c, 0));
     PIC_TRY(inject_count(c, 1));
     PIC_TRY(timer_end(c, PIC_STAGE_BIN, ev));
-    if (ns == 2) PIC_TRY(move_scan_pair(c));
-    for (int s = 0; s < ns; ++s) {
-        PIC_TRY(ns == 2 ? move_scatter(c, s) : move_finish(c, s));
-        if (s < 2) {
-            PIC_TRY(timer_begin(c, PIC_STAGE_SCATTER, &ev));
-            PIC_TRY(inject_scatter(c, s, c->sp[s].cur));
-            PIC_TRY(timer_end(c, PIC_STAGE_SCATTER, ev));
-        }
+    if (ns == 2) {
+        PIC_TRY(move_scan_pair(c));
+        PIC_TRY(move_scatter_pair(c));
+    } else {
+        for (int s = 0; s < ns; ++s) PIC_TRY(move_finish(c, s));
     }
+    PIC_TRY(timer_begin(c, PIC_STAGE_SCATTER, &ev));
+    for (int s = 0; s < std::min(ns, 2); ++s) PIC_TRY(inject_scatter(c, s, c->sp[s].cur));
+    PIC_TRY(timer_end(c, PIC_STAGE_SCATTER, ev));
     return inject_publish(c);
 }
 

@@ -218,9 +218,11 @@ int launch_move(pic_ctx* c, int s);
 // launch_move in phases, for injection between them; with `guard`, the push
 // does nothing unless guard[0] == 1 when it runs (a Poisson check passed)
 int move_push(pic_ctx* c, int s, const int* guard = nullptr);
+int move_push_all(pic_ctx* c, const int* guard = nullptr);   // every species, two per launch
 int move_finish(pic_ctx* c, int s);              // scan + scatter of species s
 int move_scan_pair(pic_ctx* c);                   // the scans of species 0 and 1 in one set of launches
 int move_scatter(pic_ctx* c, int s);              // the scatter after the scan
+int move_scatter_pair(pic_ctx* c);                // the scatters of species 0 and 1 in one launch
 int launch_zero_f64(pic_ctx* c, double* p, int64_t n);
 // CELL32: write buf[cur].cell from `start` when the step left it unwritten
 int ensure_cells(pic_ctx* c, int s);

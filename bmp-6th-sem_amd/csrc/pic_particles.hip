@@ -323,18 +323,54 @@ __global__ __launch_bounds__(kBlock) void k_push(
     store_state<L>(b, i, o);
 }
 
+// The particle kernels of the step take one or two species per launch (a
+// species set): workgroup t of the launch serves species t % ns, so both
+// species' tiles of one key range are dealt to the same XCD (their field rows
+// meet in one L2) and the launch drains once for both.
+static constexpr int kSetMax = 2;
+
+struct TileSet {
+    int ns;
+    ParticleBuf b[kSetMax];
+    SpeciesDev d[kSetMax];
+    const int32_t* start[kSetMax];
+    int32_t* counts[kSetMax];
+    int32_t* rank[kSetMax];
+    unsigned long long* absorbed[kSetMax];
+    int32_t* start_old[kSetMax];
+    int32_t* blk_first[kSetMax];
+};
+
+struct DepositSet {
+    int ns;
+    ParticleBuf b[kSetMax];
+    DepositArgs a[kSetMax];
+    const int32_t* start[kSetMax];
+    long long* cq[kSetMax];
+    long long* ce[kSetMax];
+};
+
+struct ScatterSet {
+    int ns;
+    int nb0;                         // workgroups of species 0
+    const int32_t* nsrc[kSetMax];
+    ParticleBuf src[kSetMax], dst[kSetMax];
+    SpeciesDev d[kSetMax];
+    const int32_t* rank[kSetMax];
+    const int32_t* start[kSetMax];
+    const int32_t* start_old[kSetMax];
+    const int32_t* blk_first[kSetMax];
+};
+
 // CELL32: the particles' cells come from the cell starts (the cell array is
 // neither read here nor written by k_scatter_push); the starts are kept in
 // start_old for the scatter, with the cell of every 256th particle in
-// blk_first.  kTileCodeCap fills the LDS left by five workgroups per CU
-// (32 KB each): the mean tile holds 256 cells x 16 particles = 4096.
+// blk_first.  kTileCodeCap: the mean tile holds 256 cells x 16 particles =
+// 4096, and the workgroup's LDS stays within a fifth of the CU's.
 static constexpr int kTileCodeCap = 4096;
 template <int L, int GENB>
-__global__ __launch_bounds__(kBlock) void k_push_tile(
-    ParticleBuf b, GridDev g, SpeciesDev s, int tiles_per_row, int ntiles, const double4* __restrict__ fld,
-    const int32_t* __restrict__ start, int32_t* __restrict__ counts, int32_t* __restrict__ rank,
-    unsigned long long* __restrict__ absorbed, const int* __restrict__ guard, int32_t* __restrict__ start_old,
-    int32_t* __restrict__ blk_first)
+__global__ __launch_bounds__(kBlock) void k_push_tile(TileSet ts, GridDev g, int tiles_per_row, int ntiles,
+                                                      const double4* __restrict__ fld, const int* __restrict__ guard)
 {
     // a push enqueued behind a Poisson check that has not passed does nothing
     if (guard && guard[0] != 1) return;
@@ -343,19 +379,21 @@ __global__ __launch_bounds__(kBlock) void k_push_tile(
     __shared__ unsigned long long nabs[2];
     constexpr bool WF = L == PIC_LAYOUT_CELL32;
     __shared__ double wf[WF ? 6 * (kTileC + 1) : 1];
-    push_tile_body<L, GENB, kTileC, WF, WF, kTileCodeCap>(b, g, s, tiles_per_row, xcd_tile(blockIdx.x, ntiles), fld,
-                                                          start, counts, rank, absorbed, hist, codes, nabs, wf,
-                                                          start_old, blk_first);
+    const int t = xcd_tile(blockIdx.x, ts.ns * ntiles);
+    const int sp = t % ts.ns, tile = t / ts.ns;
+    push_tile_body<L, GENB, kTileC, WF, WF, kTileCodeCap>(ts.b[sp], g, ts.d[sp], tiles_per_row, tile, fld,
+                                                          ts.start[sp], ts.counts[sp], ts.rank[sp], ts.absorbed[sp],
+                                                          hist, codes, nabs, wf, ts.start_old[sp], ts.blk_first[sp]);
 }
 
 template <int L>
-__global__ __launch_bounds__(kBlock) void k_deposit_tile(
-    ParticleBuf b, GridDev g, DepositArgs a, int tiles_per_row, int ntiles, const int32_t* __restrict__ start,
-    long long* __restrict__ cq, long long* __restrict__ ce)
+__global__ __launch_bounds__(kBlock) void k_deposit_tile(DepositSet ds, GridDev g, int tiles_per_row, int ntiles)
 {
     __shared__ unsigned long long acc[8][kDepC];
     __shared__ int sst[kDepC + 1];
-    deposit_tile_body<L>(b, g, a, tiles_per_row, xcd_tile(blockIdx.x, ntiles), start, cq, ce, acc, sst);
+    const int t = xcd_tile(blockIdx.x, ds.ns * ntiles);
+    const int sp = t % ds.ns, tile = t / ds.ns;
+    deposit_tile_body<L>(ds.b[sp], g, ds.a[sp], tiles_per_row, tile, ds.start[sp], ds.cq[sp], ds.ce[sp], acc, sst);
 }
 
 // Recompute the push and write each surviving particle to its sorted slot.
@@ -367,29 +405,32 @@ __global__ __launch_bounds__(kBlock) void k_deposit_tile(
 // LDS with the range's cell starts (start_old: the order before this sort),
 // and each particle's cell is found among those starts; otherwise by a search
 // of start_old over [kf, G).  The cell array is neither read nor written.
+// Workgroups 0 .. nb0-1 (in XCD order) serve species 0, the rest species 1.
 static constexpr int kScatterWin = 130;
 
 template <int L, int GENB>
-__global__ __launch_bounds__(kBlock) void k_scatter_push(
-    const int32_t* __restrict__ nsrc, ParticleBuf src, ParticleBuf dst, GridDev g, SpeciesDev s,
-    const double4* __restrict__ fld, const int32_t* __restrict__ rank, const int32_t* __restrict__ start,
-    const int32_t* __restrict__ start_old, const int32_t* __restrict__ blk_first)
+__global__ __launch_bounds__(kBlock) void k_scatter_push(ScatterSet ss, GridDev g, const double4* __restrict__ fld)
 {
-    const int n = *nsrc;
     // consecutive particle blocks on one XCD: the partial cache lines two
     // neighbouring blocks write at their boundary meet in one L2
-    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int sp = t < ss.nb0 ? 0 : 1;
+    const int b = t - sp * ss.nb0;
+    const ParticleBuf src = ss.src[sp];
+    const int n = *ss.nsrc[sp];
     const int i0 = b * blockDim.x;
     if (i0 >= n) return;
     const int i = i0 + threadIdx.x;
     const bool have = i < n;
     PState<L> p;
     if (have) p = load_state<L, L != PIC_LAYOUT_CELL32>(src, i);
-    const int rk = have ? rank[i] : 0;
+    const int rk = have ? ss.rank[sp][i] : 0;
     Corners c;
     if constexpr (L == PIC_LAYOUT_CELL32) {
         __shared__ double4 wf[2 * kScatterWin];
         __shared__ int sst[kScatterWin];
+        const int32_t* __restrict__ start_old = ss.start_old[sp];
+        const int32_t* __restrict__ blk_first = ss.blk_first[sp];
         const int kf = blk_first[b];
         const int G = g.gx * g.gy;
         const int kl = i0 + kBlock < n ? blk_first[b + 1] : G - 1;
@@ -419,11 +460,11 @@ __global__ __launch_bounds__(kBlock) void k_scatter_push(
         if (!have) return;
         c = gather<L>(p, g, fld);
     }
-    const Pushed<L> o = push_state<L, GENB>(p, c, g, s);
+    const Pushed<L> o = push_state<L, GENB>(p, c, g, ss.d[sp]);
     if (o.key < 0) return;
-    const int d = start[o.key] + rk;
-    store_state<L>(dst, d, o);
-    if constexpr (L != PIC_LAYOUT_CELL32) dst.cell[d] = o.key;
+    const int d = ss.start[sp][o.key] + rk;
+    store_state<L>(ss.dst[sp], d, o);
+    if constexpr (L != PIC_LAYOUT_CELL32) ss.dst[sp].cell[d] = o.key;
 }
 
 // ============================================================ launchers
@@ -594,6 +635,7 @@ static int finalize_deposit(pic_ctx* c, Species& sp) {
 // per-cell corner sums on cell-sorted storage, per-particle atomics
 // otherwise, then node sums into final_mesh / energy_mesh and rho.
 static int deposit_corners(pic_ctx* c, int s);
+static int deposit_corners_set(pic_ctx* c, const int* ids, int n);
 
 int launch_deposit(pic_ctx* c, int s) {
     PIC_TRY(deposit_corners(c, s));
@@ -613,7 +655,10 @@ int launch_deposit_all(pic_ctx* c) {
         for (int s = 0; s < ns; ++s) PIC_TRY(launch_deposit(c, s));
         return PIC_OK;
     }
-    for (int s = 0; s < ns; ++s) PIC_TRY(deposit_corners(c, s));
+    for (int s0 = 0; s0 < ns; s0 += kSetMax) {
+        const int ids[kSetMax] = {s0, s0 + 1};
+        PIC_TRY(deposit_corners_set(c, ids, std::min(kSetMax, ns - s0)));
+    }
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_FINALIZE, &ev));
     FinalizeAll a{};
@@ -636,26 +681,24 @@ int launch_deposit_all(pic_ctx* c) {
     return PIC_OK;
 }
 
-static int deposit_corners(pic_ctx* c, int s) {
-    Species& sp = c->sp[s];
+// Charge (and energy) corner sums of the species in ids[0 .. n-1]: the
+// sorted ones by one tile launch for all of them, the others (storage left
+// unsorted by an in-place push) by per-particle atomics.
+static int deposit_corners_set(pic_ctx* c, const int* ids, int n) {
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_DEPOSIT, &ev));
-    DepositArgs a = deposit_args(c, sp);
-    if (!sp.deposit_pending) {
+    DepositSet ds{};
+    for (int k = 0; k < n; ++k) {
+        Species& sp = c->sp[ids[k]];
+        if (sp.deposit_pending) continue;
+        DepositArgs a = deposit_args(c, sp);
         if (sp.sorted) {
-            const int tpr = (c->g.gx + kDepC - 1) / kDepC;
-            const unsigned ntiles = (unsigned)(tpr * c->g.gy);
-            hipEvent_t kt[2];
-            PIC_TRY(kt_begin(c, kt, PIC_STAGE_DEPOSIT));
-            if (c->p.layout == PIC_LAYOUT_F64)
-                launch_timed(k_deposit_tile<PIC_LAYOUT_F64>, dim3(ntiles), dim3(kBlock), c->stream, kt,
-                             sp.buf[sp.cur], c->g, a, tpr, (int)ntiles, (const int32_t*)sp.start,
-                             (long long*)sp.q_charge, (long long*)sp.q_energy);
-            else
-                launch_timed(k_deposit_tile<PIC_LAYOUT_CELL32>, dim3(ntiles), dim3(kBlock), c->stream, kt,
-                             sp.buf[sp.cur], c->g, a, tpr, (int)ntiles, (const int32_t*)sp.start,
-                             (long long*)sp.q_charge, (long long*)sp.q_energy);
-            kt_end(c, PIC_STAGE_DEPOSIT, kt);
+            const int j = ds.ns++;
+            ds.b[j] = sp.buf[sp.cur];
+            ds.a[j] = a;
+            ds.start[j] = sp.start;
+            ds.cq[j] = (long long*)sp.q_charge;
+            ds.ce[j] = (long long*)sp.q_energy;
         } else {
             PIC_TRY(inject_settle(c));   // the per-particle grid needs the live-count bound
             const int64_t nacc = 4 * (int64_t)c->g.gp;
@@ -673,12 +716,29 @@ static int deposit_corners(pic_ctx* c, int s) {
                                        dim3(kBlock), 0, c->stream, sp.ndev + sp.cur, sp.buf[sp.cur], c->g, a,
                                        (unsigned long long*)sp.q_charge, (unsigned long long*)sp.q_energy);
             }
+            PIC_HIP(hipGetLastError());
         }
-        PIC_HIP(hipGetLastError());
         sp.deposit_pending = 1;
+    }
+    if (ds.ns > 0) {
+        const int tpr = (c->g.gx + kDepC - 1) / kDepC;
+        const int ntiles = tpr * c->g.gy;
+        const dim3 grid((unsigned)(ds.ns * ntiles));
+        hipEvent_t kt[2];
+        PIC_TRY(kt_begin(c, kt, PIC_STAGE_DEPOSIT));
+        if (c->p.layout == PIC_LAYOUT_F64)
+            launch_timed(k_deposit_tile<PIC_LAYOUT_F64>, grid, dim3(kBlock), c->stream, kt, ds, c->g, tpr, ntiles);
+        else
+            launch_timed(k_deposit_tile<PIC_LAYOUT_CELL32>, grid, dim3(kBlock), c->stream, kt, ds, c->g, tpr, ntiles);
+        PIC_HIP(hipGetLastError());
+        kt_end(c, PIC_STAGE_DEPOSIT, kt);
     }
     PIC_TRY(timer_end(c, PIC_STAGE_DEPOSIT, ev));
     return PIC_OK;
+}
+
+static int deposit_corners(pic_ctx* c, int s) {
+    return deposit_corners_set(c, &s, 1);
 }
 
 // In-place push, no re-bin (keys left in sp.key).
@@ -714,46 +774,69 @@ int launch_push(pic_ctx* c, int s) {
 }
 
 template <int L, int GENB>
-static void tile_kernel(pic_ctx* c, Species& sp, const SpeciesDev& d, int tpr, unsigned ntiles,
-                        const hipEvent_t kt[2], const int* guard) {
-    launch_timed(k_push_tile<L, GENB>, dim3(ntiles), dim3(kBlock), c->stream, kt, sp.buf[sp.cur], c->g, d, tpr,
-                 (int)ntiles, (const double4*)c->fld, (const int32_t*)sp.start, sp.counts, sp.rank, sp.absorbed,
-                 guard, sp.start_old, sp.blk_first);
+static void tile_kernel(pic_ctx* c, const TileSet& ts, int tpr, int ntiles, const hipEvent_t kt[2],
+                        const int* guard) {
+    launch_timed(k_push_tile<L, GENB>, dim3((unsigned)(ts.ns * ntiles)), dim3(kBlock), c->stream, kt, ts, c->g, tpr,
+                 ntiles, (const double4*)c->fld, guard);
 }
 
 template <int L, int GENB>
-static void scatter_kernel(pic_ctx* c, Species& sp, const SpeciesDev& d, int src, int dst, const hipEvent_t kt[2]) {
-    launch_timed(k_scatter_push<L, GENB>, dim3(blocks_for(sp.n_upper)), dim3(kBlock), c->stream, kt,
-                 (const int32_t*)(sp.ndev + src), sp.buf[src], sp.buf[dst], c->g, d, (const double4*)c->fld,
-                 (const int32_t*)sp.rank, (const int32_t*)sp.start, (const int32_t*)sp.start_old,
-                 (const int32_t*)sp.blk_first);
+static void scatter_kernel(pic_ctx* c, const ScatterSet& ss, unsigned grid, const hipEvent_t kt[2]) {
+    launch_timed(k_scatter_push<L, GENB>, dim3(grid), dim3(kBlock), c->stream, kt, ss, c->g, (const double4*)c->fld);
 }
 
-// Gather + Boris + walls + re-bin of species s: tile push/count (move_push),
-// prefix scan and recomputing scatter into the other buffer (move_finish).
-// Injection runs between the two phases: its particles are counted into the
-// same cells before the scan.
-int move_push(pic_ctx* c, int s, const int* guard) {
+// Gather + Boris + walls + re-bin of the species in ids[0 .. n-1]: tile
+// push/count in one launch (move_push_set), prefix scan and recomputing
+// scatter into the other buffer (move_scatter_set).  Injection runs between
+// the two phases: its particles are counted into the same cells before the
+// scan.
+int move_push_set(pic_ctx* c, const int* ids, int n, const int* guard) {
     PIC_TRY(inject_settle(c));
-    Species& sp = c->sp[s];
-    if (!sp.sorted) PIC_TRY(launch_bin(c, s));   // tiles need cell-sorted storage
-    SpeciesDev d = species_dev(c, sp);
+    TileSet ts{};
+    for (int k = 0; k < n; ++k) {
+        Species& sp = c->sp[ids[k]];
+        if (!sp.sorted) PIC_TRY(launch_bin(c, ids[k]));   // tiles need cell-sorted storage
+        if (sp.n_upper == 0) continue;
+        const int j = ts.ns++;
+        ts.b[j] = sp.buf[sp.cur];
+        ts.d[j] = species_dev(c, sp);
+        ts.start[j] = sp.start;
+        ts.counts[j] = sp.counts;
+        ts.rank[j] = sp.rank;
+        ts.absorbed[j] = sp.absorbed;
+        ts.start_old[j] = sp.start_old;
+        ts.blk_first[j] = sp.blk_first;
+    }
     const int tpr = tiles_per_row(c);
-    const unsigned ntiles = (unsigned)(tpr * c->g.gy);
+    const int ntiles = tpr * c->g.gy;
     const bool f64 = c->p.layout == PIC_LAYOUT_F64;
+    const bool genb = c->p.bx != 0.0 || c->p.by != 0.0;
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_PUSH, &ev));
-    if (sp.n_upper > 0) {
+    if (ts.ns > 0) {
         hipEvent_t kt[2];
         PIC_TRY(kt_begin(c, kt, PIC_STAGE_PUSH));
-        if (f64 && !d.general_b) tile_kernel<PIC_LAYOUT_F64, 0>(c, sp, d, tpr, ntiles, kt, guard);
-        else if (f64) tile_kernel<PIC_LAYOUT_F64, 1>(c, sp, d, tpr, ntiles, kt, guard);
-        else if (!d.general_b) tile_kernel<PIC_LAYOUT_CELL32, 0>(c, sp, d, tpr, ntiles, kt, guard);
-        else tile_kernel<PIC_LAYOUT_CELL32, 1>(c, sp, d, tpr, ntiles, kt, guard);
+        if (f64 && !genb) tile_kernel<PIC_LAYOUT_F64, 0>(c, ts, tpr, ntiles, kt, guard);
+        else if (f64) tile_kernel<PIC_LAYOUT_F64, 1>(c, ts, tpr, ntiles, kt, guard);
+        else if (!genb) tile_kernel<PIC_LAYOUT_CELL32, 0>(c, ts, tpr, ntiles, kt, guard);
+        else tile_kernel<PIC_LAYOUT_CELL32, 1>(c, ts, tpr, ntiles, kt, guard);
         PIC_HIP(hipGetLastError());
         kt_end(c, PIC_STAGE_PUSH, kt);
     }
     PIC_TRY(timer_end(c, PIC_STAGE_PUSH, ev));
+    return PIC_OK;
+}
+
+int move_push(pic_ctx* c, int s, const int* guard) {
+    return move_push_set(c, &s, 1, guard);
+}
+
+int move_push_all(pic_ctx* c, const int* guard) {
+    const int ns = (int)c->sp.size();
+    for (int s0 = 0; s0 < ns; s0 += kSetMax) {
+        const int ids[kSetMax] = {s0, s0 + 1};
+        PIC_TRY(move_push_set(c, ids, std::min(kSetMax, ns - s0), guard));
+    }
     return PIC_OK;
 }
 
@@ -774,29 +857,56 @@ int move_scan_pair(pic_ctx* c) {
     return PIC_OK;
 }
 
-int move_scatter(pic_ctx* c, int s) {
-    Species& sp = c->sp[s];
-    SpeciesDev d = species_dev(c, sp);
+int move_scatter_set(pic_ctx* c, const int* ids, int n) {
+    ScatterSet ss{};
+    unsigned nb[kSetMax] = {0, 0};
+    for (int k = 0; k < n; ++k) {
+        Species& sp = c->sp[ids[k]];
+        const int src = sp.cur, dst = sp.cur ^ 1;
+        if (sp.n_upper > 0) {
+            const int j = ss.ns++;
+            nb[j] = blocks_for(sp.n_upper);
+            ss.nsrc[j] = sp.ndev + src;
+            ss.src[j] = sp.buf[src];
+            ss.dst[j] = sp.buf[dst];
+            ss.d[j] = species_dev(c, sp);
+            ss.rank[j] = sp.rank;
+            ss.start[j] = sp.start;
+            ss.start_old[j] = sp.start_old;
+            ss.blk_first[j] = sp.blk_first;
+        }
+        sp.cur = dst;
+        sp.deposit_pending = 0;
+        sp.sorted = 1;
+        sp.cells_stale = c->p.layout == PIC_LAYOUT_CELL32 ? 1 : 0;
+    }
+    ss.nb0 = (int)nb[0];
     const bool f64 = c->p.layout == PIC_LAYOUT_F64;
-    const int src = sp.cur, dst = sp.cur ^ 1;
+    const bool genb = c->p.bx != 0.0 || c->p.by != 0.0;
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_SCATTER, &ev));
-    if (sp.n_upper > 0) {
+    if (ss.ns > 0) {
+        const unsigned grid = nb[0] + nb[1];
         hipEvent_t kt[2];
         PIC_TRY(kt_begin(c, kt, PIC_STAGE_SCATTER));
-        if (f64 && !d.general_b) scatter_kernel<PIC_LAYOUT_F64, 0>(c, sp, d, src, dst, kt);
-        else if (f64) scatter_kernel<PIC_LAYOUT_F64, 1>(c, sp, d, src, dst, kt);
-        else if (!d.general_b) scatter_kernel<PIC_LAYOUT_CELL32, 0>(c, sp, d, src, dst, kt);
-        else scatter_kernel<PIC_LAYOUT_CELL32, 1>(c, sp, d, src, dst, kt);
+        if (f64 && !genb) scatter_kernel<PIC_LAYOUT_F64, 0>(c, ss, grid, kt);
+        else if (f64) scatter_kernel<PIC_LAYOUT_F64, 1>(c, ss, grid, kt);
+        else if (!genb) scatter_kernel<PIC_LAYOUT_CELL32, 0>(c, ss, grid, kt);
+        else scatter_kernel<PIC_LAYOUT_CELL32, 1>(c, ss, grid, kt);
         PIC_HIP(hipGetLastError());
         kt_end(c, PIC_STAGE_SCATTER, kt);
     }
-    sp.cur = dst;
-    sp.deposit_pending = 0;
-    sp.sorted = 1;
-    sp.cells_stale = c->p.layout == PIC_LAYOUT_CELL32 ? 1 : 0;
     PIC_TRY(timer_end(c, PIC_STAGE_SCATTER, ev));
     return PIC_OK;
+}
+
+int move_scatter(pic_ctx* c, int s) {
+    return move_scatter_set(c, &s, 1);
+}
+
+int move_scatter_pair(pic_ctx* c) {
+    const int ids[2] = {0, 1};
+    return move_scatter_set(c, ids, 2);
 }
 
 int launch_move(pic_ctx* c, int s) {

@@ -587,9 +587,9 @@ def test_step_push_behind_the_poisson_check(hip_mod, tol):
 
 
 def test_timing_modes_count_their_launches(hip_mod):
-    """Mode 2 times the tile push, the scatter and the tile deposit once per
-    species and step; mode 3 the scatter alone; mode 1 every stage; mode 0
-    nothing.  A mode outside 0..3 is rejected."""
+    """Mode 2 times the tile push, the scatter and the tile deposit, each one
+    launch per step for both species; mode 3 the scatter alone; mode 1 every
+    stage; mode 0 nothing.  A mode outside 0..3 is rejected."""
     hip = hip_mod
     c = _ctx(hip, 64, 1, geom=ELEC)
     for sp, (q, m, T) in enumerate([(-E, ME, 2.0), (E, MI, 0.03)]):
@@ -611,11 +611,11 @@ def test_timing_modes_count_their_launches(hip_mod):
     assert all(n == 0 and ms == 0 for ms, n in seen[0].values())
     for k, (ms, n) in seen[2].items():
         if k in ("push", "scatter", "deposit"):
-            assert n == 2 * steps and ms > 0, (k, ms, n)
+            assert n == steps and ms > 0, (k, ms, n)
         else:
             assert n == 0, (k, n)
     for k, (ms, n) in seen[3].items():
-        assert n == (2 * steps if k == "scatter" else 0), (k, n)
+        assert n == (steps if k == "scatter" else 0), (k, n)
         assert (ms > 0) == (k == "scatter"), (k, ms)
     assert seen[1]["poisson"][1] >= steps and seen[1]["deposit"][1] >= steps
     with pytest.raises(hip.PicError):
