@@ -726,7 +726,8 @@ extern "C" int pic_step(pic_ctx* c, int iteration, pic_solve_stats* st) {
         set_error("pic_step: null context");
         return PIC_ERR_ARG;
     }
-    PIC_TRY(launch_deposit_all(c));   // both species' tiles in one launch
+    // both species' tiles in one launch; their node pass is the solve's first kernel
+    PIC_TRY(launch_deposit_all(c, true));
     // Poisson and E; a direct solve's check is still in flight: the pushes
     // are enqueued behind it (they do nothing unless the check passed), so
     // the device has work while the host waits for the check

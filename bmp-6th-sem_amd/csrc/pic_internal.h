@@ -72,6 +72,45 @@ struct ParticleBuf {
     int32_t* cell = nullptr; // bin key gy*GRID_X + gx
 };
 
+// Node pass of the deposit for all species at once (at most kFinMax):
+// final_s = corner sums x cqf_s, energy_s likewise, and
+// rho = sum_s final_s * chrg_s in species order (the sum a zeroed rho
+// accumulates species by species).
+static constexpr int kFinMax = 4;
+struct FinalizeAll {
+    int ns;
+    const long long* cq[kFinMax];
+    const long long* ce[kFinMax];
+    double cqf[kFinMax], cef[kFinMax], chrg[kFinMax];
+    double* fm[kFinMax];
+    double* em[kFinMax];
+};
+
+#ifdef __HIPCC__
+// final_mesh / energy_mesh of node n written, rho at n returned; corner c of
+// the corner meshes sits at c * gp + n.  The products are rounded before the
+// sum (no fused multiply-add) in every file that includes this.
+__device__ __forceinline__ double finalize_node(const FinalizeAll& a, int n, int gp, int energy) {
+#pragma clang fp contract(off)
+    double r = 0.0;
+#pragma unroll
+    for (int s = 0; s < kFinMax; ++s) {
+        if (s >= a.ns) break;
+        const long long* cq = a.cq[s];
+        long long q = cq[n] + cq[gp + n] + cq[2 * gp + n] + cq[3 * gp + n];
+        double f = (double)q * a.cqf[s];
+        a.fm[s][n] = f;
+        r += f * a.chrg[s];
+        if (energy) {
+            const long long* ce = a.ce[s];
+            long long e = ce[n] + ce[gp + n] + ce[2 * gp + n] + ce[3 * gp + n];
+            a.em[s][n] = (double)e * a.cef[s];
+        }
+    }
+    return r;
+}
+#endif
+
 struct Species {
     double charge = 0, mass = 0, qm = 0, W = 0, sim = 0, actual = 0;
     int64_t capacity = 0;
@@ -203,6 +242,8 @@ struct pic_ctx {
     std::vector<hipEvent_t> event_pool;   // recorded-and-read stage events, reused
     hipEvent_t solve_ev[2] = {nullptr, nullptr};   // start / end of a Poisson solve
     pic::SolveTail tail;                             // a deferred solve end
+    pic::FinalizeAll fin{};                          // a deposit node pass left to the next solve
+    int fin_pending = 0;
 };
 
 namespace pic {
@@ -212,7 +253,9 @@ int launch_encode_f64(pic_ctx* c, int s, int64_t n, const double* x, const doubl
                       const double* vx, const double* vy, const double* vz);
 int launch_bin(pic_ctx* c, int s);
 int launch_deposit(pic_ctx* c, int s);            // adds species s to rho
-int launch_deposit_all(pic_ctx* c);               // every species; writes rho
+// every species; writes rho, or with defer_finalize leaves the node pass to
+// the next Poisson solve (c->fin)
+int launch_deposit_all(pic_ctx* c, bool defer_finalize = false);
 int launch_push(pic_ctx* c, int s);
 int launch_move(pic_ctx* c, int s);
 // launch_move in phases, for injection between them; with `guard`, the push

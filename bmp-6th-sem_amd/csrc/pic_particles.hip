@@ -268,39 +268,12 @@ __global__ __launch_bounds__(kBlock) void k_deposit_finalize(
     }
 }
 
-// All species at once (at most kFinMax): rho = sum_s final_s * chrg_s in
-// species order (the sum a zeroed rho accumulates species by species).
-static constexpr int kFinMax = 4;
-struct FinalizeAll {
-    int ns;
-    const long long* cq[kFinMax];
-    const long long* ce[kFinMax];
-    double cqf[kFinMax], cef[kFinMax], chrg[kFinMax];
-    double* fm[kFinMax];
-    double* em[kFinMax];
-};
-
 __global__ __launch_bounds__(kBlock) void k_deposit_finalize_all(int G, int gp, FinalizeAll a, int energy,
                                                                  double* __restrict__ rho)
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= G) return;
-    double r = 0.0;
-#pragma unroll
-    for (int s = 0; s < kFinMax; ++s) {
-        if (s >= a.ns) break;
-        const long long* cq = a.cq[s];
-        long long q = cq[n] + cq[gp + n] + cq[2 * gp + n] + cq[3 * gp + n];
-        double f = (double)q * a.cqf[s];
-        a.fm[s][n] = f;
-        r += f * a.chrg[s];
-        if (energy) {
-            const long long* ce = a.ce[s];
-            long long e = ce[n] + ce[gp + n] + ce[2 * gp + n] + ce[3 * gp + n];
-            a.em[s][n] = (double)e * a.cef[s];
-        }
-    }
-    rho[n] = r;
+    rho[n] = finalize_node(a, n, gp, energy);
 }
 
 // ================================================================= push
@@ -648,7 +621,7 @@ int launch_deposit(pic_ctx* c, int s) {
 
 // Every species' deposit, and rho written (not accumulated) by one
 // node pass over all of them.
-int launch_deposit_all(pic_ctx* c) {
+int launch_deposit_all(pic_ctx* c, bool defer_finalize) {
     const int ns = (int)c->sp.size();
     if (ns == 0 || ns > kFinMax) {
         PIC_TRY(launch_zero_f64(c, c->rho, c->G));
@@ -674,9 +647,15 @@ int launch_deposit_all(pic_ctx* c) {
         a.fm[s] = sp.final_mesh;
         a.em[s] = sp.energy_mesh;
     }
-    hipLaunchKernelGGL(k_deposit_finalize_all, dim3(blocks_for(c->G)), dim3(kBlock), 0, c->stream, c->G, c->g.gp, a,
-                       c->p.deposit_energy, c->rho);
-    PIC_HIP(hipGetLastError());
+    if (defer_finalize) {
+        // the next Poisson solve's first kernel finalizes (and overwrites) rho
+        c->fin = a;
+        c->fin_pending = 1;
+    } else {
+        hipLaunchKernelGGL(k_deposit_finalize_all, dim3(blocks_for(c->G)), dim3(kBlock), 0, c->stream, c->G, c->g.gp,
+                           a, c->p.deposit_energy, c->rho);
+        PIC_HIP(hipGetLastError());
+    }
     PIC_TRY(timer_end(c, PIC_STAGE_FINALIZE, ev));
     return PIC_OK;
 }

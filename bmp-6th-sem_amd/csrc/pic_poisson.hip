@@ -190,6 +190,32 @@ __global__ __launch_bounds__(kB) void k_prep_rhs(Fine f, double* __restrict__ rh
     write_partial(acc, part_bb, sh);
 }
 
+// k_prep_rhs with the deposit's node pass in front: rho[n] is the species
+// sum of the corner meshes (final_mesh, energy_mesh written on the way), then
+// overwritten as above.  One pass instead of two over the grid.
+__global__ __launch_bounds__(kB) void k_finalize_prep(FinalizeAll a, int gp, int energy, Fine f,
+                                                      double* __restrict__ rho, const uint8_t* __restrict__ dir,
+                                                      const double* __restrict__ dval, double* part_bb, int* flags) {
+    __shared__ double sh[kB / 64];
+    if (blockIdx.x == 0 && threadIdx.x < 4) flags[threadIdx.x] = 0;
+    double acc = 0;
+    const int G = f.gx * f.gy;
+    auto x0 = [&](int m) -> double { return dir[m] ? dval[m] : 0.0; };
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
+        const double dep = finalize_node(a, n, gp, energy);
+        const uint8_t d = dir[n];
+        const double r = d == 1 ? dval[n] : dep;
+        rho[n] = -r;
+        if (!d) {
+            const int so = south_n(f.per, n, f.gx, f.gy), no = north_n(f.per, n, f.gx, f.gy);
+            const double ax = f.diag * 0.0 - f.cx * (x0(n - 1) + x0(n + 1)) - f.cy * (x0(n + so) + x0(n + no));
+            const double v = -r - ax;
+            acc += v * v;
+        }
+    }
+    write_partial(acc, part_bb, sh);
+}
+
 // rr = sum of the partials; done when rr <= tol^2 bb (flags[0] = 1), NaN -> 2.
 // The first call also records bb (from part_bb, or rr itself when part_bb is
 // null) and rr0; later calls count one more iteration unless already done.
@@ -1315,6 +1341,9 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
     const Fine f = fine_of(c);
     const int G = c->G;
     hipStream_t s = c->stream;
+    // a deposit node pass left to this solve (dropped if the solve fails first)
+    const bool fin = c->fin_pending != 0;
+    c->fin_pending = 0;
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_POISSON, &ev));
     if (!c->solve_ev[0]) {
@@ -1332,7 +1361,14 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
     if (mixed) PIC_TRY(ensure_f32(c));
 
     // flags cleared, overwriteRho, ||b||^2 (b: residual of the Dirichlet-only field)
-    hipLaunchKernelGGL(k_prep_rhs, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->dir, c->dval, part + P_BB * kNP, c->flags);
+    if (fin) {
+        // the deposit's node pass, deferred to here, writes rho first
+        hipLaunchKernelGGL(k_finalize_prep, dim3(kNP), dim3(kB), 0, s, c->fin, c->g.gp, c->p.deposit_energy, f,
+                           c->rho, c->dir, c->dval, part + P_BB * kNP, c->flags);
+    } else {
+        hipLaunchKernelGGL(k_prep_rhs, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->dir, c->dval, part + P_BB * kNP,
+                           c->flags);
+    }
     // initial guess
     int guess = 0;
     const double* src = nullptr;

@@ -56,6 +56,32 @@ def test_ensemble_matches_serial_runs(hip_mod):
         c.close()
 
 
+def test_ensemble_from_runs_stepped_serially(hip_mod):
+    """Runs that have taken serial steps (their cell arrays left unwritten by
+    the single-run step) form an ensemble: its first step equals a further
+    serial step of twin runs (rho bit-identical, phi to 1e-9, counts within 2)."""
+    hip = hip_mod
+    runs = _runs()[:3]
+    ens_ctx = _contexts(hip, runs)
+    ser_ctx = _contexts(hip, runs)
+    for it in range(2):
+        for c in ens_ctx + ser_ctx:
+            c.step(it)
+    ens = hip.Ensemble(ens_ctx)
+    ens.step(1, graph=True)
+    for c in ser_ctx:
+        c.step(2)
+    for k, (a, b) in enumerate(zip(ens_ctx, ser_ctx)):
+        assert np.array_equal(a.grid(hip.GRID_RHO), b.grid(hip.GRID_RHO)), k
+        pa, pb = a.grid(hip.GRID_PHI), b.grid(hip.GRID_PHI)
+        assert np.abs(pa - pb).max() <= 1e-9 * np.abs(pb).max(), k
+        for s in range(2):
+            assert abs(a.count(s) - b.count(s)) <= 2, (k, s)
+    ens.close()
+    for c in ens_ctx + ser_ctx:
+        c.close()
+
+
 def test_ensemble_graph_equals_direct_launches_and_tuner(hip_mod):
     hip = hip_mod
     runs = _runs()
