@@ -117,30 +117,47 @@ __global__ __launch_bounds__(kBlock) void k_scan_local(ScanSet ss, int G)
     __shared__ int wsum[kBlock / 64];
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const int base = blockIdx.x * kScanChunk + t * kScanPer;
+    // a thread's 8 counts as two 16-byte accesses (32-byte aligned: the
+    // arrays come from hipMalloc), element by element at the end of the grid
+    const bool whole = base + kScanPer <= G;
     int v[kScanPer];
+    if (whole) {
+        const int4 a = *(const int4*)(counts + base), b = *(const int4*)(counts + base + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+        const int4 z = make_int4(0, 0, 0, 0);
+        *(int4*)(counts + base) = z;
+        *(int4*)(counts + base + 4) = z;
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j) {
+            const int idx = base + j;
+            v[j] = idx < G ? counts[idx] : 0;
+            if (idx < G) counts[idx] = 0;
+        }
+    }
     int s = 0;
 #pragma unroll
-    for (int j = 0; j < kScanPer; ++j) {
-        int idx = base + j;
-        v[j] = idx < G ? counts[idx] : 0;
-        s += v[j];
-    }
-#pragma unroll
-    for (int j = 0; j < kScanPer; ++j) {
-        int idx = base + j;
-        if (idx < G) counts[idx] = 0;
-    }
+    for (int j = 0; j < kScanPer; ++j) s += v[j];
     int incl = wave_incl_scan(s);
     if (lane == 63) wsum[wid] = incl;
     __syncthreads();
     int wpre = 0;
     for (int w = 0; w < wid; ++w) wpre += wsum[w];
     int ex = wpre + incl - s;
+    int o[kScanPer];
 #pragma unroll
     for (int j = 0; j < kScanPer; ++j) {
-        int idx = base + j;
-        if (idx < G) start[idx] = ex;
+        o[j] = ex;
         ex += v[j];
+    }
+    if (whole) {
+        *(int4*)(start + base) = make_int4(o[0], o[1], o[2], o[3]);
+        *(int4*)(start + base + 4) = make_int4(o[4], o[5], o[6], o[7]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j)
+            if (base + j < G) start[base + j] = o[j];
     }
     if (t == kBlock - 1) bsum[blockIdx.x] = wpre + incl;
 }
