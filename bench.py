@@ -32,12 +32,14 @@ HBM_PEAK_GBS = 8000.0
 #   k_scatter_push reads the state and the slot, writes the state to its sorted
 #                  slot; per cell it reads the old start and the new start;
 #   k_deposit_tile reads offsets (+ velocities for the energy mesh) per particle
-#                  and, per cell, its start index and four charge + four energy
-#                  corner sums.
+#                  and, per cell, its start index, and writes two charge + two
+#                  energy node-row sums (int64); the particles of the cell left
+#                  of each tile, read a second time, are 1/256 of the whole and
+#                  not counted.
 KERNEL_BYTES = {
     "push": {1: (20 + 4, 4 + 4), 0: (40 + 4, 0)},
     "scatter": {1: (20 + 4 + 20, 4 + 4), 0: (40 + 4 + 44, 4)},
-    "deposit": {1: (8 + 12, 4 + 64), 0: (16 + 24, 4 + 64)},
+    "deposit": {1: (8 + 12, 4 + 32), 0: (16 + 24, 4 + 32)},
 }
 KERNEL_NAMES = {"push": "k_push_tile", "scatter": "k_scatter_push", "deposit": "k_deposit_tile"}
 

@@ -106,7 +106,7 @@ __device__ __forceinline__ int take_item(unsigned* queue, int total, int* slot) 
 // items: (species, run, deposit tile); prefix over runs of the tiles of one species
 __global__ __launch_bounds__(kBlock) void k_ens_deposit(const EnsRun* __restrict__ runs, int R,
                                                         const int* __restrict__ prefix, unsigned* queue) {
-    __shared__ unsigned long long acc[8][kDepC];
+    __shared__ unsigned long long acc[8][kDepA];
     __shared__ int slot;
     const int per_species = prefix[R];
     while (true) {
@@ -138,12 +138,12 @@ __global__ __launch_bounds__(kBlock) void k_ens_finalize(const EnsRun* __restric
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const EnsSpecies& sp = run.sp[s];
-            const long long q = sp.cq[n] + sp.cq[gp + n] + sp.cq[2 * gp + n] + sp.cq[3 * gp + n];
+            const long long q = sp.cq[n] + sp.cq[gp + n];
             const double f = (double)q * sp.cqf;
             sp.final_mesh[n] = f;
             rho += f * sp.chrg;
             if (energy) {
-                const long long e = sp.ce[n] + sp.ce[gp + n] + sp.ce[2 * gp + n] + sp.ce[3 * gp + n];
+                const long long e = sp.ce[n] + sp.ce[gp + n];
                 sp.energy_mesh[n] = (double)e * sp.cef;
             }
         }

@@ -436,7 +436,7 @@ extern "C" int pic_add_species(pic_ctx* c, double charge, double mass, double si
     ok = ok && hipMalloc((void**)&s.start, (G + 1) * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s.start_old, (G + 1) * sizeof(int32_t)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s.blk_first, (n / 256 + 2) * sizeof(int32_t)) == hipSuccess;
-    const size_t nacc = 4 * (size_t)c->g.gp;   // four node-indexed corner meshes
+    const size_t nacc = 2 * (size_t)c->g.gp;   // two node-indexed meshes: node-row sums from below and above
     ok = ok && hipMalloc((void**)&s.q_charge, nacc * sizeof(int64_t)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s.q_energy, nacc * sizeof(int64_t)) == hipSuccess;
     ok = ok && hipMalloc((void**)&s.final_mesh, G * sizeof(double)) == hipSuccess;
@@ -668,13 +668,13 @@ extern "C" int pic_get_deposit_fixed(pic_ctx* c, int s, int64_t* qc, int64_t* qe
     PIC_TRY(check_species(c, s, "pic_get_deposit_fixed"));
     PIC_HIP(hipStreamSynchronize(c->stream));
     const size_t gp = c->g.gp;
-    std::vector<int64_t> tmp(4 * gp);
+    std::vector<int64_t> tmp(2 * gp);
     int64_t* outs[2] = {qc, qe};
     int64_t* srcs[2] = {c->sp[s].q_charge, c->sp[s].q_energy};
     for (int a = 0; a < 2; ++a) {
         if (!outs[a]) continue;
-        PIC_HIP(hipMemcpy(tmp.data(), srcs[a], 4 * gp * sizeof(int64_t), hipMemcpyDeviceToHost));
-        for (int n = 0; n < c->G; ++n) outs[a][n] = tmp[n] + tmp[gp + n] + tmp[2 * gp + n] + tmp[3 * gp + n];
+        PIC_HIP(hipMemcpy(tmp.data(), srcs[a], 2 * gp * sizeof(int64_t), hipMemcpyDeviceToHost));
+        for (int n = 0; n < c->G; ++n) outs[a][n] = tmp[n] + tmp[gp + n];
     }
     return PIC_OK;
 }

@@ -87,9 +87,10 @@ struct FinalizeAll {
 };
 
 #ifdef __HIPCC__
-// final_mesh / energy_mesh of node n written, rho at n returned; corner c of
-// the corner meshes sits at c * gp + n.  The products are rounded before the
-// sum (no fused multiply-add) in every file that includes this.
+// final_mesh / energy_mesh of node n written, rho at n returned; a node's
+// fixed-point deposit is the sum of corner meshes 0 and 1 (node-row sums of
+// the cells above and below it).  The products are rounded before the sum
+// (no fused multiply-add) in every file that includes this.
 __device__ __forceinline__ double finalize_node(const FinalizeAll& a, int n, int gp, int energy) {
 #pragma clang fp contract(off)
     double r = 0.0;
@@ -97,13 +98,13 @@ __device__ __forceinline__ double finalize_node(const FinalizeAll& a, int n, int
     for (int s = 0; s < kFinMax; ++s) {
         if (s >= a.ns) break;
         const long long* cq = a.cq[s];
-        long long q = cq[n] + cq[gp + n] + cq[2 * gp + n] + cq[3 * gp + n];
+        long long q = cq[n] + cq[gp + n];
         double f = (double)q * a.cqf[s];
         a.fm[s][n] = f;
         r += f * a.chrg[s];
         if (energy) {
             const long long* ce = a.ce[s];
-            long long e = ce[n] + ce[gp + n] + ce[2 * gp + n] + ce[3 * gp + n];
+            long long e = ce[n] + ce[gp + n];
             a.em[s][n] = (double)e * a.cef[s];
         }
     }
@@ -124,7 +125,7 @@ struct Species {
     int32_t* start = nullptr; // GRID_X*GRID_Y + 1 (start[G] = live count)
     int32_t* start_old = nullptr; // CELL32 step: `start` of the order being pushed (the scan overwrites start)
     int32_t* blk_first = nullptr; // CELL32 step: cell of particle 256 b of the order being pushed
-    int64_t* q_charge = nullptr; // fixed-point corner meshes: 4 x (G + GRID_X + 2) int64
+    int64_t* q_charge = nullptr; // fixed-point corner meshes: 2 x (G + GRID_X + 2) int64 (node-row sums)
     int64_t* q_energy = nullptr;
     double*  final_mesh = nullptr;
     double*  energy_mesh = nullptr;

@@ -125,11 +125,16 @@ __device__ __forceinline__ double particle_energy(const ParticleBuf& b, int i, c
 // a wave reads whole cache lines), adds their fixed-point corner weights in
 // registers while the cell stays the same, and adds the running sums to
 // per-cell int64 accumulators in LDS when the cell changes (a cell of ~16
-// particles is flushed by one to three lanes).  Thread t then writes cell
-// c0 + t's four corner sums to the node-indexed corner meshes: corner c of
-// cell k goes to node k, k+1, k+GX, k+GX+1.  Every cell is written by exactly
-// one thread, so no global atomics; cells on the last node row or column
-// write zeros (the reference skips them, :122).
+// particles is flushed by one to three lanes).  The tile also sums the cell
+// left of its first one (its particles are read by both tiles; each uses
+// other corners of it), so thread t can form the two node-row sums of node
+// n = k0 + t by itself:
+//   lower[n]      = corner 0 of cell n + corner 1 of cell n - 1 (node row r),
+//   upper[n + GX] = corner 2 of cell n + corner 3 of cell n - 1 (row r + 1),
+// stored as mesh 0 and mesh 1 of the corner meshes; a node's deposit is
+// lower + upper.  Every node sum is written by exactly one thread, so no
+// global atomics; cells on the last node row or column contribute zero (the
+// reference skips them, :122), and node row 0's upper sum stays zero.
 //
 // Weights are formed already scaled by 2^S: for CELL32 (1 - ox) 2^h *
 // (1 - oy) 2^(S-h) rounds exactly like ((1 - ox)(1 - oy)) 2^S (scaling by a
@@ -137,6 +142,7 @@ __device__ __forceinline__ double particle_energy(const ParticleBuf& b, int i, c
 // w 2^S * (en 2^(E-S)) like (w en) 2^E, so the sums equal the oracle's bit
 // for bit.
 static constexpr int kDepC = 256;
+static constexpr int kDepA = kDepC + 1;     // accumulated cells: the tile's and the one left of it
 static constexpr int kVec = 4;
 
 template <int L> struct Quad {
@@ -166,12 +172,13 @@ __device__ __forceinline__ void load_quad<double>(const double* __restrict__ p, 
     out[0] = u.x; out[1] = u.y; out[2] = v.x; out[3] = v.y;
 }
 
-// Deposit of one tile (kDepC cells of a key row); acc: 8 x kDepC LDS accumulators.
+// Deposit of one tile (kDepC cells of a key row); acc: 8 x kDepA LDS
+// accumulators, local cell l = cell - (k0 - 1); sst: kDepA + 1 starts.
 template <int L>
 __device__ __forceinline__ void deposit_tile_body(
     const ParticleBuf& b, const GridDev& g, const DepositArgs& a, int tiles_per_row, int tile,
     const int32_t* __restrict__ start, long long* __restrict__ cq, long long* __restrict__ ce,
-    unsigned long long (*acc)[kDepC], int* sst = nullptr)
+    unsigned long long (*acc)[kDepA], int* sst = nullptr)
 {
     using P = typename Lay<L>::P;
     using V = typename Lay<L>::V;
@@ -180,22 +187,27 @@ __device__ __forceinline__ void deposit_tile_body(
     const int c0 = (tile % tiles_per_row) * kDepC;
     const int cw = min(kDepC, g.gx - c0);
     const int k0 = r * g.gx + c0;
+    const int kl = k0 - 1;                          // local cell 0
+    for (int j = tid; j < kDepA; j += kBlock)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) acc[c][tid] = 0ull;
-    // with `sst` the particles' cells come from the tile's cell starts (LDS):
-    // the cell array is not read
+        for (int c = 0; c < 8; ++c) acc[c][j] = 0ull;
+    // the left cell of column 0 is the last cell of the row below: it holds
+    // no particles (last node column) and is left out
+    const int beg = c0 > 0 ? start[kl] : start[k0];
+    const int end = start[k0 + cw];
+    // with `sst` the particles' cells come from the cell starts (LDS): the
+    // cell array is not read; sst[l] = first particle of local cell l
     if (sst)
-        for (int j = tid; j <= cw; j += kBlock) sst[j] = start[k0 + j];
+        for (int j = tid; j <= cw + 1; j += kBlock) sst[j] = j == 0 ? beg : start[kl + j];
     __syncthreads();
-    const int beg = start[k0], end = start[k0 + cw];
     const int first = beg & ~(kVec - 1);
     for (int base = first + kVec * tid; base < end; base += kVec * kBlock) {
         Quad<L> q;
         int sc = 0;
         if (sst) {
-            // last cell c with sst[c] <= the quad's first particle of the tile
+            // last local cell l with sst[l] <= the quad's first particle of the tile
             const int p0 = max(base, beg);
-            int lo = 0, hi = cw - 1;
+            int lo = 0, hi = cw;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
                 if (sst[mid] <= p0) lo = mid;
@@ -215,7 +227,7 @@ __device__ __forceinline__ void deposit_tile_body(
         int cur = -1;
         long long s0 = 0, s1 = 0, s2 = 0, s3 = 0, e0 = 0, e1 = 0, e2 = 0, e3 = 0;
         auto flush = [&]() {
-            const int lc = cur - k0;
+            const int lc = cur - kl;
             atomicAdd(&acc[0][lc], (unsigned long long)s0);
             atomicAdd(&acc[1][lc], (unsigned long long)s1);
             atomicAdd(&acc[2][lc], (unsigned long long)s2);
@@ -234,7 +246,7 @@ __device__ __forceinline__ void deposit_tile_body(
             int cl;
             if (sst) {
                 while (sst[sc + 1] <= pi) ++sc;
-                cl = k0 + sc;
+                cl = kl + sc;
             } else {
                 cl = q.cell[i];
             }
@@ -297,18 +309,19 @@ __device__ __forceinline__ void deposit_tile_body(
     }
     __syncthreads();
     if (tid >= cw) return;
-    const bool live = (r + 1 < g.gy) && (c0 + tid + 1 < g.gx);
+    // node n = k0 + tid: its own cell (local tid + 1) and the cell left of it
+    // (local tid); a cell on the last node row or column contributes zero
+    const bool row_live = r + 1 < g.gy;
+    const bool own = row_live && (c0 + tid + 1 < g.gx);
+    const bool left = row_live && c0 + tid > 0;
     const int gp = g.gp;
-    const int k = k0 + tid;
-    cq[k] = live ? (long long)acc[0][tid] : 0;
-    cq[gp + k + 1] = live ? (long long)acc[1][tid] : 0;
-    cq[2 * gp + k + g.gx] = live ? (long long)acc[2][tid] : 0;
-    cq[3 * gp + k + g.gx + 1] = live ? (long long)acc[3][tid] : 0;
+    const int n = k0 + tid;
+    auto pick = [&](int c, int l, bool live) { return live ? (long long)acc[c][l] : 0ll; };
+    cq[n] = pick(0, tid + 1, own) + pick(1, tid, left);
+    if (row_live) cq[gp + n + g.gx] = pick(2, tid + 1, own) + pick(3, tid, left);
     if (a.energy) {
-        ce[k] = live ? (long long)acc[4][tid] : 0;
-        ce[gp + k + 1] = live ? (long long)acc[5][tid] : 0;
-        ce[2 * gp + k + g.gx] = live ? (long long)acc[6][tid] : 0;
-        ce[3 * gp + k + g.gx + 1] = live ? (long long)acc[7][tid] : 0;
+        ce[n] = pick(4, tid + 1, own) + pick(5, tid, left);
+        if (row_live) ce[gp + n + g.gx] = pick(6, tid + 1, own) + pick(7, tid, left);
     }
 }
 

@@ -275,12 +275,12 @@ __global__ __launch_bounds__(kBlock) void k_deposit_finalize(
 {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= G) return;
-    long long q = cq[n] + cq[gp + n] + cq[2 * gp + n] + cq[3 * gp + n];
+    long long q = cq[n] + cq[gp + n];
     double f = (double)q * cqf;
     final_mesh[n] = f;
     rho[n] += f * chrg;
     if (energy) {
-        long long e = ce[n] + ce[gp + n] + ce[2 * gp + n] + ce[3 * gp + n];
+        long long e = ce[n] + ce[gp + n];
         energy_mesh[n] = (double)e * cef;
     }
 }
@@ -379,8 +379,8 @@ __global__ __launch_bounds__(kBlock) void k_push_tile(TileSet ts, GridDev g, int
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_deposit_tile(DepositSet ds, GridDev g, int tiles_per_row, int ntiles)
 {
-    __shared__ unsigned long long acc[8][kDepC];
-    __shared__ int sst[kDepC + 1];
+    __shared__ unsigned long long acc[8][kDepA];
+    __shared__ int sst[kDepA + 1];
     const int t = xcd_tile(blockIdx.x, ds.ns * ntiles);
     const int sp = t % ds.ns, tile = t / ds.ns;
     deposit_tile_body<L>(ds.b[sp], g, ds.a[sp], tiles_per_row, tile, ds.start[sp], ds.cq[sp], ds.ce[sp], acc, sst);
@@ -697,7 +697,7 @@ static int deposit_corners_set(pic_ctx* c, const int* ids, int n) {
             ds.ce[j] = (long long*)sp.q_energy;
         } else {
             PIC_TRY(inject_settle(c));   // the per-particle grid needs the live-count bound
-            const int64_t nacc = 4 * (int64_t)c->g.gp;
+            const int64_t nacc = 2 * (int64_t)c->g.gp;
             hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(nacc)), dim3(kBlock), 0, c->stream,
                                (unsigned long long*)sp.q_charge, nacc);
             hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(nacc)), dim3(kBlock), 0, c->stream,
