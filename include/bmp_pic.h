@@ -65,13 +65,14 @@ extern "C" {
 #define PIC_GRID_ENERGY0   16   /* + species: Particle::energy_mesh                 */
 
 /* -------------------------------------------------------------- stages */
-#define PIC_STAGE_DEPOSIT   0   /* per-cell corner sums of the CIC deposit               */
+#define PIC_STAGE_DEPOSIT   0   /* per-cell sums of the CIC deposit (node-row sums)      */
 #define PIC_STAGE_POISSON   1   /* whole Poisson solve                                   */
 #define PIC_STAGE_EFIELD    2   /* E = -grad phi                                         */
 #define PIC_STAGE_PUSH      3   /* gather + Boris + walls + slot count (move)            */
 #define PIC_STAGE_BIN       4   /* prefix scan of the counts (+ count/scatter of a re-bin) */
 #define PIC_STAGE_SCATTER   5   /* recomputed push written to the sorted slots (move)    */
-#define PIC_STAGE_FINALIZE  6   /* corner-mesh node sums into final_mesh, energy, rho    */
+#define PIC_STAGE_FINALIZE  6   /* node sums into final_mesh, energy, rho (pic_step: in the
+                                   Poisson stage, the solve's first kernel)              */
 #define PIC_NSTAGES         7
 
 typedef struct pic_ctx pic_ctx;
@@ -198,8 +199,8 @@ int  pic_zero_rho(pic_ctx* ctx);
  * interpolation.cpp:13-176): final_mesh[s], energy_mesh[s], and
  * rho += final_mesh[s] * (-q_s / epsilon). */
 int  pic_deposit(pic_ctx* ctx, int s);
-/* Raw fixed-point accumulators of the last deposit of species s (GRID_X*GRID_Y
- * int64 each; either pointer may be NULL). */
+/* Raw fixed-point accumulators of the last deposit of species s, summed per
+ * node (GRID_X*GRID_Y int64 each; either pointer may be NULL). */
 int  pic_get_deposit_fixed(pic_ctx* ctx, int s, int64_t* q_charge, int64_t* q_energy);
 
 /* Poisson solve (poissonSolver, poissonSolver.cpp:459-544): rho is rewritten
