@@ -162,47 +162,47 @@ __global__ __launch_bounds__(kBlock) void k_scan_local(ScanSet ss, int G)
     if (t == kBlock - 1) bsum[blockIdx.x] = wpre + incl;
 }
 
-// One workgroup of 1024 threads scans up to 8192 block sums in place.
-__global__ __launch_bounds__(1024) void k_scan_top(ScanSet ss, int nb, int G)
-{
-    int32_t* __restrict__ bsum = ss.bsum[blockIdx.x];
-    int32_t* __restrict__ start = ss.start[blockIdx.x];
-    int32_t* __restrict__ ndst = ss.nout[blockIdx.x];
-    __shared__ int wsum[16];
-    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-    int v[8];
+// Sum of v over the workgroup (every thread gets it); wsum: kBlock / 64 ints.
+__device__ __forceinline__ int block_sum(int v, int* wsum) {
+    const int t = threadIdx.x;
+    const int incl = wave_incl_scan(v);
+    if ((t & 63) == 63) wsum[t >> 6] = incl;
+    __syncthreads();
     int s = 0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        int idx = t * 8 + j;
-        v[j] = idx < nb ? bsum[idx] : 0;
-        s += v[j];
-    }
-    int incl = wave_incl_scan(s);
-    if (lane == 63) wsum[wid] = incl;
+    for (int w = 0; w < kBlock / 64; ++w) s += wsum[w];
     __syncthreads();
-    int wpre = 0;
-    for (int w = 0; w < wid; ++w) wpre += wsum[w];
-    int ex = wpre + incl - s;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        int idx = t * 8 + j;
-        if (idx < nb) bsum[idx] = ex;
-        ex += v[j];
-    }
-    if (t == 1023) {
-        int total = wpre + incl;
-        start[G] = total;
-        *ndst = total;
-    }
+    return s;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_add(ScanSet ss, int G)
+// start += the sum of the block sums of the chunks before this workgroup's
+// (each workgroup adds up the ones it needs: no separate scan of the block
+// sums); the last workgroup also writes the total to start[G] and the live
+// count.
+__global__ __launch_bounds__(kBlock) void k_scan_add(ScanSet ss, int nb, int G)
 {
     int32_t* __restrict__ start = ss.start[blockIdx.y];
-    const int32_t* __restrict__ boff = ss.bsum[blockIdx.y];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < G) start[i] += boff[i / kScanChunk];
+    const int32_t* __restrict__ bsum = ss.bsum[blockIdx.y];
+    __shared__ int wsum[kBlock / 64];
+    const int t = threadIdx.x;
+    const int ch = (int)(blockIdx.x * kBlock) / kScanChunk;   // the chunk of this workgroup's counts
+    const bool last = blockIdx.x == gridDim.x - 1;
+    int before = 0, all = 0;
+    for (int j = t; j < (last ? nb : ch); j += kBlock) {
+        const int v = bsum[j];
+        if (j < ch) before += v;
+        all += v;
+    }
+    const int pre = block_sum(before, wsum);
+    const int i = blockIdx.x * kBlock + t;
+    if (i < G) start[i] += pre;
+    if (last) {
+        const int total = block_sum(all, wsum);
+        if (t == 0) {
+            start[G] = total;
+            *ss.nout[blockIdx.y] = total;
+        }
+    }
 }
 
 template <int L>
@@ -535,8 +535,7 @@ static int scan_counts2(pic_ctx* c, int s0, int s1) {
         ss.nout[k] = sp.ndev + (sp.cur ^ 1);
     }
     hipLaunchKernelGGL(k_scan_local, dim3(c->scan_nb, n), dim3(kBlock), 0, c->stream, ss, c->G);
-    hipLaunchKernelGGL(k_scan_top, dim3(n), dim3(1024), 0, c->stream, ss, c->scan_nb, c->G);
-    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(c->G), n), dim3(kBlock), 0, c->stream, ss, c->G);
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(c->G), n), dim3(kBlock), 0, c->stream, ss, c->scan_nb, c->G);
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
