@@ -97,11 +97,15 @@ int sycl_pic_init(const char* dev_type, int wg, int ps, const char* input1, cons
     (void)ps;
     (void)input1;
     (void)input2;
+    // The reference picks a SYCL CPU or GPU queue and returns -1 for anything
+    // else (sycl_pic_ctypes.cpp:53-60).  Both values run on the GPU here: the
+    // library drives no other device, and the results do not depend on it.
     std::string dt(dev_type ? dev_type : "");
-    if (dt != "gpu") {
-        std::fprintf(stderr, "sycl_pic_init: device must be 'gpu' (this library has no CPU path)\n");
+    if (dt != "gpu" && dt != "cpu") {
+        std::fprintf(stderr, "sycl_pic_init: device must be 'cpu' or 'gpu'\n");
         return -1;
     }
+    if (dt == "cpu") std::fprintf(stderr, "sycl_pic_init: device 'cpu' requested; running on the GPU\n");
     if (g.ctx) {
         std::fprintf(stderr, "sycl_pic_init: already initialised; call sycl_pic_finalize first\n");
         return PIC_ERR_STATE;

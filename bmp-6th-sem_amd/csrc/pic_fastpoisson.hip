@@ -769,8 +769,8 @@ static FdRun run_of(const pic_ctx* c, double sgn) {
 
 // The electrode correction of the runs (dimensions of f; V1/V2 lines beyond
 // kLines are left to the caller): covering lines, u0 on Q, sources, their
-// rows and the spectrum G (a single run: rocBLAS; a batch: k_fd_corr, so a
-// captured graph holds kernels only).
+// rows and the spectrum G by the FP64 tile product k_fd_corr (one run or a
+// batch, so a captured graph holds kernels only).
 template <bool B>
 static void launch_fd(void (*k)(const FdRun, const FdRun*), dim3 g, const FdRuns& r, hipStream_t s) {
     if (B) hipLaunchKernelGGL(k, dim3(g.x, g.y, (unsigned)r.n), dim3(kFdB), 0, s, FdRun{}, r.tab);
@@ -789,12 +789,8 @@ static int enqueue_correction(FastPoisson* f, const FdRuns& r, hipStream_t s) {
     launch_fd<B>(k_fd_u0q<B>, dim3(wq), r, s);
     launch_fd<B>(k_fd_sources<B>, dim3(wq), r, s);
     launch_fd<B>(k_fd_zrows<B>, dim3((unsigned)((m + 63) / 64), (unsigned)f->nq), r, s);
-    if (B) {
-        launch_fd<B>(k_fd_corr<B>, dim3((unsigned)((m + kCT - 1) / kCT), (unsigned)((n + kCT - 1) / kCT)), r, s);
-    } else {
-        // G = S_n[:, rows of Q] Z (n x nq x m) -> G^T = Z^T SnQ^T
-        PIC_TRY(gemm(f->blas, m, n, f->nq, f->Z, m, f->SnQ, f->nq, 0.0, f->G, m));
-    }
+    // G = S_n[:, rows of Q] Z (n x nq times nq x m)
+    launch_fd<B>(k_fd_corr<B>, dim3((unsigned)((m + kCT - 1) / kCT), (unsigned)((n + kCT - 1) / kCT)), r, s);
     return PIC_OK;
 }
 

@@ -270,12 +270,8 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
     // solver scalars and flags in one block (one copy brings both to the host)
     ALLOC(c->scal, PIC_STATE_BYTES);
     c->flags = (int*)(c->scal + PIC_SCAL_DOUBLES);
-    c->scan_nb = (int)((G + 2047) / 2048);
-    if (c->scan_nb > 8192) {
-        set_error("pic_create: grid too large for the two-level scan");
-        return fail(PIC_ERR_ARG);
-    }
-    ALLOC(c->scan_bsum, 2 * (size_t)c->scan_nb * sizeof(int32_t));
+    c->scan_nb = (int)((G + 2047) / 2048);   // 2048-count chunks of the cell scan (<= 8192 by the node limit)
+    ALLOC(c->scan_bsum, 2 * ((size_t)c->scan_nb + 1) * sizeof(int32_t));
 #undef ALLOC
     if (hipHostMalloc((void**)&c->scal_host, PIC_STATE_BYTES, hipHostMallocDefault) != hipSuccess) {
         set_error("pic_create: hipHostMalloc failed");

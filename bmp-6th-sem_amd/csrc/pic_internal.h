@@ -234,7 +234,7 @@ struct pic_ctx {
     double* scal_host = nullptr;    // pinned copy of the state block
     int* flags_host = nullptr;      // in it
     int scan_nb = 0;
-    int32_t* scan_bsum = nullptr;   // block sums of two scans (2 x scan_nb)
+    int32_t* scan_bsum = nullptr;   // chunk totals of two scans (2 x (scan_nb + 1))
     pic::Injection inj;
     pic::FastPoisson* fast = nullptr;   // direct solve of the current geometry (null: PCG only)
     // timing

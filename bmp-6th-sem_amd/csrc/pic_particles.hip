@@ -175,10 +175,41 @@ __device__ __forceinline__ int block_sum(int v, int* wsum) {
     return s;
 }
 
-// start += the sum of the block sums of the chunks before this workgroup's
-// (each workgroup adds up the ones it needs: no separate scan of the block
-// sums); the last workgroup also writes the total to start[G] and the live
-// count.
+// Grids of at most this many chunks let every workgroup of k_scan_add add up
+// the chunk totals before its own (at most 4 loads per thread, from L2);
+// larger grids scan the totals once first (k_scan_top), since the summing
+// workgroups' reads grow with the square of the chunk count.
+static constexpr int kScanDirectChunks = 1024;
+
+// Exclusive scan of the chunk totals in place, with the grand total stored
+// after them (bsum[nb]); one workgroup per species.
+__global__ __launch_bounds__(kBlock) void k_scan_top(ScanSet ss, int nb)
+{
+    int32_t* __restrict__ bsum = ss.bsum[blockIdx.y];
+    __shared__ int wsum[kBlock / 64];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int per = (nb + kBlock - 1) / kBlock;
+    const int b0 = min(t * per, nb), b1 = min(b0 + per, nb);
+    int s = 0;
+    for (int j = b0; j < b1; ++j) s += bsum[j];
+    const int incl = wave_incl_scan(s);
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int ex = incl - s;
+    for (int w = 0; w < wid; ++w) ex += wsum[w];
+    for (int j = b0; j < b1; ++j) {
+        const int v = bsum[j];
+        bsum[j] = ex;
+        ex += v;
+    }
+    if (t == kBlock - 1) bsum[nb] = ex;
+}
+
+// start += the sum of the chunk totals before this workgroup's chunk.  With
+// Scanned false each workgroup adds up the totals it needs; with Scanned true
+// k_scan_top has turned them into prefixes.  The last workgroup also writes
+// the total to start[G] and the live count.
+template <bool Scanned>
 __global__ __launch_bounds__(kBlock) void k_scan_add(ScanSet ss, int nb, int G)
 {
     int32_t* __restrict__ start = ss.start[blockIdx.y];
@@ -187,21 +218,25 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(ScanSet ss, int nb, int G)
     const int t = threadIdx.x;
     const int ch = (int)(blockIdx.x * kBlock) / kScanChunk;   // the chunk of this workgroup's counts
     const bool last = blockIdx.x == gridDim.x - 1;
-    int before = 0, all = 0;
-    for (int j = t; j < (last ? nb : ch); j += kBlock) {
-        const int v = bsum[j];
-        if (j < ch) before += v;
-        all += v;
+    int pre, total = 0;
+    if (Scanned) {
+        pre = bsum[ch];
+        if (last) total = bsum[nb];
+    } else {
+        int before = 0, all = 0;
+        for (int j = t; j < (last ? nb : ch); j += kBlock) {
+            const int v = bsum[j];
+            if (j < ch) before += v;
+            all += v;
+        }
+        pre = block_sum(before, wsum);
+        if (last) total = block_sum(all, wsum);
     }
-    const int pre = block_sum(before, wsum);
     const int i = blockIdx.x * kBlock + t;
     if (i < G) start[i] += pre;
-    if (last) {
-        const int total = block_sum(all, wsum);
-        if (t == 0) {
-            start[G] = total;
-            *ss.nout[blockIdx.y] = total;
-        }
+    if (last && t == 0) {
+        start[G] = total;
+        *ss.nout[blockIdx.y] = total;
     }
 }
 
@@ -228,7 +263,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(
 
 // ============================================================== deposit
 // Storage not cell-sorted (after an in-place push): per-particle 64-bit
-// integer atomics into corner mesh 0 (the other three were zeroed).
+// integer atomics into node-row mesh 0 (each particle's four corners); mesh 1
+// (the sums from the row below) was zeroed, and the node pass adds mesh 0 and
+// mesh 1 as after a tile deposit.
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_deposit_atomic(
     const int32_t* __restrict__ nsrc, ParticleBuf b, GridDev g, DepositArgs a,
@@ -531,11 +568,18 @@ static int scan_counts2(pic_ctx* c, int s0, int s1) {
         Species& sp = c->sp[ids[k]];
         ss.counts[k] = sp.counts;
         ss.start[k] = sp.start;
-        ss.bsum[k] = c->scan_bsum + (size_t)k * c->scan_nb;
+        ss.bsum[k] = c->scan_bsum + (size_t)k * (c->scan_nb + 1);
         ss.nout[k] = sp.ndev + (sp.cur ^ 1);
     }
     hipLaunchKernelGGL(k_scan_local, dim3(c->scan_nb, n), dim3(kBlock), 0, c->stream, ss, c->G);
-    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(c->G), n), dim3(kBlock), 0, c->stream, ss, c->scan_nb, c->G);
+    if (c->scan_nb <= kScanDirectChunks) {
+        hipLaunchKernelGGL(k_scan_add<false>, dim3(blocks_for(c->G), n), dim3(kBlock), 0, c->stream, ss, c->scan_nb,
+                           c->G);
+    } else {
+        hipLaunchKernelGGL(k_scan_top, dim3(1, n), dim3(kBlock), 0, c->stream, ss, c->scan_nb);
+        hipLaunchKernelGGL(k_scan_add<true>, dim3(blocks_for(c->G), n), dim3(kBlock), 0, c->stream, ss, c->scan_nb,
+                           c->G);
+    }
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }

@@ -1056,7 +1056,14 @@ int poisson_setup(pic_ctx* c) {
     PIC_HIP(hipMemcpy(c->dir, dir.data(), G, hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(c->dval, dval.data(), G * sizeof(double), hipMemcpyHostToDevice));
 
-    // Multigrid hierarchy: vertex-centred coarsening while both cell counts are even.
+    // Multigrid hierarchy: vertex-centred coarsening, coarse node (I, J) at fine
+    // node (2I, 2J).  A level with an odd cell count in a Dirichlet direction
+    // gets one more coarse node than its nodes cover: it lies half a coarse
+    // cell beyond the wall and is Dirichlet like the wall (the coarse levels
+    // only precondition, so their domain may overhang the fine one).  The
+    // transfer kernels never read a fine node of it: they skip Dirichlet
+    // coarse nodes, and a fine node at most reaches coarse node (i + 1) / 2.
+    // A periodic y direction must keep an even period to coarsen.
     poisson_free(c);
     int lx = gx, ly = gy;
     double hx = c->g.dx, hy = c->g.dy;
@@ -1081,13 +1088,14 @@ int poisson_setup(pic_ctx* c) {
         PIC_HIP(hipMemset(m.t, 0, nn * sizeof(float)));
         c->mg.push_back(m);
         int cxn = lx - 1, cyn = ly - 1;
-        bool can = (cxn % 2 == 0) && (cyn % 2 == 0) && cxn >= 4 && cyn >= 4 && nn > 25;
+        bool can = cxn >= 4 && cyn >= 4 && nn > 25 && (!per || cyn % 2 == 0);
         if (!can) break;
-        int nlx = cxn / 2 + 1, nly = cyn / 2 + 1;
+        int nlx = (cxn + 1) / 2 + 1, nly = (cyn + 1) / 2 + 1;
         std::vector<uint8_t> nxt((size_t)nlx * nly, 0);
         for (int J = 0; J < nly; ++J)
             for (int I = 0; I < nlx; ++I) {
-                uint8_t d = cur[(size_t)(2 * J) * lx + 2 * I] != 0 ? 1 : 0;
+                const bool inside = 2 * I < lx && 2 * J < ly;     // else the overhanging node
+                uint8_t d = (!inside || cur[(size_t)(2 * J) * lx + 2 * I] != 0) ? 1 : 0;
                 if (I == 0 || I == nlx - 1 || (!per && (J == 0 || J == nly - 1))) d = 1;
                 if (per && J == nly - 1 && !d) d = 2;
                 nxt[(size_t)J * nlx + I] = d;

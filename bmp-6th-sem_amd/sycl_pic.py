@@ -14,10 +14,10 @@ for it (SYCL-PIC-poissonsolver/src/main.py:75-137) runs unchanged with
   ``update_fields_from_phi`` for phi); calls that change a grid refresh its
   mirror before they return.
 * Timed calls return wall seconds of a synchronous call.
-* ``init`` raises ``ValueError`` for a device other than ``"gpu"`` (this
-  library has no CPU path; the reference accepts ``"cpu"`` too,
-  sycl_pic_module.cpp:63-71) and ``RuntimeError`` when the input files are
-  missing or malformed.
+* ``init`` accepts the reference's two device values (sycl_pic_module.cpp:63-71);
+  both run the simulation on the GPU, the only device this library drives
+  (``"cpu"`` prints a note saying so).  Any other value raises
+  ``ValueError``; missing or malformed input files raise ``RuntimeError``.
 
 The library must have been built (``python bmp-6th-sem_amd/build.py``);
 importing this module on a machine without it raises ``ImportError``.
@@ -75,8 +75,6 @@ def init(device: str, wg_size: int, prob_sz: int, input1: str, input2: str) -> N
     Poisson hierarchy (sycl_pic_module.cpp:54-93)."""
     if device not in ("cpu", "gpu"):
         raise ValueError("device must be 'cpu' or 'gpu'")
-    if device != "gpu":
-        raise ValueError("this library runs on the GPU only; use device 'gpu'")
     rc = _lib.sycl_pic_init(device.encode(), int(wg_size), int(prob_sz), input1.encode(), input2.encode())
     if rc != 0:
         raise RuntimeError(f"sycl_pic.init failed ({rc}): {_lib.pic_last_error().decode(errors='replace')}")

@@ -381,7 +381,8 @@ int  pic_ensemble_steps_done(pic_ensemble* e, int64_t* steps);
 /* =================================================== drop-in group */
 /* Same names/signatures/semantics as SYCL-PIC-poissonsolver/src/sycl_pic_ctypes.cpp.
  * init reads ./M_PICDAT.DAT and ./geometry.conf from the working directory;
- * dev_type must be "gpu" ("cpu" returns -1: this library has no CPU path);
+ * dev_type "gpu" or "cpu" (sycl_pic_ctypes.cpp:53-60): both run on the GPU,
+ * the only device this library drives; any other value returns -1;
  * wg and ps are accepted and ignored (the counting sort has no probing
  * width); input1/input2 are ignored as in the reference (particles.cpp:174-176).
  * Particles are drawn by pic_generate_particles with the seed in the

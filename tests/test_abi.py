@@ -36,11 +36,19 @@ def test_library_exports_every_declared_symbol(hip_mod):
     assert not missing, missing
 
 
-def test_drop_in_rejects_cpu_device(hip_mod):
+def test_drop_in_device_values(hip_mod, tmp_path, monkeypatch):
+    """sycl_pic_init takes the reference's two device values ("cpu" and "gpu",
+    sycl_pic_ctypes.cpp:53-60) and returns -1 for any other; in a directory
+    without M_PICDAT.DAT both accepted values go on to report the missing file
+    (PIC_ERR_FILE) before anything touches a device."""
     lib = hip_mod.load()
     lib.sycl_pic_init.restype = ctypes.c_int
     lib.sycl_pic_init.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p]
-    assert lib.sycl_pic_init(b"cpu", 256, 8, b"a", b"b") == -1
+    monkeypatch.chdir(tmp_path)
+    assert lib.sycl_pic_init(b"fpga", 256, 8, b"a", b"b") == -1
+    assert lib.sycl_pic_init(b"", 256, 8, b"a", b"b") == -1
+    assert lib.sycl_pic_init(b"cpu", 256, 8, b"a", b"b") == -2
+    assert lib.sycl_pic_init(b"gpu", 256, 8, b"a", b"b") == -2
 
 
 def test_generator_is_deterministic_and_maxwellian(hip_mod):

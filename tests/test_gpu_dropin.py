@@ -151,6 +151,97 @@ def test_module_surface_matches_reference_table(hip_mod):
     for n in names:
         assert callable(getattr(sycl_pic, n)), n
     with pytest.raises(ValueError):
-        sycl_pic.init("cpu", 256, 8, "a", "b")
-    with pytest.raises(ValueError):
         sycl_pic.init("fpga", 256, 8, "a", "b")
+    # the C entry point returns -1 for an unknown device (sycl_pic_ctypes.cpp:57-59)
+    assert sycl_pic._lib.sycl_pic_init(b"fpga", 256, 8, b"a", b"b") == -1
+
+
+@pytest.mark.gpu
+def test_diagnostics_files_match_oracle(hip_mod, oracle_mod, tmp_path, monkeypatch):
+    """Every file the reference's diagnostics write, read back with loadtxt and
+    compared with the oracle's time step on the same particles, under the
+    reference's formulas (SYCL-PIC-poissonsolver/src/utils.cpp:829-1025,
+    src/main.cpp:260-274):
+
+    * electronDensity / ionDensity: final_mesh * scaleFactor (scale 3 here);
+    * electronenergy / ionenergy: 0.66667 energy / (final + 1), except the last
+      electron column, which divides by energy + 1 (utils.cpp:914);
+    * Ex, Ey, potential: the fields;
+    * electronNumber / ionNumber (written by the mover, mover.cpp:421-425):
+      per-key particle counts after the push, exact integers;
+    * Mesh_<s>.out: final_mesh through an ostream (6 significant digits).
+
+    "%lf" keeps 6 decimals, so values compare to 1e-9 relative plus 1e-6."""
+    hip, O = hip_mod, oracle_mod
+    text = _picdat().replace("Scale factor\n1\n", "Scale factor\n3\n")
+    assert "Scale factor\n3\n" in text
+    (tmp_path / "M_PICDAT.DAT").write_text(text)
+    (tmp_path / "geometry.conf").write_text(_geometry())
+    dirs = ("electronDensity", "ionDensity", "electronenergy", "ionenergy", "Ex", "Ey", "potential",
+            "electronNumber", "ionNumber")
+    for d in dirs:
+        (tmp_path / "out" / d).mkdir(parents=True)
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("BMP_PIC_SEED", "1")
+    import sycl_pic
+
+    W = NX * DX
+    gx = NX + 1
+    sim = O.Sim(NX, NX, W, W, DT, 8.854e-12, B0_T, RECTS, deposit_energy=1, use_pardiso=True)
+    for s, (q, m, T) in enumerate([(-E, ME, 2.0), (E, MI, 0.03)]):
+        parts = hip.generate_particles(1, s, N, W, W, DX, DX, abs(q / m), T)
+        sim.add_species(q, m, N, 1e16, *parts)
+
+    def close(path, want, rel=1e-9):
+        got = np.loadtxt(path)
+        assert got.shape == (gx, gx), path
+        want = want.reshape(gx, gx)
+        err = np.abs(got - want).max()
+        assert err <= 1e-6 + rel * np.abs(want).max(), (path, err)
+
+    # the reference's other device value runs the same GPU path
+    sycl_pic.init("cpu", 256, 8, "electronParticle.inp", "ionParticle.inp")
+    try:
+        for it in range(2):
+            sycl_pic.zero_rho()
+            for sp in range(2):
+                sycl_pic.charge_deposition(sp)
+            sycl_pic.poisson_solve(it)
+            for sp in range(2):
+                sycl_pic.move_particles(sp, it)
+            sycl_pic.print_diagnostics("./out", it)
+            sim.step()
+            out = tmp_path / "out"
+            fm = [sim.grid(hip.GRID_FINAL0 + s) for s in range(2)]
+            em = [sim.grid(hip.GRID_ENERGY0 + s) for s in range(2)]
+            close(out / "electronDensity" / str(it), 3.0 * fm[0])
+            close(out / "ionDensity" / str(it), 3.0 * fm[1])
+            e_want = 0.66667 * em[0] / (fm[0] + 1.0)
+            quirk = (0.66667 * em[0] / (em[0] + 1.0)).reshape(gx, gx)[:, -1]
+            e_want = e_want.reshape(gx, gx).copy()
+            e_want[:, -1] = quirk
+            close(out / "electronenergy" / str(it), e_want)
+            close(out / "ionenergy" / str(it), 0.66667 * em[1] / (fm[1] + 1.0))
+            # the quirk is visible: the last electron column is not energy / density
+            plain = (0.66667 * em[0] / (fm[0] + 1.0)).reshape(gx, gx)[:, -1]
+            if np.abs(plain - quirk).max() > 1e-3:
+                got_last = np.loadtxt(out / "electronenergy" / str(it))[:, -1]
+                assert np.abs(got_last - quirk).max() < np.abs(got_last - plain).max()
+            close(out / "Ex" / str(it), sim.grid(hip.GRID_EX))
+            close(out / "Ey" / str(it), sim.grid(hip.GRID_EY))
+            close(out / "potential" / str(it), sim.grid(hip.GRID_PHI))
+            for s, name in enumerate(("electronNumber", "ionNumber")):
+                got = np.loadtxt(out / name / str(it), dtype=np.int64)
+                want = np.bincount(sim.particles(s)[5], minlength=gx * gx).reshape(gx, gx)
+                assert got.shape == (gx, gx)
+                assert np.array_equal(got, want), (name, it, int(np.abs(got - want).sum()))
+                assert got.sum() == sycl_pic.get_particle_counts()[s]
+        sycl_pic.save_meshes()
+        for s in range(2):
+            got = np.loadtxt(tmp_path / f"Mesh_{s}.out")
+            want = sim.grid(hip.GRID_FINAL0 + s).reshape(gx, gx)
+            assert got.shape == (gx, gx)
+            assert np.allclose(got, want, rtol=6e-6, atol=0), s
+    finally:
+        sycl_pic.finalize()
+        sim.close()

@@ -74,18 +74,47 @@ def test_bin_membership_and_counts(hip_mod, oracle_mod, layout):
     c.close()
 
 
-@pytest.mark.parametrize("layout", [0, 1])
-def test_deposit_parity(hip_mod, oracle_mod, layout):
+@pytest.mark.parametrize("NX,NY", [(2047, 1100), (1000, 999)])
+def test_bin_large_grid(hip_mod, oracle_mod, NX, NY):
+    """Counting sort on grids with more than 1024 chunks of 2048 counts (the
+    chunk totals are scanned once before they are added: 2048 x 1101 nodes)
+    and just below that (1001 x 1000 nodes, each workgroup adds up the totals
+    it needs): counts and starts equal the oracle's."""
     hip, O = hip_mod, oracle_mod
-    c = _ctx(hip, 64, layout)
+    dx = 1e-4
+    p = hip.default_params(NX=NX, NY=NY, width=NX * dx, height=NY * dx, dt=1.77e-11, layout=1)
+    c = hip.Context(p)
+    c.set_geometry(WALLS)
+    n = 400000
+    s = c.add_species(-E, ME, n, 1e16, n)
+    x, y, vx, vy, vz = hip.generate_particles(4, 0, n, c.params.width, c.params.height, c.dx, c.dy, E / ME, 2.0)
+    c.upload(s, x, y, vx, vy, vz)
+    key = O.keys(x, y, c.gx, c.dx, c.dy)
+    counts, start, _ = O.bin(key, c.G)
+    assert np.array_equal(c.cell_counts(s), counts)
+    assert np.array_equal(c.cell_start(s), start)
+    assert c.count(s) == n
+    c.close()
+
+
+@pytest.mark.parametrize("NX,n", [(64, 50000), (600, 600000)])
+@pytest.mark.parametrize("layout", [0, 1])
+def test_deposit_parity(hip_mod, oracle_mod, layout, NX, n):
+    """Tile deposit (one workgroup per 256 cells of a key row, each also
+    summing the cell left of its first one) against the oracle's fixed-point
+    restatement, bit for bit, and the FP64 reference deposit.  NX = 600: two
+    full tiles and a partial one per row, so full-width tiles at c0 > 0 add a
+    live left cell taken from the previous tile's particles."""
+    hip, O = hip_mod, oracle_mod
+    c = _ctx(hip, NX, layout)
     qm = -E / ME
-    s = c.add_species(-E, ME, 50000, 1e16, 50000)
-    x, y, vx, vy, vz = _parts(hip, c, 50000, seed=5)
+    s = c.add_species(-E, ME, n, 1e16, n)
+    x, y, vx, vy, vz = _parts(hip, c, n, seed=5)
     c.upload(s, x, y, vx, vy, vz)
     c.zero_rho()
     c.deposit(s)
     qc, qe = c.deposit_fixed(s)
-    W = 1e16 * (c.params.width * c.params.height) / (50000 * c.dx * c.dy * c.dx * c.dy)
+    W = 1e16 * (c.params.width * c.params.height) / (n * c.dx * c.dy * c.dx * c.dy)
     if layout == 0:
         key = O.keys(x, y, c.gx, c.dx, c.dy)
         rqc, rqe = O.deposit_fixed_f64(x, y, vx, vy, vz, key, c.gx, c.gy, c.dx, c.dy, 40, 32, qm)
@@ -104,6 +133,48 @@ def test_deposit_parity(hip_mod, oracle_mod, layout):
     assert np.abs(gfm - fm).max() / np.abs(fm).max() < 1e-12
     assert np.abs(grho - rho).max() / np.abs(rho).max() < 1e-12
     assert np.abs(gem - em).max() / np.abs(em).max() < 1e-9
+    c.close()
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_step_deposit_parity_three_tiles(hip_mod, oracle_mod, layout):
+    """pic_step's deposit (both species' tiles in one launch, the node pass
+    deferred into the Poisson solve's first kernel) at NX = 600 against the
+    oracle's fixed-point deposit of the state the step started from, bit for
+    bit, and final_mesh / rho against the FP64 reference deposit."""
+    hip, O = hip_mod, oracle_mod
+    NX = 600
+    c = _ctx(hip, NX, layout, geom=ELEC)
+    n = 400000
+    before = []
+    for sp, (q, m, T) in enumerate([(-E, ME, 2.0), (E, MI, 0.03)]):
+        s = c.add_species(q, m, n, 1e16, n)
+        c.upload(s, *_parts(hip, c, n, seed=6, stream=sp, qm=abs(q / m), T=T))
+        before.append(c.download_cell32(s) if layout == 1 else c.download(s))
+    c.step(0)
+    rho = np.zeros(c.G)
+    W = 1e16 * (c.params.width * c.params.height) / (n * c.dx * c.dy * c.dx * c.dy)
+    for sp, (q, m, T) in enumerate([(-E, ME, 2.0), (E, MI, 0.03)]):
+        qc, qe = c.deposit_fixed(sp)
+        if layout == 1:
+            cell, ox, oy, fvx, fvy, fvz = before[sp]
+            rqc, rqe = O.deposit_fixed_cell32(cell, ox, oy, fvx, fvy, fvz, c.gx, c.gy, 40, 32, q / m)
+            xd = ((cell % c.gx) + ox.astype(np.float64)) * c.dx
+            yd = ((cell // c.gx) + oy.astype(np.float64)) * c.dy
+            fm, _, rho = O.deposit_ref(xd, yd, fvx, fvy, fvz, cell, c.gx, c.gy, c.dx, c.dy, W, q / m, q, 8.854e-12,
+                                       rho)
+        else:
+            x, y, vx, vy, vz, cell = before[sp]
+            rqc, rqe = O.deposit_fixed_f64(x, y, vx, vy, vz, cell, c.gx, c.gy, c.dx, c.dy, 40, 32, q / m)
+            fm, _, rho = O.deposit_ref(x, y, vx, vy, vz, cell, c.gx, c.gy, c.dx, c.dy, W, q / m, q, 8.854e-12, rho)
+        assert np.array_equal(qc, rqc), sp
+        assert np.array_equal(qe, rqe), sp
+        gfm = c.grid(hip.GRID_FINAL0 + sp)
+        assert np.abs(gfm - fm).max() / np.abs(fm).max() < 1e-12
+    # rho after overwriteRho (the solve rewrote it in place)
+    _, _, mask, dval = O.geometry(c.gx, c.gy, ELEC)
+    want = O.overwrite_rho(rho, mask, dval)
+    assert np.abs(c.grid(hip.GRID_RHO) - want).max() / np.abs(want).max() < 1e-12
     c.close()
 
 
@@ -246,13 +317,17 @@ def test_cell32_consecutive_pushes_from_cell_starts(hip_mod, oracle_mod, n):
     c.close()
 
 
-@pytest.mark.parametrize("it", [0, 4])
-def test_poisson_matches_reference_pardiso(hip_mod, oracle_mod, it):
-    """validation_dumps (PARDISO) at 257^2: phi to 1e-10, rho rewritten as overwriteRho does."""
+@pytest.mark.parametrize("solver", ["direct", "pcg"])
+@pytest.mark.parametrize("it", [0, 1, 2, 3, 4])
+def test_poisson_matches_reference_pardiso(hip_mod, oracle_mod, it, solver):
+    """validation_dumps (PARDISO) at 257^2: phi to 1e-10, rho rewritten as
+    overwriteRho does; the direct solve (checked by the PCG) and the FP64 PCG
+    alone."""
     from conftest import GOLDEN
 
     hip, O = hip_mod, oracle_mod
-    p = hip.default_params(NX=256, NY=256, width=1.0, height=1.0, dt=2.5e-9, poisson_tol=1e-12)
+    p = hip.default_params(NX=256, NY=256, width=1.0, height=1.0, dt=2.5e-9, poisson_tol=1e-12,
+                           poisson_direct=1 if solver == "direct" else 0)
     c = hip.Context(p)
     c.set_geometry(WALLS)
     rho = np.load(GOLDEN / f"iter{it:04d}_rho_input.npy")
@@ -261,6 +336,9 @@ def test_poisson_matches_reference_pardiso(hip_mod, oracle_mod, it):
     st = c.poisson_solve(0)
     phi = c.grid(hip.GRID_PHI)
     assert st["final_relres"] <= 1e-12
+    assert st["guess"] == (5 if solver == "direct" else 0)
+    if solver == "pcg":
+        assert st["iterations"] > 0
     assert np.abs(phi - ref).max() / np.abs(ref).max() < 1e-10
     _, _, mask, dval = O.geometry(257, 257, WALLS)
     assert np.array_equal(c.grid(hip.GRID_RHO), O.overwrite_rho(rho, mask, dval))
@@ -470,7 +548,7 @@ def test_full_steps_vs_cpu_baseline(hip_mod, oracle_mod):
     sim.close()
 
 
-@pytest.mark.parametrize("it", [0, 2])
+@pytest.mark.parametrize("it", [0, 1, 2, 3, 4])
 def test_poisson_mixed_precision_matches_reference_pardiso(hip_mod, it):
     """Iterative refinement (FP32 correction solves, FP64 residual, dots and
     update) on the PARDISO dumps: phi to 1e-10 at tolerance 1e-12."""

@@ -135,3 +135,45 @@ def test_periodic_steps_against_oracle_stages(hip_mod, oracle_mod):
             assert np.array_equal(_canon(got[5], *got[:5]),
                                   _canon(key[live], X[live], Y[live], VX[live], VY[live], VZ[live]))
     c.close()
+
+
+@pytest.mark.parametrize("NX,NY,periodic", [(500, 256, 1), (500, 256, 0), (600, 600, 0), (97, 61, 0)])
+def test_pcg_on_grids_with_odd_cell_counts(hip_mod, oracle_mod, NX, NY, periodic):
+    """The multigrid hierarchy coarsens levels with an odd cell count by one
+    overhanging Dirichlet coarse node, so the PCG alone solves grids whose
+    sizes have few factors of two: the reference's published grid (501 x 257
+    nodes, periodic y, Manya_Vansh_BMP.md:155-167) and non-periodic grids of
+    500 x 256, 600 x 600 and 97 x 61 cells with an electrode.  Against the
+    oracle (SuperLU on the periodic matrix; MKL PARDISO or the band Cholesky
+    on the Dirichlet one) to 1e-10 at tolerance 1e-12."""
+    hip, O = hip_mod, oracle_mod
+    dx = 1e-4
+    elec = [0.4375, 0.4375, 0.5625, 0.5625, 100.0]
+    walls = [[0, 0, 0, 1, 200.0], [1, 0, 1, 1, 0.0]]
+    geom = walls + [elec] if periodic else walls + [[0, 1, 1, 1, 0.0], [0, 0, 1, 0, 0.0], elec]
+    p = hip.default_params(NX=NX, NY=NY, width=NX * dx, height=NY * dx, dt=1.77e-11, layout=0, periodic=periodic,
+                           poisson_tol=1e-12, poisson_direct=0)
+    c = hip.Context(p)
+    c.set_geometry(geom)
+    rng = np.random.default_rng(NX * NY)
+    rho = rng.normal(0, 1e6, c.G)
+    c.set_grid(hip.GRID_RHO, rho)
+    st = c.poisson_solve(0)
+    assert st["final_relres"] <= 1e-12, st
+    if periodic:
+        _, phi_ref, _, _ = O.poisson_periodic(rho, c.gx, c.gy, c.dx, c.dy, geom)
+    else:
+        _, _, mask, dval = O.geometry(c.gx, c.gy, geom)
+        rhs = O.overwrite_rho(rho, mask, dval)
+        if O.load().orc_have_pardiso():
+            P = O.Pardiso(c.gx, c.gy, c.dx, c.dy, mask)
+            phi_ref = P.solve(rhs)
+            P.close()
+        else:
+            phi_ref = O.poisson_direct(c.gx, c.gy, c.dx, c.dy, mask, dval, rhs)
+    phi = c.grid(hip.GRID_PHI)
+    err = np.abs(phi - phi_ref).max() / np.abs(phi_ref).max()
+    print(f"{NX}x{NY} periodic={periodic}: {st['iterations']} PCG iterations, error {err:.2e}")
+    assert err < 1e-10
+    assert st["iterations"] < 40, st
+    c.close()

@@ -9,7 +9,7 @@ from conftest import GOLDEN
 WALLS = [[0, 0, 0, 1, 0.0], [1, 0, 1, 1, 0.0], [0, 1, 1, 1, 0.0], [0, 0, 1, 0, 0.0]]
 
 
-@pytest.mark.parametrize("it", [0, 2, 4])
+@pytest.mark.parametrize("it", [0, 1, 2, 3, 4])
 def test_direct_solve_matches_reference_pardiso_dumps(oracle_mod, it):
     """validation_dumps/iter000{it}: PARDISO phi for rho_input at 257^2, dx = 1/256."""
     O = oracle_mod
@@ -32,7 +32,7 @@ def test_pardiso_matches_dumps(oracle_mod):
     _, _, mask, dval = O.geometry(257, 257, WALLS)
     P = O.Pardiso(257, 257, 1 / 256, 1 / 256, mask)
     try:
-        for it in (0, 4):
+        for it in range(5):
             rho = np.load(GOLDEN / f"iter{it:04d}_rho_input.npy")
             ref = np.load(GOLDEN / f"iter{it:04d}_phi_cpp.npy")
             phi = P.solve(O.overwrite_rho(rho, mask, dval))
