@@ -18,7 +18,9 @@
 
 #include <hip/hip_ext.h>
 
+#include <climits>
 #include <cmath>
+#include <mutex>
 #include <tuple>
 
 namespace pic {
@@ -1012,4 +1014,117 @@ int launch_kinetic_energy(pic_ctx* c, int s, double* out) {
     return PIC_OK;
 }
 
+// ======================================== counting sort without allocation
+// pic_bin_keys_noalloc sorts cell keys held in caller memory with the bin
+// stage's kernels; the scan's chunk totals (and its grand total and live
+// count) go to `scratch`, a fixed array of this library on every GPU, so the
+// call allocates nothing.  Host code never passes `scratch` itself to a
+// kernel: each device's address comes from hipGetSymbolAddress, taken once
+// per device on its first call.
+static constexpr int kNoAllocMaxKeys = 2048 * 8192;                   // the grid node limit
+static constexpr int kNoAllocChunks = kNoAllocMaxKeys / kScanChunk;   // 8192
+static __device__ __align__(256) int32_t scratch[kNoAllocChunks + 2];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static int32_t* g_scratch[kMaxDevices];
+
+static int scratch_address(int32_t** out) {
+    int dev = 0;
+    PIC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("pic_bin_keys_noalloc: device %d beyond the %d this library tracks", dev, kMaxDevices);
+        return PIC_ERR_RANGE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("pic_bin_keys_noalloc: hipGetSymbolAddress of the scan scratch failed on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return PIC_ERR_HIP;
+        }
+        g_scratch[dev] = (int32_t*)p;
+    }
+    *out = g_scratch[dev];
+    return PIC_OK;
+}
+
+__global__ __launch_bounds__(kBlock) void k_na_zero(int G, int32_t* __restrict__ counts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < G) counts[i] = 0;
+}
+
+// slot[i] = the key's place among the equal keys so far (-1: key outside [0, G))
+__global__ __launch_bounds__(kBlock) void k_na_count(int64_t n, const int32_t* __restrict__ key, int G,
+                                                     int32_t* __restrict__ counts, int32_t* __restrict__ slot) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = key[i];
+    slot[i] = (k >= 0 && k < G) ? atomicAdd(&counts[k], 1) : -1;
+}
+
+// counts again from the starts (the scan clears the counts it reads); slots
+// become sorted positions
+__global__ __launch_bounds__(kBlock) void k_na_finish(int64_t n, int G, const int32_t* __restrict__ key,
+                                                      const int32_t* __restrict__ start,
+                                                      int32_t* __restrict__ counts, int32_t* __restrict__ slot) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < G) counts[i] = start[i + 1] - start[i];
+    if (i < n) {
+        const int r = slot[i];
+        if (r >= 0) slot[i] = start[key[i]] + r;
+    }
+}
+
+int bin_keys_noalloc(const int32_t* key, int64_t n, int32_t G, int32_t* counts, int32_t* start, int32_t* slot,
+                     hipStream_t s) {
+    if (n < 0 || n > (int64_t)INT32_MAX) {
+        set_error("pic_bin_keys_noalloc: %lld keys; at most %d", (long long)n, INT32_MAX);
+        return PIC_ERR_RANGE;
+    }
+    if (G < 1 || G > kNoAllocMaxKeys) {
+        set_error("pic_bin_keys_noalloc: %d cells; the fixed scan scratch serves 1 to %d", G, kNoAllocMaxKeys);
+        return PIC_ERR_RANGE;
+    }
+    if (!counts || !start || (n > 0 && (!key || !slot))) {
+        set_error("pic_bin_keys_noalloc: null pointer");
+        return PIC_ERR_ARG;
+    }
+    if ((((uintptr_t)counts) | ((uintptr_t)start)) & 15) {
+        set_error("pic_bin_keys_noalloc: counts and start must be 16-byte aligned (the scan moves 4 ints at once)");
+        return PIC_ERR_ARG;
+    }
+    int32_t* sc = nullptr;
+    PIC_TRY(scratch_address(&sc));
+    const int nb = (G + kScanChunk - 1) / kScanChunk;
+    hipLaunchKernelGGL(k_na_zero, dim3(blocks_for(G)), dim3(kBlock), 0, s, G, counts);
+    if (n > 0)
+        hipLaunchKernelGGL(k_na_count, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, key, G, counts, slot);
+    ScanSet ss{};
+    for (int k = 0; k < 2; ++k) {
+        ss.counts[k] = counts;
+        ss.start[k] = start;
+        ss.bsum[k] = sc;
+        ss.nout[k] = sc + kNoAllocChunks + 1;
+    }
+    hipLaunchKernelGGL(k_scan_local, dim3(nb, 1), dim3(kBlock), 0, s, ss, G);
+    if (nb <= kScanDirectChunks) {
+        hipLaunchKernelGGL(k_scan_add<false>, dim3(blocks_for(G), 1), dim3(kBlock), 0, s, ss, nb, G);
+    } else {
+        hipLaunchKernelGGL(k_scan_top, dim3(1, 1), dim3(kBlock), 0, s, ss, nb);
+        hipLaunchKernelGGL(k_scan_add<true>, dim3(blocks_for(G), 1), dim3(kBlock), 0, s, ss, nb, G);
+    }
+    hipLaunchKernelGGL(k_na_finish, dim3(blocks_for(std::max<int64_t>(n, G))), dim3(kBlock), 0, s, n, G, key, start,
+                       counts, slot);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
 } // namespace pic
+
+extern "C" int pic_bin_keys_noalloc(const int32_t* key, int64_t n, int32_t G, int32_t* counts, int32_t* start,
+                                    int32_t* slot, void* stream) {
+    return pic::bin_keys_noalloc(key, n, G, counts, start, slot, (hipStream_t)stream);
+}

@@ -289,6 +289,24 @@ int  pic_injection_target(pic_ctx* ctx, int64_t* target, int64_t* attempts);
 int  pic_injection_stats(pic_ctx* ctx, int64_t* injections, int64_t* reinjected, int64_t* pairs,
                          int64_t* dropped, int64_t* last_electrons, int64_t* last_ions);
 
+/* Counting sort of n cell keys in device memory that allocates nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  It is the key
+ * half of pic_bin (the same count and scan kernels): counts[k] (G ints) = the
+ * keys equal to k; start (G + 1 ints) = their exclusive prefix, start[G] = the
+ * keys inside [0, G); slot[i] (n ints) = the sorted position of key i
+ * (start[key[i]] plus its place among the equal keys, in no fixed order), or
+ * -1 for a key outside [0, G) (a dead particle).  All pointers are DEVICE
+ * pointers; counts and start 16-byte aligned.  The work is enqueued on
+ * `stream` (a hipStream_t; NULL = the null stream) and the call does not wait.
+ * The scan's chunk totals live in a fixed array of this library on each GPU
+ * (33 KB, kept until the process ends): G is limited to 16,777,216 keys and n
+ * to 2^31 - 1 (PIC_ERR_RANGE beyond), and calls that overlap on one GPU race
+ * on that array.  Every call clears what it needs; the array keeps the last
+ * call's totals. */
+int  pic_bin_keys_noalloc(const int32_t* key, int64_t n, int32_t G, int32_t* counts, int32_t* start, int32_t* slot,
+                          void* stream);
+
 /* Block until all work of the context is done. */
 int  pic_sync(pic_ctx* ctx);
 /* Timing: 0 off; 1 per stage (HIP event markers around each stage's
