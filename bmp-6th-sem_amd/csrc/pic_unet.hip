@@ -8,7 +8,10 @@
 // implicit GEMM: M = 16 pixels of a row, N = 16 output channels, K = 16 input
 // channels of one tap).  Training is online: forward, backward (input
 // gradients by the same kernel with flipped, transposed weights; weight
-// gradients by MFMA over pixels) and Adam with FP32 master weights.
+// gradients by MFMA over pixels) and Adam with FP32 master weights.  Every
+// gradient and loss reduction writes per-wave or per-workgroup partials that
+// k_sum_slots adds in a fixed order (no floating-point atomics), so training
+// is reproducible bit for bit.
 //
 // Activations are NHWC with one zero pixel of padding on every side, so the
 // 3x3 taps need no bounds checks; gradients are FP32 in the same layout.
@@ -207,14 +210,15 @@ __global__ __launch_bounds__(kT) void k_conv3x3_lds(const bf16* __restrict__ X, 
     }
 }
 
-// dW[tap][co][ci] += sum_{y,x} dZT[co][y][x] XT[ci][y+1+dy][x+1+dx]
-// dZT: [cout][H][W] (unpadded), XT: [cin][H+2][W+2] (padded).  One wave: one
-// tap, one 16-channel co tile, CT ci tiles, rows [r0, r0 + rows); FP32 atomics
-// add the wave's tile into dW.
+// Partial of dW[tap][co][ci] = sum_{y,x} dZT[co][y][x] XT[ci][y+1+dy][x+1+dx]
+// over one chunk of rows.  dZT: [cout][H][W] (unpadded), XT: [cin][H+2][W+2]
+// (padded).  One wave: one tap, one 16-channel co tile, CT ci tiles, rows
+// [r0, r0 + rows); it writes its tile to slot `chunk` of part (slot stride
+// 9 cout cin), and k_sum_slots adds the slots in chunk order.
 template <int CT>
 __global__ __launch_bounds__(kT) void k_conv_wgrad(const bf16* __restrict__ dZT, const bf16* __restrict__ XT,
                                                    int cout, int cin, int H, int W, int rows,
-                                                   float* __restrict__ dW) {
+                                                   float* __restrict__ part) {
     const int lane = threadIdx.x & 63;
     const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int nchunk = (H + rows - 1) / rows;
@@ -251,17 +255,18 @@ __global__ __launch_bounds__(kT) void k_conv_wgrad(const bf16* __restrict__ dZT,
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int co = cot * 16 + 4 * kq + e, ci = c * 16 + i;
-            atomicAdd(dW + ((size_t)tap * cout + co) * cin + ci, acc[c][e]);
+            part[(size_t)chunk * 9 * cout * cin + ((size_t)tap * cout + co) * cin + ci] = acc[c][e];
         }
 }
 
-// db[co] += sum_p dZ[p][co]; dZ padded NHWC with cout channels (8 <= cout <=
-// 256, a multiple of 8).  Thread t takes the 8-channel slice t % (cout/8) of
-// pixels t / (cout/8), + kT / (cout/8), ... of a grid-stride walk (16-byte
-// loads); the block adds its per-thread sums per channel through LDS and
-// makes one atomic per channel.
+// Partials of db[co] = sum_p dZ[p][co]; dZ padded NHWC with cout channels (8 <=
+// cout <= 256, a multiple of 8).  Thread t takes the 8-channel slice
+// t % (cout/8) of pixels t / (cout/8), + kT / (cout/8), ... of a grid-stride
+// walk (16-byte loads); the block adds its per-thread sums per channel through
+// LDS in a fixed order and writes them to slot blockIdx.x of part (stride
+// cout).
 __global__ __launch_bounds__(kT) void k_bias_grad(const bf16* __restrict__ dZ, int cout, int H, int W,
-                                                  float* __restrict__ db) {
+                                                  float* __restrict__ part) {
     __shared__ float sh[kT][9];
     const int c8 = cout / 8, per = kT / c8;
     const int sl = threadIdx.x % c8, q0 = threadIdx.x / c8;
@@ -284,7 +289,7 @@ __global__ __launch_bounds__(kT) void k_bias_grad(const bf16* __restrict__ dZ, i
         const int ch = threadIdx.x, s8 = ch >> 3, k = ch & 7;
         float t = 0.f;
         for (int q = 0; q < per; ++q) t += sh[q * c8 + s8][k];
-        atomicAdd(db + ch, t);
+        part[(size_t)blockIdx.x * cout + ch] = t;
     }
 }
 
@@ -421,10 +426,11 @@ __global__ void k_head(const bf16* __restrict__ d0, int c, const float* __restri
     out[p] = s;
 }
 
-// Output gradient of the masked mean squared error against `tgt`; loss added.
+// Output gradient of the masked mean squared error against `tgt`; the block's
+// share of the loss goes to lpart[blockIdx.x].
 __global__ __launch_bounds__(kT) void k_loss_mse(const float* __restrict__ out, const float* __restrict__ tgt,
                                                  const uint8_t* __restrict__ mask, const int* __restrict__ count,
-                                                 float* __restrict__ gout, float* __restrict__ loss, int n) {
+                                                 float* __restrict__ gout, float* __restrict__ lpart, int n) {
     __shared__ float sh[kT / 64];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const float inv = *count > 0 ? 1.f / (float)*count : 0.f;
@@ -441,18 +447,18 @@ __global__ __launch_bounds__(kT) void k_loss_mse(const float* __restrict__ out, 
     for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = l;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(loss, (sh[0] + sh[1]) + (sh[2] + sh[3]));
+    if (threadIdx.x == 0) lpart[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // Residual loss: the prediction d (= out on unknown nodes, 0 elsewhere) should
 // cancel the scaled starting residual, res = r0 / max|r0| - (A / diag) d on
-// unknown nodes; loss = mean res^2.  (A / diag has unit diagonal and
-// off-diagonals -cxd, -cyd.)
+// unknown nodes; loss = mean res^2 (the block's share to lpart[blockIdx.x]).
+// (A / diag has unit diagonal and off-diagonals -cxd, -cyd.)
 __global__ __launch_bounds__(kT) void k_loss_res(const float* __restrict__ out, const double* __restrict__ r0,
                                                  const unsigned long long* __restrict__ scl,
                                                  const uint8_t* __restrict__ mask, const int* __restrict__ count,
                                                  float cxd, float cyd, int N, float* __restrict__ res,
-                                                 float* __restrict__ loss) {
+                                                 float* __restrict__ lpart) {
     __shared__ float sh[kT / 64];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const float inv = *count > 0 ? 1.f / (float)*count : 0.f;
@@ -477,7 +483,7 @@ __global__ __launch_bounds__(kT) void k_loss_res(const float* __restrict__ out, 
     for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = l;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(loss, (sh[0] + sh[1]) + (sh[2] + sh[3]));
+    if (threadIdx.x == 0) lpart[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // d loss / d out = -2 (A / diag) res / count on unknown nodes (A symmetric)
@@ -498,11 +504,12 @@ __global__ void k_loss_res_grad(const float* __restrict__ res, const uint8_t* __
     gout[p] = g;
 }
 
-// Head backward from the output gradient gout: gd0[p][c] = gout wh[c];
-// dwh[c] += sum gout d0[c]; dbh += sum gout.  c = 16 channels.
+// Head backward from the output gradient gout: gd0[p][c] = gout wh[c]; the
+// block's shares of dwh[c] = sum gout d0[c] and dbh = sum gout go to slot
+// blockIdx.x of part (17 floats: dwh then dbh).  c = 16 channels.
 __global__ __launch_bounds__(kT) void k_head_bwd(const float* __restrict__ gout, const bf16* __restrict__ d0,
                                                  const float* __restrict__ wh, float* __restrict__ gd0,
-                                                 float* __restrict__ dwh, float* __restrict__ dbh, int H, int W) {
+                                                 float* __restrict__ part, int H, int W) {
     __shared__ float sh[17][kT / 64];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     float red[17];
@@ -530,14 +537,24 @@ __global__ __launch_bounds__(kT) void k_head_bwd(const float* __restrict__ gout,
     if (threadIdx.x < 17) {
         float s = 0.f;
         for (int w = 0; w < kT / 64; ++w) s += sh[threadIdx.x][w];
-        if (threadIdx.x < 16) atomicAdd(dwh + threadIdx.x, s);
-        else atomicAdd(dbh, s);
+        part[(size_t)blockIdx.x * 17 + threadIdx.x] = s;
     }
 }
 
 __global__ void k_zero_f32(float* __restrict__ p, int64_t n) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = 0.f;
+}
+
+// out[j] = sum over slots 0, 1, ..., nslot - 1 of part[slot * len + j], added
+// in slot order: the reductions above leave their partials in slots, so a
+// gradient or loss does not depend on the order the workgroups ran in.
+__global__ void k_sum_slots(const float* __restrict__ part, int nslot, int64_t len, float* __restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= len) return;
+    float t = 0.f;
+    for (int q = 0; q < nslot; ++q) t += part[(size_t)q * len + j];
+    out[j] = t;
 }
 
 // count[0] = number of non-zero mask entries (count zeroed by the caller's kernel)
@@ -716,6 +733,9 @@ struct pic_unet {
     bool own_stream = false;
     double* r0 = nullptr;                 // starting residual on the image (FP64)
     unsigned long long* scl = nullptr;    // per-solve scales (max|r0|, max|phi|)
+    float* wpart = nullptr;               // weight-gradient partials, one slot per row chunk
+    float* bpart = nullptr;               // bias / head / loss partials, one slot per workgroup
+    size_t wpart_n = 0, bpart_n = 0;
 };
 
 namespace pic {
@@ -815,25 +835,34 @@ static int conv_any(hipStream_t s, int nt, const bf16* X, int xC, int xo, int ci
     return PIC_OK;
 }
 
+static int wgrad_rows(int H) { return H >= 256 ? 2 : 1; }
+static int wgrad_chunks(int H) { return (H + wgrad_rows(H) - 1) / wgrad_rows(H); }
+
+// dW = the weight gradient: per-chunk partials into `part`, then their sum in
+// chunk order.
 static int wgrad_any(hipStream_t s, int ct, const bf16* dZT, const bf16* XT, int cout, int cin, int H, int W,
-                     float* dW) {
-    const int rows = H >= 256 ? 2 : 1;
-    const int waves = 9 * (cout / 16) * ((H + rows - 1) / rows);
+                     float* part, float* dW) {
+    const int rows = wgrad_rows(H);
+    const int nchunk = wgrad_chunks(H);
+    const int waves = 9 * (cout / 16) * nchunk;
     const dim3 grid((unsigned)((waves + 3) / 4));
     switch (ct) {
-    case 1: hipLaunchKernelGGL(k_conv_wgrad<1>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, dW); break;
-    case 2: hipLaunchKernelGGL(k_conv_wgrad<2>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, dW); break;
-    case 3: hipLaunchKernelGGL(k_conv_wgrad<3>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, dW); break;
-    case 4: hipLaunchKernelGGL(k_conv_wgrad<4>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, dW); break;
-    case 6: hipLaunchKernelGGL(k_conv_wgrad<6>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, dW); break;
-    case 8: hipLaunchKernelGGL(k_conv_wgrad<8>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, dW); break;
+    case 1: hipLaunchKernelGGL(k_conv_wgrad<1>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, part); break;
+    case 2: hipLaunchKernelGGL(k_conv_wgrad<2>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, part); break;
+    case 3: hipLaunchKernelGGL(k_conv_wgrad<3>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, part); break;
+    case 4: hipLaunchKernelGGL(k_conv_wgrad<4>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, part); break;
+    case 6: hipLaunchKernelGGL(k_conv_wgrad<6>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, part); break;
+    case 8: hipLaunchKernelGGL(k_conv_wgrad<8>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, part); break;
     default: set_error("pic_unet: unsupported channel count %d", ct * 16); return PIC_ERR_UNSUPPORTED;
     }
+    const int64_t len = (int64_t)9 * cout * cin;
+    hipLaunchKernelGGL(k_sum_slots, dim3(nb(len)), dim3(kT), 0, s, part, nchunk, len, dW);
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
 
 static size_t tsize(const Tensor& x) { return (size_t)(x.H + 2) * (x.W + 2) * x.C; }
+static int bias_parts(int H, int W) { return std::min(512, (H * W + 4095) / 4096); }
 
 static int resample(pic_unet* u, bool up, int in, int io, int out, int oo, int c) {
     const Tensor& a = u->t[in];
@@ -901,9 +930,10 @@ static int conv_bwd(pic_unet* u, int k) {
     hipLaunchKernelGGL(k_to_chw, dim3((unsigned)((W + 2 + 63) / 64), (unsigned)(H + 2), (unsigned)((c.cin + 15) / 16)),
                        dim3(kT), 0, s, a.p, a.C, c.io, c.cin, H, W, 1, u->xt);
     PIC_HIP(hipGetLastError());
-    PIC_TRY(wgrad_any(s, c.cin / 16, u->dzt, u->xt, c.cout, c.cin, H, W, u->g + c.woff));
-    const int parts = std::min(512, (H * W + 4095) / 4096);
-    hipLaunchKernelGGL(k_bias_grad, dim3(parts), dim3(kT), 0, s, u->dz, c.cout, H, W, u->g + c.boff);
+    PIC_TRY(wgrad_any(s, c.cin / 16, u->dzt, u->xt, c.cout, c.cin, H, W, u->wpart, u->g + c.woff));
+    const int parts = bias_parts(H, W);
+    hipLaunchKernelGGL(k_bias_grad, dim3(parts), dim3(kT), 0, s, u->dz, c.cout, H, W, u->bpart);
+    hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(kT), 0, s, u->bpart, parts, (int64_t)c.cout, u->g + c.boff);
     PIC_HIP(hipGetLastError());
     if (k > 0)
         PIC_TRY((conv_any<0, float>(s, c.cin / 16, u->dz, c.cout, 0, c.cout, u->wflip + c.woff, nullptr, a.g, a.C,
@@ -917,20 +947,21 @@ static int unet_backward(pic_unet* u, int loss_kind = 0, float cxd = 0.f, float 
     hipStream_t s = u->stream;
     const int N = u->N;
     const unsigned gN = nb((int64_t)N * N);
-    hipLaunchKernelGGL(k_zero_f32, dim3(nb((int64_t)u->nparam)), dim3(kT), 0, s, u->g, (int64_t)u->nparam);
-    hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(kT), 0, s, u->loss, (int64_t)1);
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, u->count);
     hipLaunchKernelGGL(k_mask_count, dim3(gN), dim3(kT), 0, s, u->mask, N * N, u->count);
     if (loss_kind == 0) {
-        hipLaunchKernelGGL(k_loss_mse, dim3(gN), dim3(kT), 0, s, u->out, u->tgt, u->mask, u->count, u->gout, u->loss,
+        hipLaunchKernelGGL(k_loss_mse, dim3(gN), dim3(kT), 0, s, u->out, u->tgt, u->mask, u->count, u->gout, u->bpart,
                            N * N);
     } else {
         hipLaunchKernelGGL(k_loss_res, dim3(gN), dim3(kT), 0, s, u->out, u->r0, u->scl, u->mask, u->count, cxd, cyd,
-                           N, u->tgt, u->loss);
+                           N, u->tgt, u->bpart);
         hipLaunchKernelGGL(k_loss_res_grad, dim3(gN), dim3(kT), 0, s, u->tgt, u->mask, u->count, cxd, cyd, N, u->gout);
     }
+    hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(kT), 0, s, u->bpart, (int)gN, (int64_t)1, u->loss);
     hipLaunchKernelGGL(k_head_bwd, dim3(gN), dim3(kT), 0, s, u->gout, u->t[D0].p, u->w + u->head_w, u->t[D0].g,
-                       u->g + u->head_w, u->g + u->head_b, N, N);
+                       u->bpart, N, N);
+    // head_b follows head_w's 16 weights in the parameter vector, as in the slots
+    hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(kT), 0, s, u->bpart, (int)gN, (int64_t)17, u->g + u->head_w);
     PIC_HIP(hipGetLastError());
     auto up_bwd = [&](int from, int fo, int to, int c) -> int {
         const Tensor& a = u->t[from];
@@ -1082,7 +1113,14 @@ int pic_unet_create(int n, uint64_t seed, pic_unet** out) {
         maxx = std::max(maxx, (size_t)(a.H + 2) * (a.W + 2) * c.cin);
     }
     const size_t P = u->nparam;
-    bool ok = hipMalloc(&u->w, P * 4) == hipSuccess && hipMalloc(&u->g, P * 4) == hipSuccess &&
+    for (const Conv& c : u->conv) {
+        const Tensor& a = u->t[c.in];
+        u->wpart_n = std::max(u->wpart_n, (size_t)wgrad_chunks(a.H) * 9 * c.cout * c.cin);
+        u->bpart_n = std::max(u->bpart_n, (size_t)bias_parts(a.H, a.W) * c.cout);
+    }
+    u->bpart_n = std::max(u->bpart_n, (size_t)nb((int64_t)n * n) * 17);
+    bool ok = hipMalloc(&u->wpart, u->wpart_n * 4) == hipSuccess && hipMalloc(&u->bpart, u->bpart_n * 4) == hipSuccess &&
+              hipMalloc(&u->w, P * 4) == hipSuccess && hipMalloc(&u->g, P * 4) == hipSuccess &&
               hipMalloc(&u->m, P * 4) == hipSuccess && hipMalloc(&u->v, P * 4) == hipSuccess &&
               hipMalloc(&u->wb, P * 2) == hipSuccess && hipMalloc(&u->wflip, P * 2) == hipSuccess &&
               hipMalloc(&u->out, (size_t)n * n * 4) == hipSuccess && hipMalloc(&u->tgt, (size_t)n * n * 4) == hipSuccess &&
@@ -1123,7 +1161,7 @@ void pic_unet_destroy(pic_unet* u) {
         if (x.g) (void)hipFree(x.g);
     }
     void* ptrs[] = {u->w, u->g, u->m, u->v, u->wb, u->wflip, u->out, u->tgt, u->gout, u->mask, u->loss, u->count,
-                    u->dz, u->dzt, u->xt, u->r0, u->scl};
+                    u->dz, u->dzt, u->xt, u->r0, u->scl, u->wpart, u->bpart};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (u->loss_host) (void)hipHostFree(u->loss_host);

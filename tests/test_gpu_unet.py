@@ -104,9 +104,12 @@ def test_unet_adam_step_and_training(hip_mod):
 
 def test_learned_guess_in_the_solver(hip_mod):
     """256 x 256 cells, electrode geometry, tolerance 1e-8: the PCG with the
-    learned guess converges like the plain one and reaches the same potential;
-    the network's guess replaces the previous solution only when its starting
-    residual is lower.  Prints what the trained network gains."""
+    learned guess reaches the same potential as the plain one.  After 40
+    training steps the trained network earns its place: its guess is kept on
+    at least 80 % of the solves it is tried on, and over the last 20 solves it
+    starts the PCG from at most half the plain warm start's relative residual
+    (measured on MI355X: 0.0142 against 0.0396, used 39 of 39), with no more
+    iterations on average."""
     import pic_configs as cfg
     w = cfg.WORKLOADS["electrode_512"].scaled(NX=256, n=200_000, name="learned_256")
     parts = [cfg.generate(w, s) for s in range(2)]
@@ -128,9 +131,10 @@ def test_learned_guess_in_the_solver(hip_mod):
     ra = np.mean([s["initial_relres"] for s in sa[tail]])
     rb = np.mean([s["initial_relres"] for s in sb[tail]])
     print(f"iterations plain {ia:.2f} learned {ib:.2f}; initial relres plain {ra:.3e} learned {rb:.3e}")
-    assert st["trained"] == 40 and st["tried"] == 60 - 1 - 20 and 0 <= st["used"] <= st["tried"]
+    assert st["trained"] == 40 and st["tried"] == 60 - 1 - 20 and st["used"] >= 0.8 * st["tried"]
     assert all(s["final_relres"] <= 1e-8 for s in sb)
-    assert rb <= ra * (1 + 1e-9)       # the guess is used only when it starts lower
+    assert rb <= 0.5 * ra, (ra, rb)
+    assert ib <= ia, (ia, ib)
     a.close()
     b.close()
 
@@ -150,3 +154,35 @@ def test_unet_residual_loss_gradients_match_oracle(hip_mod):
     assert loss == pytest.approx(rloss, rel=1e-2)
     assert _rel(g, rg) < 5e-2
     net.close()
+
+
+def test_training_is_reproducible(hip_mod):
+    """Two networks from the same seed trained on the same data end every step
+    with the same loss, gradients and parameters, bit for bit: the weight,
+    bias, head and loss reductions add per-wave / per-workgroup partials in a
+    fixed order (no floating-point atomics)."""
+    nets = [hip_mod.UNet(N, seed=3) for _ in range(2)]
+    for net in nets:
+        _params(net, 4)
+    x = _inputs(4)
+    rng = np.random.default_rng(9)
+    tgt = rng.normal(0, 0.2, (N, N)).astype(np.float32)
+    r = rng.uniform(-1, 1, (N, N))
+    mask = np.ones((N, N), np.uint8)
+    mask[0, :] = mask[:, 0] = 0
+    for k in range(3):
+        la = nets[0].train_step(x, tgt, mask, 1e-3)
+        lb = nets[1].train_step(x, tgt, mask, 1e-3)
+        assert la == lb, k
+        assert np.array_equal(nets[0].grads(), nets[1].grads()), k
+        assert np.array_equal(nets[0].params(), nets[1].params()), k
+        la = nets[0].train_step_residual(x, r, mask, 0.25, 0.25, 1e-3)
+        lb = nets[1].train_step_residual(x, r, mask, 0.25, 0.25, 1e-3)
+        assert la == lb, k
+        assert np.array_equal(nets[0].grads(), nets[1].grads()), k
+    # the same network on the same data twice (no update): the same gradients
+    g0 = (nets[0].train_step(x, tgt, mask, 0.0), nets[0].grads())
+    g1 = (nets[0].train_step(x, tgt, mask, 0.0), nets[0].grads())
+    assert g0[0] == g1[0] and np.array_equal(g0[1], g1[1])
+    for net in nets:
+        net.close()
