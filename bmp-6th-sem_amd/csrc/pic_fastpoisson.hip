@@ -73,6 +73,13 @@ struct FastPoisson {
     double* Fb = nullptr;                    // n x m: wall values moved to the right-hand side
     FdRun* self = nullptr;                   // device: this run's descriptor (a table of one)
     FdRun* hself = nullptr;                  // host copy
+    // line form of the correction (FFT transforms, one-pass covering lines):
+    // the sources of covering line v are lt[lptr[v] .. lptr[v+1]) (indices
+    // into cq), at row-of-S index lidx (the column for a row line, the row
+    // for a column line); LV holds one vector per line (lr x m, then lc x n)
+    int line_form = 0;
+    int *lptr = nullptr, *lt = nullptr, *lidx = nullptr;
+    double* LV = nullptr;
 };
 
 static void* dmalloc(size_t bytes, bool& ok) {
@@ -88,7 +95,8 @@ void fast_free(pic_ctx* c) {
     FastPoisson* f = c->fast;
     if (!f) return;
     void* ptrs[] = {f->Sm, f->Sn, f->Dinv, f->F, f->W, f->U, f->G, f->SnR, f->V1, f->SmC, f->V2, f->SnQ, f->cq,
-                    f->rq_ptr, f->rq_idx, f->rq_col, f->Z, f->Cinv, f->VQ, f->dq, f->qri, f->qcol, f->qrow, f->part, f->part2, f->twm, f->twn, f->Fb, f->self};
+                    f->rq_ptr, f->rq_idx, f->rq_col, f->Z, f->Cinv, f->VQ, f->dq, f->qri, f->qcol, f->qrow, f->part, f->part2, f->twm, f->twn, f->Fb, f->self,
+                    f->lptr, f->lt, f->lidx, f->LV};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (f->blas) (void)rocblas_destroy_handle(f->blas);
@@ -161,6 +169,9 @@ struct FdRun {
     double *cq, *Z;
     const double* SnQ;
     const double2 *twm, *twn;
+    int cmode;                               // correction: 0 none, 1 the spectrum G, 2 line form (LV)
+    const int *lptr, *lt, *lidx;
+    double* LV;
 };
 
 // d_q = V_q - u0(r, c) at q = (r, c), u0 = S_n Uhat S_m.  A q covered by row a
@@ -351,6 +362,46 @@ __global__ __launch_bounds__(kFdB) void k_fd_corr(const FdRun one, const FdRun* 
     }
 }
 
+// Line form of the correction's spectrum.  Every q of Q lies on exactly one
+// covering line, so G = sum over row lines a of S_n[row a] (x) Zr_a plus sum
+// over column lines j of Vc_j (x) S_m[col j], with
+//   Zr_a[l] = sum_{q on row line a} c_q S_m[col q][l]   (length m),
+//   Vc_j[k] = sum_{q on column line j} c_q S_n[row q][k] (length n):
+// a handful of rank-1 terms (four for a rectangle) that the third transform
+// pass adds while it loads the spectrum, instead of a dense n x nq x m
+// product.  Workgroup (64 outputs, line): the line's sources and their S
+// rows are staged in LDS, wave w sums sources w, w + 4, ..., and the four
+// wave sums are added in a fixed order.
+template <bool B>
+__global__ __launch_bounds__(kFdB) void k_fd_linevec(const FdRun one, const FdRun* __restrict__ runs) {
+    __shared__ double cs[kFdMaxP];
+    __shared__ int li[kFdMaxP];
+    __shared__ double part[4][64];
+    const FdRun& f = B ? runs[blockIdx.z] : one;
+    const int line = blockIdx.y;
+    const bool row = line < f.lr;
+    const int len = row ? f.m : f.n;
+    if (line >= f.lr + f.lc || (int)blockIdx.x * 64 >= len) return;   // uniform over the workgroup
+    const double* S = row ? f.Sm : f.Sn;
+    const int t0 = f.lptr[line], nt = f.lptr[line + 1] - t0;
+    for (int u = threadIdx.x; u < nt; u += kFdB) {
+        cs[u] = f.cq[f.lt[t0 + u]];
+        li[u] = f.lidx[t0 + u];
+    }
+    __syncthreads();
+    const int lx = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int o = blockIdx.x * 64 + lx, oc = min(o, len - 1);
+    double sum = 0.0;
+#pragma unroll 8
+    for (int u = w; u < nt; u += 4) sum += cs[u] * S[(size_t)li[u] * len + oc];
+    part[w][lx] = sum;
+    __syncthreads();
+    if (w == 0 && o < len) {
+        const size_t base = row ? (size_t)line * f.m : (size_t)f.lr * f.m + (size_t)(line - f.lr) * f.n;
+        f.LV[base + o] = (part[0][lx] + part[1][lx]) + (part[2][lx] + part[3][lx]);
+    }
+}
+
 // full-grid solution: interior unknowns from U, every Dirichlet node its value
 __global__ __launch_bounds__(kFdB) void k_fd_out(int gx, int gy, const double* __restrict__ U,
                                                  const uint8_t* __restrict__ dir, const double* __restrict__ dval,
@@ -409,13 +460,30 @@ struct DstLoadRows {
     __device__ double operator()(int row, int col) const { return X[(size_t)row * ld + col]; }
 };
 
-// rows of U + G .* Dinv (the corrected spectrum; U alone without sources)
+// rows of U + G .* Dinv (the corrected spectrum; U alone without sources);
+// G the stored spectrum (mode 1) or its line form (mode 2: the rank-1 terms
+// of the covering lines, k_fd_linevec)
 struct DstLoadCorrected {
     const double *U, *G, *Dinv;
-    int ld, corr;
+    int ld, mode, n, lr, lc;
+    const double *SnR, *SmC, *LV;
     __device__ double operator()(int row, int col) const {
         const size_t i = (size_t)row * ld + col;
-        return corr ? U[i] + G[i] * Dinv[i] : U[i];
+        if (mode == 0) return U[i];
+        if (mode == 1) return U[i] + G[i] * Dinv[i];
+        // the first two lines of each kind unrolled (a rectangle's cover has
+        // two rows and two columns), so their loads issue together
+        double g = 0.0;
+        const double* V = LV + (size_t)lr * ld;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+            if (a < lr) g += SnR[(size_t)a * n + row] * LV[(size_t)a * ld + col];
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            if (j < lc) g += V[(size_t)j * n + row] * SmC[(size_t)j * ld + col];
+        for (int a = 2; a < lr; ++a) g += SnR[(size_t)a * n + row] * LV[(size_t)a * ld + col];
+        for (int j = 2; j < lc; ++j) g += V[(size_t)j * n + row] * SmC[(size_t)j * ld + col];
+        return U[i] + g * Dinv[i];
     }
 };
 
@@ -609,7 +677,9 @@ __global__ __launch_bounds__(kDstT) void k_fd_dst(const FdRun one, const FdRun* 
     else if constexpr (KIND == 2)
         dst_body<L>(R, nwg, DstLoadRows{f.W, f.n}, DstStoreTScaled{f.U, f.Dinv, f.m}, f.twn);
     else if constexpr (KIND == 3)
-        dst_body<L>(R, nwg, DstLoadCorrected{f.U, f.G, f.Dinv, f.m, f.P > 0}, DstStoreT{f.W, f.n}, f.twm);
+        dst_body<L>(R, nwg,
+                    DstLoadCorrected{f.U, f.G, f.Dinv, f.m, f.cmode, f.n, f.lr, f.lc, f.SnR, f.SmC, f.LV},
+                    DstStoreT{f.W, f.n}, f.twm);
     else
         dst_body<L>(R, nwg, DstLoadRows{f.W, f.n}, DstStoreGrid{f.phi, f.gx}, f.twn);
 }
@@ -764,6 +834,11 @@ static FdRun run_of(const pic_ctx* c, double sgn) {
     r.SnQ = f->SnQ;
     r.twm = f->twm;
     r.twn = f->twn;
+    r.cmode = f->P == 0 ? 0 : (f->line_form ? 2 : 1);
+    r.lptr = f->lptr;
+    r.lt = f->lt;
+    r.lidx = f->lidx;
+    r.LV = f->LV;
     return r;
 }
 
@@ -788,6 +863,11 @@ static int enqueue_correction(FastPoisson* f, const FdRuns& r, hipStream_t s) {
     const unsigned wq = (unsigned)((P * 64 + kFdB - 1) / kFdB);
     launch_fd<B>(k_fd_u0q<B>, dim3(wq), r, s);
     launch_fd<B>(k_fd_sources<B>, dim3(wq), r, s);
+    if (f->line_form) {
+        // one vector per covering line; the third transform pass adds the terms
+        launch_fd<B>(k_fd_linevec<B>, dim3((unsigned)((std::max(m, n) + 63) / 64), (unsigned)(f->nr + f->nc)), r, s);
+        return PIC_OK;
+    }
     launch_fd<B>(k_fd_zrows<B>, dim3((unsigned)((m + 63) / 64), (unsigned)f->nq), r, s);
     // G = S_n[:, rows of Q] Z (n x nq times nq x m)
     launch_fd<B>(k_fd_corr<B>, dim3((unsigned)((m + kCT - 1) / kCT), (unsigned)((n + kCT - 1) / kCT)), r, s);
@@ -1033,6 +1113,36 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
         PIC_HIP(hipMemcpy(f->Cinv, C.data(), (size_t)P * P * 8, hipMemcpyHostToDevice));
     }
     if (qrow_d) (void)hipFree(qrow_d);
+    if (P > 0 && f->fft && nr <= kLines && nc <= kLines) {
+        // line form: the sources of each covering line (rows first, then
+        // columns), q ascending; lt indexes the row-ordered sources cq
+        std::vector<int> tpos(P);
+        for (int t = 0; t < P; ++t) tpos[rq_idx[t]] = t;
+        std::vector<std::vector<int>> members(nr + nc);
+        for (int q = 0; q < P; ++q) members[qri[q] >= 0 ? qri[q] : nr + (-1 - qri[q])].push_back(q);
+        std::vector<int> lptr(1, 0), lt, lidx;
+        for (int v = 0; v < nr + nc; ++v) {
+            for (int q : members[v]) {
+                lt.push_back(tpos[q]);
+                lidx.push_back(v < nr ? qcol[q] : qrow[q]);
+            }
+            lptr.push_back((int)lt.size());
+        }
+        bool okl = true;
+        f->lptr = (int*)dmalloc(lptr.size() * 4, okl);
+        f->lt = (int*)dmalloc(lt.size() * 4, okl);
+        f->lidx = (int*)dmalloc(lidx.size() * 4, okl);
+        f->LV = (double*)dmalloc(((size_t)nr * m + (size_t)nc * n) * 8, okl);
+        if (!okl) {
+            fast_free(c);
+            set_error("direct Poisson solve: device allocation failed");
+            return PIC_ERR_ALLOC;
+        }
+        PIC_HIP(hipMemcpy(f->lptr, lptr.data(), lptr.size() * 4, hipMemcpyHostToDevice));
+        PIC_HIP(hipMemcpy(f->lt, lt.data(), lt.size() * 4, hipMemcpyHostToDevice));
+        PIC_HIP(hipMemcpy(f->lidx, lidx.data(), lidx.size() * 4, hipMemcpyHostToDevice));
+        f->line_form = 1;
+    }
     const FdRun self = run_of(c, 1.0);
     if (hipMalloc((void**)&f->self, sizeof(FdRun)) != hipSuccess) {
         fast_free(c);

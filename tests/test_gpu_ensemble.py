@@ -1,10 +1,12 @@
 """Batched ensemble of uneven runs (persistent kernels, one captured HIP graph
-per step) against serial runs of the same contexts: rho bit-identical on the
-first step (same particles, integer deposit), phi to 1e-9 relative (batched
-direct solves, or the ensemble's PCG converged to 1e-12, against the serial
-solves), live particle counts within 2; graph replay identical to direct
-launches; the autotuner's choice is valid; mixed ensembles (two batched
-groups, a GEMM-transform grid and a PCG-only run)."""
+per step) against serial runs of the same contexts.  Runs whose Poisson solve
+is a batched direct solve match their serial twins bit for bit on every step
+(rho, phi, live and per-cell counts: the same integer deposit, the same
+transform passes, the same push); runs the ensemble's one-workgroup PCG
+solves match to 1e-9 (another PCG implementation, both at 1e-12).  Graph
+replay is identical to direct launches; the autotuner's choice is valid;
+mixed ensembles (two batched groups, a GEMM-transform grid and a PCG-only
+run)."""
 import numpy as np
 import pytest
 
@@ -41,13 +43,13 @@ def test_ensemble_matches_serial_runs(hip_mod):
         for k, (a, b) in enumerate(zip(ens_ctx, ser_ctx)):
             ra, rb = a.grid(hip.GRID_RHO), b.grid(hip.GRID_RHO)
             pa, pb = a.grid(hip.GRID_PHI), b.grid(hip.GRID_PHI)
-            if it == 0:
-                assert np.array_equal(ra, rb), (it, k)
-            else:
-                assert np.abs(ra - rb).max() <= 1e-9 * np.abs(rb).max(), (it, k)
-            assert np.abs(pa - pb).max() <= 1e-9 * np.abs(pb).max(), (it, k)
+            # same kernels for the particles, the same direct-solve passes
+            # (batched over the runs of one shape): the same numbers
+            assert np.array_equal(ra, rb), (it, k)
+            assert np.array_equal(pa, pb), (it, k, np.abs(pa - pb).max())
             for s in range(2):
-                assert abs(a.count(s) - b.count(s)) <= 2, (it, k, s)
+                assert a.count(s) == b.count(s), (it, k, s)
+                assert np.array_equal(a.cell_counts(s), b.cell_counts(s)), (it, k, s)
     # the runs stay usable through the ordinary API: one more serial step each
     for c in ens_ctx:
         c.step(3)
@@ -73,10 +75,9 @@ def test_ensemble_from_runs_stepped_serially(hip_mod):
         c.step(2)
     for k, (a, b) in enumerate(zip(ens_ctx, ser_ctx)):
         assert np.array_equal(a.grid(hip.GRID_RHO), b.grid(hip.GRID_RHO)), k
-        pa, pb = a.grid(hip.GRID_PHI), b.grid(hip.GRID_PHI)
-        assert np.abs(pa - pb).max() <= 1e-9 * np.abs(pb).max(), k
+        assert np.array_equal(a.grid(hip.GRID_PHI), b.grid(hip.GRID_PHI)), k
         for s in range(2):
-            assert abs(a.count(s) - b.count(s)) <= 2, (k, s)
+            assert a.count(s) == b.count(s), (k, s)
     ens.close()
     for c in ens_ctx + ser_ctx:
         c.close()
@@ -147,11 +148,21 @@ def test_ensemble_mixed_direct_batches_and_pcg_runs(hip_mod):
             assert iters[2] > 0, iters                        # PCG from zero
         for k, (a, b) in enumerate(zip(ens_ctx, ser_ctx)):
             pa, pb = a.grid(hip.GRID_PHI), b.grid(hip.GRID_PHI)
-            if it == 0:
+            if it == 0 or k in (0, 1, 4, 5):
                 assert np.array_equal(a.grid(hip.GRID_RHO), b.grid(hip.GRID_RHO)), (it, k)
-            assert np.abs(pa - pb).max() <= 1e-9 * np.abs(pb).max(), (it, k)
-            for s in range(2):
-                assert abs(a.count(s) - b.count(s)) <= 2, (it, k, s)
+            if k in (0, 1, 4, 5):
+                # batched direct solves: the single-run passes, the same numbers
+                assert np.array_equal(pa, pb), (it, k)
+                for s in range(2):
+                    assert a.count(s) == b.count(s), (it, k, s)
+            else:
+                # solved by the ensemble's one-workgroup PCG against the serial
+                # multi-kernel PCG, both to 1e-12: phi agrees to rounding, and
+                # a particle within that rounding of a cell face or a wall may
+                # end on the other side, so the counts may differ by a few
+                assert np.abs(pa - pb).max() <= 1e-9 * np.abs(pb).max(), (it, k)
+                for s in range(2):
+                    assert abs(a.count(s) - b.count(s)) <= 2, (it, k, s)
     ens.close()
     for c in ens_ctx + ser_ctx:
         c.close()
