@@ -427,8 +427,13 @@ __global__ __launch_bounds__(kBlock) void k_ens_push(const EnsRun* __restrict__ 
         const EnsRun& run = runs[r];
         const EnsSpecies& sp = run.sp[s];
         const int tpr = (run.g.gx + TC - 1) / TC;
-        push_tile_body<PIC_LAYOUT_CELL32, 0, TC, true>(sp.src, run.g, sp.sd, tpr, k - prefix[r], run.fld, sp.start,
-                                                       sp.counts, sp.rank, sp.absorbed, hist, codes, nabs, wf);
+        // a run's uniform Bz takes the constant rotation, as its serial push does
+        if (sp.sd.ub)
+            push_tile_body<PIC_LAYOUT_CELL32, 2, TC, true>(sp.src, run.g, sp.sd, tpr, k - prefix[r], run.fld, sp.start,
+                                                           sp.counts, sp.rank, sp.absorbed, hist, codes, nabs, wf);
+        else
+            push_tile_body<PIC_LAYOUT_CELL32, 0, TC, true>(sp.src, run.g, sp.sd, tpr, k - prefix[r], run.fld, sp.start,
+                                                           sp.counts, sp.rank, sp.absorbed, hist, codes, nabs, wf);
     }
 }
 
@@ -492,7 +497,8 @@ __global__ __launch_bounds__(kBlock) void k_ens_scatter(const EnsRun* __restrict
             const PState<PIC_LAYOUT_CELL32> p = load_state<PIC_LAYOUT_CELL32>(sp.src, i);
             const int rk = sp.rank[i];
             const Corners c = gather<PIC_LAYOUT_CELL32>(p, run.g, run.fld);
-            const Pushed<PIC_LAYOUT_CELL32> o = push_state<PIC_LAYOUT_CELL32, 0>(p, c, run.g, sp.sd);
+            const Pushed<PIC_LAYOUT_CELL32> o = sp.sd.ub ? push_state<PIC_LAYOUT_CELL32, 2>(p, c, run.g, sp.sd)
+                                                         : push_state<PIC_LAYOUT_CELL32, 0>(p, c, run.g, sp.sd);
             if (o.key < 0) continue;
             const int d = sp.start[o.key] + rk;
             store_state<PIC_LAYOUT_CELL32>(sp.dst, d, o);

@@ -290,6 +290,8 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
     (void)hipMemset(c->scal, 0, 16 * sizeof(double));
     {
         std::vector<double> b(G, p->b0);   // initMagneticField: uniform B0 (particles.cpp:709-719)
+        c->bz_uniform = 1;
+        c->bz0 = p->b0;
         if (hipMemcpy(c->bz, b.data(), G * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
             set_error("pic_create: copy of Bz failed");
             return fail(PIC_ERR_HIP);
@@ -814,6 +816,16 @@ extern "C" int pic_set_grid(pic_ctx* c, int which, const double* in) {
     }
     PIC_HIP(hipStreamSynchronize(c->stream));
     PIC_HIP(hipMemcpy(p, in, c->G * sizeof(double), hipMemcpyHostToDevice));
+    if (which == PIC_GRID_BZ) {
+        // the push's constant-rotation mode needs one value on every node
+        c->bz0 = in[0];
+        c->bz_uniform = 1;
+        for (int n = 1; n < c->G; ++n)
+            if (in[n] != in[0]) {
+                c->bz_uniform = 0;
+                break;
+            }
+    }
     if (which == PIC_GRID_EX || which == PIC_GRID_EY || which == PIC_GRID_BZ) {
         hipLaunchKernelGGL(k_pack_fld, dim3((c->G + 255) / 256), dim3(256), 0, c->stream, c->G, c->ex, c->ey, c->bz,
                            c->fld);

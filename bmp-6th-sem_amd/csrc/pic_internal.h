@@ -60,6 +60,13 @@ struct SpeciesDev {
     double dtdx, dtdy; // dt / dx, dt / dy (cell-unit position update)
     double bx, by;    // uniform in-plane B (0 = reference Bz-only push)
     int    general_b; // 1 when bx or by is non-zero
+    // CELL32 with a uniform Bz (push modes 2, 3): the rotation is the same for
+    // every particle, so its factors are formed once here, and the electric
+    // half kick is ce * E with ce = (0.5 q/m) dt
+    int    ub;        // 1 when these constants apply
+    double ce;
+    double ut[3];     // t = (0.5 q/m) B dt  (Bz-only: ut[2] is t)
+    double us[3];     // s = 2 t / (1 + |t|^2)
 };
 
 // Particle storage: structure of arrays, double-buffered for the sort.
@@ -198,6 +205,8 @@ struct pic_ctx {
     double* ex = nullptr;      // separate components for the API / diagnostics
     double* ey = nullptr;
     double* bz = nullptr;
+    int bz_uniform = 1;        // every node of bz holds the same value, bz0
+    double bz0 = 0.0;
     double4* fld = nullptr;    // packed (Ex, Ey, Bz, 0) per node, padded with zeros
     uint8_t* dir = nullptr;    // Dirichlet mask (fine grid)
     double* dval = nullptr;    // Dirichlet value per node (0 elsewhere)

@@ -357,6 +357,10 @@ template <int L> struct Pushed {
     V vx, vy, vz;
 };
 
+// Push modes (template parameter GENB): bit 0 — an in-plane B, the rotation
+// about the full vector t = (q/m) B dt / 2; bit 1 — CELL32 with a uniform Bz:
+// t and s are the species' constants (SpeciesDev::ut, us), Bz is not
+// gathered and the half kick is ce * E (ce = (0.5 q/m) dt).
 template <int GENB>
 __device__ __forceinline__ void boris(double Epx, double Epy, double Bpz, const SpeciesDev& s,
                                       double& pvx, double& pvy, double& pvz) {
@@ -364,7 +368,7 @@ __device__ __forceinline__ void boris(double Epx, double Epy, double Bpz, const 
     double hy = s.half_qm * Epy * s.dt;
     double Vxminus = pvx + hx;
     double Vyminus = pvy + hy;
-    if (!GENB) {
+    if (!(GENB & 1)) {
         double t = s.half_qm * Bpz * s.dt;
         double Vxprime = Vxminus + Vyminus * t;
         double Vyprime = Vyminus - Vxminus * t;
@@ -381,6 +385,31 @@ __device__ __forceinline__ void boris(double Epx, double Epy, double Bpz, const 
         double vpz = Vzminus + (Vxminus * ty - Vyminus * tx);
         double rden = 1 / (1 + (tx * tx + ty * ty + tz * tz));
         double sx = (2 * tx) * rden, sy = (2 * ty) * rden, sz = (2 * tz) * rden;
+        pvx = Vxminus + (vpy * sz - vpz * sy) + hx;
+        pvy = Vyminus + (vpz * sx - vpx * sz) + hy;
+        pvz = Vzminus + (vpx * sy - vpy * sx);
+    }
+}
+
+// The same rotation with the species' constant t and s (uniform B).
+template <int GB>
+__device__ __forceinline__ void boris_const(double hx, double hy, const SpeciesDev& s, double& pvx, double& pvy,
+                                            double& pvz) {
+    double Vxminus = pvx + hx;
+    double Vyminus = pvy + hy;
+    if (!GB) {
+        const double t = s.ut[2], sf = s.us[2];
+        double Vxprime = Vxminus + Vyminus * t;
+        double Vyprime = Vyminus - Vxminus * t;
+        pvx = Vxminus + Vyprime * sf + hx;
+        pvy = Vyminus - Vxprime * sf + hy;
+    } else {
+        const double tx = s.ut[0], ty = s.ut[1], tz = s.ut[2];
+        const double sx = s.us[0], sy = s.us[1], sz = s.us[2];
+        double Vzminus = pvz;
+        double vpx = Vxminus + (Vyminus * tz - Vzminus * ty);
+        double vpy = Vyminus + (Vzminus * tx - Vxminus * tz);
+        double vpz = Vzminus + (Vxminus * ty - Vyminus * tx);
         pvx = Vxminus + (vpy * sz - vpz * sy) + hx;
         pvy = Vyminus + (vpz * sx - vpx * sz) + hy;
         pvz = Vzminus + (vpx * sy - vpy * sx);
@@ -458,8 +487,9 @@ __device__ __forceinline__ void load_field_window(const double4* __restrict__ fl
     }
 }
 
-// The corners of a particle in cell `cell` of the tile, from the window.
-template <int TC>
+// The corners of a particle in cell `cell` of the tile, from the window
+// (Bz only with NEEDB; push mode 2/3 does not read it).
+template <int TC, bool NEEDB = true>
 __device__ __forceinline__ Corners gather_window(int cell, int k0, int c0, int r, const double* wf) {
     constexpr int kFW = 2 * (TC + 1);
     Corners c;
@@ -467,10 +497,11 @@ __device__ __forceinline__ Corners gather_window(int cell, int k0, int c0, int r
     c.cgx = c0 + lc;
     c.cgy = r;
     const int u = TC + 1 + lc;
-    c.f00 = make_double4(wf[lc], wf[kFW + lc], wf[2 * kFW + lc], 0.0);
-    c.f10 = make_double4(wf[lc + 1], wf[kFW + lc + 1], wf[2 * kFW + lc + 1], 0.0);
-    c.f01 = make_double4(wf[u], wf[kFW + u], wf[2 * kFW + u], 0.0);
-    c.f11 = make_double4(wf[u + 1], wf[kFW + u + 1], wf[2 * kFW + u + 1], 0.0);
+    auto bz = [&](int j) { return NEEDB ? wf[2 * kFW + j] : 0.0; };
+    c.f00 = make_double4(wf[lc], wf[kFW + lc], bz(lc), 0.0);
+    c.f10 = make_double4(wf[lc + 1], wf[kFW + lc + 1], bz(lc + 1), 0.0);
+    c.f01 = make_double4(wf[u], wf[kFW + u], bz(u), 0.0);
+    c.f11 = make_double4(wf[u + 1], wf[kFW + u + 1], bz(u + 1), 0.0);
     return c;
 }
 
@@ -517,8 +548,12 @@ __device__ __forceinline__ Pushed<L> push_state(const PState<L>& p, const Corner
         double w01 = (1.0 - fx) * fy, w11 = fx * fy;
         double Epx = f00.x * w00 + f10.x * w10 + f01.x * w01 + f11.x * w11;
         double Epy = f00.y * w00 + f10.y * w10 + f01.y * w01 + f11.y * w11;
-        double Bpz = f00.z * w00 + f10.z * w10 + f01.z * w01 + f11.z * w11;
-        boris<GENB>(Epx, Epy, Bpz, s, pvx, pvy, pvz);
+        if constexpr ((GENB & 2) != 0) {
+            boris_const<GENB & 1>(s.ce * Epx, s.ce * Epy, s, pvx, pvy, pvz);
+        } else {
+            double Bpz = f00.z * w00 + f10.z * w10 + f01.z * w01 + f11.z * w11;
+            boris<GENB>(Epx, Epy, Bpz, s, pvx, pvy, pvz);
+        }
         double tx = ((double)c.cgx + fx) + pvx * s.dtdx;
         double ty = ((double)c.cgy + fy) + pvy * s.dtdy;
         if (g.periodic) ty = modulo_ref(ty, g.ny);
@@ -651,7 +686,7 @@ __device__ __forceinline__ void push_tile_body(
         const bool have = i < end;
         Corners cn;
         if (have) {
-            if constexpr (WF) cn = gather_window<TC>(cur.cell, k0, c0, r, wf);
+            if constexpr (WF) cn = gather_window<TC, (GENB & 2) == 0>(cur.cell, k0, c0, r, wf);
             else cn = gather<L>(cur, g, fld);
             if constexpr (SC)
                 if ((i & (kBlock - 1)) == 0) blk_first[i / kBlock] = cur.cell;

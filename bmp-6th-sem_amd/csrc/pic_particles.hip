@@ -434,12 +434,18 @@ __global__ __launch_bounds__(kBlock) void k_deposit_tile(DepositSet ds, GridDev 
 // LDS with the range's cell starts (start_old: the order before this sort),
 // and each particle's cell is found among those starts; otherwise by a search
 // of start_old over [kf, G).  The cell array is neither read nor written.
+// (Staging the new starts of the keys the workgroup's particles reach in LDS
+// too made the kernel slower: those loads precede the barrier, while the
+// per-particle start load overlaps the push arithmetic.)
 // Workgroups 0 .. nb0-1 (in XCD order) serve species 0, the rest species 1.
 static constexpr int kScatterWin = 130;
 
 template <int L, int GENB>
 __global__ __launch_bounds__(kBlock) void k_scatter_push(ScatterSet ss, GridDev g, const double4* __restrict__ fld)
 {
+    constexpr bool C32 = L == PIC_LAYOUT_CELL32;
+    __shared__ double4 wf[C32 ? 2 * kScatterWin : 1];
+    __shared__ int sst[C32 ? kScatterWin : 1];
     // consecutive particle blocks on one XCD: the partial cache lines two
     // neighbouring blocks write at their boundary meet in one L2
     const int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -452,12 +458,11 @@ __global__ __launch_bounds__(kBlock) void k_scatter_push(ScatterSet ss, GridDev 
     const int i = i0 + threadIdx.x;
     const bool have = i < n;
     PState<L> p;
-    if (have) p = load_state<L, L != PIC_LAYOUT_CELL32>(src, i);
+    if (have) p = load_state<L, !C32>(src, i);
     const int rk = have ? ss.rank[sp][i] : 0;
+    const int32_t* __restrict__ nstart = ss.start[sp];
     Corners c;
-    if constexpr (L == PIC_LAYOUT_CELL32) {
-        __shared__ double4 wf[2 * kScatterWin];
-        __shared__ int sst[kScatterWin];
+    if constexpr (C32) {
         const int32_t* __restrict__ start_old = ss.start_old[sp];
         const int32_t* __restrict__ blk_first = ss.blk_first[sp];
         const int kf = blk_first[b];
@@ -491,9 +496,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter_push(ScatterSet ss, GridDev 
     }
     const Pushed<L> o = push_state<L, GENB>(p, c, g, ss.d[sp]);
     if (o.key < 0) return;
-    const int d = ss.start[sp][o.key] + rk;
+    const int d = nstart[o.key] + rk;
     store_state<L>(ss.dst[sp], d, o);
-    if constexpr (L != PIC_LAYOUT_CELL32) ss.dst[sp].cell[d] = o.key;
+    if constexpr (!C32) ss.dst[sp].cell[d] = o.key;
 }
 
 // ============================================================ launchers
@@ -649,7 +654,31 @@ SpeciesDev species_dev(const pic_ctx* c, const Species& sp) {
     d.bx = c->p.bx;
     d.by = c->p.by;
     d.general_b = (c->p.bx != 0.0 || c->p.by != 0.0);
+    // constant rotation (CELL32, uniform Bz): t and s as the per-particle
+    // push forms them from the gathered Bz, here from the uniform value
+    d.ub = c->p.layout == PIC_LAYOUT_CELL32 && c->bz_uniform;
+    d.ce = d.half_qm * d.dt;
+    const double tx = d.half_qm * d.bx * d.dt, ty = d.half_qm * d.by * d.dt, tz = d.half_qm * c->bz0 * d.dt;
+    d.ut[0] = tx;
+    d.ut[1] = ty;
+    d.ut[2] = tz;
+    if (d.general_b) {
+        const double rden = 1 / (1 + (tx * tx + ty * ty + tz * tz));
+        d.us[0] = (2 * tx) * rden;
+        d.us[1] = (2 * ty) * rden;
+        d.us[2] = (2 * tz) * rden;
+    } else {
+        d.us[0] = d.us[1] = 0.0;
+        d.us[2] = 2 * tz / (1 + tz * tz);
+    }
     return d;
+}
+
+// Push mode of the launches: bit 0 in-plane B, bit 1 the constant rotation
+// of a uniform Bz (CELL32).
+static int push_mode(const pic_ctx* c) {
+    const int genb = (c->p.bx != 0.0 || c->p.by != 0.0) ? 1 : 0;
+    return genb | ((c->p.layout == PIC_LAYOUT_CELL32 && c->bz_uniform) ? 2 : 0);
 }
 
 static int tiles_per_row(const pic_ctx* c) { return (c->g.gx + kTileC - 1) / kTileC; }
@@ -794,18 +823,16 @@ int launch_push(pic_ctx* c, int s) {
     if (n > 0) {
         dim3 grid(blocks_for(n)), blk(kBlock);
         const bool f64 = c->p.layout == PIC_LAYOUT_F64;
-        if (f64 && !d.general_b)
-            hipLaunchKernelGGL((k_push<PIC_LAYOUT_F64, 0>), grid, blk, 0, c->stream,
-                               sp.ndev + sp.cur, sp.buf[sp.cur], c->g, d, c->fld, sp.key, sp.absorbed);
-        else if (f64)
-            hipLaunchKernelGGL((k_push<PIC_LAYOUT_F64, 1>), grid, blk, 0, c->stream,
-                               sp.ndev + sp.cur, sp.buf[sp.cur], c->g, d, c->fld, sp.key, sp.absorbed);
-        else if (!d.general_b)
-            hipLaunchKernelGGL((k_push<PIC_LAYOUT_CELL32, 0>), grid, blk, 0, c->stream,
-                               sp.ndev + sp.cur, sp.buf[sp.cur], c->g, d, c->fld, sp.key, sp.absorbed);
-        else
-            hipLaunchKernelGGL((k_push<PIC_LAYOUT_CELL32, 1>), grid, blk, 0, c->stream,
-                               sp.ndev + sp.cur, sp.buf[sp.cur], c->g, d, c->fld, sp.key, sp.absorbed);
+#define PIC_PUSH_ARGS grid, blk, 0, c->stream, sp.ndev + sp.cur, sp.buf[sp.cur], c->g, d, c->fld, sp.key, sp.absorbed
+        switch (f64 ? (d.general_b ? 1 : 0) : 4 + push_mode(c)) {
+        case 0: hipLaunchKernelGGL((k_push<PIC_LAYOUT_F64, 0>), PIC_PUSH_ARGS); break;
+        case 1: hipLaunchKernelGGL((k_push<PIC_LAYOUT_F64, 1>), PIC_PUSH_ARGS); break;
+        case 4: hipLaunchKernelGGL((k_push<PIC_LAYOUT_CELL32, 0>), PIC_PUSH_ARGS); break;
+        case 5: hipLaunchKernelGGL((k_push<PIC_LAYOUT_CELL32, 1>), PIC_PUSH_ARGS); break;
+        case 6: hipLaunchKernelGGL((k_push<PIC_LAYOUT_CELL32, 2>), PIC_PUSH_ARGS); break;
+        default: hipLaunchKernelGGL((k_push<PIC_LAYOUT_CELL32, 3>), PIC_PUSH_ARGS); break;
+        }
+#undef PIC_PUSH_ARGS
         PIC_HIP(hipGetLastError());
     }
     sp.deposit_pending = 0;
@@ -851,16 +878,20 @@ int move_push_set(pic_ctx* c, const int* ids, int n, const int* guard) {
     const int tpr = tiles_per_row(c);
     const int ntiles = tpr * c->g.gy;
     const bool f64 = c->p.layout == PIC_LAYOUT_F64;
-    const bool genb = c->p.bx != 0.0 || c->p.by != 0.0;
+    const int mode = push_mode(c);
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_PUSH, &ev));
     if (ts.ns > 0) {
         hipEvent_t kt[2];
         PIC_TRY(kt_begin(c, kt, PIC_STAGE_PUSH));
-        if (f64 && !genb) tile_kernel<PIC_LAYOUT_F64, 0>(c, ts, tpr, ntiles, kt, guard);
-        else if (f64) tile_kernel<PIC_LAYOUT_F64, 1>(c, ts, tpr, ntiles, kt, guard);
-        else if (!genb) tile_kernel<PIC_LAYOUT_CELL32, 0>(c, ts, tpr, ntiles, kt, guard);
-        else tile_kernel<PIC_LAYOUT_CELL32, 1>(c, ts, tpr, ntiles, kt, guard);
+        switch (f64 ? (mode & 1) : 4 + mode) {
+        case 0: tile_kernel<PIC_LAYOUT_F64, 0>(c, ts, tpr, ntiles, kt, guard); break;
+        case 1: tile_kernel<PIC_LAYOUT_F64, 1>(c, ts, tpr, ntiles, kt, guard); break;
+        case 4: tile_kernel<PIC_LAYOUT_CELL32, 0>(c, ts, tpr, ntiles, kt, guard); break;
+        case 5: tile_kernel<PIC_LAYOUT_CELL32, 1>(c, ts, tpr, ntiles, kt, guard); break;
+        case 6: tile_kernel<PIC_LAYOUT_CELL32, 2>(c, ts, tpr, ntiles, kt, guard); break;
+        default: tile_kernel<PIC_LAYOUT_CELL32, 3>(c, ts, tpr, ntiles, kt, guard); break;
+        }
         PIC_HIP(hipGetLastError());
         kt_end(c, PIC_STAGE_PUSH, kt);
     }
@@ -923,17 +954,21 @@ int move_scatter_set(pic_ctx* c, const int* ids, int n) {
     }
     ss.nb0 = (int)nb[0];
     const bool f64 = c->p.layout == PIC_LAYOUT_F64;
-    const bool genb = c->p.bx != 0.0 || c->p.by != 0.0;
+    const int mode = push_mode(c);
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_SCATTER, &ev));
     if (ss.ns > 0) {
         const unsigned grid = nb[0] + nb[1];
         hipEvent_t kt[2];
         PIC_TRY(kt_begin(c, kt, PIC_STAGE_SCATTER));
-        if (f64 && !genb) scatter_kernel<PIC_LAYOUT_F64, 0>(c, ss, grid, kt);
-        else if (f64) scatter_kernel<PIC_LAYOUT_F64, 1>(c, ss, grid, kt);
-        else if (!genb) scatter_kernel<PIC_LAYOUT_CELL32, 0>(c, ss, grid, kt);
-        else scatter_kernel<PIC_LAYOUT_CELL32, 1>(c, ss, grid, kt);
+        switch (f64 ? (mode & 1) : 4 + mode) {
+        case 0: scatter_kernel<PIC_LAYOUT_F64, 0>(c, ss, grid, kt); break;
+        case 1: scatter_kernel<PIC_LAYOUT_F64, 1>(c, ss, grid, kt); break;
+        case 4: scatter_kernel<PIC_LAYOUT_CELL32, 0>(c, ss, grid, kt); break;
+        case 5: scatter_kernel<PIC_LAYOUT_CELL32, 1>(c, ss, grid, kt); break;
+        case 6: scatter_kernel<PIC_LAYOUT_CELL32, 2>(c, ss, grid, kt); break;
+        default: scatter_kernel<PIC_LAYOUT_CELL32, 3>(c, ss, grid, kt); break;
+        }
         PIC_HIP(hipGetLastError());
         kt_end(c, PIC_STAGE_SCATTER, kt);
     }

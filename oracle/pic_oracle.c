@@ -359,8 +359,34 @@ void orc_push_f64(int64_t n, double* x, double* y, double* vx, double* vy, doubl
  * units: bilinear weights (1-fx)(1-fy), fx(1-fy), (1-fx)fy, fx*fy on nodes
  * cell, +1, +GX, +GX+1; the reference's Boris step; new position
  * t = (cgx + fx) + v * (dt * (1.0/dx)); absorbed when t >= NX or t < 0 (same
- * for y); new cell (int)t and offset (float)(t - (int)t) kept below 1. */
+ * for y); new cell (int)t and offset (float)(t - (int)t) kept below 1.
+ * When every node holds the same Bz (the reference's initMagneticField,
+ * src/particles.cpp:709-719), the rotation is the same for every particle:
+ * t = (0.5 q/m) B dt and s = 2t / (1 + |t|^2) are formed once from that
+ * value (instead of from the gathered Bz, which can differ from it in the
+ * last bit), and the half kick is ce * E with ce = (0.5 q/m) dt. */
 static inline float clamp_offset(float o) { return o < 1.0f ? o : 0x1.fffffep-1f; }
+
+/* The Boris rotation of orc_boris with constant t and s (the uniform-B form). */
+static inline void orc_boris_const(double hx, double hy, int genb, const double* t, const double* s,
+                                   double* pvx_, double* pvy_, double* pvz_) {
+    double Vxminus = *pvx_ + hx;
+    double Vyminus = *pvy_ + hy;
+    if (!genb) {
+        double Vxprime = Vxminus + Vyminus * t[2];
+        double Vyprime = Vyminus - Vxminus * t[2];
+        *pvx_ = Vxminus + Vyprime * s[2] + hx;
+        *pvy_ = Vyminus - Vxprime * s[2] + hy;
+    } else {
+        double Vzminus = *pvz_;
+        double vpx = Vxminus + (Vyminus * t[2] - Vzminus * t[1]);
+        double vpy = Vyminus + (Vzminus * t[0] - Vxminus * t[2]);
+        double vpz = Vzminus + (Vxminus * t[1] - Vyminus * t[0]);
+        *pvx_ = Vxminus + (vpy * s[2] - vpz * s[1]) + hx;
+        *pvy_ = Vyminus + (vpz * s[0] - vpx * s[2]) + hy;
+        *pvz_ = Vzminus + (vpx * s[1] - vpy * s[0]);
+    }
+}
 
 void orc_push_cell32_p(int64_t n, int32_t* cell, float* ox, float* oy, float* vx, float* vy, float* vz,
                        int32_t* key, const double* ex, const double* ey, const double* bz, int gxn, int gyn,
@@ -372,6 +398,18 @@ void orc_push_cell32_p(int64_t n, int32_t* cell, float* ox, float* oy, float* vx
     const double NX = gxn - 1, NY = gyn - 1;
     (void)width;
     (void)height;
+    int uniform = 1;
+    for (int64_t k = 1; k < G && uniform; ++k)
+        if (bz[k] != bz[0]) uniform = 0;
+    const int genb = bx != 0.0 || by != 0.0;
+    const double half_qm = 0.5 * qm, ce = half_qm * dt;
+    double ut[3] = {half_qm * bx * dt, half_qm * by * dt, half_qm * bz[0] * dt}, us[3] = {0.0, 0.0, 0.0};
+    if (genb) {
+        double rden = 1 / (1 + (ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]));
+        for (int k = 0; k < 3; ++k) us[k] = (2 * ut[k]) * rden;
+    } else {
+        us[2] = 2 * ut[2] / (1 + ut[2] * ut[2]);
+    }
 #pragma omp parallel for schedule(static) reduction(+ : a0, a1)
     for (int64_t i = 0; i < n; ++i) {
         int c = cell[i];
@@ -383,10 +421,14 @@ void orc_push_cell32_p(int64_t n, int32_t* cell, float* ox, float* oy, float* vx
                      fld_at(ex, G, n11) * w11;
         double Epy = fld_at(ey, G, n00) * w00 + fld_at(ey, G, n10) * w10 + fld_at(ey, G, n01) * w01 +
                      fld_at(ey, G, n11) * w11;
-        double Bpz = fld_at(bz, G, n00) * w00 + fld_at(bz, G, n10) * w10 + fld_at(bz, G, n01) * w01 +
-                     fld_at(bz, G, n11) * w11;
         double pvx = vx[i], pvy = vy[i], pvz = vz[i];
-        orc_boris(Epx, Epy, Bpz, qm, dt, bx, by, &pvx, &pvy, &pvz);
+        if (uniform) {
+            orc_boris_const(ce * Epx, ce * Epy, genb, ut, us, &pvx, &pvy, &pvz);
+        } else {
+            double Bpz = fld_at(bz, G, n00) * w00 + fld_at(bz, G, n10) * w10 + fld_at(bz, G, n01) * w01 +
+                         fld_at(bz, G, n11) * w11;
+            orc_boris(Epx, Epy, Bpz, qm, dt, bx, by, &pvx, &pvy, &pvz);
+        }
         double tx = ((double)cgx + fx) + pvx * dtdx;
         double ty = ((double)cgy + fy) + pvy * dtdy;
         if (periodic) ty = orc_modulo(ty, NY);   /* the wrap in cell units, period NY */

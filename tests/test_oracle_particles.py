@@ -256,11 +256,14 @@ def test_cell32_encode_decode(oracle_mod):
     assert np.abs(xd - x).max() < 1e-7 * dx
 
 
-def test_cell32_push_matches_f64_push(oracle_mod):
+@pytest.mark.parametrize("uniform_bz", [True, False])
+def test_cell32_push_matches_f64_push(oracle_mod, uniform_bz):
     """The CELL32 push (cell units, offsets from the key cell) against the
     FP64 reference push on the decoded particles: positions within a few
     float ulps of a cell, velocities to float rounding, same keys away from
-    cell faces, same absorbed counts."""
+    cell faces, same absorbed counts.  A uniform Bz takes the constant
+    rotation (t and s formed once, half kick ce * E); a Bz that varies by
+    node is gathered per particle."""
     O = oracle_mod
     g = PUSH_GRID
     G = g["gx"] * g["gy"]
@@ -270,7 +273,7 @@ def test_cell32_push_matches_f64_push(oracle_mod):
     y = rng.uniform(0, 32e-4, n)
     vx, vy, vz = rng.normal(0, 3e5, (3, n))
     ex, ey = rng.normal(0, 2e3, (2, G))
-    bz = np.full(G, 0.0433)
+    bz = np.full(G, 0.0433) if uniform_bz else 0.0433 + rng.normal(0, 1e-3, G)
     for bx in (0.0, 0.025):
         cell, ox, oy, fvx, fvy, fvz = O.encode_cell32(x, y, vx, vy, vz, g["gx"], g["dx"], g["dy"])
         xd = ((cell % g["gx"]) + ox.astype(np.float64)) * g["dx"]
