@@ -546,23 +546,75 @@ __global__ void k_zero_f32(float* __restrict__ p, int64_t n) {
     if (k < n) p[k] = 0.f;
 }
 
-// out[j] = sum over slots 0, 1, ..., nslot - 1 of part[slot * len + j], added
-// in slot order: the reductions above leave their partials in slots, so a
-// gradient or loss does not depend on the order the workgroups ran in.
-__global__ void k_sum_slots(const float* __restrict__ part, int nslot, int64_t len, float* __restrict__ out) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= len) return;
+// out[j] = the sum over slots of part[slot * len + j] in a fixed order: wave w
+// of a workgroup (64 outputs) adds slots w, w + 4, w + 8, ... in four
+// interleaved partial sums (a, b, c, d over slots w + 16k, w + 4 + 16k, ...),
+// then out = ((s0 + s1) + (s2 + s3)) over the waves.  The reductions above
+// leave their partials in slots, so a gradient or loss does not depend on the
+// order the workgroups ran in.
+__global__ __launch_bounds__(kT) void k_sum_slots(const float* __restrict__ part, int nslot, int64_t len,
+                                                  float* __restrict__ out) {
+    __shared__ float sh[4][64];
+    const int lx = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t j = (int64_t)blockIdx.x * 64 + lx;
+    const int64_t jc = j < len ? j : len - 1;
+    float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
+    int q = w;
+    for (; q + 12 < nslot; q += 16) {
+        a += part[(size_t)q * len + jc];
+        b += part[(size_t)(q + 4) * len + jc];
+        c += part[(size_t)(q + 8) * len + jc];
+        d += part[(size_t)(q + 12) * len + jc];
+    }
+    for (; q < nslot; q += 4) a += part[(size_t)q * len + jc];
+    sh[w][lx] = (a + b) + (c + d);
+    __syncthreads();
+    if (w == 0 && j < len) out[j] = (sh[0][lx] + sh[1][lx]) + (sh[2][lx] + sh[3][lx]);
+}
+static inline unsigned sum_blocks(int64_t len) { return (unsigned)((len + 63) / 64); }
+
+// The same for a few outputs over many slots (losses, head gradients): one
+// workgroup per output j, thread t adds slots t, t + kT, ..., and the
+// workgroup's sums are added by a fixed tree.
+__global__ __launch_bounds__(kT) void k_sum_slots_few(const float* __restrict__ part, int nslot, int len,
+                                                      float* __restrict__ out) {
+    __shared__ float sh[kT];
+    const int j = blockIdx.x;
     float t = 0.f;
-    for (int q = 0; q < nslot; ++q) t += part[(size_t)q * len + j];
-    out[j] = t;
+    for (int q = threadIdx.x; q < nslot; q += kT) t += part[(size_t)q * len + j];
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = kT / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = sh[0];
 }
 
-// count[0] = number of non-zero mask entries (count zeroed by the caller's kernel)
+// count[0] = number of non-zero mask entries (count zeroed by the caller's
+// kernel); n a multiple of 16: each thread counts 16 bytes, one integer
+// atomic per workgroup
+__device__ __forceinline__ int nonzero_bytes(uint32_t x) {
+    x |= x >> 4;
+    x |= x >> 2;
+    x |= x >> 1;
+    return __popc(x & 0x01010101u);
+}
 __global__ __launch_bounds__(kT) void k_mask_count(const uint8_t* __restrict__ mask, int n, int* __restrict__ count) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    int v = (p < n && mask[p]) ? 1 : 0;
+    __shared__ int sh[kT / 64];
+    const int p = (blockIdx.x * blockDim.x + threadIdx.x) * 16;
+    int v = 0;
+    if (p < n) {
+        const uint4 m = *(const uint4*)(mask + p);
+        v = nonzero_bytes(m.x) + nonzero_bytes(m.y) + nonzero_bytes(m.z) + nonzero_bytes(m.w);
+    }
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(count, v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        if (t) atomicAdd(count, t);
+    }
 }
 
 __global__ void k_zero_i32(int* p) { *p = 0; }
@@ -856,7 +908,7 @@ static int wgrad_any(hipStream_t s, int ct, const bf16* dZT, const bf16* XT, int
     default: set_error("pic_unet: unsupported channel count %d", ct * 16); return PIC_ERR_UNSUPPORTED;
     }
     const int64_t len = (int64_t)9 * cout * cin;
-    hipLaunchKernelGGL(k_sum_slots, dim3(nb(len)), dim3(kT), 0, s, part, nchunk, len, dW);
+    hipLaunchKernelGGL(k_sum_slots, dim3(sum_blocks(len)), dim3(kT), 0, s, part, nchunk, len, dW);
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
@@ -933,7 +985,7 @@ static int conv_bwd(pic_unet* u, int k) {
     PIC_TRY(wgrad_any(s, c.cin / 16, u->dzt, u->xt, c.cout, c.cin, H, W, u->wpart, u->g + c.woff));
     const int parts = bias_parts(H, W);
     hipLaunchKernelGGL(k_bias_grad, dim3(parts), dim3(kT), 0, s, u->dz, c.cout, H, W, u->bpart);
-    hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(kT), 0, s, u->bpart, parts, (int64_t)c.cout, u->g + c.boff);
+    hipLaunchKernelGGL(k_sum_slots, dim3(sum_blocks(c.cout)), dim3(kT), 0, s, u->bpart, parts, (int64_t)c.cout, u->g + c.boff);
     PIC_HIP(hipGetLastError());
     if (k > 0)
         PIC_TRY((conv_any<0, float>(s, c.cin / 16, u->dz, c.cout, 0, c.cout, u->wflip + c.woff, nullptr, a.g, a.C,
@@ -948,7 +1000,7 @@ static int unet_backward(pic_unet* u, int loss_kind = 0, float cxd = 0.f, float 
     const int N = u->N;
     const unsigned gN = nb((int64_t)N * N);
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, s, u->count);
-    hipLaunchKernelGGL(k_mask_count, dim3(gN), dim3(kT), 0, s, u->mask, N * N, u->count);
+    hipLaunchKernelGGL(k_mask_count, dim3(nb((int64_t)N * N / 16)), dim3(kT), 0, s, u->mask, N * N, u->count);
     if (loss_kind == 0) {
         hipLaunchKernelGGL(k_loss_mse, dim3(gN), dim3(kT), 0, s, u->out, u->tgt, u->mask, u->count, u->gout, u->bpart,
                            N * N);
@@ -957,11 +1009,11 @@ static int unet_backward(pic_unet* u, int loss_kind = 0, float cxd = 0.f, float 
                            N, u->tgt, u->bpart);
         hipLaunchKernelGGL(k_loss_res_grad, dim3(gN), dim3(kT), 0, s, u->tgt, u->mask, u->count, cxd, cyd, N, u->gout);
     }
-    hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(kT), 0, s, u->bpart, (int)gN, (int64_t)1, u->loss);
+    hipLaunchKernelGGL(k_sum_slots_few, dim3(1), dim3(kT), 0, s, u->bpart, (int)gN, 1, u->loss);
     hipLaunchKernelGGL(k_head_bwd, dim3(gN), dim3(kT), 0, s, u->gout, u->t[D0].p, u->w + u->head_w, u->t[D0].g,
                        u->bpart, N, N);
     // head_b follows head_w's 16 weights in the parameter vector, as in the slots
-    hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(kT), 0, s, u->bpart, (int)gN, (int64_t)17, u->g + u->head_w);
+    hipLaunchKernelGGL(k_sum_slots_few, dim3(17), dim3(kT), 0, s, u->bpart, (int)gN, 17, u->g + u->head_w);
     PIC_HIP(hipGetLastError());
     auto up_bwd = [&](int from, int fo, int to, int c) -> int {
         const Tensor& a = u->t[from];
