@@ -81,11 +81,13 @@ def unet_flops(n):
     return f + 2 * 16 * n * n
 
 
-def learned_vs_plain(w, parts, layout, train, evaluate, tol=1e-8):
-    """Poisson solve time at relative residual `tol` with the learned initial
-    guess against the plain previous-solution warm start, on two contexts
-    stepping the same particles: `train` steps of online training (the guess is
-    not offered yet), then `evaluate` steps with it offered."""
+def learned_vs_plain(w, parts, layout, train, total, tol=1e-8):
+    """The learned initial guess against the plain previous-solution warm start
+    at relative residual `tol`: two contexts step the same particles for
+    `total` steps, the learned one training online on the first `train`
+    solves (its guess is offered from then on); Poisson ms, PCG iterations,
+    starting residuals and the fallback rate are averaged over the solves
+    after training.  Both use the PCG alone (poisson_direct = 0)."""
     import numpy as np
     import pic_configs as cfg
 
@@ -94,7 +96,7 @@ def learned_vs_plain(w, parts, layout, train, evaluate, tol=1e-8):
     learn.learned_enable(train_steps=train, min_trained=train, lr=1e-3, seed=1)
     t0 = time.perf_counter()
     sp, sl = [], []
-    for it in range(train + 1 + evaluate):
+    for it in range(total):
         a = plain.step(it)
         b = learn.step(it)
         if it > train:
@@ -105,14 +107,13 @@ def learned_vs_plain(w, parts, layout, train, evaluate, tol=1e-8):
     fwd_ms, train_ms = learn.learned_benchmark(5)
     n = (min(w.NX, w.NY) // 128) * 128
     fl = unet_flops(n)
+    mean = lambda rows, k: float(np.mean([r[k] for r in rows]))
     out = {
-        "image": n, "tol": tol, "train_steps": train, "eval_steps": evaluate,
-        "poisson_ms_plain": float(np.mean([s["ms"] for s in sp])),
-        "poisson_ms_learned": float(np.mean([s["ms"] for s in sl])),
-        "iterations_plain": float(np.mean([s["iterations"] for s in sp])),
-        "iterations_learned": float(np.mean([s["iterations"] for s in sl])),
-        "initial_relres_plain": float(np.mean([s["initial_relres"] for s in sp])),
-        "initial_relres_learned": float(np.mean([s["initial_relres"] for s in sl])),
+        "grid": f"{w.NX}x{w.NY}", "particles_per_species": w.n_per_species, "image": n, "tol": tol,
+        "steps": total, "train_steps": train, "eval_steps": len(sl),
+        "poisson_ms_plain": mean(sp, "ms"), "poisson_ms_learned": mean(sl, "ms"),
+        "iterations_plain": mean(sp, "iterations"), "iterations_learned": mean(sl, "iterations"),
+        "initial_relres_plain": mean(sp, "initial_relres"), "initial_relres_learned": mean(sl, "initial_relres"),
         "guess_used": st["used"], "guess_tried": st["tried"],
         "fallback_rate": (1 - st["used"] / st["tried"]) if st["tried"] else None,
         "last_train_loss": st["loss"],
@@ -123,10 +124,76 @@ def learned_vs_plain(w, parts, layout, train, evaluate, tol=1e-8):
         "wall_s": elapsed,
     }
     out["speedup"] = out["poisson_ms_plain"] / out["poisson_ms_learned"] if out["poisson_ms_learned"] else None
+    # what a better guess can buy: the plain PCG contracts the residual by rho
+    # per iteration, so a guess that starts g times lower saves log g / log(1/rho)
+    # iterations; at most every iteration's time can go
+    ip, r0p, r0l = out["iterations_plain"], out["initial_relres_plain"], out["initial_relres_learned"]
+    if ip > 0 and r0p > 0 and r0l > 0:
+        rho = (tol / r0p) ** (1.0 / ip)
+        ms_it = out["poisson_ms_plain"] / ip
+        saved = np.log(r0p / r0l) / np.log(1.0 / rho)
+        out["pcg_contraction_per_iteration"] = rho
+        out["guess_gain"] = r0p / r0l
+        out["iterations_saved_expected"] = saved
+        out["speedup_bound_free_forward"] = ip / max(ip - saved, 1e-9)
+        out["guess_gain_for_1p5x_free_forward"] = float((1.0 / rho) ** (ip / 3.0))
+        out["ms_per_pcg_iteration"] = ms_it
     plain.close()
     learn.close()
-    log(f"learned vs plain @ {tol:g}: {out['poisson_ms_learned']:.3f} vs {out['poisson_ms_plain']:.3f} ms, "
-        f"used {st['used']}/{st['tried']}, U-Net forward {fwd_ms:.3f} ms")
+    log(f"learned vs plain @ {tol:g} on {w.NX}^2: {out['poisson_ms_learned']:.3f} vs {out['poisson_ms_plain']:.3f} ms, "
+        f"{out['iterations_learned']:.2f} vs {out['iterations_plain']:.2f} iterations, used {st['used']}/{st['tried']}, "
+        f"U-Net forward {fwd_ms:.3f} ms")
+    return out
+
+
+def precision_form(steps, warmup=3):
+    """The precision form at its configuration (electrode_1024, 2^24 + 2^24
+    particles, 2^-32 fixed-point deposit, FP32 V-cycles with weighted-Jacobi
+    2 + 2 sweeps inside FP64 iterative refinement to 1e-10): ms per step and
+    the Poisson statistics; then, on the last step's right-hand side, each
+    field's error against the library's FP64 direct solve of the same system
+    (checked against MKL PARDISO in the tests)."""
+    import numpy as np
+    import pic_configs as cfg
+    import pic_hip
+
+    w = cfg.WORKLOADS["electrode_1024"]
+    parts = [cfg.generate(w, s) for s in range(2)]
+    ctx = cfg.make_context(w, particles=parts, poisson_tol=1e-10, poisson_precision=1, mg_smoother=0, mg_pre=2,
+                           mg_post=2, charge_shift=32, poisson_direct=0)
+    for it in range(warmup):
+        ctx.step(it)
+    ctx.sync()
+    n0 = sum(ctx.count(s) for s in range(2))
+    st = []
+    t0 = time.perf_counter()
+    for it in range(warmup, warmup + steps):
+        st.append(ctx.step(it))
+    ctx.sync()
+    el = time.perf_counter() - t0
+    n1 = sum(ctx.count(s) for s in range(2))
+    rhs = ctx.grid(pic_hip.GRID_RHO)
+    fields = {k: ctx.grid(v) for k, v in (("phi", pic_hip.GRID_PHI), ("Ex", pic_hip.GRID_EX), ("Ey", pic_hip.GRID_EY))}
+    ref = pic_hip.Context(pic_hip.default_params(NX=w.NX, NY=w.NY, width=w.width, height=w.height, dt=w.dt,
+                                                 epsilon=cfg.EPS, layout=1, poisson_tol=1e-12))
+    ref.set_geometry(w.rects())
+    ref.set_grid(pic_hip.GRID_RHO, -rhs)     # overwriteRho of -rhs gives rhs again
+    ref.poisson_solve(0)
+    errs = {}
+    for k, v in (("phi", pic_hip.GRID_PHI), ("Ex", pic_hip.GRID_EX), ("Ey", pic_hip.GRID_EY)):
+        r = ref.grid(v)
+        errs[k] = float(np.abs(fields[k] - r).max() / np.abs(r).max())
+    ref.close()
+    ctx.close()
+    out = {"workload": w.name, "steps": steps, "ms_per_step": 1e3 * el / steps,
+           "particle_steps_per_s": steps * 0.5 * (n0 + n1) / el,
+           "poisson_ms": float(np.mean([s["ms"] for s in st])),
+           "refinement_steps": float(np.mean([s["iterations"] for s in st])),
+           "fp32_pcg_iterations": float(np.mean([s["inner_iterations"] for s in st])),
+           "final_relres_max": float(max(s["final_relres"] for s in st)),
+           "charge_shift": 32, "smoother": "weighted Jacobi 2+2",
+           "field_errors_vs_fp64_direct": errs}
+    log(f"precision form: {out['ms_per_step']:.3f} ms/step, Poisson {out['poisson_ms']:.3f} ms, field errors {errs}")
     return out
 
 
@@ -278,9 +345,14 @@ def main():
     ap.add_argument("--workload", default="electrode_1024")
     ap.add_argument("--layout", type=int, default=None, help="0 F64, 1 CELL32 (default: the workload's)")
     ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--learned-train", type=int, default=200,
+    ap.add_argument("--learned-train", type=int, default=2000,
                     help="online training steps of the learned-guess comparison (0 skips it)")
-    ap.add_argument("--learned-eval", type=int, default=50)
+    ap.add_argument("--learned-steps", type=int, default=5000,
+                    help="steps of the learned-guess comparison, training included")
+    ap.add_argument("--learned-grids", default="256,1024",
+                    help="grids (cells per side, electrode geometry) of the learned-guess comparison")
+    ap.add_argument("--precision-steps", type=int, default=20,
+                    help="steps of the precision-form leg (0 skips it)")
     ap.add_argument("--ensemble-runs", type=int, default=64,
                     help="runs of the batched-ensemble comparison (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
@@ -411,11 +483,25 @@ def main():
 
     learned = None
     if a.learned_train > 0:
+        learned = {}
+        base = cfg.WORKLOADS["electrode_1024"]
+        for g in (int(v) for v in a.learned_grids.split(",") if v):
+            try:
+                lw = base if g == base.NX else base.scaled(NX=g, n=base.n_per_species * g * g // (base.NX * base.NX),
+                                                           name=f"electrode_{g}")
+                lparts = parts if lw is w else [cfg.generate(lw, s) for s in range(2)]
+                learned[str(g)] = learned_vs_plain(lw, lparts, 1, a.learned_train, a.learned_steps)
+            except Exception as e:  # reported, the main measurement stands
+                log(f"learned-guess comparison on {g}^2 failed: {e}")
+                learned[str(g)] = {"error": str(e)}
+
+    precision = None
+    if a.precision_steps > 0:
         try:
-            learned = learned_vs_plain(w, parts, layout, a.learned_train, a.learned_eval)
+            precision = precision_form(a.precision_steps)
         except Exception as e:  # reported, the main measurement stands
-            log(f"learned-guess comparison failed: {e}")
-            learned = {"error": str(e)}
+            log(f"precision-form leg failed: {e}")
+            precision = {"error": str(e)}
 
     ensemble = None
     if a.ensemble_runs > 0:
@@ -459,6 +545,7 @@ def main():
         "particle_kernels": others,
         "with_injection": injection,
         "poisson_learned_vs_plain": learned,
+        "precision_form": precision,
         "ensemble": ensemble,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
