@@ -69,6 +69,25 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
 
 
+def unet_only(n, reps=5):
+    """Forward and forward + backward times of the network on an n x n image
+    with seeded inputs, one training step first (its Adam update included)."""
+    import numpy as np
+    import pic_hip
+
+    net = pic_hip.UNet(n, seed=1)
+    rng = np.random.default_rng(0)
+    x = rng.normal(0, 1, (n, n, 2)).astype(np.float32)
+    net.forward(x)
+    net.train_step(x, rng.normal(0, 0.1, (n, n)).astype(np.float32), np.ones((n, n), np.uint8), 1e-3)
+    fwd, full = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    rc = pic_hip.load().pic_unet_benchmark(net.h, reps, pic_hip._fp(fwd), pic_hip._fp(full))
+    net.close()
+    fl = unet_flops(n)
+    print(json.dumps({"unet_image": n, "rc": rc, "forward_ms": float(fwd[0]), "train_ms": float(full[0]),
+                      "forward_gflop": fl / 1e9, "forward_tflops": fl / (float(fwd[0]) * 1e-3) / 1e12}))
+
+
 def unet_flops(n):
     """Multiply-add FLOPs of one forward pass of the learned-guess network on an
     n x n image (padded input channels counted: that is the MFMA work done)."""
@@ -367,9 +386,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run three MIB-MiB device copies (counter calibration) and exit")
+    ap.add_argument("--unet-only", type=int, default=0, metavar="N",
+                    help="only time the learned-guess network on an N x N image (forward, forward + backward) "
+                         "and exit")
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
+        return
+    if a.unet_only:
+        unet_only(a.unet_only)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
