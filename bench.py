@@ -216,6 +216,71 @@ def precision_form(steps, warmup=3):
     return out
 
 
+# The reference's published times for its periodic 501 x 257 run, 10 steps
+# (Manya_Vansh_BMP.md:396-406: RTX 6000 Ada, PARDISO on the host), in ms/step
+PUBLISHED_MS = {"deposit": 54.4, "mover": 34.2, "poisson": 78.1, "total": 213.0}
+
+
+def published_workload(steps, warmup=3):
+    """The reference's own published run (periodic_501x257: 501 x 257 nodes,
+    periodic y, 9.6M electrons + 9.6M Xe+ ions) stepped on the GPU, once with
+    the direct solve (DST in x, Fourier basis in y, matrix-core GEMMs) and
+    once with the PCG alone: ms per step, particle-steps/s and the per-stage
+    device times, beside the reference's published per-step times for the
+    same workload on its own hardware."""
+    import numpy as np
+    import pic_configs as cfg
+
+    w = cfg.WORKLOADS["periodic_501x257"]
+    parts = [cfg.generate(w, s) for s in range(2)]
+    out = {"workload": w.name, "cells": f"{w.NX}x{w.NY}", "nodes": f"{w.NX + 1}x{w.NY + 1}", "periodic_y": True,
+           "particles_per_species": w.n_per_species, "steps": steps,
+           "published_ms_per_step": PUBLISHED_MS,
+           "published_hardware": "NVIDIA RTX 6000 Ada (deposit, mover) + host MKL PARDISO (Poisson)"}
+    for solver, direct in (("direct", 1), ("pcg", 0)):
+        ctx = cfg.make_context(w, particles=parts, poisson_tol=1e-10, poisson_direct=direct)
+        it = 0
+        for _ in range(warmup):
+            ctx.step(it)
+            it += 1
+        ctx.sync()
+        n0 = sum(ctx.count(s) for s in range(2))
+        st = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st.append(ctx.step(it))
+            it += 1
+        ctx.sync()
+        el = time.perf_counter() - t0
+        n1 = sum(ctx.count(s) for s in range(2))
+        nb = min(10, steps)
+        ctx.set_timing(1)
+        ctx.stage_times()
+        for _ in range(nb):
+            ctx.step(it)
+            it += 1
+        stage = {k: v[0] / nb for k, v in ctx.stage_times().items()}
+        ctx.set_timing(0)
+        ctx.close()
+        ms = 1e3 * el / steps
+        r = {"ms_per_step": ms, "particle_steps_per_s": steps * 0.5 * (n0 + n1) / el,
+             "stages_ms_per_step": {k: round(v, 4) for k, v in stage.items()},
+             "poisson_iterations_mean": float(np.mean([x["iterations"] for x in st])),
+             "poisson_final_relres_max": float(max(x["final_relres"] for x in st))}
+        if solver == "direct":
+            dep = stage["deposit"] + stage["finalize"]
+            mover = stage["push"] + stage["bin"] + stage["scatter"]
+            r["published_over_measured"] = {
+                "deposit": PUBLISHED_MS["deposit"] / dep if dep > 0 else None,
+                "mover": PUBLISHED_MS["mover"] / mover if mover > 0 else None,
+                "poisson": PUBLISHED_MS["poisson"] / stage["poisson"] if stage["poisson"] > 0 else None,
+                "total": PUBLISHED_MS["total"] / ms}
+        out[solver] = r
+        log(f"published workload, {solver} solve: {ms:.3f} ms/step, Poisson {stage['poisson']:.3f} ms "
+            f"({r['poisson_iterations_mean']:.1f} PCG iterations)")
+    return out
+
+
 def ensemble_form(nruns, steps, seed=4):
     """Aggregate particle-steps/s of an uneven ensemble stepped as one batch
     (persistent kernels, one HIP graph per step, autotuned) against the same
@@ -375,6 +440,8 @@ def main():
     ap.add_argument("--ensemble-runs", type=int, default=64,
                     help="runs of the batched-ensemble comparison (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--published-steps", type=int, default=20,
+                    help="steps of the reference's published periodic 501 x 257 run (0: skip)")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="Poisson: FP64 PCG, or FP32 correction solves refined in FP64")
     ap.add_argument("--guess", choices=("prev", "extrap"), default="prev",
@@ -536,6 +603,14 @@ def main():
             log(f"ensemble comparison failed: {e}")
             ensemble = {"error": str(e)}
 
+    published = None
+    if a.published_steps > 0:
+        try:
+            published = published_workload(a.published_steps)
+        except Exception as e:  # reported, the main measurement stands
+            log(f"published-workload leg failed: {e}")
+            published = {"error": str(e)}
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -572,6 +647,7 @@ def main():
         "poisson_learned_vs_plain": learned,
         "precision_form": precision,
         "ensemble": ensemble,
+        "reference_published_workload": published,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
     }

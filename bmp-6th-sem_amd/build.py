@@ -64,7 +64,7 @@ def build(jobs: int = 8, verbose: bool = True) -> Path:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIBDIR / "libbmp_pic.so.tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp),
-           "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocblas", "-lpthread"]
+           "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

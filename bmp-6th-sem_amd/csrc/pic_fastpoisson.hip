@@ -12,7 +12,13 @@
 //     cy (2 - 2 cos(pi (k+1)/(n+1))) + cx (2 - 2 cos(pi (l+1)/(m+1)));
 //     the products with S run as FFTs (one LDS-resident complex FFT of length
 //     2(N+1) per pair of rows) when N + 1 is a power of two, else as FP64
-//     GEMMs (MFMA, rocBLAS);
+//     GEMMs on the matrix cores (k_dgemm);
+//   * periodic y (the reference's generateCoeffsPeriodic, :82-180): the
+//     system's rows are grid rows 0 .. gy-2 with the y neighbours wrapped, so
+//     the y factor is the periodic second difference, diagonal in the real
+//     Fourier basis Y (cos / sin columns, eigenvalues 2 - 2 cos(2 pi p / N));
+//     T^-1 f = Y ((Y^T F S_m) ./ Lambda) S_m by GEMMs, every x eigenvalue is
+//     positive so Lambda has no zero;
 //   * a capacitance correction for the internal Dirichlet nodes: with Q the
 //     internal Dirichlet nodes next to an unknown (P of them), sources c on Q
 //     are chosen so that u = T^-1 (f + B c) equals V on Q:
@@ -21,13 +27,11 @@
 //     the internal Dirichlet nodes, so u solves the reference's system there.
 //
 // Per solve: four transform passes (FFT passes, or GEMMs of n x n x m /
-// m x m x n), one n x nq x m GEMM for the correction (nq: the rows holding
-// Q), a few small kernels.  The PCG of
+// m x m x n), the correction (a few line vectors folded into the third FFT
+// pass, or an n x nq x m product), a few small kernels.  The PCG of
 // pic_poisson.hip then starts from this solution and only checks (or, if
 // rounding left the residual above the tolerance, corrects) it.
 #include "pic_internal.h"
-
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <array>
@@ -46,8 +50,8 @@ struct FdRun;
 
 struct FastPoisson {
     int m = 0, n = 0, P = 0, nr = 0, nc = 0;
-    rocblas_handle blas = nullptr;
-    double *Sm = nullptr, *Sn = nullptr;     // m x m, n x n
+    int per = 0;                             // periodic y: Sn is the real Fourier basis, rows 0 .. n-1 of the grid
+    double *Sm = nullptr, *Sn = nullptr;     // m x m, n x n; row = node, column = mode
     double* Dinv = nullptr;                  // n x m
     double *F = nullptr, *W = nullptr, *U = nullptr, *G = nullptr;   // n x m work
     double* SnR = nullptr;                   // nr x n: rows of S_n at the covering rows
@@ -99,37 +103,32 @@ void fast_free(pic_ctx* c) {
                     f->lptr, f->lt, f->lidx, f->LV};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    if (f->blas) (void)rocblas_destroy_handle(f->blas);
     delete f->hself;
     delete f;
     c->fast = nullptr;
 }
 
 // ------------------------------------------------------------------ kernels
-// F (n x m, interior node (r+1, c+1)): rhs on unknowns plus the wall values
-// moved to the right-hand side; 0 on internal Dirichlet nodes.
-__global__ __launch_bounds__(kFdB) void k_fd_rhs(int gx, int gy, double cx, double cy, const double* __restrict__ rhs,
-                                                 const uint8_t* __restrict__ dir, const double* __restrict__ dval,
-                                                 double* __restrict__ F) {
-    const int m = gx - 2, n = gy - 2;
+// F (n x m, system row r at grid node (r + per ? 0 : 1, c + 1)): rhs on
+// unknowns plus the wall values moved to the right-hand side (the x walls, and
+// the y walls unless y is periodic); 0 on internal Dirichlet nodes.
+__global__ __launch_bounds__(kFdB) void k_fd_rhs(int gx, int n, int per, double cx, double cy,
+                                                 const double* __restrict__ rhs, const uint8_t* __restrict__ dir,
+                                                 const double* __restrict__ dval, double* __restrict__ F) {
+    const int m = gx - 2;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m * n) return;
     const int r = k / m, q = k - r * m;
-    const int node = (r + 1) * gx + q + 1;
+    const int node = (r + (per ? 0 : 1)) * gx + q + 1;
     double v = 0.0;
-    if (!dir[node]) {
+    if (dir[node] == 0) {
         v = rhs[node];
         if (q == 0) v += cx * dval[node - 1];
         if (q == m - 1) v += cx * dval[node + 1];
-        if (r == 0) v += cy * dval[node - gx];
-        if (r == n - 1) v += cy * dval[node + gx];
+        if (!per && r == 0) v += cy * dval[node - gx];
+        if (!per && r == n - 1) v += cy * dval[node + gx];
     }
     F[k] = v;
-}
-
-__global__ __launch_bounds__(kFdB) void k_fd_scale(int N, double* __restrict__ X, const double* __restrict__ Dinv) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < N) X[k] *= Dinv[k];
 }
 
 // X += G .* Dinv
@@ -137,6 +136,94 @@ __global__ __launch_bounds__(kFdB) void k_fd_addscaled(int N, double* __restrict
                                                        const double* __restrict__ Dinv) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < N) X[k] += G[k] * Dinv[k];
+}
+
+// ------------------------------------------------------------ FP64 GEMM
+// C (M x N, row-major, ldc) = op(A) op(B) on the matrix cores
+// (v_mfma_f64_16x16x4_f64), with op(A)[i][k] = TA ? A[k lda + i] : A[i lda + k]
+// and op(B)[k][j] = TB ? B[j ldb + k] : B[k ldb + j].  EP: 0 store, 1 store
+// times S[i ldc + j] (elementwise), 2 add to C.  A workgroup computes a 32 x 32
+// tile, one 16 x 16 quadrant per wave, over k chunks of 32 staged in LDS (the
+// next chunk is loaded into registers while the current one is multiplied);
+// every output is summed in the same order, so results are reproducible.
+// Lane l of an MFMA holds A[l % 16][l / 16], B[l / 16][l % 16] and receives
+// D[l / 16 + 4 e][l % 16], e < 4 (rows interleaved over the lane groups).
+typedef double d4v __attribute__((ext_vector_type(4)));
+static constexpr int kGT = 32, kGK = 32;
+
+template <bool TA, bool TB, int EP>
+__global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double* __restrict__ A, int lda,
+                                               const double* __restrict__ B, int ldb, double* __restrict__ C,
+                                               int ldc, const double* __restrict__ S) {
+    __shared__ double As[kGK][kGT + 1];     // As[k][i]
+    __shared__ double Bs[kGK][kGT + 1];     // Bs[k][j]
+    const int i0 = blockIdx.y * kGT, j0 = blockIdx.x * kGT;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wi = (w >> 1) * 16, wj = (w & 1) * 16;
+    double ra[4], rb[4];
+    // element e of thread t: (row, k) of the A chunk and (k, column) of the B
+    // chunk, the fast index along the stored rows so that loads coalesce
+    auto a_pos = [&](int e, int& ii, int& kk) {
+        const int idx = t + 256 * e;
+        if (TA) { kk = idx >> 5; ii = idx & 31; }
+        else { ii = idx >> 5; kk = idx & 31; }
+    };
+    auto b_pos = [&](int e, int& kk, int& jj) {
+        const int idx = t + 256 * e;
+        if (TB) { jj = idx >> 5; kk = idx & 31; }
+        else { kk = idx >> 5; jj = idx & 31; }
+    };
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int ii, kk, jj, kb;
+            a_pos(e, ii, kk);
+            const int gi = i0 + ii, gk = k0 + kk;
+            ra[e] = (gi < M && gk < K) ? (TA ? A[(size_t)gk * lda + gi] : A[(size_t)gi * lda + gk]) : 0.0;
+            b_pos(e, kb, jj);
+            const int gj = j0 + jj, gk2 = k0 + kb;
+            rb[e] = (gj < N && gk2 < K) ? (TB ? B[(size_t)gj * ldb + gk2] : B[(size_t)gk2 * ldb + gj]) : 0.0;
+        }
+    };
+    d4v acc = {0.0, 0.0, 0.0, 0.0};
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += kGK) {
+        __syncthreads();                    // the previous chunk is consumed
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int ii, kk, jj, kb;
+            a_pos(e, ii, kk);
+            As[kk][ii] = ra[e];
+            b_pos(e, kb, jj);
+            Bs[kb][jj] = rb[e];
+        }
+        __syncthreads();
+        if (k0 + kGK < K) load(k0 + kGK);
+#pragma unroll
+        for (int kk = 0; kk < kGK; kk += 4) {
+            const double a = As[kk + (lane >> 4)][wi + (lane & 15)];
+            const double b = Bs[kk + (lane >> 4)][wj + (lane & 15)];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+    }
+    const int j = j0 + wj + (lane & 15);
+    if (j >= N) return;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int i = i0 + wi + (lane >> 4) + 4 * e;
+        if (i >= M) continue;
+        const size_t o = (size_t)i * ldc + j;
+        if (EP == 0) C[o] = acc[e];
+        else if (EP == 1) C[o] = acc[e] * S[o];
+        else C[o] += acc[e];
+    }
+}
+
+template <bool TA, bool TB, int EP>
+static void dgemm(hipStream_t s, int M, int N, int K, const double* A, int lda, const double* B, int ldb, double* C,
+                  int ldc, const double* S = nullptr) {
+    hipLaunchKernelGGL((k_dgemm<TA, TB, EP>), dim3((unsigned)((N + kGT - 1) / kGT), (unsigned)((M + kGT - 1) / kGT)),
+                       dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, S);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -402,15 +489,17 @@ __global__ __launch_bounds__(kFdB) void k_fd_linevec(const FdRun one, const FdRu
     }
 }
 
-// full-grid solution: interior unknowns from U, every Dirichlet node its value
-__global__ __launch_bounds__(kFdB) void k_fd_out(int gx, int gy, const double* __restrict__ U,
+// full-grid solution: the system's unknowns (rows j0 .. j0 + n - 1) from U,
+// every other node its Dirichlet value (the periodic image row is copied from
+// row 0 afterwards, with the Dirichlet values)
+__global__ __launch_bounds__(kFdB) void k_fd_out(int gx, int gy, int j0, int n, const double* __restrict__ U,
                                                  const uint8_t* __restrict__ dir, const double* __restrict__ dval,
                                                  double* __restrict__ out) {
     const int node = blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= gx * gy) return;
     const int j = node / gx, i = node - j * gx;
     double v = dval[node];
-    if (!dir[node] && i > 0 && j > 0 && i < gx - 1 && j < gy - 1) v = U[(size_t)(j - 1) * (gx - 2) + (i - 1)];
+    if (dir[node] == 0 && i > 0 && i < gx - 1 && j >= j0 && j < j0 + n) v = U[(size_t)(j - j0) * (gx - 2) + (i - 1)];
     out[node] = v;
 }
 
@@ -738,23 +827,32 @@ static std::vector<double> dst_matrix(int N) {
     return S;
 }
 
-#define FD_BLAS(call)                                                                            \
-    do {                                                                                         \
-        rocblas_status s_ = (call);                                                              \
-        if (s_ != rocblas_status_success) {                                                      \
-            set_error("%s:%d rocBLAS %s: %s", __FILE__, __LINE__, #call, rocblas_status_to_string(s_)); \
-            return PIC_ERR_HIP;                                                                  \
-        }                                                                                        \
-    } while (0)
-
-// Column-major C (mr x nc) = A (mr x k) B (k x nc); row-major arrays are their
-// transposes, so row-major products are issued with the operands swapped.
-static int gemm(rocblas_handle h, int mr, int nc, int k, const double* A, int lda, const double* B, int ldb,
-                double beta, double* C, int ldc) {
-    const double one = 1.0;
-    FD_BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, mr, nc, k, &one, A, lda, B, ldb, &beta,
-                          C, ldc));
-    return PIC_OK;
+// Real orthonormal eigenbasis of the periodic second difference of period N
+// (row j = node, column k = mode): 1/sqrt(N); sqrt(2/N) cos(2 pi p j / N) and
+// sqrt(2/N) sin(2 pi p j / N) for 0 < p < N/2; (-1)^j / sqrt(N) for even N.
+// lam[k] = 2 - 2 cos(2 pi p / N) for column k's frequency p.
+static std::vector<double> fourier_matrix(int N, std::vector<double>& lam) {
+    std::vector<double> Y((size_t)N * N);
+    lam.assign(N, 0.0);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    const double a0 = 1.0 / std::sqrt((double)N), a = std::sqrt(2.0 / N);
+    for (int j = 0; j < N; ++j) Y[(size_t)j * N] = a0;
+    int k = 1;
+    for (int p = 1; 2 * p < N; ++p, k += 2) {
+        lam[k] = lam[k + 1] = (double)(2.0L - 2.0L * cosl(2.0L * pi * (long double)p / (long double)N));
+        for (int j = 0; j < N; ++j) {
+            // reduce p j modulo N so the argument stays below 2 pi
+            const long long t = ((long long)p * j) % N;
+            const long double ang = 2.0L * pi * (long double)t / (long double)N;
+            Y[(size_t)j * N + k] = a * (double)cosl(ang);
+            Y[(size_t)j * N + k + 1] = a * (double)sinl(ang);
+        }
+    }
+    if (N % 2 == 0) {
+        lam[N - 1] = 4.0;
+        for (int j = 0; j < N; ++j) Y[(size_t)j * N + N - 1] = (j % 2) ? -a0 : a0;
+    }
+    return Y;
 }
 
 // Explicit inverse of the SPD matrix C (P x P, row-major) by Cholesky.
@@ -881,16 +979,22 @@ static int enqueue_correction(FastPoisson* f, const FdRuns& r, hipStream_t s) {
 int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<double>& dval) {
     fast_free(c);
     const int gx = c->g.gx, gy = c->g.gy;
-    const int m = gx - 2, n = gy - 2;
-    if (!c->p.poisson_direct || c->p.periodic || m < 1 || n < 1 || m > kFdMaxN || n > kFdMaxN) return PIC_OK;
+    // Periodic y: the system's rows are grid rows 0 .. gy-2 (row gy-1 is the
+    // image of row 0), their y neighbours wrap, and only the x walls move to
+    // the right-hand side.  Otherwise the rows are 1 .. gy-2.
+    const int per = c->p.periodic ? 1 : 0;
+    const int m = gx - 2, n = per ? gy - 1 : gy - 2, j0 = per ? 0 : 1;
+    if (!c->p.poisson_direct || m < 1 || n < 1 + per || m > kFdMaxN || n > kFdMaxN) return PIC_OK;
     std::vector<int> qrow, qcol;
     std::vector<double> vq;
-    for (int r = 1; r < gy - 1; ++r)
+    for (int r = 0; r < n; ++r)
         for (int q = 1; q < gx - 1; ++q) {
-            const int node = r * gx + q;
-            if (!dir[node]) continue;
-            if (!dir[node - 1] || !dir[node + 1] || !dir[node - gx] || !dir[node + gx]) {
-                qrow.push_back(r - 1);
+            const int node = (r + j0) * gx + q;
+            if (dir[node] != 1) continue;
+            const int south = (r == 0 && per) ? (n - 1) * gx + q : node - gx;
+            const int north = (r == n - 1 && per) ? q : node + gx;
+            if (!dir[node - 1] || !dir[node + 1] || !dir[south] || !dir[north]) {
+                qrow.push_back(r);
                 qcol.push_back(q - 1);
                 vq.push_back(dval[node]);
             }
@@ -902,16 +1006,7 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
     f->m = m;
     f->n = n;
     f->P = P;
-    if (rocblas_create_handle(&f->blas) != rocblas_status_success) {
-        fast_free(c);
-        set_error("rocBLAS handle creation failed");
-        return PIC_ERR_HIP;
-    }
-    if (rocblas_set_stream(f->blas, c->stream) != rocblas_status_success) {
-        fast_free(c);
-        set_error("rocBLAS set stream failed");
-        return PIC_ERR_HIP;
-    }
+    f->per = per;
     // Cover Q greedily with rows and columns (the line holding most uncovered
     // nodes first): u0 on Q then needs one product per covering line, e.g. two
     // rows and two columns for a rectangular electrode.
@@ -944,11 +1039,17 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
     const int nr = f->nr, nc = f->nc;
 
     const std::vector<double> Sm = dst_matrix(m);
-    const std::vector<double> Sn = (n == m) ? Sm : dst_matrix(n);
+    std::vector<double> lamy(n);
+    std::vector<double> Sn;
+    if (per) Sn = fourier_matrix(n, lamy);
+    else {
+        Sn = (n == m) ? Sm : dst_matrix(n);
+        for (int k = 0; k < n; ++k) lamy[k] = 2.0 - 2.0 * std::cos(M_PI * (k + 1) / (n + 1));
+    }
     const double cx = 1.0 / (c->g.dx * c->g.dx), cy = 1.0 / (c->g.dy * c->g.dy);
     std::vector<double> Dinv((size_t)n * m);
     for (int k = 0; k < n; ++k) {
-        const double ly = cy * (2.0 - 2.0 * std::cos(M_PI * (k + 1) / (n + 1)));
+        const double ly = cy * lamy[k];
         for (int l = 0; l < m; ++l) {
             const double lx = cx * (2.0 - 2.0 * std::cos(M_PI * (l + 1) / (m + 1)));
             Dinv[(size_t)k * m + l] = 1.0 / (ly + lx);
@@ -1000,7 +1101,7 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
     };
     f->lm = fft_log(m);
     f->ln = fft_log(n);
-    f->fft = c->p.poisson_direct == 1 && f->lm > 0 && f->ln > 0;
+    f->fft = !per && c->p.poisson_direct == 1 && f->lm > 0 && f->ln > 0;
     std::vector<double2> twm, twn;
     if (f->fft) {
         auto table = [](int L) {
@@ -1086,11 +1187,10 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
             const size_t tot = (size_t)P * kc * m;
             hipLaunchKernelGGL(k_fd_amat, dim3((unsigned)((tot + kFdB - 1) / kFdB)), dim3(kFdB), 0, c->stream, P, n, m,
                                k0, kc, f->Sn, f->Sm, f->Dinv, qrow_d, f->qcol, A);
-            // column-major view: A^T is (kc m) x P with ld kc m; C = (A^T)^T A^T
-            const double one = 1.0, beta = k0 == 0 ? 0.0 : 1.0;
-            if (rocblas_dgemm(f->blas, rocblas_operation_transpose, rocblas_operation_none, P, P, kc * m, &one, A,
-                              kc * m, A, kc * m, &beta, Cd, P) != rocblas_status_success)
-                rc = PIC_ERR_HIP;
+            // C (+)= A A^T over this chunk of the spectral index
+            if (k0 == 0) dgemm<false, true, 0>(c->stream, P, P, kc * m, A, kc * m, A, kc * m, Cd, P);
+            else dgemm<false, true, 2>(c->stream, P, P, kc * m, A, kc * m, A, kc * m, Cd, P);
+            if (hipGetLastError() != hipSuccess) rc = PIC_ERR_HIP;
         }
         std::vector<double> C((size_t)P * P);
         if (rc == PIC_OK && hipMemcpyAsync(C.data(), Cd, (size_t)P * P * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
@@ -1176,23 +1276,18 @@ int fast_solve(pic_ctx* c, double* out) {
         dst_pass<1>(f->lm, one, n, s);
         dst_pass<2>(f->ln, one, m, s);
     } else {
-        hipLaunchKernelGGL(k_fd_rhs, dim3(nb), dim3(kFdB), 0, s, gx, gy, cx, cy, c->rho, c->dir, c->dval, f->F);
-        // row-major X (n x m) is the column-major m x n matrix X^T:
-        // W^T = S_m F^T, then Fhat^T = W^T S_n
-        PIC_TRY(gemm(f->blas, m, n, m, f->Sm, m, f->F, m, 0.0, f->W, m));
-        PIC_TRY(gemm(f->blas, m, n, n, f->W, m, f->Sn, n, 0.0, f->U, m));
-        hipLaunchKernelGGL(k_fd_scale, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->Dinv);
+        hipLaunchKernelGGL(k_fd_rhs, dim3(nb), dim3(kFdB), 0, s, gx, n, f->per, cx, cy, c->rho, c->dir, c->dval,
+                           f->F);
+        // W = F S_m, Uhat = (Y^T W) ./ Lambda (Y = S_n, or the Fourier basis)
+        dgemm<false, false, 0>(s, n, m, m, f->F, m, f->Sm, m, f->W, m);
+        dgemm<true, false, 1>(s, n, m, n, f->Sn, n, f->W, m, f->U, m, f->Dinv);
     }
     if (P > 0) {
         // u0 on Q through the covering lines: V1 = S_n[rows] Uhat (nr x m),
-        // V1^T = Uhat^T S_n[rows]^T; V2 = Uhat S_m[cols]^T (n x nc, column-major)
+        // V2 = S_m[cols] Uhat^T (nc x n)
         // (a few lines: one pass over Uhat; many lines: GEMMs)
-        if (nr > kLines) PIC_TRY(gemm(f->blas, m, nr, n, f->U, m, f->SnR, n, 0.0, f->V1, m));
-        if (nc > kLines) {
-            const double one = 1.0, zero = 0.0;
-            FD_BLAS(rocblas_dgemm(f->blas, rocblas_operation_transpose, rocblas_operation_none, n, nc, m, &one, f->U,
-                                  m, f->SmC, m, &zero, f->V2, n));
-        }
+        if (nr > kLines) dgemm<false, false, 0>(s, nr, m, n, f->SnR, n, f->U, m, f->V1, m);
+        if (nc > kLines) dgemm<false, true, 0>(s, nc, n, m, f->SmC, m, f->U, m, f->V2, n);
         PIC_TRY(enqueue_correction<false>(f, one, s));
         if (!f->fft) hipLaunchKernelGGL(k_fd_addscaled, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->G, f->Dinv);
     }
@@ -1203,11 +1298,11 @@ int fast_solve(pic_ctx* c, double* out) {
         PIC_HIP(hipGetLastError());
         return PIC_OK;
     }
-    // U = S_n Uhat S_m -> W^T = S_m Uhat^T, U^T = W^T S_n
-    PIC_TRY(gemm(f->blas, m, n, m, f->Sm, m, f->U, m, 0.0, f->W, m));
-    PIC_TRY(gemm(f->blas, m, n, n, f->W, m, f->Sn, n, 0.0, f->G, m));
-    hipLaunchKernelGGL(k_fd_out, dim3((unsigned)((gx * gy + kFdB - 1) / kFdB)), dim3(kFdB), 0, s, gx, gy, f->G,
-                       c->dir, c->dval, out);
+    // U = Y Uhat S_m
+    dgemm<false, false, 0>(s, n, m, n, f->Sn, n, f->U, m, f->W, m);
+    dgemm<false, false, 0>(s, n, m, m, f->W, m, f->Sm, m, f->G, m);
+    hipLaunchKernelGGL(k_fd_out, dim3((unsigned)((gx * gy + kFdB - 1) / kFdB)), dim3(kFdB), 0, s, gx, gy,
+                       f->per ? 0 : 1, n, f->G, c->dir, c->dval, out);
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
@@ -1286,3 +1381,44 @@ int fast_batch_enqueue(const FdBatch* b, hipStream_t s) {
 }
 
 } // namespace pic
+
+using namespace pic;
+
+extern "C" int pic_dgemm_check(int M, int N, int K, int ta, int tb, int ep, const double* a, const double* b,
+                               const double* sc, double* cmat) {
+    if (M < 1 || N < 1 || K < 1 || M > 4096 || N > 4096 || K > 65536 || ep < 0 || ep > 2 || !a || !b || !cmat ||
+        (ep == 1 && !sc)) {
+        set_error("pic_dgemm_check: invalid arguments (M %d, N %d, K %d, ep %d)", M, N, K, ep);
+        return PIC_ERR_ARG;
+    }
+    const size_t na = (size_t)M * K, nb_ = (size_t)K * N, nc = (size_t)M * N;
+    double *da = nullptr, *db = nullptr, *dc = nullptr, *ds = nullptr;
+    PIC_HIP(hipMalloc(&da, na * 8));
+    PIC_HIP(hipMalloc(&db, nb_ * 8));
+    PIC_HIP(hipMalloc(&dc, nc * 8));
+    PIC_HIP(hipMalloc(&ds, nc * 8));
+    PIC_HIP(hipMemcpy(da, a, na * 8, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(db, b, nb_ * 8, hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(dc, cmat, nc * 8, hipMemcpyHostToDevice));
+    if (sc) PIC_HIP(hipMemcpy(ds, sc, nc * 8, hipMemcpyHostToDevice));
+    const int lda = ta ? M : K, ldb = tb ? K : N;
+    hipStream_t s = nullptr;
+#define PIC_DG(TA, TB)                                                                   \
+    do {                                                                                 \
+        if (ep == 0) dgemm<TA, TB, 0>(s, M, N, K, da, lda, db, ldb, dc, N);              \
+        else if (ep == 1) dgemm<TA, TB, 1>(s, M, N, K, da, lda, db, ldb, dc, N, ds);     \
+        else dgemm<TA, TB, 2>(s, M, N, K, da, lda, db, ldb, dc, N);                      \
+    } while (0)
+    if (ta && tb) PIC_DG(true, true);
+    else if (ta) PIC_DG(true, false);
+    else if (tb) PIC_DG(false, true);
+    else PIC_DG(false, false);
+#undef PIC_DG
+    PIC_HIP(hipGetLastError());
+    PIC_HIP(hipMemcpy(cmat, dc, nc * 8, hipMemcpyDeviceToHost));
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dc);
+    (void)hipFree(ds);
+    return PIC_OK;
+}

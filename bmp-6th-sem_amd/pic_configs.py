@@ -16,6 +16,16 @@ Workloads:
                   seed 2, FP32 particle state (CELL32 storage), FP64 fields.
   electrode_1024  1024 x 1024 cells, centred 128 x 128-cell electrode at +100 V,
                   16M + 16M (2^24 each), B as above, seed 3, CELL32 storage.
+  periodic_501x257  the reference's published run (Manya_Vansh_BMP.md:155-167):
+                  500 x 256 cells (501 x 257 nodes) over 0.025 m x 0.0128 m
+                  (dx = dy = 5e-5 m), periodic y, dt = 5e-12 s, electrons and
+                  Xe+ ions (131.29 amu), Te = 10 eV, Ti = 0.5 eV, 9.6M of each,
+                  200 V on the left wall and 0 V on the right and bottom walls
+                  (a periodic geometry holds at most three rectangles,
+                  particles.cpp:651-654; the bottom row bounds the top through
+                  the wrap).  The published table gives no density or B: here
+                  n0 = 1e16 m^-3 (Debye length 4.7 cells, omega_pe dt = 0.028)
+                  and B = 0.  Seed 5, CELL32 storage.
 """
 from __future__ import annotations
 
@@ -54,18 +64,25 @@ class Workload:
     layout: int = 0                # 0 F64, 1 CELL32
     steps: int = 100
     extra: dict = field(default_factory=dict)
+    dx: float = DX                 # cell size (both directions)
+    dt_s: float | None = None      # time step (default: omega_pe dt = 0.1 at N0)
+    te_ev: float = TE_EV
+    ti_ev: float = TI_EV
+    ion_mass: float = M_ION
+    periodic: int = 0              # periodic y
+    walls: list | None = None      # boundary rectangles (default: all four walls at 0 V)
 
     @property
     def width(self):
-        return self.NX * DX
+        return self.NX * self.dx
 
     @property
     def height(self):
-        return self.NY * DX
+        return self.NY * self.dx
 
     @property
     def dt(self):
-        return plasma_dt()
+        return self.dt_s if self.dt_s is not None else plasma_dt()
 
     @property
     def bx(self):
@@ -76,7 +93,7 @@ class Workload:
         return self.b_total * math.cos(math.radians(self.b_tilt_deg))
 
     def rects(self):
-        r = [list(w) for w in WALLS]
+        r = [list(w) for w in (self.walls if self.walls is not None else WALLS)]
         if self.electrode:
             x1, y1, x2, y2 = self.electrode
             r.append([x1, y1, x2, y2, self.electrode_v])
@@ -90,7 +107,7 @@ class Workload:
 
     def species(self):
         """(charge, mass, temperature eV) per species, electrons first."""
-        return [(-E_CHARGE, M_E, TE_EV), (E_CHARGE, M_ION, TI_EV)]
+        return [(-E_CHARGE, M_E, self.te_ev), (E_CHARGE, self.ion_mass, self.ti_ev)]
 
     def scaled(self, NX=None, n=None, name=None):
         """Same physics and geometry fractions at another size."""
@@ -110,6 +127,9 @@ WORKLOADS = {
     "electrode_512": Workload("electrode_512", 512, 512, 1 << 22, 2, 0.05, 30.0, ELECTRODE, layout=1, steps=500),
     "electrode_1024": Workload("electrode_1024", 1024, 1024, 1 << 24, 3, 0.05, 30.0, ELECTRODE, layout=1,
                                steps=1000),
+    "periodic_501x257": Workload("periodic_501x257", 500, 256, 9_600_000, 5, 0.0, 0.0, None, layout=1, steps=10,
+                                 dx=5e-5, dt_s=5e-12, te_ev=10.0, ti_ev=0.5, ion_mass=131.29 * AMU, periodic=1,
+                                 walls=[[0, 0, 0, 1, 200.0], [1, 0, 1, 1, 0.0], [0, 0, 1, 0, 0.0]]),
 }
 
 
@@ -135,7 +155,7 @@ def generate(w: Workload, s: int):
     """Particles of species s of workload w from the library's generator."""
     import pic_hip
     q, m, t = w.species()[s]
-    return pic_hip.generate_particles(w.seed, s, w.n_per_species, w.width, w.height, DX, DX, abs(q / m), t,
+    return pic_hip.generate_particles(w.seed, s, w.n_per_species, w.width, w.height, w.dx, w.dx, abs(q / m), t,
                                       excl=w.electrode_nodes())
 
 
@@ -149,7 +169,7 @@ def make_context(w: Workload, layout: int | None = None, particles=None, capacit
     import pic_hip
     lay = w.layout if layout is None else layout
     p = pic_hip.default_params(NX=w.NX, NY=w.NY, width=w.width, height=w.height, dt=w.dt, epsilon=EPS,
-                               b0=w.bz, bx=w.bx, by=0.0, layout=lay, **params)
+                               b0=w.bz, bx=w.bx, by=0.0, layout=lay, periodic=w.periodic, **params)
     ctx = pic_hip.Context(p)
     ctx.set_geometry(w.rects())
     for s, (q, m, _) in enumerate(w.species()):

@@ -134,6 +134,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "pic_unet_forward": (i32, [vp, fp, fp]),
         "pic_unet_train_step": (i32, [vp, fp, fp, C.POINTER(C.c_uint8), C.c_float, fp]),
         "pic_unet_mfma_probe": (i32, [fp, fp, fp]),
+        "pic_dgemm_check": (i32, [i32, i32, i32, i32, i32, i32, dp, dp, dp, dp]),
         "pic_unet_benchmark": (i32, [vp, i32, fp, fp]),
         "pic_unet_train_step_residual": (i32, [vp, fp, dp, C.POINTER(C.c_uint8), C.c_float, C.c_float, C.c_float,
                                                fp]),
@@ -439,6 +440,24 @@ def mfma_probe(a, b):
     d = np.empty((16, 16), np.float32)
     _chk(lib.pic_unet_mfma_probe(_fp(a), _fp(b), _fp(d)))
     return d
+
+
+def dgemm_check(a, b, ta=False, tb=False, scale=None, c=None):
+    """op(a) op(b) on the direct solve's FP64 matrix-core GEMM kernel: op(a) =
+    a.T if ta, op(b) = b.T if tb; times `scale` elementwise, or added to `c`."""
+    lib = load()
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    M, K = (a.shape[1], a.shape[0]) if ta else a.shape
+    K2, N = (b.shape[1], b.shape[0]) if tb else b.shape
+    if K != K2:
+        raise ValueError(f"inner dimensions differ: {K} vs {K2}")
+    ep = 1 if scale is not None else (2 if c is not None else 0)
+    out = np.ascontiguousarray(c, np.float64).copy() if c is not None else np.zeros((M, N), np.float64)
+    sc = np.ascontiguousarray(scale, np.float64) if scale is not None else None
+    _chk(lib.pic_dgemm_check(M, N, K, int(ta), int(tb), ep, _dp(a), _dp(b), _dp(sc) if sc is not None else None,
+                             _dp(out)))
+    return out
 
 
 class UNet:

@@ -359,6 +359,13 @@ int     pic_unet_train_step_residual(pic_unet* u, const float* x, const double* 
 int     pic_unet_benchmark(pic_unet* u, int reps, float* fwd_ms, float* train_ms);
 /* 16x16x16 BF16 MFMA on row-major 16x16 inputs (layout check). */
 int     pic_unet_mfma_probe(const float* a, const float* b, float* d);
+/* The direct Poisson solve's FP64 matrix-core GEMM on host arrays (check of
+ * the kernel): C (M x N, row-major) = op(A) op(B) with op(A) = A (M x K) or,
+ * ta != 0, A^T of an array stored K x M; op(B) = B (K x N) or, tb != 0, B^T of
+ * an array stored N x K.  ep 0: C = product; 1: C = product .* S; 2: C +=
+ * product (C read first).  M, N <= 4096, K <= 65536. */
+int     pic_dgemm_check(int M, int N, int K, int ta, int tb, int ep, const double* a, const double* b,
+                        const double* s, double* c);
 
 /* Use a learned guess in pic_poisson_solve: train on the first `train_steps`
  * solves (Adam, learning rate lr), offer the prediction from solve

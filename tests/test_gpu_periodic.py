@@ -137,22 +137,25 @@ def test_periodic_steps_against_oracle_stages(hip_mod, oracle_mod):
     c.close()
 
 
+@pytest.mark.parametrize("solver", ["pcg", "direct"])
 @pytest.mark.parametrize("NX,NY,periodic", [(500, 256, 1), (500, 256, 0), (600, 600, 0), (97, 61, 0)])
-def test_pcg_on_grids_with_odd_cell_counts(hip_mod, oracle_mod, NX, NY, periodic):
+def test_grids_with_odd_cell_counts(hip_mod, oracle_mod, NX, NY, periodic, solver):
     """The multigrid hierarchy coarsens levels with an odd cell count by one
     overhanging Dirichlet coarse node, so the PCG alone solves grids whose
     sizes have few factors of two: the reference's published grid (501 x 257
     nodes, periodic y, Manya_Vansh_BMP.md:155-167) and non-periodic grids of
     500 x 256, 600 x 600 and 97 x 61 cells with an electrode.  Against the
     oracle (SuperLU on the periodic matrix; MKL PARDISO or the band Cholesky
-    on the Dirichlet one) to 1e-10 at tolerance 1e-12."""
+    on the Dirichlet one) to 1e-10 at tolerance 1e-12.  The same grids by the
+    direct solve, whose transforms are GEMMs here (no size is 2^k - 1): no PCG
+    iteration left to do."""
     hip, O = hip_mod, oracle_mod
     dx = 1e-4
     elec = [0.4375, 0.4375, 0.5625, 0.5625, 100.0]
     walls = [[0, 0, 0, 1, 200.0], [1, 0, 1, 1, 0.0]]
     geom = walls + [elec] if periodic else walls + [[0, 1, 1, 1, 0.0], [0, 0, 1, 0, 0.0], elec]
     p = hip.default_params(NX=NX, NY=NY, width=NX * dx, height=NY * dx, dt=1.77e-11, layout=0, periodic=periodic,
-                           poisson_tol=1e-12, poisson_direct=0)
+                           poisson_tol=1e-12, poisson_direct=1 if solver == "direct" else 0)
     c = hip.Context(p)
     c.set_geometry(geom)
     rng = np.random.default_rng(NX * NY)
@@ -173,7 +176,71 @@ def test_pcg_on_grids_with_odd_cell_counts(hip_mod, oracle_mod, NX, NY, periodic
             phi_ref = O.poisson_direct(c.gx, c.gy, c.dx, c.dy, mask, dval, rhs)
     phi = c.grid(hip.GRID_PHI)
     err = np.abs(phi - phi_ref).max() / np.abs(phi_ref).max()
-    print(f"{NX}x{NY} periodic={periodic}: {st['iterations']} PCG iterations, error {err:.2e}")
+    print(f"{NX}x{NY} periodic={periodic} {solver}: {st['iterations']} PCG iterations, error {err:.2e}")
     assert err < 1e-10
-    assert st["iterations"] < 40, st
+    if solver == "direct":
+        assert st["guess"] == 5 and st["iterations"] == 0, st
+    else:
+        assert st["iterations"] < 40, st
+    c.close()
+
+
+# the published voltages (200 V left, 0 V right and bottom) in the three
+# rectangles a periodic geometry may hold (particles.cpp:651-654)
+PUBLISHED_WALLS = [[0, 0, 0, 1, 200.0], [1, 0, 1, 1, 0.0], [0, 0, 1, 0, 0.0]]
+
+
+@pytest.mark.parametrize("case", ["per_64", "per_128x64_electrode", "published_500x256_electrode",
+                                  "published_500x256_bottom_wall", "electrode_at_the_seam"])
+def test_periodic_direct_solve(hip_mod, oracle_mod, case):
+    """The direct solve with periodic y (DST-I in x, the real Fourier basis in
+    y, capacitance correction for the internal Dirichlet nodes, transforms on
+    the FP64 matrix-core GEMM) against the oracle's SuperLU solve of the
+    reference's periodic matrix (generateCoeffsPeriodic,
+    poissonSolver.cpp:82-180), to 1e-10 relative, with the PCG left nothing
+    to do: the reference's published grid (501 x 257 nodes, periodic,
+    Manya_Vansh_BMP.md:155-167) with an electrode and with its published
+    boundary voltages (200 V left, 0 V right and bottom: row 0 held, so the
+    last system row meets it through the wrap), and an electrode on rows 0-1
+    whose neighbours lie across the periodic seam."""
+    hip, O = hip_mod, oracle_mod
+    dx = 1e-4
+    NX, NY = 64, 64
+    geom = PER_GEOM
+    if case == "per_128x64_electrode":
+        NX, NY = 128, 64
+        geom = PER_GEOM[:2] + [[0.2, 0.1, 0.3, 0.3, -40.0]]
+    elif case == "published_500x256_electrode":
+        NX, NY = 500, 256
+        geom = [[0, 0, 0, 1, 200.0], [1, 0, 1, 1, 0.0], [0.4375, 0.4375, 0.5625, 0.5625, 100.0]]
+    elif case == "published_500x256_bottom_wall":
+        NX, NY = 500, 256
+        geom = PUBLISHED_WALLS
+    elif case == "electrode_at_the_seam":
+        NX, NY = 96, 128
+        # rows 0 and 1: the row below it is the last system row (127)
+        geom = PER_GEOM[:2] + [[0.25, 0.0, 0.5, 1 / 128, 30.0]]
+    p = hip.default_params(NX=NX, NY=NY, width=NX * dx, height=NY * dx, dt=1.77e-11, layout=0, periodic=1,
+                           poisson_tol=1e-12, poisson_direct=1)
+    c = hip.Context(p)
+    c.set_geometry(geom)
+    rng = np.random.default_rng(NX + 3 * NY)
+    for it in range(2):
+        rho = rng.normal(0, 1e6, c.G)
+        c.set_grid(hip.GRID_RHO, rho)
+        st = c.poisson_solve(it)
+        rhs, phi_ref, _, _ = O.poisson_periodic(rho, c.gx, c.gy, c.dx, c.dy, geom)
+        phi = c.grid(hip.GRID_PHI)
+        err = np.abs(phi - phi_ref).max() / np.abs(phi_ref).max()
+        print(f"{case} solve {it}: error {err:.2e}, relres {st['final_relres']:.1e}")
+        assert err < 1e-10, (case, err, st)
+        assert st["guess"] == 5 and st["iterations"] == 0, st
+        assert st["final_relres"] <= 1e-12
+        # the image row copies row 0 (its corners belong to the x walls, whose
+        # potential may differ from a wall on row 0)
+        p2 = phi.reshape(c.gy, c.gx)
+        assert np.array_equal(p2[-1, 1:-1], p2[0, 1:-1])
+        assert np.array_equal(c.grid(hip.GRID_RHO), rhs)
+        ex, ey = O.efield_periodic(phi, c.gx, c.gy, c.dx, c.dy)
+        assert np.array_equal(c.grid(hip.GRID_EX), ex) and np.array_equal(c.grid(hip.GRID_EY), ey)
     c.close()
