@@ -174,7 +174,10 @@ __device__ __forceinline__ void load_quad<double>(const double* __restrict__ p, 
 
 // Deposit of one tile (kDepC cells of a key row); acc: 8 x kDepA LDS
 // accumulators, local cell l = cell - (k0 - 1); sst: kDepA + 1 starts.
-template <int L>
+// NQ: consecutive quads per thread (4 NQ particles).  Dense cells hold many
+// quads; with NQ > 1 fewer lanes flush into one cell's accumulators, so the
+// same-address LDS atomics of a wave serialise less.
+template <int L, int NQ = 1>
 __device__ __forceinline__ void deposit_tile_body(
     const ParticleBuf& b, const GridDev& g, const DepositArgs& a, int tiles_per_row, int tile,
     const int32_t* __restrict__ start, long long* __restrict__ cq, long long* __restrict__ ce,
@@ -201,30 +204,9 @@ __device__ __forceinline__ void deposit_tile_body(
         for (int j = tid; j <= cw + 1; j += kBlock) sst[j] = j == 0 ? beg : start[kl + j];
     __syncthreads();
     const int first = beg & ~(kVec - 1);
-    for (int base = first + kVec * tid; base < end; base += kVec * kBlock) {
-        Quad<L> q;
-        int sc = 0;
-        if (sst) {
-            // last local cell l with sst[l] <= the quad's first particle of the tile
-            const int p0 = max(base, beg);
-            int lo = 0, hi = cw;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (sst[mid] <= p0) lo = mid;
-                else hi = mid - 1;
-            }
-            sc = lo;
-        } else {
-            load_quad<int>(b.cell, base, q.cell);
-        }
-        load_quad<P>((const P*)b.pa, base, q.a);
-        load_quad<P>((const P*)b.pb, base, q.b);
-        if (a.energy) {
-            load_quad<V>((const V*)b.vx, base, q.vx);
-            load_quad<V>((const V*)b.vy, base, q.vy);
-            load_quad<V>((const V*)b.vz, base, q.vz);
-        }
-        int cur = -1;
+    constexpr int kSpan = kVec * NQ;
+    for (int base0 = first + kSpan * tid; base0 < end; base0 += kSpan * kBlock) {
+        int cur = -1, sc = 0;
         long long s0 = 0, s1 = 0, s2 = 0, s3 = 0, e0 = 0, e1 = 0, e2 = 0, e3 = 0;
         auto flush = [&]() {
             const int lc = cur - kl;
@@ -239,69 +221,96 @@ __device__ __forceinline__ void deposit_tile_body(
                 atomicAdd(&acc[7][lc], (unsigned long long)e3);
             }
         };
-#pragma unroll
-        for (int i = 0; i < kVec; ++i) {
-            const int pi = base + i;
-            if (pi < beg || pi >= end) continue;
-            int cl;
+        for (int qd = 0; qd < NQ; ++qd) {
+            const int base = base0 + kVec * qd;
+            if (base >= end) break;
+            Quad<L> q;
             if (sst) {
-                while (sst[sc + 1] <= pi) ++sc;
-                cl = kl + sc;
+                if (qd == 0) {
+                    // last local cell l with sst[l] <= the span's first particle of the tile
+                    const int p0 = max(base, beg);
+                    int lo = 0, hi = cw;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (sst[mid] <= p0) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    sc = lo;
+                }
             } else {
-                cl = q.cell[i];
+                load_quad<int>(b.cell, base, q.cell);
             }
-            if (cl != cur) {
-                if (cur >= 0) flush();
-                cur = cl;
-                s0 = s1 = s2 = s3 = e0 = e1 = e2 = e3 = 0;
-            }
-            double w0, w1, w2, w3;
-            if (L == PIC_LAYOUT_F64) {
-                int kx, ky;
-                split_key(cl, g, kx, ky);
-                double lx = (double)q.a[i] - kx * g.dx;
-                double ly = (double)q.b[i] - ky * g.dy;
-                w0 = (((g.dx - lx) * (g.dy - ly)) * a.inv_area) * a.scale_q;
-                w1 = ((lx * (g.dy - ly)) * a.inv_area) * a.scale_q;
-                w2 = (((g.dx - lx) * ly) * a.inv_area) * a.scale_q;
-                w3 = ((lx * ly) * a.inv_area) * a.scale_q;
-            } else {
-                const double fx = (double)q.a[i], fy = (double)q.b[i];
-                const double ux = (1.0 - fx) * a.half_x, vx_ = fx * a.half_x;
-                const double uy = (1.0 - fy) * a.half_y, vy_ = fy * a.half_y;
-                w0 = ux * uy;
-                w1 = vx_ * uy;
-                w2 = ux * vy_;
-                w3 = vx_ * vy_;
-            }
-            // the range tests of round_fixed are taken once per workgroup
-            // (charge) or once per particle (energy): |w| < 2 * 2^S
-            if (a.q_fast) {
-                s0 += round_small(w0);
-                s1 += round_small(w1);
-                s2 += round_small(w2);
-                s3 += round_small(w3);
-            } else {
-                s0 += round_fixed(w0);
-                s1 += round_fixed(w1);
-                s2 += round_fixed(w2);
-                s3 += round_fixed(w3);
-            }
+            load_quad<P>((const P*)b.pa, base, q.a);
+            load_quad<P>((const P*)b.pb, base, q.b);
             if (a.energy) {
-                const double pvx = (double)q.vx[i], pvy = (double)q.vy[i], pvz = (double)q.vz[i];
-                const double v2 = pvx * pvx + pvy * pvy + pvz * pvz;
-                const double en = 0.5 * v2 * a.denom;
-                const double es = en * a.scale_eq;    // exact: power of two
-                if (es < a.e_fast) {
-                    e0 += round_small(w0 * es);
-                    e1 += round_small(w1 * es);
-                    e2 += round_small(w2 * es);
-                    e3 += round_small(w3 * es);
+                load_quad<V>((const V*)b.vx, base, q.vx);
+                load_quad<V>((const V*)b.vy, base, q.vy);
+                load_quad<V>((const V*)b.vz, base, q.vz);
+            }
+#pragma unroll
+            for (int i = 0; i < kVec; ++i) {
+                const int pi = base + i;
+                if (pi < beg || pi >= end) continue;
+                int cl;
+                if (sst) {
+                    while (sst[sc + 1] <= pi) ++sc;
+                    cl = kl + sc;
                 } else {
-                    e0 += round_fixed(w0 * es);
-                    e1 += round_fixed(w1 * es);
-                    e2 += round_fixed(w2 * es);
-                    e3 += round_fixed(w3 * es);
+                    cl = q.cell[i];
+                }
+                if (cl != cur) {
+                    if (cur >= 0) flush();
+                    cur = cl;
+                    s0 = s1 = s2 = s3 = e0 = e1 = e2 = e3 = 0;
+                }
+                double w0, w1, w2, w3;
+                if (L == PIC_LAYOUT_F64) {
+                    int kx, ky;
+                    split_key(cl, g, kx, ky);
+                    double lx = (double)q.a[i] - kx * g.dx;
+                    double ly = (double)q.b[i] - ky * g.dy;
+                    w0 = (((g.dx - lx) * (g.dy - ly)) * a.inv_area) * a.scale_q;
+                    w1 = ((lx * (g.dy - ly)) * a.inv_area) * a.scale_q;
+                    w2 = (((g.dx - lx) * ly) * a.inv_area) * a.scale_q;
+                    w3 = ((lx * ly) * a.inv_area) * a.scale_q;
+                } else {
+                    const double fx = (double)q.a[i], fy = (double)q.b[i];
+                    const double ux = (1.0 - fx) * a.half_x, vx_ = fx * a.half_x;
+                    const double uy = (1.0 - fy) * a.half_y, vy_ = fy * a.half_y;
+                    w0 = ux * uy;
+                    w1 = vx_ * uy;
+                    w2 = ux * vy_;
+                    w3 = vx_ * vy_;
+                }
+                // the range tests of round_fixed are taken once per workgroup
+                // (charge) or once per particle (energy): |w| < 2 * 2^S
+                if (a.q_fast) {
+                    s0 += round_small(w0);
+                    s1 += round_small(w1);
+                    s2 += round_small(w2);
+                    s3 += round_small(w3);
+                } else {
+                    s0 += round_fixed(w0);
+                    s1 += round_fixed(w1);
+                    s2 += round_fixed(w2);
+                    s3 += round_fixed(w3);
+                }
+                if (a.energy) {
+                    const double pvx = (double)q.vx[i], pvy = (double)q.vy[i], pvz = (double)q.vz[i];
+                    const double v2 = pvx * pvx + pvy * pvy + pvz * pvz;
+                    const double en = 0.5 * v2 * a.denom;
+                    const double es = en * a.scale_eq;    // exact: power of two
+                    if (es < a.e_fast) {
+                        e0 += round_small(w0 * es);
+                        e1 += round_small(w1 * es);
+                        e2 += round_small(w2 * es);
+                        e3 += round_small(w3 * es);
+                    } else {
+                        e0 += round_fixed(w0 * es);
+                        e1 += round_fixed(w1 * es);
+                        e2 += round_fixed(w2 * es);
+                        e3 += round_fixed(w3 * es);
+                    }
                 }
             }
         }

@@ -396,33 +396,36 @@ struct ScatterSet {
 // start_old for the scatter, with the cell of every 256th particle in
 // blk_first.  kTileCodeCap: the mean tile holds 256 cells x 16 particles =
 // 4096, and the workgroup's LDS stays within a fifth of the CU's.
+// TC: tile width in cells (256 at the usual occupancy; narrower tiles keep
+// dense runs' tiles within the code table).
 static constexpr int kTileCodeCap = 4096;
-template <int L, int GENB>
+template <int L, int GENB, int TC>
 __global__ __launch_bounds__(kBlock) void k_push_tile(TileSet ts, GridDev g, int tiles_per_row, int ntiles,
                                                       const double4* __restrict__ fld, const int* __restrict__ guard)
 {
     // a push enqueued behind a Poisson check that has not passed does nothing
     if (guard && guard[0] != 1) return;
-    __shared__ int hist[kHR * (kTileC + 2)];
+    __shared__ int hist[kHR * (TC + 2)];
     __shared__ int codes[kTileCodeCap];
     __shared__ unsigned long long nabs[2];
     constexpr bool WF = L == PIC_LAYOUT_CELL32;
-    __shared__ double wf[WF ? 6 * (kTileC + 1) : 1];
+    __shared__ double wf[WF ? 6 * (TC + 1) : 1];
     const int t = xcd_tile(blockIdx.x, ts.ns * ntiles);
     const int sp = t % ts.ns, tile = t / ts.ns;
-    push_tile_body<L, GENB, kTileC, WF, WF, kTileCodeCap>(ts.b[sp], g, ts.d[sp], tiles_per_row, tile, fld,
+    push_tile_body<L, GENB, TC, WF, WF, kTileCodeCap>(ts.b[sp], g, ts.d[sp], tiles_per_row, tile, fld,
                                                           ts.start[sp], ts.counts[sp], ts.rank[sp], ts.absorbed[sp],
                                                           hist, codes, nabs, wf, ts.start_old[sp], ts.blk_first[sp]);
 }
 
-template <int L>
+template <int L, int NQ>
 __global__ __launch_bounds__(kBlock) void k_deposit_tile(DepositSet ds, GridDev g, int tiles_per_row, int ntiles)
 {
     __shared__ unsigned long long acc[8][kDepA];
     __shared__ int sst[kDepA + 1];
     const int t = xcd_tile(blockIdx.x, ds.ns * ntiles);
     const int sp = t % ds.ns, tile = t / ds.ns;
-    deposit_tile_body<L>(ds.b[sp], g, ds.a[sp], tiles_per_row, tile, ds.start[sp], ds.cq[sp], ds.ce[sp], acc, sst);
+    deposit_tile_body<L, NQ>(ds.b[sp], g, ds.a[sp], tiles_per_row, tile, ds.start[sp], ds.cq[sp], ds.ce[sp], acc,
+                             sst);
 }
 
 // Recompute the push and write each surviving particle to its sorted slot.
@@ -681,7 +684,36 @@ static int push_mode(const pic_ctx* c) {
     return genb | ((c->p.layout == PIC_LAYOUT_CELL32 && c->bz_uniform) ? 2 : 0);
 }
 
-static int tiles_per_row(const pic_ctx* c) { return (c->g.gx + kTileC - 1) / kTileC; }
+// Mean particles per cell of the densest species among ids[0 .. n-1].
+static double max_occupancy(const pic_ctx* c, const int* ids, int n) {
+    const double cells = (double)c->p.NX * (double)c->p.NY;
+    double m = 0.0;
+    for (int k = 0; k < n; ++k) m = std::max(m, (double)c->sp[ids[k]].n_upper / cells);
+    return m;
+}
+
+// Tile width of the push: params.tile_cells, else the widest of 256, 128, 64,
+// 32 cells whose mean tile holds at most kTileTarget particles, well inside
+// the code table (the table's particles find their cells in LDS, the rest
+// search the starts).  Measured on MI355X at 16 particles per cell (2 x 2^24
+// particles): push 257, 229, 219, 274 us for tiles of 256, 128, 64, 32 cells;
+// at 75 (2 x 9.6M): 273, 228, 136, 124, 136 us for 256 .. 16 cells.
+static constexpr int kTileTarget = 1024;
+static int push_tile_cells(const pic_ctx* c, const int* ids, int n) {
+    if (c->p.tile_cells) return c->p.tile_cells;
+    const double ppc = max_occupancy(c, ids, n);
+    int tc = kTileC;
+    while (tc > 32 && tc * ppc > kTileTarget) tc >>= 1;
+    return tc;
+}
+
+// Consecutive quads per deposit thread: params.deposit_group, else 1 up to
+// 32 particles per cell and 2 above (fewer lanes per cell's accumulators;
+// measured at 75 per cell: 127 us with 2, 170 with 1, 213 with 4).
+static int deposit_quads(const pic_ctx* c, const int* ids, int n) {
+    if (c->p.deposit_group) return c->p.deposit_group;
+    return max_occupancy(c, ids, n) > 32.0 ? 2 : 1;
+}
 
 static int finalize_deposit(pic_ctx* c, Species& sp) {
     const double dx = c->g.dx, dy = c->g.dy;
@@ -794,12 +826,20 @@ static int deposit_corners_set(pic_ctx* c, const int* ids, int n) {
         const int tpr = (c->g.gx + kDepC - 1) / kDepC;
         const int ntiles = tpr * c->g.gy;
         const dim3 grid((unsigned)(ds.ns * ntiles));
+        const int nq = deposit_quads(c, ids, n);
         hipEvent_t kt[2];
         PIC_TRY(kt_begin(c, kt, PIC_STAGE_DEPOSIT));
-        if (c->p.layout == PIC_LAYOUT_F64)
-            launch_timed(k_deposit_tile<PIC_LAYOUT_F64>, grid, dim3(kBlock), c->stream, kt, ds, c->g, tpr, ntiles);
-        else
-            launch_timed(k_deposit_tile<PIC_LAYOUT_CELL32>, grid, dim3(kBlock), c->stream, kt, ds, c->g, tpr, ntiles);
+#define PIC_DEP(LAY, NQ) launch_timed(k_deposit_tile<LAY, NQ>, grid, dim3(kBlock), c->stream, kt, ds, c->g, tpr, ntiles)
+        if (c->p.layout == PIC_LAYOUT_F64) {
+            if (nq == 4) PIC_DEP(PIC_LAYOUT_F64, 4);
+            else if (nq == 2) PIC_DEP(PIC_LAYOUT_F64, 2);
+            else PIC_DEP(PIC_LAYOUT_F64, 1);
+        } else {
+            if (nq == 4) PIC_DEP(PIC_LAYOUT_CELL32, 4);
+            else if (nq == 2) PIC_DEP(PIC_LAYOUT_CELL32, 2);
+            else PIC_DEP(PIC_LAYOUT_CELL32, 1);
+        }
+#undef PIC_DEP
         PIC_HIP(hipGetLastError());
         kt_end(c, PIC_STAGE_DEPOSIT, kt);
     }
@@ -842,10 +882,16 @@ int launch_push(pic_ctx* c, int s) {
 }
 
 template <int L, int GENB>
-static void tile_kernel(pic_ctx* c, const TileSet& ts, int tpr, int ntiles, const hipEvent_t kt[2],
-                        const int* guard) {
-    launch_timed(k_push_tile<L, GENB>, dim3((unsigned)(ts.ns * ntiles)), dim3(kBlock), c->stream, kt, ts, c->g, tpr,
-                 ntiles, (const double4*)c->fld, guard);
+static void tile_kernel(pic_ctx* c, const TileSet& ts, int tc, const hipEvent_t kt[2], const int* guard) {
+    const int tpr = (c->g.gx + tc - 1) / tc, ntiles = tpr * c->g.gy;
+    const dim3 grid((unsigned)(ts.ns * ntiles));
+    const double4* fld = (const double4*)c->fld;
+    switch (tc) {
+    case 32: launch_timed(k_push_tile<L, GENB, 32>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
+    case 64: launch_timed(k_push_tile<L, GENB, 64>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
+    case 128: launch_timed(k_push_tile<L, GENB, 128>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
+    default: launch_timed(k_push_tile<L, GENB, kTileC>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
+    }
 }
 
 template <int L, int GENB>
@@ -875,8 +921,7 @@ int move_push_set(pic_ctx* c, const int* ids, int n, const int* guard) {
         ts.start_old[j] = sp.start_old;
         ts.blk_first[j] = sp.blk_first;
     }
-    const int tpr = tiles_per_row(c);
-    const int ntiles = tpr * c->g.gy;
+    const int tc = push_tile_cells(c, ids, n);
     const bool f64 = c->p.layout == PIC_LAYOUT_F64;
     const int mode = push_mode(c);
     hipEvent_t ev;
@@ -885,12 +930,12 @@ int move_push_set(pic_ctx* c, const int* ids, int n, const int* guard) {
         hipEvent_t kt[2];
         PIC_TRY(kt_begin(c, kt, PIC_STAGE_PUSH));
         switch (f64 ? (mode & 1) : 4 + mode) {
-        case 0: tile_kernel<PIC_LAYOUT_F64, 0>(c, ts, tpr, ntiles, kt, guard); break;
-        case 1: tile_kernel<PIC_LAYOUT_F64, 1>(c, ts, tpr, ntiles, kt, guard); break;
-        case 4: tile_kernel<PIC_LAYOUT_CELL32, 0>(c, ts, tpr, ntiles, kt, guard); break;
-        case 5: tile_kernel<PIC_LAYOUT_CELL32, 1>(c, ts, tpr, ntiles, kt, guard); break;
-        case 6: tile_kernel<PIC_LAYOUT_CELL32, 2>(c, ts, tpr, ntiles, kt, guard); break;
-        default: tile_kernel<PIC_LAYOUT_CELL32, 3>(c, ts, tpr, ntiles, kt, guard); break;
+        case 0: tile_kernel<PIC_LAYOUT_F64, 0>(c, ts, tc, kt, guard); break;
+        case 1: tile_kernel<PIC_LAYOUT_F64, 1>(c, ts, tc, kt, guard); break;
+        case 4: tile_kernel<PIC_LAYOUT_CELL32, 0>(c, ts, tc, kt, guard); break;
+        case 5: tile_kernel<PIC_LAYOUT_CELL32, 1>(c, ts, tc, kt, guard); break;
+        case 6: tile_kernel<PIC_LAYOUT_CELL32, 2>(c, ts, tc, kt, guard); break;
+        default: tile_kernel<PIC_LAYOUT_CELL32, 3>(c, ts, tc, kt, guard); break;
         }
         PIC_HIP(hipGetLastError());
         kt_end(c, PIC_STAGE_PUSH, kt);

@@ -105,13 +105,19 @@ typedef struct pic_params {
     double poisson_inner_tol; /* relative residual of each FP32 correction solve (precision 1)  */
     int    mg_smoother;       /* 0: weighted Jacobi (mg_omega); 1: Chebyshev steps on the upper
                                  spectrum of D^-1 A (with mg_pre = mg_post = 2)                */
-    int    poisson_direct;    /* 1: solve exactly (FP64 sine transforms + capacitance correction
-                                 for internal Dirichlet nodes) and let the PCG only check the
-                                 result; used for non-periodic grids of at most 2047 interior
-                                 nodes per direction and 1024 internal Dirichlet nodes next to
+    int    poisson_direct;    /* 1: solve exactly (FP64 sine transforms in x and y, or a real
+                                 Fourier basis in a periodic y, + capacitance correction for
+                                 internal Dirichlet nodes) and let the PCG only check the
+                                 result; used for grids of at most 2047 interior nodes (or
+                                 period) per direction and 1024 internal Dirichlet nodes next to
                                  unknowns, with precision 0 and no learned guess.  The
-                                 transforms run as FFTs when NX and NY are powers of two, else
-                                 as GEMMs; 2: GEMMs always.  0: PCG alone                     */
+                                 transforms run as FFTs when NX and NY are powers of two and y
+                                 is not periodic, else as FP64 matrix-core GEMMs; 2: GEMMs
+                                 always.  0: PCG alone                                         */
+    int    tile_cells;        /* cells per tile of the push: 32, 64, 128 or 256; 0 = the widest
+                                 whose mean tile holds at most 1024 particles (at least 32)    */
+    int    deposit_group;     /* consecutive groups of 4 particles per deposit thread: 1, 2 or 4;
+                                 0 = 1 up to 32 particles per cell, 2 above                    */
 } pic_params;
 
 /* Per-solve statistics of pic_poisson_solve. */

@@ -97,16 +97,19 @@ def test_bin_large_grid(hip_mod, oracle_mod, NX, NY):
     c.close()
 
 
-@pytest.mark.parametrize("NX,n", [(64, 50000), (600, 600000)])
+@pytest.mark.parametrize("group", [0, 1, 2, 4])
+@pytest.mark.parametrize("NX,n", [(64, 50000), (600, 600000), (64, 300000)])
 @pytest.mark.parametrize("layout", [0, 1])
-def test_deposit_parity(hip_mod, oracle_mod, layout, NX, n):
+def test_deposit_parity(hip_mod, oracle_mod, layout, NX, n, group):
     """Tile deposit (one workgroup per 256 cells of a key row, each also
     summing the cell left of its first one) against the oracle's fixed-point
     restatement, bit for bit, and the FP64 reference deposit.  NX = 600: two
     full tiles and a partial one per row, so full-width tiles at c0 > 0 add a
-    live left cell taken from the previous tile's particles."""
+    live left cell taken from the previous tile's particles; 300000 particles
+    on 64 x 64 cells: 73 per cell.  Each thread sums 1, 2 or 4 consecutive
+    groups of four particles (0: the occupancy's choice)."""
     hip, O = hip_mod, oracle_mod
-    c = _ctx(hip, NX, layout)
+    c = _ctx(hip, NX, layout, deposit_group=group)
     qm = -E / ME
     s = c.add_species(-E, ME, n, 1e16, n)
     x, y, vx, vy, vz = _parts(hip, c, n, seed=5)
@@ -274,18 +277,19 @@ def test_push_parity(hip_mod, oracle_mod, layout, bx):
     c.close()
 
 
-@pytest.mark.parametrize("n", [2000, 300000])
-def test_cell32_consecutive_pushes_from_cell_starts(hip_mod, oracle_mod, n):
+@pytest.mark.parametrize("n,tile", [(2000, 0), (300000, 0), (300000, 64), (300000, 128), (300000, 256)])
+def test_cell32_consecutive_pushes_from_cell_starts(hip_mod, oracle_mod, n, tile):
     """Three re-binning CELL32 pushes in a row with no download between them:
     after the first, the storage's cell array is not written and the tile push
     and the scatter take every particle's cell from the cell starts (sparse
-    storage: the scatter's search over the starts in global memory; dense: its
-    LDS range, and tiles of ~4700 particles, past the tile push's LDS table).
-    The result equals three oracle pushes, as sets, and the cell array written
-    on download equals the oracle's keys."""
+    storage: the scatter's search over the starts in global memory; dense, 73
+    particles per cell: its LDS range, and push tiles of 32 cells by the
+    occupancy's choice, or of 64-256 cells, whose ~4700-19000 particles pass
+    the tile push's LDS table).  The result equals three oracle pushes, as
+    sets, and the cell array written on download equals the oracle's keys."""
     hip, O = hip_mod, oracle_mod
     NX = 64
-    c = _ctx(hip, NX, 1, bx=0.025)
+    c = _ctx(hip, NX, 1, bx=0.025, tile_cells=tile)
     rng = np.random.default_rng(21)
     ex = rng.normal(0, 2e3, c.G)
     ey = rng.normal(0, 2e3, c.G)
@@ -449,11 +453,14 @@ def test_direct_poisson_solve(hip_mod, oracle_mod, case, transforms):
 
 
 @pytest.mark.parametrize("layout", [0, 1])
-def test_move_fused_deposit_parity(hip_mod, oracle_mod, layout):
+@pytest.mark.parametrize("tile", [0, 32, 256])
+def test_move_fused_deposit_parity(hip_mod, oracle_mod, layout, tile):
     """The deposit accumulated inside the fused move (next step's charge)
-    equals the oracle's fixed-point deposit of the pushed particles."""
+    equals the oracle's fixed-point deposit of the pushed particles; push
+    tiles of 64 cells (the occupancy's choice at 15 particles per cell) and
+    of 32 and 256."""
     hip, O = hip_mod, oracle_mod
-    c = _ctx(hip, 64, layout, bx=0.025)
+    c = _ctx(hip, 64, layout, bx=0.025, tile_cells=tile)
     rng = np.random.default_rng(21)
     ex, ey = rng.normal(0, 3e3, (2, c.G))
     c.set_grid(hip.GRID_EX, ex)
