@@ -103,6 +103,7 @@ extern "C" void pic_params_default(pic_params* p) {
     p->poisson_inner_tol = 1e-4;
     p->tile_cells = 0;
     p->deposit_group = 0;
+    p->push_group = 0;
 }
 
 static void free_species(pic::Species& s) {
@@ -206,6 +207,10 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
     if (p->tile_cells != 0 && p->tile_cells != 32 && p->tile_cells != 64 && p->tile_cells != 128 &&
         p->tile_cells != 256) {
         set_error("pic_create: tile_cells must be 0 (automatic), 32, 64, 128 or 256 (got %d)", p->tile_cells);
+        return PIC_ERR_ARG;
+    }
+    if (p->push_group != 0 && p->push_group != 1 && p->push_group != 2 && p->push_group != 4) {
+        set_error("pic_create: push_group must be 0 (automatic), 1, 2 or 4 (got %d)", p->push_group);
         return PIC_ERR_ARG;
     }
     if (p->deposit_group != 0 && p->deposit_group != 1 && p->deposit_group != 2 && p->deposit_group != 4) {

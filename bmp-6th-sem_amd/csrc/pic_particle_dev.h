@@ -746,6 +746,138 @@ __device__ __forceinline__ void push_tile_body(
     }
 }
 
+// QN consecutive floats starting at a multiple of QN (QN-aligned: one load).
+template <int QN>
+__device__ __forceinline__ void load_run(const float* __restrict__ p, int i, float out[QN]) {
+    if constexpr (QN == 4) {
+        const float4 v = *(const float4*)(p + i);
+        out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
+    } else if constexpr (QN == 2) {
+        const float2 v = *(const float2*)(p + i);
+        out[0] = v.x; out[1] = v.y;
+    } else {
+        out[0] = p[i];
+    }
+}
+
+// The CELL32 tile push (cells from the starts, field window) with QN
+// consecutive particles per thread: the thread loads its particles with one
+// QN-wide load per field and pushes them in order; consecutive particles bound
+// for the same window bin form a run that takes its slots with one LDS atomic
+// (the slots of a run are consecutive).  Dense cells put many particles of
+// one cell in a wave, whose same-address atomics serialise; runs cut them by
+// up to QN.  Otherwise as push_tile_body<CELL32, GENB, TC, true, true, CAP>.
+template <int GENB, int TC, int QN, int CAP = kCodeCap>
+__device__ __forceinline__ void push_tile_body_runs(
+    const ParticleBuf& b, const GridDev& g, const SpeciesDev& s, int tiles_per_row, int tile,
+    const double4* __restrict__ fld, const int32_t* __restrict__ start, int32_t* __restrict__ counts,
+    int32_t* __restrict__ rank, unsigned long long* __restrict__ absorbed, int* hist, int* codes,
+    unsigned long long* nabs, double* wf, int32_t* __restrict__ start_old, int32_t* __restrict__ blk_first)
+{
+    constexpr int L = PIC_LAYOUT_CELL32;
+    constexpr int kHW = TC + 2;
+    const int tid = threadIdx.x;
+    const int r = tile / tiles_per_row;
+    const int c0 = (tile % tiles_per_row) * TC;
+    const int cw = min(TC, g.gx - c0);
+    const int k0 = r * g.gx + c0;
+    load_field_window<TC>(fld, g, k0, cw, wf);
+    const int beg = start[k0], end = start[k0 + cw];
+    for (int j = tid; j < cw; j += kBlock) {
+        const int e = min(start[k0 + j + 1] - beg, CAP);
+        for (int li = start[k0 + j] - beg; li < e; ++li) codes[li] = j;
+    }
+    for (int j = tid; j < kHR * kHW; j += kBlock) hist[j] = 0;
+    if (tid < 2) nabs[tid] = 0ull;
+    __syncthreads();
+    for (int j = tid; j <= cw; j += kBlock) start_old[k0 + j] = start[k0 + j];
+    auto put_code = [&](int i, int v) {
+        const int li = i - beg;
+        if (li < CAP) codes[li] = v;
+        else rank[i] = v;
+    };
+    const int first = beg & ~(QN - 1);
+    for (int q0 = first + QN * tid; q0 < end; q0 += QN * kBlock) {
+        float pa[QN], pb[QN], pvx[QN], pvy[QN], pvz[QN];
+        load_run<QN>((const float*)b.pa, q0, pa);
+        load_run<QN>((const float*)b.pb, q0, pb);
+        load_run<QN>((const float*)b.vx, q0, pvx);
+        load_run<QN>((const float*)b.vy, q0, pvy);
+        load_run<QN>((const float*)b.vz, q0, pvz);
+        int cells[QN];
+#pragma unroll
+        for (int e = 0; e < QN; ++e) {
+            const int i = q0 + e, li = i - beg;
+            cells[e] = (i >= beg && i < end) ? k0 + (li < CAP ? codes[li] : cell_of(start + k0, cw - 1, i)) : 0;
+        }
+        int rbin = -1, rn = 0, rfirst = 0;       // pending run: window bin, length, first element
+        auto flush = [&]() {
+            if (rn > 0) {
+                const int s0 = atomicAdd(&hist[rbin], rn);
+                for (int k = 0; k < rn; ++k) put_code(q0 + rfirst + k, (rbin << kRankBits) | (s0 + k));
+                rn = 0;
+            }
+        };
+#pragma unroll
+        for (int e = 0; e < QN; ++e) {
+            const int i = q0 + e;
+            if (i < beg || i >= end) continue;
+            PState<L> cur;
+            cur.cell = cells[e];
+            cur.a = pa[e];
+            cur.b = pb[e];
+            cur.vx = pvx[e];
+            cur.vy = pvy[e];
+            cur.vz = pvz[e];
+            if ((i & (kBlock - 1)) == 0) blk_first[i / kBlock] = cur.cell;
+            const Corners cn = gather_window<TC, (GENB & 2) == 0>(cur.cell, k0, c0, r, wf);
+            const Pushed<L> o = push_state<L, GENB>(cur, cn, g, s);
+            int bin = -1, code = -1;
+            if (o.key < 0) {
+                if (o.left) atomicAdd(&nabs[0], 1ull);
+                atomicAdd(&nabs[1], 1ull);
+            } else {
+                const int dr = o.ky - (r - 1), dc = o.kx - (c0 - 1);
+                if (dr >= 0 && dr < kHR && dc >= 0 && dc < kHW) bin = dr * kHW + dc;
+                else code = kFallback | atomicAdd(&counts[o.key], 1);
+            }
+            if (bin >= 0 && bin == rbin && rfirst + rn == e) {
+                ++rn;
+                continue;
+            }
+            flush();
+            if (bin >= 0) {
+                rbin = bin;
+                rn = 1;
+                rfirst = e;
+            } else {
+                put_code(i, code);
+            }
+        }
+        flush();
+    }
+    __syncthreads();
+    for (int j = tid; j < kHR * kHW; j += kBlock) {
+        int cnt = hist[j];
+        if (cnt) {
+            int k = (r - 1 + j / kHW) * g.gx + (c0 - 1 + j % kHW);
+            hist[j] = atomicAdd(&counts[k], cnt);
+        }
+    }
+    if (tid == 0) {
+        if (nabs[0]) atomicAdd(absorbed, nabs[0]);
+        if (nabs[1]) atomicAdd(absorbed + 1, nabs[1]);
+    }
+    __syncthreads();
+    for (int i = beg + tid; i < end; i += kBlock) {
+        const int li = i - beg;
+        const int v = li < CAP ? codes[li] : rank[i];
+        if (v == -1) continue;                      // absorbed: the scatter skips it
+        if (v & kFallback) rank[i] = v & ~kFallback;
+        else rank[i] = hist[v >> kRankBits] + (v & ((1 << kRankBits) - 1));
+    }
+}
+
 // host helpers (pic_particles.hip)
 DepositArgs deposit_args(const pic_ctx* c, const Species& sp);
 SpeciesDev species_dev(const pic_ctx* c, const Species& sp);

@@ -399,7 +399,7 @@ struct ScatterSet {
 // TC: tile width in cells (256 at the usual occupancy; narrower tiles keep
 // dense runs' tiles within the code table).
 static constexpr int kTileCodeCap = 4096;
-template <int L, int GENB, int TC>
+template <int L, int GENB, int TC, int QN = 1>
 __global__ __launch_bounds__(kBlock) void k_push_tile(TileSet ts, GridDev g, int tiles_per_row, int ntiles,
                                                       const double4* __restrict__ fld, const int* __restrict__ guard)
 {
@@ -412,6 +412,13 @@ __global__ __launch_bounds__(kBlock) void k_push_tile(TileSet ts, GridDev g, int
     __shared__ double wf[WF ? 6 * (TC + 1) : 1];
     const int t = xcd_tile(blockIdx.x, ts.ns * ntiles);
     const int sp = t % ts.ns, tile = t / ts.ns;
+    if constexpr (QN > 1) {
+        static_assert(L == PIC_LAYOUT_CELL32, "runs of particles per thread serve the CELL32 tile push");
+        push_tile_body_runs<GENB, TC, QN, kTileCodeCap>(ts.b[sp], g, ts.d[sp], tiles_per_row, tile, fld,
+                                                        ts.start[sp], ts.counts[sp], ts.rank[sp], ts.absorbed[sp],
+                                                        hist, codes, nabs, wf, ts.start_old[sp], ts.blk_first[sp]);
+        return;
+    }
     push_tile_body<L, GENB, TC, WF, WF, kTileCodeCap>(ts.b[sp], g, ts.d[sp], tiles_per_row, tile, fld,
                                                           ts.start[sp], ts.counts[sp], ts.rank[sp], ts.absorbed[sp],
                                                           hist, codes, nabs, wf, ts.start_old[sp], ts.blk_first[sp]);
@@ -881,17 +888,36 @@ int launch_push(pic_ctx* c, int s) {
     return PIC_OK;
 }
 
-template <int L, int GENB>
-static void tile_kernel(pic_ctx* c, const TileSet& ts, int tc, const hipEvent_t kt[2], const int* guard) {
+template <int L, int GENB, int QN>
+static void tile_kernel_q(pic_ctx* c, const TileSet& ts, int tc, const hipEvent_t kt[2], const int* guard) {
     const int tpr = (c->g.gx + tc - 1) / tc, ntiles = tpr * c->g.gy;
     const dim3 grid((unsigned)(ts.ns * ntiles));
     const double4* fld = (const double4*)c->fld;
     switch (tc) {
-    case 32: launch_timed(k_push_tile<L, GENB, 32>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
-    case 64: launch_timed(k_push_tile<L, GENB, 64>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
-    case 128: launch_timed(k_push_tile<L, GENB, 128>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
-    default: launch_timed(k_push_tile<L, GENB, kTileC>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
+    case 32: launch_timed(k_push_tile<L, GENB, 32, QN>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
+    case 64: launch_timed(k_push_tile<L, GENB, 64, QN>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
+    case 128: launch_timed(k_push_tile<L, GENB, 128, QN>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
+    default: launch_timed(k_push_tile<L, GENB, kTileC, QN>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
     }
+}
+
+// Consecutive particles per thread of the CELL32 tile push:
+// params.push_group, else 1 up to 32 particles per cell and 2 above (measured
+// at 75 per cell: push 113 us with 2, 122 with 1, 130 with 4; at 16: 218 with
+// 1, 238 with 2, 264 with 4 — one particle per thread keeps the next
+// particle's loads in flight behind the current push).
+static int push_run(const pic_ctx* c, double ppc) {
+    if (c->p.push_group) return c->p.push_group;
+    return ppc > 32.0 ? 2 : 1;
+}
+
+template <int L, int GENB>
+static void tile_kernel(pic_ctx* c, const TileSet& ts, int tc, int qn, const hipEvent_t kt[2], const int* guard) {
+    if constexpr (L == PIC_LAYOUT_CELL32) {
+        if (qn == 4) return tile_kernel_q<L, GENB, 4>(c, ts, tc, kt, guard);
+        if (qn == 2) return tile_kernel_q<L, GENB, 2>(c, ts, tc, kt, guard);
+    }
+    tile_kernel_q<L, GENB, 1>(c, ts, tc, kt, guard);
 }
 
 template <int L, int GENB>
@@ -922,6 +948,7 @@ int move_push_set(pic_ctx* c, const int* ids, int n, const int* guard) {
         ts.blk_first[j] = sp.blk_first;
     }
     const int tc = push_tile_cells(c, ids, n);
+    const int qn = push_run(c, max_occupancy(c, ids, n));
     const bool f64 = c->p.layout == PIC_LAYOUT_F64;
     const int mode = push_mode(c);
     hipEvent_t ev;
@@ -930,12 +957,12 @@ int move_push_set(pic_ctx* c, const int* ids, int n, const int* guard) {
         hipEvent_t kt[2];
         PIC_TRY(kt_begin(c, kt, PIC_STAGE_PUSH));
         switch (f64 ? (mode & 1) : 4 + mode) {
-        case 0: tile_kernel<PIC_LAYOUT_F64, 0>(c, ts, tc, kt, guard); break;
-        case 1: tile_kernel<PIC_LAYOUT_F64, 1>(c, ts, tc, kt, guard); break;
-        case 4: tile_kernel<PIC_LAYOUT_CELL32, 0>(c, ts, tc, kt, guard); break;
-        case 5: tile_kernel<PIC_LAYOUT_CELL32, 1>(c, ts, tc, kt, guard); break;
-        case 6: tile_kernel<PIC_LAYOUT_CELL32, 2>(c, ts, tc, kt, guard); break;
-        default: tile_kernel<PIC_LAYOUT_CELL32, 3>(c, ts, tc, kt, guard); break;
+        case 0: tile_kernel<PIC_LAYOUT_F64, 0>(c, ts, tc, qn, kt, guard); break;
+        case 1: tile_kernel<PIC_LAYOUT_F64, 1>(c, ts, tc, qn, kt, guard); break;
+        case 4: tile_kernel<PIC_LAYOUT_CELL32, 0>(c, ts, tc, qn, kt, guard); break;
+        case 5: tile_kernel<PIC_LAYOUT_CELL32, 1>(c, ts, tc, qn, kt, guard); break;
+        case 6: tile_kernel<PIC_LAYOUT_CELL32, 2>(c, ts, tc, qn, kt, guard); break;
+        default: tile_kernel<PIC_LAYOUT_CELL32, 3>(c, ts, tc, qn, kt, guard); break;
         }
         PIC_HIP(hipGetLastError());
         kt_end(c, PIC_STAGE_PUSH, kt);

@@ -52,7 +52,7 @@ class Params(C.Structure):
         ("poisson_guess", C.c_int),
         ("poisson_precision", C.c_int), ("poisson_inner_tol", C.c_double),
         ("mg_smoother", C.c_int), ("poisson_direct", C.c_int),
-        ("tile_cells", C.c_int), ("deposit_group", C.c_int),
+        ("tile_cells", C.c_int), ("deposit_group", C.c_int), ("push_group", C.c_int),
     ]
 
 

@@ -118,6 +118,8 @@ typedef struct pic_params {
                                  whose mean tile holds at most 1024 particles (at least 32)    */
     int    deposit_group;     /* consecutive groups of 4 particles per deposit thread: 1, 2 or 4;
                                  0 = 1 up to 32 particles per cell, 2 above                    */
+    int    push_group;        /* consecutive particles per thread of the CELL32 tile push: 1, 2 or 4;
+                                 0 = 1 up to 32 particles per cell, 2 above                    */
 } pic_params;
 
 /* Per-solve statistics of pic_poisson_solve. */

@@ -277,8 +277,10 @@ def test_push_parity(hip_mod, oracle_mod, layout, bx):
     c.close()
 
 
-@pytest.mark.parametrize("n,tile", [(2000, 0), (300000, 0), (300000, 64), (300000, 128), (300000, 256)])
-def test_cell32_consecutive_pushes_from_cell_starts(hip_mod, oracle_mod, n, tile):
+@pytest.mark.parametrize("n,tile,group", [(2000, 0, 0), (300000, 0, 0), (300000, 64, 0), (300000, 128, 0),
+                                          (300000, 256, 0), (2000, 0, 4), (300000, 0, 2), (300000, 0, 4),
+                                          (300000, 256, 4), (60000, 64, 4)])
+def test_cell32_consecutive_pushes_from_cell_starts(hip_mod, oracle_mod, n, tile, group):
     """Three re-binning CELL32 pushes in a row with no download between them:
     after the first, the storage's cell array is not written and the tile push
     and the scatter take every particle's cell from the cell starts (sparse
@@ -286,10 +288,12 @@ def test_cell32_consecutive_pushes_from_cell_starts(hip_mod, oracle_mod, n, tile
     particles per cell: its LDS range, and push tiles of 32 cells by the
     occupancy's choice, or of 64-256 cells, whose ~4700-19000 particles pass
     the tile push's LDS table).  The result equals three oracle pushes, as
-    sets, and the cell array written on download equals the oracle's keys."""
+    sets, and the cell array written on download equals the oracle's keys.
+    Each thread of the tile push takes 1, 2 or 4 consecutive particles
+    (group; 0: the default)."""
     hip, O = hip_mod, oracle_mod
     NX = 64
-    c = _ctx(hip, NX, 1, bx=0.025, tile_cells=tile)
+    c = _ctx(hip, NX, 1, bx=0.025, tile_cells=tile, push_group=group)
     rng = np.random.default_rng(21)
     ex = rng.normal(0, 2e3, c.G)
     ey = rng.normal(0, 2e3, c.G)
