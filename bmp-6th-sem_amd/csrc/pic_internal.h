@@ -206,6 +206,7 @@ struct pic_ctx {
     double* ey = nullptr;
     double* bz = nullptr;
     int bz_uniform = 1;        // every node of bz holds the same value, bz0
+    int sweep = 0;             // direction of the next particle pass (xcd_tile's rev); alternates
     double bz0 = 0.0;
     double4* fld = nullptr;    // packed (Ex, Ey, Bz, 0) per node, padded with zeros
     uint8_t* dir = nullptr;    // Dirichlet mask (fine grid)

@@ -21,15 +21,19 @@ static inline unsigned blocks_for(int64_t n, int per = kBlock) {
 
 // Workgroup -> tile order.  Workgroups are dealt round-robin to the 8 XCDs
 // (each with its own L2); giving each XCD a contiguous band of tiles keeps the
-// field rows and cell starts a tile touches in that XCD's L2.
+// field rows and cell starts a tile touches in that XCD's L2.  With rev the
+// XCD walks its band backwards: the particle passes alternate direction, so
+// each starts on the data the previous pass touched last, which the
+// memory-side cache still holds.
 static constexpr int kXcd = 8;
-__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
+__device__ __forceinline__ int xcd_tile(int b, int ntiles, int rev = 0) {
     const int per = (ntiles + kXcd - 1) / kXcd;
     const int x = b % kXcd, k = b / kXcd;
     const int full = ntiles - (per - 1) * kXcd;     // XCDs that get `per` tiles
     // XCD x owns tiles [first(x), first(x) + count(x)); workgroup b is its k-th
     const int first = x < full ? x * per : full * per + (x - full) * (per - 1);
-    return first + k;
+    const int count = x < full ? per : per - 1;
+    return first + (rev ? count - 1 - k : k);
 }
 
 // Round to nearest (ties to even) of 0 <= v < 2^51 by the 1.5 * 2^52

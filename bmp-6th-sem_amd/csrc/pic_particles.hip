@@ -401,7 +401,8 @@ struct ScatterSet {
 static constexpr int kTileCodeCap = 4096;
 template <int L, int GENB, int TC, int QN = 1>
 __global__ __launch_bounds__(kBlock) void k_push_tile(TileSet ts, GridDev g, int tiles_per_row, int ntiles,
-                                                      const double4* __restrict__ fld, const int* __restrict__ guard)
+                                                      const double4* __restrict__ fld, const int* __restrict__ guard,
+                                                      int rev)
 {
     // a push enqueued behind a Poisson check that has not passed does nothing
     if (guard && guard[0] != 1) return;
@@ -410,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_push_tile(TileSet ts, GridDev g, int
     __shared__ unsigned long long nabs[2];
     constexpr bool WF = L == PIC_LAYOUT_CELL32;
     __shared__ double wf[WF ? 6 * (TC + 1) : 1];
-    const int t = xcd_tile(blockIdx.x, ts.ns * ntiles);
+    const int t = xcd_tile(blockIdx.x, ts.ns * ntiles, rev);
     const int sp = t % ts.ns, tile = t / ts.ns;
     if constexpr (QN > 1) {
         static_assert(L == PIC_LAYOUT_CELL32, "runs of particles per thread serve the CELL32 tile push");
@@ -425,11 +426,12 @@ __global__ __launch_bounds__(kBlock) void k_push_tile(TileSet ts, GridDev g, int
 }
 
 template <int L, int NQ>
-__global__ __launch_bounds__(kBlock) void k_deposit_tile(DepositSet ds, GridDev g, int tiles_per_row, int ntiles)
+__global__ __launch_bounds__(kBlock) void k_deposit_tile(DepositSet ds, GridDev g, int tiles_per_row, int ntiles,
+                                                         int rev)
 {
     __shared__ unsigned long long acc[8][kDepA];
     __shared__ int sst[kDepA + 1];
-    const int t = xcd_tile(blockIdx.x, ds.ns * ntiles);
+    const int t = xcd_tile(blockIdx.x, ds.ns * ntiles, rev);
     const int sp = t % ds.ns, tile = t / ds.ns;
     deposit_tile_body<L, NQ>(ds.b[sp], g, ds.a[sp], tiles_per_row, tile, ds.start[sp], ds.cq[sp], ds.ce[sp], acc,
                              sst);
@@ -451,14 +453,15 @@ __global__ __launch_bounds__(kBlock) void k_deposit_tile(DepositSet ds, GridDev 
 static constexpr int kScatterWin = 130;
 
 template <int L, int GENB>
-__global__ __launch_bounds__(kBlock) void k_scatter_push(ScatterSet ss, GridDev g, const double4* __restrict__ fld)
+__global__ __launch_bounds__(kBlock) void k_scatter_push(ScatterSet ss, GridDev g, const double4* __restrict__ fld,
+                                                         int rev)
 {
     constexpr bool C32 = L == PIC_LAYOUT_CELL32;
     __shared__ double4 wf[C32 ? 2 * kScatterWin : 1];
     __shared__ int sst[C32 ? kScatterWin : 1];
     // consecutive particle blocks on one XCD: the partial cache lines two
     // neighbouring blocks write at their boundary meet in one L2
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int t = xcd_tile(blockIdx.x, gridDim.x, rev);
     const int sp = t < ss.nb0 ? 0 : 1;
     const int b = t - sp * ss.nb0;
     const ParticleBuf src = ss.src[sp];
@@ -836,7 +839,9 @@ static int deposit_corners_set(pic_ctx* c, const int* ids, int n) {
         const int nq = deposit_quads(c, ids, n);
         hipEvent_t kt[2];
         PIC_TRY(kt_begin(c, kt, PIC_STAGE_DEPOSIT));
-#define PIC_DEP(LAY, NQ) launch_timed(k_deposit_tile<LAY, NQ>, grid, dim3(kBlock), c->stream, kt, ds, c->g, tpr, ntiles)
+        const int rev = c->sweep;
+        c->sweep ^= 1;
+#define PIC_DEP(LAY, NQ) launch_timed(k_deposit_tile<LAY, NQ>, grid, dim3(kBlock), c->stream, kt, ds, c->g, tpr, ntiles, rev)
         if (c->p.layout == PIC_LAYOUT_F64) {
             if (nq == 4) PIC_DEP(PIC_LAYOUT_F64, 4);
             else if (nq == 2) PIC_DEP(PIC_LAYOUT_F64, 2);
@@ -893,11 +898,13 @@ static void tile_kernel_q(pic_ctx* c, const TileSet& ts, int tc, const hipEvent_
     const int tpr = (c->g.gx + tc - 1) / tc, ntiles = tpr * c->g.gy;
     const dim3 grid((unsigned)(ts.ns * ntiles));
     const double4* fld = (const double4*)c->fld;
+    const int rev = c->sweep;
+    c->sweep ^= 1;
     switch (tc) {
-    case 32: launch_timed(k_push_tile<L, GENB, 32, QN>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
-    case 64: launch_timed(k_push_tile<L, GENB, 64, QN>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
-    case 128: launch_timed(k_push_tile<L, GENB, 128, QN>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
-    default: launch_timed(k_push_tile<L, GENB, kTileC, QN>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard); break;
+    case 32: launch_timed(k_push_tile<L, GENB, 32, QN>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard, rev); break;
+    case 64: launch_timed(k_push_tile<L, GENB, 64, QN>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard, rev); break;
+    case 128: launch_timed(k_push_tile<L, GENB, 128, QN>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard, rev); break;
+    default: launch_timed(k_push_tile<L, GENB, kTileC, QN>, grid, dim3(kBlock), c->stream, kt, ts, c->g, tpr, ntiles, fld, guard, rev); break;
     }
 }
 
@@ -922,7 +929,10 @@ static void tile_kernel(pic_ctx* c, const TileSet& ts, int tc, int qn, const hip
 
 template <int L, int GENB>
 static void scatter_kernel(pic_ctx* c, const ScatterSet& ss, unsigned grid, const hipEvent_t kt[2]) {
-    launch_timed(k_scatter_push<L, GENB>, dim3(grid), dim3(kBlock), c->stream, kt, ss, c->g, (const double4*)c->fld);
+    const int rev = c->sweep;
+    c->sweep ^= 1;
+    launch_timed(k_scatter_push<L, GENB>, dim3(grid), dim3(kBlock), c->stream, kt, ss, c->g, (const double4*)c->fld,
+                 rev);
 }
 
 // Gather + Boris + walls + re-bin of the species in ids[0 .. n-1]: tile
