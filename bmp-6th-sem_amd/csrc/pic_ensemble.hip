@@ -695,6 +695,7 @@ void advance_host_state(pic_ensemble* e, int n) {
         }
         c->have_prev_phi = 1;
         c->guess_set = 0;
+        if (n > 0) c->meshes_stale = 0;     // the ensemble's node pass writes the meshes
     }
 }
 

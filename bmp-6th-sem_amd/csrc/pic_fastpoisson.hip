@@ -257,6 +257,7 @@ struct FdRun {
     const double* SnQ;
     const double2 *twm, *twn;
     int cmode;                               // correction: 0 none, 1 the spectrum G, 2 line form (LV)
+    int rl2;                                 // the row lines (1 .. kPassLines of them) come from the second FFT pass
     const int *lptr, *lt, *lidx;
     double* LV;
 };
@@ -266,9 +267,10 @@ struct FdRun {
 // One wave per q.
 template <bool B>
 __global__ __launch_bounds__(kFdB) void k_fd_u0q(const FdRun one, const FdRun* __restrict__ runs) {
+    __shared__ double part[4];
     const FdRun& f = B ? runs[blockIdx.z] : one;
-    const int q = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-    if (q >= f.P) return;
+    const int q = blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (q >= f.P) return;                    // uniform over the workgroup
     const int line = f.qri[q];
     const double *a, *b;
     int len;
@@ -282,9 +284,12 @@ __global__ __launch_bounds__(kFdB) void k_fd_u0q(const FdRun one, const FdRun* _
         len = f.n;
     }
     double s = 0.0;
-    for (int l = lane; l < len; l += 64) s += a[l] * b[l];
+#pragma unroll 4
+    for (int l = threadIdx.x; l < len; l += kFdB) s += a[l] * b[l];
     s = wave_sum(s);
-    if (lane == 0) f.dq[q] = f.VQ[q] - s;
+    if (lane == 0) part[wv] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) f.dq[q] = f.VQ[q] - ((part[0] + part[1]) + (part[2] + part[3]));
 }
 
 // Covering lines in one pass over Uhat (lr, lc <= kLines; either may be 0):
@@ -353,15 +358,56 @@ __global__ __launch_bounds__(kFdB) void k_fd_linesum(const FdRun one, const FdRu
 // The sources in row order: c_t = (C^-1 d)_q for q = rq_idx[t]; one wave per t
 template <bool B>
 __global__ __launch_bounds__(kFdB) void k_fd_sources(const FdRun one, const FdRun* __restrict__ runs) {
+    __shared__ double part[4];
     const FdRun& f = B ? runs[blockIdx.z] : one;
     const int P = f.P;
-    const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-    if (t >= P) return;
+    const int t = blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (t >= P) return;                      // uniform over the workgroup
     const int q = f.rq_idx[t];
     double s = 0.0;
-    for (int p = lane; p < P; p += 64) s += f.Cinv[(size_t)q * P + p] * f.dq[p];
+#pragma unroll 4
+    for (int p = threadIdx.x; p < P; p += kFdB) s += f.Cinv[(size_t)q * P + p] * f.dq[p];
     s = wave_sum(s);
-    if (lane == 0) f.cq[t] = s;
+    if (lane == 0) part[wv] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) f.cq[t] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
+// Column lines alone (lc <= kLines), one workgroup per row k of Uhat (its
+// four waves split the row; every load of a thread issues at once):
+//   V2[j][k] = sum_l Uhat[k][l] S_m[col j][l]
+template <bool B>
+__global__ __launch_bounds__(kFdB) void k_fd_collines(const FdRun one, const FdRun* __restrict__ runs) {
+    __shared__ double part[4][kLines];
+    const FdRun& f = B ? runs[blockIdx.z] : one;
+    const int k = blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nc = f.lc, m = f.m;
+    if (k >= f.n) return;                    // uniform over the workgroup
+    const double* __restrict__ u = f.U + (size_t)k * m;
+    constexpr int kIt = (kFdMaxN + kFdB - 1) / kFdB;
+    double x[kIt];
+#pragma unroll
+    for (int i = 0; i < kIt; ++i) {
+        const int l = threadIdx.x + kFdB * i;
+        x[i] = l < m ? u[l] : 0.0;
+    }
+#pragma unroll
+    for (int jj = 0; jj < kLines; ++jj) {
+        if (jj >= nc) break;
+        const double* __restrict__ sm = f.SmC + (size_t)jj * m;
+        double a = 0.0;
+#pragma unroll
+        for (int i = 0; i < kIt; ++i) {
+            const int l = threadIdx.x + kFdB * i;
+            if (l < m) a += x[i] * sm[l];
+        }
+        a = wave_sum(a);
+        if (lane == 0) part[wv][jj] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < nc)
+        f.V2[(size_t)threadIdx.x * f.n + k] =
+            (part[0][threadIdx.x] + part[1][threadIdx.x]) + (part[2][threadIdx.x] + part[3][threadIdx.x]);
 }
 
 // Z[a][l] = sum over the sources t of distinct row a of c_t S_m[col t][l].
@@ -599,7 +645,10 @@ struct DstStoreT {
     int R;
     struct Aux {};
     __device__ Aux fetch(int, int) const { return {}; }
-    __device__ void put(int t, int row, double v, Aux) const { Y[(size_t)t * R + row] = v; }
+    __device__ double put(int t, int row, double v, Aux) const {
+        Y[(size_t)t * R + row] = v;
+        return v;
+    }
 };
 
 // transposed and scaled by Dinv (the same layout as Y)
@@ -609,7 +658,11 @@ struct DstStoreTScaled {
     int R;
     using Aux = double;
     __device__ Aux fetch(int t, int row) const { return Dinv[(size_t)t * R + row]; }
-    __device__ void put(int t, int row, double v, Aux d) const { Y[(size_t)t * R + row] = v * d; }
+    __device__ double put(int t, int row, double v, Aux d) const {
+        const double y = v * d;
+        Y[(size_t)t * R + row] = y;
+        return y;
+    }
 };
 
 // the grid's interior node (t + 1, row + 1) (the caller imposes the
@@ -619,16 +672,29 @@ struct DstStoreGrid {
     int gx;
     struct Aux {};
     __device__ Aux fetch(int, int) const { return {}; }
-    __device__ void put(int t, int row, double v, Aux) const { out[(t + 1) * gx + row + 1] = v; }
+    __device__ double put(int t, int row, double v, Aux) const {
+        out[(t + 1) * gx + row + 1] = v;
+        return v;
+    }
+};
+
+// Row lines of the covering (at most kPassLines) taken by the second pass
+// while it stores the spectrum: V1[a][row] = sum_t S_n[row a][t] Uhat[t][row]
+// for the pass's two rows (columns of Uhat), block-reduced in a fixed order.
+static constexpr int kPassLines = 2;
+struct PassLines {
+    int lr;                                  // 0: none
+    const double* SnR;                       // lr x N
+    double* V1;                              // lr x R
 };
 
 // One pass: R rows of N = 2^(L-1) - 1 values in, N rows of R values out.
 // Workgroup w (after the XCD-aware remap: the workgroups of one XCD take
 // consecutive row pairs, so the transposed stores of one cache line meet in
 // one L2) transforms NB = max(1, 2048 / M) row pairs.
-template <int L, class Load, class Store>
+template <int L, class Load, class Store, bool LINES = false>
 __device__ __forceinline__ void dst_body(int R, int nwg, const Load& ld, const Store& st,
-                                         const double2* __restrict__ tw) {
+                                         const double2* __restrict__ tw, const PassLines pl = PassLines{}) {
     constexpr int M = 1 << L;
     constexpr int N = M / 2 - 1;
     constexpr int NB = M >= kDstPts ? 1 : kDstPts / M;
@@ -733,6 +799,7 @@ __device__ __forceinline__ void dst_body(int R, int nwg, const Load& ld, const S
     }
     // the storer's gathers first, then the stores
     typename Store::Aux xa[IT], xb[IT];
+    double sn[LINES ? kPassLines : 1][IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int e = min(tid + kDstT * it, NE - 1);
@@ -740,8 +807,13 @@ __device__ __forceinline__ void dst_body(int R, int nwg, const Load& ld, const S
         const int ra = r0 + 2 * b;
         xa[it] = st.fetch(t, min(ra, R - 1));
         xb[it] = st.fetch(t, min(ra + 1, R - 1));
+        if constexpr (LINES) {
+#pragma unroll
+            for (int a = 0; a < kPassLines; ++a) sn[a][it] = a < pl.lr ? pl.SnR[(size_t)a * N + t] : 0.0;
+        }
     }
     const double sc = 0.5 * sqrt(2.0 / (double)(N + 1));
+    double acc[LINES ? kPassLines : 1][2] = {};
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int e = tid + kDstT * it;
@@ -749,8 +821,36 @@ __device__ __forceinline__ void dst_body(int R, int nwg, const Load& ld, const S
             const int b = e / N, t = e - b * N;
             const int ra = r0 + 2 * b;
             const double2 v = z[b * M + t + 1];
-            if (ra < R) st.put(t, ra, -v.y * sc, xa[it]);
-            if (ra + 1 < R) st.put(t, ra + 1, v.x * sc, xb[it]);
+            double ya = 0.0, yb = 0.0;
+            if (ra < R) ya = st.put(t, ra, -v.y * sc, xa[it]);
+            if (ra + 1 < R) yb = st.put(t, ra + 1, v.x * sc, xb[it]);
+            if constexpr (LINES) {
+#pragma unroll
+                for (int a = 0; a < kPassLines; ++a) {
+                    acc[a][0] += sn[a][it] * ya;
+                    acc[a][1] += sn[a][it] * yb;
+                }
+            }
+        }
+    }
+    if constexpr (LINES) {
+        static_assert(NB == 1 && kDstT == 256, "the line sums assume one row pair per workgroup of four waves");
+        // the sums of the four waves, added in a fixed order (the FFT's LDS is free now)
+        __syncthreads();
+        double* red = (double*)lds;
+        const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+        for (int a = 0; a < kPassLines; ++a)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const double w = wave_sum(acc[a][c]);
+                if (lane == 0) red[(a * 2 + c) * 4 + wv] = w;
+            }
+        __syncthreads();
+        if (tid < 2 * pl.lr) {
+            const int a = tid >> 1, c = tid & 1;
+            const double* q = red + (a * 2 + c) * 4;
+            if (r0 + c < R) pl.V1[(size_t)a * R + r0 + c] = (q[0] + q[1]) + (q[2] + q[3]);
         }
     }
 }
@@ -763,8 +863,18 @@ __global__ __launch_bounds__(kDstT) void k_fd_dst(const FdRun one, const FdRun* 
     const FdRun& f = B ? runs[blockIdx.y] : one;
     if constexpr (KIND == 1)
         dst_body<L>(R, nwg, DstLoadRhs{f.gx, f.m, f.sgn, f.rho, f.dir, f.Fb}, DstStoreT{f.W, f.n}, f.twm);
-    else if constexpr (KIND == 2)
+    else if constexpr (KIND == 2) {
+        // the row lines ride along where a workgroup holds one row pair (L >= 11)
+        if constexpr (L >= 11) {
+            if (f.rl2) {
+                dst_body<L, DstLoadRows, DstStoreTScaled, true>(R, nwg, DstLoadRows{f.W, f.n},
+                                                                DstStoreTScaled{f.U, f.Dinv, f.m}, f.twn,
+                                                                PassLines{f.lr, f.SnR, f.V1});
+                return;
+            }
+        }
         dst_body<L>(R, nwg, DstLoadRows{f.W, f.n}, DstStoreTScaled{f.U, f.Dinv, f.m}, f.twn);
+    }
     else if constexpr (KIND == 3)
         dst_body<L>(R, nwg,
                     DstLoadCorrected{f.U, f.G, f.Dinv, f.m, f.cmode, f.n, f.lr, f.lc, f.SnR, f.SmC, f.LV},
@@ -933,6 +1043,7 @@ static FdRun run_of(const pic_ctx* c, double sgn) {
     r.twm = f->twm;
     r.twn = f->twn;
     r.cmode = f->P == 0 ? 0 : (f->line_form ? 2 : 1);
+    r.rl2 = (f->fft && f->ln >= 11 && f->nr >= 1 && f->nr <= kPassLines) ? 1 : 0;
     r.lptr = f->lptr;
     r.lt = f->lt;
     r.lidx = f->lidx;
@@ -954,13 +1065,16 @@ template <bool B>
 static int enqueue_correction(FastPoisson* f, const FdRuns& r, hipStream_t s) {
     const int m = f->m, n = f->n, P = f->P;
     const int lr = f->nr <= kLines ? f->nr : 0, lc = f->nc <= kLines ? f->nc : 0;
-    if (lr + lc > 0) {
+    const bool rows_in_pass = f->fft && f->ln >= 11 && f->nr >= 1 && f->nr <= kPassLines;   // FdRun::rl2
+    if (rows_in_pass || lr == 0) {
+        // the row lines came from the second FFT pass (or there are none)
+        if (lc > 0) launch_fd<B>(k_fd_collines<B>, dim3((unsigned)n), r, s);
+    } else {
         launch_fd<B>(k_fd_lines<B>, dim3((unsigned)((m + 63) / 64), kSeg), r, s);
         launch_fd<B>(k_fd_linesum<B>, dim3((unsigned)((lr * m + lc * n + kFdB - 1) / kFdB)), r, s);
     }
-    const unsigned wq = (unsigned)((P * 64 + kFdB - 1) / kFdB);
-    launch_fd<B>(k_fd_u0q<B>, dim3(wq), r, s);
-    launch_fd<B>(k_fd_sources<B>, dim3(wq), r, s);
+    launch_fd<B>(k_fd_u0q<B>, dim3((unsigned)P), r, s);
+    launch_fd<B>(k_fd_sources<B>, dim3((unsigned)P), r, s);
     if (f->line_form) {
         // one vector per covering line; the third transform pass adds the terms
         launch_fd<B>(k_fd_linevec<B>, dim3((unsigned)((std::max(m, n) + 63) / 64), (unsigned)(f->nr + f->nc)), r, s);

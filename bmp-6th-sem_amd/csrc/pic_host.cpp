@@ -810,6 +810,7 @@ extern "C" int pic_get_grid(pic_ctx* c, int which, double* out) {
         set_error("pic_get_grid: unknown grid %d", which);
         return PIC_ERR_ARG;
     }
+    if (which >= PIC_GRID_FINAL0) PIC_TRY(materialize_meshes(c));   // final_mesh / energy_mesh
     PIC_HIP(hipStreamSynchronize(c->stream));
     PIC_HIP(hipMemcpy(out, p, c->G * sizeof(double), hipMemcpyDeviceToHost));
     return PIC_OK;
@@ -830,6 +831,8 @@ extern "C" int pic_set_grid(pic_ctx* c, int which, const double* in) {
         set_error("pic_set_grid: unknown grid %d", which);
         return PIC_ERR_ARG;
     }
+    // the other meshes first, so that the caller's values are not overwritten later
+    if (which >= PIC_GRID_FINAL0) PIC_TRY(materialize_meshes(c));
     PIC_HIP(hipStreamSynchronize(c->stream));
     PIC_HIP(hipMemcpy(p, in, c->G * sizeof(double), hipMemcpyHostToDevice));
     if (which == PIC_GRID_BZ) {

@@ -94,10 +94,11 @@ struct FinalizeAll {
 };
 
 #ifdef __HIPCC__
-// final_mesh / energy_mesh of node n written, rho at n returned; a node's
-// fixed-point deposit is the sum of corner meshes 0 and 1 (node-row sums of
-// the cells above and below it).  The products are rounded before the sum
-// (no fused multiply-add) in every file that includes this.
+// rho at node n returned and, with MESH, final_mesh / energy_mesh of node n
+// written; a node's fixed-point deposit is the sum of corner meshes 0 and 1
+// (node-row sums of the cells above and below it).  The products are rounded
+// before the sum (no fused multiply-add) in every file that includes this.
+template <bool MESH = true>
 __device__ __forceinline__ double finalize_node(const FinalizeAll& a, int n, int gp, int energy) {
 #pragma clang fp contract(off)
     double r = 0.0;
@@ -107,9 +108,9 @@ __device__ __forceinline__ double finalize_node(const FinalizeAll& a, int n, int
         const long long* cq = a.cq[s];
         long long q = cq[n] + cq[gp + n];
         double f = (double)q * a.cqf[s];
-        a.fm[s][n] = f;
+        if (MESH) a.fm[s][n] = f;
         r += f * a.chrg[s];
-        if (energy) {
+        if (MESH && energy) {
             const long long* ce = a.ce[s];
             long long e = ce[n] + ce[gp + n];
             a.em[s][n] = (double)e * a.cef[s];
@@ -255,6 +256,10 @@ struct pic_ctx {
     pic::SolveTail tail;                             // a deferred solve end
     pic::FinalizeAll fin{};                          // a deposit node pass left to the next solve
     int fin_pending = 0;
+    // final_mesh / energy_mesh lag the fixed-point sums of the deposit in
+    // `fin` (the step's node pass writes rho only); materialize_meshes writes
+    // them when a caller reads or writes one
+    int meshes_stale = 0;
 };
 
 namespace pic {
@@ -267,6 +272,7 @@ int launch_deposit(pic_ctx* c, int s);            // adds species s to rho
 // every species; writes rho, or with defer_finalize leaves the node pass to
 // the next Poisson solve (c->fin)
 int launch_deposit_all(pic_ctx* c, bool defer_finalize = false);
+int materialize_meshes(pic_ctx* c);        // final_mesh / energy_mesh of the deferred node pass
 int launch_push(pic_ctx* c, int s);
 int launch_move(pic_ctx* c, int s);
 // launch_move in phases, for injection between them; with `guard`, the push

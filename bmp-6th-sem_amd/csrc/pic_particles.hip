@@ -788,8 +788,24 @@ int launch_deposit_all(pic_ctx* c, bool defer_finalize) {
         hipLaunchKernelGGL(k_deposit_finalize_all, dim3(blocks_for(c->G)), dim3(kBlock), 0, c->stream, c->G, c->g.gp,
                            a, c->p.deposit_energy, c->rho);
         PIC_HIP(hipGetLastError());
+        c->meshes_stale = 0;
     }
     PIC_TRY(timer_end(c, PIC_STAGE_FINALIZE, ev));
+    return PIC_OK;
+}
+
+// final_mesh / energy_mesh of the deposit in `a` (rho untouched).
+__global__ __launch_bounds__(kBlock) void k_finalize_meshes(int G, int gp, FinalizeAll a, int energy) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < G) (void)finalize_node<true>(a, n, gp, energy);
+}
+
+int materialize_meshes(pic_ctx* c) {
+    if (!c->meshes_stale) return PIC_OK;
+    hipLaunchKernelGGL(k_finalize_meshes, dim3(blocks_for(c->G)), dim3(kBlock), 0, c->stream, c->G, c->g.gp, c->fin,
+                       c->p.deposit_energy);
+    PIC_HIP(hipGetLastError());
+    c->meshes_stale = 0;
     return PIC_OK;
 }
 

@@ -191,8 +191,9 @@ __global__ __launch_bounds__(kB) void k_prep_rhs(Fine f, double* __restrict__ rh
 }
 
 // k_prep_rhs with the deposit's node pass in front: rho[n] is the species
-// sum of the corner meshes (final_mesh, energy_mesh written on the way), then
-// overwritten as above.  One pass instead of two over the grid.
+// sum of the corner meshes, then overwritten as above.  One pass instead of
+// two over the grid; final_mesh and energy_mesh are left to
+// materialize_meshes (read by diagnostics only).
 __global__ __launch_bounds__(kB) void k_finalize_prep(FinalizeAll a, int gp, int energy, Fine f,
                                                       double* __restrict__ rho, const uint8_t* __restrict__ dir,
                                                       const double* __restrict__ dval, double* part_bb, int* flags) {
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(kB) void k_finalize_prep(FinalizeAll a, int gp, int
     const int G = f.gx * f.gy;
     auto x0 = [&](int m) -> double { return dir[m] ? dval[m] : 0.0; };
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
-        const double dep = finalize_node(a, n, gp, energy);
+        const double dep = finalize_node<false>(a, n, gp, energy);
         const uint8_t d = dir[n];
         const double r = d == 1 ? dval[n] : dep;
         rho[n] = -r;
@@ -1373,6 +1374,7 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
         // the deposit's node pass, deferred to here, writes rho first
         hipLaunchKernelGGL(k_finalize_prep, dim3(kNP), dim3(kB), 0, s, c->fin, c->g.gp, c->p.deposit_energy, f,
                            c->rho, c->dir, c->dval, part + P_BB * kNP, c->flags);
+        c->meshes_stale = 1;
     } else {
         hipLaunchKernelGGL(k_prep_rhs, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->dir, c->dval, part + P_BB * kNP,
                            c->flags);

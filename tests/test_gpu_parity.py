@@ -710,3 +710,36 @@ def test_timing_modes_count_their_launches(hip_mod):
     with pytest.raises(hip.PicError):
         c.set_timing(4)
     c.close()
+
+
+def test_step_meshes_written_on_demand(hip_mod, oracle_mod):
+    """pic_step's node pass writes rho only; final_mesh and energy_mesh are
+    formed from the step's fixed-point sums when read.  After two steps they
+    equal the deposit of the state the second step started from; a caller's
+    pic_set_grid of a mesh stands until the next step replaces it."""
+    hip, O = hip_mod, oracle_mod
+    c = _ctx(hip, 64, 1, geom=ELEC)
+    n = 40000
+    for sp, (q, m, T) in enumerate([(-E, ME, 2.0), (E, MI, 0.03)]):
+        s = c.add_species(q, m, n, 1e16, n)
+        c.upload(s, *_parts(hip, c, n, seed=9, stream=sp, qm=abs(q / m), T=T))
+    c.step(0)
+    before = [c.download_cell32(s) for s in range(2)]
+    c.step(1)
+    for sp, (q, m, T) in enumerate([(-E, ME, 2.0), (E, MI, 0.03)]):
+        cell, ox, oy, fvx, fvy, fvz = before[sp]
+        W = 1e16 * (c.params.width * c.params.height) / (n * c.dx * c.dy * c.dx * c.dy)
+        xd = ((cell % c.gx) + ox.astype(np.float64)) * c.dx
+        yd = ((cell // c.gx) + oy.astype(np.float64)) * c.dy
+        fm, em, _ = O.deposit_ref(xd, yd, fvx, fvy, fvz, cell, c.gx, c.gy, c.dx, c.dy, W, q / m, q, 8.854e-12)
+        assert np.abs(c.grid(hip.GRID_FINAL0 + sp) - fm).max() / np.abs(fm).max() < 1e-12
+        # the 2^-32 eV fixed-point step is relatively larger for the 0.03 eV ions
+        etol = 1e-9 if sp == 0 else 1e-8
+        assert np.abs(c.grid(hip.GRID_ENERGY0 + sp) - em).max() / np.abs(em).max() < etol
+    mine = np.linspace(0.0, 1.0, c.G)
+    c.set_grid(hip.GRID_FINAL0 + 1, mine)
+    assert np.array_equal(c.grid(hip.GRID_FINAL0 + 1), mine)
+    assert np.array_equal(c.grid(hip.GRID_FINAL0 + 1), mine)
+    c.step(2)
+    assert not np.array_equal(c.grid(hip.GRID_FINAL0 + 1), mine)
+    c.close()
