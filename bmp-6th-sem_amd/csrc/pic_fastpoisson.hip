@@ -274,6 +274,22 @@ __global__ __launch_bounds__(kFdB) void k_fd_u0q(const FdRun one, const FdRun* _
     const int line = f.qri[q];
     const double *a, *b;
     int len;
+    if (line >= 0 && f.rl2) {
+        // the second FFT pass left four wave partials per value
+        const double4* __restrict__ a4 = (const double4*)(f.V1 + (size_t)line * f.m * 4);
+        b = f.Sm + (size_t)f.qcol[q] * f.m;
+        double s = 0.0;
+#pragma unroll 4
+        for (int l = threadIdx.x; l < f.m; l += kFdB) {
+            const double4 p = a4[l];
+            s += ((p.x + p.y) + (p.z + p.w)) * b[l];
+        }
+        s = wave_sum(s);
+        if (lane == 0) part[wv] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) f.dq[q] = f.VQ[q] - ((part[0] + part[1]) + (part[2] + part[3]));
+        return;
+    }
     if (line >= 0) {
         a = f.V1 + (size_t)line * f.m;
         b = f.Sm + (size_t)f.qcol[q] * f.m;
@@ -680,20 +696,31 @@ struct DstStoreGrid {
 
 // Row lines of the covering (at most kPassLines) taken by the second pass
 // while it stores the spectrum: V1[a][row] = sum_t S_n[row a][t] Uhat[t][row]
-// for the pass's two rows (columns of Uhat), block-reduced in a fixed order.
+// for the pass's two rows (columns of Uhat).  Each of the workgroup's four
+// waves leaves its partial sum (V1 as lr x R x 4); their reader adds the four
+// in a fixed order, so the pass needs no extra barrier.
 static constexpr int kPassLines = 2;
 struct PassLines {
     int lr;                                  // 0: none
     const double* SnR;                       // lr x N
-    double* V1;                              // lr x R
+    double* V1;                              // lr x R x 4 wave partials
 };
 
 // One pass: R rows of N = 2^(L-1) - 1 values in, N rows of R values out.
 // Workgroup w (after the XCD-aware remap: the workgroups of one XCD take
 // consecutive row pairs, so the transposed stores of one cache line meet in
 // one L2) transforms NB = max(1, 2048 / M) row pairs.
+// LDS of a pass: the workgroup's points and, up to L = 11, the twiddles
+// (64 KB in all); declared once by the kernel, whatever bodies it inlines.
+template <int L>
+struct DstLds {
+    static constexpr int M = 1 << L;
+    static constexpr int PTS = (M >= kDstPts ? 1 : kDstPts / M) * M;
+    static constexpr int size = PTS + (L <= 11 ? M : 0);
+};
+
 template <int L, class Load, class Store, bool LINES = false>
-__device__ __forceinline__ void dst_body(int R, int nwg, const Load& ld, const Store& st,
+__device__ __forceinline__ void dst_body(double2* lds, int R, int nwg, const Load& ld, const Store& st,
                                          const double2* __restrict__ tw, const PassLines pl = PassLines{}) {
     constexpr int M = 1 << L;
     constexpr int N = M / 2 - 1;
@@ -703,7 +730,7 @@ __device__ __forceinline__ void dst_body(int R, int nwg, const Load& ld, const S
     constexpr int NE = NB * N;                           // values per row of the workgroup's pairs
     constexpr int IT = (NE + kDstT - 1) / kDstT;
     constexpr bool kTwLds = L <= 11;                     // twiddles staged in LDS (64 KB in all)
-    __shared__ double2 lds[PTS + (kTwLds ? M : 0)];
+    static_assert(DstLds<L>::size == PTS + (kTwLds ? M : 0), "LDS layout of a pass");
     double2* z = lds;
     double2* tws = lds + PTS;
     const double2* twp = kTwLds ? tws : tw;
@@ -714,7 +741,18 @@ __device__ __forceinline__ void dst_body(int R, int nwg, const Load& ld, const S
     __builtin_assume(tid < kDstT);
     const int r0 = w * 2 * NB;
     // loads: indices clamped into the matrix so that none waits behind a branch
+    // (with LINES the S_n rows of the line sums too, so that they arrive during
+    // the FFT)
     double va[IT], vb[IT];
+    double sn[LINES ? kPassLines : 1][IT];
+    if constexpr (LINES) {
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int t = min(tid + kDstT * it, NE - 1);
+#pragma unroll
+            for (int a = 0; a < kPassLines; ++a) sn[a][it] = a < pl.lr ? pl.SnR[(size_t)a * N + t] : 0.0;
+        }
+    }
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int e = min(tid + kDstT * it, NE - 1);
@@ -799,7 +837,6 @@ __device__ __forceinline__ void dst_body(int R, int nwg, const Load& ld, const S
     }
     // the storer's gathers first, then the stores
     typename Store::Aux xa[IT], xb[IT];
-    double sn[LINES ? kPassLines : 1][IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int e = min(tid + kDstT * it, NE - 1);
@@ -807,10 +844,6 @@ __device__ __forceinline__ void dst_body(int R, int nwg, const Load& ld, const S
         const int ra = r0 + 2 * b;
         xa[it] = st.fetch(t, min(ra, R - 1));
         xb[it] = st.fetch(t, min(ra + 1, R - 1));
-        if constexpr (LINES) {
-#pragma unroll
-            for (int a = 0; a < kPassLines; ++a) sn[a][it] = a < pl.lr ? pl.SnR[(size_t)a * N + t] : 0.0;
-        }
     }
     const double sc = 0.5 * sqrt(2.0 / (double)(N + 1));
     double acc[LINES ? kPassLines : 1][2] = {};
@@ -835,23 +868,14 @@ __device__ __forceinline__ void dst_body(int R, int nwg, const Load& ld, const S
     }
     if constexpr (LINES) {
         static_assert(NB == 1 && kDstT == 256, "the line sums assume one row pair per workgroup of four waves");
-        // the sums of the four waves, added in a fixed order (the FFT's LDS is free now)
-        __syncthreads();
-        double* red = (double*)lds;
         const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
         for (int a = 0; a < kPassLines; ++a)
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 const double w = wave_sum(acc[a][c]);
-                if (lane == 0) red[(a * 2 + c) * 4 + wv] = w;
+                if (lane == 0 && a < pl.lr && r0 + c < R) pl.V1[((size_t)a * R + r0 + c) * 4 + wv] = w;
             }
-        __syncthreads();
-        if (tid < 2 * pl.lr) {
-            const int a = tid >> 1, c = tid & 1;
-            const double* q = red + (a * 2 + c) * 4;
-            if (r0 + c < R) pl.V1[(size_t)a * R + r0 + c] = (q[0] + q[1]) + (q[2] + q[3]);
-        }
     }
 }
 
@@ -860,27 +884,28 @@ __device__ __forceinline__ void dst_body(int R, int nwg, const Load& ld, const S
 //   3: rows of Uhat (+ G ./ Lambda) -> W      4: rows of W -> the grid's interior
 template <int L, int KIND, bool B>
 __global__ __launch_bounds__(kDstT) void k_fd_dst(const FdRun one, const FdRun* __restrict__ runs, int R, int nwg) {
+    __shared__ double2 lds[DstLds<L>::size];
     const FdRun& f = B ? runs[blockIdx.y] : one;
     if constexpr (KIND == 1)
-        dst_body<L>(R, nwg, DstLoadRhs{f.gx, f.m, f.sgn, f.rho, f.dir, f.Fb}, DstStoreT{f.W, f.n}, f.twm);
+        dst_body<L>(lds, R, nwg, DstLoadRhs{f.gx, f.m, f.sgn, f.rho, f.dir, f.Fb}, DstStoreT{f.W, f.n}, f.twm);
     else if constexpr (KIND == 2) {
         // the row lines ride along where a workgroup holds one row pair (L >= 11)
         if constexpr (L >= 11) {
             if (f.rl2) {
-                dst_body<L, DstLoadRows, DstStoreTScaled, true>(R, nwg, DstLoadRows{f.W, f.n},
+                dst_body<L, DstLoadRows, DstStoreTScaled, true>(lds, R, nwg, DstLoadRows{f.W, f.n},
                                                                 DstStoreTScaled{f.U, f.Dinv, f.m}, f.twn,
                                                                 PassLines{f.lr, f.SnR, f.V1});
                 return;
             }
         }
-        dst_body<L>(R, nwg, DstLoadRows{f.W, f.n}, DstStoreTScaled{f.U, f.Dinv, f.m}, f.twn);
+        dst_body<L>(lds, R, nwg, DstLoadRows{f.W, f.n}, DstStoreTScaled{f.U, f.Dinv, f.m}, f.twn);
     }
     else if constexpr (KIND == 3)
-        dst_body<L>(R, nwg,
+        dst_body<L>(lds, R, nwg,
                     DstLoadCorrected{f.U, f.G, f.Dinv, f.m, f.cmode, f.n, f.lr, f.lc, f.SnR, f.SmC, f.LV},
                     DstStoreT{f.W, f.n}, f.twm);
     else
-        dst_body<L>(R, nwg, DstLoadRows{f.W, f.n}, DstStoreGrid{f.phi, f.gx}, f.twn);
+        dst_body<L>(lds, R, nwg, DstLoadRows{f.W, f.n}, DstStoreGrid{f.phi, f.gx}, f.twn);
 }
 
 // Runs of a launch: one run's descriptor by value (kernel argument), or a
@@ -1202,7 +1227,7 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
     f->U = (double*)dmalloc(nm * 8, ok);
     f->G = (double*)dmalloc(nm * 8, ok);
     f->SnR = (double*)dmalloc(SnR.size() * 8, ok);
-    f->V1 = (double*)dmalloc((size_t)std::max(nr, 1) * m * 8, ok);
+    f->V1 = (double*)dmalloc((size_t)std::max(nr, 1) * m * 4 * 8, ok);   // room for the pass's wave partials
     f->SmC = (double*)dmalloc(SmC.size() * 8, ok);
     f->V2 = (double*)dmalloc((size_t)std::max(nc, 1) * n * 8, ok);
     f->qrow = (int*)dmalloc((size_t)std::max(P, 1) * 4, ok);

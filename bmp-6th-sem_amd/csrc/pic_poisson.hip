@@ -164,6 +164,37 @@ __global__ __launch_bounds__(kB) void k_residual(Fine f, const double* __restric
     write_partial(acc, part, sh);
 }
 
+// k_residual after the direct solve, with overwritePhi in the same pass:
+// Dirichlet nodes take their values (the image row of a periodic grid row
+// 0's), and the stencil of an unknown node reads a Dirichlet neighbour's
+// value from dval, so no node is read while it is written (an unknown node's
+// stencil never reaches the image row).  The same residual as k_overwrite_phi
+// followed by k_residual.
+__global__ __launch_bounds__(kB) void k_residual_fix(Fine f, const double* __restrict__ rhs, double* __restrict__ x,
+                                                     const uint8_t* __restrict__ dir, const double* __restrict__ dval,
+                                                     double* __restrict__ r, double* part) {
+    __shared__ double sh[kB / 64];
+    double acc = 0;
+    const int G = f.gx * f.gy;
+    auto xe = [&](int m) { return dir[m] == 1 ? dval[m] : x[m]; };
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
+        double v = 0;
+        const uint8_t d = dir[n];
+        if (!d) {
+            const int so = south_n(f.per, n, f.gx, f.gy), no = north_n(f.per, n, f.gx, f.gy);
+            double ax = f.diag * x[n] - f.cx * (xe(n - 1) + xe(n + 1)) - f.cy * (xe(n + so) + xe(n + no));
+            v = rhs[n] - ax;
+        } else if (d == 1) {
+            x[n] = dval[n];
+        } else {
+            x[n] = xe(n % f.gx);
+        }
+        r[n] = v;
+        acc += v * v;
+    }
+    write_partial(acc, part, sh);
+}
+
 // The start of every solve in one pass: flags cleared; overwriteRho
 // (rho[geom] = V, rho = -rho); the partial of b.b, b = the residual of the
 // Dirichlet-only field (unknowns at zero) on the unknown nodes.
@@ -1423,14 +1454,16 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
     const bool direct = c->fast && !mixed && !learned && !c->guess_set;
     if (direct) {
         if (keep) PIC_HIP(hipMemcpyAsync(keep, c->phi, (size_t)G * sizeof(double), hipMemcpyDeviceToDevice, s));
-        PIC_TRY(fast_solve(c, c->phi));   // the interior nodes
-        PIC_TRY(launch_overwrite_phi(c)); // the Dirichlet values
+        PIC_TRY(fast_solve(c, c->phi));   // the interior nodes; the residual imposes the Dirichlet values
         guess = 5;
+        hipLaunchKernelGGL(k_residual_fix, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, c->dval, R[0],
+                           part + P_RR * kNP);
     } else {
         hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, src, prev, keep, c->dir,
                            c->dval);
+        hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, R[0],
+                           part + P_RR * kNP);
     }
-    hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, R[0], part + P_RR * kNP);
     hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2, c->flags,
                        c->scal, 1, S_BB, S_RR, S_RR0);
     PIC_HIP(hipGetLastError());
