@@ -696,6 +696,7 @@ void advance_host_state(pic_ensemble* e, int n) {
         c->have_prev_phi = 1;
         c->guess_set = 0;
         if (n > 0) c->meshes_stale = 0;     // the ensemble's node pass writes the meshes
+        if (n > 0) c->e_stale = 0;          // and its field kernel ex, ey with fld
     }
 }
 

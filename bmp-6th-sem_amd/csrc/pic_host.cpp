@@ -811,6 +811,7 @@ extern "C" int pic_get_grid(pic_ctx* c, int which, double* out) {
         return PIC_ERR_ARG;
     }
     if (which >= PIC_GRID_FINAL0) PIC_TRY(materialize_meshes(c));   // final_mesh / energy_mesh
+    if (which == PIC_GRID_EX || which == PIC_GRID_EY) PIC_TRY(materialize_e(c));
     PIC_HIP(hipStreamSynchronize(c->stream));
     PIC_HIP(hipMemcpy(out, p, c->G * sizeof(double), hipMemcpyDeviceToHost));
     return PIC_OK;
@@ -820,6 +821,26 @@ __global__ void k_pack_fld(int G, const double* ex, const double* ey, const doub
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n < G) fld[n] = make_double4(ex[n], ey[n], bz[n], 0.0);
 }
+
+__global__ void k_unpack_e(int G, const double4* fld, double* ex, double* ey) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < G) {
+        const double4 v = fld[n];
+        ex[n] = v.x;
+        ey[n] = v.y;
+    }
+}
+
+namespace pic {
+int materialize_e(pic_ctx* c) {
+    if (!c->e_stale) return PIC_OK;
+    hipLaunchKernelGGL(k_unpack_e, dim3((c->G + 255) / 256), dim3(256), 0, c->stream, c->G, (const double4*)c->fld,
+                       c->ex, c->ey);
+    PIC_HIP(hipGetLastError());
+    c->e_stale = 0;
+    return PIC_OK;
+}
+} // namespace pic
 
 extern "C" int pic_set_grid(pic_ctx* c, int which, const double* in) {
     if (!c || !in) {
@@ -831,8 +852,10 @@ extern "C" int pic_set_grid(pic_ctx* c, int which, const double* in) {
         set_error("pic_set_grid: unknown grid %d", which);
         return PIC_ERR_ARG;
     }
-    // the other meshes first, so that the caller's values are not overwritten later
+    // the lagging grids first, so that the caller's values are not overwritten
+    // later and a repack of fld sees current ex / ey
     if (which >= PIC_GRID_FINAL0) PIC_TRY(materialize_meshes(c));
+    if (which == PIC_GRID_EX || which == PIC_GRID_EY || which == PIC_GRID_BZ) PIC_TRY(materialize_e(c));
     PIC_HIP(hipStreamSynchronize(c->stream));
     PIC_HIP(hipMemcpy(p, in, c->G * sizeof(double), hipMemcpyHostToDevice));
     if (which == PIC_GRID_BZ) {

@@ -260,6 +260,10 @@ struct pic_ctx {
     // `fin` (the step's node pass writes rho only); materialize_meshes writes
     // them when a caller reads or writes one
     int meshes_stale = 0;
+    // ex / ey lag the gather array fld (the direct solve's residual pass
+    // writes E into fld only); materialize_e forms them when a caller reads
+    // or writes a field grid
+    int e_stale = 0;
 };
 
 namespace pic {
@@ -273,6 +277,7 @@ int launch_deposit(pic_ctx* c, int s);            // adds species s to rho
 // the next Poisson solve (c->fin)
 int launch_deposit_all(pic_ctx* c, bool defer_finalize = false);
 int materialize_meshes(pic_ctx* c);        // final_mesh / energy_mesh of the deferred node pass
+int materialize_e(pic_ctx* c);             // ex / ey from fld
 int launch_push(pic_ctx* c, int s);
 int launch_move(pic_ctx* c, int s);
 // launch_move in phases, for injection between them; with `guard`, the push

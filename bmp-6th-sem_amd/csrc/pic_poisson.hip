@@ -170,14 +170,35 @@ __global__ __launch_bounds__(kB) void k_residual(Fine f, const double* __restric
 // value from dval, so no node is read while it is written (an unknown node's
 // stencil never reaches the image row).  The same residual as k_overwrite_phi
 // followed by k_residual.
+// With EF (non-periodic grids) the same pass writes E = -grad phi as
+// k_efield does (central differences at the clamped interior node, the same
+// expressions), reading every phi through the same Dirichlet substitution,
+// into the gather array fld only: ex / ey are formed from fld when a caller
+// reads them (pic_ctx::e_stale).
+struct EfOut {
+    double dx, dy;
+    const double* bz;
+    double *ex, *ey;
+    double4* fld;
+};
+
+template <bool EF>
 __global__ __launch_bounds__(kB) void k_residual_fix(Fine f, const double* __restrict__ rhs, double* __restrict__ x,
                                                      const uint8_t* __restrict__ dir, const double* __restrict__ dval,
-                                                     double* __restrict__ r, double* part) {
+                                                     double* __restrict__ r, double* part, EfOut e) {
     __shared__ double sh[kB / 64];
     double acc = 0;
     const int G = f.gx * f.gy;
     auto xe = [&](int m) { return dir[m] == 1 ? dval[m] : x[m]; };
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
+        if constexpr (EF) {
+            const int j = n % f.gx, i = n / f.gx;
+            const int jj = min(max(j, 1), f.gx - 2), ii = min(max(i, 1), f.gy - 2);
+            const int m = ii * f.gx + jj;
+            const double fx = -((xe(m + 1) - xe(m - 1)) / (2 * e.dx));
+            const double fy = -((xe(m + f.gx) - xe(m - f.gx)) / (2 * e.dy));
+            e.fld[n] = make_double4(fx, fy, e.bz[n], 0.0);
+        }
         double v = 0;
         const uint8_t d = dir[n];
         if (!d) {
@@ -1171,6 +1192,7 @@ int launch_overwrite_phi(pic_ctx* c) {
 }
 
 int launch_efield(pic_ctx* c) {
+    c->e_stale = 0;
     hipEvent_t ev;
     PIC_TRY(timer_begin(c, PIC_STAGE_EFIELD, &ev));
     if (c->p.periodic)
@@ -1452,12 +1474,22 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
     // direct solve: the PCG below starts from it and only checks (or
     // corrects) the residual
     const bool direct = c->fast && !mixed && !learned && !c->guess_set;
+    bool efield_done = false;
     if (direct) {
         if (keep) PIC_HIP(hipMemcpyAsync(keep, c->phi, (size_t)G * sizeof(double), hipMemcpyDeviceToDevice, s));
         PIC_TRY(fast_solve(c, c->phi));   // the interior nodes; the residual imposes the Dirichlet values
         guess = 5;
-        hipLaunchKernelGGL(k_residual_fix, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, c->dval, R[0],
-                           part + P_RR * kNP);
+        if (f.per) {
+            hipLaunchKernelGGL(k_residual_fix<false>, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, c->dval,
+                               R[0], part + P_RR * kNP, EfOut{});
+        } else {
+            // and E in the same pass
+            hipLaunchKernelGGL(k_residual_fix<true>, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, c->dval,
+                               R[0], part + P_RR * kNP,
+                               EfOut{c->g.dx, c->g.dy, c->bz, c->ex, c->ey, (double4*)c->fld});
+            efield_done = true;
+            c->e_stale = 1;
+        }
     } else {
         hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, src, prev, keep, c->dir,
                            c->dval);
@@ -1480,7 +1512,7 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
         PIC_HIP(hipMemcpyAsync(c->scal_host, c->scal, PIC_STATE_BYTES, hipMemcpyDeviceToHost, s));
         PIC_HIP(hipEventRecord(t1, s));
         PIC_TRY(timer_end(c, PIC_STAGE_POISSON, ev));
-        PIC_TRY(launch_efield(c));
+        if (!efield_done) PIC_TRY(launch_efield(c));
         if (defer) {
             // the caller enqueues more work first and calls poisson_finish
             tail.active = 1;
