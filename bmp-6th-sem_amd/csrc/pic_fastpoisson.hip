@@ -149,7 +149,8 @@ __global__ __launch_bounds__(kFdB) void k_fd_addscaled(int N, double* __restrict
 // Lane l of an MFMA holds A[l % 16][l / 16], B[l / 16][l % 16] and receives
 // D[l / 16 + 4 e][l % 16], e < 4 (rows interleaved over the lane groups).
 typedef double d4v __attribute__((ext_vector_type(4)));
-static constexpr int kGT = 32, kGK = 32;
+static constexpr int kGT = 32, kGK = 64;
+static constexpr int kGE = kGT * kGK / 256;   // elements of each operand chunk per thread
 
 template <bool TA, bool TB, int EP>
 __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double* __restrict__ A, int lda,
@@ -160,22 +161,22 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double
     const int i0 = blockIdx.y * kGT, j0 = blockIdx.x * kGT;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wi = (w >> 1) * 16, wj = (w & 1) * 16;
-    double ra[4], rb[4];
+    double ra[kGE], rb[kGE];
     // element e of thread t: (row, k) of the A chunk and (k, column) of the B
     // chunk, the fast index along the stored rows so that loads coalesce
     auto a_pos = [&](int e, int& ii, int& kk) {
         const int idx = t + 256 * e;
-        if (TA) { kk = idx >> 5; ii = idx & 31; }
-        else { ii = idx >> 5; kk = idx & 31; }
+        if (TA) { kk = idx / kGT; ii = idx % kGT; }
+        else { ii = idx / kGK; kk = idx % kGK; }
     };
     auto b_pos = [&](int e, int& kk, int& jj) {
         const int idx = t + 256 * e;
-        if (TB) { jj = idx >> 5; kk = idx & 31; }
-        else { kk = idx >> 5; jj = idx & 31; }
+        if (TB) { jj = idx / kGK; kk = idx % kGK; }
+        else { kk = idx / kGT; jj = idx % kGT; }
     };
     auto load = [&](int k0) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < kGE; ++e) {
             int ii, kk, jj, kb;
             a_pos(e, ii, kk);
             const int gi = i0 + ii, gk = k0 + kk;
@@ -185,12 +186,14 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double
             rb[e] = (gj < N && gk2 < K) ? (TB ? B[(size_t)gj * ldb + gk2] : B[(size_t)gk2 * ldb + gj]) : 0.0;
         }
     };
-    d4v acc = {0.0, 0.0, 0.0, 0.0};
+    // two accumulators (even and odd k steps) so that consecutive MFMAs do not
+    // wait for each other; added at the end
+    d4v acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
     load(0);
     for (int k0 = 0; k0 < K; k0 += kGK) {
         __syncthreads();                    // the previous chunk is consumed
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < kGE; ++e) {
             int ii, kk, jj, kb;
             a_pos(e, ii, kk);
             As[kk][ii] = ra[e];
@@ -200,12 +203,16 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, const double
         __syncthreads();
         if (k0 + kGK < K) load(k0 + kGK);
 #pragma unroll
-        for (int kk = 0; kk < kGK; kk += 4) {
+        for (int kk = 0; kk < kGK; kk += 8) {
             const double a = As[kk + (lane >> 4)][wi + (lane & 15)];
             const double b = Bs[kk + (lane >> 4)][wj + (lane & 15)];
+            const double a2 = As[kk + 4 + (lane >> 4)][wi + (lane & 15)];
+            const double b2 = Bs[kk + 4 + (lane >> 4)][wj + (lane & 15)];
             acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b2, acc2, 0, 0, 0);
         }
     }
+    acc += acc2;
     const int j = j0 + wj + (lane & 15);
     if (j >= N) return;
 #pragma unroll
@@ -637,6 +644,19 @@ struct DstLoadCorrected {
         return U[i] + g * Dinv[i];
     }
 };
+
+// The line form of the correction on the GEMM path: Uhat += G ./ Lambda with
+// G the covering lines' rank-1 terms (the loader of the third FFT pass,
+// applied in place)
+template <bool B>
+__global__ __launch_bounds__(kFdB) void k_fd_addlines(const FdRun one, const FdRun* __restrict__ runs) {
+    const FdRun& f = B ? runs[blockIdx.z] : one;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= f.n * f.m) return;
+    const int row = k / f.m, col = k - row * f.m;
+    const DstLoadCorrected ld{f.U, f.G, f.Dinv, f.m, 2, f.n, f.lr, f.lc, f.SnR, f.SmC, f.LV};
+    f.U[k] = ld(row, col);
+}
 
 // the right-hand side F at interior node (r + 1, c + 1): rhs on unknowns plus
 // the wall terms Fb (fixed by the geometry), 0 on internal Dirichlet nodes.
@@ -1352,7 +1372,7 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
         PIC_HIP(hipMemcpy(f->Cinv, C.data(), (size_t)P * P * 8, hipMemcpyHostToDevice));
     }
     if (qrow_d) (void)hipFree(qrow_d);
-    if (P > 0 && f->fft && nr <= kLines && nc <= kLines) {
+    if (P > 0 && nr <= kLines && nc <= kLines) {
         // line form: the sources of each covering line (rows first, then
         // columns), q ascending; lt indexes the row-ordered sources cq
         std::vector<int> tpos(P);
@@ -1428,7 +1448,10 @@ int fast_solve(pic_ctx* c, double* out) {
         if (nr > kLines) dgemm<false, false, 0>(s, nr, m, n, f->SnR, n, f->U, m, f->V1, m);
         if (nc > kLines) dgemm<false, true, 0>(s, nc, n, m, f->SmC, m, f->U, m, f->V2, n);
         PIC_TRY(enqueue_correction<false>(f, one, s));
-        if (!f->fft) hipLaunchKernelGGL(k_fd_addscaled, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->G, f->Dinv);
+        if (!f->fft) {
+            if (f->line_form) launch_fd<false>(k_fd_addlines<false>, dim3(nb), one, s);
+            else hipLaunchKernelGGL(k_fd_addscaled, dim3(nb), dim3(kFdB), 0, s, nm, f->U, f->G, f->Dinv);
+        }
     }
     if (f->fft) {
         // rows of Uhat (+ G ./ Lambda) -> W (m x n) -> the grid's interior nodes
