@@ -135,6 +135,8 @@ extern "C" void pic_destroy(pic_ctx* c) {
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c->scal_host) (void)hipHostFree(c->scal_host);
+    if (c->mail) (void)hipHostFree(c->mail);
+    if (c->tstamp) (void)hipFree(c->tstamp);
     for (auto& t : c->timers)
         for (auto& e : t.pending) {
             (void)hipEventDestroy(e.first);
@@ -294,6 +296,23 @@ extern "C" int pic_create(const pic_params* p, pic_ctx** out) {
         return fail(PIC_ERR_ALLOC);
     }
     c->flags_host = (int*)(c->scal_host + PIC_SCAL_DOUBLES);
+    if (hipHostMalloc((void**)&c->mail, sizeof(SolveMail), hipHostMallocMapped | hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostGetDevicePointer((void**)&c->mail_dev, c->mail, 0) != hipSuccess) {
+        set_error("pic_create: mapped host memory for the solve check failed");
+        return fail(PIC_ERR_ALLOC);
+    }
+    std::memset(c->mail, 0, sizeof(SolveMail));
+    if (hipMalloc((void**)&c->tstamp, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        set_error("pic_create: device allocation failed");
+        return fail(PIC_ERR_ALLOC);
+    }
+    {
+        int dev = 0, khz = 0;
+        PIC_HIP(hipGetDevice(&dev));
+        PIC_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        c->wall_khz = khz > 0 ? (double)khz : 100000.0;
+    }
     (void)hipMemset(c->rho, 0, G * sizeof(double));
     (void)hipMemset(c->phi, 0, G * sizeof(double));
     (void)hipMemset(c->ex, 0, G * sizeof(double));

@@ -17,6 +17,15 @@
 #define PIC_SCAL_DOUBLES 16
 #define PIC_STATE_BYTES (PIC_SCAL_DOUBLES * 8 + 4 * 4)
 
+// the state block as the check kernel publishes it to the host
+struct SolveMail {
+    double scal[PIC_SCAL_DOUBLES];
+    int flags[4];
+    unsigned long long t0, t1;     // device wall clock at the solve's start and at the check
+    unsigned seq;
+    unsigned pad;
+};
+
 namespace pic {
 
 // ------------------------------------------------------------------ errors
@@ -253,6 +262,15 @@ struct pic_ctx {
     pic::StageTimer timers[PIC_NSTAGES];
     std::vector<hipEvent_t> event_pool;   // recorded-and-read stage events, reused
     hipEvent_t solve_ev[2] = {nullptr, nullptr};   // start / end of a Poisson solve
+    // The direct solve's check is published by the check kernel itself into
+    // mapped, coherent host memory (state block, device wall-clock stamps,
+    // then a sequence number after a system-scope fence): no copy and no
+    // event in the step; the host waits for the sequence number.
+    SolveMail* mail = nullptr;                     // host view
+    SolveMail* mail_dev = nullptr;                 // device view of the same memory
+    unsigned mail_seq = 0;                         // sequence number of the last publication
+    unsigned long long* tstamp = nullptr;          // device: wall clock at the solve's start
+    double wall_khz = 0;                           // wall-clock rate
     pic::SolveTail tail;                             // a deferred solve end
     pic::FinalizeAll fin{};                          // a deposit node pass left to the next solve
     int fin_pending = 0;

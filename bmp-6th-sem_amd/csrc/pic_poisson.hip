@@ -41,7 +41,9 @@
 #include "pic_unet.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstring>
 
 namespace pic {
 
@@ -220,8 +222,10 @@ __global__ __launch_bounds__(kB) void k_residual_fix(Fine f, const double* __res
 // (rho[geom] = V, rho = -rho); the partial of b.b, b = the residual of the
 // Dirichlet-only field (unknowns at zero) on the unknown nodes.
 __global__ __launch_bounds__(kB) void k_prep_rhs(Fine f, double* __restrict__ rho, const uint8_t* __restrict__ dir,
-                                                 const double* __restrict__ dval, double* part_bb, int* flags) {
+                                                 const double* __restrict__ dval, double* part_bb, int* flags,
+                                                 unsigned long long* ts) {
     __shared__ double sh[kB / 64];
+    if (ts && blockIdx.x == 0 && threadIdx.x == 0) ts[0] = wall_clock64();   // the solve's start
     // flags: [0] outer converged (1) / failed (2), [1] outer iterations,
     // [2] inner (mixed precision) converged / failed, [3] inner iterations
     if (blockIdx.x == 0 && threadIdx.x < 4) flags[threadIdx.x] = 0;
@@ -248,9 +252,11 @@ __global__ __launch_bounds__(kB) void k_prep_rhs(Fine f, double* __restrict__ rh
 // materialize_meshes (read by diagnostics only).
 __global__ __launch_bounds__(kB) void k_finalize_prep(FinalizeAll a, int gp, int energy, Fine f,
                                                       double* __restrict__ rho, const uint8_t* __restrict__ dir,
-                                                      const double* __restrict__ dval, double* part_bb, int* flags) {
+                                                      const double* __restrict__ dval, double* part_bb, int* flags,
+                                                      unsigned long long* ts) {
     __shared__ double sh[kB / 64];
     if (blockIdx.x == 0 && threadIdx.x < 4) flags[threadIdx.x] = 0;
+    if (ts && blockIdx.x == 0 && threadIdx.x == 0) ts[0] = wall_clock64();   // the solve's start
     double acc = 0;
     const int G = f.gx * f.gy;
     auto x0 = [&](int m) -> double { return dir[m] ? dval[m] : 0.0; };
@@ -273,9 +279,8 @@ __global__ __launch_bounds__(kB) void k_finalize_prep(FinalizeAll a, int gp, int
 // The first call also records bb (from part_bb, or rr itself when part_bb is
 // null) and rr0; later calls count one more iteration unless already done.
 // sbb, srr, srr0: the scalar slots (outer solve or inner correction solve).
-__global__ __launch_bounds__(1024) void k_check(const double* part_rr, const double* part_bb, double tol2,
-                                                int* flags, double* scal, int first, int sbb, int srr, int srr0) {
-    __shared__ double sh[16];
+__device__ __forceinline__ void check_body(const double* part_rr, const double* part_bb, double tol2, int* flags,
+                                           double* scal, int first, int sbb, int srr, int srr0, double* sh) {
     if (flags[0]) return;
     double rr = sum_partials(part_rr, sh);
     double bb = first ? (part_bb ? sum_partials(part_bb, sh) : rr) : scal[sbb];
@@ -289,6 +294,31 @@ __global__ __launch_bounds__(1024) void k_check(const double* part_rr, const dou
         scal[srr] = rr;
         if (!(rr == rr)) flags[0] = 2;
         else if (rr <= tol2 * bb || bb == 0.0) flags[0] = 1;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_check(const double* part_rr, const double* part_bb, double tol2,
+                                                int* flags, double* scal, int first, int sbb, int srr, int srr0) {
+    __shared__ double sh[16];
+    check_body(part_rr, part_bb, tol2, flags, scal, first, sbb, srr, srr0, sh);
+}
+
+// k_check, then the state block published to the host: copied into the
+// mapped host memory `mail` with the wall-clock stamps, a system-scope
+// release, then the sequence number the host waits for.
+__global__ __launch_bounds__(1024) void k_check_publish(const double* part_rr, const double* part_bb, double tol2,
+                                                        int* flags, double* scal, int first, int sbb, int srr,
+                                                        int srr0, SolveMail* mail, const unsigned long long* ts,
+                                                        unsigned seq) {
+    __shared__ double sh[16];
+    check_body(part_rr, part_bb, tol2, flags, scal, first, sbb, srr, srr0, sh);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < PIC_SCAL_DOUBLES; ++k) mail->scal[k] = scal[k];
+        for (int k = 0; k < 4; ++k) mail->flags[k] = flags[k];
+        mail->t0 = ts[0];
+        mail->t1 = wall_clock64();
+        __threadfence_system();
+        __hip_atomic_store(&mail->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -1412,8 +1442,7 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
         PIC_HIP(hipEventCreate(&c->solve_ev[0]));
         PIC_HIP(hipEventCreate(&c->solve_ev[1]));
     }
-    hipEvent_t t0 = c->solve_ev[0], t1 = c->solve_ev[1];
-    PIC_HIP(hipEventRecord(t0, s));
+    hipEvent_t t0 = c->solve_ev[0];
     double* part = c->part;
     const double tol2 = c->p.poisson_tol * c->p.poisson_tol;
     double* const R[2] = {c->pr, c->pr2};
@@ -1421,16 +1450,23 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
     float* const Z[2] = {c->zf0, c->zf1};
     const bool mixed = c->p.poisson_precision == 1;
     if (mixed) PIC_TRY(ensure_f32(c));
+    // the path, from host state only: the direct solve unless a form asks for the PCG
+    const bool learned = c->unet && !c->guess_set && iteration > 0 && c->have_prev_phi;
+    const bool direct = c->fast && !mixed && !learned && !c->guess_set;
+    // the direct solve is timed by device wall-clock stamps in its first and
+    // last kernels, the PCG paths by events
+    if (!direct) PIC_HIP(hipEventRecord(t0, s));
+    unsigned long long* ts = direct ? c->tstamp : nullptr;
 
     // flags cleared, overwriteRho, ||b||^2 (b: residual of the Dirichlet-only field)
     if (fin) {
         // the deposit's node pass, deferred to here, writes rho first
         hipLaunchKernelGGL(k_finalize_prep, dim3(kNP), dim3(kB), 0, s, c->fin, c->g.gp, c->p.deposit_energy, f,
-                           c->rho, c->dir, c->dval, part + P_BB * kNP, c->flags);
+                           c->rho, c->dir, c->dval, part + P_BB * kNP, c->flags, ts);
         c->meshes_stale = 1;
     } else {
         hipLaunchKernelGGL(k_prep_rhs, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->dir, c->dval, part + P_BB * kNP,
-                           c->flags);
+                           c->flags, ts);
     }
     // initial guess
     int guess = 0;
@@ -1452,7 +1488,6 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
         if (extrap) keep = c->phi_prev;
     }
     // learned guess: keep phi_{n-1} for training; try the prediction against it
-    const bool learned = c->unet && !c->guess_set && iteration > 0 && c->have_prev_phi;
     if (learned) {
         PIC_HIP(hipMemcpyAsync(c->phi_prev, c->phi, (size_t)G * sizeof(double), hipMemcpyDeviceToDevice, s));
         if (c->learn_trained >= c->learn_min) {
@@ -1473,7 +1508,6 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
     }
     // direct solve: the PCG below starts from it and only checks (or
     // corrects) the residual
-    const bool direct = c->fast && !mixed && !learned && !c->guess_set;
     bool efield_done = false;
     if (direct) {
         if (keep) PIC_HIP(hipMemcpyAsync(keep, c->phi, (size_t)G * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -1496,8 +1530,16 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
         hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, R[0],
                            part + P_RR * kNP);
     }
-    hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2, c->flags,
-                       c->scal, 1, S_BB, S_RR, S_RR0);
+    if (direct) {
+        // the check publishes its state to the host itself (no copy, no event)
+        c->mail_seq += 1;
+        hipLaunchKernelGGL(k_check_publish, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2,
+                           c->flags, c->scal, 1, S_BB, S_RR, S_RR0, c->mail_dev, (const unsigned long long*)c->tstamp,
+                           c->mail_seq);
+    } else {
+        hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_BB * kNP, tol2, c->flags,
+                           c->scal, 1, S_BB, S_RR, S_RR0);
+    }
     PIC_HIP(hipGetLastError());
     SolveTail tail;
     tail.direct = direct;
@@ -1507,10 +1549,8 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
     tail.guess = guess;
     tail.ev = ev;
     if (direct) {
-        // The check's flags go to the host while the device already writes E:
-        // the host waits for the flags only, not for the field kernel.
-        PIC_HIP(hipMemcpyAsync(c->scal_host, c->scal, PIC_STATE_BYTES, hipMemcpyDeviceToHost, s));
-        PIC_HIP(hipEventRecord(t1, s));
+        // The host waits for the check's publication only, not for the work
+        // enqueued after it.
         PIC_TRY(timer_end(c, PIC_STAGE_POISSON, ev));
         if (!efield_done) PIC_TRY(launch_efield(c));
         if (defer) {
@@ -1533,6 +1573,37 @@ int poisson_finish(pic_ctx* c, pic_solve_stats* st, int* redo) {
 
 // The rest of a solve after its first check: the wait for the check of a
 // direct solve, PCG iterations where needed, the statistics.
+// Wait for the check kernel's publication (sequence number c->mail_seq) and
+// take its state block; the device time of the solve up to the check from the
+// wall-clock stamps.  Should the stream finish without the publication being
+// visible, the state block is copied instead.
+static int wait_mail(pic_ctx* c, double* ms) {
+    const unsigned want = c->mail_seq;
+    // the stream is queried only after a while (a query on every spin would
+    // disturb the stream's own progress)
+    auto next_query = std::chrono::steady_clock::now() + std::chrono::milliseconds(20);
+    for (unsigned spin = 1;; ++spin) {
+        if (__atomic_load_n(&c->mail->seq, __ATOMIC_ACQUIRE) == want) break;
+        if ((spin & 255u) == 0 && std::chrono::steady_clock::now() >= next_query) {
+            next_query = std::chrono::steady_clock::now() + std::chrono::milliseconds(20);
+            const hipError_t q = hipStreamQuery(c->stream);
+            if (q != hipSuccess && q != hipErrorNotReady) {
+                set_error("Poisson check: %s", hipGetErrorString(q));
+                return PIC_ERR_HIP;
+            }
+            if (q == hipSuccess && __atomic_load_n(&c->mail->seq, __ATOMIC_ACQUIRE) != want) {
+                PIC_HIP(hipMemcpy(c->scal_host, c->scal, PIC_STATE_BYTES, hipMemcpyDeviceToHost));
+                *ms = 0.0;
+                return PIC_OK;
+            }
+        }
+    }
+    std::memcpy(c->scal_host, c->mail->scal, sizeof(c->mail->scal));
+    std::memcpy(c->flags_host, c->mail->flags, sizeof(c->mail->flags));
+    *ms = (double)(c->mail->t1 - c->mail->t0) / c->wall_khz;
+    return PIC_OK;
+}
+
 static int solve_finish(pic_ctx* c, const SolveTail& t, pic_solve_stats* st, int* redo) {
     const Fine f = fine_of(c);
     const int G = c->G;
@@ -1547,12 +1618,14 @@ static int solve_finish(pic_ctx* c, const SolveTail& t, pic_solve_stats* st, int
     const bool mixed = t.mixed, learned = t.learned;
     const int guess = t.guess;
     bool fields_done = false;
+    double direct_ms = 0.0;
     if (t.direct) {
-        PIC_HIP(hipEventSynchronize(t1));
+        PIC_TRY(wait_mail(c, &direct_ms));
         fields_done = c->flags_host[0] == 1;
         if (!fields_done) {
             if (redo) *redo = 1;
             PIC_TRY(timer_begin(c, PIC_STAGE_POISSON, &ev));
+            PIC_HIP(hipEventRecord(t0, s));    // the correcting PCG's time is added
         }
     }
     int iters = 0, inner_total = 0;
@@ -1609,13 +1682,15 @@ static int solve_finish(pic_ctx* c, const SolveTail& t, pic_solve_stats* st, int
                   iters, rel);
         return PIC_ERR_SOLVER;
     }
+    double ms = direct_ms;
     if (!fields_done) {
         PIC_TRY(launch_overwrite_phi(c));
         PIC_HIP(hipEventRecord(t1, s));
         PIC_HIP(hipEventSynchronize(t1));
+        float e = 0;
+        PIC_HIP(hipEventElapsedTime(&e, t0, t1));
+        ms += e;
     }
-    float ms = 0;
-    PIC_HIP(hipEventElapsedTime(&ms, t0, t1));
     c->guess_set = 0;
     c->have_prev_phi = t.extrap_keep ? 2 : 1;
     if (learned && c->learn_trained < c->learn_train) {
