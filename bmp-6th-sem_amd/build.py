@@ -3,10 +3,11 @@
 Usage: python bmp-6th-sem_amd/build.py [--jobs N]
 
 Every source under csrc/ is compiled for --offload-arch=gfx950 and linked into
-bmp-6th-sem_amd/lib/libbmp_pic.so (with rocBLAS for the FP64 transform GEMMs
-of the direct Poisson solve).  Packed FP32 instructions are disabled for
-all device code; the particle kernels are also built with -ffp-contract=off so
-their FP64 expressions round exactly as written.
+bmp-6th-sem_amd/lib/libbmp_pic.so, which links the HIP runtime only (the
+direct Poisson solve's FP64 transforms run on the library's own FFT and GEMM
+kernels).  Packed FP32 instructions are disabled for all device code; the
+particle kernels are also built with -ffp-contract=off so their FP64
+expressions round exactly as written.
 """
 from __future__ import annotations
 
