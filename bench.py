@@ -90,13 +90,14 @@ def unet_only(n, reps=5):
 
 def unet_flops(n):
     """Multiply-add FLOPs of one forward pass of the learned-guess network on an
-    n x n image (padded input channels counted: that is the MFMA work done)."""
+    n x n image (the first layer's 2 input channels, not its 16-channel
+    padding: the forward pass computes that layer with 2)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import unet_oracle as U
     f = 0
-    for (src, _, cin, _, _, _, cout) in U.CONVS:
+    for (src, _, _, cin_real, _, _, cout) in U.CONVS:
         lvl = U.TENSORS[src][0]
-        f += 2 * 9 * cin * cout * (n >> lvl) ** 2
+        f += 2 * 9 * cin_real * cout * (n >> lvl) ** 2
     return f + 2 * 16 * n * n
 
 

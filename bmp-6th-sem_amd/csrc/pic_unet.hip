@@ -210,6 +210,293 @@ __global__ __launch_bounds__(kT) void k_conv3x3_lds(const bf16* __restrict__ X, 
     }
 }
 
+// ------------------------------------------------ forward convolution (x32)
+// The forward pass's 3x3 convolutions on v_mfma_f32_16x16x32_bf16 with the
+// roles of the operands chosen for the epilogue: A = weights (M = 16 output
+// channels), B = input (N = 16 pixels: 2 rows x 8 columns), K = 32 = two taps
+// x 16 input channels (taps (0,1), (2,3), (4,5), (6,7), (8, none): five K
+// steps per 16 channels).  Lane (i, q) (i = lane % 16, q = lane / 16) holds
+// the 8 K values 8q .. 8q + 7 of its A row and B column, i.e. tap 2j + q/2 and
+// channels 8(q%2) .. 8(q%2) + 7 of K step j, and receives D[4q + e][i]: four
+// consecutive output channels of one pixel, stored as one 8-byte word.
+//
+// A workgroup (4 waves) computes a tile of 4 rows x 64 pixels for NT*16
+// output channels; wave w takes rows 2(w/2), 2(w/2) + 1 and columns
+// 32(w%2) .. + 31 as four pixel tiles.  Per fill of up to 32 input channels
+// the padded input tile (6 rows x 66 pixels) goes to LDS as 16-byte channel
+// groups in [row][group][pixel] order with a row stride of 8 (mod 16) slots,
+// so the 16 lanes of a quarter-wave (8 pixels of one row, 8 of the next) read
+// 16 distinct slots of every bank set; the weights go to LDS in the lane order
+// of the A operand.  Epilogue: bias, ReLU, BF16 rounding, then any of: the
+// output (8 bytes per lane), the 2x2 average pool of the rounded values
+// (lanes i, i^1, i^8, i^9 hold a pooling window; (a + b) + (c + d) in FP32,
+// a, b the upper pair), and the 1x1 head (cout = 16: the pixel's 16 channels gathered in
+// one lane and summed in channel order onto the head bias).  The input may be the
+// concatenation of a 2x nearest upsampling of U (first cu channels) and X, read
+// in place (no upsampled tensor).
+typedef int i4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+static constexpr int kFTH = 4;    // tile rows
+static constexpr int kFPX = 66;   // tile columns with the halo
+static constexpr int kFCK = 32;   // input channels per LDS fill
+
+struct ConvFwdArgs {
+    const bf16* X;                 // input (padded NHWC); with U: the channels after the first cu
+    int xC, xo;
+    const bf16* U;                 // null, or the (H/2 x W/2) tensor whose upsampling gives channels 0 .. cu-1
+    int uC, uo, cu;
+    int cin;                       // input channels (multiple of 16)
+    const bf16* Wt;                // [9][coutT][cin]
+    const float* bias;
+    int coutT;
+    bf16* Y;                       // output (padded NHWC), or null
+    int yC, yo;
+    bf16* P;                       // 2x2 average-pooled output (H/2 x W/2, padded NHWC), or null
+    int pC, po;
+    const float* wh;               // 1x1 head (16 weights, bias bh) into out[H][W] (FP32), or out null
+    const float* bh;
+    float* out;
+    int H, W;
+};
+
+// LDS slots (16 bytes) per tile row for g channel groups: g * 66 rounded up to 8 (mod 16)
+__host__ __device__ constexpr int fwd_row_stride(int g) { return g * kFPX + ((8 - (g * kFPX) % 16) + 16) % 16; }
+
+__device__ __forceinline__ int fwd_tile(int H, int W, int& y0, int& x0) {
+    const int nxt = (W + 63) >> 6;
+    const int ntile = (H / kFTH) * nxt;
+    int tile = (int)blockIdx.x;
+    // tiles that share halo rows go to one XCD (workgroups are dealt to the 8 XCDs in turn)
+    if ((ntile & 7) == 0) tile = (tile & 7) * (ntile >> 3) + (tile >> 3);
+    y0 = (tile / nxt) * kFTH;
+    x0 = (tile % nxt) * 64;
+    return tile;
+}
+
+// hardware round-to-nearest-even conversion (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ bf16 f2bf_hw(float f) { return __builtin_bit_cast(bf16, (__bf16)f); }
+// lane l receives v of lane l^1 (quad_perm [1,0,3,2]) / of lane l^8 (rotate a row of 16 by 8)
+__device__ __forceinline__ float dpp_xor1(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_xor8(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x128, 0xF, 0xF, false));
+}
+
+// Epilogue of one pixel tile m and channel tile t: v = the FP32 sums of lane
+// (i, q) for channels co .. co+3 of pixel (y, x).  EPI 0: output only; 1:
+// output and pooled output; 2: head (and the output when a.Y is set).
+template <int EPI>
+__device__ __forceinline__ void fwd_epilogue(const ConvFwdArgs& a, f4v acc, int co, int y, int x, int lane) {
+    const float4 bb = *(const float4*)(a.bias + co);
+    float v[4] = {acc[0] + bb.x, acc[1] + bb.y, acc[2] + bb.z, acc[3] + bb.w};
+    uint2 packed;
+    bf16* h = (bf16*)&packed;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        h[e] = f2bf_hw(fmaxf(v[e], 0.f));
+        v[e] = bf2f(h[e]);
+    }
+    const bool in = x < a.W;
+    const int Wp = a.W + 2;
+    if ((EPI != 2 || a.Y) && in) *(uint2*)(a.Y + ((size_t)(y + 1) * Wp + x + 1) * a.yC + a.yo + co) = packed;
+    if (EPI == 1) {
+        uint2 pk;
+        bf16* ph = (bf16*)&pk;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float r = v[e] + dpp_xor1(v[e]);
+            const float s = r + dpp_xor8(r);
+            ph[e] = f2bf_hw(0.25f * s);
+        }
+        if (in && (lane & 9) == 0)
+            *(uint2*)(a.P + ((size_t)(y / 2 + 1) * (a.W / 2 + 2) + x / 2 + 1) * a.pC + a.po + co) = pk;
+    }
+    if (EPI == 2) {
+        // head: each lane's four channels, then the sum over the four lanes of the pixel
+        // (lanes i, i+16, i+32, i+48) as ((q0 + q1) + (q2 + q3)) plus the bias
+        float s = a.wh[co] * v[0];
+#pragma unroll
+        for (int e = 1; e < 4; ++e) s += a.wh[co + e] * v[e];
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+        if (in && lane < 16) a.out[(size_t)y * a.W + x] = s + a.bh[0];
+    }
+}
+
+// LDS slot offset of tap t relative to the centre tap (row stride rs)
+__host__ __device__ constexpr int tap_off(int t, int rs) { return t > 8 ? 0 : (t / 3 - 1) * rs + (t % 3 - 1); }
+
+// CK: input channels per LDS fill (16 or 32); cin must be a multiple of CK.
+template <int CK, int NT, int EPI>
+__global__ __launch_bounds__(256, CK == 16 ? 8 : 4) void k_conv_fwd(ConvFwdArgs a) {
+    constexpr int NG = CK / 8;                 // 16-byte channel groups per fill
+    constexpr int RS = fwd_row_stride(NG);
+    constexpr int NW = (CK / 16) * 5 * NT * 64;
+    __shared__ i4v xs[(kFTH + 2) * RS];
+    __shared__ i4v ws[NW];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = lane & 15, q = lane >> 4;
+    int y0, x0;
+    fwd_tile(a.H, a.W, y0, x0);
+    const int cob = (int)blockIdx.y * NT * 16;
+    const int Wp = a.W + 2;
+    const int lrow = 2 * (wv >> 1) + (i >> 3), lcol = 32 * (wv & 1) + (i & 7);
+    const int tq = q >> 1;               // which tap of the pair
+    const int lbase = (lrow + 1) * RS + (q & 1) * kFPX + lcol + 1;
+    f4v acc[4][NT];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][t] = (f4v){0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < a.cin; c0 += CK) {
+        __syncthreads();
+        // padded rows y0 .. y0+5, padded columns x0 .. x0+65 of channels c0 .. c0+CK-1: the tile's
+        // own columns x0+1 .. x0+64 (thread: column x0+1+p, one (row, group) per wave and pass), then
+        // the two halo columns
+        {
+            const int p = threadIdx.x & 63;
+            const bool colin = x0 + p < a.W;          // an image column
+            const bool colpad = x0 + p <= a.W;        // an image column or the right halo column
+            const bf16* xb = a.X + (size_t)(x0 + 1 + p) * a.xC + a.xo + c0 - a.cu;
+            const bf16* ub = a.U ? a.U + (size_t)(((x0 + p) >> 1) + 1) * a.uC + a.uo + c0 : nullptr;
+#pragma unroll
+            for (int k = 0; k < (kFTH + 2) * NG / 4; ++k) {
+                const int rg = 4 * k + (int)(threadIdx.x >> 6);
+                const int r = rg / NG, g = rg % NG;
+                const int Yp = y0 + r;
+                i4v v = {0, 0, 0, 0};
+                if (c0 + 8 * g < a.cu) {
+                    if (colin && Yp >= 1 && Yp <= a.H)
+                        v = *(const i4v*)(ub + (size_t)(((Yp - 1) >> 1) + 1) * ((a.W >> 1) + 2) * a.uC + 8 * g);
+                } else if (colpad) {
+                    v = *(const i4v*)(xb + (size_t)Yp * Wp * a.xC + 8 * g);
+                }
+                xs[r * RS + g * kFPX + 1 + p] = v;
+            }
+            if (threadIdx.x < (kFTH + 2) * NG * 2) {
+                const int h = threadIdx.x & 1, rg = threadIdx.x >> 1;
+                const int r = rg / NG, g = rg % NG;
+                const int Yp = y0 + r, Xp = x0 + 65 * h;
+                const int ch = c0 + 8 * g;
+                i4v v = {0, 0, 0, 0};
+                if (Xp <= a.W + 1) {
+                    if (ch < a.cu) {
+                        if (Yp >= 1 && Yp <= a.H && Xp >= 1 && Xp <= a.W)
+                            v = *(const i4v*)(a.U + ((size_t)(((Yp - 1) >> 1) + 1) * ((a.W >> 1) + 2) +
+                                                     ((Xp - 1) >> 1) + 1) * a.uC + a.uo + ch);
+                    } else {
+                        v = *(const i4v*)(a.X + ((size_t)Yp * Wp + Xp) * a.xC + a.xo + ch - a.cu);
+                    }
+                }
+                xs[r * RS + g * kFPX + 65 * h] = v;
+            }
+        }
+#pragma unroll
+        for (int e0 = 0; e0 < NW; e0 += 256) {
+            const int e = e0 + (int)threadIdx.x;
+            if (NW % 256 == 0 || e < NW) {
+                const int l = e & 63;
+                const int t = (e >> 6) % NT, j = (e >> 6) / NT % 5, s = (e >> 6) / NT / 5;
+                const int tap = 2 * j + ((l >> 4) >> 1);
+                i4v v = {0, 0, 0, 0};
+                if (tap < 9)
+                    v = *(const i4v*)(a.Wt + ((size_t)tap * a.coutT + cob + 16 * t + (l & 15)) * a.cin + c0 + 16 * s +
+                                      8 * ((l >> 4) & 1));
+                ws[e] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < CK / 16; ++s) {
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                const int base = lbase + 2 * s * kFPX + (tq ? tap_off(2 * j + 1, RS) : tap_off(2 * j, RS));
+                bf8v wa[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) wa[t] = __builtin_bit_cast(bf8v, ws[((s * 5 + j) * NT + t) * 64 + lane]);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    i4v xv = xs[base + 8 * m];
+                    if (j == 4 && tq) xv = (i4v){0, 0, 0, 0};   // the pair's missing tenth tap
+                    const bf8v xb = __builtin_bit_cast(bf8v, xv);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[t], xb, acc[m][t], 0, 0, 0);
+                }
+            }
+        }
+    }
+    const int y = y0 + lrow;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) fwd_epilogue<EPI>(a, acc[m][t], cob + 16 * t + 4 * q, y, x0 + lcol + 8 * m, lane);
+}
+
+// The first layer: 2 input channels (X = the compact input, 2 channels per
+// pixel, padded), 16 output channels; K = 9 taps x 2 channels in one K step
+// of 32 (lane (i, q) holds taps 4q .. 4q+3, both channels; K 18 .. 31 zero).
+// Weights from Wt [9][16][cin] (channels 0 and 1 used).
+__global__ __launch_bounds__(256) void k_conv0_fwd(ConvFwdArgs a) {
+    constexpr int RS0 = 72;   // 32-bit words per tile row (== 8 mod 64: two rows of 8 pixels hit distinct banks)
+    __shared__ uint32_t xs[(kFTH + 2) * RS0];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = lane & 15, q = lane >> 4;
+    // A: W[co = i][k = 8q + e] = Wt[tap = 4q + e/2][i][e % 2], two channels per 32-bit load
+    uint32_t w4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int tap = 4 * q + k;
+        w4[k] = tap < 9 ? *(const uint32_t*)(a.Wt + ((size_t)tap * a.coutT + i) * a.cin) : 0u;
+    }
+    int y0, x0;
+    fwd_tile(a.H, a.W, y0, x0);
+    const int Wp = a.W + 2;
+    for (int e = threadIdx.x; e < (kFTH + 2) * kFPX; e += 256) {
+        const int r = e / kFPX, p = e - r * kFPX;
+        const int Xp = x0 + p;
+        xs[r * RS0 + p] = Xp <= a.W + 1 ? *(const uint32_t*)(a.X + ((size_t)(y0 + r) * Wp + Xp) * 2) : 0u;
+    }
+    const bf8v wa = __builtin_bit_cast(bf8v, *(const i4v*)w4);
+    __syncthreads();
+    const int lrow = 2 * (wv >> 1) + (i >> 3), lcol = 32 * (wv & 1) + (i & 7);
+    const int y = y0 + lrow;
+    const int lbase = (lrow + 1) * RS0 + lcol + 1;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        uint32_t x4[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            // tap 4q + k: q = 0: taps 0..3, q = 1: 4..7, q = 2: 8 and zeros, q = 3: zeros
+            const int off = q == 0 ? tap_off(k, RS0) : q == 1 ? tap_off(4 + k, RS0) : tap_off(8, RS0);
+            const uint32_t v = xs[lbase + off + 8 * m];
+            x4[k] = (q < 2 || (q == 2 && k == 0)) ? v : 0u;
+        }
+        const bf8v xb = __builtin_bit_cast(bf8v, *(const i4v*)x4);
+        f4v acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, xb, acc, 0, 0, 0);
+        fwd_epilogue<0>(a, acc, 4 * q, y, x0 + lcol + 8 * m, lane);
+    }
+}
+
+// 16x16x32 BF16 MFMA lane check: A (16 x 32) and B (32 x 16) row-major; lane
+// l holds A[l%16][8(l/16)+e] and B[8(l/16)+e][l%16] and receives
+// D[4(l/16)+e][l%16].
+__global__ void k_mfma32_probe(const float* a, const float* b, float* d) {
+    const int l = threadIdx.x, i = l & 15, q = l >> 4;
+    uint16_t av[8], bv[8];
+    for (int e = 0; e < 8; ++e) {
+        av[e] = f2bf(a[i * 32 + 8 * q + e]);
+        bv[e] = f2bf(b[(8 * q + e) * 16 + i]);
+    }
+    f4v acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, *(const i4v*)av),
+                                                  __builtin_bit_cast(bf8v, *(const i4v*)bv), acc, 0, 0, 0);
+    for (int e = 0; e < 4; ++e) d[(4 * q + e) * 16 + i] = acc[e];
+}
+
 // Partial of dW[tap][co][ci] = sum_{y,x} dZT[co][y][x] XT[ci][y+1+dy][x+1+dx]
 // over one chunk of rows.  dZT: [cout][H][W] (unpadded), XT: [cin][H+2][W+2]
 // (padded).  One wave: one tap, one 16-channel co tile, CT ci tiles, rows
@@ -345,29 +632,6 @@ __global__ __launch_bounds__(kT) void k_to_chw(const bf16* __restrict__ in, int 
     }
 }
 
-// 2x2 average pooling: out (H/2 x W/2) <- in (H x W), c channels.
-// Eight channels (16 bytes) per thread; c, io and oo are multiples of 8.
-__global__ void k_pool2(const bf16* __restrict__ in, int iC, int io, bf16* __restrict__ out, int oC, int oo, int c,
-                        int H, int W) {
-    const int h = H / 2, w = W / 2, c8 = c / 8;
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= (int64_t)h * w * c8) return;
-    const int ch = (int)(n % c8) * 8;
-    const int64_t p = n / c8;
-    const int y = (int)(p / w), x = (int)(p - (int64_t)y * w);
-    auto at = [&](int yy, int xx) { return *(const int4*)(in + ((size_t)(yy + 1) * (W + 2) + xx + 1) * iC + io + ch); };
-    const int4 a = at(2 * y, 2 * x), b = at(2 * y, 2 * x + 1), c2 = at(2 * y + 1, 2 * x), d = at(2 * y + 1, 2 * x + 1);
-    const bf16 *pa = (const bf16*)&a, *pb = (const bf16*)&b, *pc = (const bf16*)&c2, *pd = (const bf16*)&d;
-    int4 o;
-    bf16* po = (bf16*)&o;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const float s = (bf2f(pa[k]) + bf2f(pb[k])) + (bf2f(pc[k]) + bf2f(pd[k]));
-        po[k] = f2bf(0.25f * s);
-    }
-    *(int4*)(out + ((size_t)(y + 1) * (w + 2) + x + 1) * oC + oo + ch) = o;
-}
-
 // gin (H x W) += 0.25 gout (H/2 x W/2) at each of the four children.
 __global__ void k_pool2_bwd(const float* __restrict__ gout, int oC, int oo, float* __restrict__ gin, int iC, int io,
                             int c, int H, int W) {
@@ -407,23 +671,6 @@ __global__ void k_up2_bwd(const float* __restrict__ gout, int oC, int oo, float*
     auto at = [&](int yy, int xx) { return gout[((size_t)(yy + 1) * (W + 2) + xx + 1) * oC + oo + ch]; };
     gin[((size_t)(y + 1) * (w + 2) + x + 1) * iC + io + ch] =
         (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1)) + (at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1));
-}
-
-// out[y][x] = bh + sum_c wh[c] d0[y][x][c]  (FP32)
-__global__ void k_head(const bf16* __restrict__ d0, int c, const float* __restrict__ wh, const float* __restrict__ bh,
-                       float* __restrict__ out, int H, int W) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= H * W) return;
-    const int y = p / W, x = p - y * W;
-    const int4* v4 = (const int4*)(d0 + ((size_t)(y + 1) * (W + 2) + x + 1) * c);
-    float s = bh[0];
-    for (int k8 = 0; k8 < c / 8; ++k8) {
-        const int4 q = v4[k8];
-        const bf16* v = (const bf16*)&q;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) s += wh[8 * k8 + k] * bf2f(v[k]);
-    }
-    out[p] = s;
 }
 
 // Output gradient of the masked mean squared error against `tgt`; the block's
@@ -697,15 +944,23 @@ __device__ __forceinline__ double scl_get(const unsigned long long* scl, int k) 
     return v > 0.0 ? v : 1.0;
 }
 
+// x0c: the compact 2-channel input (padded); x0 (training only, else null):
+// the same two values in channels 0, 1 of the 16-channel input tensor read by
+// the first layer's weight gradient.
 __global__ void k_unet_input(const double* __restrict__ r0, const double* __restrict__ phi, int gx, int N,
-                             const unsigned long long* __restrict__ scl, bf16* __restrict__ x0) {
+                             const unsigned long long* __restrict__ scl, bf16* __restrict__ x0c,
+                             bf16* __restrict__ x0) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N * N) return;
     const int y = p / N, x = p - y * N;
     const double sr = 1.0 / scl_get(scl, 0), sp = 1.0 / scl_get(scl, 1);
-    bf16* d = x0 + ((size_t)(y + 1) * (N + 2) + x + 1) * 16;
-    d[0] = f2bf((float)(r0[p] * sr));
-    d[1] = f2bf((float)(phi[y * gx + x] * sp));
+    const size_t px = (size_t)(y + 1) * (N + 2) + x + 1;
+    const bf16 a = f2bf((float)(r0[p] * sr)), b = f2bf((float)(phi[y * gx + x] * sp));
+    *(uint32_t*)(x0c + px * 2) = (uint32_t)a | ((uint32_t)b << 16);
+    if (x0) {
+        x0[px * 16] = a;
+        x0[px * 16 + 1] = b;
+    }
 }
 
 // target (phi_new - phi_old) diag / max|r0| on the image; mask = not Dirichlet
@@ -741,6 +996,7 @@ struct Tensor {
     bf16* p = nullptr;   // padded NHWC
     int H = 0, W = 0, C = 0;
     float* g = nullptr;  // gradient (same layout, FP32), when needed
+    bool grad = true;    // false: forward-only tensor, no gradient buffer
 };
 
 struct Conv {
@@ -801,11 +1057,12 @@ static int unet_fail(pic_unet* u, const char* what) {
 // layer table, built for an N x N image (N a multiple of 64)
 static int unet_build(pic_unet* u) {
     const int N = u->N;
-    auto add = [&](int H, int C) {
+    auto add = [&](int H, int C, bool grad = true) {
         Tensor x;
         x.H = H;
         x.W = H;
         x.C = C;
+        x.grad = grad;
         u->t.push_back(x);
         return (int)u->t.size() - 1;
     };
@@ -816,6 +1073,11 @@ static int unet_build(pic_unet* u) {
     const int A4 = add(N / 4, 64), D2 = add(N / 4, 64);
     const int A5 = add(N / 2, 32), D1 = add(N / 2, 32);
     const int A6 = add(N, 16), D0 = add(N, 16);
+    // inference only: compact skip outputs and the 2-channel input of the first layer
+    add(N, 16, false);
+    add(N / 2, 32, false);
+    add(N / 4, 64, false);
+    add(N, 2, false);
     size_t off = 0;
     auto conv = [&](int in, int io, int cin, int cin_real, int out, int oo, int cout) {
         Conv c{in, io, cin, cin_real, out, oo, cout, 0, 0};
@@ -851,7 +1113,7 @@ static int unet_build(pic_unet* u) {
 }
 
 // tensor indices used by the fixed schedule below
-enum { X0 = 0, A0, CAT0, P0, A1, CAT1, P1, A2, CAT2, P2, A3, B3, A4, D2, A5, D1, A6, D0 };
+enum { X0 = 0, A0, CAT0, P0, A1, CAT1, P1, A2, CAT2, P2, A3, B3, A4, D2, A5, D1, A6, D0, S0, S1, S2, X0C };
 
 template <int NT, int RELU, typename TO>
 static void conv_launch(hipStream_t s, const bf16* X, int xC, int xo, int cin, const bf16* Wt, const float* bias,
@@ -916,53 +1178,125 @@ static int wgrad_any(hipStream_t s, int ct, const bf16* dZT, const bf16* XT, int
 static size_t tsize(const Tensor& x) { return (size_t)(x.H + 2) * (x.W + 2) * x.C; }
 static int bias_parts(int H, int W) { return std::min(512, (H * W + 4095) / 4096); }
 
-static int resample(pic_unet* u, bool up, int in, int io, int out, int oo, int c) {
+// nearest 2x upsampling of `in` (channels io ..) into `out` (channels oo ..)
+static int upsample(pic_unet* u, int in, int io, int out, int oo, int c) {
     const Tensor& a = u->t[in];
     const Tensor& b = u->t[out];
-    if (!up) {
-        const int64_t n = (int64_t)b.H * b.W * (c / 8);
-        hipLaunchKernelGGL(k_pool2, dim3(nb(n)), dim3(kT), 0, u->stream, a.p, a.C, io, b.p, b.C, oo, c, a.H, a.W);
+    const int64_t n = (int64_t)b.H * b.W * (c / 8);
+    hipLaunchKernelGGL(k_up2, dim3(nb(n)), dim3(kT), 0, u->stream, a.p, a.C, io, b.p, b.C, oo, c, a.H, a.W);
+    PIC_HIP(hipGetLastError());
+    return PIC_OK;
+}
+
+template <int NT, int EPI>
+static void conv_fwd_nt(hipStream_t s, const ConvFwdArgs& a, unsigned tiles) {
+    const dim3 grid(tiles, (unsigned)(a.coutT / (16 * NT)));
+    // launches of at most 4 workgroups per CU: 32 channels per LDS fill (half the waits for
+    // global memory); larger ones: 16 (less LDS and fewer registers: 8 waves per SIMD)
+    if (grid.x * grid.y <= 1024 && a.cin % 32 == 0)
+        hipLaunchKernelGGL((k_conv_fwd<32, NT, EPI>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_conv_fwd<16, NT, EPI>), grid, dim3(256), 0, s, a);
+}
+
+// Forward convolution k.  Training keeps every activation the backward pass
+// reads (skip outputs inside the concatenated tensors, the upsampled halves,
+// the last layer's output); inference writes the skip outputs to compact
+// tensors, reads the upsampled halves in place and keeps only the head output.
+static int conv_fwd(pic_unet* u, int k, bool train) {
+    const Conv& c = u->conv[k];
+    const Tensor& in = u->t[c.in];
+    const Tensor& out = u->t[c.out];
+    ConvFwdArgs a{};
+    a.X = in.p;
+    a.xC = in.C;
+    a.xo = c.io;
+    a.cin = c.cin;
+    a.Wt = u->wb + c.woff;
+    a.bias = u->w + c.boff;
+    a.coutT = c.cout;
+    a.Y = out.p;
+    a.yC = out.C;
+    a.yo = c.oo;
+    a.H = in.H;
+    a.W = in.W;
+    auto skip_out = [&](int S, int P) {
+        if (!train) {
+            a.Y = u->t[S].p;
+            a.yC = u->t[S].C;
+            a.yo = 0;
+        }
+        a.P = u->t[P].p;
+        a.pC = u->t[P].C;
+        a.po = 0;
+    };
+    auto up_in = [&](int U, int S, int cu) {
+        if (train) return;
+        a.U = u->t[U].p;
+        a.uC = u->t[U].C;
+        a.uo = 0;
+        a.cu = cu;
+        a.X = u->t[S].p;
+        a.xC = u->t[S].C;
+        a.xo = 0;
+    };
+    switch (k) {
+    case 1: skip_out(S0, P0); break;
+    case 3: skip_out(S1, P1); break;
+    case 5: skip_out(S2, P2); break;
+    case 8: up_in(B3, S2, 64); break;
+    case 10: up_in(D2, S1, 64); break;
+    case 12: up_in(D1, S0, 32); break;
+    case 13:
+        a.wh = u->w + u->head_w;
+        a.bh = u->w + u->head_b;
+        a.out = u->out;
+        if (!train) a.Y = nullptr;
+        break;
+    default: break;
+    }
+    const unsigned tiles = (unsigned)((a.H / kFTH) * ((a.W + 63) / 64));
+    if (k == 0) {
+        a.X = u->t[X0C].p;
+        a.xC = 2;
+        a.xo = 0;
+        hipLaunchKernelGGL(k_conv0_fwd, dim3(tiles), dim3(256), 0, u->stream, a);
     } else {
-        const int64_t n = (int64_t)b.H * b.W * (c / 8);
-        hipLaunchKernelGGL(k_up2, dim3(nb(n)), dim3(kT), 0, u->stream, a.p, a.C, io, b.p, b.C, oo, c, a.H, a.W);
+        const int nt = tiles >= 1024 ? c.cout / 16 : 1;
+        if (a.out) {
+            if (c.cout != 16) {
+                set_error("pic_unet: the head needs 16 channels (got %d)", c.cout);
+                return PIC_ERR_UNSUPPORTED;
+            }
+            conv_fwd_nt<1, 2>(u->stream, a, tiles);
+        } else if (a.P) {
+            switch (nt) {
+            case 1: conv_fwd_nt<1, 1>(u->stream, a, tiles); break;
+            case 2: conv_fwd_nt<2, 1>(u->stream, a, tiles); break;
+            case 4: conv_fwd_nt<4, 1>(u->stream, a, tiles); break;
+            default: set_error("pic_unet: unsupported channel count %d", c.cout); return PIC_ERR_UNSUPPORTED;
+            }
+        } else {
+            switch (nt) {
+            case 1: conv_fwd_nt<1, 0>(u->stream, a, tiles); break;
+            case 2: conv_fwd_nt<2, 0>(u->stream, a, tiles); break;
+            case 4: conv_fwd_nt<4, 0>(u->stream, a, tiles); break;
+            default: set_error("pic_unet: unsupported channel count %d", c.cout); return PIC_ERR_UNSUPPORTED;
+            }
+        }
     }
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
 
-static int conv_fwd(pic_unet* u, int k) {
-    const Conv& c = u->conv[k];
-    const Tensor& a = u->t[c.in];
-    Tensor& b = u->t[c.out];
-    return conv_any<1, bf16>(u->stream, c.cout / 16, a.p, a.C, c.io, c.cin, u->wb + c.woff, u->w + c.boff, b.p, b.C,
-                             c.oo, a.H, a.W);
-}
-
-static int unet_forward(pic_unet* u) {
-    PIC_TRY(conv_fwd(u, 0));
-    PIC_TRY(conv_fwd(u, 1));
-    PIC_TRY(resample(u, false, CAT0, 32, P0, 0, 16));
-    PIC_TRY(conv_fwd(u, 2));
-    PIC_TRY(conv_fwd(u, 3));
-    PIC_TRY(resample(u, false, CAT1, 64, P1, 0, 32));
-    PIC_TRY(conv_fwd(u, 4));
-    PIC_TRY(conv_fwd(u, 5));
-    PIC_TRY(resample(u, false, CAT2, 64, P2, 0, 64));
-    PIC_TRY(conv_fwd(u, 6));
-    PIC_TRY(conv_fwd(u, 7));
-    PIC_TRY(resample(u, true, B3, 0, CAT2, 0, 64));
-    PIC_TRY(conv_fwd(u, 8));
-    PIC_TRY(conv_fwd(u, 9));
-    PIC_TRY(resample(u, true, D2, 0, CAT1, 0, 64));
-    PIC_TRY(conv_fwd(u, 10));
-    PIC_TRY(conv_fwd(u, 11));
-    PIC_TRY(resample(u, true, D1, 0, CAT0, 0, 32));
-    PIC_TRY(conv_fwd(u, 12));
-    PIC_TRY(conv_fwd(u, 13));
-    const int N = u->N;
-    hipLaunchKernelGGL(k_head, dim3(nb((int64_t)N * N)), dim3(kT), 0, u->stream, u->t[D0].p, 16, u->w + u->head_w,
-                       u->w + u->head_b, u->out, N, N);
-    PIC_HIP(hipGetLastError());
+// The forward pass from the compact input (X0C) to u->out.
+static int unet_forward(pic_unet* u, bool train) {
+    for (int k = 0; k < 14; ++k) {
+        if (train && k == 8) PIC_TRY(upsample(u, B3, 0, CAT2, 0, 64));
+        if (train && k == 10) PIC_TRY(upsample(u, D2, 0, CAT1, 0, 64));
+        if (train && k == 12) PIC_TRY(upsample(u, D1, 0, CAT0, 0, 32));
+        PIC_TRY(conv_fwd(u, k, train));
+    }
     return PIC_OK;
 }
 
@@ -1106,6 +1440,31 @@ static double gauss(uint64_t& s) {
     return std::sqrt(-2.0 * std::log(a)) * std::cos(6.283185307179586 * b);
 }
 
+// Upload host inputs [N][N][2] into the compact input and, for training, the
+// 16-channel input; run the forward pass; copy the output [N][N] to `out`.
+static int unet_forward_host(pic_unet* u, const float* x, float* out, bool train) {
+    const int N = u->N;
+    std::vector<bf16> hc((size_t)(N + 2) * (N + 2) * 2, 0);
+    for (int y = 0; y < N; ++y)
+        for (int xx = 0; xx < N; ++xx)
+            for (int c = 0; c < 2; ++c)
+                hc[((size_t)(y + 1) * (N + 2) + xx + 1) * 2 + c] = f2bf_host(x[((size_t)y * N + xx) * 2 + c]);
+    PIC_HIP(hipMemcpyAsync(u->t[X0C].p, hc.data(), hc.size() * 2, hipMemcpyHostToDevice, u->stream));
+    std::vector<bf16> h;
+    if (train) {
+        h.assign((size_t)(N + 2) * (N + 2) * 16, 0);
+        for (size_t p = 0; p < (size_t)(N + 2) * (N + 2); ++p) {
+            h[p * 16] = hc[p * 2];
+            h[p * 16 + 1] = hc[p * 2 + 1];
+        }
+        PIC_HIP(hipMemcpyAsync(u->t[X0].p, h.data(), h.size() * 2, hipMemcpyHostToDevice, u->stream));
+    }
+    PIC_TRY(unet_forward(u, train));
+    PIC_HIP(hipMemcpyAsync(out, u->out, (size_t)N * N * 4, hipMemcpyDeviceToHost, u->stream));
+    PIC_HIP(hipStreamSynchronize(u->stream));
+    return PIC_OK;
+}
+
 } // namespace pic
 
 using namespace pic;
@@ -1124,6 +1483,26 @@ int pic_unet_mfma_probe(const float* a, const float* b, float* d) {
     PIC_HIP(hipMemcpy(da, a, 256 * sizeof(float), hipMemcpyHostToDevice));
     PIC_HIP(hipMemcpy(db, b, 256 * sizeof(float), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, da, db, dd);
+    PIC_HIP(hipGetLastError());
+    PIC_HIP(hipMemcpy(d, dd, 256 * sizeof(float), hipMemcpyDeviceToHost));
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dd);
+    return PIC_OK;
+}
+
+int pic_unet_mfma32_probe(const float* a, const float* b, float* d) {
+    if (!a || !b || !d) {
+        set_error("pic_unet_mfma32_probe: null argument");
+        return PIC_ERR_ARG;
+    }
+    float *da, *db, *dd;
+    PIC_HIP(hipMalloc(&da, 512 * sizeof(float)));
+    PIC_HIP(hipMalloc(&db, 512 * sizeof(float)));
+    PIC_HIP(hipMalloc(&dd, 256 * sizeof(float)));
+    PIC_HIP(hipMemcpy(da, a, 512 * sizeof(float), hipMemcpyHostToDevice));
+    PIC_HIP(hipMemcpy(db, b, 512 * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mfma32_probe, dim3(1), dim3(64), 0, 0, da, db, dd);
     PIC_HIP(hipGetLastError());
     PIC_HIP(hipMemcpy(d, dd, 256 * sizeof(float), hipMemcpyDeviceToHost));
     (void)hipFree(da);
@@ -1155,8 +1534,9 @@ int pic_unet_create(int n, uint64_t seed, pic_unet** out) {
     for (Tensor& x : u->t) {
         const size_t sz = tsize(x);
         if (hipMalloc(&x.p, sz * sizeof(bf16)) != hipSuccess) return cleanup(unet_fail(u, "activation allocation"));
-        if (hipMalloc(&x.g, sz * sizeof(float)) != hipSuccess) return cleanup(unet_fail(u, "gradient allocation"));
         if (hipMemset(x.p, 0, sz * sizeof(bf16)) != hipSuccess) return cleanup(unet_fail(u, "memset"));
+        if (!x.grad) continue;
+        if (hipMalloc(&x.g, sz * sizeof(float)) != hipSuccess) return cleanup(unet_fail(u, "gradient allocation"));
         if (hipMemset(x.g, 0, sz * sizeof(float)) != hipSuccess) return cleanup(unet_fail(u, "memset"));
     }
     for (const Conv& c : u->conv) {
@@ -1261,17 +1641,7 @@ int pic_unet_forward(pic_unet* u, const float* x, float* out) {
         set_error("pic_unet_forward: null argument");
         return PIC_ERR_ARG;
     }
-    const int N = u->N;
-    std::vector<bf16> h((size_t)(N + 2) * (N + 2) * 16, 0);
-    for (int y = 0; y < N; ++y)
-        for (int xx = 0; xx < N; ++xx)
-            for (int c = 0; c < 2; ++c)
-                h[((size_t)(y + 1) * (N + 2) + xx + 1) * 16 + c] = f2bf_host(x[((size_t)y * N + xx) * 2 + c]);
-    PIC_HIP(hipMemcpyAsync(u->t[X0].p, h.data(), h.size() * 2, hipMemcpyHostToDevice, u->stream));
-    PIC_TRY(unet_forward(u));
-    PIC_HIP(hipMemcpyAsync(out, u->out, (size_t)N * N * 4, hipMemcpyDeviceToHost, u->stream));
-    PIC_HIP(hipStreamSynchronize(u->stream));
-    return PIC_OK;
+    return unet_forward_host(u, x, out, false);
 }
 
 // One training step on host data: forward, backward, and (lr > 0) Adam.
@@ -1283,7 +1653,7 @@ int pic_unet_train_step(pic_unet* u, const float* x, const float* tgt, const uin
     }
     const int N = u->N;
     std::vector<float> o((size_t)N * N);
-    PIC_TRY(pic_unet_forward(u, x, o.data()));
+    PIC_TRY(unet_forward_host(u, x, o.data(), true));
     PIC_HIP(hipMemcpyAsync(u->tgt, tgt, (size_t)N * N * 4, hipMemcpyHostToDevice, u->stream));
     PIC_HIP(hipMemcpyAsync(u->mask, mask, (size_t)N * N, hipMemcpyHostToDevice, u->stream));
     PIC_TRY(unet_backward(u));
@@ -1305,7 +1675,7 @@ int pic_unet_train_step_residual(pic_unet* u, const float* x, const double* r, c
     }
     const int N = u->N;
     std::vector<float> o((size_t)N * N);
-    PIC_TRY(pic_unet_forward(u, x, o.data()));
+    PIC_TRY(unet_forward_host(u, x, o.data(), true));
     const double one = 1.0;
     unsigned long long sc[2];
     std::memcpy(&sc[0], &one, 8);
@@ -1333,12 +1703,12 @@ int pic_unet_benchmark(pic_unet* u, int reps, float* fwd_ms, float* train_ms) {
     PIC_HIP(hipEventCreate(&e0));
     PIC_HIP(hipEventCreate(&e1));
     PIC_HIP(hipEventCreate(&e2));
-    PIC_TRY(unet_forward(u));                      // warm-up
+    PIC_TRY(unet_forward(u, false));               // warm-up
     PIC_HIP(hipEventRecord(e0, u->stream));
-    for (int k = 0; k < reps; ++k) PIC_TRY(unet_forward(u));
+    for (int k = 0; k < reps; ++k) PIC_TRY(unet_forward(u, false));
     PIC_HIP(hipEventRecord(e1, u->stream));
     for (int k = 0; k < reps; ++k) {
-        PIC_TRY(unet_forward(u));
+        PIC_TRY(unet_forward(u, true));
         PIC_TRY(unet_backward(u));
     }
     PIC_HIP(hipEventRecord(e2, u->stream));
@@ -1363,7 +1733,7 @@ pic_unet* pic_learned_unet(pic_ctx* c) { return c ? c->unet : nullptr; }
 namespace pic {
 
 // r0 and the per-solve scales for (rhs = c->rho after the Dirichlet overwrite, phi_old)
-static int unet_prepare(pic_ctx* c, pic_unet* u, const double* phi_old) {
+static int unet_prepare(pic_ctx* c, pic_unet* u, const double* phi_old, bool train) {
     const int N = u->N, gx = c->g.gx;
     hipStream_t s = c->stream;
     const double cx = 1.0 / (c->g.dx * c->g.dx), cy = 1.0 / (c->g.dy * c->g.dy);
@@ -1371,7 +1741,7 @@ static int unet_prepare(pic_ctx* c, pic_unet* u, const double* phi_old) {
     hipLaunchKernelGGL(k_unet_resid, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, c->rho, phi_old, c->dir, gx, N, cx, cy,
                        2 * (cx + cy), u->r0, u->scl);
     hipLaunchKernelGGL(k_unet_input, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, u->r0, phi_old, gx, N, u->scl,
-                       u->t[X0].p);
+                       u->t[X0C].p, train ? u->t[X0].p : (bf16*)nullptr);
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
@@ -1379,10 +1749,10 @@ static int unet_prepare(pic_ctx* c, pic_unet* u, const double* phi_old) {
 int unet_guess(pic_ctx* c, pic_unet* u, double* guess) {
     const int N = u->N, gx = c->g.gx;
     hipStream_t s = c->stream;
-    PIC_TRY(unet_prepare(c, u, c->phi));
+    PIC_TRY(unet_prepare(c, u, c->phi, false));
     hipStream_t keep = u->stream;
     u->stream = s;
-    const int rc = unet_forward(u);
+    const int rc = unet_forward(u, false);
     u->stream = keep;
     PIC_TRY(rc);
     const double diag = 2.0 / (c->g.dx * c->g.dx) + 2.0 / (c->g.dy * c->g.dy);
@@ -1395,7 +1765,7 @@ int unet_guess(pic_ctx* c, pic_unet* u, double* guess) {
 int unet_train_pair(pic_ctx* c, pic_unet* u, const double* phi_old, float lr, float* loss) {
     const int N = u->N, gx = c->g.gx;
     hipStream_t s = c->stream;
-    PIC_TRY(unet_prepare(c, u, phi_old));
+    PIC_TRY(unet_prepare(c, u, phi_old, true));
     const double cx = 1.0 / (c->g.dx * c->g.dx), cy = 1.0 / (c->g.dy * c->g.dy);
     const double diag = 2 * (cx + cy);
     // the unknown-node mask (the MSE target it also writes is not used)
@@ -1404,7 +1774,7 @@ int unet_train_pair(pic_ctx* c, pic_unet* u, const double* phi_old, float lr, fl
     PIC_HIP(hipGetLastError());
     hipStream_t keep = u->stream;
     u->stream = s;
-    int rc = unet_forward(u);
+    int rc = unet_forward(u, true);
     if (rc == PIC_OK) rc = unet_backward(u, 1, (float)(cx / diag), (float)(cy / diag));
     if (rc == PIC_OK) rc = adam_step(u, lr);
     u->stream = keep;

@@ -135,6 +135,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "pic_unet_forward": (i32, [vp, fp, fp]),
         "pic_unet_train_step": (i32, [vp, fp, fp, C.POINTER(C.c_uint8), C.c_float, fp]),
         "pic_unet_mfma_probe": (i32, [fp, fp, fp]),
+        "pic_unet_mfma32_probe": (i32, [fp, fp, fp]),
         "pic_dgemm_check": (i32, [i32, i32, i32, i32, i32, i32, dp, dp, dp, dp]),
         "pic_unet_benchmark": (i32, [vp, i32, fp, fp]),
         "pic_unet_train_step_residual": (i32, [vp, fp, dp, C.POINTER(C.c_uint8), C.c_float, C.c_float, C.c_float,
@@ -440,6 +441,17 @@ def mfma_probe(a, b):
     b = np.ascontiguousarray(b, np.float32)
     d = np.empty((16, 16), np.float32)
     _chk(lib.pic_unet_mfma_probe(_fp(a), _fp(b), _fp(d)))
+    return d
+
+
+def mfma32_probe(a, b):
+    """16x16x32 BF16 MFMA of row-major a (16 x 32) and b (32 x 16) float inputs (exact in BF16)."""
+    lib = load()
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    assert a.shape == (16, 32) and b.shape == (32, 16)
+    d = np.empty((16, 16), np.float32)
+    _chk(lib.pic_unet_mfma32_probe(_fp(a), _fp(b), _fp(d)))
     return d
 
 

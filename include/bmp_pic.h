@@ -367,6 +367,9 @@ int     pic_unet_train_step_residual(pic_unet* u, const float* x, const double* 
 int     pic_unet_benchmark(pic_unet* u, int reps, float* fwd_ms, float* train_ms);
 /* 16x16x16 BF16 MFMA on row-major 16x16 inputs (layout check). */
 int     pic_unet_mfma_probe(const float* a, const float* b, float* d);
+/* 16x16x32 BF16 MFMA (the forward convolutions' instruction) on row-major
+ * a (16 x 32) and b (32 x 16): d (16 x 16) = a b (layout check). */
+int     pic_unet_mfma32_probe(const float* a, const float* b, float* d);
 /* The direct Poisson solve's FP64 matrix-core GEMM on host arrays (check of
  * the kernel): C (M x N, row-major) = op(A) op(B) with op(A) = A (M x K) or,
  * ta != 0, A^T of an array stored K x M; op(B) = B (K x N) or, tb != 0, B^T of

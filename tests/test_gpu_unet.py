@@ -18,11 +18,11 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def _inputs(seed=0):
+def _inputs(seed=0, n=N):
     rng = np.random.default_rng(seed)
-    yy, xx = np.mgrid[0:N, 0:N] / N
-    x = np.stack([np.sin(6 * xx) * np.cos(4 * yy) + 0.3 * rng.normal(size=(N, N)),
-                  np.cos(3 * xx + yy) + 0.1 * rng.normal(size=(N, N))], axis=-1)
+    yy, xx = np.mgrid[0:n, 0:n] / n
+    x = np.stack([np.sin(6 * xx) * np.cos(4 * yy) + 0.3 * rng.normal(size=(n, n)),
+                  np.cos(3 * xx + yy) + 0.1 * rng.normal(size=(n, n))], axis=-1)
     return x.astype(np.float32)
 
 
@@ -45,14 +45,35 @@ def test_mfma_lane_layout(hip_mod):
     assert np.array_equal(d, a @ b)
 
 
-def test_unet_forward_matches_oracle(hip_mod):
+def test_mfma32_lane_layout(hip_mod):
+    """The forward convolutions' 16x16x32 instruction: lane l holds A[l%16][8(l/16)+e]
+    and B[8(l/16)+e][l%16] and receives D[4(l/16)+e][l%16]."""
+    rng = np.random.default_rng(2)
+    a = rng.integers(-8, 8, (16, 32)).astype(np.float32)
+    b = rng.integers(-8, 8, (32, 16)).astype(np.float32)
+    d = hip_mod.mfma32_probe(a, b)
+    assert np.array_equal(d, a @ b)
+
+
+@pytest.mark.parametrize("n", [N, 1024])
+def test_unet_forward_matches_oracle(hip_mod, n):
+    """Inference forward pass (fused pooling, upsampling read in place, fused
+    head) at 128 (images narrower than a 64-pixel tile on the coarse levels)
+    and 1024 (the learned guess's image, two channel tiles per workgroup on
+    the 512 level), and the training forward pass (every activation kept),
+    both against the oracle."""
     import unet_oracle as U
-    net = hip_mod.UNet(N, seed=7)
+    net = hip_mod.UNet(n, seed=7)
     p = _params(net, 1)
-    x = _inputs(1)
+    x = _inputs(1, n)
     out = net.forward(x)
     ref, _ = U.forward(p, x.astype(np.float64))
+    print(f"forward error at {n}: {_rel(out, ref):.3e}")
     assert _rel(out, ref) < 2e-2
+    if n == N:
+        tgt = np.zeros((n, n), np.float32)
+        loss = net.train_step(x, tgt, np.ones((n, n), np.uint8), 0.0)
+        assert loss == pytest.approx(float(np.mean(ref ** 2)), rel=2e-2)
     net.close()
 
 
