@@ -62,154 +62,6 @@ __global__ void k_mfma_probe(const float* a, const float* b, float* d) {
     for (int e = 0; e < 4; ++e) d[(4 * kq + e) * 16 + i] = acc[e];
 }
 
-// ----------------------------------------------------------- convolution
-// Y[p][co] = act(bias[co] + sum_{tap, ci} Wt[tap][co][ci] X[p + off(tap)][ci])
-// X: padded NHWC, channel stride xC, first channel xo, cin channels (multiple
-// of 16).  Wt: [9][NT*16][cin].  Y: padded NHWC (stride yC, first channel yo),
-// BF16 (forward) or FP32 (input gradients).  One wave computes 64 pixels of a
-// row (4 M tiles) by NT*16 output channels.
-template <int NT, int RELU, typename TO>
-__global__ __launch_bounds__(kT) void k_conv3x3(const bf16* __restrict__ X, int xC, int xo, int cin,
-                                                const bf16* __restrict__ Wt, const float* __restrict__ bias,
-                                                TO* __restrict__ Y, int yC, int yo, int H, int W) {
-    const int lane = threadIdx.x & 63;
-    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int nseg = (W + 63) >> 6;
-    const int y = gw / nseg;
-    if (y >= H) return;
-    const int x0 = (gw - y * nseg) * 64;
-    const int i = lane & 15, kq = lane >> 4;
-    constexpr int cout = NT * 16;
-    const int Wp = W + 2;
-    f4v acc[4][NT];
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[m][t] = (f4v){0.f, 0.f, 0.f, 0.f};
-    for (int tap = 0; tap < 9; ++tap) {
-        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-        const bf16* xrow = X + (size_t)(y + 1 + dy) * Wp * xC + xo;
-        const bf16* wtap = Wt + (size_t)tap * cout * cin;
-        for (int cc = 0; cc < cin; cc += 16) {
-            s4v a[4], b[NT];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const int x = x0 + m * 16 + i;
-                if (x < W) a[m] = *(const s4v*)(xrow + (size_t)(x + 1 + dx) * xC + cc + 4 * kq);
-                else a[m] = (s4v){0, 0, 0, 0};
-            }
-#pragma unroll
-            for (int t = 0; t < NT; ++t) b[t] = *(const s4v*)(wtap + (size_t)(t * 16 + i) * cin + cc + 4 * kq);
-#pragma unroll
-            for (int m = 0; m < 4; ++m)
-#pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b[t], acc[m][t], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + i;
-        const float bb = bias ? bias[co] : 0.f;
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int x = x0 + m * 16 + 4 * kq + e;
-                if (x >= W) continue;
-                float v = acc[m][t][e] + bb;
-                if (RELU) v = fmaxf(v, 0.f);
-                TO* dst = Y + ((size_t)(y + 1) * Wp + x + 1) * yC + yo + co;
-                if constexpr (sizeof(TO) == 2) *dst = f2bf(v);
-                else *dst = v;
-            }
-    }
-}
-
-// The same convolution with the operands staged in LDS: a workgroup computes
-// a tile of kCH rows x 64 pixels (wave w: row y0 + w) for all NT*16 output
-// channels.  Per 16-channel input chunk the padded input tile (kCH + 2 rows x
-// 66 pixels) and the chunk's weights [9][cout][16] are copied to LDS with
-// 16-byte loads (each input value leaves HBM once per tile instead of once
-// per tap), then every MFMA operand is an 8-byte LDS read: A = 16 pixels x
-// 4 channels per lane group at [row][pixel][channel], B = 16 output channels
-// x 4 input channels at [tap][co][channel] (both 512 contiguous bytes per
-// wave, no bank conflicts).  H must be a multiple of kCH.  Small images split
-// the output channels over blockIdx.y (coutT in all, NT*16 per workgroup) so
-// that the launch still fills the GPU.
-static constexpr int kCH = 4;
-static constexpr int kCPW = 66;   // tile width with the halo
-
-template <int NT, int RELU, typename TO>
-__global__ __launch_bounds__(kT) void k_conv3x3_lds(const bf16* __restrict__ X, int xC, int xo, int cin,
-                                                    const bf16* __restrict__ Wt, const float* __restrict__ bias,
-                                                    TO* __restrict__ Y, int yC, int yo, int H, int W, int coutT) {
-    constexpr int cout = NT * 16;
-    const int cob = (int)blockIdx.y * cout;
-    __shared__ s4v xin[(kCH + 2) * kCPW * 4];
-    __shared__ s4v wsh[9 * cout * 4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int nxt = (W + 63) >> 6;
-    const int y0 = (int)(blockIdx.x / nxt) * kCH, x0 = (int)(blockIdx.x % nxt) * 64;
-    const int i = lane & 15, kq = lane >> 4;
-    const int Wp = W + 2;
-    f4v acc[4][NT];
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[m][t] = (f4v){0.f, 0.f, 0.f, 0.f};
-    for (int cc = 0; cc < cin; cc += 16) {
-        __syncthreads();
-        // padded input rows y0 .. y0 + kCH + 1, padded columns x0 .. x0 + 65
-        for (int e = threadIdx.x; e < (kCH + 2) * kCPW * 2; e += kT) {
-            const int pix = e >> 1, half = e & 1;
-            const int r = pix / kCPW, pc = pix - r * kCPW;
-            const int gxp = x0 + pc;
-            int4 v = make_int4(0, 0, 0, 0);
-            if (gxp <= W + 1) v = *(const int4*)(X + ((size_t)(y0 + r) * Wp + gxp) * xC + xo + cc + 8 * half);
-            ((int4*)xin)[e] = v;
-        }
-        for (int e = threadIdx.x; e < 9 * cout * 2; e += kT) {
-            const int tc = e >> 1, half = e & 1;
-            const int tap = tc / cout, col = tc - tap * cout;
-            ((int4*)wsh)[e] = *(const int4*)(Wt + ((size_t)tap * coutT + cob + col) * cin + cc + 8 * half);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-            s4v a[4], b[NT];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) a[m] = xin[((wv + 1 + dy) * kCPW + 16 * m + i + 1 + dx) * 4 + kq];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) b[t] = wsh[(tap * cout + 16 * t + i) * 4 + kq];
-#pragma unroll
-            for (int m = 0; m < 4; ++m)
-#pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b[t], acc[m][t], 0, 0, 0);
-        }
-    }
-    const int y = y0 + wv;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int co = cob + t * 16 + i;
-        const float bb = bias ? bias[co] : 0.f;
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int x = x0 + m * 16 + 4 * kq + e;
-                if (x >= W) continue;
-                float v = acc[m][t][e] + bb;
-                if (RELU) v = fmaxf(v, 0.f);
-                TO* dst = Y + ((size_t)(y + 1) * Wp + x + 1) * yC + yo + co;
-                if constexpr (sizeof(TO) == 2) *dst = f2bf(v);
-                else *dst = v;
-            }
-    }
-}
-
 // ------------------------------------------------ forward convolution (x32)
 // The forward pass's 3x3 convolutions on v_mfma_f32_16x16x32_bf16 with the
 // roles of the operands chosen for the epilogue: A = weights (M = 16 output
@@ -257,6 +109,7 @@ struct ConvFwdArgs {
     const float* wh;               // 1x1 head (16 weights, bias bh) into out[H][W] (FP32), or out null
     const float* bh;
     float* out;
+    float* Yf;                     // FP32 output without bias or ReLU (input gradients), padded NHWC
     int H, W;
 };
 
@@ -286,9 +139,16 @@ __device__ __forceinline__ float dpp_xor8(float v) {
 
 // Epilogue of one pixel tile m and channel tile t: v = the FP32 sums of lane
 // (i, q) for channels co .. co+3 of pixel (y, x).  EPI 0: output only; 1:
-// output and pooled output; 2: head (and the output when a.Y is set).
+// output and pooled output; 2: head (and the output when a.Y is set); 3: the
+// sums as FP32 into a.Yf (stride a.yC, first channel a.yo), no bias or ReLU.
 template <int EPI>
 __device__ __forceinline__ void fwd_epilogue(const ConvFwdArgs& a, f4v acc, int co, int y, int x, int lane) {
+    if constexpr (EPI == 3) {
+        if (x < a.W)
+            *(float4*)(a.Yf + ((size_t)(y + 1) * (a.W + 2) + x + 1) * a.yC + a.yo + co) =
+                make_float4(acc[0], acc[1], acc[2], acc[3]);
+        return;
+    }
     const float4 bb = *(const float4*)(a.bias + co);
     float v[4] = {acc[0] + bb.x, acc[1] + bb.y, acc[2] + bb.z, acc[3] + bb.w};
     uint2 packed;
@@ -497,86 +357,77 @@ __global__ void k_mfma32_probe(const float* a, const float* b, float* d) {
     for (int e = 0; e < 4; ++e) d[(4 * q + e) * 16 + i] = acc[e];
 }
 
-// Partial of dW[tap][co][ci] = sum_{y,x} dZT[co][y][x] XT[ci][y+1+dy][x+1+dx]
-// over one chunk of rows.  dZT: [cout][H][W] (unpadded), XT: [cin][H+2][W+2]
-// (padded).  One wave: one tap, one 16-channel co tile, CT ci tiles, rows
-// [r0, r0 + rows); it writes its tile to slot `chunk` of part (slot stride
-// 9 cout cin), and k_sum_slots adds the slots in chunk order.
-template <int CT>
-__global__ __launch_bounds__(kT) void k_conv_wgrad(const bf16* __restrict__ dZT, const bf16* __restrict__ XT,
-                                                   int cout, int cin, int H, int W, int rows,
-                                                   float* __restrict__ part) {
-    const int lane = threadIdx.x & 63;
-    const int gw = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int nchunk = (H + rows - 1) / rows;
-    const int cot_n = cout >> 4;
-    const int chunk = gw % nchunk;
-    const int t2 = gw / nchunk;
-    const int cot = t2 % cot_n, tap = t2 / cot_n;
-    if (tap >= 9) return;
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    const int i = lane & 15, kq = lane >> 4;
-    const int Wp = W + 2;
-    f4v acc[CT];
+// Partial of dW[tap][co][ci] = sum_{y,x} dZT[co][y][x] XT[ci][y+dyy][x+dxx]
+// (tap = 3 dyy + dxx) over one chunk of rows, on v_mfma_f32_16x16x32_bf16
+// with K = 32 pixels of a row: A = dZT (M = 16 output channels), B = the
+// input row shifted by the tap (N = 16 input channels).  dZT: [cout][H][W]
+// (unpadded, W a multiple of 8), XT: [cin][H+2][xs] (padded columns 0 ..
+// W+1, row stride xs a multiple of 8 with at least 8 columns past W+1).  One
+// workgroup: one (co, ci) tile pair (blockIdx.y), rows [r0, r0 + rows) of
+// chunk blockIdx.x; wave w takes a quarter of each row's columns (in 32-pixel
+// steps) and all nine taps (nine accumulators): per 32 pixels it loads its dZ
+// chunk once and, for each of the three input rows, two aligned 16-byte
+// chunks from which the three column shifts are formed by byte alignment
+// (lane (i, q) holds pixels 8q .. 8q+7 of channel i).  The four waves' tiles
+// are added in a fixed order through LDS and written to slot `chunk` of part
+// (slot stride 9 cout cin); k_sum_slots adds the slots in chunk order.
+__device__ __forceinline__ i4v shift_elems(i4v lo, i4v hi, int s) {
+    // elements s .. s+7 of the 16 elements lo:hi (s = 0, 1, 2)
+    if (s == 0) return lo;
+    if (s == 2) return (i4v){lo[1], lo[2], lo[3], hi[0]};
+    return (i4v){(int)__builtin_amdgcn_alignbyte((unsigned)lo[1], (unsigned)lo[0], 2),
+                 (int)__builtin_amdgcn_alignbyte((unsigned)lo[2], (unsigned)lo[1], 2),
+                 (int)__builtin_amdgcn_alignbyte((unsigned)lo[3], (unsigned)lo[2], 2),
+                 (int)__builtin_amdgcn_alignbyte((unsigned)hi[0], (unsigned)lo[3], 2)};
+}
+
+__global__ __launch_bounds__(kT) void k_wgrad(const bf16* __restrict__ dZT, const bf16* __restrict__ XT, int cout,
+                                              int cin, int H, int W, int xs, int rows, float* __restrict__ part) {
+    __shared__ f4v red[3][9][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int chunk = blockIdx.x, pair = blockIdx.y;
+    const int cit = cin >> 4;
+    const int cot = pair / cit, cti = pair % cit;
+    const int i = lane & 15, q = lane >> 4;
+    f4v acc[9];
 #pragma unroll
-    for (int c = 0; c < CT; ++c) acc[c] = (f4v){0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < 9; ++t) acc[t] = (f4v){0.f, 0.f, 0.f, 0.f};
+    const bf16* zc = dZT + (size_t)(cot * 16 + i) * H * W + 8 * q;
+    const bf16* xc = XT + (size_t)(cti * 16 + i) * (H + 2) * xs + 8 * q;
     const int r0 = chunk * rows, r1 = min(H, r0 + rows);
+    const int seg = (((W + 3) / 4) + 31) / 32 * 32;   // columns per wave, in 32-pixel steps
+    const int xa = wv * seg, xb = min(W, xa + seg);
     for (int y = r0; y < r1; ++y) {
-        const bf16* arow = dZT + ((size_t)(cot * 16 + i) * H + y) * W;
-        for (int x0 = 0; x0 < W; x0 += 16) {
-            const s4v a = *(const s4v*)(arow + x0 + 4 * kq);
+        for (int px0 = xa; px0 < xb; px0 += 32) {
+            const bool in = px0 + 8 * q < xb;   // W is a multiple of 8
+            const i4v zv = in ? *(const i4v*)(zc + (size_t)y * W + px0) : (i4v){0, 0, 0, 0};
+            const bf8v a = __builtin_bit_cast(bf8v, zv);
 #pragma unroll
-            for (int c = 0; c < CT; ++c) {
-                const bf16* src = XT + ((size_t)(c * 16 + i) * (H + 2) + y + 1 + dy) * Wp + x0 + 4 * kq + 1 + dx;
-                s4v b;
-                b[0] = (short)src[0];
-                b[1] = (short)src[1];
-                b[2] = (short)src[2];
-                b[3] = (short)src[3];
-                acc[c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[c], 0, 0, 0);
+            for (int dyy = 0; dyy < 3; ++dyy) {
+                const bf16* xr = xc + (size_t)(y + dyy) * xs + px0;
+                i4v lo = {0, 0, 0, 0}, hi = {0, 0, 0, 0};
+                if (in) {
+                    lo = *(const i4v*)xr;
+                    hi = *(const i4v*)(xr + 8);
+                }
+#pragma unroll
+                for (int dxx = 0; dxx < 3; ++dxx)
+                    acc[3 * dyy + dxx] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        a, __builtin_bit_cast(bf8v, shift_elems(lo, hi, dxx)), acc[3 * dyy + dxx], 0, 0, 0);
             }
         }
     }
+    if (wv > 0)
 #pragma unroll
-    for (int c = 0; c < CT; ++c)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int co = cot * 16 + 4 * kq + e, ci = c * 16 + i;
-            part[(size_t)chunk * 9 * cout * cin + ((size_t)tap * cout + co) * cin + ci] = acc[c][e];
-        }
-}
-
-// Partials of db[co] = sum_p dZ[p][co]; dZ padded NHWC with cout channels (8 <=
-// cout <= 256, a multiple of 8).  Thread t takes the 8-channel slice
-// t % (cout/8) of pixels t / (cout/8), + kT / (cout/8), ... of a grid-stride
-// walk (16-byte loads); the block adds its per-thread sums per channel through
-// LDS in a fixed order and writes them to slot blockIdx.x of part (stride
-// cout).
-__global__ __launch_bounds__(kT) void k_bias_grad(const bf16* __restrict__ dZ, int cout, int H, int W,
-                                                  float* __restrict__ part) {
-    __shared__ float sh[kT][9];
-    const int c8 = cout / 8, per = kT / c8;
-    const int sl = threadIdx.x % c8, q0 = threadIdx.x / c8;
-    const int HW = H * W;
-    float acc[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) acc[k] = 0.f;
-    if (q0 < per)
-        for (int p = blockIdx.x * per + q0; p < HW; p += gridDim.x * per) {
-            const int y = p / W, x = p - y * W;
-            const int4 v = *(const int4*)(dZ + ((size_t)(y + 1) * (W + 2) + x + 1) * cout + 8 * sl);
-            const bf16* b = (const bf16*)&v;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) acc[k] += bf2f(b[k]);
-        }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) sh[threadIdx.x][k] = acc[k];
+        for (int t = 0; t < 9; ++t) red[wv - 1][t][lane] = acc[t];
     __syncthreads();
-    if ((int)threadIdx.x < cout) {
-        const int ch = threadIdx.x, s8 = ch >> 3, k = ch & 7;
-        float t = 0.f;
-        for (int q = 0; q < per; ++q) t += sh[q * c8 + s8][k];
-        part[(size_t)blockIdx.x * cout + ch] = t;
+    if (wv > 0) return;
+    float* dst = part + (size_t)chunk * 9 * cout * cin;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const f4v v = ((acc[t] + red[0][t][lane]) + red[1][t][lane]) + red[2][t][lane];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) dst[((size_t)t * cout + cot * 16 + 4 * q + e) * cin + cti * 16 + i] = v[e];
     }
 }
 
@@ -584,21 +435,93 @@ __global__ __launch_bounds__(kT) void k_bias_grad(const bf16* __restrict__ dZ, i
 // (stride yC, first channel yo), dZ padded NHWC with c channels.  The whole
 // padded grid is written (zero halo): the buffer is shared by layers of
 // different shapes and the input-gradient convolution reads the halo.
-__global__ void k_relu_bwd(const float* __restrict__ dY, int gC, int go, const bf16* __restrict__ Y, int yC, int yo,
-                           bf16* __restrict__ dZ, int c, int H, int W) {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t tot = (int64_t)(H + 2) * (W + 2) * c;
-    if (n >= tot) return;
-    const int ch = (int)(n % c);
-    const int64_t q = n / c;
-    const int yy = (int)(q / (W + 2)), xx = (int)(q - (int64_t)yy * (W + 2));
-    bf16 r = 0;
-    if (yy >= 1 && yy <= H && xx >= 1 && xx <= W) {
-        const float g = dY[q * gC + go + ch];
-        const float v = bf2f(Y[q * yC + yo + ch]);
-        r = f2bf(v > 0.f ? g : 0.f);
+// With it, its CHW transpose dZT [c][H][W] and the bias-gradient partials,
+// in one pass.  Workgroup: 64 padded columns (blockIdx.x) of a
+// chunk of kRB padded rows (blockIdx.y), 16 channels (blockIdx.z); per row
+// each thread forms 4 channels of one pixel (16-byte dY and 8-byte Y loads),
+// the tile is transposed through LDS, and thread t writes channel t / 16,
+// pixels 4 (t % 16) .. +3 of dZT and adds them to its bias sum (rows in
+// order); the 16 sums of a channel are added in a fixed order into slot
+// (blockIdx.y * gridDim.x + blockIdx.x) of part (stride c).
+static constexpr int kRB = 16;
+__global__ __launch_bounds__(kT) void k_relu_bwd_t(const float* __restrict__ dY, int gC, int go,
+                                                   const bf16* __restrict__ Y, int yC, int yo, bf16* __restrict__ dZ,
+                                                   bf16* __restrict__ dZT, float* __restrict__ part, int c, int H,
+                                                   int W) {
+    __shared__ bf16 tile[16][68];
+    __shared__ float bs[16][17];
+    const int t = threadIdx.x;
+    const int c0 = blockIdx.z * 16;
+    const int X0 = blockIdx.x * 64;            // first padded column
+    const int lp = t >> 2, lc = (t & 3) * 4;   // load: pixel, first of 4 channels
+    const int wc = t >> 4, wp = (t & 15) * 4;  // transposed write: channel, first of 4 pixels
+    float bsum = 0.f;
+    const int y0 = blockIdx.y * kRB, y1 = min(H + 2, y0 + kRB);
+    const int xx = X0 + lp;
+    const bool colin = xx >= 1 && xx <= W;
+    // the next row's dY and Y are loaded while the current row is transposed
+    auto load = [&](int yy, float4& g, uint2& yb) {
+        g = make_float4(0.f, 0.f, 0.f, 0.f);
+        yb = make_uint2(0u, 0u);
+        if (colin && yy >= 1 && yy <= H) {
+            const size_t q = (size_t)yy * (W + 2) + xx;
+            g = *(const float4*)(dY + q * gC + go + c0 + lc);
+            yb = *(const uint2*)(Y + q * yC + yo + c0 + lc);
+        }
+    };
+    float4 gn;
+    uint2 ybn;
+    load(y0, gn, ybn);
+    for (int yy = y0; yy < y1; ++yy) {
+        const float4 g = gn;
+        const uint2 yb = ybn;
+        if (yy + 1 < y1) load(yy + 1, gn, ybn);
+        uint2 zv = make_uint2(0u, 0u);
+        if (xx < W + 2) {
+            const size_t q = (size_t)yy * (W + 2) + xx;
+            const bf16* yv = (const bf16*)&yb;
+            bf16* z = (bf16*)&zv;
+            z[0] = f2bf(bf2f(yv[0]) > 0.f ? g.x : 0.f);
+            z[1] = f2bf(bf2f(yv[1]) > 0.f ? g.y : 0.f);
+            z[2] = f2bf(bf2f(yv[2]) > 0.f ? g.z : 0.f);
+            z[3] = f2bf(bf2f(yv[3]) > 0.f ? g.w : 0.f);
+            *(uint2*)(dZ + q * c + c0 + lc) = zv;
+        }
+        const bf16* z = (const bf16*)&zv;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) tile[lc + k][lp] = z[k];
+        __syncthreads();
+        if (yy >= 1 && yy <= H) {
+            // interior columns of this block: padded X0 + wp + k, image column X0 + wp + k - 1
+            bf16 o[4];
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int xx2 = X0 + wp + k;
+                o[k] = tile[wc][wp + k];
+                if (xx2 >= 1 && xx2 <= W) {
+                    bsum += bf2f(o[k]);
+                    any = true;
+                }
+            }
+            if (any) {
+                bf16* dst = dZT + ((size_t)(c0 + wc) * H + (yy - 1)) * W;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int xx2 = X0 + wp + k;
+                    if (xx2 >= 1 && xx2 <= W) dst[xx2 - 1] = o[k];
+                }
+            }
+        }
+        __syncthreads();
     }
-    dZ[n] = r;
+    bs[wc][t & 15] = bsum;
+    __syncthreads();
+    if (t < 16) {
+        float v = 0.f;
+        for (int k = 0; k < 16; ++k) v += bs[t][k];
+        part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * c + c0 + t] = v;
+    }
 }
 
 // out[c][y][x] = in[(y * (W + 2) + x) * C + o + c] over the padded grid (pad = 1)
@@ -607,7 +530,7 @@ __global__ void k_relu_bwd(const float* __restrict__ dY, int gC, int go, const b
 // LDS so both the NHWC reads (16 bytes per pixel) and the CHW writes (64
 // consecutive pixels of a channel) are contiguous.  c, o, C multiples of 8.
 __global__ __launch_bounds__(kT) void k_to_chw(const bf16* __restrict__ in, int C, int o, int c, int H, int W, int pad,
-                                               bf16* __restrict__ out) {
+                                               int os, bf16* __restrict__ out) {
     __shared__ bf16 tile[16][66];
     const int Ho = H + 2 * pad, Wo = W + 2 * pad;
     const int x0 = blockIdx.x * 64, y = blockIdx.y, c0 = blockIdx.z * 16;
@@ -628,7 +551,7 @@ __global__ __launch_bounds__(kT) void k_to_chw(const bf16* __restrict__ in, int 
         const int idx = t + kT * k;
         const int ch = idx >> 6, px = idx & 63;
         const int x = x0 + px;
-        if (x < Wo && c0 + ch < c) out[((size_t)(c0 + ch) * Ho + y) * Wo + x] = tile[ch][px];
+        if (x < Wo && c0 + ch < c) out[((size_t)(c0 + ch) * Ho + y) * os + x] = tile[ch][px];
     }
 }
 
@@ -1115,60 +1038,22 @@ static int unet_build(pic_unet* u) {
 // tensor indices used by the fixed schedule below
 enum { X0 = 0, A0, CAT0, P0, A1, CAT1, P1, A2, CAT2, P2, A3, B3, A4, D2, A5, D1, A6, D0, S0, S1, S2, X0C };
 
-template <int NT, int RELU, typename TO>
-static void conv_launch(hipStream_t s, const bf16* X, int xC, int xo, int cin, const bf16* Wt, const float* bias,
-                        TO* Y, int yC, int yo, int H, int W) {
-    if (H % kCH == 0) {
-        const unsigned tiles = (unsigned)((H / kCH) * ((W + 63) / 64));
-        if (NT > 1 && tiles < 1024)
-            hipLaunchKernelGGL((k_conv3x3_lds<1, RELU, TO>), dim3(tiles, NT), dim3(kT), 0, s, X, xC, xo, cin, Wt, bias,
-                               Y, yC, yo, H, W, NT * 16);
-        else
-            hipLaunchKernelGGL((k_conv3x3_lds<NT, RELU, TO>), dim3(tiles), dim3(kT), 0, s, X, xC, xo, cin, Wt, bias, Y,
-                               yC, yo, H, W, NT * 16);
-        return;
-    }
-    const int waves = H * ((W + 63) / 64);
-    hipLaunchKernelGGL((k_conv3x3<NT, RELU, TO>), dim3((unsigned)((waves + 3) / 4)), dim3(kT), 0, s, X, xC, xo, cin,
-                       Wt, bias, Y, yC, yo, H, W);
-}
-
-template <int RELU, typename TO>
-static int conv_any(hipStream_t s, int nt, const bf16* X, int xC, int xo, int cin, const bf16* Wt, const float* bias,
-                    TO* Y, int yC, int yo, int H, int W) {
-    switch (nt) {
-    case 1: conv_launch<1, RELU, TO>(s, X, xC, xo, cin, Wt, bias, Y, yC, yo, H, W); break;
-    case 2: conv_launch<2, RELU, TO>(s, X, xC, xo, cin, Wt, bias, Y, yC, yo, H, W); break;
-    case 3: conv_launch<3, RELU, TO>(s, X, xC, xo, cin, Wt, bias, Y, yC, yo, H, W); break;
-    case 4: conv_launch<4, RELU, TO>(s, X, xC, xo, cin, Wt, bias, Y, yC, yo, H, W); break;
-    case 6: conv_launch<6, RELU, TO>(s, X, xC, xo, cin, Wt, bias, Y, yC, yo, H, W); break;
-    case 8: conv_launch<8, RELU, TO>(s, X, xC, xo, cin, Wt, bias, Y, yC, yo, H, W); break;
-    default: set_error("pic_unet: unsupported channel count %d", nt * 16); return PIC_ERR_UNSUPPORTED;
-    }
-    PIC_HIP(hipGetLastError());
-    return PIC_OK;
-}
-
-static int wgrad_rows(int H) { return H >= 256 ? 2 : 1; }
-static int wgrad_chunks(int H) { return (H + wgrad_rows(H) - 1) / wgrad_rows(H); }
+// Rows per workgroup of the weight gradient: one, or more when the layer has
+// many tile pairs (about 2048 workgroups either way).
+static int wgrad_rows(int H, int pairs) { return std::max(1, pairs * H / 2048); }
+static int wgrad_chunks(int H, int pairs) { return (H + wgrad_rows(H, pairs) - 1) / wgrad_rows(H, pairs); }
+// row stride of the transposed, padded input: W + 2 columns rounded up to 8, plus 8
+static int xt_stride(int W) { return (W + 2 + 7) / 8 * 8 + 8; }
 
 // dW = the weight gradient: per-chunk partials into `part`, then their sum in
 // chunk order.
-static int wgrad_any(hipStream_t s, int ct, const bf16* dZT, const bf16* XT, int cout, int cin, int H, int W,
-                     float* part, float* dW) {
-    const int rows = wgrad_rows(H);
-    const int nchunk = wgrad_chunks(H);
-    const int waves = 9 * (cout / 16) * nchunk;
-    const dim3 grid((unsigned)((waves + 3) / 4));
-    switch (ct) {
-    case 1: hipLaunchKernelGGL(k_conv_wgrad<1>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, part); break;
-    case 2: hipLaunchKernelGGL(k_conv_wgrad<2>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, part); break;
-    case 3: hipLaunchKernelGGL(k_conv_wgrad<3>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, part); break;
-    case 4: hipLaunchKernelGGL(k_conv_wgrad<4>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, part); break;
-    case 6: hipLaunchKernelGGL(k_conv_wgrad<6>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, part); break;
-    case 8: hipLaunchKernelGGL(k_conv_wgrad<8>, grid, dim3(kT), 0, s, dZT, XT, cout, cin, H, W, rows, part); break;
-    default: set_error("pic_unet: unsupported channel count %d", ct * 16); return PIC_ERR_UNSUPPORTED;
-    }
+static int wgrad_any(hipStream_t s, const bf16* dZT, const bf16* XT, int cout, int cin, int H, int W, float* part,
+                     float* dW) {
+    const int pairs = (cout / 16) * (cin / 16);
+    const int rows = wgrad_rows(H, pairs);
+    const int nchunk = wgrad_chunks(H, pairs);
+    hipLaunchKernelGGL(k_wgrad, dim3((unsigned)nchunk, (unsigned)pairs), dim3(kT), 0, s, dZT, XT, cout, cin, H, W,
+                       xt_stride(W), rows, part);
     const int64_t len = (int64_t)9 * cout * cin;
     hipLaunchKernelGGL(k_sum_slots, dim3(sum_blocks(len)), dim3(kT), 0, s, part, nchunk, len, dW);
     PIC_HIP(hipGetLastError());
@@ -1176,7 +1061,8 @@ static int wgrad_any(hipStream_t s, int ct, const bf16* dZT, const bf16* XT, int
 }
 
 static size_t tsize(const Tensor& x) { return (size_t)(x.H + 2) * (x.W + 2) * x.C; }
-static int bias_parts(int H, int W) { return std::min(512, (H * W + 4095) / 4096); }
+// bias-gradient slots of k_relu_bwd_t: one per (64 padded columns, kRB padded rows)
+static int bias_parts(int H, int W) { return ((W + 2 + 63) / 64) * ((H + 2 + kRB - 1) / kRB); }
 
 // nearest 2x upsampling of `in` (channels io ..) into `out` (channels oo ..)
 static int upsample(pic_unet* u, int in, int io, int out, int oo, int c) {
@@ -1309,21 +1195,41 @@ static int conv_bwd(pic_unet* u, int k) {
     const Tensor& b = u->t[c.out];
     const int H = a.H, W = a.W;
     hipStream_t s = u->stream;
-    const int64_t n = (int64_t)(H + 2) * (W + 2) * c.cout;
-    hipLaunchKernelGGL(k_relu_bwd, dim3(nb(n)), dim3(kT), 0, s, b.g, b.C, c.oo, b.p, b.C, c.oo, u->dz, c.cout, H, W);
-    hipLaunchKernelGGL(k_to_chw, dim3((unsigned)((W + 63) / 64), (unsigned)H, (unsigned)((c.cout + 15) / 16)), dim3(kT),
-                       0, s, u->dz, c.cout, 0, c.cout, H, W, 0, u->dzt);
+    // dZ (padded NHWC, for the input gradient), its CHW transpose and the bias partials in one pass
+    const dim3 rg((unsigned)((W + 2 + 63) / 64), (unsigned)((H + 2 + kRB - 1) / kRB), (unsigned)(c.cout / 16));
+    hipLaunchKernelGGL(k_relu_bwd_t, rg, dim3(kT), 0, s, b.g, b.C, c.oo, b.p, b.C, c.oo, u->dz, u->dzt, u->bpart,
+                       c.cout, H, W);
     hipLaunchKernelGGL(k_to_chw, dim3((unsigned)((W + 2 + 63) / 64), (unsigned)(H + 2), (unsigned)((c.cin + 15) / 16)),
-                       dim3(kT), 0, s, a.p, a.C, c.io, c.cin, H, W, 1, u->xt);
+                       dim3(kT), 0, s, a.p, a.C, c.io, c.cin, H, W, 1, xt_stride(W), u->xt);
     PIC_HIP(hipGetLastError());
-    PIC_TRY(wgrad_any(s, c.cin / 16, u->dzt, u->xt, c.cout, c.cin, H, W, u->wpart, u->g + c.woff));
-    const int parts = bias_parts(H, W);
-    hipLaunchKernelGGL(k_bias_grad, dim3(parts), dim3(kT), 0, s, u->dz, c.cout, H, W, u->bpart);
-    hipLaunchKernelGGL(k_sum_slots, dim3(sum_blocks(c.cout)), dim3(kT), 0, s, u->bpart, parts, (int64_t)c.cout, u->g + c.boff);
+    PIC_TRY(wgrad_any(s, u->dzt, u->xt, c.cout, c.cin, H, W, u->wpart, u->g + c.woff));
+    hipLaunchKernelGGL(k_sum_slots_few, dim3((unsigned)c.cout), dim3(kT), 0, s, u->bpart, (int)(rg.x * rg.y), c.cout,
+                       u->g + c.boff);
     PIC_HIP(hipGetLastError());
-    if (k > 0)
-        PIC_TRY((conv_any<0, float>(s, c.cin / 16, u->dz, c.cout, 0, c.cout, u->wflip + c.woff, nullptr, a.g, a.C,
-                                    c.io, H, W)));
+    if (k > 0) {
+        // input gradient: the 3x3 convolution of dZ with the flipped, transposed weights
+        ConvFwdArgs d{};
+        d.X = u->dz;
+        d.xC = c.cout;
+        d.cin = c.cout;
+        d.Wt = u->wflip + c.woff;
+        d.coutT = c.cin;
+        d.Yf = a.g;
+        d.yC = a.C;
+        d.yo = c.io;
+        d.H = H;
+        d.W = W;
+        const unsigned tiles = (unsigned)((H / kFTH) * ((W + 63) / 64));
+        const int ct = c.cin / 16;
+        const int nt = tiles < 1024 ? 1 : ct % 4 == 0 ? 4 : ct % 2 == 0 ? 2 : 1;
+        switch (nt) {
+        case 1: conv_fwd_nt<1, 3>(s, d, tiles); break;
+        case 2: conv_fwd_nt<2, 3>(s, d, tiles); break;
+        case 4: conv_fwd_nt<4, 3>(s, d, tiles); break;
+        default: set_error("pic_unet: unsupported channel count %d", c.cin); return PIC_ERR_UNSUPPORTED;
+        }
+        PIC_HIP(hipGetLastError());
+    }
     return PIC_OK;
 }
 
@@ -1542,12 +1448,12 @@ int pic_unet_create(int n, uint64_t seed, pic_unet** out) {
     for (const Conv& c : u->conv) {
         const Tensor& a = u->t[c.in];
         maxz = std::max(maxz, (size_t)(a.H + 2) * (a.W + 2) * c.cout);
-        maxx = std::max(maxx, (size_t)(a.H + 2) * (a.W + 2) * c.cin);
+        maxx = std::max(maxx, (size_t)(a.H + 2) * xt_stride(a.W) * c.cin);
     }
     const size_t P = u->nparam;
     for (const Conv& c : u->conv) {
         const Tensor& a = u->t[c.in];
-        u->wpart_n = std::max(u->wpart_n, (size_t)wgrad_chunks(a.H) * 9 * c.cout * c.cin);
+        u->wpart_n = std::max(u->wpart_n, (size_t)wgrad_chunks(a.H, (c.cout / 16) * (c.cin / 16)) * 9 * c.cout * c.cin);
         u->bpart_n = std::max(u->bpart_n, (size_t)bias_parts(a.H, a.W) * c.cout);
     }
     u->bpart_n = std::max(u->bpart_n, (size_t)nb((int64_t)n * n) * 17);
