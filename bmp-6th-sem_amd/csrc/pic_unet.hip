@@ -4,11 +4,13 @@
 // channels, ReLU, 2x2 average pooling, nearest upsampling with skip
 // concatenation, a 1x1 head) maps (rho_n, phi_{n-1}) to the change
 // phi_n - phi_{n-1}.  Weights and activations are BF16, every convolution
-// accumulates in FP32 on the MFMA units (v_mfma_f32_16x16x16_bf16 as an
-// implicit GEMM: M = 16 pixels of a row, N = 16 output channels, K = 16 input
-// channels of one tap).  Training is online: forward, backward (input
-// gradients by the same kernel with flipped, transposed weights; weight
-// gradients by MFMA over pixels) and Adam with FP32 master weights.  Every
+// accumulates in FP32 on the MFMA units (v_mfma_f32_16x16x32_bf16 as an
+// implicit GEMM: M = 16 output channels, N = 16 pixels, K = two taps x 16
+// input channels; pooling, upsampling and the head are fused into the
+// convolutions).  Training is online: forward, backward (input gradients by
+// the same kernel with flipped, transposed weights; weight gradients by MFMA
+// with K = 32 pixels, all nine taps per wave) and Adam with FP32 master
+// weights.  Every
 // gradient and loss reduction writes per-wave or per-workgroup partials that
 // k_sum_slots adds in a fixed order (no floating-point atomics), so training
 // is reproducible bit for bit.
