@@ -2,7 +2,7 @@
 // of uneven sizes (grids, particle counts, electrode voltage, Bz) in one step.
 //
 // Every stage of the step is one launch for the whole ensemble:
-//   deposit   persistent: workgroups take (run, tile) items from a queue
+//   deposit   persistent over (run, tile) items, dealt by workgroup stride
 //   finalize  persistent over (run, node chunk): corner sums -> meshes, rho
 //   Poisson   one workgroup per run: the run's whole MG-preconditioned PCG
 //             (FP64 vectors, FP32 V(2,2) weighted-Jacobi multigrid, direct
@@ -88,30 +88,19 @@ __device__ __forceinline__ int find_run(const int* __restrict__ prefix, int R, i
     return lo;
 }
 
-// queue counters are cleared by a kernel (no memset nodes in the graph)
-__global__ void k_ens_reset(unsigned* q, int n) {
-    if ((int)threadIdx.x < n) q[threadIdx.x] = 0u;
-}
-
-// one item per workgroup at a time; returns -1 when the queue is empty
-__device__ __forceinline__ int take_item(unsigned* queue, int total, int* slot) {
-    __syncthreads();
-    if (threadIdx.x == 0) *slot = (int)atomicAdd(queue, 1u);
-    __syncthreads();
-    const int it = *slot;
-    return it < total ? it : -1;
-}
+// Items are dealt by workgroup stride: workgroup b of a persistent launch
+// takes items b, b + G, b + 2G, ... (G workgroups).  A shared queue counter
+// would balance uneven items better, but its same-address atomics serialise
+// at about 110 ns per item, which the ~20k items of a kernel cannot afford.
 
 // ----------------------------------------------------------------- deposit
 // items: (species, run, deposit tile); prefix over runs of the tiles of one species
 __global__ __launch_bounds__(kBlock) void k_ens_deposit(const EnsRun* __restrict__ runs, int R,
-                                                        const int* __restrict__ prefix, unsigned* queue) {
+                                                        const int* __restrict__ prefix) {
     __shared__ unsigned long long acc[8][kDepA];
-    __shared__ int slot;
     const int per_species = prefix[R];
-    while (true) {
-        const int it = take_item(queue, 2 * per_species, &slot);
-        if (it < 0) break;
+    for (int it = (int)blockIdx.x; it < 2 * per_species; it += (int)gridDim.x) {
+        __syncthreads();   // the previous item's LDS use is over
         const int s = it / per_species, k = it - s * per_species;
         const int r = find_run(prefix, R, k);
         const EnsRun& run = runs[r];
@@ -124,11 +113,9 @@ __global__ __launch_bounds__(kBlock) void k_ens_deposit(const EnsRun* __restrict
 // corner sums -> final/energy meshes of both species; rho = (0 + f0 c0) + f1 c1
 // (the serial step's order); items: (run, chunk of 256 nodes)
 __global__ __launch_bounds__(kBlock) void k_ens_finalize(const EnsRun* __restrict__ runs, int R,
-                                                         const int* __restrict__ prefix, unsigned* queue, int energy) {
-    __shared__ int slot;
-    while (true) {
-        const int it = take_item(queue, prefix[R], &slot);
-        if (it < 0) break;
+                                                         const int* __restrict__ prefix, int energy) {
+    for (int it = (int)blockIdx.x; it < prefix[R]; it += (int)gridDim.x) {
+        __syncthreads();   // the previous item's LDS use is over
         const int r = find_run(prefix, R, it);
         const EnsRun& run = runs[r];
         const int n = (it - prefix[r]) * kBlock + threadIdx.x;
@@ -386,11 +373,9 @@ __global__ __launch_bounds__(1024) void k_ens_poisson(const EnsRun* __restrict__
 
 // E = -grad phi with the reference's edge copies, packed gather array
 __global__ __launch_bounds__(kBlock) void k_ens_efield(const EnsRun* __restrict__ runs, int R,
-                                                       const int* __restrict__ prefix, unsigned* queue) {
-    __shared__ int slot;
-    while (true) {
-        const int it = take_item(queue, prefix[R], &slot);
-        if (it < 0) break;
+                                                       const int* __restrict__ prefix) {
+    for (int it = (int)blockIdx.x; it < prefix[R]; it += (int)gridDim.x) {
+        __syncthreads();   // the previous item's LDS use is over
         const int r = find_run(prefix, R, it);
         const EnsRun& run = runs[r];
         const int n = (it - prefix[r]) * kBlock + threadIdx.x;
@@ -412,16 +397,14 @@ __global__ __launch_bounds__(kBlock) void k_ens_efield(const EnsRun* __restrict_
 // items: (species, run, push tile)
 template <int TC>
 __global__ __launch_bounds__(kBlock) void k_ens_push(const EnsRun* __restrict__ runs, int R,
-                                                     const int* __restrict__ prefix, unsigned* queue) {
+                                                     const int* __restrict__ prefix) {
     __shared__ int hist[kHR * (TC + 2)];
     __shared__ int codes[kCodeCap];
     __shared__ unsigned long long nabs[2];
     __shared__ double wf[6 * (TC + 1)];
-    __shared__ int slot;
     const int per_species = prefix[R];
-    while (true) {
-        const int it = take_item(queue, 2 * per_species, &slot);
-        if (it < 0) break;
+    for (int it = (int)blockIdx.x; it < 2 * per_species; it += (int)gridDim.x) {
+        __syncthreads();   // the previous item's LDS use is over
         const int s = it / per_species, k = it - s * per_species;
         const int r = find_run(prefix, R, k);
         const EnsRun& run = runs[r];
@@ -478,12 +461,10 @@ __global__ __launch_bounds__(1024) void k_ens_scan(const EnsRun* __restrict__ ru
 // items: (species, run, chunk of kEnsChunk x 256 particles of the source
 // storage); each thread takes every 256th particle of the chunk
 __global__ __launch_bounds__(kBlock) void k_ens_scatter(const EnsRun* __restrict__ runs, int R,
-                                                        const int* __restrict__ prefix, unsigned* queue) {
-    __shared__ int slot;
+                                                        const int* __restrict__ prefix) {
     const int per_species = prefix[R];
-    while (true) {
-        const int it = take_item(queue, 2 * per_species, &slot);
-        if (it < 0) break;
+    for (int it = (int)blockIdx.x; it < 2 * per_species; it += (int)gridDim.x) {
+        __syncthreads();   // the previous item's LDS use is over
         const int s = it / per_species, k = it - s * per_species;
         const int r = find_run(prefix, R, k);
         const EnsRun& run = runs[r];
@@ -518,7 +499,6 @@ struct pic_ensemble {
     hipStream_t stream = nullptr;
     EnsRun* table[2] = {nullptr, nullptr};    // device, per parity
     int *dep_prefix = nullptr, *node_prefix = nullptr, *push_prefix = nullptr, *part_prefix = nullptr;
-    unsigned* queues = nullptr;
     // per-run Poisson state (device)
     int* warm = nullptr;
     int* iters = nullptr;
@@ -537,7 +517,6 @@ struct pic_ensemble {
 
 namespace {
 
-enum { Q_DEP = 0, Q_FIN, Q_EF, Q_PUSH, Q_SCAT, Q_N };
 
 void destroy_graphs(pic_ensemble* e) {
     for (int k = 0; k < 2; ++k) {
@@ -640,30 +619,24 @@ int fill_tables(pic_ensemble* e) {
 
 template <int TC>
 void launch_push(pic_ensemble* e, const EnsRun* tab, hipStream_t s) {
-    hipLaunchKernelGGL(k_ens_push<TC>, dim3(e->blocks), dim3(kBlock), 0, s, tab, e->R, e->push_prefix,
-                       e->queues + Q_PUSH);
+    hipLaunchKernelGGL(k_ens_push<TC>, dim3(e->blocks), dim3(kBlock), 0, s, tab, e->R, e->push_prefix);
 }
 
 // The whole step for the given parity on stream s.
 int enqueue_step(pic_ensemble* e, int par, hipStream_t s) {
     const EnsRun* tab = e->table[par];
-    hipLaunchKernelGGL(k_ens_reset, dim3(1), dim3(64), 0, s, e->queues, (int)Q_N);
-    hipLaunchKernelGGL(k_ens_deposit, dim3(e->blocks), dim3(kBlock), 0, s, tab, e->R, e->dep_prefix,
-                       e->queues + Q_DEP);
-    hipLaunchKernelGGL(k_ens_finalize, dim3(e->blocks), dim3(kBlock), 0, s, tab, e->R, e->node_prefix,
-                       e->queues + Q_FIN, e->energy);
+    hipLaunchKernelGGL(k_ens_deposit, dim3(e->blocks), dim3(kBlock), 0, s, tab, e->R, e->dep_prefix);
+    hipLaunchKernelGGL(k_ens_finalize, dim3(e->blocks), dim3(kBlock), 0, s, tab, e->R, e->node_prefix, e->energy);
     if (e->fdb) PIC_TRY(fast_batch_enqueue(e->fdb, s));
     hipLaunchKernelGGL(k_ens_poisson, dim3(e->R), dim3(1024), 0, s, tab);
-    hipLaunchKernelGGL(k_ens_efield, dim3(e->blocks), dim3(kBlock), 0, s, tab, e->R, e->node_prefix,
-                       e->queues + Q_EF);
+    hipLaunchKernelGGL(k_ens_efield, dim3(e->blocks), dim3(kBlock), 0, s, tab, e->R, e->node_prefix);
     switch (e->tile) {
     case 64: launch_push<64>(e, tab, s); break;
     case 128: launch_push<128>(e, tab, s); break;
     default: launch_push<256>(e, tab, s); break;
     }
     hipLaunchKernelGGL(k_ens_scan, dim3(2 * e->R), dim3(1024), 0, s, tab);
-    hipLaunchKernelGGL(k_ens_scatter, dim3(e->blocks), dim3(kBlock), 0, s, tab, e->R, e->part_prefix,
-                       e->queues + Q_SCAT);
+    hipLaunchKernelGGL(k_ens_scatter, dim3(e->blocks), dim3(kBlock), 0, s, tab, e->R, e->part_prefix);
     PIC_HIP(hipGetLastError());
     return PIC_OK;
 }
@@ -750,7 +723,6 @@ int pic_ensemble_create(pic_ctx** runs, int nruns, pic_ensemble** out) {
               hipMalloc(&e->node_prefix, (R + 1) * sizeof(int)) == hipSuccess &&
               hipMalloc(&e->push_prefix, (R + 1) * sizeof(int)) == hipSuccess &&
               hipMalloc(&e->part_prefix, (R + 1) * sizeof(int)) == hipSuccess &&
-              hipMalloc(&e->queues, 64 * sizeof(unsigned)) == hipSuccess &&
               hipMalloc(&e->warm, R * sizeof(int)) == hipSuccess && hipMalloc(&e->iters, R * sizeof(int)) == hipSuccess &&
               hipMalloc(&e->relres, 2 * R * sizeof(double)) == hipSuccess;
     if (!ok) {
@@ -806,7 +778,7 @@ void pic_ensemble_destroy(pic_ensemble* e) {
     destroy_graphs(e);
     fast_batch_free(e->fdb);
     void* ptrs[] = {e->table[0], e->table[1], e->dep_prefix, e->node_prefix, e->push_prefix, e->part_prefix,
-                    e->queues, e->warm, e->iters, e->relres};
+                    e->warm, e->iters, e->relres};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (double* w : e->pwork)

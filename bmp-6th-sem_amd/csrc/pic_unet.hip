@@ -823,11 +823,13 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
 // target and the output are the change of phi in units of max|r0| / diag,
 // diag = 2(1/dx^2 + 1/dy^2), so the map the network learns does not depend on
 // the solve's amplitude (the change is linear in r0).
-// scl[0] = max|r0| (as ordered bits), scl[1] = max|phi|.
+// Each workgroup's max|r0| and max|phi| go to mpart[2 b], mpart[2 b + 1];
+// k_unet_scale reduces them (one atomic per workgroup on two addresses would
+// serialise: 96 us at 1024^2).
 __global__ __launch_bounds__(kT) void k_unet_resid(const double* __restrict__ rhs, const double* __restrict__ phi,
                                                    const uint8_t* __restrict__ dir, int gx, int N, double cx,
                                                    double cy, double diag, double* __restrict__ r0,
-                                                   unsigned long long* __restrict__ scl) {
+                                                   double* __restrict__ mpart) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0, a = 0.0;
     if (p < N * N) {
@@ -854,14 +856,38 @@ __global__ __launch_bounds__(kT) void k_unet_resid(const double* __restrict__ rh
             m = fmax(m, sm[w]);
             a = fmax(a, sa[w]);
         }
-        atomicMax(scl, (unsigned long long)__double_as_longlong(m));
-        atomicMax(scl + 1, (unsigned long long)__double_as_longlong(a));
+        mpart[2 * blockIdx.x] = m;
+        mpart[2 * blockIdx.x + 1] = a;
     }
 }
 
-__global__ void k_zero_u64x2(unsigned long long* p) {
-    p[0] = 0ull;
-    p[1] = 0ull;
+// scl[0] = max|r0|, scl[1] = max|phi| (as bits) over the nblk partials; one
+// workgroup of kT threads.
+__global__ __launch_bounds__(kT) void k_unet_scale(const double* __restrict__ mpart, int nblk,
+                                                   unsigned long long* __restrict__ scl) {
+    double m = 0.0, a = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kT) {
+        m = fmax(m, mpart[2 * b]);
+        a = fmax(a, mpart[2 * b + 1]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        m = fmax(m, __shfl_down(m, off, 64));
+        a = fmax(a, __shfl_down(a, off, 64));
+    }
+    __shared__ double sm[kT / 64], sa[kT / 64];
+    if ((threadIdx.x & 63) == 0) {
+        sm[threadIdx.x >> 6] = m;
+        sa[threadIdx.x >> 6] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kT / 64; ++w) {
+            m = fmax(m, sm[w]);
+            a = fmax(a, sa[w]);
+        }
+        scl[0] = (unsigned long long)__double_as_longlong(m);
+        scl[1] = (unsigned long long)__double_as_longlong(a);
+    }
 }
 
 __device__ __forceinline__ double scl_get(const unsigned long long* scl, int k) {
@@ -966,6 +992,7 @@ struct pic_unet {
     bool own_stream = false;
     double* r0 = nullptr;                 // starting residual on the image (FP64)
     unsigned long long* scl = nullptr;    // per-solve scales (max|r0|, max|phi|)
+    double* mpart = nullptr;              // per-workgroup maxima for the scales
     float* wpart = nullptr;               // weight-gradient partials, one slot per row chunk
     float* bpart = nullptr;               // bias / head / loss partials, one slot per workgroup
     size_t wpart_n = 0, bpart_n = 0;
@@ -1468,6 +1495,7 @@ int pic_unet_create(int n, uint64_t seed, pic_unet** out) {
               hipMalloc(&u->gout, (size_t)n * n * 4) == hipSuccess &&
               hipMalloc(&u->count, 4) == hipSuccess && hipMalloc(&u->r0, (size_t)n * n * 8) == hipSuccess &&
               hipMalloc(&u->scl, 16) == hipSuccess &&
+              hipMalloc(&u->mpart, (size_t)nb((int64_t)n * n) * 2 * sizeof(double)) == hipSuccess &&
               hipMalloc(&u->dz, maxz * 2) == hipSuccess && hipMalloc(&u->dzt, maxz * 2) == hipSuccess &&
               hipMalloc(&u->xt, maxx * 2) == hipSuccess &&
               hipHostMalloc((void**)&u->loss_host, 4, hipHostMallocDefault) == hipSuccess;
@@ -1501,7 +1529,7 @@ void pic_unet_destroy(pic_unet* u) {
         if (x.g) (void)hipFree(x.g);
     }
     void* ptrs[] = {u->w, u->g, u->m, u->v, u->wb, u->wflip, u->out, u->tgt, u->gout, u->mask, u->loss, u->count,
-                    u->dz, u->dzt, u->xt, u->r0, u->scl, u->wpart, u->bpart};
+                    u->dz, u->dzt, u->xt, u->r0, u->scl, u->mpart, u->wpart, u->bpart};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (u->loss_host) (void)hipHostFree(u->loss_host);
@@ -1645,9 +1673,10 @@ static int unet_prepare(pic_ctx* c, pic_unet* u, const double* phi_old, bool tra
     const int N = u->N, gx = c->g.gx;
     hipStream_t s = c->stream;
     const double cx = 1.0 / (c->g.dx * c->g.dx), cy = 1.0 / (c->g.dy * c->g.dy);
-    hipLaunchKernelGGL(k_zero_u64x2, dim3(1), dim3(1), 0, s, u->scl);
-    hipLaunchKernelGGL(k_unet_resid, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, c->rho, phi_old, c->dir, gx, N, cx, cy,
-                       2 * (cx + cy), u->r0, u->scl);
+    const unsigned nblk = nb((int64_t)N * N);
+    hipLaunchKernelGGL(k_unet_resid, dim3(nblk), dim3(kT), 0, s, c->rho, phi_old, c->dir, gx, N, cx, cy,
+                       2 * (cx + cy), u->r0, u->mpart);
+    hipLaunchKernelGGL(k_unet_scale, dim3(1), dim3(kT), 0, s, u->mpart, (int)nblk, u->scl);
     hipLaunchKernelGGL(k_unet_input, dim3(nb((int64_t)N * N)), dim3(kT), 0, s, u->r0, phi_old, gx, N, u->scl,
                        u->t[X0C].p, train ? u->t[X0].p : (bf16*)nullptr);
     PIC_HIP(hipGetLastError());
