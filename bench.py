@@ -117,8 +117,13 @@ def learned_vs_plain(w, parts, layout, train, total, tol=1e-8):
     t0 = time.perf_counter()
     sp, sl = [], []
     for it in range(total):
+        # each context's step ends on the device before the other's begins
+        # (a step returns with its push and scatter still running on its
+        # stream): neither solve shares the GPU with the other's particles
         a = plain.step(it)
+        plain.sync()
         b = learn.step(it)
+        learn.sync()
         if it > train:
             sp.append(a)
             sl.append(b)

@@ -73,7 +73,7 @@ struct FastPoisson {
     int* qrow = nullptr;                     // P: row of q
     // sine transforms by FFT when m + 1 and n + 1 are powers of two (else GEMMs)
     int fft = 0, lm = 0, ln = 0;             // log2(2 (m + 1)), log2(2 (n + 1))
-    double2 *twm = nullptr, *twn = nullptr;  // exp(-2 pi i t / M), t < M, for both lengths
+    double2 *twm = nullptr, *twn = nullptr;  // FFT twiddles of both lengths, in the order the stages read them
     double* Fb = nullptr;                    // n x m: wall values moved to the right-hand side
     FdRun* self = nullptr;                   // device: this run's descriptor (a table of one)
     FdRun* hself = nullptr;                  // host copy
@@ -607,6 +607,13 @@ __global__ __launch_bounds__(kFdB) void k_fd_amat(int P, int n, int m, int k0, i
 static constexpr int kDstT = 256;          // threads per workgroup
 static constexpr int kDstPts = 2048;       // complex points per workgroup (at least)
 
+// LDS slot of point x: within each row of 16 slots (one pass over the 64
+// banks by 16 lanes of 16 bytes) the low four bits are XORed with 5 (row mod 4).
+// Contiguous runs of 16 points stay in one row; the radix-4 stores of the
+// first two stages (lane stride 4 slots, and runs of 4 at a row stride of 16)
+// land on 16 distinct slots per 16 lanes instead of 4.
+__device__ __forceinline__ int zsw(int x) { return x ^ (((x >> 4) & 3) * 5); }
+
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -798,13 +805,13 @@ __device__ __forceinline__ void dst_body(double2* lds, int R, int nwg, const Loa
             const int ra = r0 + 2 * b;
             const double a = ra < R ? va[it] : 0.0;
             const double c = ra + 1 < R ? vb[it] : 0.0;
-            z[b * M + j + 1] = make_double2(a, c);
-            z[b * M + M - 1 - j] = make_double2(-a, -c);
+            z[zsw(b * M + j + 1)] = make_double2(a, c);
+            z[zsw(b * M + M - 1 - j)] = make_double2(-a, -c);
         }
     }
     if (tid < NB) {
-        z[tid * M] = make_double2(0.0, 0.0);
-        z[tid * M + M / 2] = make_double2(0.0, 0.0);
+        z[zsw(tid * M)] = make_double2(0.0, 0.0);
+        z[zsw(tid * M + M / 2)] = make_double2(0.0, 0.0);
     }
     __syncthreads();
 #pragma unroll
@@ -819,10 +826,10 @@ __device__ __forceinline__ void dst_body(double2* lds, int R, int nwg, const Loa
             const int i = ig & (M / 4 - 1);
             const int k = i & (p - 1);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) u[t][r] = z[base + i + r * (M / 4)];
+            for (int r = 0; r < 4; ++r) u[t][r] = z[zsw(base + i + r * (M / 4))];
             if (s > 0) {
 #pragma unroll
-                for (int r = 1; r < 4; ++r) u[t][r] = cmul(u[t][r], twp[(r * k) << (L - 2 - 2 * s)]);
+                for (int r = 1; r < 4; ++r) u[t][r] = cmul(u[t][r], twp[r * p + k - 4]);
             }
             const double2 v0 = make_double2(u[t][0].x + u[t][2].x, u[t][0].y + u[t][2].y);
             const double2 v1 = make_double2(u[t][0].x - u[t][2].x, u[t][0].y - u[t][2].y);
@@ -838,7 +845,7 @@ __device__ __forceinline__ void dst_body(double2* lds, int R, int nwg, const Loa
 #pragma unroll
         for (int t = 0; t < NQ; ++t)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) z[dst[t] + r * p] = u[t][r];
+            for (int r = 0; r < 4; ++r) z[zsw(dst[t] + r * p)] = u[t][r];
         __syncthreads();
     }
     if (L & 1) {
@@ -848,10 +855,10 @@ __device__ __forceinline__ void dst_body(double2* lds, int R, int nwg, const Loa
             const int ig = tid + kDstT * t;
             const int base = (ig >> (L - 1)) << L;
             const int i = ig & (M / 2 - 1);
-            const double2 a = z[base + i];
-            const double2 b = cmul(z[base + i + M / 2], twp[i]);
-            z[base + i] = make_double2(a.x + b.x, a.y + b.y);
-            z[base + i + M / 2] = make_double2(a.x - b.x, a.y - b.y);
+            const double2 a = z[zsw(base + i)];
+            const double2 b = cmul(z[zsw(base + i + M / 2)], twp[M / 2 - 4 + i]);
+            z[zsw(base + i)] = make_double2(a.x + b.x, a.y + b.y);
+            z[zsw(base + i + M / 2)] = make_double2(a.x - b.x, a.y - b.y);
         }
         __syncthreads();
     }
@@ -873,7 +880,7 @@ __device__ __forceinline__ void dst_body(double2* lds, int R, int nwg, const Loa
         if (e < NE) {
             const int b = e / N, t = e - b * N;
             const int ra = r0 + 2 * b;
-            const double2 v = z[b * M + t + 1];
+            const double2 v = z[zsw(b * M + t + 1)];
             double ya = 0.0, yb = 0.0;
             if (ra < R) ya = st.put(t, ra, -v.y * sc, xa[it]);
             if (ra + 1 < R) yb = st.put(t, ra + 1, v.x * sc, xb[it]);
@@ -1263,14 +1270,25 @@ int fast_setup(pic_ctx* c, const std::vector<uint8_t>& dir, const std::vector<do
     f->fft = !per && c->p.poisson_direct == 1 && f->lm > 0 && f->ln > 0;
     std::vector<double2> twm, twn;
     if (f->fft) {
+        // w^e, w = exp(-2 pi i / M), in the order the FFT reads them: radix-4
+        // stage s (p = 4^s, s >= 1) takes w^((r k) 2^(L-2-2s)) for r = 1..3,
+        // k < p at r p + k - 4 (consecutive lanes read consecutive entries);
+        // a final radix-2 stage (odd L) takes w^i, i < M/2, at M/2 - 4 + i
         auto table = [](int L) {
             const int M = 1 << L;
             const long double pi = 3.141592653589793238462643383279502884L;
-            std::vector<double2> t(M);
-            for (int i = 0; i < M; ++i) {
-                const long double a = 2.0L * pi * (long double)i / (long double)M;
-                t[i] = make_double2((double)cosl(a), (double)-sinl(a));
+            auto w = [&](int e) {
+                const long double a = 2.0L * pi * (long double)e / (long double)M;
+                return make_double2((double)cosl(a), (double)-sinl(a));
+            };
+            std::vector<double2> t(M, make_double2(0.0, 0.0));
+            for (int st = 1; st < L / 2; ++st) {
+                const int p = 1 << (2 * st);
+                for (int r = 1; r < 4; ++r)
+                    for (int k = 0; k < p; ++k) t[r * p + k - 4] = w((r * k) << (L - 2 - 2 * st));
             }
+            if (L & 1)
+                for (int i = 0; i < M / 2; ++i) t[M / 2 - 4 + i] = w(i);
             return t;
         };
         twm = table(f->lm);
