@@ -1,4 +1,4 @@
-"""The build's five workloads at their stated sizes on the GPU, against the
+"""The build's workloads at their stated sizes on the GPU, against the
 oracle (pic_configs.py names them; the physics is the one described there):
 
 * plain_cpu_128 — 128^2 cells, 100k + 100k particles, Bz = 0.01 T, F64
@@ -15,6 +15,9 @@ oracle (pic_configs.py names them; the physics is the one described there):
   one step stage by stage with each field's error against the FP64 oracle and
   the deposit's error against the FP64 reference deposit, bounded by the
   fixed-point quantisation;
+* electrode_1024, throughput form — the same grid and particles with the
+  defaults the bench runs (2^-40 deposit, direct solve checked by the FP64
+  PCG): 50 steps, then one step stage by stage, phi against PARDISO;
 * the 64-run uneven ensemble (seed 4): three batched steps, four runs of
   different sizes checked stage by stage after every step.
 
@@ -198,6 +201,39 @@ def test_electrode_1024_precision_form_1000_steps(hip_mod, oracle_mod):
         print(f"electrode_1024 precision form, step {w.steps}: field errors vs the FP64 oracle {errs}; "
               f"refinement steps {mean_it[0]:.2f}, FP32 PCG iterations {mean_it[1]:.2f} per solve")
         assert errs["phi"] <= 1e-8 and errs["Ex"] <= 1e-6 and errs["Ey"] <= 1e-6, errs
+    finally:
+        pref.close()
+        ctx.close()
+
+
+@pytest.mark.timeout(600)
+def test_electrode_1024_throughput_form_stage_by_stage(hip_mod, oracle_mod):
+    """The throughput workload exactly as the bench steps it: electrode_1024
+    (1024^2 cells, 2^24 + 2^24 particles, CELL32, 2^-40 deposit, B tilted 30
+    deg), the direct solve checked by the FP64 PCG at 1e-10 (Chebyshev
+    smoothing, previous-phi guess).  50 steps, every one leaving the PCG no
+    iteration, with property checks; then one step stage by stage against the
+    oracle (deposit, rho, E and push bit for bit) and phi against PARDISO."""
+    hip, O = hip_mod, oracle_mod
+    cfg, w = _workload("electrode_1024")
+    assert (w.NX, w.n_per_species, w.layout) == (1024, 1 << 24, 1)
+    tol, steps = 1e-10, 50
+    ctx, _ = _context(cfg, w, poisson_tol=tol, poisson_guess=0, poisson_precision=0, mg_smoother=1,
+                      capacity_factor=1.05)
+    pref = SC.PoissonRef(O, ctx.gx, ctx.gy, ctx.dx, ctx.dy, w.rects())
+    spec = SC.species_table(w, cfg.N0)
+    try:
+        for it in range(steps):
+            st = ctx.step(it)
+            assert st["iterations"] == 0, (it, st)
+            if it % 10 == 9 or it < 2:
+                _properties(hip, ctx, st, tol, full=(it == steps - 1))
+        before, ab = SC.snapshot(ctx, 2)
+        st = ctx.step(steps)
+        assert st["iterations"] == 0 and st["final_relres"] <= tol, st
+        errs = SC.check_step(hip, O, ctx, before, ab, spec, pref)
+        print(f"electrode_1024 throughput form, step {steps}: field errors vs PARDISO {errs}")
+        assert errs["phi"] <= 1e-10 and errs["Ex"] <= 1e-8 and errs["Ey"] <= 1e-8, errs
     finally:
         pref.close()
         ctx.close()
