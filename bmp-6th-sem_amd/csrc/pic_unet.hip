@@ -713,11 +713,6 @@ __global__ __launch_bounds__(kT) void k_head_bwd(const float* __restrict__ gout,
     }
 }
 
-__global__ void k_zero_f32(float* __restrict__ p, int64_t n) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) p[k] = 0.f;
-}
-
 // out[j] = the sum over slots of part[slot * len + j] in a fixed order: wave w
 // of a workgroup (64 outputs) adds slots w, w + 4, w + 8, ... in four
 // interleaved partial sums (a, b, c, d over slots w + 16k, w + 4 + 16k, ...),
