@@ -182,6 +182,7 @@ struct SolveTail {
     int active = 0;
     bool direct = false, mixed = false, learned = false, extrap_keep = false;
     int guess = 0;
+    bool picked = false;            // the device chose between a learned guess and phi_{n-1}
     hipEvent_t ev = nullptr;        // stage timer start
 };
 

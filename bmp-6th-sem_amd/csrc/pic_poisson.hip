@@ -51,7 +51,8 @@ static constexpr int kB = 256;
 static constexpr int kNP = PIC_PART_BLOCKS;  // blocks of the reduction kernels (partials)
 
 // device scalars
-enum Scal { S_RR = 0, S_BB, S_RR0, S_RZ, S_ALPHA, S_BETA, S_G0, S_G1, S_IRR, S_IBB, S_IRR0, S_N };
+enum Scal { S_RR = 0, S_BB, S_RR0, S_RZ, S_ALPHA, S_BETA, S_G0, S_G1, S_IRR, S_IBB, S_IRR0, S_GPICK, S_N };
+static_assert(S_N <= PIC_SCAL_DOUBLES, "the device scalars fit the state block");
 // partial-sum arrays, kNP doubles each
 enum Part { P_RR = 0, P_BB, P_PQ, P_RZ, P_RZO, P_N };
 
@@ -134,14 +135,15 @@ __global__ void k_overwrite_phi(int G, int gx, double* phi, const uint8_t* dir, 
 // x <- guess (or 0) with Dirichlet values imposed; `keep` (optional) receives
 // the previous x.  With `prev` the guess is 2 x - prev (x, prev: the last two
 // solutions).
+// With `pick`, src is taken only where *pick is set (else x stays).
 __global__ void k_init_guess(int G, double* x, const double* src, const double* prev, double* keep,
-                             const uint8_t* dir, const double* dval) {
+                             const uint8_t* dir, const double* dval, const double* pick) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= G) return;
     const double cur = x[n];
     double v = 0.0;
     if (prev) v = 2.0 * cur - prev[n];
-    else if (src) v = src[n];
+    else if (src) v = (pick && !(*pick > 0.5)) ? cur : src[n];
     if (keep) keep[n] = cur;
     x[n] = dir[n] ? dval[n] : v;
 }
@@ -370,7 +372,8 @@ __global__ __launch_bounds__(1024) void k_beta(const double* part_rz, const doub
     }
 }
 
-// scal[S_G0], scal[S_G1] = sums of two partial arrays (residuals of two guesses)
+// scal[S_G0], scal[S_G1] = sums of two partial arrays (squared residuals of
+// two guesses); scal[S_GPICK] = 1 where the second is lower (else 0)
 __global__ __launch_bounds__(1024) void k_sum2(const double* pa, const double* pb, double* scal) {
     __shared__ double sh[16];
     const double a = sum_partials(pa, sh);
@@ -378,6 +381,7 @@ __global__ __launch_bounds__(1024) void k_sum2(const double* pa, const double* p
     if (threadIdx.x == 0) {
         scal[S_G0] = a;
         scal[S_G1] = b;
+        scal[S_GPICK] = b < a ? 1.0 : 0.0;
     }
 }
 
@@ -1487,7 +1491,10 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
         }
         if (extrap) keep = c->phi_prev;
     }
-    // learned guess: keep phi_{n-1} for training; try the prediction against it
+    // learned guess: keep phi_{n-1} for training; try the prediction against
+    // it (the device compares the two starting residuals and keeps the lower;
+    // the host learns the choice with the solve's state)
+    const double* pick = nullptr;
     if (learned) {
         PIC_HIP(hipMemcpyAsync(c->phi_prev, c->phi, (size_t)G * sizeof(double), hipMemcpyDeviceToDevice, s));
         if (c->learn_trained >= c->learn_min) {
@@ -1497,13 +1504,10 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
             hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->guess, c->dir, R[1],
                                part + P_RZO * kNP);
             hipLaunchKernelGGL(k_sum2, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_RZO * kNP, c->scal);
-            PIC_TRY(read_state(c));
             c->learn_tried += 1;
-            if (c->scal_host[S_G1] < c->scal_host[S_G0]) {
-                src = c->guess;
-                guess = 4;
-                c->learn_used += 1;
-            }
+            src = c->guess;
+            pick = c->scal + S_GPICK;
+            guess = 4;
         }
     }
     // direct solve: the PCG below starts from it and only checks (or
@@ -1526,7 +1530,7 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
         }
     } else {
         hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, src, prev, keep, c->dir,
-                           c->dval);
+                           c->dval, pick);
         hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, R[0],
                            part + P_RR * kNP);
     }
@@ -1547,6 +1551,7 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
     tail.learned = learned;
     tail.extrap_keep = extrap && keep;
     tail.guess = guess;
+    tail.picked = pick != nullptr;
     tail.ev = ev;
     if (direct) {
         // The host waits for the check's publication only, not for the work
@@ -1616,7 +1621,7 @@ static int solve_finish(pic_ctx* c, const SolveTail& t, pic_solve_stats* st, int
     double* const P[2] = {c->pp, c->pp2};
     float* const Z[2] = {c->zf0, c->zf1};
     const bool mixed = t.mixed, learned = t.learned;
-    const int guess = t.guess;
+    int guess = t.guess;
     bool fields_done = false;
     double direct_ms = 0.0;
     if (t.direct) {
@@ -1702,6 +1707,11 @@ static int solve_finish(pic_ctx* c, const SolveTail& t, pic_solve_stats* st, int
     if (!fields_done) {
         PIC_TRY(timer_end(c, PIC_STAGE_POISSON, ev));
         PIC_TRY(launch_efield(c));
+    }
+    if (t.picked) {
+        // the device's choice between the prediction and phi_{n-1}
+        if (c->scal_host[S_GPICK] > 0.5) c->learn_used += 1;
+        else guess = 1;
     }
     if (st) {
         st->iterations = iters;
