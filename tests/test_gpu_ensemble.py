@@ -3,7 +3,8 @@ per step) against serial runs of the same contexts.  Runs whose Poisson solve
 is a batched direct solve match their serial twins bit for bit on every step
 (rho, phi, live and per-cell counts: the same integer deposit, the same
 transform passes, the same push); runs the ensemble's one-workgroup PCG
-solves match to 1e-9 (another PCG implementation, both at 1e-12).  Graph
+solves match to 1e-9 (another PCG implementation, both at 1e-12) and are
+checked stage by stage against the oracle on their own inputs.  Graph
 replay is identical to direct launches; the autotuner's choice is valid;
 mixed ensembles (two batched groups, a GEMM-transform grid and a PCG-only
 run)."""
@@ -108,14 +109,19 @@ def test_ensemble_graph_equals_direct_launches_and_tuner(hip_mod):
         c.close()
 
 
-def test_ensemble_mixed_direct_batches_and_pcg_runs(hip_mod):
+def test_ensemble_mixed_direct_batches_and_pcg_runs(hip_mod, oracle_mod):
     """Runs of one ensemble solved different ways: batched direct solves in two
     groups of the same grid (the default electrode and a second geometry), a
     96-cell grid (sine transforms by GEMM: not batched, the ensemble's PCG
     solves it) and a run with the direct solve turned off.  Every run against
-    its own serial steps, as above; the batched runs need no PCG iteration."""
+    its own serial steps, as above; the batched runs need no PCG iteration.
+    The two PCG-solved runs are also checked stage by stage against the oracle
+    on their own inputs (deposit, rho, E, push and absorbed counts bit for bit,
+    phi against PARDISO): where their counts differ from the serial twins', the
+    only cause is phi's rounding."""
     import pic_configs as cfg
-    hip = hip_mod
+    import stage_check as SC
+    hip, O = hip_mod, oracle_mod
     runs = cfg.ensemble(nruns=6, seed=12, grids=(64, 96), n_min=5e3, n_max=4e4)
     runs[0] = cfg.Workload("mix0", 64, 64, 20000, 12000, 0.03, 0.0, cfg.ELECTRODE, electrode_v=50.0, layout=1)
     runs[1] = cfg.Workload("mix1", 64, 64, 15000, 12001, 0.02, 0.0, cfg.ELECTRODE, electrode_v=80.0, layout=1)
@@ -137,8 +143,15 @@ def test_ensemble_mixed_direct_batches_and_pcg_runs(hip_mod):
 
     ens_ctx, ser_ctx = build(), build()
     ens = hip.Ensemble(ens_ctx)
+    pcg_runs = (2, 3)
+    prefs = {k: SC.PoissonRef(O, ens_ctx[k].gx, ens_ctx[k].gy, ens_ctx[k].dx, ens_ctx[k].dy, runs[k].rects())
+             for k in pcg_runs}
     for it in range(2):
+        snaps = {k: SC.snapshot(ens_ctx[k], 2) for k in pcg_runs}
         ens.step(1, graph=True)
+        for k in pcg_runs:
+            errs = SC.check_step(hip, O, ens_ctx[k], *snaps[k], SC.species_table(runs[k], cfg.N0), prefs[k])
+            assert errs["phi"] <= 1e-10, (it, k, errs)
         for c in ser_ctx:
             c.step(it)
         iters, rr = ens.poisson_stats()
@@ -163,6 +176,8 @@ def test_ensemble_mixed_direct_batches_and_pcg_runs(hip_mod):
                 assert np.abs(pa - pb).max() <= 1e-9 * np.abs(pb).max(), (it, k)
                 for s in range(2):
                     assert abs(a.count(s) - b.count(s)) <= 2, (it, k, s)
+    for pr in prefs.values():
+        pr.close()
     ens.close()
     for c in ens_ctx + ser_ctx:
         c.close()
