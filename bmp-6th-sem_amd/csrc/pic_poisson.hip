@@ -135,15 +135,14 @@ __global__ void k_overwrite_phi(int G, int gx, double* phi, const uint8_t* dir, 
 // x <- guess (or 0) with Dirichlet values imposed; `keep` (optional) receives
 // the previous x.  With `prev` the guess is 2 x - prev (x, prev: the last two
 // solutions).
-// With `pick`, src is taken only where *pick is set (else x stays).
 __global__ void k_init_guess(int G, double* x, const double* src, const double* prev, double* keep,
-                             const uint8_t* dir, const double* dval, const double* pick) {
+                             const uint8_t* dir, const double* dval) {
     int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= G) return;
     const double cur = x[n];
     double v = 0.0;
     if (prev) v = 2.0 * cur - prev[n];
-    else if (src) v = (pick && !(*pick > 0.5)) ? cur : src[n];
+    else if (src) v = src[n];
     if (keep) keep[n] = cur;
     x[n] = dir[n] ? dval[n] : v;
 }
@@ -166,6 +165,76 @@ __global__ __launch_bounds__(kB) void k_residual(Fine f, const double* __restric
         acc += v * v;
     }
     write_partial(acc, part, sh);
+}
+
+// The learned guess and the starting residuals of both candidates in one
+// pass over the grid (the grid, loop and partial sums of k_residual): with
+// Dirichlet nodes at their values dval, guess = phi + out * (max|r0| / diag)
+// on unknown nodes of the N x N image, phi elsewhere; r_phi = rhs - A phi
+// into r0 with the partials of |r_phi|^2, r_guess = rhs - A guess into r1
+// with those of |r_guess|^2 (the guess at the stencil's nodes formed from phi
+// and out on the spot).  Non-periodic grids (the learned guess's).
+__global__ __launch_bounds__(kB) void k_guess_residuals(Fine f, const double* __restrict__ rhs,
+                                                        const double* __restrict__ phi,
+                                                        const uint8_t* __restrict__ dir,
+                                                        const double* __restrict__ dval,
+                                                        const float* __restrict__ out,
+                                                        const unsigned long long* __restrict__ scl, int N,
+                                                        double diag, double* __restrict__ guess,
+                                                        double* __restrict__ r0, double* __restrict__ r1,
+                                                        double* part0, double* part1) {
+    __shared__ double sh[kB / 64];
+    const double m0 = __longlong_as_double((long long)scl[0]);
+    const double sc = (m0 > 0.0 ? m0 : 1.0) / diag;
+    auto pv = [&](int n) -> double { return dir[n] ? dval[n] : phi[n]; };
+    auto gv = [&](int n, int x, int y) -> double {
+        if (dir[n]) return dval[n];
+        double v = phi[n];
+        if (x < N && y < N) v += (double)out[y * N + x] * sc;
+        return v;
+    };
+    double acc0 = 0, acc1 = 0;
+    const int G = f.gx * f.gy;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < G; n += gridDim.x * blockDim.x) {
+        const int y = n / f.gx, x = n - y * f.gx;
+        double v0 = 0, v1 = 0;
+        const double g = gv(n, x, y);
+        if (!dir[n]) {
+            const double a0 = f.diag * phi[n] - f.cx * (pv(n - 1) + pv(n + 1)) - f.cy * (pv(n - f.gx) + pv(n + f.gx));
+            v0 = rhs[n] - a0;
+            const double a1 = f.diag * g - f.cx * (gv(n - 1, x - 1, y) + gv(n + 1, x + 1, y))
+                            - f.cy * (gv(n - f.gx, x, y - 1) + gv(n + f.gx, x, y + 1));
+            v1 = rhs[n] - a1;
+        }
+        guess[n] = g;
+        r0[n] = v0;
+        r1[n] = v1;
+        acc0 += v0 * v0;
+        acc1 += v1 * v1;
+    }
+    write_partial(acc0, part0, sh);
+    write_partial(acc1, part1, sh);
+}
+
+// Where the device picked the learned guess (*pick set by k_sum2): phi and
+// r0 take the guess and its residual, and its partials replace phi's; else
+// phi keeps its values with the Dirichlet nodes at dval (as r0 assumed).
+__global__ __launch_bounds__(kB) void k_take_guess(int G, double* __restrict__ phi, double* __restrict__ r0,
+                                                   const double* __restrict__ guess, const double* __restrict__ r1,
+                                                   double* __restrict__ part0, const double* __restrict__ part1,
+                                                   const uint8_t* __restrict__ dir, const double* __restrict__ dval,
+                                                   const double* __restrict__ pick) {
+    const bool take = *pick > 0.5;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int n = k; n < G; n += gridDim.x * blockDim.x) {
+        if (take) {
+            phi[n] = guess[n];
+            r0[n] = r1[n];
+        } else if (dir[n]) {
+            phi[n] = dval[n];
+        }
+    }
+    if (take && k < kNP) part0[k] = part1[k];
 }
 
 // k_residual after the direct solve, with overwritePhi in the same pass:
@@ -1491,22 +1560,32 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
         }
         if (extrap) keep = c->phi_prev;
     }
-    // learned guess: keep phi_{n-1} for training; try the prediction against
-    // it (the device compares the two starting residuals and keeps the lower;
-    // the host learns the choice with the solve's state)
-    const double* pick = nullptr;
+    // learned guess (in place of an extrapolated one): keep phi_{n-1} while
+    // pairs are trained on; once the network has trained enough, form the
+    // prediction and both starting residuals in one pass, and let the device
+    // keep the lower (the host learns the choice with the solve's state)
+    bool picked = false;
     if (learned) {
-        PIC_HIP(hipMemcpyAsync(c->phi_prev, c->phi, (size_t)G * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (prev) {
+            prev = nullptr;
+            src = c->phi;
+            guess = 1;
+        }
+        keep = nullptr;
+        if (c->learn_trained < c->learn_train)
+            PIC_HIP(hipMemcpyAsync(c->phi_prev, c->phi, (size_t)G * sizeof(double), hipMemcpyDeviceToDevice, s));
         if (c->learn_trained >= c->learn_min) {
-            PIC_TRY(unet_guess(c, c->unet, c->guess));
-            hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, R[0],
-                               part + P_RR * kNP);
-            hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->guess, c->dir, R[1],
+            UnetPrediction pr{};
+            PIC_TRY(unet_predict(c, c->unet, &pr));
+            const double diag = 2.0 / (c->g.dx * c->g.dx) + 2.0 / (c->g.dy * c->g.dy);
+            hipLaunchKernelGGL(k_guess_residuals, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, c->dval,
+                               pr.out, pr.scl, pr.N, diag, c->guess, R[0], R[1], part + P_RR * kNP,
                                part + P_RZO * kNP);
             hipLaunchKernelGGL(k_sum2, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, part + P_RZO * kNP, c->scal);
+            hipLaunchKernelGGL(k_take_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, R[0], c->guess, R[1],
+                               part + P_RR * kNP, part + P_RZO * kNP, c->dir, c->dval, c->scal + S_GPICK);
             c->learn_tried += 1;
-            src = c->guess;
-            pick = c->scal + S_GPICK;
+            picked = true;
             guess = 4;
         }
     }
@@ -1528,9 +1607,9 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
             efield_done = true;
             c->e_stale = 1;
         }
-    } else {
+    } else if (!picked) {
         hipLaunchKernelGGL(k_init_guess, dim3(nblk(G)), dim3(kB), 0, s, G, c->phi, src, prev, keep, c->dir,
-                           c->dval, pick);
+                           c->dval);
         hipLaunchKernelGGL(k_residual, dim3(kNP), dim3(kB), 0, s, f, c->rho, c->phi, c->dir, R[0],
                            part + P_RR * kNP);
     }
@@ -1551,7 +1630,7 @@ int poisson_solve(pic_ctx* c, int iteration, pic_solve_stats* st, int defer) {
     tail.learned = learned;
     tail.extrap_keep = extrap && keep;
     tail.guess = guess;
-    tail.picked = pick != nullptr;
+    tail.picked = picked;
     tail.ev = ev;
     if (direct) {
         // The host waits for the check's publication only, not for the work

@@ -922,19 +922,6 @@ __global__ void k_unet_target(const double* __restrict__ phi_new, const double* 
     mask[p] = dir[n] ? 0 : 1;
 }
 
-// guess = phi_old + out max|r0| / diag on unknown nodes of the image; the rest
-// of the grid keeps phi_old (Dirichlet values are imposed by the solver).
-__global__ void k_unet_guess(const double* __restrict__ phi_old, const float* __restrict__ out,
-                             const uint8_t* __restrict__ dir, int gx, int gy, int N, double diag,
-                             const unsigned long long* __restrict__ scl, double* __restrict__ guess) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= gx * gy) return;
-    const int y = n / gx, x = n - y * gx;
-    double v = phi_old[n];
-    if (x < N && y < N && !dir[n]) v += (double)out[y * N + x] * (scl_get(scl, 0) / diag);
-    guess[n] = v;
-}
-
 // ================================================================= host side
 namespace {
 
@@ -1678,19 +1665,16 @@ static int unet_prepare(pic_ctx* c, pic_unet* u, const double* phi_old, bool tra
     return PIC_OK;
 }
 
-int unet_guess(pic_ctx* c, pic_unet* u, double* guess) {
-    const int N = u->N, gx = c->g.gx;
-    hipStream_t s = c->stream;
+int unet_predict(pic_ctx* c, pic_unet* u, UnetPrediction* o) {
     PIC_TRY(unet_prepare(c, u, c->phi, false));
     hipStream_t keep = u->stream;
-    u->stream = s;
+    u->stream = c->stream;
     const int rc = unet_forward(u, false);
     u->stream = keep;
     PIC_TRY(rc);
-    const double diag = 2.0 / (c->g.dx * c->g.dx) + 2.0 / (c->g.dy * c->g.dy);
-    hipLaunchKernelGGL(k_unet_guess, dim3(nb(c->G)), dim3(kT), 0, s, c->phi, u->out, c->dir, gx, c->g.gy, N, diag,
-                       u->scl, guess);
-    PIC_HIP(hipGetLastError());
+    o->out = u->out;
+    o->scl = u->scl;
+    o->N = u->N;
     return PIC_OK;
 }
 
