@@ -1426,21 +1426,20 @@ static int read_state(pic_ctx* c) {
 // Flexible PCG from the residual already in R[0] (type T; its check already
 // done): z0 = M r0, then iterations until flags[0] is set or max_it.  x, R, P,
 // q are T; z is FP32 (the V-cycle output); dots and scalars are FP64.  The
-// host reads the flag between batches (the first as long as *batch_hint).
+// host reads the flag between batches (the first as long as *batch_hint); a
+// start that already meets the tolerance is found by the device (every kernel
+// returns at once once flags[0] is set), so nothing is read before the first
+// batch.  count0: the device's iteration counter flags[1] before this run.
 template <typename T>
 static int run_pcg(pic_ctx* c, T* x, T* const R[2], T* const P[2], T* q, float* const Z[2], int* flags,
-                   double tol2, int sbb, int srr, int srr0, int max_it, int* batch_hint) {
+                   double tol2, int sbb, int srr, int srr0, int max_it, int* batch_hint, int count0) {
     const Fine f = fine_of(c);
     hipStream_t s = c->stream;
     double* part = c->part;
     MgLevel& m0 = c->mg[0];
     const Lvl l0 = lvl_pre(c, m0);
     const int flag_off = (int)(flags - c->flags);
-    // the starting residual may already satisfy the tolerance (a direct solve,
-    // a converged warm start): then nothing more is enqueued
-    PIC_TRY(read_state(c));
-    const int count0 = c->flags_host[flag_off + 1];
-    if (c->flags_host[flag_off] || max_it <= 0) {
+    if (max_it <= 0) {
         *batch_hint = 0;
         return PIC_OK;
     }
@@ -1716,8 +1715,9 @@ static int solve_finish(pic_ctx* c, const SolveTail& t, pic_solve_stats* st, int
     if (fields_done) {
         // the direct solution met the tolerance: no PCG iteration
     } else if (!mixed) {
+        // the solve's first kernel cleared the iteration counter
         PIC_TRY(run_pcg<double>(c, c->phi, R, P, c->pq, Z, c->flags, tol2, S_BB, S_RR, S_RR0, c->p.poisson_max_iter,
-                                &c->last_iters));
+                                &c->last_iters, 0));
         iters = c->flags_host[1];
         inner_total = iters;
     } else {
@@ -1736,8 +1736,9 @@ static int solve_finish(pic_ctx* c, const SolveTail& t, pic_solve_stats* st, int
                                eta * eta, c->flags + 2, c->scal, 1, S_IBB, S_IRR, S_IRR0);
             hipLaunchKernelGGL(k_zero_f32v, dim3(nblk(G)), dim3(kB), 0, s, G, c->df);
             PIC_HIP(hipGetLastError());
+            // the inner counter flags[3] as read at the top of this step
             PIC_TRY(run_pcg<float>(c, c->df, RF, PF, c->qf, Z, c->flags + 2, eta * eta, S_IBB, S_IRR, S_IRR0,
-                                   c->p.poisson_max_iter, &c->last_inner));
+                                   c->p.poisson_max_iter, &c->last_inner, c->flags_host[3]));
             if (c->flags_host[2] == 2) {
                 set_error("Poisson inner FP32 PCG diverged (NaN residual)");
                 return PIC_ERR_SOLVER;
