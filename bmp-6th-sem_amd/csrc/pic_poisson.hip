@@ -14,8 +14,9 @@
 // One PCG iteration k is, on the fine grid:
 //   k_apply_p     p_k = z_{k-1} + beta p_{k-1} (formed at the node and its
 //                 neighbours), q = A p_k, partials of p.q
-//   k_alpha       alpha = (r.z)_{k-1} / (p.q)
-//   k_update_pre2 x += alpha p; r_k = r_{k-1} - alpha q (node and neighbours);
+//   k_update_pre2 alpha = (r.z)_{k-1} / (p.q) (each workgroup adds the p.q
+//                 partials itself); x += alpha p; r_k = r_{k-1} - alpha q
+//                 (node and neighbours);
 //                 the V-cycle's two pre-smoothing sweeps from zero on r_k in
 //                 the same pass; partials of r.r
 //   k_check       convergence flag
@@ -26,8 +27,8 @@
 //   k_beta        Polak-Ribiere beta = (r.z_k - r.z_{k-1}) / (r.z)_{k-1}
 // r, p and z alternate between two buffers so that no kernel overwrites what a
 // neighbouring workgroup still reads.  Dot products leave one partial sum per
-// block; a one-block kernel re-adds them in a fixed order (deterministic, no
-// atomics).  Convergence is a device flag: once set, every kernel of the
+// block; a one-block kernel (for p.q, every workgroup of the next kernel)
+// re-adds them in a fixed order (deterministic, no atomics).  Convergence is a device flag: once set, every kernel of the
 // remaining enqueued iterations returns at once, and the host reads the flag
 // only between batches of iterations.  Levels of at most 65 x 65 nodes run
 // inside one workgroup's LDS.
@@ -53,6 +54,7 @@ static constexpr int kNP = PIC_PART_BLOCKS;  // blocks of the reduction kernels 
 // device scalars
 enum Scal { S_RR = 0, S_BB, S_RR0, S_RZ, S_ALPHA, S_BETA, S_G0, S_G1, S_IRR, S_IBB, S_IRR0, S_GPICK, S_N };
 static_assert(S_N <= PIC_SCAL_DOUBLES, "the device scalars fit the state block");
+static_assert(kNP % kB == 0, "a kB-thread block adds kNP / kB partials per thread");
 // partial-sum arrays, kNP doubles each
 enum Part { P_RR = 0, P_BB, P_PQ, P_RZ, P_RZO, P_N };
 
@@ -420,14 +422,6 @@ __global__ __launch_bounds__(kB) void k_apply_p(Fine f, const float* __restrict_
     write_partial(acc, part_pq, sh);
 }
 
-// alpha = (r.z) / (p.q)
-__global__ __launch_bounds__(1024) void k_alpha(const double* part_pq, double* scal, const int* flags) {
-    __shared__ double sh[16];
-    if (flags[0]) return;
-    const double pq = sum_partials(part_pq, sh);
-    if (threadIdx.x == 0) scal[S_ALPHA] = scal[S_RZ] / pq;
-}
-
 // beta = (r.z - r.zold) / (r.z)_prev (Polak-Ribiere; first: 0); keeps r.z.
 __global__ __launch_bounds__(1024) void k_beta(const double* part_rz, const double* part_rzo, double* scal,
                                                const int* flags, int first) {
@@ -472,11 +466,18 @@ __global__ __launch_bounds__(kB) void k_update_pre2(Fine f, Lvl L, T* __restrict
                                                     const T* __restrict__ p, const T* __restrict__ rold,
                                                     T* __restrict__ rnew, const T* __restrict__ q,
                                                     const uint8_t* __restrict__ dir, float* __restrict__ b0,
-                                                    float* __restrict__ e0, double* part_rr, const double* scal,
-                                                    const int* flags) {
+                                                    float* __restrict__ e0, double* part_rr, double* scal,
+                                                    const double* __restrict__ part_pq, const int* flags) {
     __shared__ double sh[kB / 64];
     if (flags[0]) return;
-    const double alpha = scal[S_ALPHA];
+    // alpha = r.z / p.q, every workgroup adding the p.q partials in the same
+    // fixed order (workgroup 0 also keeps it in the state block)
+    double pq = 0.0;
+#pragma unroll
+    for (int k = 0; k < kNP / kB; ++k) pq += part_pq[threadIdx.x + k * kB];
+    pq = block_sum_all(pq, sh);
+    const double alpha = scal[S_RZ] / pq;
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_ALPHA] = alpha;
     double acc = 0;
     const int G = f.gx * f.gy;
     auto R = [&](int m) -> double { return (double)rold[m] - alpha * (double)q[m]; };
@@ -1466,13 +1467,12 @@ static int run_pcg(pic_ctx* c, T* x, T* const R[2], T* const P[2], T* q, float* 
             const int cur = it & 1, old = cur ^ 1;
             hipLaunchKernelGGL(k_apply_p<T>, dim3(kNP), dim3(kB), 0, s, f, Z[old], P[old], P[cur], c->dir, q,
                                part + P_PQ * kNP, c->scal, flags, it == 1 ? 1 : 0);
-            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1024), 0, s, part + P_PQ * kNP, c->scal, flags);
             if (c->p.mg_pre == 2)
                 hipLaunchKernelGGL((k_update_pre2<1, T>), dim3(kNP), dim3(kB), 0, s, f, l0, x, P[cur], R[old], R[cur],
-                                   q, c->dir, m0.b, m0.e, part + P_RR * kNP, c->scal, flags);
+                                   q, c->dir, m0.b, m0.e, part + P_RR * kNP, c->scal, part + P_PQ * kNP, flags);
             else
                 hipLaunchKernelGGL((k_update_pre2<0, T>), dim3(kNP), dim3(kB), 0, s, f, l0, x, P[cur], R[old], R[cur],
-                                   q, c->dir, m0.b, m0.e, part + P_RR * kNP, c->scal, flags);
+                                   q, c->dir, m0.b, m0.e, part + P_RR * kNP, c->scal, part + P_PQ * kNP, flags);
             hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, part + P_RR * kNP, (const double*)nullptr, tol2,
                                flags, c->scal, 0, sbb, srr, srr0);
             if (c->p.mg_pre != 2) presmooth(c, m0, flags);
