@@ -160,6 +160,30 @@ def test_learned_guess_in_the_solver(hip_mod):
     b.close()
 
 
+def test_learned_guess_replaces_the_extrapolated_one(hip_mod):
+    """With poisson_guess = 1 (linear extrapolation) and the learned guess on,
+    the learned guess is the one offered once the network has trained (the
+    extrapolation no longer overrides it): the guess is tried and used, and
+    every solve reaches the same potential as a plain run at 1e-8."""
+    import pic_configs as cfg
+    w = cfg.WORKLOADS["electrode_512"].scaled(NX=256, n=100_000, name="learned_extrap_256")
+    parts = [cfg.generate(w, s) for s in range(2)]
+    a = cfg.make_context(w, particles=parts, poisson_tol=1e-8, poisson_direct=0)
+    b = cfg.make_context(w, particles=parts, poisson_tol=1e-8, poisson_direct=0, poisson_guess=1)
+    b.learned_enable(train_steps=12, min_trained=8, lr=1e-3, seed=2)
+    for it in range(20):
+        a.step(it)
+        sb = b.step(it)
+        assert sb["final_relres"] <= 1e-8, (it, sb)
+        pa, pb = a.grid(hip_mod.GRID_PHI), b.grid(hip_mod.GRID_PHI)
+        assert np.abs(pa - pb).max() <= 1e-6 * np.abs(pa).max(), it
+    st = b.learned_stats()
+    print("learned stats with extrapolation requested:", st)
+    assert st["tried"] == 20 - 1 - 8 and st["used"] >= 1
+    a.close()
+    b.close()
+
+
 def test_unet_residual_loss_gradients_match_oracle(hip_mod):
     import unet_oracle as U
     net = hip_mod.UNet(N, seed=10)
