@@ -743,3 +743,61 @@ def test_step_meshes_written_on_demand(hip_mod, oracle_mod):
     c.step(2)
     assert not np.array_equal(c.grid(hip.GRID_FINAL0 + 1), mine)
     c.close()
+
+
+def test_extrapolated_guess(hip_mod):
+    """poisson_guess = 1: from the third solve the PCG starts from the linear
+    extrapolation 2 phi_{n-1} - phi_{n-2} (guess 3), reaches the same potential
+    as the previous-phi start (guess 1) and needs no more iterations on
+    average; the direct solve does not depend on the guess, so there the
+    potential is the same bit for bit (guess 5 for both)."""
+    hip = hip_mod
+    import pic_configs as cfg
+    w = cfg.WORKLOADS["electrode_512"].scaled(NX=128, n=60_000, name="extrap_128")
+    parts = [cfg.generate(w, s) for s in range(2)]
+    for direct in (0, 1):
+        a = cfg.make_context(w, particles=parts, poisson_tol=1e-10, poisson_direct=direct, poisson_guess=0)
+        b = cfg.make_context(w, particles=parts, poisson_tol=1e-10, poisson_direct=direct, poisson_guess=1)
+        ia, ib = [], []
+        for it in range(8):
+            sa, sb = a.step(it), b.step(it)
+            pa, pb = a.grid(hip.GRID_PHI), b.grid(hip.GRID_PHI)
+            if direct:
+                assert sa["guess"] == 5 and sb["guess"] == 5, (it, sa, sb)
+                assert np.array_equal(pa, pb), it
+            else:
+                assert np.abs(pa - pb).max() <= 1e-8 * np.abs(pa).max(), it
+                if it >= 2:
+                    assert sa["guess"] == 1 and sb["guess"] == 3, (it, sa["guess"], sb["guess"])
+                    ia.append(sa["iterations"])
+                    ib.append(sb["iterations"])
+        if not direct:
+            print(f"iterations: previous phi {np.mean(ia):.2f}, extrapolated {np.mean(ib):.2f}")
+            assert np.mean(ib) <= np.mean(ia), (ia, ib)
+        a.close()
+        b.close()
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_step_without_energy_deposit(hip_mod, oracle_mod, layout):
+    """deposit_energy = 0: the step deposits charge only; two steps stage by
+    stage against the oracle (charge, rho, E, push bit for bit; phi against
+    PARDISO) and the energy meshes stay zero."""
+    hip, O = hip_mod, oracle_mod
+    import pic_configs as cfg
+    import stage_check as SC
+    w = cfg.WORKLOADS["electrode_512"].scaled(NX=64, n=20_000, name="no_energy_64")
+    parts = [cfg.generate(w, s) for s in range(2)]
+    ctx = cfg.make_context(w, layout=layout, particles=parts, poisson_tol=1e-12, deposit_energy=0)
+    pref = SC.PoissonRef(O, ctx.gx, ctx.gy, ctx.dx, ctx.dy, w.rects())
+    try:
+        for it in range(2):
+            before, ab = SC.snapshot(ctx, 2)
+            ctx.step(it)
+            errs = SC.check_step(hip, O, ctx, before, ab, SC.species_table(w, cfg.N0), pref, energy=False)
+            assert errs["phi"] <= 1e-10, (it, errs)
+            for s in range(2):
+                assert not ctx.grid(hip.GRID_ENERGY0 + s).any(), (it, s)
+    finally:
+        pref.close()
+        ctx.close()
