@@ -230,11 +230,14 @@ def test_efield_parity(hip_mod, oracle_mod):
     c.close()
 
 
-@pytest.mark.parametrize("layout,bx", [(0, 0.0), (0, 0.025), (1, 0.0), (1, 0.025)])
-def test_push_parity(hip_mod, oracle_mod, layout, bx):
+@pytest.mark.parametrize("layout,bx,by", [(0, 0.0, 0.0), (0, 0.025, 0.0), (0, 0.025, -0.015), (1, 0.0, 0.0),
+                                          (1, 0.025, 0.0), (1, 0.025, -0.015)])
+def test_push_parity(hip_mod, oracle_mod, layout, bx, by):
+    """Gather + Boris + walls + re-bin against the oracle bit for bit, with a
+    non-uniform Bz and none, one or both in-plane components of B."""
     hip, O = hip_mod, oracle_mod
     NX = 64
-    c = _ctx(hip, NX, layout, bx=bx)
+    c = _ctx(hip, NX, layout, bx=bx, by=by)
     rng = np.random.default_rng(8)
     ex = rng.normal(0, 2e3, c.G)
     ey = rng.normal(0, 2e3, c.G)
@@ -252,11 +255,12 @@ def test_push_parity(hip_mod, oracle_mod, layout, bx):
         dt = c.params.dt
         g = dict(gx=c.gx, gy=c.gy, dx=c.dx, dy=c.dy, width=c.params.width, height=c.params.height)
         if layout == 0:
-            X, Y, VX, VY, VZ, key, ab = O.push_f64(x, y, vx, vy, vz, ex, ey, bz, qm=q / m, dt=dt, bx=bx, **g)
+            X, Y, VX, VY, VZ, key, ab = O.push_f64(x, y, vx, vy, vz, ex, ey, bz, qm=q / m, dt=dt, bx=bx, by=by,
+                                                   **g)
         else:
             cell, ox, oy, fvx, fvy, fvz = O.encode_cell32(x, y, vx, vy, vz, c.gx, c.dx, c.dy)
             rc, rox, roy, rvx, rvy, rvz, key, ab = O.push_cell32(cell, ox, oy, fvx, fvy, fvz, ex, ey, bz,
-                                                                 qm=q / m, dt=dt, bx=bx, **g)
+                                                                 qm=q / m, dt=dt, bx=bx, by=by, **g)
         c.push(s, rebin=True)
         live = key >= 0
         assert c.count(s) == live.sum()
@@ -801,3 +805,30 @@ def test_step_without_energy_deposit(hip_mod, oracle_mod, layout):
     finally:
         pref.close()
         ctx.close()
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_step_with_uniform_tilted_b(hip_mod, oracle_mod, layout):
+    """A uniform B with Bz and both in-plane components (the step's uniform-B
+    push constants): two steps with the electrode, stage by stage against the
+    oracle (deposit, rho, E and push bit for bit, phi against PARDISO)."""
+    hip, O = hip_mod, oracle_mod
+    import stage_check as SC
+    c = _ctx(hip, 64, layout, geom=ELEC, bx=0.02, by=-0.015, poisson_tol=1e-12)
+    spec = []
+    n = 20000
+    for sp, (q, m, T) in enumerate([(-E, ME, 2.0), (E, MI, 0.03)]):
+        s = c.add_species(q, m, n, 1e16, n)
+        x, y, vx, vy, vz = _parts(hip, c, n, seed=31, stream=sp, qm=abs(q / m), T=T)
+        c.upload(s, x, y, vx, vy, vz)
+        spec.append((q, m, 1e16 * (c.params.width * c.params.height) / (n * c.dx * c.dy * c.dx * c.dy)))
+    pref = SC.PoissonRef(O, c.gx, c.gy, c.dx, c.dy, ELEC)
+    try:
+        for it in range(2):
+            before, ab = SC.snapshot(c, 2)
+            c.step(it)
+            errs = SC.check_step(hip, O, c, before, ab, spec, pref)
+            assert errs["phi"] <= 1e-10, (it, errs)
+    finally:
+        pref.close()
+        c.close()
